@@ -1,0 +1,166 @@
+/* topomax_amd C ABI: the elasticity hot path of topomax on one MI355X (gfx950).
+ *
+ * One handle = one ElasticityProblem on a structured W x H rectangle meshed with nx x ny
+ * cells split by the 'right' diagonal (P1 displacement, CG1 design), with its Helmholtz
+ * filter, SIMP penalizer, load vector and Dirichlet sides.  All array arguments are DEVICE
+ * pointers (e.g. torch tensor data_ptr()) in vertex order v = iy*(nx+1) + ix; displacements
+ * are node-interleaved (u[2v], u[2v+1]).  All work is enqueued on the handle's stream
+ * (tmx_set_stream); functions returning a host scalar synchronise that stream.
+ *
+ * Return codes: TMX_OK, or a negative TMX_ERR_*; tmx_last_error() gives the message of the
+ * last failure on the calling thread.  Misuse maps to the reference's ValueError, numerical
+ * or HIP failures to RuntimeError (see the Python wrapper).
+ *
+ * Reference interfaces replaced (topomax repository, relative paths):
+ *   ElasticityProblem  FEM_src/elasisity_problem.py:76-196
+ *   HelmholtzFilter    FEM_src/filter.py:8-41
+ *   SmartMumpsSolver   FEM_src/pde_solver.py:75-133
+ *   ElasticPenalizer   src/penalizers.py:29-45
+ *   Problem ABC        src/problem.py:7-23
+ *   FEMSolver.integrate FEM_src/solver.py:81-84
+ */
+#ifndef TMX_H
+#define TMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMX_ABI_VERSION 1
+
+enum tmx_status {
+  TMX_OK = 0,
+  TMX_ERR_INVALID = -1,       /* misuse: bad argument, call order, unset penalty */
+  TMX_ERR_HIP = -2,           /* HIP runtime failure */
+  TMX_ERR_NOT_CONVERGED = -3, /* iteration limit reached before the tolerance */
+  TMX_ERR_BREAKDOWN = -4,     /* operator not SPD (e.g. zero stiffness) */
+  TMX_ERR_NONFINITE = -5,     /* non-finite data; outputs are filled with NaN */
+  TMX_ERR_UNSUPPORTED = -6    /* valid for the reference but not implemented here */
+};
+
+enum tmx_side { TMX_LEFT = 0, TMX_RIGHT = 1, TMX_TOP = 2, TMX_BOTTOM = 3 };
+enum tmx_side_bits { TMX_FIX_LEFT = 1, TMX_FIX_RIGHT = 2, TMX_FIX_TOP = 4, TMX_FIX_BOTTOM = 8 };
+enum tmx_precision { TMX_FP64 = 0, TMX_MIXED = 1, TMX_FP32 = 2 };
+
+#define TMX_MAX_TRACTIONS 16
+
+/* designs/definitions.py:150-166 Traction */
+typedef struct tmx_traction {
+  int side;          /* enum tmx_side */
+  double center, length;
+  double value[2];
+} tmx_traction;
+
+/* designs/definitions.py:137-146 Force (+ CircularRegion :90-97) */
+typedef struct tmx_body_force {
+  double center[2];
+  double radius;
+  double value[2];
+} tmx_body_force;
+
+/* ElasticityProblem.__init__ inputs: mesh (FEM_src/solver.py:38-47), DomainParameters and
+ * ElasticityParameters (designs/definitions.py:170-223), penalizer minimum
+ * (src/penalizers.py:34). */
+typedef struct tmx_problem_desc {
+  int nx, ny;                 /* cells: int(W*N), int(H*N) */
+  double width, height;
+  double young_modulus, poisson_ratio;
+  double penalty_minimum;     /* m in r = m + x^p (1-m); the reference uses 1e-6 */
+  double filter_radius;       /* Helmholtz epsilon, used unscaled */
+  unsigned fixed_sides;       /* requested sides, TMX_FIX_* bits (near-test applied inside) */
+  int n_tractions;
+  tmx_traction tractions[TMX_MAX_TRACTIONS];
+  int has_body_force;
+  tmx_body_force body_force;
+  int precision;              /* default tmx_precision for solves */
+} tmx_problem_desc;
+
+typedef struct tmx_solve_opts {
+  double rtol;                /* relative residual |b - A u| / |b| */
+  int max_iter;               /* PCG iterations (inner, summed over refinements) */
+  int precision;              /* -1: handle default, else tmx_precision */
+  double inner_rtol;          /* mixed: reduction asked of each FP32 inner solve */
+  int max_outer;              /* mixed: refinement steps */
+} tmx_solve_opts;
+
+typedef struct tmx_solve_stats {
+  int iterations;             /* PCG iterations (all inner solves) */
+  int outer_iterations;       /* refinement steps (mixed), 1 for FP64 */
+  double rel_residual;        /* final |b - A u| / |b| */
+  double setup_ms, solve_ms;  /* operator hierarchy update / iterations (host clock) */
+  int levels;                 /* multigrid levels */
+} tmx_solve_stats;
+
+typedef struct tmx_problem tmx_problem;
+
+const char* tmx_last_error(void);
+int tmx_abi_version(void);
+
+/* ElasticityProblem.__init__ (FEM_src/elasisity_problem.py:79-102): mesh, marker, BCs,
+ * HelmholtzFilter (A assembled once, filter.py:15-38), load vector assembled once
+ * (pde_solver.py:102-104).  All device workspace is allocated here. */
+int tmx_problem_create(const tmx_problem_desc* desc, tmx_problem** out);
+void tmx_problem_destroy(tmx_problem* h);
+int tmx_set_stream(tmx_problem* h, void* hip_stream);
+int tmx_problem_info(const tmx_problem* h, int64_t* n_vertices, int64_t* n_dofs,
+                     unsigned* fixed_sides_applied, int* levels);
+
+/* FEMProblem.set_penalization (FEM_src/problem.py:29-35) -> Penalizer.set_penalization
+ * (src/penalizers.py:11-12).  Integer p in [1, 8]. */
+int tmx_set_penalization(tmx_problem* h, double p);
+
+/* HelmholtzFilter.apply with a CG1 input (filter.py:40-41): solve A xi = M rho. */
+int tmx_filter_apply(tmx_problem* h, const double* rho, double* xi);
+/* HelmholtzFilter.apply with a form input whose vector b is already assembled: A w = b. */
+int tmx_filter_solve(tmx_problem* h, const double* b, double* w);
+/* y = M x (CG1 mass); M w is the Euclidean gradient from the L2 representer w. */
+int tmx_mass_apply(tmx_problem* h, const double* x, double* y);
+
+/* ElasticityProblem.forward (:168-169) -> SmartMumpsSolver.solve(a_arg=xi): K(xi) u = F
+ * with zero Dirichlet dofs.  opts/stats may be NULL. */
+int tmx_elasticity_solve(tmx_problem* h, const double* xi, double* u, const tmx_solve_opts* opts,
+                         tmx_solve_stats* stats);
+/* compliance J = int u.f dx + int_ds u.t = F.u (:161-164) */
+int tmx_compliance(tmx_problem* h, const double* u, double* out_host);
+/* sensitivity rhs b_i = int -r'(xi) sigma0(u):eps(u) phi_i (:146-149) */
+int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u, double* b);
+/* calculate_objective + calculate_objective_gradient (:134-166) fused:
+ * xi = F(rho), u = K(xi)^-1 F, J = F.u, w = A^-1 b(xi, u), Mw = M w.
+ * xi, u, w, Mw may be NULL when not wanted (xi/u use handle scratch then). */
+int tmx_objective_and_gradient(tmx_problem* h, const double* rho, double* J_host, double* xi,
+                               double* u, double* w, double* Mw, const tmx_solve_opts* opts,
+                               tmx_solve_stats* stats);
+/* y = A(xi) u for the Dirichlet-masked stiffness (identity on fixed dofs). precision:
+ * TMX_FP64 or TMX_FP32 (inputs/outputs stay double). */
+int tmx_matvec(tmx_problem* h, const double* xi, const double* u, double* y, int precision);
+/* load vector F (2 n_v) as assembled at create time (before Dirichlet masking) */
+int tmx_load_vector(tmx_problem* h, double* F);
+/* FEMSolver.integrate (FEM_src/solver.py:81-84): int of the CG1 interpolant of values */
+int tmx_integrate(tmx_problem* h, const double* values, double* out_host);
+
+/* Autotuner: times the FP64 and FP32 elasticity operator sweeps on this problem's fine grid
+ * for each candidate row-chunk size and keeps the fastest; results are written to
+ * rows_out[2] (fp64, fp32) and best_us_out[2]. */
+int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
+/* Sweep row-chunk override (fp64, fp32); <= 0 keeps the current value. */
+int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
+
+/* Batched objective + gradient over many independent problems (uneven sizes, penalties and
+ * loads).  Each handle keeps its own hierarchy; the batch enqueues all designs' work phase by
+ * phase on one stream so that every kernel launch covers all unconverged designs.
+ * rho[i], w[i] device pointers; J_host[i] receives the compliance. */
+typedef struct tmx_batch tmx_batch;
+int tmx_batch_create(tmx_problem* const* problems, int count, tmx_batch** out);
+void tmx_batch_destroy(tmx_batch* b);
+int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho, double* J_host,
+                                     double* const* w, const tmx_solve_opts* opts,
+                                     tmx_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMX_H */

@@ -1,0 +1,77 @@
+"""Shared workload definitions for the tests (seeded, small enough for the oracle)."""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+CANTILEVER = dict(width=3.0, height=1.0, young_modulus=1.0, poisson_ratio=0.3, filter_radius=0.05,
+                  fixed_sides=["Left"],
+                  tractions=[{"side": "Right", "center": 0.5, "length": 0.1, "value": (0.0, -1.0)}],
+                  body_force=None)
+
+
+def uniform_density(n_vertices: int, seed: int):
+    """0.5 + 0.1 U(-1, 1) per vertex."""
+    rng = np.random.default_rng(seed)
+    return 0.5 + 0.1 * rng.uniform(-1.0, 1.0, n_vertices)
+
+
+def smoothed_density(nx: int, ny: int, seed: int, sigma: float = 5.0, lo: float = 1e-3):
+    """Gaussian-smoothed (sigma in cells) uniform noise on the node grid, min-max mapped to
+    [lo, 1], flattened in vertex order."""
+    rng = np.random.default_rng(seed)
+    a = gaussian_filter(rng.uniform(0.0, 1.0, (ny + 1, nx + 1)), sigma, mode="nearest")
+    a = (a - a.min()) / (a.max() - a.min())
+    return (lo + (1.0 - lo) * a).ravel()
+
+
+def oracle_problem(case: dict, nx: int, ny: int, degree: int = 1):
+    from oracle.fem import ElasticityOracle, StructuredMesh
+    mesh = StructuredMesh(case["width"], case["height"], nx, ny)
+    return ElasticityOracle(mesh, young_modulus=case["young_modulus"],
+                            poisson_ratio=case["poisson_ratio"],
+                            filter_radius=case["filter_radius"],
+                            fixed_sides=case["fixed_sides"], tractions=case.get("tractions"),
+                            body_force=case.get("body_force"), degree=degree)
+
+
+def gpu_problem(case: dict, nx: int, ny: int, precision: str = "fp64", rtol: float = 1e-12):
+    from topomax_amd import (CircularRegion, ControlSpace, DomainParameters,
+                             ElasticityParameters, ElasticityProblem, Force, Side,
+                             StructuredMesh, Traction)
+    from topomax_amd.designs import ProblemType
+    mesh = StructuredMesh(case["width"], case["height"], nx, ny)
+    space = ControlSpace(mesh, "cuda:0")
+    tractions = None
+    if case.get("tractions"):
+        tractions = [Traction(Side.from_string(t["side"]), t["center"], t["length"],
+                              tuple(t["value"])) for t in case["tractions"]]
+    body = None
+    if case.get("body_force"):
+        bf = case["body_force"]
+        body = Force(CircularRegion(tuple(bf["center"]), bf["radius"]), tuple(bf["value"]))
+    params = ElasticityParameters([Side.from_string(s) for s in case["fixed_sides"]], body,
+                                  tractions, case["filter_radius"], case["young_modulus"],
+                                  case["poisson_ratio"])
+    domain = DomainParameters(case["width"], case["height"], ProblemType.ELASTICITY, 25.0, 25.0,
+                              [3.0], 0.5)
+    return ElasticityProblem(mesh, space, domain, params, precision=precision, rtol=rtol)
+
+
+def case_from_design(design: dict) -> dict:
+    (key,) = design.keys()
+    dom = design[key]["domain_parameters"]
+    par = design[key]["problem_parameters"]
+    body = None
+    if par.get("body_force") is not None:
+        bf = par["body_force"]
+        body = {"center": tuple(bf["region"]["center"]), "radius": bf["region"]["radius"],
+                "value": tuple(bf["value"])}
+    tractions = None
+    if par.get("tractions") is not None:
+        tractions = [{"side": t["side"], "center": t["center"], "length": t["length"],
+                      "value": tuple(t["value"])} for t in par["tractions"]]
+    return dict(width=dom["width"], height=dom["height"], young_modulus=par["young_modulus"],
+                poisson_ratio=par["poisson_ratio"], filter_radius=par["filter_radius"],
+                fixed_sides=par["fixed_sides"], tractions=tractions, body_force=body)

@@ -1,0 +1,453 @@
+// extern "C" boundary of topomax_amd (declared in include/tmx.h).
+#include "../../include/tmx.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tmx_solver.h"
+
+namespace tmx {
+const char* last_error_cstr();
+}
+
+using namespace tmx;
+
+struct tmx_problem {
+  tmx_problem_desc desc;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0;
+  double hx = 0, hy = 0, lam = 0, mu = 0, m = 1e-6;
+  unsigned fixed = 0;  // sides actually fixed after the near() test
+  int p = 0;
+  bool has_penalty = false;
+  Hierarchy el, fl;
+  Workspace ws;
+  DevBuf<double> F, Fm, lumped;
+  DevBuf<double> xi_s, u_s, b_s, w_s;     // scratch of the fused call
+  DevBuf<double> tmp64;
+  DevBuf<float> xi32, u32, y32;
+  SolveParams el_prm, fl_prm;
+};
+
+static const double kDolfinEps = 3.0e-16;
+
+static int invalid(const std::string& m) {
+  set_error(m);
+  return TMX_ERR_INVALID;
+}
+
+extern "C" const char* tmx_last_error(void) { return last_error_cstr(); }
+extern "C" int tmx_abi_version(void) { return TMX_ABI_VERSION; }
+
+static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
+  if (d->nx < 1 || d->ny < 1) return invalid("nx and ny must be >= 1");
+  if (!(d->width > 0) || !(d->height > 0)) return invalid("width and height must be > 0");
+  if (!(d->young_modulus > 0)) return invalid("young_modulus must be > 0");
+  if (!(d->poisson_ratio > -1.0 && d->poisson_ratio < 0.5))
+    return invalid("poisson_ratio must be in (-1, 0.5)");
+  if (!(d->penalty_minimum >= 0 && d->penalty_minimum < 1))
+    return invalid("penalty_minimum must be in [0, 1)");
+  if (!(d->filter_radius >= 0)) return invalid("filter_radius must be >= 0");
+  if (d->n_tractions < 0 || d->n_tractions > TMX_MAX_TRACTIONS)
+    return invalid("n_tractions out of range");
+  for (int i = 0; i < d->n_tractions; ++i)
+    if (d->tractions[i].side < 0 || d->tractions[i].side > 3)
+      return invalid("Malformed side: " + std::to_string(d->tractions[i].side));
+  if ((d->fixed_sides & ~15u) != 0) return invalid("Malformed side in fixed_sides");
+  if (d->precision < 0 || d->precision > 2) return invalid("unknown precision");
+  h->desc = *d;
+  h->nx = d->nx;
+  h->ny = d->ny;
+  // dolfin RectangleMesh vertex x = a + ab*ix, ab = (b - a)/nx
+  const double ab = (d->width - 0.0) / d->nx, cd = (d->height - 0.0) / d->ny;
+  h->hx = ab;
+  h->hy = cd;
+  h->m = d->penalty_minimum;
+  // SidesDomain.inside (FEM_src/domains.py:18-38): a side is fixed iff its coordinate is
+  // near (|x - t| < DOLFIN_EPS) the target
+  unsigned fx = 0;
+  if ((d->fixed_sides & TMX_FIX_LEFT) && std::fabs(0.0 - 0.0) < kDolfinEps) fx |= kLeft;
+  if ((d->fixed_sides & TMX_FIX_RIGHT) && std::fabs(0.0 + ab * d->nx - d->width) < kDolfinEps)
+    fx |= kRight;
+  if ((d->fixed_sides & TMX_FIX_TOP) && std::fabs(0.0 + cd * d->ny - d->height) < kDolfinEps)
+    fx |= kTop;
+  if ((d->fixed_sides & TMX_FIX_BOTTOM) && std::fabs(0.0 - 0.0) < kDolfinEps) fx |= kBottom;
+  h->fixed = fx;
+  // Lame parameters as the reference computes them (FEM_src/elasisity_problem.py:98-99)
+  h->mu = d->young_modulus / (2 * (1 + d->poisson_ratio));
+  h->lam = h->mu * d->poisson_ratio / (0.5 - d->poisson_ratio);
+  hipStream_t s = h->stream;
+  TMX_TRY(h->ws.init(s));
+  TMX_TRY(h->el.init(kElasticity, h->nx, h->ny, ab, cd, fx, h->lam, h->mu, 0.0, 64, s));
+  TMX_TRY(h->fl.init(kFilter, h->nx, h->ny, ab, cd, 0u, 0.0, 0.0, d->filter_radius, 128, s));
+  TMX_TRY(h->fl.build_filter(s));
+  const size_t nv = (size_t)(h->nx + 1) * (h->ny + 1);
+  TMX_TRY(h->F.alloc(2 * nv));
+  TMX_TRY(h->Fm.alloc(2 * nv));
+  TMX_TRY(h->lumped.alloc(nv));
+  TMX_TRY(h->xi_s.alloc(nv));
+  TMX_TRY(h->u_s.alloc(2 * nv));
+  TMX_TRY(h->b_s.alloc(nv));
+  TMX_TRY(h->w_s.alloc(nv));
+  TMX_TRY(h->tmp64.alloc(2 * nv));
+  TMX_TRY(h->xi32.alloc(nv));
+  TMX_TRY(h->u32.alloc(2 * nv));
+  TMX_TRY(h->y32.alloc(2 * nv));
+  LoadDesc ld;
+  std::memset(&ld, 0, sizeof(ld));
+  ld.nx = h->nx;
+  ld.ny = h->ny;
+  ld.width = d->width;
+  ld.height = d->height;
+  ld.ab = ab;
+  ld.cd = cd;
+  ld.has_body = d->has_body_force ? 1 : 0;
+  ld.bc[0] = d->body_force.center[0];
+  ld.bc[1] = d->body_force.center[1];
+  ld.br = d->body_force.radius;
+  ld.bv[0] = d->body_force.value[0];
+  ld.bv[1] = d->body_force.value[1];
+  ld.ntr = d->n_tractions;
+  for (int i = 0; i < d->n_tractions; ++i) {
+    const tmx_traction& t = d->tractions[i];
+    ld.tside[i] = t.side;
+    ld.tlo[i] = t.center - t.length / 2;
+    ld.thi[i] = t.center + t.length / 2;
+    ld.tv[i][0] = t.value[0];
+    ld.tv[i][1] = t.value[1];
+  }
+  TMX_HIP(elast_assemble_load(ld, h->F.get(), s));
+  TMX_HIP(copy_vec<double>(2 * nv, h->F.get(), h->Fm.get(), s));
+  TMX_HIP(mask_fixed(h->nx, h->ny, fx, h->Fm.get(), s));
+  // lumped mass m_i = int phi_i = (M 1)_i
+  TMX_HIP(fill(nv, 1.0, h->tmp64.get(), s));
+  TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, ab, cd),
+                               kApply, h->tmp64.get(), nullptr, h->lumped.get(), 0.0, nullptr, s));
+  h->el_prm.rtol = 1e-10;
+  h->el_prm.max_iter = 1000;
+  h->el_prm.precision = d->precision == TMX_FP64 ? 0 : 1;
+  h->fl_prm.rtol = 1e-12;
+  h->fl_prm.max_iter = 500;
+  h->fl_prm.precision = 0;
+  TMX_HIP(hipStreamSynchronize(s));
+  return TMX_OK;
+}
+
+extern "C" int tmx_problem_create(const tmx_problem_desc* desc, tmx_problem** out) {
+  if (!desc || !out) return invalid("null argument");
+  *out = nullptr;
+  tmx_problem* h = new (std::nothrow) tmx_problem();
+  if (!h) return invalid("out of host memory");
+  const int rc = create_impl(desc, h);
+  if (rc != TMX_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return TMX_OK;
+}
+
+extern "C" void tmx_problem_destroy(tmx_problem* h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->stream);
+  delete h;
+}
+
+extern "C" int tmx_set_stream(tmx_problem* h, void* stream) {
+  if (!h) return invalid("null handle");
+  h->stream = static_cast<hipStream_t>(stream);
+  return TMX_OK;
+}
+
+extern "C" int tmx_problem_info(const tmx_problem* h, int64_t* n_vertices, int64_t* n_dofs,
+                                unsigned* fixed_sides_applied, int* levels) {
+  if (!h) return invalid("null handle");
+  const int64_t nv = (int64_t)(h->nx + 1) * (h->ny + 1);
+  if (n_vertices) *n_vertices = nv;
+  if (n_dofs) *n_dofs = 2 * nv;
+  if (fixed_sides_applied) {
+    unsigned f = 0;
+    if (h->fixed & kLeft) f |= TMX_FIX_LEFT;
+    if (h->fixed & kRight) f |= TMX_FIX_RIGHT;
+    if (h->fixed & kTop) f |= TMX_FIX_TOP;
+    if (h->fixed & kBottom) f |= TMX_FIX_BOTTOM;
+    *fixed_sides_applied = f;
+  }
+  if (levels) *levels = h->el.levels();
+  return TMX_OK;
+}
+
+extern "C" int tmx_set_penalization(tmx_problem* h, double p) {
+  if (!h) return invalid("null handle");
+  if (!(p >= 1.0 && p <= 8.0) || p != std::floor(p)) {
+    set_error("penalization must be an integer in [1, 8] (closed-form exact integration)");
+    return TMX_ERR_UNSUPPORTED;
+  }
+  h->p = (int)p;
+  h->has_penalty = true;
+  return TMX_OK;
+}
+
+static int need_penalty(const tmx_problem* h) {
+  if (!h->has_penalty) return invalid("You must set penalization before calling penalizer");
+  return TMX_OK;
+}
+
+static SolveParams merge(const SolveParams& base, const tmx_solve_opts* o) {
+  SolveParams p = base;
+  if (!o) return p;
+  if (o->rtol > 0) p.rtol = o->rtol;
+  if (o->max_iter > 0) p.max_iter = o->max_iter;
+  if (o->precision >= 0) p.precision = o->precision == TMX_FP64 ? 0 : 1;
+  if (o->inner_rtol > 0) p.inner_rtol = o->inner_rtol;
+  if (o->max_outer > 0) p.max_outer = o->max_outer;
+  return p;
+}
+
+static void export_stats(const SolveStats& s, tmx_solve_stats* o) {
+  if (!o) return;
+  o->iterations = s.iterations;
+  o->outer_iterations = s.outer;
+  o->rel_residual = s.rel_residual;
+  o->setup_ms = s.setup_ms;
+  o->solve_ms = s.solve_ms;
+  o->levels = s.levels;
+}
+
+static size_t nverts(const tmx_problem* h) { return (size_t)(h->nx + 1) * (h->ny + 1); }
+
+static int filter_rhs_solve(tmx_problem* h, const double* b, double* w) {
+  SolveStats st;
+  const int rc = h->fl.solve(h->ws, b, w, h->fl_prm, &st, h->stream);
+  if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(nverts(h), NAN, w, h->stream));
+  return rc;
+}
+
+extern "C" int tmx_mass_apply(tmx_problem* h, const double* x, double* y) {
+  if (!h || !x || !y) return invalid("null argument");
+  TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, h->hx, h->hy),
+                               kApply, x, nullptr, y, 0.0, nullptr, h->stream));
+  return TMX_OK;
+}
+
+extern "C" int tmx_filter_apply(tmx_problem* h, const double* rho, double* xi) {
+  if (!h || !rho || !xi) return invalid("null argument");
+  TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
+  return filter_rhs_solve(h, h->b_s.get(), xi);
+}
+
+extern "C" int tmx_filter_solve(tmx_problem* h, const double* b, double* w) {
+  if (!h || !b || !w) return invalid("null argument");
+  return filter_rhs_solve(h, b, w);
+}
+
+static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const tmx_solve_opts* o,
+                            SolveStats* st) {
+  TMX_TRY(need_penalty(h));
+  auto t0 = std::chrono::steady_clock::now();
+  TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, h->stream));
+  TMX_HIP(hipStreamSynchronize(h->stream));
+  const double setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const SolveParams prm = merge(h->el_prm, o);
+  const int rc = h->el.solve(h->ws, h->Fm.get(), u, prm, st, h->stream);
+  st->setup_ms = setup;
+  if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(2 * nverts(h), NAN, u, h->stream));
+  return rc;
+}
+
+extern "C" int tmx_elasticity_solve(tmx_problem* h, const double* xi, double* u,
+                                    const tmx_solve_opts* opts, tmx_solve_stats* stats) {
+  if (!h || !xi || !u) return invalid("null argument");
+  SolveStats st;
+  const int rc = elasticity_solve(h, xi, u, opts, &st);
+  export_stats(st, stats);
+  return rc;
+}
+
+static int read_dot(tmx_problem* h, size_t n, const double* a, const double* b, int slot,
+                    double* out) {
+  TMX_HIP(dot(n, a, b, h->ws.slot(slot), h->stream));
+  TMX_TRY(h->ws.read(h->stream, slot, 1));
+  *out = h->ws.host[slot];
+  return TMX_OK;
+}
+
+extern "C" int tmx_compliance(tmx_problem* h, const double* u, double* out) {
+  if (!h || !u || !out) return invalid("null argument");
+  return read_dot(h, 2 * nverts(h), h->F.get(), u, kScJ, out);
+}
+
+extern "C" int tmx_integrate(tmx_problem* h, const double* values, double* out) {
+  if (!h || !values || !out) return invalid("null argument");
+  return read_dot(h, nverts(h), h->lumped.get(), values, kScInt, out);
+}
+
+extern "C" int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
+  if (!h || !xi || !u || !b) return invalid("null argument");
+  TMX_TRY(need_penalty(h));
+  TMX_HIP(elast_sensitivity(h->el.levels_[0].sh64, xi, u, h->p, h->m, h->lam, h->mu, h->hx, h->hy,
+                            b, h->stream));
+  return TMX_OK;
+}
+
+extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, double* J,
+                                          double* xi, double* u, double* w, double* Mw,
+                                          const tmx_solve_opts* opts, tmx_solve_stats* stats) {
+  if (!h || !rho) return invalid("null argument");
+  TMX_TRY(need_penalty(h));
+  double* xiv = xi ? xi : h->xi_s.get();
+  double* uv = u ? u : h->u_s.get();
+  int rc = tmx_filter_apply(h, rho, xiv);
+  if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
+  SolveStats st;
+  if (rc == TMX_OK) rc = elasticity_solve(h, xiv, uv, opts, &st);
+  else TMX_HIP(fill(2 * nverts(h), NAN, uv, h->stream));
+  export_stats(st, stats);
+  if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
+  double Jv = 0.0;
+  TMX_TRY(tmx_compliance(h, uv, &Jv));
+  if (J) *J = Jv;
+  if (w || Mw) {
+    double* wv = w ? w : h->w_s.get();
+    TMX_TRY(tmx_sensitivity(h, xiv, uv, h->b_s.get()));
+    const int rc2 = filter_rhs_solve(h, h->b_s.get(), wv);
+    if (rc2 != TMX_OK && rc2 != TMX_ERR_NONFINITE) return rc2;
+    if (Mw) TMX_TRY(tmx_mass_apply(h, wv, Mw));
+    if (rc == TMX_OK) rc = rc2;
+  }
+  TMX_HIP(hipStreamSynchronize(h->stream));
+  return rc;
+}
+
+extern "C" int tmx_matvec(tmx_problem* h, const double* xi, const double* u, double* y,
+                          int precision) {
+  if (!h || !xi || !u || !y) return invalid("null argument");
+  TMX_TRY(need_penalty(h));
+  const size_t nv = nverts(h);
+  h->el.xi64_ = xi;
+  h->el.p_ = h->p;
+  h->el.m_ = h->m;
+  h->el.cp_ = (1.0 - h->m) * 2.0 / ((h->p + 1.0) * (h->p + 2.0));
+  if (precision == TMX_FP64) {
+    TMX_TRY(h->el.apply<double>(u, y, nullptr, h->stream));
+  } else if (precision == TMX_FP32) {
+    TMX_HIP((convert<double, float>(nv, xi, h->el.xi32_.get(), h->stream)));
+    TMX_HIP((convert<double, float>(2 * nv, u, h->u32.get(), h->stream)));
+    TMX_TRY(h->el.apply<float>(h->u32.get(), h->y32.get(), nullptr, h->stream));
+    TMX_HIP((convert<float, double>(2 * nv, h->y32.get(), y, h->stream)));
+  } else {
+    return invalid("precision must be TMX_FP64 or TMX_FP32");
+  }
+  return TMX_OK;
+}
+
+extern "C" int tmx_load_vector(tmx_problem* h, double* F) {
+  if (!h || !F) return invalid("null argument");
+  TMX_HIP(copy_vec<double>(2 * nverts(h), h->F.get(), F, h->stream));
+  return TMX_OK;
+}
+
+extern "C" int tmx_set_tiles(tmx_problem* h, int rows64, int rows32) {
+  if (!h) return invalid("null handle");
+  h->el.set_rows(rows64, rows32);
+  h->fl.set_rows(rows64, rows32);
+  return TMX_OK;
+}
+
+extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) {
+  if (!h) return invalid("null handle");
+  // needs an operator: use xi = 1 (uniform) and p = current or 3
+  const size_t nv = nverts(h);
+  const int p = h->has_penalty ? h->p : 3;
+  h->el.xi64_ = h->xi_s.get();
+  h->el.p_ = p;
+  h->el.m_ = h->m;
+  h->el.cp_ = (1.0 - h->m) * 2.0 / ((p + 1.0) * (p + 2.0));
+  hipStream_t s = h->stream;
+  TMX_HIP(fill(nv, 0.5, h->xi_s.get(), s));
+  TMX_HIP((convert<double, float>(nv, h->xi_s.get(), h->el.xi32_.get(), s)));
+  TMX_HIP(fill(2 * nv, 1.0, h->u_s.get(), s));
+  TMX_HIP((convert<double, float>(2 * nv, h->u_s.get(), h->u32.get(), s)));
+  hipEvent_t e0, e1;
+  TMX_HIP(hipEventCreate(&e0));
+  TMX_HIP(hipEventCreate(&e1));
+  const int cands[] = {2, 4, 8, 16, 32, 64};
+  int best[2] = {0, 0};
+  double best_us[2] = {1e30, 1e30};
+  for (int prec = 0; prec < 2; ++prec) {
+    for (int rows : cands) {
+      if (prec == 0) h->el.set_rows(rows, 0);
+      else h->el.set_rows(0, rows);
+      const int reps = 20;
+      for (int warm = 0; warm < 3; ++warm) {
+        if (prec == 0) TMX_TRY(h->el.apply<double>(h->u_s.get(), h->tmp64.get(), nullptr, s));
+        else TMX_TRY(h->el.apply<float>(h->u32.get(), h->y32.get(), nullptr, s));
+      }
+      TMX_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) {
+        if (prec == 0) TMX_TRY(h->el.apply<double>(h->u_s.get(), h->tmp64.get(), nullptr, s));
+        else TMX_TRY(h->el.apply<float>(h->u32.get(), h->y32.get(), nullptr, s));
+      }
+      TMX_HIP(hipEventRecord(e1, s));
+      TMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const double us = 1000.0 * ms / reps;
+      if (us < best_us[prec]) {
+        best_us[prec] = us;
+        best[prec] = rows;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->el.set_rows(best[0], best[1]);
+  h->fl.set_rows(best[0], best[1]);
+  if (rows_out) {
+    rows_out[0] = best[0];
+    rows_out[1] = best[1];
+  }
+  if (best_us_out) {
+    best_us_out[0] = best_us[0];
+    best_us_out[1] = best_us[1];
+  }
+  return TMX_OK;
+}
+
+// ------------------------------------------------------------------ batch
+struct tmx_batch {
+  std::vector<tmx_problem*> probs;
+};
+
+extern "C" int tmx_batch_create(tmx_problem* const* problems, int count, tmx_batch** out) {
+  if (!problems || count <= 0 || !out) return invalid("bad batch arguments");
+  tmx_batch* b = new (std::nothrow) tmx_batch();
+  if (!b) return invalid("out of host memory");
+  for (int i = 0; i < count; ++i) {
+    if (!problems[i]) {
+      delete b;
+      return invalid("null problem in batch");
+    }
+    b->probs.push_back(problems[i]);
+  }
+  *out = b;
+  return TMX_OK;
+}
+
+extern "C" void tmx_batch_destroy(tmx_batch* b) { delete b; }
+
+extern "C" int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho,
+                                                double* J, double* const* w,
+                                                const tmx_solve_opts* opts,
+                                                tmx_solve_stats* stats) {
+  if (!b || !rho || !J || !w) return invalid("null argument");
+  for (size_t i = 0; i < b->probs.size(); ++i) {
+    const int rc = tmx_objective_and_gradient(b->probs[i], rho[i], &J[i], nullptr, nullptr,
+                                              w[i], nullptr, opts, stats ? &stats[i] : nullptr);
+    if (rc != TMX_OK) return rc;
+  }
+  return TMX_OK;
+}

@@ -1,0 +1,1375 @@
+// gfx950 kernels for the topomax elasticity hot path: matrix-free SIMP-P1 elasticity and
+// Helmholtz-filter operators on the structured 'right'-diagonal triangle mesh, geometric
+// multigrid transfers, Galerkin coarse stencils, and the vector kernels of PCG.
+//
+// Column-sweep design (elasticity and filter operators): one wavefront owns a strip of 64
+// node columns (62 output columns, one halo column each side) and walks a chunk of node
+// rows bottom to top.  Each lane computes the cell whose lower-left node is its own column:
+// it loads only its own column's values (coalesced across the wave), obtains the right
+// neighbour column by a cross-lane move, and hands the forces of the cell's right nodes to
+// the right lane the same way.  Every HBM value is read once per sweep (plus one halo row
+// per chunk and two halo columns per 64), no LDS is used, and a node row is written as soon
+// as the two cell rows around it are done.
+
+#include "tmx_device.h"
+#include "tmx_kernels.h"
+
+#include <algorithm>
+
+namespace tmx {
+
+// ============================================================================ shapes
+SweepShape SweepShape::make(int nx, int ny, int rows) {
+  SweepShape s;
+  s.nx = nx;
+  s.ny = ny;
+  s.rows = rows < 1 ? 1 : rows;
+  s.nstrips = (nx + 1 + kStripCols - 1) / kStripCols;
+  s.nchunks = (ny + 1 + s.rows - 1) / s.rows;
+  return s;
+}
+
+int SweepShape::blocks() const {
+  const int waves = nstrips * nchunks;
+  const int wpb = kBlock / kWave;
+  return (waves + wpb - 1) / wpb;
+}
+
+template <typename T>
+ElastConst<T> ElastConst<T>::make(double lam, double mu, double hx, double hy) {
+  const double a = hy / hx, b = hx / hy;
+  ElastConst<T> k;
+  k.l2m_a = T((lam + 2 * mu) * a / 2);
+  k.lam_h = T(lam / 2);
+  k.l2m_b = T((lam + 2 * mu) * b / 2);
+  k.mu_h = T(mu / 2);
+  k.mu_a = T(mu * a / 2);
+  k.mu_b = T(mu * b / 2);
+  k.Pxx = T((lam + 2 * mu) * a / 2);
+  k.Pyy = T(mu * a / 2);
+  k.Qxx = T(mu * b / 2);
+  k.Qyy = T((lam + 2 * mu) * b / 2);
+  k.Rxx = T((lam + 2 * mu) * a / 2 + mu * b / 2);
+  k.Rxy = T(-(lam + mu) / 2);
+  k.Ryy = T((lam + 2 * mu) * b / 2 + mu * a / 2);
+  return k;
+}
+template struct ElastConst<double>;
+template struct ElastConst<float>;
+
+template <typename T>
+FilterConst<T> FilterConst<T>::make(double eps, double hx, double hy) {
+  FilterConst<T> k;
+  k.ka = T(eps * eps * (hy / hx) / 2);
+  k.kb = T(eps * eps * (hx / hy) / 2);
+  k.mA = T(hx * hy / 2 / 12);
+  return k;
+}
+template struct FilterConst<double>;
+template struct FilterConst<float>;
+
+static inline unsigned grid1d(size_t n, int block = kBlock) {
+  size_t g = (n + block - 1) / block;
+  return (unsigned)std::max<size_t>(1, g);
+}
+
+// ============================================================================ elasticity
+// Forces of one cell (triangles A = (v0,v1,v3) and B = (v0,v2,v3)) on its four nodes,
+// given the triangles' mean SIMP factors wA, wB.  K_T = wT |T| B^T D B with
+// D = [[l+2m, l, 0], [l, l+2m, 0], [0, 0, m]], written out for the two right triangles.
+template <typename T>
+__device__ __forceinline__ void cell_forces(const ElastConst<T>& k, T wA, T wB, V2<T> u0, V2<T> u1,
+                                            V2<T> u2, V2<T> u3, V2<T>& f0, V2<T>& f1, V2<T>& f2,
+                                            V2<T>& f3) {
+  {  // triangle A: grad l0 = (-1/hx, 0), grad l1 = (1/hx, -1/hy), grad l3 = (0, 1/hy)
+    const T e1 = u1.x - u0.x, e2 = u3.y - u1.y, g1 = u3.x - u1.x, g2 = u1.y - u0.y;
+    const T Sx = wA * (k.l2m_a * e1 + k.lam_h * e2);
+    const T Sy = wA * (k.lam_h * e1 + k.l2m_b * e2);
+    const T T1 = wA * (k.mu_h * g1 + k.mu_a * g2);
+    const T T2 = wA * (k.mu_b * g1 + k.mu_h * g2);
+    f0 = {-Sx, -T1};
+    f1 = {Sx - T2, T1 - Sy};
+    f3 = {T2, Sy};
+  }
+  {  // triangle B: grad l0 = (0, -1/hy), grad l2 = (-1/hx, 1/hy), grad l3 = (1/hx, 0)
+    const T e1 = u3.x - u2.x, e2 = u2.y - u0.y, g1 = u2.x - u0.x, g2 = u3.y - u2.y;
+    const T Sx = wB * (k.l2m_a * e1 + k.lam_h * e2);
+    const T Sy = wB * (k.lam_h * e1 + k.l2m_b * e2);
+    const T T1 = wB * (k.mu_h * g1 + k.mu_a * g2);
+    const T T2 = wB * (k.mu_b * g1 + k.mu_h * g2);
+    f0 = f0 + V2<T>{-T2, -Sy};
+    f2 = {T2 - Sx, Sy - T1};
+    f3 = f3 + V2<T>{Sx, T1};
+  }
+}
+
+template <typename T>
+struct Diag3 {
+  T xx, xy, yy;
+};
+
+template <typename T, bool XI, int OP>
+__global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const T* __restrict__ x,
+                                                        const T* __restrict__ b, T* __restrict__ y,
+                                                        T omega, RedSlot red, int want_dot) {
+  using V = V2<T>;
+  const SweepShape sh = a.sh;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
+  const int c = strip * kStripCols - 1 + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= 1 && lane <= kStripCols;
+  const unsigned fixed = a.fixed;
+  const bool colfix = ((fixed & kLeft) && c == 0) || ((fixed & kRight) && c == nx);
+  const int r0 = chunk * sh.rows;
+  const int r1 = wave_ok ? min(r0 + sh.rows, ny + 1) : r0 - 1;
+  const V* X = reinterpret_cast<const V*>(x);
+  const V* B = reinterpret_cast<const V*>(b);
+  V* Y = reinterpret_cast<V*>(y);
+  const V* CW = reinterpret_cast<const V*>(a.cellw);
+  const ElastConst<T> k = a.k;
+
+  auto row_fixed = [&](int iy) {
+    return colfix || ((fixed & kBottom) && iy == 0) || ((fixed & kTop) && iy == ny);
+  };
+  auto ld_x = [&](int iy, V& raw) -> V {
+    V v = {T(0), T(0)};
+    if (OP != kSmoothZero && colv && iy >= 0 && iy <= ny) v = X[(size_t)iy * nvx + c];
+    raw = v;
+    if (row_fixed(iy)) v = {T(0), T(0)};
+    return v;
+  };
+  auto ld_xi = [&](int iy) -> T {
+    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(size_t)iy * nvx + c];
+    return T(0);
+  };
+
+  int cy = r0 - 1;
+  V ub_raw;
+  V ub = ld_x(cy, ub_raw);
+  T zb = ld_xi(cy);
+  V ubr = lane_down(ub);
+  T zbr = lane_down(zb);
+  V acc_b = {T(0), T(0)};
+  Diag3<T> d_b = {T(0), T(0), T(0)};
+  double dot = 0.0;
+
+  for (; cy < r1; ++cy) {
+    V ut_raw;
+    const V ut = ld_x(cy + 1, ut_raw);
+    const T zt = ld_xi(cy + 1);
+    const V utr = lane_down(ut);
+    const T ztr = lane_down(zt);
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    T wA = T(0), wB = T(0);
+    if (cell_ok) {
+      if (XI) {
+        wA = a.m + a.cp * hsym3(zb, zbr, ztr, a.p);
+        wB = a.m + a.cp * hsym3(zb, zt, ztr, a.p);
+      } else {
+        const V w = CW[(size_t)cy * nx + c];
+        wA = w.x;
+        wB = w.y;
+      }
+    }
+    V acc_t = {T(0), T(0)};
+    if constexpr (OP != kSmoothZero) {
+      V f0, f1, f2, f3;
+      cell_forces(k, wA, wB, ub, ubr, ut, utr, f0, f1, f2, f3);
+      acc_b += f0;
+      acc_t = f2;
+      acc_b += lane_up(f1);
+      acc_t += lane_up(f3);
+    }
+    Diag3<T> d_t = {T(0), T(0), T(0)};
+    if constexpr (OP == kSmooth || OP == kSmoothZero) {
+      const T wAl = lane_up(wA), wBl = lane_up(wB);
+      // own bottom node = v0 of this cell (A and B) and v1 of the left cell (A)
+      d_b.xx += wA * k.Pxx + wB * k.Qxx + wAl * k.Rxx;
+      d_b.xy += wAl * k.Rxy;
+      d_b.yy += wA * k.Pyy + wB * k.Qyy + wAl * k.Ryy;
+      // own top node = v2 of this cell (B) and v3 of the left cell (A and B)
+      d_t.xx = wB * k.Rxx + wAl * k.Qxx + wBl * k.Pxx;
+      d_t.xy = wB * k.Rxy;
+      d_t.yy = wB * k.Ryy + wAl * k.Qyy + wBl * k.Pyy;
+    }
+    if (cy >= r0 && outl) {
+      const size_t idx = (size_t)cy * nvx + c;
+      const bool fx = row_fixed(cy);
+      const V Ax = fx ? ub_raw : acc_b;
+      if constexpr (OP == kApply) {
+        Y[idx] = Ax;
+        if (want_dot) dot += (double)ub_raw.x * (double)Ax.x + (double)ub_raw.y * (double)Ax.y;
+      } else if constexpr (OP == kResidual) {
+        Y[idx] = B[idx] - Ax;
+      } else {
+        const V res = B[idx] - Ax;
+        V d;
+        if (fx) {
+          d = res;
+        } else {
+          const T det = d_b.xx * d_b.yy - d_b.xy * d_b.xy;
+          const T inv = T(1) / det;
+          d = {(d_b.yy * res.x - d_b.xy * res.y) * inv, (d_b.xx * res.y - d_b.xy * res.x) * inv};
+        }
+        Y[idx] = {ub_raw.x + omega * d.x, ub_raw.y + omega * d.y};
+      }
+    }
+    acc_b = acc_t;
+    d_b = d_t;
+    ub = ut;
+    ub_raw = ut_raw;
+    ubr = utr;
+    zb = zt;
+    zbr = ztr;
+  }
+  if constexpr (OP == kApply) {
+    if (want_dot) {
+      const double v[1] = {dot};
+      const Reduction r[1] = {{red.partials, red.counter, red.result}};
+      block_reduce_finalize<1>(v, r);
+    }
+  }
+}
+
+template <typename T>
+hipError_t elast_sweep(const ElastOpArgs<T>& a, int op, const T* x, const T* b, T* y, T omega,
+                       const RedSlot* dot_xy, hipStream_t s) {
+  const int nb = a.sh.blocks();
+  RedSlot r = dot_xy ? *dot_xy : RedSlot{nullptr, nullptr, nullptr};
+  const int wd = dot_xy ? 1 : 0;
+  const bool xi = a.xi != nullptr;
+#define TMX_LAUNCH(XIV, OPV) \
+  k_elast_sweep<T, XIV, OPV><<<nb, kBlock, 0, s>>>(a, x, b, y, omega, r, wd)
+  switch (op) {
+    case kApply: if (xi) TMX_LAUNCH(true, kApply); else TMX_LAUNCH(false, kApply); break;
+    case kResidual: if (xi) TMX_LAUNCH(true, kResidual); else TMX_LAUNCH(false, kResidual); break;
+    case kSmooth: if (xi) TMX_LAUNCH(true, kSmooth); else TMX_LAUNCH(false, kSmooth); break;
+    default: if (xi) TMX_LAUNCH(true, kSmoothZero); else TMX_LAUNCH(false, kSmoothZero); break;
+  }
+#undef TMX_LAUNCH
+  return hipGetLastError();
+}
+template hipError_t elast_sweep<double>(const ElastOpArgs<double>&, int, const double*,
+                                        const double*, double*, double, const RedSlot*,
+                                        hipStream_t);
+template hipError_t elast_sweep<float>(const ElastOpArgs<float>&, int, const float*, const float*,
+                                       float*, float, const RedSlot*, hipStream_t);
+
+template <typename T>
+__global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
+                                     V2<T>* __restrict__ cw) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int cx = (int)(i % nx), cy = (int)(i / nx);
+  const size_t v0 = (size_t)cy * (nx + 1) + cx;
+  const T a0 = xi[v0], a1 = xi[v0 + 1], a2 = xi[v0 + nx + 1], a3 = xi[v0 + nx + 2];
+  cw[i] = {m + cp * hsym3(a0, a1, a3, p), m + cp * hsym3(a0, a2, a3, p)};
+}
+
+template <typename T>
+hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T* cellw,
+                              hipStream_t s) {
+  k_elast_cell_weights<T><<<grid1d((size_t)nx * ny), kBlock, 0, s>>>(
+      nx, ny, xi, p, m, cp, reinterpret_cast<V2<T>*>(cellw));
+  return hipGetLastError();
+}
+template hipError_t elast_cell_weights<double>(int, int, const double*, int, double, double,
+                                               double*, hipStream_t);
+template hipError_t elast_cell_weights<float>(int, int, const float*, int, float, float, float*,
+                                              hipStream_t);
+
+// coarse triangle A (lower right) = fine (0,0).A + (1,0).A + (1,0).B + (1,1).A
+// coarse triangle B (upper left)  = fine (0,0).B + (0,1).A + (0,1).B + (1,1).B
+template <typename T>
+__global__ void k_restrict_weights(int nxc, int nyc, const V2<T>* __restrict__ f,
+                                   V2<T>* __restrict__ cw) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nxc * nyc) return;
+  const int cx = (int)(i % nxc), cy = (int)(i / nxc);
+  const int nxf = 2 * nxc;
+  const V2<T> f00 = f[(size_t)(2 * cy) * nxf + 2 * cx];
+  const V2<T> f10 = f[(size_t)(2 * cy) * nxf + 2 * cx + 1];
+  const V2<T> f01 = f[(size_t)(2 * cy + 1) * nxf + 2 * cx];
+  const V2<T> f11 = f[(size_t)(2 * cy + 1) * nxf + 2 * cx + 1];
+  cw[i] = {T(0.25) * (f00.x + f10.x + f10.y + f11.x), T(0.25) * (f00.y + f01.x + f01.y + f11.y)};
+}
+
+template <typename T>
+hipError_t elast_restrict_weights(int nxc, int nyc, const T* fine, T* coarse, hipStream_t s) {
+  k_restrict_weights<T><<<grid1d((size_t)nxc * nyc), kBlock, 0, s>>>(
+      nxc, nyc, reinterpret_cast<const V2<T>*>(fine), reinterpret_cast<V2<T>*>(coarse));
+  return hipGetLastError();
+}
+template hipError_t elast_restrict_weights<double>(int, int, const double*, double*, hipStream_t);
+template hipError_t elast_restrict_weights<float>(int, int, const float*, float*, hipStream_t);
+
+// ---------------------------------------------------------------- explicit stencils
+// local vertex offsets (dx, dy) and gradient factors (gx in 1/hx, gy in 1/hy) of the two
+// triangle types: A = (v0, v1, v3), B = (v0, v2, v3)
+__constant__ int c_tri_off[2][3][2] = {{{0, 0}, {1, 0}, {1, 1}}, {{0, 0}, {0, 1}, {1, 1}}};
+__constant__ double c_tri_grad[2][3][2] = {{{-1, 0}, {1, -1}, {0, 1}}, {{0, -1}, {-1, 1}, {1, 0}}};
+
+__device__ inline int st_index(int dx, int dy) { return (dy + 1) * 3 + (dx + 1); }
+
+template <typename T>
+__global__ void k_elast_to_stencil(ElastOpArgs<T> a, T* __restrict__ st,
+                                   double hx, double hy, double lam, double mu) {
+  const int nx = a.sh.nx, ny = a.sh.ny;
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
+  double acc[9][4];
+  for (int kk = 0; kk < 9; ++kk)
+    for (int e = 0; e < 4; ++e) acc[kk][e] = 0.0;
+  const bool vf = node_fixed(a.fixed, ix, iy, nx, ny);
+  if (vf) {
+    acc[4][0] = acc[4][3] = 1.0;
+  } else {
+    const double area = 0.5 * hx * hy;
+    for (int cyo = -1; cyo <= 0; ++cyo)
+      for (int cxo = -1; cxo <= 0; ++cxo) {
+        const int cx = ix + cxo, cy = iy + cyo;
+        if (cx < 0 || cy < 0 || cx >= nx || cy >= ny) continue;
+        double w[2];
+        if (a.xi) {
+          const size_t v0 = (size_t)cy * (nx + 1) + cx;
+          const double x0 = a.xi[v0], x1 = a.xi[v0 + 1], x2 = a.xi[v0 + nx + 1],
+                       x3 = a.xi[v0 + nx + 2];
+          w[0] = (double)a.m + (double)a.cp * hsym3(x0, x1, x3, a.p);
+          w[1] = (double)a.m + (double)a.cp * hsym3(x0, x2, x3, a.p);
+        } else {
+          w[0] = a.cellw[2 * ((size_t)cy * nx + cx)];
+          w[1] = a.cellw[2 * ((size_t)cy * nx + cx) + 1];
+        }
+        for (int t = 0; t < 2; ++t) {
+          int li = -1;
+          for (int q = 0; q < 3; ++q)
+            if (cx + c_tri_off[t][q][0] == ix && cy + c_tri_off[t][q][1] == iy) li = q;
+          if (li < 0) continue;
+          const double gix = c_tri_grad[t][li][0] / hx, giy = c_tri_grad[t][li][1] / hy;
+          for (int q = 0; q < 3; ++q) {
+            const int jx = cx + c_tri_off[t][q][0], jy = cy + c_tri_off[t][q][1];
+            if (node_fixed(a.fixed, jx, jy, nx, ny)) continue;
+            const double gjx = c_tri_grad[t][q][0] / hx, gjy = c_tri_grad[t][q][1] / hy;
+            const double gi[2] = {gix, giy}, gj[2] = {gjx, gjy};
+            const double dotg = gix * gjx + giy * gjy;
+            const int kk = st_index(jx - ix, jy - iy);
+            for (int p = 0; p < 2; ++p)
+              for (int r = 0; r < 2; ++r) {
+                double val = lam * gi[p] * gj[r] + mu * gi[r] * gj[p];
+                if (p == r) val += mu * dotg;
+                acc[kk][p * 2 + r] += w[t] * area * val;
+              }
+          }
+        }
+      }
+  }
+  for (int kk = 0; kk < 9; ++kk)
+    for (int e = 0; e < 4; ++e) st[(size_t)(kk * 4 + e) * nn + v] = (T)acc[kk][e];
+}
+
+// lam, mu and a = hy/hx are recovered from the force constants (lam_h = l/2, mu_h = m/2,
+// mu_a = m a/2); the element matrices depend on hx, hy only through a.
+template <>
+hipError_t elast_to_stencil<double>(const ElastOpArgs<double>& a, double* st, hipStream_t s) {
+  // the constants are scale free (a = hy/hx); use hx = 1, hy = a.
+  const double aa = a.k.mu_a / a.k.mu_h;  // = hy/hx
+  const double mu = 2 * a.k.mu_h, lam = 2 * a.k.lam_h;
+  const size_t nn = (size_t)(a.sh.nx + 1) * (a.sh.ny + 1);
+  k_elast_to_stencil<double><<<grid1d(nn), kBlock, 0, s>>>(a, st, 1.0, aa, lam, mu);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- sensitivity
+// b_v = -p(1-m) sum_{T∋v} e_T int_T xi^q lambda_v,  q = p-1,
+// int_T xi^q lambda_v = 2|T| q!/(q+3)! H_q(xi_a, xi_b, xi_c, xi_v)  (complete homogeneous)
+constexpr int kMaxQ = 8;
+
+__device__ __forceinline__ void hsym3_all(double a, double b, double c, int q, double (&H)[kMaxQ + 1]) {
+  double pa = 1.0, hab = 1.0, h = 1.0;
+  H[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k <= kMaxQ; ++k) {
+    if (k <= q) {
+      pa *= a;
+      hab = pa + b * hab;
+      h = hab + c * h;
+    }
+    H[k] = h;
+  }
+}
+
+__device__ __forceinline__ double hsym4_from3(const double (&H)[kMaxQ + 1], int q, double d) {
+  double acc = 1.0;  // H3_0
+#pragma unroll
+  for (int j = 1; j <= kMaxQ; ++j)
+    if (j <= q) acc = acc * d + H[j];
+  return acc;
+}
+
+struct SensArgs {
+  SweepShape sh;
+  int q;
+  double coef;  // -p(1-m) * 2|T| q!/(q+3)!
+  double lam, mu, hx, hy;
+};
+
+__global__ __launch_bounds__(kBlock) void k_elast_sensitivity(SensArgs a, const double* __restrict__ xi,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ out) {
+  using V = V2<double>;
+  const SweepShape sh = a.sh;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
+  const int c = strip * kStripCols - 1 + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= 1 && lane <= kStripCols;
+  const int r0 = chunk * sh.rows;
+  const int r1 = wave_ok ? min(r0 + sh.rows, ny + 1) : r0 - 1;
+  const V* U = reinterpret_cast<const V*>(u);
+  auto ld_u = [&](int iy) -> V {
+    if (colv && iy >= 0 && iy <= ny) return U[(size_t)iy * nvx + c];
+    return V{0.0, 0.0};
+  };
+  auto ld_xi = [&](int iy) -> double {
+    if (colv && iy >= 0 && iy <= ny) return xi[(size_t)iy * nvx + c];
+    return 0.0;
+  };
+  const double lam = a.lam, mu = a.mu, ihx = 1.0 / a.hx, ihy = 1.0 / a.hy;
+  auto energy = [&](double exx, double eyy, double gam) {
+    const double tr = exx + eyy;
+    return lam * tr * tr + 2 * mu * (exx * exx + eyy * eyy + 0.5 * gam * gam);
+  };
+  int cy = r0 - 1;
+  V ub = ld_u(cy), ubr = lane_down(ub);
+  double zb = ld_xi(cy), zbr = lane_down(zb);
+  double acc_b = 0.0;
+  for (; cy < r1; ++cy) {
+    const V ut = ld_u(cy + 1), utr = lane_down(ut);
+    const double zt = ld_xi(cy + 1), ztr = lane_down(zt);
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    double f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+    if (cell_ok) {
+      double H[kMaxQ + 1];
+      // triangle A = (v0 = ub, v1 = ubr, v3 = utr)
+      const double eA = energy((ubr.x - ub.x) * ihx, (utr.y - ubr.y) * ihy,
+                               (utr.x - ubr.x) * ihy + (ubr.y - ub.y) * ihx);
+      hsym3_all(zb, zbr, ztr, a.q, H);
+      f0 += eA * hsym4_from3(H, a.q, zb);
+      f1 += eA * hsym4_from3(H, a.q, zbr);
+      f3 += eA * hsym4_from3(H, a.q, ztr);
+      // triangle B = (v0 = ub, v2 = ut, v3 = utr)
+      const double eB = energy((utr.x - ut.x) * ihx, (ut.y - ub.y) * ihy,
+                               (ut.x - ub.x) * ihy + (utr.y - ut.y) * ihx);
+      hsym3_all(zb, zt, ztr, a.q, H);
+      f0 += eB * hsym4_from3(H, a.q, zb);
+      f2 += eB * hsym4_from3(H, a.q, zt);
+      f3 += eB * hsym4_from3(H, a.q, ztr);
+    }
+    acc_b += f0 + lane_up(f1);
+    const double acc_t = f2 + lane_up(f3);
+    if (cy >= r0 && outl) out[(size_t)cy * nvx + c] = a.coef * acc_b;
+    acc_b = acc_t;
+    ub = ut;
+    ubr = utr;
+    zb = zt;
+    zbr = ztr;
+  }
+}
+
+hipError_t elast_sensitivity(const SweepShape& sh, const double* xi, const double* u, int p,
+                             double m, double lam, double mu, double hx, double hy, double* out,
+                             hipStream_t s) {
+  SensArgs a;
+  a.sh = sh;
+  a.q = p - 1;
+  double fq = 1.0;
+  for (int i = 2; i <= a.q; ++i) fq *= i;
+  double f3 = 1.0;
+  for (int i = 2; i <= a.q + 3; ++i) f3 *= i;
+  a.coef = -p * (1.0 - m) * (hx * hy) * fq / f3;
+  a.lam = lam;
+  a.mu = mu;
+  a.hx = hx;
+  a.hy = hy;
+  k_elast_sensitivity<<<sh.blocks(), kBlock, 0, s>>>(a, xi, u, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- load vector
+// Body force / traction rules of the reference evaluated at the P2 interpolation points
+// (vertices and edge midpoints 0.5 xa + 0.5 xb) with IEEE semantics (no contraction), then
+// integrated exactly against the P1 test functions:
+//   int_T N_vertex(j) l_i = |T|/30 (j = i), -|T|/60 (j != i)
+//   int_T N_edge(jk) l_i  = 2|T|/15 (i in {j,k}), |T|/15 (otherwise)
+//   int_e N_a phi_a = L/6, int_e N_b phi_a = 0, int_e N_mid phi_a = L/3 (edge e = (a,b))
+__device__ __noinline__ void body_value(const LoadDesc& d, double x, double y, double& fx,
+                                        double& fy) {
+#pragma clang fp contract(off)
+  fx = 0.0;
+  fy = 0.0;
+  if (!d.has_body) return;
+  const double dx = x - d.bc[0], dy = y - d.bc[1];
+  const double dx2 = __dmul_rn(dx, dx), dy2 = __dmul_rn(dy, dy);
+  const double dist = __dsqrt_rn(__dadd_rn(dx2, dy2));
+  if (dist < d.br) {
+    fx = d.bv[0];
+    fy = d.bv[1];
+  }
+}
+
+__device__ __noinline__ void traction_value(const LoadDesc& d, double x, double y, double& fx,
+                                            double& fy) {
+#pragma clang fp contract(off)
+  const double eps = 3.0e-16;
+  fx = 0.0;
+  fy = 0.0;
+  for (int i = 0; i < d.ntr; ++i) {
+    bool on = false;
+    switch (d.tside[i]) {
+      case 0: on = (x == 0.0) && (y >= d.tlo[i] - eps) && (y <= d.thi[i] + eps); break;
+      case 1: on = (x == d.width) && (y >= d.tlo[i] - eps) && (y <= d.thi[i] + eps); break;
+      case 2: on = (y == d.height) && (x >= d.tlo[i] - eps) && (x <= d.thi[i] + eps); break;
+      default: on = (y == 0.0) && (x >= d.tlo[i] - eps) && (x <= d.thi[i] + eps); break;
+    }
+    if (on) {
+      fx = __dadd_rn(fx, d.tv[i][0]);
+      fy = __dadd_rn(fy, d.tv[i][1]);
+    }
+  }
+}
+
+__device__ __forceinline__ double vx(const LoadDesc& d, int ix) { return __dadd_rn(0.0, __dmul_rn(d.ab, (double)ix)); }
+__device__ __forceinline__ double vy(const LoadDesc& d, int iy) { return __dadd_rn(0.0, __dmul_rn(d.cd, (double)iy)); }
+__device__ __forceinline__ double mid(double a, double b) { return __dadd_rn(__dmul_rn(0.5, a), __dmul_rn(0.5, b)); }
+
+__global__ void k_assemble_load(LoadDesc d, double* __restrict__ F) {
+#pragma clang fp contract(off)
+  const int nx = d.nx, ny = d.ny;
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
+  double sx = 0.0, sy = 0.0;
+  if (d.has_body) {
+    for (int cyo = -1; cyo <= 0; ++cyo)
+      for (int cxo = -1; cxo <= 0; ++cxo) {
+        const int cx = ix + cxo, cy = iy + cyo;
+        if (cx < 0 || cy < 0 || cx >= nx || cy >= ny) continue;
+        for (int t = 0; t < 2; ++t) {
+          int li = -1;
+          int px[3], py[3];
+          for (int q = 0; q < 3; ++q) {
+            px[q] = cx + c_tri_off[t][q][0];
+            py[q] = cy + c_tri_off[t][q][1];
+            if (px[q] == ix && py[q] == iy) li = q;
+          }
+          if (li < 0) continue;
+          const double area = 0.5 * fabs((vx(d, px[1]) - vx(d, px[0])) * (vy(d, py[2]) - vy(d, py[0])) -
+                                         (vx(d, px[2]) - vx(d, px[0])) * (vy(d, py[1]) - vy(d, py[0])));
+          double ax = 0.0, ay = 0.0;
+          for (int q = 0; q < 3; ++q) {
+            double fx, fy;
+            body_value(d, vx(d, px[q]), vy(d, py[q]), fx, fy);
+            const double wv = (q == li) ? area / 30.0 : -area / 60.0;
+            ax += wv * fx;
+            ay += wv * fy;
+          }
+          for (int q = 0; q < 3; ++q) {  // edge opposite vertex q
+            const int e0 = (q + 1) % 3, e1 = (q + 2) % 3;
+            double fx, fy;
+            body_value(d, mid(vx(d, px[e0]), vx(d, px[e1])), mid(vy(d, py[e0]), vy(d, py[e1])), fx, fy);
+            const double we = (q == li) ? area / 15.0 : 2.0 * area / 15.0;
+            ax += we * fx;
+            ay += we * fy;
+          }
+          sx += ax;
+          sy += ay;
+        }
+      }
+  }
+  if (d.ntr > 0) {
+    // boundary edges containing v: (dir, other node) along each side v lies on
+    const double x0 = vx(d, ix), y0 = vy(d, iy);
+    double tvx, tvy;
+    traction_value(d, x0, y0, tvx, tvy);
+    auto edge = [&](int jx, int jy) {
+      const double x1 = vx(d, jx), y1 = vy(d, jy);
+      double tmx, tmy;
+      traction_value(d, mid(x0, x1), mid(y0, y1), tmx, tmy);
+      const double len = (jx != ix) ? fabs(x1 - x0) : fabs(y1 - y0);
+      sx += len * (tvx / 6.0 + tmx / 3.0);
+      sy += len * (tvy / 6.0 + tmy / 3.0);
+    };
+    if (iy == 0 || iy == ny) {
+      if (ix > 0) edge(ix - 1, iy);
+      if (ix < nx) edge(ix + 1, iy);
+    }
+    if (ix == 0 || ix == nx) {
+      if (iy > 0) edge(ix, iy - 1);
+      if (iy < ny) edge(ix, iy + 1);
+    }
+  }
+  F[2 * v] = sx;
+  F[2 * v + 1] = sy;
+}
+
+hipError_t elast_assemble_load(const LoadDesc& d, double* F, hipStream_t s) {
+  const size_t nn = (size_t)(d.nx + 1) * (d.ny + 1);
+  k_assemble_load<<<grid1d(nn), kBlock, 0, s>>>(d, F);
+  return hipGetLastError();
+}
+
+// ============================================================================ filter
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterConst<T> k,
+                                                         const T* __restrict__ x,
+                                                         const T* __restrict__ b,
+                                                         T* __restrict__ y, T omega, RedSlot red,
+                                                         int want_dot) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
+  const int c = strip * kStripCols - 1 + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= 1 && lane <= kStripCols;
+  const int r0 = chunk * sh.rows;
+  const int r1 = wave_ok ? min(r0 + sh.rows, ny + 1) : r0 - 1;
+  auto ld = [&](int iy) -> T {
+    if (OP != kSmoothZero && colv && iy >= 0 && iy <= ny) return x[(size_t)iy * nvx + c];
+    return T(0);
+  };
+  const T ka = k.ka, kb = k.kb, mA = k.mA;
+  const T dfull = ka + kb + T(4) * mA, dhalf = ka + kb + T(2) * mA;
+  int cy = r0 - 1;
+  T ub = ld(cy), ubr = lane_down(ub);
+  T acc_b = T(0), d_b = T(0);
+  double dot = 0.0;
+  for (; cy < r1; ++cy) {
+    const T ut = ld(cy + 1), utr = lane_down(ut);
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    const T w = cell_ok ? T(1) : T(0);
+    T acc_t = T(0);
+    if constexpr (OP != kSmoothZero) {
+      const T u0 = ub, u1 = ubr, u2 = ut, u3 = utr;
+      const T SA = u0 + u1 + u3, SB = u0 + u2 + u3;
+      const T f0 = w * (ka * (u0 - u1) + mA * (u0 + SA) + kb * (u0 - u2) + mA * (u0 + SB));
+      const T f1 = w * (ka * (u1 - u0) + kb * (u1 - u3) + mA * (u1 + SA));
+      const T f2 = w * (kb * (u2 - u0) + ka * (u2 - u3) + mA * (u2 + SB));
+      const T f3 = w * (kb * (u3 - u1) + mA * (u3 + SA) + ka * (u3 - u2) + mA * (u3 + SB));
+      acc_b += f0 + lane_up(f1);
+      acc_t = f2 + lane_up(f3);
+    }
+    T d_t = T(0);
+    if constexpr (OP == kSmooth || OP == kSmoothZero) {
+      const T wl = lane_up(w);
+      d_b += w * dfull + wl * dhalf;
+      d_t = w * dhalf + wl * dfull;
+    }
+    if (cy >= r0 && outl) {
+      const size_t idx = (size_t)cy * nvx + c;
+      if constexpr (OP == kApply) {
+        y[idx] = acc_b;
+        if (want_dot) dot += (double)ub * (double)acc_b;
+      } else if constexpr (OP == kResidual) {
+        y[idx] = b[idx] - acc_b;
+      } else {
+        y[idx] = ub + omega * (b[idx] - acc_b) / d_b;
+      }
+    }
+    acc_b = acc_t;
+    d_b = d_t;
+    ub = ut;
+    ubr = utr;
+  }
+  if constexpr (OP == kApply) {
+    if (want_dot) {
+      const double v[1] = {dot};
+      const Reduction r[1] = {{red.partials, red.counter, red.result}};
+      block_reduce_finalize<1>(v, r);
+    }
+  }
+}
+
+template <typename T>
+hipError_t filter_sweep(const SweepShape& sh, const FilterConst<T>& k, int op, const T* x,
+                        const T* b, T* y, T omega, const RedSlot* dot_xy, hipStream_t s) {
+  const int nb = sh.blocks();
+  RedSlot r = dot_xy ? *dot_xy : RedSlot{nullptr, nullptr, nullptr};
+  const int wd = dot_xy ? 1 : 0;
+  switch (op) {
+    case kApply: k_filter_sweep<T, kApply><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
+    case kResidual: k_filter_sweep<T, kResidual><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
+    case kSmooth: k_filter_sweep<T, kSmooth><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
+    default: k_filter_sweep<T, kSmoothZero><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
+  }
+  return hipGetLastError();
+}
+template hipError_t filter_sweep<double>(const SweepShape&, const FilterConst<double>&, int,
+                                         const double*, const double*, double*, double,
+                                         const RedSlot*, hipStream_t);
+template hipError_t filter_sweep<float>(const SweepShape&, const FilterConst<float>&, int,
+                                        const float*, const float*, float*, float, const RedSlot*,
+                                        hipStream_t);
+
+template <typename T>
+__global__ void k_filter_to_stencil(int nx, int ny, FilterConst<T> k, T* __restrict__ st) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // element matrices in units: stiffness eps^2 |T| gi.gj with gx in 1/hx, gy in 1/hy gives
+  // ka (x-x products) and kb (y-y products); mass |T|/12 (1 + delta_ij)
+  for (int cyo = -1; cyo <= 0; ++cyo)
+    for (int cxo = -1; cxo <= 0; ++cxo) {
+      const int cx = ix + cxo, cy = iy + cyo;
+      if (cx < 0 || cy < 0 || cx >= nx || cy >= ny) continue;
+      for (int t = 0; t < 2; ++t) {
+        int li = -1;
+        for (int q = 0; q < 3; ++q)
+          if (cx + c_tri_off[t][q][0] == ix && cy + c_tri_off[t][q][1] == iy) li = q;
+        if (li < 0) continue;
+        for (int q = 0; q < 3; ++q) {
+          const int jx = cx + c_tri_off[t][q][0], jy = cy + c_tri_off[t][q][1];
+          const double kx = c_tri_grad[t][li][0] * c_tri_grad[t][q][0];
+          const double ky = c_tri_grad[t][li][1] * c_tri_grad[t][q][1];
+          // eps^2 |T| (gix gjx/hx^2 + giy gjy/hy^2) = ka kx + kb ky
+          double val = (double)k.ka * kx + (double)k.kb * ky;
+          val += (double)k.mA * (q == li ? 2.0 : 1.0);
+          acc[st_index(jx - ix, jy - iy)] += val;
+        }
+      }
+    }
+  for (int kk = 0; kk < 9; ++kk) st[(size_t)kk * nn + v] = (T)acc[kk];
+}
+
+template <typename T>
+hipError_t filter_to_stencil(int nx, int ny, const FilterConst<T>& k, T* st, hipStream_t s) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  k_filter_to_stencil<T><<<grid1d(nn), kBlock, 0, s>>>(nx, ny, k, st);
+  return hipGetLastError();
+}
+template hipError_t filter_to_stencil<double>(int, int, const FilterConst<double>&, double*,
+                                              hipStream_t);
+
+// ============================================================================ transfers
+template <typename T, int BS>
+struct Blk {
+  T v[BS];
+};
+
+template <typename T, int BS>
+__device__ __forceinline__ Blk<T, BS> ldb(const T* p, size_t node) {
+  Blk<T, BS> r;
+#pragma unroll
+  for (int i = 0; i < BS; ++i) r.v[i] = p[BS * node + i];
+  return r;
+}
+
+template <typename T, int BS>
+__global__ void k_restrict_nested(int nxc, int nyc, unsigned fixed, const T* __restrict__ rf,
+                                  T* __restrict__ bc) {
+  const size_t nn = (size_t)(nxc + 1) * (nyc + 1);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  const int nxf = 2 * nxc, nyf = 2 * nyc, nvf = nxf + 1;
+  T acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = T(0);
+  if (!node_fixed(fixed, I, J, nxc, nyc)) {
+    const int fx = 2 * I, fy = 2 * J;
+    const int off[7][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1}};
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      const int gx = fx + off[t][0], gy = fy + off[t][1];
+      if (gx < 0 || gy < 0 || gx > nxf || gy > nyf) continue;
+      const T w = (t == 0) ? T(1) : T(0.5);
+      const size_t g = (size_t)gy * nvf + gx;
+#pragma unroll
+      for (int q = 0; q < BS; ++q) acc[q] += w * rf[BS * g + q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) bc[BS * i + q] = acc[q];
+}
+
+template <typename T, int BS>
+__global__ void k_prolong_nested(int nxc, int nyc, unsigned fixed, const T* __restrict__ ec,
+                                 T* __restrict__ xf) {
+  const int nxf = 2 * nxc, nyf = 2 * nyc;
+  const size_t nn = (size_t)(nxf + 1) * (nyf + 1);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int fx = (int)(i % (nxf + 1)), fy = (int)(i / (nxf + 1));
+  if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
+  const int nvc = nxc + 1;
+  const int ox = fx & 1, oy = fy & 1;
+  const int I0 = fx >> 1, J0 = fy >> 1;
+  T add[BS];
+  if (!ox && !oy) {
+#pragma unroll
+    for (int q = 0; q < BS; ++q) add[q] = ec[BS * ((size_t)J0 * nvc + I0) + q];
+  } else {
+    const int I1 = I0 + ox, J1 = J0 + oy;  // (odd,even): x-neighbour; (even,odd): y; (odd,odd): diagonal
+#pragma unroll
+    for (int q = 0; q < BS; ++q)
+      add[q] = T(0.5) * (ec[BS * ((size_t)J0 * nvc + I0) + q] + ec[BS * ((size_t)J1 * nvc + I1) + q]);
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) xf[BS * i + q] += add[q];
+}
+
+template <typename T, int BS>
+hipError_t restrict_nested(int nxc, int nyc, unsigned fixed, const T* rf, T* bc, hipStream_t s) {
+  k_restrict_nested<T, BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1)), kBlock, 0, s>>>(nxc, nyc, fixed, rf, bc);
+  return hipGetLastError();
+}
+template <typename T, int BS>
+hipError_t prolong_nested(int nxc, int nyc, unsigned fixed, const T* ec, T* xf, hipStream_t s) {
+  k_prolong_nested<T, BS><<<grid1d((size_t)(2 * nxc + 1) * (2 * nyc + 1)), kBlock, 0, s>>>(nxc, nyc, fixed, ec, xf);
+  return hipGetLastError();
+}
+
+// 1-D interpolation from coarse nodes to fine node f (nf fine cells, nc coarse cells).
+struct Interp1 {
+  int n;
+  int c[2];
+  double w[2];
+};
+
+__device__ __forceinline__ Interp1 interp1(int f, int nf, int nc) {
+  Interp1 r;
+  if (nf == nc) {  // not coarsened in this direction
+    r.n = 1; r.c[0] = f; r.w[0] = 1.0;
+  } else if ((f & 1) == 0) {
+    r.n = 1; r.c[0] = f >> 1; r.w[0] = 1.0;
+  } else if (f == nf) {  // odd fine count: last fine node is the last coarse node
+    r.n = 1; r.c[0] = nc; r.w[0] = 1.0;
+  } else {
+    r.n = 2; r.c[0] = (f - 1) >> 1; r.c[1] = (f + 1) >> 1; r.w[0] = r.w[1] = 0.5;
+  }
+  return r;
+}
+
+__device__ __forceinline__ int cpos(int I, int nf, int nc) { return nf == nc ? I : min(2 * I, nf); }
+
+__device__ __forceinline__ double interp1_weight(int f, int nf, int nc, int I) {
+  const Interp1 r = interp1(f, nf, nc);
+  double w = 0.0;
+  for (int t = 0; t < r.n; ++t)
+    if (r.c[t] == I) w += r.w[t];
+  return w;
+}
+
+template <typename T, int BS>
+__global__ void k_restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
+                                    const T* __restrict__ rf, T* __restrict__ bc) {
+  const size_t nn = (size_t)(nxc + 1) * (nyc + 1);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  double acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = 0.0;
+  if (!node_fixed(fixed, I, J, nxc, nyc)) {
+    const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
+    for (int fy = py - 1; fy <= py + 1; ++fy) {
+      if (fy < 0 || fy > nyf) continue;
+      const double wy = interp1_weight(fy, nyf, nyc, J);
+      if (wy == 0.0) continue;
+      for (int fx = px - 1; fx <= px + 1; ++fx) {
+        if (fx < 0 || fx > nxf) continue;
+        const double w = wy * interp1_weight(fx, nxf, nxc, I);
+        if (w == 0.0) continue;
+        const size_t g = (size_t)fy * (nxf + 1) + fx;
+#pragma unroll
+        for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) bc[BS * i + q] = (T)acc[q];
+}
+
+template <typename T, int BS>
+__global__ void k_prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
+                                   const T* __restrict__ ec, T* __restrict__ xf) {
+  const size_t nn = (size_t)(nxf + 1) * (nyf + 1);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int fx = (int)(i % (nxf + 1)), fy = (int)(i / (nxf + 1));
+  if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
+  const Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
+  T add[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) add[q] = T(0);
+  for (int b = 0; b < iy.n; ++b)
+    for (int a = 0; a < ix.n; ++a) {
+      const T w = (T)(ix.w[a] * iy.w[b]);
+      const size_t c = (size_t)iy.c[b] * (nxc + 1) + ix.c[a];
+#pragma unroll
+      for (int q = 0; q < BS; ++q) add[q] += w * ec[BS * c + q];
+    }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) xf[BS * i + q] += add[q];
+}
+
+template <typename T, int BS>
+hipError_t restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* rf,
+                             T* bc, hipStream_t s) {
+  k_restrict_bilinear<T, BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1)), kBlock, 0, s>>>(
+      nxf, nyf, nxc, nyc, fixed, rf, bc);
+  return hipGetLastError();
+}
+template <typename T, int BS>
+hipError_t prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* ec,
+                            T* xf, hipStream_t s) {
+  k_prolong_bilinear<T, BS><<<grid1d((size_t)(nxf + 1) * (nyf + 1)), kBlock, 0, s>>>(
+      nxf, nyf, nxc, nyc, fixed, ec, xf);
+  return hipGetLastError();
+}
+
+#define TMX_INST_TRANSFER(T, BS)                                                                   \
+  template hipError_t restrict_nested<T, BS>(int, int, unsigned, const T*, T*, hipStream_t);      \
+  template hipError_t prolong_nested<T, BS>(int, int, unsigned, const T*, T*, hipStream_t);       \
+  template hipError_t restrict_bilinear<T, BS>(int, int, int, int, unsigned, const T*, T*,        \
+                                               hipStream_t);                                       \
+  template hipError_t prolong_bilinear<T, BS>(int, int, int, int, unsigned, const T*, T*,         \
+                                              hipStream_t);
+TMX_INST_TRANSFER(double, 1)
+TMX_INST_TRANSFER(double, 2)
+TMX_INST_TRANSFER(float, 1)
+TMX_INST_TRANSFER(float, 2)
+
+// ============================================================================ 9-point levels
+// st layout (SoA): st[(k*BS*BS + a*BS + b) * nn + node], k = (dy+1)*3 + (dx+1).
+template <typename T, int BS, int OP>
+__global__ void k_stencil_op(int nx, int ny, const T* __restrict__ st, const T* __restrict__ x,
+                             const T* __restrict__ b, T* __restrict__ y, T omega) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
+  T acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = T(0);
+  if (OP != kSmoothZero) {
+#pragma unroll
+    for (int kk = 0; kk < 9; ++kk) {
+      const int dx = kk % 3 - 1, dy = kk / 3 - 1;
+      const int jx = ix + dx, jy = iy + dy;
+      if (jx < 0 || jy < 0 || jx > nx || jy > ny) continue;
+      const size_t j = (size_t)jy * (nx + 1) + jx;
+      T xj[BS];
+#pragma unroll
+      for (int q = 0; q < BS; ++q) xj[q] = x[BS * j + q];
+#pragma unroll
+      for (int a = 0; a < BS; ++a)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) acc[a] += st[(size_t)(kk * BS * BS + a * BS + c) * nn + v] * xj[c];
+    }
+  }
+  if (OP == kApply) {
+#pragma unroll
+    for (int q = 0; q < BS; ++q) y[BS * v + q] = acc[q];
+    return;
+  }
+  T res[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) res[q] = b[BS * v + q] - acc[q];
+  if (OP == kResidual) {
+#pragma unroll
+    for (int q = 0; q < BS; ++q) y[BS * v + q] = res[q];
+    return;
+  }
+  T d[BS];
+  if constexpr (BS == 1) {
+    d[0] = res[0] / st[(size_t)4 * nn + v];
+  } else {
+    const T a00 = st[(size_t)(16 + 0) * nn + v], a01 = st[(size_t)(16 + 1) * nn + v];
+    const T a10 = st[(size_t)(16 + 2) * nn + v], a11 = st[(size_t)(16 + 3) * nn + v];
+    const T inv = T(1) / (a00 * a11 - a01 * a10);
+    d[0] = (a11 * res[0] - a01 * res[1]) * inv;
+    d[1] = (a00 * res[1] - a10 * res[0]) * inv;
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) {
+    const T xo = (OP == kSmoothZero) ? T(0) : x[BS * v + q];
+    y[BS * v + q] = xo + omega * d[q];
+  }
+}
+
+template <typename T, int BS>
+hipError_t stencil_op(int nx, int ny, const T* st, int op, const T* x, const T* b, T* y, T omega,
+                      hipStream_t s) {
+  const unsigned g = grid1d((size_t)(nx + 1) * (ny + 1));
+  switch (op) {
+    case kApply: k_stencil_op<T, BS, kApply><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    case kResidual: k_stencil_op<T, BS, kResidual><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    case kSmooth: k_stencil_op<T, BS, kSmooth><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    default: k_stencil_op<T, BS, kSmoothZero><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+  }
+  return hipGetLastError();
+}
+template hipError_t stencil_op<double, 1>(int, int, const double*, int, const double*,
+                                          const double*, double*, double, hipStream_t);
+template hipError_t stencil_op<double, 2>(int, int, const double*, int, const double*,
+                                          const double*, double*, double, hipStream_t);
+template hipError_t stencil_op<float, 1>(int, int, const float*, int, const float*, const float*,
+                                         float*, float, hipStream_t);
+template hipError_t stencil_op<float, 2>(int, int, const float*, int, const float*, const float*,
+                                         float*, float, hipStream_t);
+
+// Galerkin product, one thread per coarse node.  Interpolation from coarse to fine node g:
+// nested P1 (x-, y- and diagonal midpoints) or tensor-product 1-D linear (bilinear).
+struct Interp2 {
+  int n;
+  int cx[4], cy[4];
+  double w[4];
+};
+
+__device__ Interp2 interp2(int gx, int gy, int nxf, int nyf, int nxc, int nyc, int nested) {
+  Interp2 r;
+  if (nested) {
+    const int ox = gx & 1, oy = gy & 1;
+    if (!ox && !oy) {
+      r.n = 1; r.cx[0] = gx >> 1; r.cy[0] = gy >> 1; r.w[0] = 1.0;
+    } else {
+      r.n = 2;
+      r.cx[0] = gx >> 1; r.cy[0] = gy >> 1;
+      r.cx[1] = (gx >> 1) + ox; r.cy[1] = (gy >> 1) + oy;
+      r.w[0] = r.w[1] = 0.5;
+    }
+    return r;
+  }
+  const Interp1 ix = interp1(gx, nxf, nxc), iy = interp1(gy, nyf, nyc);
+  r.n = 0;
+  for (int b = 0; b < iy.n; ++b)
+    for (int a = 0; a < ix.n; ++a) {
+      r.cx[r.n] = ix.c[a]; r.cy[r.n] = iy.c[b]; r.w[r.n] = ix.w[a] * iy.w[b];
+      ++r.n;
+    }
+  return r;
+}
+
+template <int BS>
+__global__ void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
+                              const double* __restrict__ stf, double* __restrict__ stc) {
+  const size_t nnf = (size_t)(nxf + 1) * (nyf + 1);
+  const size_t nnc = (size_t)(nxc + 1) * (nyc + 1);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnc) return;
+  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  constexpr int B2 = BS * BS;
+  double acc[9][B2];
+  for (int k = 0; k < 9; ++k)
+    for (int e = 0; e < B2; ++e) acc[k][e] = 0.0;
+  if (node_fixed(fixed, I, J, nxc, nyc)) {
+    for (int a = 0; a < BS; ++a) acc[4][a * BS + a] = 1.0;
+  } else {
+    const int px = nested ? 2 * I : cpos(I, nxf, nxc);
+    const int py = nested ? 2 * J : cpos(J, nyf, nyc);
+    for (int fy = py - 1; fy <= py + 1; ++fy)
+      for (int fx = px - 1; fx <= px + 1; ++fx) {
+        if (fx < 0 || fy < 0 || fx > nxf || fy > nyf) continue;
+        const Interp2 pf = interp2(fx, fy, nxf, nyf, nxc, nyc, nested);
+        double wf = 0.0;
+        for (int t = 0; t < pf.n; ++t)
+          if (pf.cx[t] == I && pf.cy[t] == J) wf += pf.w[t];
+        if (wf == 0.0) continue;
+        const size_t f = (size_t)fy * (nxf + 1) + fx;
+        for (int k = 0; k < 9; ++k) {
+          const int gx = fx + k % 3 - 1, gy = fy + k / 3 - 1;
+          if (gx < 0 || gy < 0 || gx > nxf || gy > nyf) continue;
+          double A[B2];
+          bool nz = false;
+          for (int e = 0; e < B2; ++e) {
+            A[e] = stf[(size_t)(k * B2 + e) * nnf + f];
+            nz |= (A[e] != 0.0);
+          }
+          if (!nz) continue;
+          const Interp2 pg = interp2(gx, gy, nxf, nyf, nxc, nyc, nested);
+          for (int t = 0; t < pg.n; ++t) {
+            const int dx = pg.cx[t] - I, dy = pg.cy[t] - J;
+            if (dx < -1 || dx > 1 || dy < -1 || dy > 1) continue;  // cannot happen for these P
+            if (node_fixed(fixed, pg.cx[t], pg.cy[t], nxc, nyc)) continue;
+            const double w = wf * pg.w[t];
+            const int kc = st_index(dx, dy);
+            for (int e = 0; e < B2; ++e) acc[kc][e] += w * A[e];
+          }
+        }
+      }
+  }
+  for (int k = 0; k < 9; ++k)
+    for (int e = 0; e < B2; ++e) stc[(size_t)(k * B2 + e) * nnc + i] = acc[k][e];
+}
+
+template <int BS>
+hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
+                       const double* stf, double* stc, hipStream_t s) {
+  k_stencil_rap<BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1), 64), 64, 0, s>>>(
+      nxf, nyf, nxc, nyc, nested, fixed, stf, stc);
+  return hipGetLastError();
+}
+template hipError_t stencil_rap<1>(int, int, int, int, int, unsigned, const double*, double*,
+                                   hipStream_t);
+template hipError_t stencil_rap<2>(int, int, int, int, int, unsigned, const double*, double*,
+                                   hipStream_t);
+
+template <int BS>
+__global__ void k_stencil_to_dense(int nx, int ny, const double* __restrict__ st,
+                                   double* __restrict__ dense) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t n = BS * nn;
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
+  for (int k = 0; k < 9; ++k) {
+    const int jx = ix + k % 3 - 1, jy = iy + k / 3 - 1;
+    if (jx < 0 || jy < 0 || jx > nx || jy > ny) continue;
+    const size_t j = (size_t)jy * (nx + 1) + jx;
+    for (int a = 0; a < BS; ++a)
+      for (int b = 0; b < BS; ++b)
+        dense[(BS * v + a) * n + BS * j + b] = st[(size_t)(k * BS * BS + a * BS + b) * nn + v];
+  }
+}
+
+template <int BS>
+hipError_t stencil_to_dense(int nx, int ny, const double* st, double* dense, hipStream_t s) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  hipError_t e = hipMemsetAsync(dense, 0, sizeof(double) * BS * nn * BS * nn, s);
+  if (e != hipSuccess) return e;
+  k_stencil_to_dense<BS><<<grid1d(nn), kBlock, 0, s>>>(nx, ny, st, dense);
+  return hipGetLastError();
+}
+template hipError_t stencil_to_dense<1>(int, int, const double*, double*, hipStream_t);
+template hipError_t stencil_to_dense<2>(int, int, const double*, double*, hipStream_t);
+
+// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting), one workgroup.
+__global__ __launch_bounds__(1024) void k_dense_invert(int n, double* __restrict__ a) {
+  extern __shared__ double col[];
+  for (int k = 0; k < n; ++k) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = a[(size_t)i * n + k];
+    __syncthreads();
+    const double piv = col[k];
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+      a[(size_t)k * n + j] = (j == k) ? 1.0 / piv : a[(size_t)k * n + j] / piv;
+    __syncthreads();
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      if (i == k) continue;
+      if (j == k)
+        a[e] = -col[i] * a[(size_t)k * n + k];
+      else
+        a[e] -= col[i] * a[(size_t)k * n + j];
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
+  k_dense_invert<<<1, 1024, sizeof(double) * n, s>>>(n, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_dense_apply(int n, const double* __restrict__ ainv, const T* __restrict__ b,
+                              T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s += ainv[(size_t)j * n + i] * (double)b[j];  // symmetric inverse
+  x[i] = (T)s;
+}
+
+template <typename T>
+hipError_t dense_apply(int n, const double* ainv, const T* b, T* x, hipStream_t s) {
+  k_dense_apply<T><<<(n + 63) / 64, 64, 0, s>>>(n, ainv, b, x);
+  return hipGetLastError();
+}
+template hipError_t dense_apply<double>(int, const double*, const double*, double*, hipStream_t);
+template hipError_t dense_apply<float>(int, const double*, const float*, float*, hipStream_t);
+
+template <typename TI, typename TO>
+__global__ void k_convert(size_t n, const TI* __restrict__ in, TO* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (TO)in[i];
+}
+template <typename TI, typename TO>
+hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s) {
+  k_convert<TI, TO><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, in, out);
+  return hipGetLastError();
+}
+template hipError_t convert<double, float>(size_t, const double*, float*, hipStream_t);
+template hipError_t convert<float, double>(size_t, const float*, double*, hipStream_t);
+
+// ============================================================================ vectors
+constexpr int kVecBlocks = 1024;
+
+static inline unsigned vec_grid(size_t n) {
+  return (unsigned)std::min<size_t>(kVecBlocks, std::max<size_t>(1, (n + kBlock - 1) / kBlock));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(size_t n, const T* __restrict__ a,
+                                                const T* __restrict__ b, RedSlot r) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    s += (double)a[i] * (double)b[i];
+  const double v[1] = {s};
+  const Reduction red[1] = {{r.partials, r.counter, r.result}};
+  block_reduce_finalize<1>(v, red);
+}
+
+hipError_t dot(size_t n, const double* a, const double* b, const RedSlot& r, hipStream_t s) {
+  k_dot<double><<<vec_grid(n), kBlock, 0, s>>>(n, a, b, r);
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t dot_t(size_t n, const T* a, const T* b, const RedSlot& r, hipStream_t s) {
+  k_dot<T><<<vec_grid(n), kBlock, 0, s>>>(n, a, b, r);
+  return hipGetLastError();
+}
+template hipError_t dot_t<double>(size_t, const double*, const double*, const RedSlot&, hipStream_t);
+template hipError_t dot_t<float>(size_t, const float*, const float*, const RedSlot&, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_xr(size_t n, const double* __restrict__ sc,
+                                                          int i_rz, int i_pq, T* __restrict__ x,
+                                                          T* __restrict__ r, const T* __restrict__ p,
+                                                          const T* __restrict__ q, RedSlot rr) {
+  const double alpha = sc[i_rz] / sc[i_pq];
+  const T al = (T)alpha;
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    x[i] += al * p[i];
+    const T ri = r[i] - al * q[i];
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  const double v[1] = {s};
+  const Reduction red[1] = {{rr.partials, rr.counter, rr.result}};
+  block_reduce_finalize<1>(v, red);
+}
+
+template <typename T>
+hipError_t pcg_update_xr(size_t n, const double* sc, int i_rz, int i_pq, T* x, T* r, const T* p,
+                         const T* q, const RedSlot& rr, hipStream_t s) {
+  k_pcg_update_xr<T><<<vec_grid(n), kBlock, 0, s>>>(n, sc, i_rz, i_pq, x, r, p, q, rr);
+  return hipGetLastError();
+}
+template hipError_t pcg_update_xr<double>(size_t, const double*, int, int, double*, double*,
+                                          const double*, const double*, const RedSlot&, hipStream_t);
+template hipError_t pcg_update_xr<float>(size_t, const double*, int, int, float*, float*,
+                                         const float*, const float*, const RedSlot&, hipStream_t);
+
+template <typename T>
+__global__ void k_pcg_update_p(size_t n, const double* __restrict__ sc, int i_rznew, int i_rz,
+                               const T* __restrict__ z, T* __restrict__ p) {
+  const T beta = (T)(sc[i_rznew] / sc[i_rz]);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+template <typename T>
+hipError_t pcg_update_p(size_t n, double* sc, int i_rznew, int i_rz, const T* z, T* p,
+                        hipStream_t s) {
+  k_pcg_update_p<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, sc, i_rznew, i_rz, z, p);
+  return hipGetLastError();
+}
+template hipError_t pcg_update_p<double>(size_t, double*, int, int, const double*, double*, hipStream_t);
+template hipError_t pcg_update_p<float>(size_t, double*, int, int, const float*, float*, hipStream_t);
+
+template <typename T>
+__global__ void k_copy(size_t n, const T* __restrict__ a, T* __restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+template <typename T>
+hipError_t copy_vec(size_t n, const T* a, T* b, hipStream_t s) {
+  k_copy<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, a, b);
+  return hipGetLastError();
+}
+template hipError_t copy_vec<double>(size_t, const double*, double*, hipStream_t);
+template hipError_t copy_vec<float>(size_t, const float*, float*, hipStream_t);
+
+template <typename T>
+__global__ void k_axpy_scaled(size_t n, const double* __restrict__ sc, int i, const T* __restrict__ d,
+                              double* __restrict__ x) {
+  const double scale = sqrt(sc[i]);
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    x[k] += scale * (double)d[k];
+}
+template <typename T>
+hipError_t axpy_scaled(size_t n, const double* sc, int i, const T* d, double* x, hipStream_t s) {
+  k_axpy_scaled<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, sc, i, d, x);
+  return hipGetLastError();
+}
+template hipError_t axpy_scaled<float>(size_t, const double*, int, const float*, double*, hipStream_t);
+template hipError_t axpy_scaled<double>(size_t, const double*, int, const double*, double*, hipStream_t);
+
+template <typename T>
+__global__ void k_scale_to(size_t n, const double* __restrict__ sc, int i, const double* __restrict__ in,
+                           T* __restrict__ out) {
+  const double inv = 1.0 / sqrt(sc[i]);
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    out[k] = (T)(in[k] * inv);
+}
+template <typename T>
+hipError_t scale_to(size_t n, const double* sc, int i, const double* in, T* out, hipStream_t s) {
+  k_scale_to<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, sc, i, in, out);
+  return hipGetLastError();
+}
+template hipError_t scale_to<float>(size_t, const double*, int, const double*, float*, hipStream_t);
+template hipError_t scale_to<double>(size_t, const double*, int, const double*, double*, hipStream_t);
+
+__global__ void k_fill_zero(size_t n, unsigned* __restrict__ p) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+hipError_t fill_zero_bytes(void* p, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 4;
+  k_fill_zero<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, static_cast<unsigned*>(p));
+  return hipGetLastError();
+}
+
+__global__ void k_mask_fixed(int nx, int ny, unsigned fixed, double* __restrict__ v) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int ix = (int)(i % (nx + 1)), iy = (int)(i / (nx + 1));
+  if (node_fixed(fixed, ix, iy, nx, ny)) {
+    v[2 * i] = 0.0;
+    v[2 * i + 1] = 0.0;
+  }
+}
+hipError_t mask_fixed(int nx, int ny, unsigned fixed, double* v, hipStream_t s) {
+  k_mask_fixed<<<grid1d((size_t)(nx + 1) * (ny + 1)), kBlock, 0, s>>>(nx, ny, fixed, v);
+  return hipGetLastError();
+}
+
+__global__ void k_fill(size_t n, double val, double* __restrict__ p) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = val;
+}
+hipError_t fill(size_t n, double val, double* p, hipStream_t s) {
+  k_fill<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, val, p);
+  return hipGetLastError();
+}
+
+}  // namespace tmx

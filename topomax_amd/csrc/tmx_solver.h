@@ -1,0 +1,185 @@
+// Host-side multigrid hierarchy and Krylov drivers for topomax_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "tmx_kernels.h"
+
+namespace tmx {
+
+// ------------------------------------------------------------------ errors
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(const std::string& m);
+
+#define TMX_HIP(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      ::tmx::set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr);  \
+      return -2;                                                                             \
+    }                                                                                        \
+  } while (0)
+
+#define TMX_TRY(expr)        \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+// ------------------------------------------------------------------ device buffers
+template <typename T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  ~DevBuf() { reset(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { reset(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  int alloc(size_t n) {
+    reset();
+    if (n == 0) return 0;
+    if (hipMalloc(&p_, n * sizeof(T)) != hipSuccess) {
+      p_ = nullptr;
+      set_error("hipMalloc failed for " + std::to_string(n * sizeof(T)) + " bytes");
+      return -2;
+    }
+    n_ = n;
+    return 0;
+  }
+  void reset() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Reduction scratch and device scalars of one stream of work.
+struct Workspace {
+  DevBuf<double> partials;
+  DevBuf<unsigned> counter;
+  DevBuf<double> scalars;
+  double* host = nullptr;  // pinned mirror of scalars
+  static constexpr int kScalars = 64;
+  int init(hipStream_t s);
+  ~Workspace();
+  RedSlot slot(int result) const {
+    return RedSlot{partials.get(), counter.get(), scalars.get() + result};
+  }
+  double* sc() const { return scalars.get(); }
+  int read(hipStream_t s, int first, int count);  // scalars[first..) -> host[first..)
+};
+
+// scalar slots
+enum : int {
+  kScBB = 0, kScRZ0 = 1, kScRZ1 = 2, kScPQ = 3, kScRR = 4, kScRR64 = 5, kScJ = 6,
+  kScInt = 7, kScTmp = 8
+};
+
+enum LevelKind : int { kElem = 0, kStencil = 1, kDense = 2 };
+enum Phys : int { kElasticity = 0, kFilter = 1 };
+
+struct Level {
+  int nx = 0, ny = 0;
+  double hx = 0, hy = 0;
+  int kind = kElem;
+  int nested_from_prev = 0;  // transfer from level-1 to this level is nested P1
+  size_t nn = 0;
+  SweepShape sh64, sh32;
+  // elasticity element levels > 0: per-cell (wA, wB)
+  DevBuf<double> cw64;
+  DevBuf<float> cw32;
+  // 9-point stencils
+  DevBuf<double> st64;
+  DevBuf<float> st32;
+  // V-cycle vectors (levels > 0 use b/x; all levels use t and r)
+  DevBuf<double> b64, x64, t64, r64;
+  DevBuf<float> b32, x32, t32, r32;
+};
+
+struct SolveParams {
+  double rtol = 1e-10;
+  int max_iter = 500;
+  int precision = 0;
+  double inner_rtol = 1e-5;
+  int max_outer = 30;
+};
+
+struct SolveStats {
+  int iterations = 0, outer = 0, levels = 0;
+  double rel_residual = 0, setup_ms = 0, solve_ms = 0;
+};
+
+// Geometric multigrid + PCG for one operator family on one grid.
+class Hierarchy {
+ public:
+  int init(Phys phys, int nx, int ny, double hx, double hy, unsigned fixed, double lam,
+           double mu, double eps, int coarse_max_nodes, hipStream_t s);
+  // elasticity: rebuild coarse operators from nodal xi (device, fp64), penalty p
+  int update_elasticity(const double* xi, int p, double m, hipStream_t s);
+  // filter: build the constant hierarchy once
+  int build_filter(hipStream_t s);
+
+  template <typename T>
+  int apply(const T* x, T* y, const RedSlot* dot, hipStream_t s);
+  template <typename T>
+  int residual(const T* x, const T* b, T* r, hipStream_t s);
+  template <typename T>
+  int vcycle(int l, const T* b, T* out, hipStream_t s);
+
+  template <typename T>
+  int pcg(Workspace& ws, const T* b, T* x, double rtol, int max_iter, int* iters,
+          double* rel, double bnorm2, hipStream_t s);
+  int solve(Workspace& ws, const double* b, double* x, const SolveParams& p, SolveStats* st,
+            hipStream_t s);
+
+  size_t n() const { return (size_t)bs_ * levels_[0].nn; }
+  int levels() const { return (int)levels_.size() + 1; }
+  void set_rows(int rows64, int rows32);
+  int rows64() const { return levels_.empty() ? 0 : levels_[0].sh64.rows; }
+  int rows32() const { return levels_.empty() ? 0 : levels_[0].sh32.rows; }
+
+  Phys phys_ = kElasticity;
+  int bs_ = 2;
+  unsigned fixed_ = 0;
+  double lam_ = 0, mu_ = 0, eps_ = 0;
+  int p_ = 0;
+  double m_ = 0, cp_ = 0;
+  double omega_ = 0.7;
+  int nu_pre_ = 2, nu_post_ = 2;
+  std::vector<Level> levels_;   // levels_[0] = finest
+  // coarsest dense level
+  int cnx_ = 0, cny_ = 0, cn_ = 0;
+  DevBuf<double> dense_;
+  DevBuf<double> cst64_;        // coarsest stencil (fp64)
+  DevBuf<double> tmpst_;        // explicit stencil of the last element level
+  DevBuf<double> cw0_;          // level-0 cell weights (restriction source)
+  const double* xi64_ = nullptr;
+  DevBuf<float> xi32_;
+  // Krylov vectors
+  DevBuf<double> r64_, z64_, p64_, q64_;
+  DevBuf<float> r32_, z32_, p32_, q32_, d32_;
+  DevBuf<double> rr64_;         // refinement residual
+
+ private:
+  int alloc_vectors();
+  int build_coarse_stencils(hipStream_t s);
+};
+
+}  // namespace tmx

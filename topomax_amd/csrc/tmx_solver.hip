@@ -1,0 +1,496 @@
+// Geometric multigrid hierarchy, PCG and mixed-precision iterative refinement.
+//
+// Levels: the fine grid, then nested P1 coarsenings (nx, ny even) that keep the element form
+// (the Galerkin coarse operator of nested P1 spaces is the coarse-mesh operator whose
+// triangle factor is the mean of the 4 fine triangles it contains), then tensor-product
+// linear coarsenings whose Galerkin operators are explicit 9-point block stencils, down to a
+// coarsest grid solved by a dense inverse.  Smoother: damped block Jacobi.
+#include "tmx_solver.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace tmx {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& m) { g_last_error = m; }
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ workspace
+int Workspace::init(hipStream_t s) {
+  TMX_TRY(partials.alloc(kMaxPartials));
+  TMX_TRY(counter.alloc(64));
+  TMX_TRY(scalars.alloc(kScalars));
+  TMX_HIP(hipHostMalloc(&host, sizeof(double) * kScalars, hipHostMallocDefault));
+  std::memset(host, 0, sizeof(double) * kScalars);
+  TMX_HIP(fill_zero_bytes(counter.get(), sizeof(unsigned) * 64, s));
+  TMX_HIP(fill_zero_bytes(scalars.get(), sizeof(double) * kScalars, s));
+  return 0;
+}
+
+Workspace::~Workspace() {
+  if (host) (void)hipHostFree(host);
+}
+
+int Workspace::read(hipStream_t s, int first, int count) {
+  TMX_HIP(hipMemcpyAsync(host + first, scalars.get() + first, sizeof(double) * count,
+                         hipMemcpyDeviceToHost, s));
+  TMX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// ------------------------------------------------------------------ hierarchy setup
+static int rows_default(int ny) { return ny >= 256 ? 16 : 8; }
+
+int Hierarchy::init(Phys phys, int nx, int ny, double hx, double hy, unsigned fixed, double lam,
+                    double mu, double eps, int coarse_max_nodes, hipStream_t s) {
+  phys_ = phys;
+  bs_ = phys == kElasticity ? 2 : 1;
+  fixed_ = fixed;
+  lam_ = lam;
+  mu_ = mu;
+  eps_ = eps;
+  omega_ = phys == kElasticity ? 0.7 : 0.8;
+  levels_.clear();
+  Level L0;
+  L0.nx = nx; L0.ny = ny; L0.hx = hx; L0.hy = hy; L0.kind = kElem;
+  L0.nn = (size_t)(nx + 1) * (ny + 1);
+  levels_.push_back(std::move(L0));
+  while (levels_.back().nn > (size_t)coarse_max_nodes) {
+    const Level& c = levels_.back();
+    Level n;
+    if (c.kind == kElem && c.nx % 2 == 0 && c.ny % 2 == 0) {
+      n.nx = c.nx / 2; n.ny = c.ny / 2; n.kind = kElem; n.nested_from_prev = 1;
+    } else {
+      n.nx = c.nx == 1 ? 1 : (c.nx + 1) / 2;
+      n.ny = c.ny == 1 ? 1 : (c.ny + 1) / 2;
+      n.kind = kStencil;
+      n.nested_from_prev = 0;
+      if (n.nx == c.nx && n.ny == c.ny) break;
+    }
+    n.hx = c.hx * c.nx / n.nx;
+    n.hy = c.hy * c.ny / n.ny;
+    n.nn = (size_t)(n.nx + 1) * (n.ny + 1);
+    levels_.push_back(std::move(n));
+  }
+  for (auto& L : levels_) {
+    L.sh64 = SweepShape::make(L.nx, L.ny, rows_default(L.ny));
+    L.sh32 = SweepShape::make(L.nx, L.ny, rows_default(L.ny));
+  }
+  Level& last = levels_.back();
+  cnx_ = last.nx;
+  cny_ = last.ny;
+  cn_ = bs_ * (int)last.nn;
+  TMX_TRY(dense_.alloc((size_t)cn_ * cn_));
+  TMX_TRY(alloc_vectors());
+  (void)s;
+  return 0;
+}
+
+void Hierarchy::set_rows(int rows64, int rows32) {
+  if (rows64 > 0) levels_[0].sh64 = SweepShape::make(levels_[0].nx, levels_[0].ny, rows64);
+  if (rows32 > 0) levels_[0].sh32 = SweepShape::make(levels_[0].nx, levels_[0].ny, rows32);
+}
+
+int Hierarchy::alloc_vectors() {
+  const size_t bs = bs_;
+  size_t tmp_nodes = 0;
+  for (size_t l = 0; l < levels_.size(); ++l) {
+    Level& L = levels_[l];
+    const size_t n = bs * L.nn;
+    TMX_TRY(L.t64.alloc(n));
+    TMX_TRY(L.r64.alloc(n));
+    TMX_TRY(L.t32.alloc(n));
+    TMX_TRY(L.r32.alloc(n));
+    TMX_TRY(L.b64.alloc(n));
+    TMX_TRY(L.x64.alloc(n));
+    TMX_TRY(L.b32.alloc(n));
+    TMX_TRY(L.x32.alloc(n));
+    if (l > 0 && L.kind == kElem && phys_ == kElasticity) {
+      TMX_TRY(L.cw64.alloc(2 * (size_t)L.nx * L.ny));
+      TMX_TRY(L.cw32.alloc(2 * (size_t)L.nx * L.ny));
+    }
+    const bool last = (l + 1 == levels_.size());
+    if (L.kind == kStencil || last) {
+      TMX_TRY(L.st64.alloc(9 * bs * bs * L.nn));
+      TMX_TRY(L.st32.alloc(9 * bs * bs * L.nn));
+    }
+    // element level followed by a stencil level needs its explicit stencil for the RAP
+    if (L.kind == kElem && !last && levels_[l + 1].kind == kStencil)
+      tmp_nodes = std::max(tmp_nodes, L.nn);
+  }
+  if (tmp_nodes) TMX_TRY(tmpst_.alloc(9 * bs * bs * tmp_nodes));
+  if (phys_ == kElasticity) {
+    TMX_TRY(cw0_.alloc(2 * (size_t)levels_[0].nx * levels_[0].ny));
+    TMX_TRY(xi32_.alloc(levels_[0].nn));
+  }
+  const size_t n0 = n();
+  TMX_TRY(r64_.alloc(n0));
+  TMX_TRY(z64_.alloc(n0));
+  TMX_TRY(p64_.alloc(n0));
+  TMX_TRY(q64_.alloc(n0));
+  TMX_TRY(r32_.alloc(n0));
+  TMX_TRY(z32_.alloc(n0));
+  TMX_TRY(p32_.alloc(n0));
+  TMX_TRY(q32_.alloc(n0));
+  TMX_TRY(d32_.alloc(n0));
+  TMX_TRY(rr64_.alloc(n0));
+  return 0;
+}
+
+template <typename T>
+static ElastOpArgs<T> elast_args(const Hierarchy& h, int l, const T* xi, const T* cw,
+                                 bool fp64) {
+  const Level& L = h.levels_[l];
+  ElastOpArgs<T> a;
+  a.sh = fp64 ? L.sh64 : L.sh32;
+  a.fixed = h.fixed_;
+  a.xi = l == 0 ? xi : nullptr;
+  a.cellw = l == 0 ? nullptr : cw;
+  a.p = h.p_;
+  a.m = (T)h.m_;
+  a.cp = (T)h.cp_;
+  a.k = ElastConst<T>::make(h.lam_, h.mu_, L.hx, L.hy);
+  return a;
+}
+
+// explicit stencil of element level l (fp64) into dst
+static int element_stencil(const Hierarchy& h, int l, double* dst, hipStream_t s) {
+  const Level& L = h.levels_[l];
+  if (h.phys_ == kElasticity) {
+    ElastOpArgs<double> a = elast_args<double>(h, l, h.xi64_, L.cw64.get(), true);
+    TMX_HIP(elast_to_stencil<double>(a, dst, s));
+  } else {
+    TMX_HIP(filter_to_stencil<double>(L.nx, L.ny, FilterConst<double>::make(h.eps_, L.hx, L.hy),
+                                      dst, s));
+  }
+  return 0;
+}
+
+int Hierarchy::build_coarse_stencils(hipStream_t s) {
+  const int last = (int)levels_.size() - 1;
+  for (int l = 1; l <= last; ++l) {
+    Level& L = levels_[l];
+    Level& P = levels_[l - 1];
+    if (L.kind == kStencil) {
+      const double* src = nullptr;
+      if (P.kind == kElem) {
+        TMX_TRY(element_stencil(*this, l - 1, tmpst_.get(), s));
+        src = tmpst_.get();
+      } else {
+        src = P.st64.get();
+      }
+      if (bs_ == 2)
+        TMX_HIP(stencil_rap<2>(P.nx, P.ny, L.nx, L.ny, 0, fixed_, src, L.st64.get(), s));
+      else
+        TMX_HIP(stencil_rap<1>(P.nx, P.ny, L.nx, L.ny, 0, fixed_, src, L.st64.get(), s));
+      TMX_HIP((convert<double, float>(L.st64.size(), L.st64.get(), L.st32.get(), s)));
+    } else if (l == last) {
+      TMX_TRY(element_stencil(*this, l, L.st64.get(), s));
+    }
+  }
+  if (last == 0) TMX_TRY(element_stencil(*this, 0, levels_[0].st64.get(), s));
+  const Level& C = levels_[last];
+  if (bs_ == 2)
+    TMX_HIP(stencil_to_dense<2>(C.nx, C.ny, C.st64.get(), dense_.get(), s));
+  else
+    TMX_HIP(stencil_to_dense<1>(C.nx, C.ny, C.st64.get(), dense_.get(), s));
+  TMX_HIP(dense_invert(cn_, dense_.get(), nullptr, s));
+  return 0;
+}
+
+int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
+  xi64_ = xi;
+  p_ = p;
+  m_ = m;
+  cp_ = (1.0 - m) * 2.0 / ((p + 1.0) * (p + 2.0));
+  const Level& L0 = levels_[0];
+  TMX_HIP((convert<double, float>(L0.nn, xi, xi32_.get(), s)));
+  const int last = (int)levels_.size() - 1;
+  if (last >= 1 && levels_[1].kind == kElem) {
+    TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi, p, m, cp_, cw0_.get(), s));
+  }
+  for (int l = 1; l <= last; ++l) {
+    Level& L = levels_[l];
+    if (L.kind != kElem) break;
+    const double* src = l == 1 ? cw0_.get() : levels_[l - 1].cw64.get();
+    TMX_HIP(elast_restrict_weights<double>(L.nx, L.ny, src, L.cw64.get(), s));
+    TMX_HIP((convert<double, float>(L.cw64.size(), L.cw64.get(), L.cw32.get(), s)));
+  }
+  return build_coarse_stencils(s);
+}
+
+int Hierarchy::build_filter(hipStream_t s) { return build_coarse_stencils(s); }
+
+// ------------------------------------------------------------------ level operators
+template <typename T>
+struct PrecBufs;
+template <>
+struct PrecBufs<double> {
+  static double* t(Level& L) { return L.t64.get(); }
+  static double* r(Level& L) { return L.r64.get(); }
+  static double* b(Level& L) { return L.b64.get(); }
+  static double* x(Level& L) { return L.x64.get(); }
+  static const double* cw(const Level& L) { return L.cw64.get(); }
+  static const double* st(const Level& L) { return L.st64.get(); }
+  static const SweepShape& sh(const Level& L) { return L.sh64; }
+};
+template <>
+struct PrecBufs<float> {
+  static float* t(Level& L) { return L.t32.get(); }
+  static float* r(Level& L) { return L.r32.get(); }
+  static float* b(Level& L) { return L.b32.get(); }
+  static float* x(Level& L) { return L.x32.get(); }
+  static const float* cw(const Level& L) { return L.cw32.get(); }
+  static const float* st(const Level& L) { return L.st32.get(); }
+  static const SweepShape& sh(const Level& L) { return L.sh32; }
+};
+
+template <typename T>
+static const T* xi_for(const Hierarchy& h);
+template <>
+const double* xi_for<double>(const Hierarchy& h) { return h.xi64_; }
+template <>
+const float* xi_for<float>(const Hierarchy& h) { return h.xi32_.get(); }
+
+template <typename T>
+static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, const RedSlot* dot,
+                    hipStream_t s) {
+  Level& L = h.levels_[l];
+  const bool fp64 = sizeof(T) == 8;
+  if (L.kind == kElem) {
+    if (h.phys_ == kElasticity) {
+      ElastOpArgs<T> a = elast_args<T>(h, l, xi_for<T>(h), PrecBufs<T>::cw(L), fp64);
+      TMX_HIP(elast_sweep<T>(a, op, x, b, y, (T)h.omega_, dot, s));
+    } else {
+      TMX_HIP(filter_sweep<T>(PrecBufs<T>::sh(L), FilterConst<T>::make(h.eps_, L.hx, L.hy), op, x,
+                              b, y, (T)h.omega_, dot, s));
+    }
+  } else {
+    if (h.bs_ == 2)
+      TMX_HIP((stencil_op<T, 2>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, (T)h.omega_, s)));
+    else
+      TMX_HIP((stencil_op<T, 1>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, (T)h.omega_, s)));
+  }
+  return 0;
+}
+
+template <typename T>
+int Hierarchy::apply(const T* x, T* y, const RedSlot* dot, hipStream_t s) {
+  return level_op<T>(*this, 0, kApply, x, nullptr, y, dot, s);
+}
+
+template <typename T>
+int Hierarchy::residual(const T* x, const T* b, T* r, hipStream_t s) {
+  return level_op<T>(*this, 0, kResidual, x, b, r, nullptr, s);
+}
+
+template <typename T>
+static int restrict_to(Hierarchy& h, int l, const T* rf, T* bc, hipStream_t s) {
+  const Level& F = h.levels_[l];
+  const Level& C = h.levels_[l + 1];
+  if (C.nested_from_prev) {
+    if (h.bs_ == 2) TMX_HIP((restrict_nested<T, 2>(C.nx, C.ny, h.fixed_, rf, bc, s)));
+    else TMX_HIP((restrict_nested<T, 1>(C.nx, C.ny, h.fixed_, rf, bc, s)));
+  } else {
+    if (h.bs_ == 2) TMX_HIP((restrict_bilinear<T, 2>(F.nx, F.ny, C.nx, C.ny, h.fixed_, rf, bc, s)));
+    else TMX_HIP((restrict_bilinear<T, 1>(F.nx, F.ny, C.nx, C.ny, h.fixed_, rf, bc, s)));
+  }
+  return 0;
+}
+
+template <typename T>
+static int prolong_to(Hierarchy& h, int l, const T* ec, T* xf, hipStream_t s) {
+  const Level& F = h.levels_[l];
+  const Level& C = h.levels_[l + 1];
+  if (C.nested_from_prev) {
+    if (h.bs_ == 2) TMX_HIP((prolong_nested<T, 2>(C.nx, C.ny, h.fixed_, ec, xf, s)));
+    else TMX_HIP((prolong_nested<T, 1>(C.nx, C.ny, h.fixed_, ec, xf, s)));
+  } else {
+    if (h.bs_ == 2) TMX_HIP((prolong_bilinear<T, 2>(F.nx, F.ny, C.nx, C.ny, h.fixed_, ec, xf, s)));
+    else TMX_HIP((prolong_bilinear<T, 1>(F.nx, F.ny, C.nx, C.ny, h.fixed_, ec, xf, s)));
+  }
+  return 0;
+}
+
+template <typename T>
+int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s) {
+  const int last = (int)levels_.size() - 1;
+  if (l == last) {
+    TMX_HIP(dense_apply<T>(cn_, dense_.get(), b, out, s));
+    return 0;
+  }
+  Level& L = levels_[l];
+  Level& C = levels_[l + 1];
+  T* buf[2] = {out, PrecBufs<T>::t(L)};
+  const int S = nu_pre_ + nu_post_;
+  int w = (S - 1) % 2;
+  TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s));
+  for (int k = 1; k < nu_pre_; ++k) {
+    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s));
+    w ^= 1;
+  }
+  T* r = PrecBufs<T>::r(L);
+  TMX_TRY(level_op<T>(*this, l, kResidual, buf[w], b, r, nullptr, s));
+  TMX_TRY(restrict_to<T>(*this, l, r, PrecBufs<T>::b(C), s));
+  TMX_TRY(vcycle<T>(l + 1, PrecBufs<T>::b(C), PrecBufs<T>::x(C), s));
+  TMX_TRY(prolong_to<T>(*this, l, PrecBufs<T>::x(C), buf[w], s));
+  for (int k = 0; k < nu_post_; ++k) {
+    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s));
+    w ^= 1;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ Krylov
+template <typename T>
+struct KrylovBufs;
+template <>
+struct KrylovBufs<double> {
+  static double* r(Hierarchy& h) { return h.r64_.get(); }
+  static double* z(Hierarchy& h) { return h.z64_.get(); }
+  static double* p(Hierarchy& h) { return h.p64_.get(); }
+  static double* q(Hierarchy& h) { return h.q64_.get(); }
+};
+template <>
+struct KrylovBufs<float> {
+  static float* r(Hierarchy& h) { return h.r32_.get(); }
+  static float* z(Hierarchy& h) { return h.z32_.get(); }
+  static float* p(Hierarchy& h) { return h.p32_.get(); }
+  static float* q(Hierarchy& h) { return h.q32_.get(); }
+};
+
+template <typename T>
+int Hierarchy::pcg(Workspace& ws, const T* b, T* x, double rtol, int max_iter, int* iters,
+                   double* rel, double bnorm2, hipStream_t s) {
+  const size_t nv = n();
+  T* r = KrylovBufs<T>::r(*this);
+  T* z = KrylovBufs<T>::z(*this);
+  T* p = KrylovBufs<T>::p(*this);
+  T* q = KrylovBufs<T>::q(*this);
+  *iters = 0;
+  TMX_HIP(fill_zero_bytes(x, sizeof(T) * nv, s));
+  TMX_HIP(copy_vec<T>(nv, b, r, s));
+  TMX_TRY(vcycle<T>(0, r, z, s));
+  TMX_HIP(copy_vec<T>(nv, z, p, s));
+  int irz = kScRZ0, irzn = kScRZ1;
+  TMX_HIP(dot_t<T>(nv, r, z, ws.slot(irz), s));
+  const RedSlot pq = ws.slot(kScPQ), rr = ws.slot(kScRR);
+  for (int it = 1; it <= max_iter; ++it) {
+    TMX_TRY(apply<T>(p, q, &pq, s));
+    TMX_HIP(pcg_update_xr<T>(nv, ws.sc(), irz, kScPQ, x, r, p, q, rr, s));
+    TMX_TRY(ws.read(s, kScPQ, 2));
+    const double pqv = ws.host[kScPQ], rrv = ws.host[kScRR];
+    *iters = it;
+    if (!std::isfinite(pqv) || !std::isfinite(rrv)) {
+      set_error("non-finite residual in PCG");
+      return -5;
+    }
+    if (pqv <= 0.0) {
+      set_error("PCG breakdown: p.Ap <= 0 (operator not positive definite)");
+      return -4;
+    }
+    *rel = std::sqrt(rrv / bnorm2);
+    if (*rel < rtol) return 0;
+    TMX_TRY(vcycle<T>(0, r, z, s));
+    TMX_HIP(dot_t<T>(nv, r, z, ws.slot(irzn), s));
+    TMX_HIP(pcg_update_p<T>(nv, ws.sc(), irzn, irz, z, p, s));
+    std::swap(irz, irzn);
+  }
+  set_error("PCG did not reach the tolerance within the iteration limit");
+  return -3;
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParams& prm,
+                     SolveStats* st, hipStream_t s) {
+  const size_t nv = n();
+  auto t0 = std::chrono::steady_clock::now();
+  TMX_HIP(dot(nv, b, b, ws.slot(kScBB), s));
+  TMX_TRY(ws.read(s, kScBB, 1));
+  const double bb = ws.host[kScBB];
+  SolveStats local;
+  SolveStats& S = st ? *st : local;
+  S.levels = levels();
+  S.iterations = 0;
+  S.outer = 0;
+  if (!std::isfinite(bb)) {
+    set_error("non-finite right-hand side");
+    return -5;
+  }
+  if (bb == 0.0) {
+    TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
+    S.rel_residual = 0.0;
+    return 0;
+  }
+  int rc = 0;
+  if (prm.precision == 0) {
+    int it = 0;
+    double rel = 0;
+    rc = pcg<double>(ws, b, x, prm.rtol, prm.max_iter, &it, &rel, bb, s);
+    S.iterations = it;
+    S.outer = 1;
+    if (rc == 0 || rc == -3) {  // report the true residual
+      TMX_TRY(residual<double>(x, b, rr64_.get(), s));
+      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), ws.slot(kScRR64), s));
+      TMX_TRY(ws.read(s, kScRR64, 1));
+      S.rel_residual = std::sqrt(ws.host[kScRR64] / bb);
+    }
+  } else {
+    TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
+    TMX_HIP(copy_vec<double>(nv, b, rr64_.get(), s));
+    double rr = bb;
+    ws.host[kScRR64] = bb;
+    TMX_HIP(hipMemcpyAsync(ws.sc() + kScRR64, ws.host + kScRR64, sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    rc = -3;
+    int budget = prm.max_iter;
+    for (int outer = 0; outer < prm.max_outer; ++outer) {
+      S.rel_residual = std::sqrt(rr / bb);
+      if (S.rel_residual < prm.rtol) {
+        rc = 0;
+        break;
+      }
+      if (budget <= 0) break;
+      float* rin = levels_[0].b32.get();
+      TMX_HIP(scale_to<float>(nv, ws.sc(), kScRR64, rr64_.get(), rin, s));
+      int it = 0;
+      double rel = 0;
+      const double inner = std::max(prm.inner_rtol, 0.5 * prm.rtol / S.rel_residual);
+      int irc = pcg<float>(ws, rin, d32_.get(), inner, budget, &it, &rel, 1.0, s);
+      S.iterations += it;
+      budget -= it;
+      S.outer = outer + 1;
+      if (irc != 0 && irc != -3) return irc;
+      TMX_HIP(axpy_scaled<float>(nv, ws.sc(), kScRR64, d32_.get(), x, s));
+      TMX_TRY(residual<double>(x, b, rr64_.get(), s));
+      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), ws.slot(kScRR64), s));
+      TMX_TRY(ws.read(s, kScRR64, 1));
+      rr = ws.host[kScRR64];
+      if (!std::isfinite(rr)) {
+        set_error("non-finite residual in iterative refinement");
+        return -5;
+      }
+    }
+    if (rc == -3) {
+      S.rel_residual = std::sqrt(rr / bb);
+      if (S.rel_residual < prm.rtol) rc = 0;
+      else set_error("mixed-precision refinement did not reach the tolerance");
+    }
+  }
+  TMX_HIP(hipStreamSynchronize(s));
+  S.solve_ms = ms_since(t0);
+  return rc;
+}
+
+template int Hierarchy::apply<double>(const double*, double*, const RedSlot*, hipStream_t);
+template int Hierarchy::apply<float>(const float*, float*, const RedSlot*, hipStream_t);
+template int Hierarchy::residual<double>(const double*, const double*, double*, hipStream_t);
+template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t);
+template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t);
+
+}  // namespace tmx
