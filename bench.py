@@ -1,0 +1,187 @@
+"""Benchmark: solve + gradient of topomax's elasticity hot path at N=1000 on one MI355X.
+
+Workload (one "step"): the 3x1 cantilever (Left side fixed, traction (0,-1) on the Right side
+centred at 0.5 with length 0.1; E=1, nu=0.3, SIMP p=3, m=1e-6, filter radius 0.05) meshed
+with N=1000 cells per unit length (3000 x 1000 cells, 6,000,000 P1 triangles, 3,004,001
+vertices, 6,008,002 DOFs); design = Gaussian-smoothed (sigma = 5 cells) seeded uniform noise
+mapped to [1e-3, 1].  One step = Helmholtz filter solve -> SIMP elasticity solve to 1e-8
+relative residual -> compliance -> sensitivity -> filter adjoint solve -> M w, with every
+input resident in HBM.  The headline line is the mixed-precision path (FP32 multigrid-PCG
+inside FP64 iterative refinement); the FP64 path is timed beside it.
+
+Prints one JSON line (rank 0, N=1).  `value` = GDOF/s = DOFs / (solve+grad seconds) / 1e9.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def time_steps(problem, rho, steps, warmup, precision, rtol):
+    import torch
+    for _ in range(warmup):
+        problem.objective_and_gradient(rho, rtol=rtol, precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    J = None
+    for _ in range(steps):
+        J, w, mw = problem.objective_and_gradient(rho, rtol=rtol, precision=precision)
+        stats.append(dict(problem.last_stats))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, J, w, stats
+
+
+def bench_kernel(problem, kernel, precision, reps=50):
+    import ctypes
+    from topomax_amd import _lib
+    h = problem._h
+    us, nbytes = ctypes.c_double(), ctypes.c_double()
+    h.stream()
+    prec = {"fp64": _lib.TMX_FP64, "fp32": _lib.TMX_FP32}[precision]
+    _lib.check(h.lib.tmx_bench_kernel(h.ptr, kernel, prec, reps, ctypes.byref(us),
+                                      ctypes.byref(nbytes)))
+    return us.value, nbytes.value
+
+
+def cpu_baseline(n_cpu: int, seconds_budget: float = 30.0):
+    """The CPU oracle (numpy/scipy, SuperLU direct solves, single thread) on a bounded sample
+    of the same workload: one objective + gradient of the same cantilever at N=n_cpu."""
+    from tests.cases import CANTILEVER, oracle_problem, smoothed_density
+    nx, ny = 3 * n_cpu, n_cpu
+    orc = oracle_problem(CANTILEVER, nx, ny)
+    orc.set_penalization(3)
+    rho = smoothed_density(nx, ny, seed=0, sigma=5.0)
+    t0 = time.perf_counter()
+    orc.calculate_objective(rho)
+    w = orc.calculate_objective_gradient()
+    _ = orc.mass @ w
+    dt = time.perf_counter() - t0
+    ndof = 2 * (nx + 1) * (ny + 1)
+    return {
+        "value": ndof / dt / 1e9,
+        "unit": "GDOF/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"CPU oracle (numpy/scipy SuperLU, single-threaded, not FEniCS) P1 FP64 "
+                   f"filter+solve+compliance+gradient of the same cantilever at N={n_cpu} "
+                   f"({ndof} DOFs): {dt:.2f} s per step on {cpu_model()} "
+                   f"({len(os.sched_getaffinity(0))} cores visible)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=1000)
+    ap.add_argument("--p", type=int, default=3)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--cpu-N", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(independent designs shard with zero communication, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tests.cases import CANTILEVER, gpu_problem, smoothed_density
+
+    N = args.N
+    nx, ny = 3 * N, N
+    problem = gpu_problem(CANTILEVER, nx, ny, precision="mixed", rtol=args.rtol)
+    problem.set_penalization(args.p)
+    tune = problem.autotune()
+    rho = torch.as_tensor(smoothed_density(nx, ny, seed=0, sigma=5.0), device="cuda:0")
+    ndof = 2 * (nx + 1) * (ny + 1)
+
+    t_mixed, J_mixed, w_mixed, st_mixed = time_steps(problem, rho, args.steps, args.warmup,
+                                                     "mixed", args.rtol)
+    kernels = {}
+    for name, kid, prec in (("apply_fp32", 0, "fp32"), ("smooth_fp32", 1, "fp32"),
+                            ("apply_fp64", 0, "fp64"), ("smooth_fp64", 1, "fp64")):
+        us, nbytes = bench_kernel(problem, kid, prec)
+        kernels[name] = {"us": us, "bytes": nbytes, "GBs": nbytes / us / 1e3}
+
+    result_fp64 = None
+    if not args.no_fp64:
+        t64, J64, w64, st64 = time_steps(problem, rho, max(2, args.steps // 2), 1, "fp64",
+                                         args.rtol)
+        dw = float(torch.linalg.norm(w_mixed - w64) / torch.linalg.norm(w64))
+        result_fp64 = {"ms_per_step": t64 * 1e3, "GDOF_s": ndof / t64 / 1e9,
+                       "iterations": st64[-1]["iterations"],
+                       "compliance_rel_err_mixed_vs_fp64": abs(J_mixed - J64) / abs(J64),
+                       "gradient_rel_err_mixed_vs_fp64": dw,
+                       "speedup_mixed_over_fp64": t64 / t_mixed}
+
+    # dominant kernel of the mixed step: the FP32 fine-grid block-Jacobi sweep
+    dom = kernels["smooth_fp32"]
+    roofline = {"kernel": "k_elast_sweep<float,true,kSmooth> (fine-grid FP32 block-Jacobi sweep)",
+                "bound": "hbm", "achieved": dom["GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": dom["GBs"] / PEAK_HBM_GBS, "traffic": None,
+                "algorithmic_bytes_per_launch": dom["bytes"], "avg_launch_us": dom["us"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_N)
+
+    line = {
+        "metric": "GDOF/s & solve+grad time at N=1000 to 1e-8 rel. residual, FP64 vs mixed",
+        "value": ndof / t_mixed / 1e9,
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-in-f64 (mixed)",
+        "data": "synthetic (seeded smoothed random density)",
+        "config": {"workload": f"cantilever 3x1, N={N} ({nx}x{ny} cells, {ndof} P1 DOFs), "
+                               f"p={args.p}, filter->solve->compliance->gradient, "
+                               f"rtol {args.rtol:g}",
+                   "N": N, "dofs": ndof, "p": args.p, "rtol": args.rtol,
+                   "precision": "mixed", "tiles": tune},
+        "iterations": st_mixed[-1]["iterations"],
+        "outer_iterations": st_mixed[-1]["outer_iterations"],
+        "rel_residual": st_mixed[-1]["rel_residual"],
+        "compliance": J_mixed,
+        "fp64": result_fp64,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -145,6 +145,12 @@ int tmx_integrate(tmx_problem* h, const double* values, double* out_host);
  * for each candidate row-chunk size and keeps the fastest; results are written to
  * rows_out[2] (fp64, fp32) and best_us_out[2]. */
 int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
+/* Average duration (microseconds, HIP events on the handle's stream) of `reps` back-to-back
+ * launches of one fine-grid kernel with the operator of the last solve:
+ * kernel 0 = elasticity K(xi)u apply, 1 = elasticity block-Jacobi sweep, 2 = filter apply.
+ * precision TMX_FP64 or TMX_FP32.  Writes the per-launch algorithmic bytes to *bytes. */
+int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps, double* avg_us,
+                     double* bytes);
 /* Sweep row-chunk override (fp64, fp32); <= 0 keeps the current value. */
 int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
 

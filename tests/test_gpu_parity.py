@@ -34,7 +34,7 @@ def host(t):
 
 
 # ------------------------------------------------------------------ load vector
-@pytest.mark.parametrize("name,n", [("triangle", 10), ("cantilever", 20), ("short_cantilever", 4),
+@pytest.mark.parametrize("name,n", [("triangle", 10), ("cantilever", 20), ("short_cantilever", 10),
                                     ("bridge", 4)])
 def test_load_vector_matches_oracle(name, n):
     design = json.load(open(design_path(name)))
@@ -47,8 +47,10 @@ def test_load_vector_matches_oracle(name, n):
     print(name, "load nnz", np.count_nonzero(orc.load), flush=True)
     assert np.count_nonzero(orc.load) > 0
     assert rel(F, orc.load) < 1e-13
-    # same support (membership of the body force / traction points is exact)
-    assert np.array_equal(np.abs(F) > 0, np.abs(orc.load) > 0)
+    # same support (membership of the body force / traction points is exact); the oracle's
+    # quadrature leaves ~1e-16 relative residue where the exact weight is 0
+    cut = 1e-12 * np.abs(orc.load).max()
+    assert np.array_equal(np.abs(F) > cut, np.abs(orc.load) > cut)
 
 
 # ------------------------------------------------------------------ operators
@@ -146,6 +148,6 @@ def test_smoothed_density_high_contrast(p):
     u = host(gp.forward(cuda(xi)))
     print("p", p, "u err", rel(u, u_ref), gp.last_stats, flush=True)
     assert rel(u, u_ref) < 1e-8
-    um = host(gp.forward(cuda(xi), precision="mixed", rtol=1e-12))
+    um = host(gp.forward(cuda(xi), precision="mixed", rtol=1e-10))
     print("p", p, "mixed u err", rel(um, u_ref), gp.last_stats, flush=True)
     assert rel(um, u_ref) < 1e-6

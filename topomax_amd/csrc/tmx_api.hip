@@ -417,6 +417,53 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
   return TMX_OK;
 }
 
+extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps,
+                                double* avg_us, double* bytes) {
+  if (!h || !avg_us || reps < 1) return invalid("bad arguments");
+  if (kernel < 0 || kernel > 2) return invalid("unknown kernel id");
+  if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
+  if (kernel < 2 && !h->el.xi64_) return invalid("no elasticity operator yet: solve first");
+  const size_t nv = nverts(h);
+  hipStream_t s = h->stream;
+  const bool f64 = precision == TMX_FP64;
+  const double sz = f64 ? 8.0 : 4.0;
+  // inputs: u / x = current displacement scratch, b = masked load
+  TMX_HIP(fill(2 * nv, 1.0, h->u_s.get(), s));
+  TMX_HIP((convert<double, float>(2 * nv, h->u_s.get(), h->u32.get(), s)));
+  TMX_HIP((convert<double, float>(2 * nv, h->Fm.get(), h->y32.get(), s)));
+  auto launch = [&]() -> int {
+    if (kernel == 0) {
+      if (f64) return h->el.apply<double>(h->u_s.get(), h->tmp64.get(), nullptr, s);
+      return h->el.apply<float>(h->u32.get(), h->el.q32_.get(), nullptr, s);
+    }
+    if (kernel == 1) {
+      if (f64) return h->el.smooth0<double>(h->u_s.get(), h->Fm.get(), h->tmp64.get(), s);
+      return h->el.smooth0<float>(h->u32.get(), h->y32.get(), h->el.q32_.get(), s);
+    }
+    if (f64) return h->fl.apply<double>(h->u_s.get(), h->tmp64.get(), nullptr, s);
+    return h->fl.apply<float>(h->u32.get(), h->el.q32_.get(), nullptr, s);
+  };
+  for (int i = 0; i < 3; ++i) TMX_TRY(launch());
+  hipEvent_t e0, e1;
+  TMX_HIP(hipEventCreate(&e0));
+  TMX_HIP(hipEventCreate(&e1));
+  TMX_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) TMX_TRY(launch());
+  TMX_HIP(hipEventRecord(e1, s));
+  TMX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_us = 1000.0 * ms / reps;
+  if (bytes) {
+    // algorithmic bytes per node: apply u (2) + y (2) + xi (1); sweep adds b (2); filter x + y
+    const double per = kernel == 0 ? 5.0 : (kernel == 1 ? 7.0 : 2.0);
+    *bytes = per * sz * (double)nv;
+  }
+  return TMX_OK;
+}
+
 // ------------------------------------------------------------------ batch
 struct tmx_batch {
   std::vector<tmx_problem*> probs;

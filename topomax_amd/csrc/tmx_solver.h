@@ -141,6 +141,8 @@ class Hierarchy {
   template <typename T>
   int residual(const T* x, const T* b, T* r, hipStream_t s);
   template <typename T>
+  int smooth0(const T* x, const T* b, T* y, hipStream_t s);
+  template <typename T>
   int vcycle(int l, const T* b, T* out, hipStream_t s);
 
   template <typename T>

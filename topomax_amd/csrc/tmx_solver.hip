@@ -9,11 +9,14 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace tmx {
 
 static thread_local std::string g_last_error;
+static const bool g_trace = std::getenv("TMX_TRACE") != nullptr;
 void set_error(const std::string& m) { g_last_error = m; }
 const char* last_error_cstr() { return g_last_error.c_str(); }
 
@@ -287,6 +290,11 @@ int Hierarchy::residual(const T* x, const T* b, T* r, hipStream_t s) {
 }
 
 template <typename T>
+int Hierarchy::smooth0(const T* x, const T* b, T* y, hipStream_t s) {
+  return level_op<T>(*this, 0, kSmooth, x, b, y, nullptr, s);
+}
+
+template <typename T>
 static int restrict_to(Hierarchy& h, int l, const T* rf, T* bc, hipStream_t s) {
   const Level& F = h.levels_[l];
   const Level& C = h.levels_[l + 1];
@@ -449,11 +457,19 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
                            hipMemcpyHostToDevice, s));
     rc = -3;
     int budget = prm.max_iter;
+    double best = 1.0;
+    int stalled = 0;
     for (int outer = 0; outer < prm.max_outer; ++outer) {
       S.rel_residual = std::sqrt(rr / bb);
       if (S.rel_residual < prm.rtol) {
         rc = 0;
         break;
+      }
+      // the FP64 residual has a round-off floor; stop when two steps gain < 2x
+      if (outer > 0) {
+        stalled = (S.rel_residual > 0.5 * best) ? stalled + 1 : 0;
+        best = std::min(best, S.rel_residual);
+        if (stalled >= 2) break;
       }
       if (budget <= 0) break;
       float* rin = levels_[0].b32.get();
@@ -462,6 +478,9 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       double rel = 0;
       const double inner = std::max(prm.inner_rtol, 0.5 * prm.rtol / S.rel_residual);
       int irc = pcg<float>(ws, rin, d32_.get(), inner, budget, &it, &rel, 1.0, s);
+      if (g_trace)
+        std::fprintf(stderr, "[tmx] refine %d: |r|/|b| = %.3e, inner its %d rel %.3e rc %d\n",
+                     outer, S.rel_residual, it, rel, irc);
       S.iterations += it;
       budget -= it;
       S.outer = outer + 1;
@@ -479,7 +498,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     if (rc == -3) {
       S.rel_residual = std::sqrt(rr / bb);
       if (S.rel_residual < prm.rtol) rc = 0;
-      else set_error("mixed-precision refinement did not reach the tolerance");
+      else set_error("mixed-precision refinement stopped at relative residual " +
+                     std::to_string(S.rel_residual) + " above the tolerance");
     }
   }
   TMX_HIP(hipStreamSynchronize(s));
@@ -490,6 +510,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
 template int Hierarchy::apply<double>(const double*, double*, const RedSlot*, hipStream_t);
 template int Hierarchy::apply<float>(const float*, float*, const RedSlot*, hipStream_t);
 template int Hierarchy::residual<double>(const double*, const double*, double*, hipStream_t);
+template int Hierarchy::smooth0<double>(const double*, const double*, double*, hipStream_t);
+template int Hierarchy::smooth0<float>(const float*, const float*, float*, hipStream_t);
 template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t);
 template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t);
 
