@@ -151,6 +151,26 @@ int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
  * precision TMX_FP64 or TMX_FP32.  Writes the per-launch algorithmic bytes to *bytes. */
 int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps, double* avg_us,
                      double* bytes);
+/* Solver options (both operator families).  TMX_OPT_FUSED: 1 (default) fuses residual with
+ * restriction and prolongation with the next smoothing sweep on element levels; TMX_OPT_TAIL:
+ * 0 off, 1 on (default), > 1 on with that many nodes as the largest level run by the
+ * one-workgroup bottom-of-cycle kernel; TMX_OPT_NU_PRE / TMX_OPT_NU_POST: elasticity Jacobi
+ * sweeps on the finest level (>= 1); TMX_OPT_OMEGA_ELAST / TMX_OPT_OMEGA_FILTER: Jacobi
+ * damping. */
+enum tmx_option {
+  TMX_OPT_FUSED = 0, TMX_OPT_TAIL = 1, TMX_OPT_NU_PRE = 2, TMX_OPT_NU_POST = 3,
+  TMX_OPT_OMEGA_ELAST = 4, TMX_OPT_OMEGA_FILTER = 5,
+  TMX_OPT_NU_COARSE = 6,    /* pre- and post-sweeps on the coarser levels (PRE/POST: finest) */
+  TMX_OPT_FILTER_RTOL = 7,  /* relative residual of the Helmholtz filter solves (1e-12) */
+  TMX_OPT_NU_FINE_FILTER = 8,   /* filter: pre- and post-sweeps on the finest level */
+  TMX_OPT_NU_COARSE_FILTER = 9  /* filter: pre- and post-sweeps on the coarser levels */
+};
+int tmx_set_option(tmx_problem* h, int option, double value);
+/* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the
+ * operator of xi (device, n_v), 1 = Helmholtz filter (xi ignored).  precision TMX_FP64 or
+ * TMX_FP32 (r, z stay double). */
+int tmx_precondition(tmx_problem* h, int which, int precision, const double* xi,
+                     const double* r, double* z);
 /* Sweep row-chunk override (fp64, fp32); <= 0 keeps the current value. */
 int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
 

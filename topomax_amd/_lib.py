@@ -24,6 +24,10 @@ TMX_LEFT, TMX_RIGHT, TMX_TOP, TMX_BOTTOM = 0, 1, 2, 3
 TMX_FIX_LEFT, TMX_FIX_RIGHT, TMX_FIX_TOP, TMX_FIX_BOTTOM = 1, 2, 4, 8
 TMX_FP64, TMX_MIXED, TMX_FP32 = 0, 1, 2
 TMX_MAX_TRACTIONS = 16
+TMX_OPT_FUSED, TMX_OPT_TAIL, TMX_OPT_NU_PRE, TMX_OPT_NU_POST = 0, 1, 2, 3
+TMX_OPT_OMEGA_ELAST, TMX_OPT_OMEGA_FILTER = 4, 5
+TMX_OPT_NU_COARSE, TMX_OPT_FILTER_RTOL = 6, 7
+TMX_OPT_NU_FINE_FILTER, TMX_OPT_NU_COARSE_FILTER = 8, 9
 
 SIDE_CODES = {"Left": TMX_LEFT, "Right": TMX_RIGHT, "Top": TMX_TOP, "Bottom": TMX_BOTTOM}
 FIX_BITS = {"Left": TMX_FIX_LEFT, "Right": TMX_FIX_RIGHT, "Top": TMX_FIX_TOP,
@@ -96,6 +100,8 @@ SIGNATURES = {
     "tmx_integrate": (_I, [_P, _P, ctypes.POINTER(_D)]),
     "tmx_autotune": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "tmx_set_tiles": (_I, [_P, _I, _I]),
+    "tmx_set_option": (_I, [_P, _I, _D]),
+    "tmx_precondition": (_I, [_P, _I, _I, _P, _P, _P]),
     "tmx_bench_kernel": (_I, [_P, _I, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "tmx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
     "tmx_batch_destroy": (None, [_P]),

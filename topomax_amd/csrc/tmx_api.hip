@@ -17,7 +17,9 @@ using namespace tmx;
 
 struct tmx_problem {
   tmx_problem_desc desc;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // internal non-blocking stream (all work, graph capture)
+  hipStream_t user = nullptr;    // caller's stream (tmx_set_stream), joined by events
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   int nx = 0, ny = 0;
   double hx = 0, hy = 0, lam = 0, mu = 0, m = 1e-6;
   unsigned fixed = 0;  // sides actually fixed after the near() test
@@ -30,6 +32,26 @@ struct tmx_problem {
   DevBuf<double> tmp64;
   DevBuf<float> xi32, u32, y32;
   SolveParams el_prm, fl_prm;
+  ~tmx_problem() {
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Orders an entry point's work after the caller's stream and the caller's stream after it.
+class Scope {
+ public:
+  explicit Scope(tmx_problem* h) : h_(h) {
+    if (hipEventRecord(h->ev_in, h->user) == hipSuccess) (void)hipStreamWaitEvent(h->stream, h->ev_in, 0);
+  }
+  ~Scope() {
+    if (hipEventRecord(h_->ev_out, h_->stream) == hipSuccess)
+      (void)hipStreamWaitEvent(h_->user, h_->ev_out, 0);
+  }
+
+ private:
+  tmx_problem* h_;
 };
 
 static const double kDolfinEps = 3.0e-16;
@@ -79,6 +101,9 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   // Lame parameters as the reference computes them (FEM_src/elasisity_problem.py:98-99)
   h->mu = d->young_modulus / (2 * (1 + d->poisson_ratio));
   h->lam = h->mu * d->poisson_ratio / (0.5 - d->poisson_ratio);
+  TMX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  TMX_HIP(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+  TMX_HIP(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
   hipStream_t s = h->stream;
   TMX_TRY(h->ws.init(s));
   TMX_TRY(h->el.init(kElasticity, h->nx, h->ny, ab, cd, fx, h->lam, h->mu, 0.0, 64, s));
@@ -126,6 +151,7 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   TMX_HIP(fill(nv, 1.0, h->tmp64.get(), s));
   TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, ab, cd),
                                kApply, h->tmp64.get(), nullptr, h->lumped.get(), 0.0, nullptr, s));
+  TMX_HIP(hipStreamSynchronize(s));
   h->el_prm.rtol = 1e-10;
   h->el_prm.max_iter = 1000;
   h->el_prm.precision = d->precision == TMX_FP64 ? 0 : 1;
@@ -152,13 +178,13 @@ extern "C" int tmx_problem_create(const tmx_problem_desc* desc, tmx_problem** ou
 
 extern "C" void tmx_problem_destroy(tmx_problem* h) {
   if (!h) return;
-  (void)hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   delete h;
 }
 
 extern "C" int tmx_set_stream(tmx_problem* h, void* stream) {
   if (!h) return invalid("null handle");
-  h->stream = static_cast<hipStream_t>(stream);
+  h->user = static_cast<hipStream_t>(stream);
   return TMX_OK;
 }
 
@@ -228,6 +254,7 @@ static int filter_rhs_solve(tmx_problem* h, const double* b, double* w) {
 
 extern "C" int tmx_mass_apply(tmx_problem* h, const double* x, double* y) {
   if (!h || !x || !y) return invalid("null argument");
+  Scope scope(h);
   TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, h->hx, h->hy),
                                kApply, x, nullptr, y, 0.0, nullptr, h->stream));
   return TMX_OK;
@@ -235,12 +262,14 @@ extern "C" int tmx_mass_apply(tmx_problem* h, const double* x, double* y) {
 
 extern "C" int tmx_filter_apply(tmx_problem* h, const double* rho, double* xi) {
   if (!h || !rho || !xi) return invalid("null argument");
+  Scope scope(h);
   TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
   return filter_rhs_solve(h, h->b_s.get(), xi);
 }
 
 extern "C" int tmx_filter_solve(tmx_problem* h, const double* b, double* w) {
   if (!h || !b || !w) return invalid("null argument");
+  Scope scope(h);
   return filter_rhs_solve(h, b, w);
 }
 
@@ -261,6 +290,7 @@ static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const t
 extern "C" int tmx_elasticity_solve(tmx_problem* h, const double* xi, double* u,
                                     const tmx_solve_opts* opts, tmx_solve_stats* stats) {
   if (!h || !xi || !u) return invalid("null argument");
+  Scope scope(h);
   SolveStats st;
   const int rc = elasticity_solve(h, xi, u, opts, &st);
   export_stats(st, stats);
@@ -269,7 +299,8 @@ extern "C" int tmx_elasticity_solve(tmx_problem* h, const double* xi, double* u,
 
 static int read_dot(tmx_problem* h, size_t n, const double* a, const double* b, int slot,
                     double* out) {
-  TMX_HIP(dot(n, a, b, h->ws.slot(slot), h->stream));
+  RedSlot r = h->ws.region(3);
+  TMX_HIP(dot(n, a, b, r, h->ws.sc() + slot, h->stream));
   TMX_TRY(h->ws.read(h->stream, slot, 1));
   *out = h->ws.host[slot];
   return TMX_OK;
@@ -277,16 +308,19 @@ static int read_dot(tmx_problem* h, size_t n, const double* a, const double* b, 
 
 extern "C" int tmx_compliance(tmx_problem* h, const double* u, double* out) {
   if (!h || !u || !out) return invalid("null argument");
+  Scope scope(h);
   return read_dot(h, 2 * nverts(h), h->F.get(), u, kScJ, out);
 }
 
 extern "C" int tmx_integrate(tmx_problem* h, const double* values, double* out) {
   if (!h || !values || !out) return invalid("null argument");
+  Scope scope(h);
   return read_dot(h, nverts(h), h->lumped.get(), values, kScInt, out);
 }
 
 extern "C" int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
   if (!h || !xi || !u || !b) return invalid("null argument");
+  Scope scope(h);
   TMX_TRY(need_penalty(h));
   TMX_HIP(elast_sensitivity(h->el.levels_[0].sh64, xi, u, h->p, h->m, h->lam, h->mu, h->hx, h->hy,
                             b, h->stream));
@@ -298,6 +332,7 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
                                           const tmx_solve_opts* opts, tmx_solve_stats* stats) {
   if (!h || !rho) return invalid("null argument");
   TMX_TRY(need_penalty(h));
+  Scope scope(h);
   double* xiv = xi ? xi : h->xi_s.get();
   double* uv = u ? u : h->u_s.get();
   int rc = tmx_filter_apply(h, rho, xiv);
@@ -326,27 +361,93 @@ extern "C" int tmx_matvec(tmx_problem* h, const double* xi, const double* u, dou
                           int precision) {
   if (!h || !xi || !u || !y) return invalid("null argument");
   TMX_TRY(need_penalty(h));
+  if (precision != TMX_FP64 && precision != TMX_FP32)
+    return invalid("precision must be TMX_FP64 or TMX_FP32");
+  Scope scope(h);
   const size_t nv = nverts(h);
-  h->el.xi64_ = xi;
-  h->el.p_ = h->p;
-  h->el.m_ = h->m;
-  h->el.cp_ = (1.0 - h->m) * 2.0 / ((h->p + 1.0) * (h->p + 2.0));
+  TMX_TRY(h->el.set_fine_operator(xi, h->p, h->m, h->stream));
   if (precision == TMX_FP64) {
     TMX_TRY(h->el.apply<double>(u, y, nullptr, h->stream));
-  } else if (precision == TMX_FP32) {
-    TMX_HIP((convert<double, float>(nv, xi, h->el.xi32_.get(), h->stream)));
+  } else {
     TMX_HIP((convert<double, float>(2 * nv, u, h->u32.get(), h->stream)));
     TMX_TRY(h->el.apply<float>(h->u32.get(), h->y32.get(), nullptr, h->stream));
     TMX_HIP((convert<float, double>(2 * nv, h->y32.get(), y, h->stream)));
-  } else {
-    return invalid("precision must be TMX_FP64 or TMX_FP32");
   }
   return TMX_OK;
 }
 
 extern "C" int tmx_load_vector(tmx_problem* h, double* F) {
   if (!h || !F) return invalid("null argument");
+  Scope scope(h);
   TMX_HIP(copy_vec<double>(2 * nverts(h), h->F.get(), F, h->stream));
+  return TMX_OK;
+}
+
+extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
+  if (!h) return invalid("null handle");
+  Hierarchy* hs[2] = {&h->el, &h->fl};
+  switch (option) {
+    case TMX_OPT_FUSED:
+      for (auto* x : hs) x->fused_ = value != 0.0;
+      break;
+    case TMX_OPT_TAIL:
+      for (auto* x : hs) {
+        x->use_tail_ = value != 0.0;
+        if (value > 1.0) {
+          x->tail_nodes_ = (int)value;
+          x->find_tail();
+        }
+      }
+      break;
+    case TMX_OPT_NU_PRE:
+    case TMX_OPT_NU_POST:
+    case TMX_OPT_NU_COARSE:
+    case TMX_OPT_NU_FINE_FILTER:
+    case TMX_OPT_NU_COARSE_FILTER:
+      if (value < 1 || value > 8 || value != std::floor(value)) return invalid("sweeps must be 1..8");
+      if (option == TMX_OPT_NU_PRE) h->el.nu_pre_ = (int)value;
+      if (option == TMX_OPT_NU_POST) h->el.nu_post_ = (int)value;
+      if (option == TMX_OPT_NU_COARSE) h->el.nu_coarse_ = (int)value;
+      if (option == TMX_OPT_NU_FINE_FILTER) h->fl.nu_pre_ = h->fl.nu_post_ = (int)value;
+      if (option == TMX_OPT_NU_COARSE_FILTER) h->fl.nu_coarse_ = (int)value;
+      break;
+    case TMX_OPT_FILTER_RTOL:
+      if (!(value > 0 && value < 1)) return invalid("filter rtol must be in (0, 1)");
+      h->fl_prm.rtol = value;
+      break;
+    case TMX_OPT_OMEGA_ELAST:
+    case TMX_OPT_OMEGA_FILTER:
+      if (!(value > 0 && value < 2)) return invalid("omega must be in (0, 2)");
+      (option == TMX_OPT_OMEGA_ELAST ? h->el : h->fl).omega_ = value;
+      break;
+    default:
+      return invalid("unknown option");
+  }
+  for (auto* x : hs) x->drop_graphs();
+  return TMX_OK;
+}
+
+extern "C" int tmx_precondition(tmx_problem* h, int which, int precision, const double* xi,
+                                const double* r, double* z) {
+  if (!h || !r || !z) return invalid("null argument");
+  if (which != 0 && which != 1) return invalid("which must be 0 (elasticity) or 1 (filter)");
+  if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
+  if (which == 0 && !xi) return invalid("xi is required for the elasticity preconditioner");
+  if (which == 0) TMX_TRY(need_penalty(h));
+  Scope scope(h);
+  hipStream_t s = h->stream;
+  Hierarchy& hy = which == 0 ? h->el : h->fl;
+  if (which == 0) TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, s));
+  const size_t n = hy.n();
+  if (precision == TMX_FP64) {
+    TMX_HIP(copy_vec<double>(n, r, hy.r64_.get(), s));
+    TMX_TRY(hy.vcycle<double>(0, hy.r64_.get(), hy.z64_.get(), s));
+    TMX_HIP(copy_vec<double>(n, hy.z64_.get(), z, s));
+  } else {
+    TMX_HIP((convert<double, float>(n, r, hy.r32_.get(), s)));
+    TMX_TRY(hy.vcycle<float>(0, hy.r32_.get(), hy.z32_.get(), s));
+    TMX_HIP((convert<float, double>(n, hy.z32_.get(), z, s)));
+  }
   return TMX_OK;
 }
 
@@ -359,16 +460,13 @@ extern "C" int tmx_set_tiles(tmx_problem* h, int rows64, int rows32) {
 
 extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) {
   if (!h) return invalid("null handle");
-  // needs an operator: use xi = 1 (uniform) and p = current or 3
+  Scope scope(h);
+  // times the fine operator with xi = 0.5 and p = current (or 3)
   const size_t nv = nverts(h);
   const int p = h->has_penalty ? h->p : 3;
-  h->el.xi64_ = h->xi_s.get();
-  h->el.p_ = p;
-  h->el.m_ = h->m;
-  h->el.cp_ = (1.0 - h->m) * 2.0 / ((p + 1.0) * (p + 2.0));
   hipStream_t s = h->stream;
   TMX_HIP(fill(nv, 0.5, h->xi_s.get(), s));
-  TMX_HIP((convert<double, float>(nv, h->xi_s.get(), h->el.xi32_.get(), s)));
+  TMX_TRY(h->el.set_fine_operator(h->xi_s.get(), p, h->m, s));
   TMX_HIP(fill(2 * nv, 1.0, h->u_s.get(), s));
   TMX_HIP((convert<double, float>(2 * nv, h->u_s.get(), h->u32.get(), s)));
   hipEvent_t e0, e1;
@@ -422,7 +520,8 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
   if (!h || !avg_us || reps < 1) return invalid("bad arguments");
   if (kernel < 0 || kernel > 2) return invalid("unknown kernel id");
   if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
-  if (kernel < 2 && !h->el.xi64_) return invalid("no elasticity operator yet: solve first");
+  if (kernel < 2 && !h->el.has_operator_) return invalid("no elasticity operator yet: solve first");
+  Scope scope(h);
   const size_t nv = nverts(h);
   hipStream_t s = h->stream;
   const bool f64 = precision == TMX_FP64;

@@ -59,35 +59,22 @@ __device__ __forceinline__ T hsym3(T a, T b, T c, int p) {
 }
 
 // ---------------------------------------------------------------- reductions
-// Sum a double over the block, write the block's partial, and let the last block to
-// finish add all partials in block order (deterministic) into *result.
+// Each block writes its partial sum with a plain store; a separate one-block kernel adds the
+// partials in block order (deterministic).  Kernel boundaries order the two, so no fences
+// or atomics are needed.
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
   return v;
 }
 
-struct Reduction {
-  double* partials;   // >= gridDim.x entries
-  unsigned* counter;  // zero before the launch; the last block resets it
-  double* result;     // where the final sum goes
-};
-
-__device__ __forceinline__ double load_coherent(const double* p) {
-  unsigned long long bits =
-      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                        __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)bits);
-}
-
 template <int NRED>
-__device__ void block_reduce_finalize(const double (&v)[NRED], const Reduction (&red)[NRED]) {
+__device__ __forceinline__ void block_partials(const double (&v)[NRED], double* const (&out)[NRED]) {
   __shared__ double sh[NRED][kBlock / kWave];
-  __shared__ bool last;
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
 #pragma unroll
   for (int k = 0; k < NRED; ++k) {
-    double s = wave_sum(v[k]);
+    const double s = wave_sum(v[k]);
     if (lane == 0) sh[k][w] = s;
   }
   __syncthreads();
@@ -96,33 +83,24 @@ __device__ void block_reduce_finalize(const double (&v)[NRED], const Reduction (
     for (int k = 0; k < NRED; ++k) {
       double s = 0.0;
       for (int i = 0; i < nw; ++i) s += sh[k][i];
-      __hip_atomic_store(reinterpret_cast<unsigned long long*>(&red[k].partials[blockIdx.x]),
-                         (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      out[k][blockIdx.x] = s;
     }
-    __threadfence();
-    unsigned prev = atomicAdd(red[0].counter, 1u);
-    last = (prev == gridDim.x - 1);
   }
+}
+
+// fixed-order sum of n partials by one block (all threads receive the result)
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int n) {
+  __shared__ double sh[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  s = wave_sum(s);
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < NRED; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) s += load_coherent(&red[k].partials[i]);
-    // fixed-order tree over the block
-    s = wave_sum(s);
-    __syncthreads();
-    if (lane == 0) sh[k][w] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int i = 0; i < nw; ++i) t += sh[k][i];
-      *red[k].result = t;
-    }
-  }
-  if (threadIdx.x == 0) *red[0].counter = 0u;
+  if (lane == 0) sh[w] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < (int)(blockDim.x / kWave); ++i) t += sh[i];
+  return t;
 }
 
 }  // namespace tmx

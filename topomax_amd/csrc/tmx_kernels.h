@@ -14,13 +14,19 @@ namespace tmx {
 // Dirichlet side bits (same values as TMX_FIX_* in include/tmx.h)
 enum SideBits : unsigned { kLeft = 1u, kRight = 2u, kTop = 4u, kBottom = 8u };
 
-struct RedSlot {          // one deterministic reduction target
-  double* partials;       // scratch, >= kMaxPartials doubles
-  unsigned* counter;      // zero-initialised, reset by the kernel
-  double* result;         // device scalar
+// Deterministic two-stage reductions: a kernel writes one partial per block into
+// `partials` (kMaxPartials doubles) and records the block count; a one-block kernel adds them.
+struct RedSlot {
+  double* partials;
+  int nblocks;
 };
 
-constexpr int kMaxPartials = 65536;
+constexpr int kMaxPartials = 16384;
+
+// Device scalar slots of one PCG run (indices into a double array).
+struct PcgSlots {
+  int rz, pq, rr, alpha, beta, flag, tol2, iters;
+};
 
 // ------------------------------------------------------------------ elasticity
 template <typename T>
@@ -33,12 +39,28 @@ struct ElastConst {       // level constants, see ElastConst<T>::make
 struct SweepShape {
   int nx, ny;
   int rows;      // node rows per wavefront chunk
+  int stripw;    // output columns per wavefront (64 minus the halo lanes)
   int nstrips, nchunks;
-  static SweepShape make(int nx, int ny, int rows);
+  static SweepShape make(int nx, int ny, int rows, int stripw = 62);
   int blocks() const;
 };
 
-enum SweepOp : int { kApply = 0, kResidual = 1, kSmooth = 2, kSmoothZero = 3 };
+// kResidualRestrict: y = P^T (b - A x) on the nested coarse grid (nx, ny even);
+// kProlongSmooth: one Jacobi sweep applied to x + P e (e = nested coarse correction).
+// kApplyUpdate (PCG direction update fused with the operator): reads p_old = x and z = b,
+// forms p = z + beta p_old with beta = *fused.beta, writes p to fused.pnew and y = A p.
+// A dot slot receives partials of x.y (kApply / kApplyUpdate: p.Ap) or b.y (sweeps: r.z).
+enum SweepOp : int {
+  kApply = 0, kResidual = 1, kSmooth = 2, kSmoothZero = 3, kResidualRestrict = 4,
+  kProlongSmooth = 5, kApplyUpdate = 6
+};
+
+template <typename T>
+struct SweepExtra {
+  const T* ec = nullptr;         // kProlongSmooth: coarse correction
+  T* pnew = nullptr;             // kApplyUpdate: updated direction
+  const double* beta = nullptr;  // kApplyUpdate: device scalar
+};
 
 template <typename T>
 struct ElastOpArgs {
@@ -53,7 +75,7 @@ struct ElastOpArgs {
 
 template <typename T>
 hipError_t elast_sweep(const ElastOpArgs<T>& a, int op, const T* x, const T* b, T* y, T omega,
-                       const RedSlot* dot_xy, hipStream_t s);
+                       RedSlot* dot, hipStream_t s, const SweepExtra<T>& ex = SweepExtra<T>());
 
 // per-cell mean r from nodal xi (for restriction of level-0 weights)
 template <typename T>
@@ -92,7 +114,8 @@ struct FilterConst {
 
 template <typename T>
 hipError_t filter_sweep(const SweepShape& sh, const FilterConst<T>& k, int op, const T* x,
-                        const T* b, T* y, T omega, const RedSlot* dot_xy, hipStream_t s);
+                        const T* b, T* y, T omega, RedSlot* dot, hipStream_t s,
+                        const SweepExtra<T>& ex = SweepExtra<T>());
 template <typename T>
 hipError_t filter_to_stencil(int nx, int ny, const FilterConst<T>& k, T* st, hipStream_t s);
 
@@ -114,12 +137,41 @@ hipError_t prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, 
 template <typename T, int BS>
 hipError_t stencil_op(int nx, int ny, const T* st, int op, const T* x, const T* b, T* y,
                       T omega, hipStream_t s);
+// fused bilinear transfers of a 9-point level: bc = P^T (b - A x); y = sweep(x + P e)
+template <typename T, int BS>
+hipError_t stencil_resrestrict(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* st,
+                               const T* x, const T* b, T* bc, hipStream_t s);
+template <typename T, int BS>
+hipError_t stencil_prolong_smooth(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* st,
+                                  const T* x, const T* ec, const T* b, T* y, T omega,
+                                  hipStream_t s);
 // Galerkin coarse stencil Ac = P^T Af P for the given transfer (nested or bilinear)
 template <int BS>
 hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
                        const double* stf, double* stc, hipStream_t s);
 template <int BS>
 hipError_t stencil_to_dense(int nx, int ny, const double* st, double* dense, hipStream_t s);
+
+// Bottom of a V-cycle in one workgroup: 9-point levels lev[0..nlev-2] (bilinear transfers)
+// and the dense coarsest lev[nlev-1] (x = Ainv b).  lev[0].b is the input, lev[0].x the output.
+constexpr int kMaxTail = 8;
+constexpr int kTailNodes = 1024;
+template <typename T>
+struct TailLevel {
+  int nx, ny;
+  const T* st;
+  T *x, *t, *b, *r;
+};
+template <typename T>
+struct TailDesc {
+  int nlev, cn, nu_pre, nu_post;
+  unsigned fixed;
+  T omega;
+  const double* dense;
+  TailLevel<T> lev[kMaxTail];
+};
+template <typename T, int BS>
+hipError_t tail_vcycle(const TailDesc<T>& d, hipStream_t s);
 hipError_t dense_invert(int n, double* a, double* work, hipStream_t s);
 template <typename T>
 hipError_t dense_apply(int n, const double* ainv, const T* b, T* x, hipStream_t s);
@@ -127,17 +179,36 @@ template <typename TI, typename TO>
 hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s);
 
 // ------------------------------------------------------------------ vectors
-hipError_t dot(size_t n, const double* a, const double* b, const RedSlot& r, hipStream_t s);
 template <typename T>
-hipError_t dot_t(size_t n, const T* a, const T* b, const RedSlot& r, hipStream_t s);
-// PCG: alpha = sc[rz]/sc[pq]; x += alpha p; r -= alpha q; rr = r.r
+hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s);
+// out = sum of r's partials (one block)
+hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s);
+// dot_t + reduce_to
+hipError_t dot(size_t n, const double* a, const double* b, RedSlot& r, double* out,
+               hipStream_t s);
+// PCG scalars on the device.  flag: 0 running, 1 converged, 2 p.Ap <= 0, 3 non-finite.
+// alpha: pq = sum; alpha = rz/pq (0 once stopped).  rr: rr = sum; converged if rr <= tol2.
+// beta: rznew = sum; beta = rznew/rz; rz = rznew.
+hipError_t pcg_alpha(const RedSlot& r, double* sc, const PcgSlots& q, hipStream_t s);
+hipError_t pcg_rr(const RedSlot& r, double* sc, const PcgSlots& q, hipStream_t s);
+hipError_t pcg_beta(const RedSlot& r, double* sc, const PcgSlots& q, hipStream_t s);
+// x += alpha p; r -= alpha q (alpha = sc[i_alpha]); partials of r.r
 template <typename T>
-hipError_t pcg_update_xr(size_t n, const double* sc, int i_rz, int i_pq, T* x, T* r, const T* p,
-                         const T* q, const RedSlot& rr, hipStream_t s);
-// PCG: beta = sc[rznew]/sc[rz]; p = z + beta p (the caller alternates the two rz slots)
+hipError_t pcg_update_xr(size_t n, const double* sc, int i_alpha, T* x, T* r, const T* p,
+                         const T* q, RedSlot& rr, hipStream_t s);
+// as pcg_update_xr, with alpha = rz / sum(pq partials) formed by every block (0 once stopped
+// or if p.Ap <= 0)
 template <typename T>
-hipError_t pcg_update_p(size_t n, double* sc, int i_rznew, int i_rz, const T* z, T* p,
-                        hipStream_t s);
+hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, const RedSlot& pq,
+                               T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s);
+// end of a fused iteration: pq, rr, rz from partials; flags; beta = rz_new / rz; rz = rz_new
+hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
+                        const PcgSlots& q, hipStream_t s);
+// start of a fused run: rz from partials, beta = 0, flag = 0, iters = 0
+hipError_t pcg_start(const RedSlot& rz, double* sc, const PcgSlots& q, hipStream_t s);
+// p = z + beta p, beta = sc[i_beta]
+template <typename T>
+hipError_t pcg_update_p(size_t n, const double* sc, int i_beta, const T* z, T* p, hipStream_t s);
 template <typename T>
 hipError_t copy_vec(size_t n, const T* a, T* b, hipStream_t s);
 // x64 += sqrt(sc[i]) * d (d in T): undoes the scaling of scale_to
@@ -148,6 +219,8 @@ template <typename T>
 hipError_t scale_to(size_t n, const double* sc, int i, const double* in, T* out, hipStream_t s);
 hipError_t fill_zero_bytes(void* p, size_t bytes, hipStream_t s);
 hipError_t fill(size_t n, double val, double* p, hipStream_t s);
+// sc[i] = v (kernel argument, safe to enqueue before earlier work completes)
+hipError_t set_scalar(double* sc, int i, double v, hipStream_t s);
 // zero both components of the Dirichlet nodes of an interleaved vector
 hipError_t mask_fixed(int nx, int ny, unsigned fixed, double* v, hipStream_t s);
 

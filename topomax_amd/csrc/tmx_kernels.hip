@@ -19,12 +19,13 @@
 namespace tmx {
 
 // ============================================================================ shapes
-SweepShape SweepShape::make(int nx, int ny, int rows) {
+SweepShape SweepShape::make(int nx, int ny, int rows, int stripw) {
   SweepShape s;
   s.nx = nx;
   s.ny = ny;
   s.rows = rows < 1 ? 1 : rows;
-  s.nstrips = (nx + 1 + kStripCols - 1) / kStripCols;
+  s.stripw = stripw;
+  s.nstrips = (nx + 1 + stripw - 1) / stripw;
   s.nchunks = (ny + 1 + s.rows - 1) / s.rows;
   return s;
 }
@@ -111,26 +112,41 @@ struct Diag3 {
 template <typename T, bool XI, int OP>
 __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const T* __restrict__ x,
                                                         const T* __restrict__ b, T* __restrict__ y,
-                                                        T omega, RedSlot red, int want_dot) {
+                                                        T omega, double* __restrict__ dpart,
+                                                        SweepExtra<T> ex) {
   using V = V2<T>;
+  constexpr bool RR = (OP == kResidualRestrict);  // residual -> nested restriction (y coarse)
+  constexpr bool PS = (OP == kProlongSmooth);     // x + P e, then one Jacobi sweep
+  constexpr bool AU = (OP == kApplyUpdate);       // p = z + beta p_old, y = A p
+  const T* __restrict__ ec = ex.ec;
+  constexpr bool SMOOTH = (OP == kSmooth || OP == kSmoothZero || PS);
+  constexpr int H = RR ? 2 : 1;                   // halo lanes on each side of the strip
   const SweepShape sh = a.sh;
+  const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
-  const int c = strip * kStripCols - 1 + lane;
+  const int nxc = nx >> 1;
+  const int c = strip * sh.stripw - H + lane;
   const bool colv = wave_ok && c >= 0 && c <= nx;
   const bool cellc = wave_ok && c >= 0 && c < nx;
-  const bool outl = colv && lane >= 1 && lane <= kStripCols;
+  const bool outl = colv && lane >= H && lane < kWave - H;
   const unsigned fixed = a.fixed;
   const bool colfix = ((fixed & kLeft) && c == 0) || ((fixed & kRight) && c == nx);
   const int r0 = chunk * sh.rows;
-  const int r1 = wave_ok ? min(r0 + sh.rows, ny + 1) : r0 - 1;
+  const int rend = min(r0 + sh.rows, ny + 1);  // owned node rows [r0, rend)
+  // cell rows swept: a node row is complete once the cell rows below and above it are done
+  const int cy_begin = RR ? r0 - 2 : r0 - 1;
+  const int cy_end = !wave_ok ? cy_begin : (RR ? min(rend, ny) + 1 : rend);
   const V* X = reinterpret_cast<const V*>(x);
   const V* B = reinterpret_cast<const V*>(b);
   V* Y = reinterpret_cast<V*>(y);
   const V* CW = reinterpret_cast<const V*>(a.cellw);
+  const V* EC = reinterpret_cast<const V*>(ec);
+  V* PN = reinterpret_cast<V*>(ex.pnew);
+  const T beta = AU ? (T)*ex.beta : T(0);
   const ElastConst<T> k = a.k;
 
   auto row_fixed = [&](int iy) {
@@ -138,7 +154,26 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   };
   auto ld_x = [&](int iy, V& raw) -> V {
     V v = {T(0), T(0)};
-    if (OP != kSmoothZero && colv && iy >= 0 && iy <= ny) v = X[(size_t)iy * nvx + c];
+    const bool ok = colv && iy >= 0 && iy <= ny;
+    if (OP != kSmoothZero && ok) v = X[(size_t)iy * nvx + c];
+    if constexpr (AU) {
+      if (ok) {
+        const V z = B[(size_t)iy * nvx + c];
+        v = {z.x + beta * v.x, z.y + beta * v.y};
+      }
+    }
+    if constexpr (PS) {
+      // nested P1 interpolation of the coarse correction at (c, iy)
+      if (ok && !row_fixed(iy)) {
+        const int I0 = c >> 1, J0 = iy >> 1, ox = c & 1, oy = iy & 1;
+        V e = EC[(size_t)J0 * (nxc + 1) + I0];
+        if (ox | oy) {
+          const V e1 = EC[(size_t)(J0 + oy) * (nxc + 1) + I0 + ox];
+          e = {T(0.5) * (e.x + e1.x), T(0.5) * (e.y + e1.y)};
+        }
+        v += e;
+      }
+    }
     raw = v;
     if (row_fixed(iy)) v = {T(0), T(0)};
     return v;
@@ -148,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     return T(0);
   };
 
-  int cy = r0 - 1;
+  int cy = cy_begin;
   V ub_raw;
   V ub = ld_x(cy, ub_raw);
   T zb = ld_xi(cy);
@@ -157,8 +192,9 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   V acc_b = {T(0), T(0)};
   Diag3<T> d_b = {T(0), T(0), T(0)};
   double dot = 0.0;
+  V rA = {T(0), T(0)}, rB = {T(0), T(0)};  // residual rows cy-2, cy-1 (RR)
 
-  for (; cy < r1; ++cy) {
+  for (; cy < cy_end; ++cy) {
     V ut_raw;
     const V ut = ld_x(cy + 1, ut_raw);
     const T zt = ld_xi(cy + 1);
@@ -186,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
       acc_t += lane_up(f3);
     }
     Diag3<T> d_t = {T(0), T(0), T(0)};
-    if constexpr (OP == kSmooth || OP == kSmoothZero) {
+    if constexpr (SMOOTH) {
       const T wAl = lane_up(wA), wBl = lane_up(wB);
       // own bottom node = v0 of this cell (A and B) and v1 of the left cell (A)
       d_b.xx += wA * k.Pxx + wB * k.Qxx + wAl * k.Rxx;
@@ -197,17 +233,45 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
       d_t.xy = wB * k.Rxy;
       d_t.yy = wB * k.Ryy + wAl * k.Qyy + wBl * k.Pyy;
     }
-    if (cy >= r0 && outl) {
-      const size_t idx = (size_t)cy * nvx + c;
-      const bool fx = row_fixed(cy);
+    // node row cy is complete
+    const bool rowv = colv && cy >= 0 && cy <= ny;
+    const size_t idx = (size_t)cy * nvx + c;
+    const bool fx = row_fixed(cy);
+    if constexpr (RR) {
       const V Ax = fx ? ub_raw : acc_b;
-      if constexpr (OP == kApply) {
+      V rC = {T(0), T(0)};
+      if (rowv) rC = B[idx] - Ax;
+      // coarse row J: centre fine row 2J = cy-1 (cy odd), or 2J = ny at the top (ny even)
+      const bool odd_emit = (cy & 1) && cy - 1 >= r0 && cy - 1 < rend;
+      const bool top_emit = cy == ny && ny >= r0 && ny < rend;
+      if (odd_emit || top_emit) {
+        const V lo = odd_emit ? rA : rB, mid = odd_emit ? rB : rC;
+        const V hi = odd_emit ? rC : V{T(0), T(0)};
+        const V mr = lane_down(mid), ml = lane_up(mid), hr = lane_down(hi), ll = lane_up(lo);
+        const int J = odd_emit ? (cy - 1) >> 1 : ny >> 1;
+        if (outl && !(c & 1)) {
+          const int I = c >> 1;
+          V o = {T(0), T(0)};
+          if (!node_fixed(fixed, I, J, nxc, ny >> 1)) {
+            o.x = mid.x + T(0.5) * (mr.x + ml.x + hi.x + lo.x + hr.x + ll.x);
+            o.y = mid.y + T(0.5) * (mr.y + ml.y + hi.y + lo.y + hr.y + ll.y);
+          }
+          Y[(size_t)J * (nxc + 1) + I] = o;
+        }
+      }
+      rA = rB;
+      rB = rC;
+    } else if (cy >= r0 && outl) {
+      const V Ax = fx ? ub_raw : acc_b;
+      if constexpr (OP == kApply || AU) {
         Y[idx] = Ax;
+        if constexpr (AU) PN[idx] = ub_raw;
         if (want_dot) dot += (double)ub_raw.x * (double)Ax.x + (double)ub_raw.y * (double)Ax.y;
       } else if constexpr (OP == kResidual) {
         Y[idx] = B[idx] - Ax;
       } else {
-        const V res = B[idx] - Ax;
+        const V bb = B[idx];
+        const V res = bb - Ax;
         V d;
         if (fx) {
           d = res;
@@ -216,7 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
           const T inv = T(1) / det;
           d = {(d_b.yy * res.x - d_b.xy * res.y) * inv, (d_b.xx * res.y - d_b.xy * res.x) * inv};
         }
-        Y[idx] = {ub_raw.x + omega * d.x, ub_raw.y + omega * d.y};
+        const V yo = {ub_raw.x + omega * d.x, ub_raw.y + omega * d.y};
+        Y[idx] = yo;
+        if (want_dot) dot += (double)bb.x * (double)yo.x + (double)bb.y * (double)yo.y;
       }
     }
     acc_b = acc_t;
@@ -227,38 +293,59 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     zb = zt;
     zbr = ztr;
   }
-  if constexpr (OP == kApply) {
+  if constexpr (!RR) {
     if (want_dot) {
       const double v[1] = {dot};
-      const Reduction r[1] = {{red.partials, red.counter, red.result}};
-      block_reduce_finalize<1>(v, r);
+      double* const o[1] = {dpart};
+      block_partials<1>(v, o);
     }
   }
 }
 
 template <typename T>
-hipError_t elast_sweep(const ElastOpArgs<T>& a, int op, const T* x, const T* b, T* y, T omega,
-                       const RedSlot* dot_xy, hipStream_t s) {
+hipError_t elast_sweep(const ElastOpArgs<T>& a0, int op, const T* x, const T* b, T* y, T omega,
+                       RedSlot* dot_xy, hipStream_t s, const SweepExtra<T>& ex) {
+  const T* ec = ex.ec;
+  ElastOpArgs<T> a = a0;
+  if (op == kResidualRestrict) {
+    if ((a.sh.nx & 1) || (a.sh.ny & 1)) return hipErrorInvalidValue;
+    a.sh = SweepShape::make(a.sh.nx, a.sh.ny, std::max(2, a.sh.rows & ~1), kStripCols - 2);
+  }
+  if (op == kProlongSmooth && (!ec || (a.sh.nx & 1) || (a.sh.ny & 1))) return hipErrorInvalidValue;
+  if (op == kApplyUpdate && (!ex.pnew || !ex.beta || !b)) return hipErrorInvalidValue;
+  if (op == kResidualRestrict && dot_xy) return hipErrorInvalidValue;
   const int nb = a.sh.blocks();
-  RedSlot r = dot_xy ? *dot_xy : RedSlot{nullptr, nullptr, nullptr};
-  const int wd = dot_xy ? 1 : 0;
+  double* dp = nullptr;
+  if (dot_xy) {
+    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    dp = dot_xy->partials;
+    dot_xy->nblocks = nb;
+  }
   const bool xi = a.xi != nullptr;
 #define TMX_LAUNCH(XIV, OPV) \
-  k_elast_sweep<T, XIV, OPV><<<nb, kBlock, 0, s>>>(a, x, b, y, omega, r, wd)
+  k_elast_sweep<T, XIV, OPV><<<nb, kBlock, 0, s>>>(a, x, b, y, omega, dp, ex)
+#define TMX_CASE(OPV) \
+  case OPV: if (xi) TMX_LAUNCH(true, OPV); else TMX_LAUNCH(false, OPV); break;
   switch (op) {
-    case kApply: if (xi) TMX_LAUNCH(true, kApply); else TMX_LAUNCH(false, kApply); break;
-    case kResidual: if (xi) TMX_LAUNCH(true, kResidual); else TMX_LAUNCH(false, kResidual); break;
-    case kSmooth: if (xi) TMX_LAUNCH(true, kSmooth); else TMX_LAUNCH(false, kSmooth); break;
-    default: if (xi) TMX_LAUNCH(true, kSmoothZero); else TMX_LAUNCH(false, kSmoothZero); break;
+    TMX_CASE(kApply)
+    TMX_CASE(kResidual)
+    TMX_CASE(kSmooth)
+    TMX_CASE(kSmoothZero)
+    TMX_CASE(kResidualRestrict)
+    TMX_CASE(kProlongSmooth)
+    TMX_CASE(kApplyUpdate)
+    default: return hipErrorInvalidValue;
   }
+#undef TMX_CASE
 #undef TMX_LAUNCH
   return hipGetLastError();
 }
 template hipError_t elast_sweep<double>(const ElastOpArgs<double>&, int, const double*,
-                                        const double*, double*, double, const RedSlot*,
-                                        hipStream_t);
+                                        const double*, double*, double, RedSlot*, hipStream_t,
+                                        const SweepExtra<double>&);
 template hipError_t elast_sweep<float>(const ElastOpArgs<float>&, int, const float*, const float*,
-                                       float*, float, const RedSlot*, hipStream_t);
+                                       float*, float, RedSlot*, hipStream_t,
+                                       const SweepExtra<float>&);
 
 template <typename T>
 __global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
@@ -430,10 +517,10 @@ __global__ __launch_bounds__(kBlock) void k_elast_sensitivity(SensArgs a, const 
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
-  const int c = strip * kStripCols - 1 + lane;
+  const int c = strip * sh.stripw - 1 + lane;
   const bool colv = wave_ok && c >= 0 && c <= nx;
   const bool cellc = wave_ok && c >= 0 && c < nx;
-  const bool outl = colv && lane >= 1 && lane <= kStripCols;
+  const bool outl = colv && lane >= 1 && lane <= sh.stripw;
   const int r0 = chunk * sh.rows;
   const int r1 = wave_ok ? min(r0 + sh.rows, ny + 1) : r0 - 1;
   const V* U = reinterpret_cast<const V*>(u);
@@ -635,30 +722,56 @@ template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterConst<T> k,
                                                          const T* __restrict__ x,
                                                          const T* __restrict__ b,
-                                                         T* __restrict__ y, T omega, RedSlot red,
-                                                         int want_dot) {
+                                                         T* __restrict__ y, T omega,
+                                                         double* __restrict__ dpart,
+                                                         SweepExtra<T> ex) {
+  constexpr bool RR = (OP == kResidualRestrict);
+  constexpr bool PS = (OP == kProlongSmooth);
+  constexpr bool AU = (OP == kApplyUpdate);
+  const T* __restrict__ ec = ex.ec;
+  const T beta = AU ? (T)*ex.beta : T(0);
+  constexpr bool SMOOTH = (OP == kSmooth || OP == kSmoothZero || PS);
+  constexpr int H = RR ? 2 : 1;
+  const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
-  const int c = strip * kStripCols - 1 + lane;
+  const int nxc = nx >> 1;
+  const int c = strip * sh.stripw - H + lane;
   const bool colv = wave_ok && c >= 0 && c <= nx;
   const bool cellc = wave_ok && c >= 0 && c < nx;
-  const bool outl = colv && lane >= 1 && lane <= kStripCols;
+  const bool outl = colv && lane >= H && lane < kWave - H;
   const int r0 = chunk * sh.rows;
-  const int r1 = wave_ok ? min(r0 + sh.rows, ny + 1) : r0 - 1;
+  const int rend = min(r0 + sh.rows, ny + 1);
+  const int cy_begin = RR ? r0 - 2 : r0 - 1;
+  const int cy_end = !wave_ok ? cy_begin : (RR ? min(rend, ny) + 1 : rend);
   auto ld = [&](int iy) -> T {
-    if (OP != kSmoothZero && colv && iy >= 0 && iy <= ny) return x[(size_t)iy * nvx + c];
-    return T(0);
+    T v = T(0);
+    const bool ok = colv && iy >= 0 && iy <= ny;
+    if (OP != kSmoothZero && ok) v = x[(size_t)iy * nvx + c];
+    if constexpr (AU) {
+      if (ok) v = b[(size_t)iy * nvx + c] + beta * v;
+    }
+    if constexpr (PS) {
+      if (ok) {
+        const int I0 = c >> 1, J0 = iy >> 1, ox = c & 1, oy = iy & 1;
+        T e = ec[(size_t)J0 * (nxc + 1) + I0];
+        if (ox | oy) e = T(0.5) * (e + ec[(size_t)(J0 + oy) * (nxc + 1) + I0 + ox]);
+        v += e;
+      }
+    }
+    return v;
   };
   const T ka = k.ka, kb = k.kb, mA = k.mA;
   const T dfull = ka + kb + T(4) * mA, dhalf = ka + kb + T(2) * mA;
-  int cy = r0 - 1;
+  int cy = cy_begin;
   T ub = ld(cy), ubr = lane_down(ub);
   T acc_b = T(0), d_b = T(0);
+  T rA = T(0), rB = T(0);
   double dot = 0.0;
-  for (; cy < r1; ++cy) {
+  for (; cy < cy_end; ++cy) {
     const T ut = ld(cy + 1), utr = lane_down(ut);
     const bool cell_ok = cellc && cy >= 0 && cy < ny;
     const T w = cell_ok ? T(1) : T(0);
@@ -674,20 +787,39 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
       acc_t = f2 + lane_up(f3);
     }
     T d_t = T(0);
-    if constexpr (OP == kSmooth || OP == kSmoothZero) {
+    if constexpr (SMOOTH) {
       const T wl = lane_up(w);
       d_b += w * dfull + wl * dhalf;
       d_t = w * dhalf + wl * dfull;
     }
-    if (cy >= r0 && outl) {
-      const size_t idx = (size_t)cy * nvx + c;
-      if constexpr (OP == kApply) {
+    const bool rowv = colv && cy >= 0 && cy <= ny;
+    const size_t idx = (size_t)cy * nvx + c;
+    if constexpr (RR) {
+      T rC = T(0);
+      if (rowv) rC = b[idx] - acc_b;
+      const bool odd_emit = (cy & 1) && cy - 1 >= r0 && cy - 1 < rend;
+      const bool top_emit = cy == ny && ny >= r0 && ny < rend;
+      if (odd_emit || top_emit) {
+        const T lo = odd_emit ? rA : rB, mid = odd_emit ? rB : rC, hi = odd_emit ? rC : T(0);
+        const T mr = lane_down(mid), ml = lane_up(mid), hr = lane_down(hi), ll = lane_up(lo);
+        const int J = odd_emit ? (cy - 1) >> 1 : ny >> 1;
+        if (outl && !(c & 1))
+          y[(size_t)J * (nxc + 1) + (c >> 1)] = mid + T(0.5) * (mr + ml + hi + lo + hr + ll);
+      }
+      rA = rB;
+      rB = rC;
+    } else if (cy >= r0 && outl) {
+      if constexpr (OP == kApply || AU) {
         y[idx] = acc_b;
+        if constexpr (AU) ex.pnew[idx] = ub;
         if (want_dot) dot += (double)ub * (double)acc_b;
       } else if constexpr (OP == kResidual) {
         y[idx] = b[idx] - acc_b;
       } else {
-        y[idx] = ub + omega * (b[idx] - acc_b) / d_b;
+        const T bb = b[idx];
+        const T yo = ub + omega * (bb - acc_b) / d_b;
+        y[idx] = yo;
+        if (want_dot) dot += (double)bb * (double)yo;
       }
     }
     acc_b = acc_t;
@@ -695,35 +827,56 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
     ub = ut;
     ubr = utr;
   }
-  if constexpr (OP == kApply) {
+  if constexpr (!RR) {
     if (want_dot) {
       const double v[1] = {dot};
-      const Reduction r[1] = {{red.partials, red.counter, red.result}};
-      block_reduce_finalize<1>(v, r);
+      double* const o[1] = {dpart};
+      block_partials<1>(v, o);
     }
   }
 }
 
 template <typename T>
-hipError_t filter_sweep(const SweepShape& sh, const FilterConst<T>& k, int op, const T* x,
-                        const T* b, T* y, T omega, const RedSlot* dot_xy, hipStream_t s) {
-  const int nb = sh.blocks();
-  RedSlot r = dot_xy ? *dot_xy : RedSlot{nullptr, nullptr, nullptr};
-  const int wd = dot_xy ? 1 : 0;
-  switch (op) {
-    case kApply: k_filter_sweep<T, kApply><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
-    case kResidual: k_filter_sweep<T, kResidual><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
-    case kSmooth: k_filter_sweep<T, kSmooth><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
-    default: k_filter_sweep<T, kSmoothZero><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, r, wd); break;
+hipError_t filter_sweep(const SweepShape& sh0, const FilterConst<T>& k, int op, const T* x,
+                        const T* b, T* y, T omega, RedSlot* dot_xy, hipStream_t s,
+                        const SweepExtra<T>& ex) {
+  const T* ec = ex.ec;
+  SweepShape sh = sh0;
+  if (op == kResidualRestrict) {
+    if ((sh.nx & 1) || (sh.ny & 1)) return hipErrorInvalidValue;
+    sh = SweepShape::make(sh.nx, sh.ny, std::max(2, sh.rows & ~1), kStripCols - 2);
   }
+  if (op == kProlongSmooth && (!ec || (sh.nx & 1) || (sh.ny & 1))) return hipErrorInvalidValue;
+  if (op == kApplyUpdate && (!ex.pnew || !ex.beta || !b)) return hipErrorInvalidValue;
+  if (op == kResidualRestrict && dot_xy) return hipErrorInvalidValue;
+  const int nb = sh.blocks();
+  double* dp = nullptr;
+  if (dot_xy) {
+    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    dp = dot_xy->partials;
+    dot_xy->nblocks = nb;
+  }
+#define TMX_CASE(OPV) \
+  case OPV: k_filter_sweep<T, OPV><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, dp, ex); break;
+  switch (op) {
+    TMX_CASE(kApply)
+    TMX_CASE(kResidual)
+    TMX_CASE(kSmooth)
+    TMX_CASE(kSmoothZero)
+    TMX_CASE(kResidualRestrict)
+    TMX_CASE(kProlongSmooth)
+    TMX_CASE(kApplyUpdate)
+    default: return hipErrorInvalidValue;
+  }
+#undef TMX_CASE
   return hipGetLastError();
 }
 template hipError_t filter_sweep<double>(const SweepShape&, const FilterConst<double>&, int,
                                          const double*, const double*, double*, double,
-                                         const RedSlot*, hipStream_t);
+                                         RedSlot*, hipStream_t, const SweepExtra<double>&);
 template hipError_t filter_sweep<float>(const SweepShape&, const FilterConst<float>&, int,
-                                        const float*, const float*, float*, float, const RedSlot*,
-                                        hipStream_t);
+                                        const float*, const float*, float*, float, RedSlot*,
+                                        hipStream_t, const SweepExtra<float>&);
 
 template <typename T>
 __global__ void k_filter_to_stencil(int nx, int ny, FilterConst<T> k, T* __restrict__ st) {
@@ -957,13 +1110,14 @@ TMX_INST_TRANSFER(float, 1)
 TMX_INST_TRANSFER(float, 2)
 
 // ============================================================================ 9-point levels
-// st layout (SoA): st[(k*BS*BS + a*BS + b) * nn + node], k = (dy+1)*3 + (dx+1).
+// st layout (SoA): st[(k*BS*BS + a*BS + b) * nn + node], k = (dy+1)*3 + (dx+1).  Boundary
+// nodes carry zero coefficients for missing neighbours, so neighbour indices are clamped
+// and every load is issued unconditionally (no divergent branches around loads).
 template <typename T, int BS, int OP>
-__global__ void k_stencil_op(int nx, int ny, const T* __restrict__ st, const T* __restrict__ x,
-                             const T* __restrict__ b, T* __restrict__ y, T omega) {
+__device__ __forceinline__ void s9_node(int nx, int ny, const T* __restrict__ st,
+                                        const T* __restrict__ x, const T* __restrict__ b,
+                                        T* __restrict__ y, T omega, size_t v) {
   const size_t nn = (size_t)(nx + 1) * (ny + 1);
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
   const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
   T acc[BS];
 #pragma unroll
@@ -971,17 +1125,12 @@ __global__ void k_stencil_op(int nx, int ny, const T* __restrict__ st, const T* 
   if (OP != kSmoothZero) {
 #pragma unroll
     for (int kk = 0; kk < 9; ++kk) {
-      const int dx = kk % 3 - 1, dy = kk / 3 - 1;
-      const int jx = ix + dx, jy = iy + dy;
-      if (jx < 0 || jy < 0 || jx > nx || jy > ny) continue;
+      const int jx = min(max(ix + kk % 3 - 1, 0), nx), jy = min(max(iy + kk / 3 - 1, 0), ny);
       const size_t j = (size_t)jy * (nx + 1) + jx;
-      T xj[BS];
-#pragma unroll
-      for (int q = 0; q < BS; ++q) xj[q] = x[BS * j + q];
 #pragma unroll
       for (int a = 0; a < BS; ++a)
 #pragma unroll
-        for (int c = 0; c < BS; ++c) acc[a] += st[(size_t)(kk * BS * BS + a * BS + c) * nn + v] * xj[c];
+        for (int c = 0; c < BS; ++c) acc[a] += st[(size_t)(kk * BS * BS + a * BS + c) * nn + v] * x[BS * j + c];
     }
   }
   if (OP == kApply) {
@@ -1014,15 +1163,23 @@ __global__ void k_stencil_op(int nx, int ny, const T* __restrict__ st, const T* 
   }
 }
 
+template <typename T, int BS, int OP>
+__global__ void k_stencil_op(int nx, int ny, const T* __restrict__ st, const T* __restrict__ x,
+                             const T* __restrict__ b, T* __restrict__ y, T omega) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nn) s9_node<T, BS, OP>(nx, ny, st, x, b, y, omega, v);
+}
+
 template <typename T, int BS>
 hipError_t stencil_op(int nx, int ny, const T* st, int op, const T* x, const T* b, T* y, T omega,
                       hipStream_t s) {
-  const unsigned g = grid1d((size_t)(nx + 1) * (ny + 1));
+  const unsigned g = grid1d((size_t)(nx + 1) * (ny + 1), 64);
   switch (op) {
-    case kApply: k_stencil_op<T, BS, kApply><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
-    case kResidual: k_stencil_op<T, BS, kResidual><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
-    case kSmooth: k_stencil_op<T, BS, kSmooth><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
-    default: k_stencil_op<T, BS, kSmoothZero><<<g, kBlock, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    case kApply: k_stencil_op<T, BS, kApply><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    case kResidual: k_stencil_op<T, BS, kResidual><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    case kSmooth: k_stencil_op<T, BS, kSmooth><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    default: k_stencil_op<T, BS, kSmoothZero><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
   }
   return hipGetLastError();
 }
@@ -1034,6 +1191,287 @@ template hipError_t stencil_op<float, 1>(int, int, const float*, int, const floa
                                          float*, float, hipStream_t);
 template hipError_t stencil_op<float, 2>(int, int, const float*, int, const float*, const float*,
                                          float*, float, hipStream_t);
+
+// per-node bilinear transfers (bodies of k_restrict_bilinear / k_prolong_bilinear)
+template <typename T, int BS>
+__device__ __forceinline__ void restrict_bilinear_node(int nxf, int nyf, int nxc, int nyc,
+                                                       unsigned fixed, const T* __restrict__ rf,
+                                                       T* __restrict__ bc, size_t i) {
+  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  double acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = 0.0;
+  if (!node_fixed(fixed, I, J, nxc, nyc)) {
+    const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int fy = py + dy;
+      const double wy = (fy < 0 || fy > nyf) ? 0.0 : interp1_weight(fy, nyf, nyc, J);
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int fx = px + dx;
+        const double w = (fx < 0 || fx > nxf) ? 0.0 : wy * interp1_weight(fx, nxf, nxc, I);
+        const size_t g = (size_t)min(max(fy, 0), nyf) * (nxf + 1) + min(max(fx, 0), nxf);
+#pragma unroll
+        for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) bc[BS * i + q] = (T)acc[q];
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void prolong_bilinear_node(int nxf, int nyf, int nxc, int nyc,
+                                                      unsigned fixed, const T* __restrict__ ec,
+                                                      T* __restrict__ xf, size_t i) {
+  const int fx = (int)(i % (nxf + 1)), fy = (int)(i / (nxf + 1));
+  if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
+  const Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
+  T add[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) add[q] = T(0);
+  for (int b = 0; b < iy.n; ++b)
+    for (int a = 0; a < ix.n; ++a) {
+      const T w = (T)(ix.w[a] * iy.w[b]);
+      const size_t c = (size_t)iy.c[b] * (nxc + 1) + ix.c[a];
+#pragma unroll
+      for (int q = 0; q < BS; ++q) add[q] += w * ec[BS * c + q];
+    }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) xf[BS * i + q] += add[q];
+}
+
+// fused: coarse node i receives P^T (b - A x) of the fine 9-point level
+template <typename T, int BS>
+__device__ __forceinline__ void s9_resrestrict_node(int nxf, int nyf, int nxc, int nyc,
+                                                    unsigned fixed, const T* __restrict__ st,
+                                                    const T* __restrict__ x,
+                                                    const T* __restrict__ b, T* __restrict__ bc,
+                                                    size_t i) {
+  const size_t nnf = (size_t)(nxf + 1) * (nyf + 1);
+  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  double acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = 0.0;
+  if (!node_fixed(fixed, I, J, nxc, nyc)) {
+    const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int fy = py + dy;
+      if (fy < 0 || fy > nyf) continue;
+      const double wy = interp1_weight(fy, nyf, nyc, J);
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int fx = px + dx;
+        if (fx < 0 || fx > nxf) continue;
+        const double w = wy * interp1_weight(fx, nxf, nxc, I);
+        if (w == 0.0) continue;
+        const size_t f = (size_t)fy * (nxf + 1) + fx;
+        T r[BS];
+#pragma unroll
+        for (int q = 0; q < BS; ++q) r[q] = b[BS * f + q];
+#pragma unroll
+        for (int kk = 0; kk < 9; ++kk) {
+          const int jx = min(max(fx + kk % 3 - 1, 0), nxf), jy = min(max(fy + kk / 3 - 1, 0), nyf);
+          const size_t j = (size_t)jy * (nxf + 1) + jx;
+#pragma unroll
+          for (int a = 0; a < BS; ++a)
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+              r[a] -= st[(size_t)(kk * BS * BS + a * BS + c) * nnf + f] * x[BS * j + c];
+        }
+#pragma unroll
+        for (int q = 0; q < BS; ++q) acc[q] += w * (double)r[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) bc[BS * i + q] = (T)acc[q];
+}
+
+// x + P e (bilinear) at fine node (fx, fy); fixed nodes are not corrected
+template <typename T, int BS>
+__device__ __forceinline__ void corrected(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
+                                          const T* __restrict__ x, const T* __restrict__ ec,
+                                          int fx, int fy, T (&out)[BS]) {
+  const size_t f = (size_t)fy * (nxf + 1) + fx;
+#pragma unroll
+  for (int q = 0; q < BS; ++q) out[q] = x[BS * f + q];
+  if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
+  const Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
+  for (int bb = 0; bb < iy.n; ++bb)
+    for (int aa = 0; aa < ix.n; ++aa) {
+      const T w = (T)(ix.w[aa] * iy.w[bb]);
+      const size_t cidx = (size_t)iy.c[bb] * (nxc + 1) + ix.c[aa];
+#pragma unroll
+      for (int q = 0; q < BS; ++q) out[q] += w * ec[BS * cidx + q];
+    }
+}
+
+// fused: one Jacobi sweep of the fine 9-point level applied to x + P e
+template <typename T, int BS>
+__device__ __forceinline__ void s9_prolong_smooth_node(int nxf, int nyf, int nxc, int nyc,
+                                                       unsigned fixed, const T* __restrict__ st,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ ec,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ y, T omega, size_t v) {
+  const size_t nn = (size_t)(nxf + 1) * (nyf + 1);
+  const int ix = (int)(v % (nxf + 1)), iy = (int)(v / (nxf + 1));
+  T acc[BS], xv[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = T(0);
+#pragma unroll
+  for (int kk = 0; kk < 9; ++kk) {
+    const int jx = min(max(ix + kk % 3 - 1, 0), nxf), jy = min(max(iy + kk / 3 - 1, 0), nyf);
+    T xj[BS];
+    corrected<T, BS>(nxf, nyf, nxc, nyc, fixed, x, ec, jx, jy, xj);
+    if (kk == 4) {
+#pragma unroll
+      for (int q = 0; q < BS; ++q) xv[q] = xj[q];
+    }
+#pragma unroll
+    for (int a = 0; a < BS; ++a)
+#pragma unroll
+      for (int c = 0; c < BS; ++c) acc[a] += st[(size_t)(kk * BS * BS + a * BS + c) * nn + v] * xj[c];
+  }
+  T res[BS], d[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) res[q] = b[BS * v + q] - acc[q];
+  if constexpr (BS == 1) {
+    d[0] = res[0] / st[(size_t)4 * nn + v];
+  } else {
+    const T a00 = st[(size_t)(16 + 0) * nn + v], a01 = st[(size_t)(16 + 1) * nn + v];
+    const T a10 = st[(size_t)(16 + 2) * nn + v], a11 = st[(size_t)(16 + 3) * nn + v];
+    const T inv = T(1) / (a00 * a11 - a01 * a10);
+    d[0] = (a11 * res[0] - a01 * res[1]) * inv;
+    d[1] = (a00 * res[1] - a10 * res[0]) * inv;
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) y[BS * v + q] = xv[q] + omega * d[q];
+}
+
+template <typename T, int BS>
+__global__ void k_stencil_resrestrict(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
+                                      const T* __restrict__ st, const T* __restrict__ x,
+                                      const T* __restrict__ b, T* __restrict__ bc) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)(nxc + 1) * (nyc + 1))
+    s9_resrestrict_node<T, BS>(nxf, nyf, nxc, nyc, fixed, st, x, b, bc, i);
+}
+
+template <typename T, int BS>
+__global__ void k_stencil_prolong_smooth(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
+                                         const T* __restrict__ st, const T* __restrict__ x,
+                                         const T* __restrict__ ec, const T* __restrict__ b,
+                                         T* __restrict__ y, T omega) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < (size_t)(nxf + 1) * (nyf + 1))
+    s9_prolong_smooth_node<T, BS>(nxf, nyf, nxc, nyc, fixed, st, x, ec, b, y, omega, v);
+}
+
+template <typename T, int BS>
+hipError_t stencil_resrestrict(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* st,
+                               const T* x, const T* b, T* bc, hipStream_t s) {
+  k_stencil_resrestrict<T, BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1), 64), 64, 0, s>>>(
+      nxf, nyf, nxc, nyc, fixed, st, x, b, bc);
+  return hipGetLastError();
+}
+template <typename T, int BS>
+hipError_t stencil_prolong_smooth(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* st,
+                                  const T* x, const T* ec, const T* b, T* y, T omega,
+                                  hipStream_t s) {
+  k_stencil_prolong_smooth<T, BS><<<grid1d((size_t)(nxf + 1) * (nyf + 1), 64), 64, 0, s>>>(
+      nxf, nyf, nxc, nyc, fixed, st, x, ec, b, y, omega);
+  return hipGetLastError();
+}
+#define TMX_INST_S9F(T, BS)                                                                       \
+  template hipError_t stencil_resrestrict<T, BS>(int, int, int, int, unsigned, const T*, const T*, \
+                                                 const T*, T*, hipStream_t);                       \
+  template hipError_t stencil_prolong_smooth<T, BS>(int, int, int, int, unsigned, const T*,        \
+                                                    const T*, const T*, const T*, T*, T,           \
+                                                    hipStream_t);
+TMX_INST_S9F(double, 1)
+TMX_INST_S9F(double, 2)
+TMX_INST_S9F(float, 1)
+TMX_INST_S9F(float, 2)
+
+// ---------------------------------------------------------------- tail V-cycle
+// The bottom of the hierarchy (9-point levels small enough for one workgroup, then the
+// dense coarsest solve) runs as a single workgroup that steps through the whole V-cycle,
+// with a barrier between steps: one launch instead of ~7 per level.
+template <typename T, int BS, int OP>
+__device__ __forceinline__ void tail_sweep(const TailLevel<T>& L, const T* x, const T* b, T* y,
+                                           T omega) {
+  const size_t nn = (size_t)(L.nx + 1) * (L.ny + 1);
+  for (size_t v = threadIdx.x; v < nn; v += blockDim.x)
+    s9_node<T, BS, OP>(L.nx, L.ny, L.st, x, b, y, omega, v);
+  __syncthreads();
+}
+
+template <typename T, int BS>
+__global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc<T> d) {
+  const int nl = d.nlev;  // levels 0..nl-2 are 9-point, level nl-1 is the dense coarsest
+  const int S = d.nu_pre + d.nu_post;
+  int wbuf[kMaxTail];
+  for (int l = 0; l + 1 < nl; ++l) {
+    const TailLevel<T>& L = d.lev[l];
+    T* buf[2] = {L.x, L.t};
+    int w = (S - 1) % 2;
+    tail_sweep<T, BS, kSmoothZero>(L, nullptr, L.b, buf[w], d.omega);
+    for (int k = 1; k < d.nu_pre; ++k) {
+      tail_sweep<T, BS, kSmooth>(L, buf[w], L.b, buf[w ^ 1], d.omega);
+      w ^= 1;
+    }
+    wbuf[l] = w;
+    tail_sweep<T, BS, kResidual>(L, buf[w], L.b, L.r, d.omega);
+    const TailLevel<T>& C = d.lev[l + 1];
+    const size_t nnc = (size_t)(C.nx + 1) * (C.ny + 1);
+    for (size_t i = threadIdx.x; i < nnc; i += blockDim.x)
+      restrict_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, L.r, C.b, i);
+    __syncthreads();
+  }
+  {  // dense coarsest: x = Ainv b, 8 lanes per row
+    const TailLevel<T>& C = d.lev[nl - 1];
+    const int n = d.cn;
+    for (int base = 0; base < n * 8; base += blockDim.x) {
+      const int t = base + threadIdx.x;
+      const int row = t >> 3, part = t & 7;
+      double s = 0.0;
+      if (row < n)
+        for (int j = part; j < n; j += 8) s += d.dense[(size_t)row * n + j] * (double)C.b[j];
+      s += __shfl_down(s, 4, 8);
+      s += __shfl_down(s, 2, 8);
+      s += __shfl_down(s, 1, 8);
+      if (row < n && part == 0) C.x[row] = (T)s;
+    }
+    __syncthreads();
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    const TailLevel<T>& L = d.lev[l];
+    const TailLevel<T>& C = d.lev[l + 1];
+    T* buf[2] = {L.x, L.t};
+    int w = wbuf[l];
+    const size_t nn = (size_t)(L.nx + 1) * (L.ny + 1);
+    for (size_t i = threadIdx.x; i < nn; i += blockDim.x)
+      prolong_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, C.x, buf[w], i);
+    __syncthreads();
+    for (int k = 0; k < d.nu_post; ++k) {
+      tail_sweep<T, BS, kSmooth>(L, buf[w], L.b, buf[w ^ 1], d.omega);
+      w ^= 1;
+    }
+  }
+}
+
+template <typename T, int BS>
+hipError_t tail_vcycle(const TailDesc<T>& d, hipStream_t s) {
+  if (d.nlev < 1 || d.nlev > kMaxTail) return hipErrorInvalidValue;
+  k_tail_vcycle<T, BS><<<1, 1024, 0, s>>>(d);
+  return hipGetLastError();
+}
+template hipError_t tail_vcycle<double, 1>(const TailDesc<double>&, hipStream_t);
+template hipError_t tail_vcycle<double, 2>(const TailDesc<double>&, hipStream_t);
+template hipError_t tail_vcycle<float, 1>(const TailDesc<float>&, hipStream_t);
+template hipError_t tail_vcycle<float, 2>(const TailDesc<float>&, hipStream_t);
 
 // Galerkin product, one thread per coarse node.  Interpolation from coarse to fine node g:
 // nested P1 (x-, y- and diagonal midpoints) or tensor-product 1-D linear (bilinear).
@@ -1190,16 +1628,21 @@ hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
 template <typename T>
 __global__ void k_dense_apply(int n, const double* __restrict__ ainv, const T* __restrict__ b,
                               T* __restrict__ x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  // 8 lanes per row (rows of the symmetric inverse read contiguously)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = t >> 3, part = t & 7;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s += ainv[(size_t)j * n + i] * (double)b[j];  // symmetric inverse
-  x[i] = (T)s;
+  if (row < n)
+    for (int j = part; j < n; j += 8) s += ainv[(size_t)row * n + j] * (double)b[j];
+  s += __shfl_down(s, 4, 8);
+  s += __shfl_down(s, 2, 8);
+  s += __shfl_down(s, 1, 8);
+  if (row < n && part == 0) x[row] = (T)s;
 }
 
 template <typename T>
 hipError_t dense_apply(int n, const double* ainv, const T* b, T* x, hipStream_t s) {
-  k_dense_apply<T><<<(n + 63) / 64, 64, 0, s>>>(n, ainv, b, x);
+  k_dense_apply<T><<<(8 * n + 255) / 256, 256, 0, s>>>(n, ainv, b, x);
   return hipGetLastError();
 }
 template hipError_t dense_apply<double>(int, const double*, const double*, double*, hipStream_t);
@@ -1227,34 +1670,98 @@ static inline unsigned vec_grid(size_t n) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(size_t n, const T* __restrict__ a,
-                                                const T* __restrict__ b, RedSlot r) {
+                                                const T* __restrict__ b, double* __restrict__ part) {
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     s += (double)a[i] * (double)b[i];
   const double v[1] = {s};
-  const Reduction red[1] = {{r.partials, r.counter, r.result}};
-  block_reduce_finalize<1>(v, red);
+  double* const o[1] = {part};
+  block_partials<1>(v, o);
 }
 
-hipError_t dot(size_t n, const double* a, const double* b, const RedSlot& r, hipStream_t s) {
-  k_dot<double><<<vec_grid(n), kBlock, 0, s>>>(n, a, b, r);
-  return hipGetLastError();
-}
 template <typename T>
-hipError_t dot_t(size_t n, const T* a, const T* b, const RedSlot& r, hipStream_t s) {
-  k_dot<T><<<vec_grid(n), kBlock, 0, s>>>(n, a, b, r);
+hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  r.nblocks = (int)g;
+  k_dot<T><<<g, kBlock, 0, s>>>(n, a, b, r.partials);
   return hipGetLastError();
 }
-template hipError_t dot_t<double>(size_t, const double*, const double*, const RedSlot&, hipStream_t);
-template hipError_t dot_t<float>(size_t, const float*, const float*, const RedSlot&, hipStream_t);
+template hipError_t dot_t<double>(size_t, const double*, const double*, RedSlot&, hipStream_t);
+template hipError_t dot_t<float>(size_t, const float*, const float*, RedSlot&, hipStream_t);
+
+__global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__ part, int n,
+                                                      double* __restrict__ out) {
+  const double t = sum_partials(part, n);
+  if (threadIdx.x == 0) *out = t;
+}
+
+hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s) {
+  k_reduce_to<<<1, kBlock, 0, s>>>(r.partials, r.nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t dot(size_t n, const double* a, const double* b, RedSlot& r, double* out,
+               hipStream_t s) {
+  hipError_t e = dot_t<double>(n, a, b, r, s);
+  if (e != hipSuccess) return e;
+  return reduce_to(r, out, s);
+}
+
+// PCG scalar steps (one block).  sc layout: see PcgSlots.
+__global__ __launch_bounds__(kBlock) void k_pcg_alpha(const double* __restrict__ part, int n,
+                                                      double* __restrict__ sc, PcgSlots q) {
+  const double pq = sum_partials(part, n);
+  if (threadIdx.x == 0) {
+    sc[q.pq] = pq;
+    const bool stop = sc[q.flag] != 0.0;
+    if (!stop && !(pq > 0.0)) sc[q.flag] = (pq != pq) ? 3.0 : 2.0;  // NaN : not SPD
+    sc[q.alpha] = (stop || !(pq > 0.0)) ? 0.0 : sc[q.rz] / pq;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_rr(const double* __restrict__ part, int n,
+                                                   double* __restrict__ sc, PcgSlots q) {
+  const double rr = sum_partials(part, n);
+  if (threadIdx.x == 0) {
+    if (sc[q.flag] == 0.0) {
+      sc[q.rr] = rr;
+      sc[q.iters] += 1.0;
+      if (!(rr == rr) || rr > 1e300) sc[q.flag] = 3.0;
+      else if (rr <= sc[q.tol2]) sc[q.flag] = 1.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_beta(const double* __restrict__ part, int n,
+                                                     double* __restrict__ sc, PcgSlots q) {
+  const double rzn = sum_partials(part, n);
+  if (threadIdx.x == 0) {
+    const bool stop = sc[q.flag] != 0.0;
+    sc[q.beta] = stop ? 0.0 : rzn / sc[q.rz];
+    if (!stop) sc[q.rz] = rzn;
+  }
+}
+
+hipError_t pcg_alpha(const RedSlot& r, double* sc, const PcgSlots& q, hipStream_t s) {
+  k_pcg_alpha<<<1, kBlock, 0, s>>>(r.partials, r.nblocks, sc, q);
+  return hipGetLastError();
+}
+hipError_t pcg_rr(const RedSlot& r, double* sc, const PcgSlots& q, hipStream_t s) {
+  k_pcg_rr<<<1, kBlock, 0, s>>>(r.partials, r.nblocks, sc, q);
+  return hipGetLastError();
+}
+hipError_t pcg_beta(const RedSlot& r, double* sc, const PcgSlots& q, hipStream_t s) {
+  k_pcg_beta<<<1, kBlock, 0, s>>>(r.partials, r.nblocks, sc, q);
+  return hipGetLastError();
+}
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_xr(size_t n, const double* __restrict__ sc,
-                                                          int i_rz, int i_pq, T* __restrict__ x,
+                                                          int i_alpha, T* __restrict__ x,
                                                           T* __restrict__ r, const T* __restrict__ p,
-                                                          const T* __restrict__ q, RedSlot rr) {
-  const double alpha = sc[i_rz] / sc[i_pq];
-  const T al = (T)alpha;
+                                                          const T* __restrict__ q,
+                                                          double* __restrict__ part) {
+  const T al = (T)sc[i_alpha];
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     x[i] += al * p[i];
@@ -1263,37 +1770,124 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr(size_t n, const double
     s += (double)ri * (double)ri;
   }
   const double v[1] = {s};
-  const Reduction red[1] = {{rr.partials, rr.counter, rr.result}};
-  block_reduce_finalize<1>(v, red);
+  double* const o[1] = {part};
+  block_partials<1>(v, o);
 }
 
 template <typename T>
-hipError_t pcg_update_xr(size_t n, const double* sc, int i_rz, int i_pq, T* x, T* r, const T* p,
-                         const T* q, const RedSlot& rr, hipStream_t s) {
-  k_pcg_update_xr<T><<<vec_grid(n), kBlock, 0, s>>>(n, sc, i_rz, i_pq, x, r, p, q, rr);
+hipError_t pcg_update_xr(size_t n, const double* sc, int i_alpha, T* x, T* r, const T* p,
+                         const T* q, RedSlot& rr, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  rr.nblocks = (int)g;
+  k_pcg_update_xr<T><<<g, kBlock, 0, s>>>(n, sc, i_alpha, x, r, p, q, rr.partials);
   return hipGetLastError();
 }
-template hipError_t pcg_update_xr<double>(size_t, const double*, int, int, double*, double*,
-                                          const double*, const double*, const RedSlot&, hipStream_t);
-template hipError_t pcg_update_xr<float>(size_t, const double*, int, int, float*, float*,
-                                         const float*, const float*, const RedSlot&, hipStream_t);
+template hipError_t pcg_update_xr<double>(size_t, const double*, int, double*, double*,
+                                          const double*, const double*, RedSlot&, hipStream_t);
+template hipError_t pcg_update_xr<float>(size_t, const double*, int, float*, float*,
+                                         const float*, const float*, RedSlot&, hipStream_t);
 
 template <typename T>
-__global__ void k_pcg_update_p(size_t n, const double* __restrict__ sc, int i_rznew, int i_rz,
+__global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
+    size_t n, const double* __restrict__ sc, PcgSlots q, const double* __restrict__ pqpart, int npq,
+    T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ qv,
+    double* __restrict__ part) {
+  __shared__ double alpha_sh;
+  const double pq = sum_partials(pqpart, npq);
+  if (threadIdx.x == 0) {
+    const bool stop = sc[q.flag] != 0.0;
+    alpha_sh = (stop || !(pq > 0.0)) ? 0.0 : sc[q.rz] / pq;
+  }
+  __syncthreads();
+  const T al = (T)alpha_sh;
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    x[i] += al * p[i];
+    const T ri = r[i] - al * qv[i];
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  const double v[1] = {s};
+  double* const o[1] = {part};
+  block_partials<1>(v, o);
+}
+
+template <typename T>
+hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, const RedSlot& pq,
+                               T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  rr.nblocks = (int)g;
+  k_pcg_update_xr_fused<T><<<g, kBlock, 0, s>>>(n, sc, q, pq.partials, pq.nblocks, x, r, p, qv,
+                                               rr.partials);
+  return hipGetLastError();
+}
+template hipError_t pcg_update_xr_fused<double>(size_t, const double*, const PcgSlots&,
+                                                const RedSlot&, double*, double*, const double*,
+                                                const double*, RedSlot&, hipStream_t);
+template hipError_t pcg_update_xr_fused<float>(size_t, const double*, const PcgSlots&,
+                                               const RedSlot&, float*, float*, const float*,
+                                               const float*, RedSlot&, hipStream_t);
+
+__global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
+                                                         const double* __restrict__ rrp, int nrr,
+                                                         const double* __restrict__ rzp, int nrz,
+                                                         double* __restrict__ sc, PcgSlots q) {
+  const double pq = sum_partials(pqp, npq);
+  const double rr = sum_partials(rrp, nrr);
+  const double rz = sum_partials(rzp, nrz);
+  if (threadIdx.x == 0) {
+    if (sc[q.flag] == 0.0) {
+      sc[q.pq] = pq;
+      sc[q.rr] = rr;
+      sc[q.iters] += 1.0;
+      if (!(pq == pq) || !(rr == rr) || !(rz == rz) || rr > 1e300) sc[q.flag] = 3.0;
+      else if (!(pq > 0.0)) sc[q.flag] = 2.0;
+      else if (rr <= sc[q.tol2]) sc[q.flag] = 1.0;
+    }
+    const bool stop = sc[q.flag] != 0.0;
+    sc[q.beta] = stop ? 0.0 : rz / sc[q.rz];
+    if (!stop) sc[q.rz] = rz;
+  }
+}
+
+hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
+                        const PcgSlots& q, hipStream_t s) {
+  k_pcg_finalize<<<1, kBlock, 0, s>>>(pq.partials, pq.nblocks, rr.partials, rr.nblocks,
+                                      rz.partials, rz.nblocks, sc, q);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__ rzp, int nrz,
+                                                      double* __restrict__ sc, PcgSlots q) {
+  const double rz = sum_partials(rzp, nrz);
+  if (threadIdx.x == 0) {
+    sc[q.rz] = rz;
+    sc[q.beta] = 0.0;
+    sc[q.flag] = (rz == rz) ? 0.0 : 3.0;
+    sc[q.iters] = 0.0;
+  }
+}
+
+hipError_t pcg_start(const RedSlot& rz, double* sc, const PcgSlots& q, hipStream_t s) {
+  k_pcg_start<<<1, kBlock, 0, s>>>(rz.partials, rz.nblocks, sc, q);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_pcg_update_p(size_t n, const double* __restrict__ sc, int i_beta,
                                const T* __restrict__ z, T* __restrict__ p) {
-  const T beta = (T)(sc[i_rznew] / sc[i_rz]);
+  const T beta = (T)sc[i_beta];
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
 }
 
 template <typename T>
-hipError_t pcg_update_p(size_t n, double* sc, int i_rznew, int i_rz, const T* z, T* p,
-                        hipStream_t s) {
-  k_pcg_update_p<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, sc, i_rznew, i_rz, z, p);
+hipError_t pcg_update_p(size_t n, const double* sc, int i_beta, const T* z, T* p, hipStream_t s) {
+  k_pcg_update_p<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, sc, i_beta, z, p);
   return hipGetLastError();
 }
-template hipError_t pcg_update_p<double>(size_t, double*, int, int, const double*, double*, hipStream_t);
-template hipError_t pcg_update_p<float>(size_t, double*, int, int, const float*, float*, hipStream_t);
+template hipError_t pcg_update_p<double>(size_t, const double*, int, const double*, double*, hipStream_t);
+template hipError_t pcg_update_p<float>(size_t, const double*, int, const float*, float*, hipStream_t);
 
 template <typename T>
 __global__ void k_copy(size_t n, const T* __restrict__ a, T* __restrict__ b) {
@@ -1360,6 +1954,14 @@ __global__ void k_mask_fixed(int nx, int ny, unsigned fixed, double* __restrict_
 }
 hipError_t mask_fixed(int nx, int ny, unsigned fixed, double* v, hipStream_t s) {
   k_mask_fixed<<<grid1d((size_t)(nx + 1) * (ny + 1)), kBlock, 0, s>>>(nx, ny, fixed, v);
+  return hipGetLastError();
+}
+
+__global__ void k_set_scalar(double* __restrict__ sc, int i, double v) {
+  if (threadIdx.x == 0) sc[i] = v;
+}
+hipError_t set_scalar(double* sc, int i, double v, hipStream_t s) {
+  k_set_scalar<<<1, 64, 0, s>>>(sc, i, v);
   return hipGetLastError();
 }
 

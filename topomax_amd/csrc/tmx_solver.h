@@ -72,25 +72,22 @@ class DevBuf {
 
 // Reduction scratch and device scalars of one stream of work.
 struct Workspace {
-  DevBuf<double> partials;
-  DevBuf<unsigned> counter;
-  DevBuf<double> scalars;
-  double* host = nullptr;  // pinned mirror of scalars
+  static constexpr int kRegions = 4;
   static constexpr int kScalars = 64;
+  DevBuf<double> partials;  // kRegions x kMaxPartials
+  DevBuf<double> scalars;
+  double* host = nullptr;   // pinned: [0, kScalars) mirror + ring for lagged reads
   int init(hipStream_t s);
   ~Workspace();
-  RedSlot slot(int result) const {
-    return RedSlot{partials.get(), counter.get(), scalars.get() + result};
-  }
+  RedSlot region(int k) const { return RedSlot{partials.get() + (size_t)k * kMaxPartials, 0}; }
   double* sc() const { return scalars.get(); }
-  int read(hipStream_t s, int first, int count);  // scalars[first..) -> host[first..)
+  int read(hipStream_t s, int first, int count);  // scalars[first..) -> host[first..), sync
 };
 
-// scalar slots
-enum : int {
-  kScBB = 0, kScRZ0 = 1, kScRZ1 = 2, kScPQ = 3, kScRR = 4, kScRR64 = 5, kScJ = 6,
-  kScInt = 7, kScTmp = 8
-};
+// scalar slots used outside PCG
+enum : int { kScBB = 0, kScRR64 = 1, kScJ = 2, kScInt = 3, kScTmp = 4 };
+// PCG slots (flag, rr, iters contiguous for one async read)
+constexpr PcgSlots kPcg = {10, 11, 16, 12, 13, 15, 14, 17};
 
 enum LevelKind : int { kElem = 0, kStencil = 1, kDense = 2 };
 enum Phys : int { kElasticity = 0, kFilter = 1 };
@@ -131,23 +128,33 @@ class Hierarchy {
  public:
   int init(Phys phys, int nx, int ny, double hx, double hy, unsigned fixed, double lam,
            double mu, double eps, int coarse_max_nodes, hipStream_t s);
-  // elasticity: rebuild coarse operators from nodal xi (device, fp64), penalty p
+  // elasticity: copy nodal xi (device, fp64) and rebuild the coarse operators, penalty p
   int update_elasticity(const double* xi, int p, double m, hipStream_t s);
+  // elasticity fine operator only (copies xi; no coarse rebuild)
+  int set_fine_operator(const double* xi, int p, double m, hipStream_t s);
   // filter: build the constant hierarchy once
   int build_filter(hipStream_t s);
 
   template <typename T>
-  int apply(const T* x, T* y, const RedSlot* dot, hipStream_t s);
+  int apply(const T* x, T* y, RedSlot* dot, hipStream_t s);
   template <typename T>
   int residual(const T* x, const T* b, T* r, hipStream_t s);
   template <typename T>
   int smooth0(const T* x, const T* b, T* y, hipStream_t s);
+  // z = M^-1 b; at level 0 the last smoothing sweep also writes partials of b.z to dot_bz
   template <typename T>
-  int vcycle(int l, const T* b, T* out, hipStream_t s);
+  int vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz = nullptr);
+  // q = A p with p = z + beta p_old written to p_new; partials of p.q to dot
+  template <typename T>
+  int apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta, RedSlot* dot,
+                   hipStream_t s);
 
+  // PCG from x0 = 0 on A x = b; result left in the hierarchy's Krylov x buffer (x_buf<T>())
   template <typename T>
-  int pcg(Workspace& ws, const T* b, T* x, double rtol, int max_iter, int* iters,
-          double* rel, double bnorm2, hipStream_t s);
+  int pcg(Workspace& ws, const T* b, double rtol, int max_iter, double bnorm2, int* iters,
+          double* rel, hipStream_t s);
+  template <typename T>
+  T* x_buf();
   int solve(Workspace& ws, const double* b, double* x, const SolveParams& p, SolveStats* st,
             hipStream_t s);
 
@@ -164,7 +171,12 @@ class Hierarchy {
   int p_ = 0;
   double m_ = 0, cp_ = 0;
   double omega_ = 0.7;
-  int nu_pre_ = 2, nu_post_ = 2;
+  int nu_pre_ = 2, nu_post_ = 2;  // Jacobi sweeps on the finest level
+  int nu_coarse_ = 2;              // pre- and post-sweeps on every coarser level
+  bool fused_ = true, use_tail_ = true;
+  int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
+  void find_tail();
+  int tail_ = -1;               // first level handled by the one-workgroup tail kernel
   std::vector<Level> levels_;   // levels_[0] = finest
   // coarsest dense level
   int cnx_ = 0, cny_ = 0, cn_ = 0;
@@ -172,12 +184,19 @@ class Hierarchy {
   DevBuf<double> cst64_;        // coarsest stencil (fp64)
   DevBuf<double> tmpst_;        // explicit stencil of the last element level
   DevBuf<double> cw0_;          // level-0 cell weights (restriction source)
-  const double* xi64_ = nullptr;
+  const double* xi64_ = nullptr;  // = xi64buf_ once an operator is set
+  DevBuf<double> xi64buf_;
   DevBuf<float> xi32_;
+  bool has_operator_ = false;
   // Krylov vectors
-  DevBuf<double> r64_, z64_, p64_, q64_;
-  DevBuf<float> r32_, z32_, p32_, q32_, d32_;
+  DevBuf<double> x64_, r64_, z64_, p64_, q64_, pb64_;
+  DevBuf<float> r32_, z32_, p32_, q32_, d32_, pb32_;
   DevBuf<double> rr64_;         // refinement residual
+  // captured PCG graphs: [precision][0 = init, 1 = even iteration, 2 = odd iteration]
+  hipGraphExec_t graphs_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  void drop_graphs();
+  ~Hierarchy();
 
  private:
   int alloc_vectors();

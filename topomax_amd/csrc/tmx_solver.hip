@@ -22,12 +22,10 @@ const char* last_error_cstr() { return g_last_error.c_str(); }
 
 // ------------------------------------------------------------------ workspace
 int Workspace::init(hipStream_t s) {
-  TMX_TRY(partials.alloc(kMaxPartials));
-  TMX_TRY(counter.alloc(64));
+  TMX_TRY(partials.alloc((size_t)kRegions * kMaxPartials));
   TMX_TRY(scalars.alloc(kScalars));
-  TMX_HIP(hipHostMalloc(&host, sizeof(double) * kScalars, hipHostMallocDefault));
-  std::memset(host, 0, sizeof(double) * kScalars);
-  TMX_HIP(fill_zero_bytes(counter.get(), sizeof(unsigned) * 64, s));
+  TMX_HIP(hipHostMalloc(&host, sizeof(double) * 2 * kScalars, hipHostMallocDefault));
+  std::memset(host, 0, sizeof(double) * 2 * kScalars);
   TMX_HIP(fill_zero_bytes(scalars.get(), sizeof(double) * kScalars, s));
   return 0;
 }
@@ -44,7 +42,12 @@ int Workspace::read(hipStream_t s, int first, int count) {
 }
 
 // ------------------------------------------------------------------ hierarchy setup
-static int rows_default(int ny) { return ny >= 256 ? 16 : 8; }
+// node rows per wavefront chunk: enough wavefronts (~4096) to fill 256 CUs, 2..16 rows
+static int rows_default(int nx, int ny) {
+  const long strips = (nx + 1 + 61) / 62;  // 62 output columns per wavefront
+  const long r = (strips * (ny + 1)) / 4096;
+  return (int)std::max(2L, std::min(16L, r));
+}
 
 int Hierarchy::init(Phys phys, int nx, int ny, double hx, double hy, unsigned fixed, double lam,
                     double mu, double eps, int coarse_max_nodes, hipStream_t s) {
@@ -78,9 +81,10 @@ int Hierarchy::init(Phys phys, int nx, int ny, double hx, double hy, unsigned fi
     levels_.push_back(std::move(n));
   }
   for (auto& L : levels_) {
-    L.sh64 = SweepShape::make(L.nx, L.ny, rows_default(L.ny));
-    L.sh32 = SweepShape::make(L.nx, L.ny, rows_default(L.ny));
+    L.sh64 = SweepShape::make(L.nx, L.ny, rows_default(L.nx, L.ny));
+    L.sh32 = SweepShape::make(L.nx, L.ny, rows_default(L.nx, L.ny));
   }
+  find_tail();
   Level& last = levels_.back();
   cnx_ = last.nx;
   cny_ = last.ny;
@@ -91,7 +95,36 @@ int Hierarchy::init(Phys phys, int nx, int ny, double hx, double hy, unsigned fi
   return 0;
 }
 
+// tail: the longest suffix of small 9-point levels ending at the dense coarsest
+void Hierarchy::find_tail() {
+  const int lastl = (int)levels_.size() - 1;
+  tail_ = -1;
+  if (lastl >= 1) {
+    int t = lastl;
+    while (t - 1 >= 1 && lastl - (t - 1) + 1 <= kMaxTail && levels_[t - 1].kind == kStencil &&
+           levels_[t - 1].nn <= (size_t)tail_nodes_ && !levels_[t].nested_from_prev)
+      --t;
+    tail_ = t;
+  }
+}
+
+void Hierarchy::drop_graphs() {
+  for (auto& g : graphs_)
+    for (auto& e : g)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+}
+
+Hierarchy::~Hierarchy() {
+  drop_graphs();
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
 void Hierarchy::set_rows(int rows64, int rows32) {
+  drop_graphs();
   if (rows64 > 0) levels_[0].sh64 = SweepShape::make(levels_[0].nx, levels_[0].ny, rows64);
   if (rows32 > 0) levels_[0].sh32 = SweepShape::make(levels_[0].nx, levels_[0].ny, rows32);
 }
@@ -127,12 +160,19 @@ int Hierarchy::alloc_vectors() {
   if (phys_ == kElasticity) {
     TMX_TRY(cw0_.alloc(2 * (size_t)levels_[0].nx * levels_[0].ny));
     TMX_TRY(xi32_.alloc(levels_[0].nn));
+    TMX_TRY(xi64buf_.alloc(levels_[0].nn));
+    xi64_ = xi64buf_.get();
   }
+  TMX_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
+  TMX_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
   const size_t n0 = n();
+  TMX_TRY(x64_.alloc(n0));
   TMX_TRY(r64_.alloc(n0));
   TMX_TRY(z64_.alloc(n0));
   TMX_TRY(p64_.alloc(n0));
   TMX_TRY(q64_.alloc(n0));
+  TMX_TRY(pb64_.alloc(n0));
+  TMX_TRY(pb32_.alloc(n0));
   TMX_TRY(r32_.alloc(n0));
   TMX_TRY(z32_.alloc(n0));
   TMX_TRY(p32_.alloc(n0));
@@ -203,16 +243,24 @@ int Hierarchy::build_coarse_stencils(hipStream_t s) {
   return 0;
 }
 
-int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
-  xi64_ = xi;
+int Hierarchy::set_fine_operator(const double* xi, int p, double m, hipStream_t s) {
+  if (p != p_ || m != m_) drop_graphs();  // p, m are launch arguments of captured kernels
   p_ = p;
   m_ = m;
   cp_ = (1.0 - m) * 2.0 / ((p + 1.0) * (p + 2.0));
   const Level& L0 = levels_[0];
-  TMX_HIP((convert<double, float>(L0.nn, xi, xi32_.get(), s)));
+  if (xi != xi64buf_.get()) TMX_HIP(copy_vec<double>(L0.nn, xi, xi64buf_.get(), s));
+  TMX_HIP((convert<double, float>(L0.nn, xi64buf_.get(), xi32_.get(), s)));
+  has_operator_ = true;
+  return 0;
+}
+
+int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
+  TMX_TRY(set_fine_operator(xi, p, m, s));
+  const Level& L0 = levels_[0];
   const int last = (int)levels_.size() - 1;
   if (last >= 1 && levels_[1].kind == kElem) {
-    TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi, p, m, cp_, cw0_.get(), s));
+    TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p, m, cp_, cw0_.get(), s));
   }
   for (int l = 1; l <= last; ++l) {
     Level& L = levels_[l];
@@ -258,19 +306,23 @@ template <>
 const float* xi_for<float>(const Hierarchy& h) { return h.xi32_.get(); }
 
 template <typename T>
-static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, const RedSlot* dot,
-                    hipStream_t s) {
+static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, RedSlot* dot,
+                    hipStream_t s, const SweepExtra<T>& ex = SweepExtra<T>()) {
   Level& L = h.levels_[l];
   const bool fp64 = sizeof(T) == 8;
   if (L.kind == kElem) {
     if (h.phys_ == kElasticity) {
       ElastOpArgs<T> a = elast_args<T>(h, l, xi_for<T>(h), PrecBufs<T>::cw(L), fp64);
-      TMX_HIP(elast_sweep<T>(a, op, x, b, y, (T)h.omega_, dot, s));
+      TMX_HIP(elast_sweep<T>(a, op, x, b, y, (T)h.omega_, dot, s, ex));
     } else {
       TMX_HIP(filter_sweep<T>(PrecBufs<T>::sh(L), FilterConst<T>::make(h.eps_, L.hx, L.hy), op, x,
-                              b, y, (T)h.omega_, dot, s));
+                              b, y, (T)h.omega_, dot, s, ex));
     }
   } else {
+    if (dot || op > kSmoothZero) {
+      set_error("internal: 9-point levels have no fused dot / transfer ops");
+      return -1;
+    }
     if (h.bs_ == 2)
       TMX_HIP((stencil_op<T, 2>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, (T)h.omega_, s)));
     else
@@ -280,13 +332,22 @@ static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, c
 }
 
 template <typename T>
-int Hierarchy::apply(const T* x, T* y, const RedSlot* dot, hipStream_t s) {
+int Hierarchy::apply(const T* x, T* y, RedSlot* dot, hipStream_t s) {
   return level_op<T>(*this, 0, kApply, x, nullptr, y, dot, s);
 }
 
 template <typename T>
 int Hierarchy::residual(const T* x, const T* b, T* r, hipStream_t s) {
   return level_op<T>(*this, 0, kResidual, x, b, r, nullptr, s);
+}
+
+template <typename T>
+int Hierarchy::apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta,
+                            RedSlot* dot, hipStream_t s) {
+  SweepExtra<T> ex;
+  ex.pnew = p_new;
+  ex.beta = beta;
+  return level_op<T>(*this, 0, kApplyUpdate, p_old, z, q, dot, s, ex);
 }
 
 template <typename T>
@@ -323,29 +384,101 @@ static int prolong_to(Hierarchy& h, int l, const T* ec, T* xf, hipStream_t s) {
 }
 
 template <typename T>
-int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s) {
+static int tail_launch(Hierarchy& h, hipStream_t s) {
+  const int last = (int)h.levels_.size() - 1;
+  TailDesc<T> d;
+  std::memset(&d, 0, sizeof(d));
+  d.nlev = last - h.tail_ + 1;
+  d.cn = h.cn_;
+  d.nu_pre = h.nu_coarse_;
+  d.nu_post = h.nu_coarse_;
+  d.fixed = h.fixed_;
+  d.omega = (T)h.omega_;
+  d.dense = h.dense_.get();
+  for (int l = h.tail_; l <= last; ++l) {
+    Level& L = h.levels_[l];
+    TailLevel<T>& t = d.lev[l - h.tail_];
+    t.nx = L.nx;
+    t.ny = L.ny;
+    t.st = PrecBufs<T>::st(L);
+    t.x = PrecBufs<T>::x(L);
+    t.t = PrecBufs<T>::t(L);
+    t.b = PrecBufs<T>::b(L);
+    t.r = PrecBufs<T>::r(L);
+  }
+  if (h.bs_ == 2) TMX_HIP((tail_vcycle<T, 2>(d, s)));
+  else TMX_HIP((tail_vcycle<T, 1>(d, s)));
+  return 0;
+}
+
+template <typename T>
+int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz) {
   const int last = (int)levels_.size() - 1;
   if (l == last) {
     TMX_HIP(dense_apply<T>(cn_, dense_.get(), b, out, s));
+    if (dot_bz) TMX_HIP(dot_t<T>((size_t)cn_, b, out, *dot_bz, s));
     return 0;
+  }
+  if (use_tail_ && l == tail_ && l > 0) {  // b and out are this level's b / x buffers
+    return tail_launch<T>(*this, s);
   }
   Level& L = levels_[l];
   Level& C = levels_[l + 1];
   T* buf[2] = {out, PrecBufs<T>::t(L)};
-  const int S = nu_pre_ + nu_post_;
+  const int npre = l == 0 ? nu_pre_ : nu_coarse_;
+  const int npost = l == 0 ? nu_post_ : nu_coarse_;
+  const int S = npre + npost;
   int w = (S - 1) % 2;
   TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s));
-  for (int k = 1; k < nu_pre_; ++k) {
+  for (int k = 1; k < npre; ++k) {
     TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s));
     w ^= 1;
   }
-  T* r = PrecBufs<T>::r(L);
-  TMX_TRY(level_op<T>(*this, l, kResidual, buf[w], b, r, nullptr, s));
-  TMX_TRY(restrict_to<T>(*this, l, r, PrecBufs<T>::b(C), s));
-  TMX_TRY(vcycle<T>(l + 1, PrecBufs<T>::b(C), PrecBufs<T>::x(C), s));
-  TMX_TRY(prolong_to<T>(*this, l, PrecBufs<T>::x(C), buf[w], s));
-  for (int k = 0; k < nu_post_; ++k) {
-    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s));
+  T* cb = PrecBufs<T>::b(C);
+  T* cx = PrecBufs<T>::x(C);
+  const bool fused_el = fused_ && L.kind == kElem && C.nested_from_prev;  // nested transfer
+  // (9-point levels keep separate residual / restriction / prolongation launches: recomputing
+  // the fine residuals per coarse node costs more than the extra launch there)
+  const bool fused_st = false;
+  if (fused_el) {
+    TMX_TRY(level_op<T>(*this, l, kResidualRestrict, buf[w], b, cb, nullptr, s));
+  } else if (fused_st) {
+    if (bs_ == 2)
+      TMX_HIP((stencil_resrestrict<T, 2>(L.nx, L.ny, C.nx, C.ny, fixed_, PrecBufs<T>::st(L),
+                                         buf[w], b, cb, s)));
+    else
+      TMX_HIP((stencil_resrestrict<T, 1>(L.nx, L.ny, C.nx, C.ny, fixed_, PrecBufs<T>::st(L),
+                                         buf[w], b, cb, s)));
+  } else {
+    T* r = PrecBufs<T>::r(L);
+    TMX_TRY(level_op<T>(*this, l, kResidual, buf[w], b, r, nullptr, s));
+    TMX_TRY(restrict_to<T>(*this, l, r, cb, s));
+  }
+  TMX_TRY(vcycle<T>(l + 1, cb, cx, s));
+  int post = npost;
+  RedSlot* last_dot = l == 0 ? dot_bz : nullptr;
+  if (fused_el) {
+    SweepExtra<T> ex;
+    ex.ec = cx;
+    TMX_TRY(level_op<T>(*this, l, kProlongSmooth, buf[w], b, buf[w ^ 1],
+                        post == 1 ? last_dot : nullptr, s, ex));
+    w ^= 1;
+    --post;
+  } else if (fused_st) {
+    if (bs_ == 2)
+      TMX_HIP((stencil_prolong_smooth<T, 2>(L.nx, L.ny, C.nx, C.ny, fixed_, PrecBufs<T>::st(L),
+                                            buf[w], cx, b, buf[w ^ 1], (T)omega_, s)));
+    else
+      TMX_HIP((stencil_prolong_smooth<T, 1>(L.nx, L.ny, C.nx, C.ny, fixed_, PrecBufs<T>::st(L),
+                                            buf[w], cx, b, buf[w ^ 1], (T)omega_, s)));
+    w ^= 1;
+    --post;
+  } else {
+    TMX_TRY(prolong_to<T>(*this, l, cx, buf[w], s));
+  }
+  for (int k = 0; k < post; ++k) {
+    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1],
+                        k == post - 1 ? last_dot : nullptr, s));
     w ^= 1;
   }
   return 0;
@@ -356,55 +489,130 @@ template <typename T>
 struct KrylovBufs;
 template <>
 struct KrylovBufs<double> {
+  static double* x(Hierarchy& h) { return h.x64_.get(); }
   static double* r(Hierarchy& h) { return h.r64_.get(); }
   static double* z(Hierarchy& h) { return h.z64_.get(); }
   static double* p(Hierarchy& h) { return h.p64_.get(); }
   static double* q(Hierarchy& h) { return h.q64_.get(); }
+  static double* pb(Hierarchy& h) { return h.pb64_.get(); }
 };
 template <>
 struct KrylovBufs<float> {
+  static float* x(Hierarchy& h) { return h.d32_.get(); }
   static float* r(Hierarchy& h) { return h.r32_.get(); }
   static float* z(Hierarchy& h) { return h.z32_.get(); }
   static float* p(Hierarchy& h) { return h.p32_.get(); }
   static float* q(Hierarchy& h) { return h.q32_.get(); }
+  static float* pb(Hierarchy& h) { return h.pb32_.get(); }
 };
 
 template <typename T>
-int Hierarchy::pcg(Workspace& ws, const T* b, T* x, double rtol, int max_iter, int* iters,
-                   double* rel, double bnorm2, hipStream_t s) {
+T* Hierarchy::x_buf() {
+  return KrylovBufs<T>::x(*this);
+}
+
+// PCG start, given r = b already in place: z = M r, rz = r.z, beta = 0, p_old = 0
+template <typename T>
+static int enqueue_pcg_init(Hierarchy& h, Workspace& ws, hipStream_t s) {
+  const size_t nv = h.n();
+  T* r = KrylovBufs<T>::r(h);
+  T* z = KrylovBufs<T>::z(h);
+  RedSlot rz = ws.region(2);
+  TMX_TRY(h.vcycle<T>(0, r, z, s, &rz));
+  TMX_HIP(fill_zero_bytes(KrylovBufs<T>::p(h), sizeof(T) * nv, s));
+  TMX_HIP(pcg_start(rz, ws.sc(), kPcg, s));
+  return 0;
+}
+
+// One PCG iteration, all scalars on the device, four launches plus the V-cycle:
+//   p = z + beta p_old and q = A p (partials p.q); alpha; x += alpha p, r -= alpha q
+//   (partials r.r); z = M r (partials r.z from the last sweep); rr, beta, stop flags.
+// parity selects which of the two direction buffers holds p_old.
+template <typename T>
+static int enqueue_pcg_iteration(Hierarchy& h, Workspace& ws, int parity, hipStream_t s) {
+  const size_t nv = h.n();
+  T* x = KrylovBufs<T>::x(h);
+  T* r = KrylovBufs<T>::r(h);
+  T* z = KrylovBufs<T>::z(h);
+  T* q = KrylovBufs<T>::q(h);
+  T* p_old = parity ? KrylovBufs<T>::pb(h) : KrylovBufs<T>::p(h);
+  T* p_new = parity ? KrylovBufs<T>::p(h) : KrylovBufs<T>::pb(h);
+  double* sc = ws.sc();
+  RedSlot pq = ws.region(0), rr = ws.region(1), rz = ws.region(2);
+  TMX_TRY(h.apply_update<T>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s));
+  TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, x, r, p_new, q, rr, s));
+  TMX_TRY(h.vcycle<T>(0, r, z, s, &rz));
+  TMX_HIP(pcg_finalize(pq, rr, rz, sc, kPcg, s));
+  return 0;
+}
+
+template <typename T>
+static int capture(Hierarchy& h, Workspace& ws, int which, hipGraphExec_t* out, hipStream_t s) {
+  hipGraph_t g = nullptr;
+  TMX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = which == 0 ? enqueue_pcg_init<T>(h, ws, s)
+                            : enqueue_pcg_iteration<T>(h, ws, which - 1, s);
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != 0) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  TMX_HIP(e);
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  TMX_HIP(ei);
+  return 0;
+}
+
+template <typename T>
+int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double bnorm2,
+                   int* iters, double* rel, hipStream_t s) {
   const size_t nv = n();
-  T* r = KrylovBufs<T>::r(*this);
-  T* z = KrylovBufs<T>::z(*this);
-  T* p = KrylovBufs<T>::p(*this);
-  T* q = KrylovBufs<T>::q(*this);
+  const int prec = sizeof(T) == 8 ? 0 : 1;
+  for (int w = 0; w < 3; ++w)
+    if (!graphs_[prec][w]) TMX_TRY(capture<T>(*this, ws, w, &graphs_[prec][w], s));
+  double* sc = ws.sc();
   *iters = 0;
-  TMX_HIP(fill_zero_bytes(x, sizeof(T) * nv, s));
-  TMX_HIP(copy_vec<T>(nv, b, r, s));
-  TMX_TRY(vcycle<T>(0, r, z, s));
-  TMX_HIP(copy_vec<T>(nv, z, p, s));
-  int irz = kScRZ0, irzn = kScRZ1;
-  TMX_HIP(dot_t<T>(nv, r, z, ws.slot(irz), s));
-  const RedSlot pq = ws.slot(kScPQ), rr = ws.slot(kScRR);
+  *rel = 1.0;
+  TMX_HIP(fill_zero_bytes(KrylovBufs<T>::x(*this), sizeof(T) * nv, s));
+  TMX_HIP(copy_vec<T>(nv, b, KrylovBufs<T>::r(*this), s));
+  TMX_HIP(set_scalar(sc, kPcg.rr, bnorm2, s));
+  TMX_HIP(set_scalar(sc, kPcg.tol2, rtol * rtol * bnorm2, s));
+  TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
+  // iteration k is launched before the flag of iteration k-1 is read, so the device always
+  // has queued work; once a stop flag is set the remaining queued iteration changes nothing
+  double* ring = ws.host + Workspace::kScalars;
+  double flag = 0.0;
+  int launched = 0;
   for (int it = 1; it <= max_iter; ++it) {
-    TMX_TRY(apply<T>(p, q, &pq, s));
-    TMX_HIP(pcg_update_xr<T>(nv, ws.sc(), irz, kScPQ, x, r, p, q, rr, s));
-    TMX_TRY(ws.read(s, kScPQ, 2));
-    const double pqv = ws.host[kScPQ], rrv = ws.host[kScRR];
-    *iters = it;
-    if (!std::isfinite(pqv) || !std::isfinite(rrv)) {
-      set_error("non-finite residual in PCG");
-      return -5;
+    TMX_HIP(hipGraphLaunch(graphs_[prec][1 + ((it - 1) & 1)], s));
+    TMX_HIP(hipMemcpyAsync(ring + 4 * (it & 1), sc + kPcg.flag, 3 * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    TMX_HIP(hipEventRecord(ev_[it & 1], s));
+    launched = it;
+    if (it > 1) {
+      TMX_HIP(hipEventSynchronize(ev_[(it - 1) & 1]));
+      flag = ring[4 * ((it - 1) & 1)];
+      if (flag != 0.0) break;
     }
-    if (pqv <= 0.0) {
-      set_error("PCG breakdown: p.Ap <= 0 (operator not positive definite)");
-      return -4;
-    }
-    *rel = std::sqrt(rrv / bnorm2);
-    if (*rel < rtol) return 0;
-    TMX_TRY(vcycle<T>(0, r, z, s));
-    TMX_HIP(dot_t<T>(nv, r, z, ws.slot(irzn), s));
-    TMX_HIP(pcg_update_p<T>(nv, ws.sc(), irzn, irz, z, p, s));
-    std::swap(irz, irzn);
+  }
+  TMX_HIP(hipEventSynchronize(ev_[launched & 1]));
+  flag = ring[4 * (launched & 1)];
+  const double rrv = ring[4 * (launched & 1) + 1];
+  *iters = (int)ring[4 * (launched & 1) + 2];
+  *rel = std::sqrt(std::max(rrv, 0.0) / bnorm2);
+  if (g_trace)
+    std::fprintf(stderr, "[tmx] pcg<%s> %s n=%zu: %d its, rel %.3e, flag %g\n",
+                 sizeof(T) == 8 ? "f64" : "f32", phys_ == kElasticity ? "elast" : "filter", nv,
+                 *iters, *rel, flag);
+  if (flag == 1.0) return 0;
+  if (flag == 2.0) {
+    set_error("PCG breakdown: p.Ap <= 0 (operator not positive definite)");
+    return -4;
+  }
+  if (flag == 3.0) {
+    set_error("non-finite residual in PCG");
+    return -5;
   }
   set_error("PCG did not reach the tolerance within the iteration limit");
   return -3;
@@ -418,7 +626,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
                      SolveStats* st, hipStream_t s) {
   const size_t nv = n();
   auto t0 = std::chrono::steady_clock::now();
-  TMX_HIP(dot(nv, b, b, ws.slot(kScBB), s));
+  RedSlot gen = ws.region(3);
+  TMX_HIP(dot(nv, b, b, gen, ws.sc() + kScBB, s));
   TMX_TRY(ws.read(s, kScBB, 1));
   const double bb = ws.host[kScBB];
   SolveStats local;
@@ -439,12 +648,13 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   if (prm.precision == 0) {
     int it = 0;
     double rel = 0;
-    rc = pcg<double>(ws, b, x, prm.rtol, prm.max_iter, &it, &rel, bb, s);
+    rc = pcg<double>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s);
     S.iterations = it;
     S.outer = 1;
     if (rc == 0 || rc == -3) {  // report the true residual
+      TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
       TMX_TRY(residual<double>(x, b, rr64_.get(), s));
-      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), ws.slot(kScRR64), s));
+      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScRR64, s));
       TMX_TRY(ws.read(s, kScRR64, 1));
       S.rel_residual = std::sqrt(ws.host[kScRR64] / bb);
     }
@@ -452,9 +662,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
     TMX_HIP(copy_vec<double>(nv, b, rr64_.get(), s));
     double rr = bb;
-    ws.host[kScRR64] = bb;
-    TMX_HIP(hipMemcpyAsync(ws.sc() + kScRR64, ws.host + kScRR64, sizeof(double),
-                           hipMemcpyHostToDevice, s));
+    TMX_HIP(set_scalar(ws.sc(), kScRR64, bb, s));
     rc = -3;
     int budget = prm.max_iter;
     double best = 1.0;
@@ -477,7 +685,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       int it = 0;
       double rel = 0;
       const double inner = std::max(prm.inner_rtol, 0.5 * prm.rtol / S.rel_residual);
-      int irc = pcg<float>(ws, rin, d32_.get(), inner, budget, &it, &rel, 1.0, s);
+      int irc = pcg<float>(ws, rin, inner, budget, 1.0, &it, &rel, s);
       if (g_trace)
         std::fprintf(stderr, "[tmx] refine %d: |r|/|b| = %.3e, inner its %d rel %.3e rc %d\n",
                      outer, S.rel_residual, it, rel, irc);
@@ -487,7 +695,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       if (irc != 0 && irc != -3) return irc;
       TMX_HIP(axpy_scaled<float>(nv, ws.sc(), kScRR64, d32_.get(), x, s));
       TMX_TRY(residual<double>(x, b, rr64_.get(), s));
-      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), ws.slot(kScRR64), s));
+      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScRR64, s));
       TMX_TRY(ws.read(s, kScRR64, 1));
       rr = ws.host[kScRR64];
       if (!std::isfinite(rr)) {
@@ -507,12 +715,14 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   return rc;
 }
 
-template int Hierarchy::apply<double>(const double*, double*, const RedSlot*, hipStream_t);
-template int Hierarchy::apply<float>(const float*, float*, const RedSlot*, hipStream_t);
+template int Hierarchy::apply<double>(const double*, double*, RedSlot*, hipStream_t);
+template int Hierarchy::apply<float>(const float*, float*, RedSlot*, hipStream_t);
+template double* Hierarchy::x_buf<double>();
+template float* Hierarchy::x_buf<float>();
 template int Hierarchy::residual<double>(const double*, const double*, double*, hipStream_t);
 template int Hierarchy::smooth0<double>(const double*, const double*, double*, hipStream_t);
 template int Hierarchy::smooth0<float>(const float*, const float*, float*, hipStream_t);
-template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t);
-template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t);
+template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t, RedSlot*);
+template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t, RedSlot*);
 
 }  // namespace tmx

@@ -385,6 +385,36 @@ class ElasticityProblem:
         _lib.check(self._h.lib.tmx_autotune(self._h.ptr, rows, us))
         return {"rows_fp64": rows[0], "rows_fp32": rows[1], "us_fp64": us[0], "us_fp32": us[1]}
 
+    def set_option(self, name: str, value):
+        """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
+        nu_coarse_filter, omega_elast, omega_filter, filter_rtol."""
+        opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
+               "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
+               "nu_coarse": _lib.TMX_OPT_NU_COARSE,
+               "omega_elast": _lib.TMX_OPT_OMEGA_ELAST,
+               "omega_filter": _lib.TMX_OPT_OMEGA_FILTER,
+               "filter_rtol": _lib.TMX_OPT_FILTER_RTOL,
+               "nu_fine_filter": _lib.TMX_OPT_NU_FINE_FILTER,
+               "nu_coarse_filter": _lib.TMX_OPT_NU_COARSE_FILTER}[name]
+        _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
+
+    def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
+        """One multigrid V-cycle z = M^-1 r of the elasticity (operator of xi) or filter
+        hierarchy."""
+        h = self._h
+        w = {"elasticity": 0, "filter": 1}[which]
+        n = h.n_dofs if w == 0 else h.n_vertices
+        r = _as_field(r, n, h.device, "r")
+        z = torch.empty_like(r)
+        xi_p = None
+        if xi is not None:
+            xi_t = _as_field(xi, h.n_vertices, h.device, "xi")
+            xi_p = _ptr(xi_t)
+        prec = {"fp64": _lib.TMX_FP64, "fp32": _lib.TMX_FP32}[precision]
+        h.stream()
+        _lib.check(h.lib.tmx_precondition(h.ptr, w, prec, xi_p, _ptr(r), _ptr(z)))
+        return z
+
     def set_tiles(self, rows_fp64: int = 0, rows_fp32: int = 0):
         _lib.check(self._h.lib.tmx_set_tiles(self._h.ptr, int(rows_fp64), int(rows_fp32)))
 
