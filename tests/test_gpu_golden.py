@@ -1,0 +1,77 @@
+"""GPU results against the committed oracle fixtures (tests/golden/make_cfg_golden.py) and
+mixed precision against FP64 at the full N=1000 size.
+
+Tolerances (stated per the workload): FP64 solves to 1e-12 relative residual must reproduce
+the oracle's direct solve to 1e-8 relative (displacement, compliance, gradient); the mixed
+FP32-in-FP64 path to 1e-6.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests.cases import CANTILEVER, gpu_problem, smoothed_density
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-6)])
+def test_cfg1_fixtures(precision, tol):
+    g = np.load(os.path.join(GOLDEN, "cfg1_n20.npz"))
+    gp = gpu_problem(CANTILEVER, 60, 20, precision=precision, rtol=1e-12 if precision == "fp64" else 1e-10)
+    gp.set_penalization(3)
+    for seed in range(3):
+        J, w, mw = gp.objective_and_gradient(torch.as_tensor(g[f"rho_{seed}"], device="cuda:0"))
+        print(precision, seed, J, float(g[f"J_{seed}"]), gp.last_stats, flush=True)
+        assert abs(J - float(g[f"J_{seed}"])) < tol * abs(float(g[f"J_{seed}"]))
+        assert rel(gp.filtered_rho.cpu(), g[f"xi_{seed}"]) < 1e-10
+        assert rel(gp.u.cpu(), g[f"u_{seed}"]) < tol
+        assert rel(w.cpu(), g[f"w_{seed}"]) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-6)])
+def test_cfg2_n400_fixture(precision, tol):
+    g = json.load(open(os.path.join(GOLDEN, "cfg2_n400.json")))
+    N = 400
+    gp = gpu_problem(CANTILEVER, 3 * N, N, precision=precision,
+                     rtol=1e-12 if precision == "fp64" else 1e-10)
+    gp.set_penalization(3)
+    rho = torch.as_tensor(smoothed_density(3 * N, N, seed=0, sigma=5.0), device="cuda:0")
+    J, w, _ = gp.objective_and_gradient(rho)
+    u = gp.u.cpu().numpy()
+    w = w.cpu().numpy()
+    print(precision, J, g["J"], gp.last_stats, flush=True)
+    assert abs(J - g["J"]) < tol * g["J"]
+    assert abs(np.linalg.norm(u) - g["u_norm"]) < tol * g["u_norm"]
+    assert np.abs(u[g["u_index"]] - np.array(g["u_values"])).max() < tol * g["u_max"]
+    assert abs(np.linalg.norm(w) - g["w_norm"]) < tol * g["w_norm"]
+    assert np.abs(w[g["w_index"]] - np.array(g["w_values"])).max() < tol * np.abs(w).max()
+
+
+@pytest.mark.parametrize("p", [3, 5])
+def test_cfg3_mixed_vs_fp64_n1000(p):
+    """N=1000 (6,008,002 DOFs): mixed precision to 1e-8 relative residual reproduces the FP64
+    solve (1e-10) to 1e-6 in displacement and compliance."""
+    N = 1000
+    gp = gpu_problem(CANTILEVER, 3 * N, N, rtol=1e-10)
+    gp.set_penalization(p)
+    xi = gp.filter.apply(torch.as_tensor(smoothed_density(3 * N, N, seed=0, sigma=5.0),
+                                         device="cuda:0"))
+    u64 = gp.forward(xi, precision="fp64", rtol=1e-10)
+    J64 = gp.compliance(u64)
+    um = gp.forward(xi, precision="mixed", rtol=1e-8)
+    st = gp.last_stats
+    Jm = gp.compliance(um)
+    du = float(torch.linalg.norm(um - u64) / torch.linalg.norm(u64))
+    print(f"p={p} J64={J64} Jm={Jm} dJ={abs(Jm - J64) / J64:.2e} du={du:.2e} {st}", flush=True)
+    assert abs(Jm - J64) < 1e-6 * J64
+    assert du < 1e-6

@@ -1,0 +1,144 @@
+"""CPU tests of the host side: the C ABI library, the Python boundary and the input schema.
+No GPU compute is issued here (the calls used return before touching the device)."""
+
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+from tests.conftest import ROOT, design_path
+
+HEADER = os.path.join(ROOT, "include", "tmx.h")
+LIB = os.path.join(ROOT, "topomax_amd", "libtmx.so")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tmx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build libtmx.so first (python -m topomax_amd.build)"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (tmx_[a-z0-9_]+)$", out, flags=re.M))
+    declared = header_functions()
+    assert declared, "no declarations parsed"
+    missing = [f for f in declared if f not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_ctypes_binding_covers_header():
+    from topomax_amd import _lib
+    assert sorted(_lib.SIGNATURES) == header_functions()
+    lib = _lib.load()
+    assert lib.tmx_abi_version() == 1
+
+
+def test_desc_layout_matches_c_struct():
+    # field offsets of tmx_problem_desc as the C compiler sees them
+    src = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "tmx.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(tmx_problem_desc), offsetof(tmx_problem_desc, fixed_sides),
+         offsetof(tmx_problem_desc, tractions), offsetof(tmx_problem_desc, body_force),
+         offsetof(tmx_problem_desc, precision), sizeof(tmx_solve_stats));
+  return 0;
+}
+'''
+    import tempfile
+    from topomax_amd import _lib
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                               check=True).stdout.split()]
+    D = _lib.ProblemDesc
+    assert vals == [ctypes.sizeof(D), D.fixed_sides.offset, D.tractions.offset,
+                    D.body_force.offset, D.precision.offset, ctypes.sizeof(_lib.SolveStats)]
+
+
+def test_error_mapping_without_gpu():
+    from topomax_amd import _lib
+    lib = _lib.load()
+    # invalid descriptor is rejected before any device work
+    d = _lib.ProblemDesc()
+    d.nx, d.ny = 0, 10
+    h = ctypes.c_void_p()
+    rc = lib.tmx_problem_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _lib.TMX_ERR_INVALID and not h.value
+    with pytest.raises(ValueError, match="nx and ny"):
+        _lib.check(rc)
+    d.nx, d.ny, d.width, d.height = 10, 10, 1.0, 1.0
+    d.young_modulus, d.poisson_ratio, d.penalty_minimum = 1.0, 0.6, 1e-6
+    with pytest.raises(ValueError, match="poisson"):
+        _lib.check(lib.tmx_problem_create(ctypes.byref(d), ctypes.byref(h)))
+    d.poisson_ratio = 0.3
+    d.n_tractions = 1
+    d.tractions[0].side = 7
+    with pytest.raises(ValueError, match="Malformed side"):
+        _lib.check(lib.tmx_problem_create(ctypes.byref(d), ctypes.byref(h)))
+    with pytest.raises(ValueError):
+        _lib.check(lib.tmx_set_penalization(None, 3.0))
+    with pytest.raises(_lib.NotConvergedError):
+        _lib.check(_lib.TMX_ERR_NOT_CONVERGED)
+    with pytest.raises(RuntimeError):
+        _lib.check(_lib.TMX_ERR_BREAKDOWN)
+
+
+def test_parse_reference_designs():
+    from topomax_amd import Side, parse_design
+    dom, par = parse_design(design_path("triangle"))
+    assert (dom.width, dom.height, dom.volume_fraction) == (1.0, 1.0, 0.5)
+    assert par.fixed_sides == [Side.BOTTOM] and par.tractions is None
+    assert par.body_force.region.center == (0.5, 0.5) and par.body_force.value == (0.0, -10.0)
+    dom, par = parse_design(design_path("bridge"))
+    assert par.fixed_sides == [Side.LEFT, Side.RIGHT] and par.body_force is None
+    assert par.tractions[0].side == Side.TOP and par.tractions[0].value == (0.0, -2000.0)
+
+
+def test_broken_design_raises():
+    """tests/test_design_parser.py of the reference: a 3-tuple value raises ValueError."""
+    from topomax_amd import parse_design
+    with pytest.raises(ValueError):
+        parse_design(design_path("broken_design"))
+
+
+def test_penalizer_reference_semantics():
+    from topomax_amd import ElasticPenalizer
+    pen = ElasticPenalizer()
+    with pytest.raises(ValueError, match="penalization"):
+        pen(0.5)
+    pen.set_penalization(3.0)
+    assert abs(pen(0.5) - (1e-6 + 0.125 * (1 - 1e-6))) < 1e-16
+    assert abs(pen.derivative(0.5) - 3 * 0.25 * (1 - 1e-6)) < 1e-16
+
+
+def test_mesh_from_N_matches_reference_rule():
+    from topomax_amd import StructuredMesh
+    m = StructuredMesh.from_N(3.0, 1.0, 20)
+    assert (m.nx, m.ny) == (60, 20)
+    m = StructuredMesh.from_N(10.0, 5.0, 10)   # N per unit length = int(10 / 5) = 2
+    assert (m.nx, m.ny) == (20, 10)
+
+
+def test_bench_refuses_multi_gpu():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True)
+    assert p.returncode == 2 and "one GPU" in p.stderr
+
+
+def test_product_does_not_import_oracle():
+    pkg = os.path.join(ROOT, "topomax_amd")
+    for name in os.listdir(pkg):
+        if name.endswith(".py"):
+            src = open(os.path.join(pkg, name)).read()
+            assert "oracle" not in src.replace("no CPU fallback", ""), name

@@ -211,7 +211,9 @@ class HelmholtzFilter:
         rho = _as_field(input_function, h.n_vertices, h.device, "input_function")
         xi = torch.empty_like(rho)
         h.stream()
-        _lib.check(h.lib.tmx_filter_apply(h.ptr, _ptr(rho), _ptr(xi)))
+        rc = h.lib.tmx_filter_apply(h.ptr, _ptr(rho), _ptr(xi))
+        if rc != _lib.TMX_ERR_NONFINITE:  # non-finite input: NaN output, as a direct solve
+            _lib.check(rc)
         return xi
 
     def solve(self, rhs) -> torch.Tensor:
@@ -219,7 +221,9 @@ class HelmholtzFilter:
         b = _as_field(rhs, h.n_vertices, h.device, "rhs")
         w = torch.empty_like(b)
         h.stream()
-        _lib.check(h.lib.tmx_filter_solve(h.ptr, _ptr(b), _ptr(w)))
+        rc = h.lib.tmx_filter_solve(h.ptr, _ptr(b), _ptr(w))
+        if rc != _lib.TMX_ERR_NONFINITE:
+            _lib.check(rc)
         return w
 
     def mass(self, x) -> torch.Tensor:
@@ -299,6 +303,8 @@ class ElasticityProblem:
         self.penalizer.assert_has_penalization()
         self.filtered_rho = self.filter.apply(rho)
         self.u = self.forward(self.filtered_rho)
+        if not bool(torch.isfinite(self.u).all()):
+            return math.nan
         return self.compliance(self.u)
 
     def calculate_objective_gradient(self) -> torch.Tensor:
