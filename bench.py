@@ -26,6 +26,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the roofline kernel at N=1000, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) of `bench.py --roofline-only`, each scaled by the same counter's
+# reading for a plain copy of 48,064,016 bytes (FETCH x1.9989, WRITE x1.0000).
+PMC_TRAFFIC_BYTES = {1000: 96.56e6}
 
 
 def cpu_model() -> str:
@@ -67,6 +71,41 @@ def bench_kernel(problem, kernel, precision, reps=50):
     return us.value, nbytes.value
 
 
+def batch_throughput(n_designs: int, N: int, steps: int, rtol: float):
+    """Many independent designs: n_designs cantilevers at N (penalties 1..5, seeds differ),
+    one objective+gradient each, run one after another vs in one tmx_batch call."""
+    import torch
+    from tests.cases import CANTILEVER, gpu_problem, smoothed_density
+    from topomax_amd import ProblemBatch
+    nx, ny = 3 * N, N
+    probs, rhos = [], []
+    for k in range(n_designs):
+        gp = gpu_problem(CANTILEVER, nx, ny, precision="mixed", rtol=rtol)
+        gp.set_penalization(1 + k % 5)
+        probs.append(gp)
+        rhos.append(torch.as_tensor(smoothed_density(nx, ny, seed=100 + k), device="cuda:0"))
+    batch = ProblemBatch(probs, workers=4)
+    for gp, r in zip(probs, rhos):
+        gp.objective_and_gradient(r)
+    batch.objective_and_gradient(rhos)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for gp, r in zip(probs, rhos):
+            gp.objective_and_gradient(r)
+    torch.cuda.synchronize()
+    t_serial = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.objective_and_gradient(rhos)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    ndof = n_designs * 2 * (nx + 1) * (ny + 1)
+    return {"designs": n_designs, "N": N, "workers": 4, "ms_serial": t_serial * 1e3,
+            "ms_batched": t_batch * 1e3, "GDOF_s_serial": ndof / t_serial / 1e9,
+            "GDOF_s_batched": ndof / t_batch / 1e9, "speedup": t_serial / t_batch}
+
+
 def cpu_baseline(n_cpu: int, seconds_budget: float = 30.0):
     """The CPU oracle (numpy/scipy, SuperLU direct solves, single thread) on a bounded sample
     of the same workload: one objective + gradient of the same cantilever at N=n_cpu."""
@@ -104,6 +143,13 @@ def main():
     ap.add_argument("--cpu-N", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-designs", type=int, default=16)
+    ap.add_argument("--batch-N", type=int, default=250)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="set up the N operator, then launch only the roofline kernel (FP32 "
+                         "fine-grid sweep, 20x) and a plain copy of the 2 n_v load vector "
+                         "(calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,9 +165,18 @@ def main():
     nx, ny = 3 * N, N
     problem = gpu_problem(CANTILEVER, nx, ny, precision="mixed", rtol=args.rtol)
     problem.set_penalization(args.p)
-    tune = problem.autotune()
     rho = torch.as_tensor(smoothed_density(nx, ny, seed=0, sigma=5.0), device="cuda:0")
     ndof = 2 * (nx + 1) * (ny + 1)
+    if args.roofline_only:
+        problem.objective_and_gradient(rho)
+        torch.cuda.synchronize()
+        us, nbytes = bench_kernel(problem, 1, "fp32", reps=20)
+        F = problem.load_vector()  # one k_copy<double> of 2 n_v doubles
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline_kernel_us": us, "algorithmic_bytes": nbytes,
+                          "copy_bytes_each_way": F.numel() * 8}), flush=True)
+        return
+    tune = problem.autotune()
 
     t_mixed, J_mixed, w_mixed, st_mixed = time_steps(problem, rho, args.steps, args.warmup,
                                                      "mixed", args.rtol)
@@ -146,8 +201,13 @@ def main():
     dom = kernels["smooth_fp32"]
     roofline = {"kernel": "k_elast_sweep<float,true,kSmooth> (fine-grid FP32 block-Jacobi sweep)",
                 "bound": "hbm", "achieved": dom["GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": dom["GBs"] / PEAK_HBM_GBS, "traffic": None,
+                "frac": dom["GBs"] / PEAK_HBM_GBS, "traffic": PMC_TRAFFIC_BYTES.get(N),
                 "algorithmic_bytes_per_launch": dom["bytes"], "avg_launch_us": dom["us"]}
+
+    batch = None
+    if not args.no_batch:
+        batch = batch_throughput(args.batch_designs, args.batch_N, max(2, args.steps // 2),
+                                 args.rtol)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -176,6 +236,7 @@ def main():
         "rel_residual": st_mixed[-1]["rel_residual"],
         "compliance": J_mixed,
         "fp64": result_fp64,
+        "batch": batch,
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

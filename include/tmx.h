@@ -175,12 +175,16 @@ int tmx_precondition(tmx_problem* h, int which, int precision, const double* xi,
 int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
 
 /* Batched objective + gradient over many independent problems (uneven sizes, penalties and
- * loads).  Each handle keeps its own hierarchy; the batch enqueues all designs' work phase by
- * phase on one stream so that every kernel launch covers all unconverged designs.
- * rho[i], w[i] device pointers; J_host[i] receives the compliance. */
+ * loads; each problem keeps its own penalty, tiles and operators).  Designs run concurrently,
+ * each on its own stream, driven by `workers` host threads (default 4), largest first, so the
+ * latency-bound coarse-grid kernels of different designs overlap.  The handles must not be
+ * used by other threads during the call.  rho[i], w[i] are device pointers (n_v of design i);
+ * J_host[i] receives the compliance (NaN for a non-finite design).  stats may be NULL or
+ * point to `count` entries.  No counterpart in the reference (it solves one design). */
 typedef struct tmx_batch tmx_batch;
 int tmx_batch_create(tmx_problem* const* problems, int count, tmx_batch** out);
 void tmx_batch_destroy(tmx_batch* b);
+int tmx_batch_set_workers(tmx_batch* b, int workers);
 int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho, double* J_host,
                                      double* const* w, const tmx_solve_opts* opts,
                                      tmx_solve_stats* stats);

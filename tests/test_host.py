@@ -92,6 +92,16 @@ def test_error_mapping_without_gpu():
         _lib.check(_lib.TMX_ERR_NOT_CONVERGED)
     with pytest.raises(RuntimeError):
         _lib.check(_lib.TMX_ERR_BREAKDOWN)
+    # batch argument checks happen before any device work
+    b = ctypes.c_void_p()
+    with pytest.raises(ValueError, match="batch"):
+        _lib.check(lib.tmx_batch_create(None, 2, ctypes.byref(b)))
+    nulls = (ctypes.c_void_p * 2)(None, None)
+    with pytest.raises(ValueError, match="null problem"):
+        _lib.check(lib.tmx_batch_create(nulls, 2, ctypes.byref(b)))
+    assert not b.value
+    with pytest.raises(ValueError):
+        _lib.check(lib.tmx_batch_set_workers(None, 4))
 
 
 def test_parse_reference_designs():

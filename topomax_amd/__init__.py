@@ -5,11 +5,12 @@ ElasticityProblem / HelmholtzFilter call surface."""
 from .designs import (DomainParameters, ElasticityParameters, Force, CircularRegion, Side,
                       Traction, parse_design, parse_design_dict)
 from .elasticity_problem import (ControlSpace, ElasticPenalizer, ElasticityProblem,
-                                 HelmholtzFilter, StructuredMesh, problem_from_design, solve_opts)
+                                 HelmholtzFilter, ProblemBatch, StructuredMesh,
+                                 problem_from_design, solve_opts)
 from .autograd import compliance
 
 __all__ = [
     "DomainParameters", "ElasticityParameters", "Force", "CircularRegion", "Side", "Traction",
     "parse_design", "parse_design_dict", "ControlSpace", "ElasticPenalizer", "ElasticityProblem",
-    "HelmholtzFilter", "StructuredMesh", "problem_from_design", "solve_opts", "compliance",
+    "HelmholtzFilter", "ProblemBatch", "StructuredMesh", "problem_from_design", "solve_opts", "compliance",
 ]

@@ -105,6 +105,7 @@ SIGNATURES = {
     "tmx_bench_kernel": (_I, [_P, _I, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "tmx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
     "tmx_batch_destroy": (None, [_P]),
+    "tmx_batch_set_workers": (_I, [_P, _I]),
     "tmx_batch_objective_and_gradient": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D),
                                               ctypes.POINTER(_P), ctypes.POINTER(SolveOpts),
                                               ctypes.POINTER(SolveStats)]),

@@ -23,6 +23,7 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    "-pthread",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-result",
@@ -58,7 +59,7 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
     while procs:
         _wait(procs.pop(0))
     tmp = LIB + ".tmp"
-    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     if verbose:
         print(" ".join(link), flush=True)
     _run(link)

@@ -1,10 +1,13 @@
 // extern "C" boundary of topomax_amd (declared in include/tmx.h).
 #include "../../include/tmx.h"
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tmx_solver.h"
@@ -20,6 +23,7 @@ struct tmx_problem {
   hipStream_t stream = nullptr;  // internal non-blocking stream (all work, graph capture)
   hipStream_t user = nullptr;    // caller's stream (tmx_set_stream), joined by events
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  int device = 0;  // HIP device current at creation; all of the handle's memory lives there
   int nx = 0, ny = 0;
   double hx = 0, hy = 0, lam = 0, mu = 0, m = 1e-6;
   unsigned fixed = 0;  // sides actually fixed after the near() test
@@ -101,6 +105,7 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   // Lame parameters as the reference computes them (FEM_src/elasisity_problem.py:98-99)
   h->mu = d->young_modulus / (2 * (1 + d->poisson_ratio));
   h->lam = h->mu * d->poisson_ratio / (0.5 - d->poisson_ratio);
+  TMX_HIP(hipGetDevice(&h->device));
   TMX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   TMX_HIP(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
   TMX_HIP(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
@@ -564,8 +569,14 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
 }
 
 // ------------------------------------------------------------------ batch
+// Independent designs run concurrently: each worker thread drives whole objective+gradient
+// steps of one design at a time on that design's own stream, so the latency-bound coarse
+// multigrid kernels of different designs overlap on the device.  Workers take designs
+// largest first from a shared counter.
 struct tmx_batch {
   std::vector<tmx_problem*> probs;
+  std::vector<int> order;  // largest first
+  int workers = 4;         // hardware queues per process on the target boxes
 };
 
 extern "C" int tmx_batch_create(tmx_problem* const* problems, int count, tmx_batch** out) {
@@ -577,23 +588,66 @@ extern "C" int tmx_batch_create(tmx_problem* const* problems, int count, tmx_bat
       delete b;
       return invalid("null problem in batch");
     }
+    for (int j = 0; j < i; ++j)
+      if (problems[j] == problems[i]) {
+        delete b;
+        return invalid("a problem appears twice in the batch");
+      }
     b->probs.push_back(problems[i]);
+    b->order.push_back(i);
   }
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
+    return nverts(b->probs[a]) > nverts(b->probs[c]);
+  });
   *out = b;
   return TMX_OK;
 }
 
 extern "C" void tmx_batch_destroy(tmx_batch* b) { delete b; }
 
+extern "C" int tmx_batch_set_workers(tmx_batch* b, int workers) {
+  if (!b) return invalid("null batch");
+  if (workers < 1 || workers > 64) return invalid("workers must be in [1, 64]");
+  b->workers = workers;
+  return TMX_OK;
+}
+
 extern "C" int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho,
                                                 double* J, double* const* w,
                                                 const tmx_solve_opts* opts,
                                                 tmx_solve_stats* stats) {
   if (!b || !rho || !J || !w) return invalid("null argument");
-  for (size_t i = 0; i < b->probs.size(); ++i) {
-    const int rc = tmx_objective_and_gradient(b->probs[i], rho[i], &J[i], nullptr, nullptr,
-                                              w[i], nullptr, opts, stats ? &stats[i] : nullptr);
-    if (rc != TMX_OK) return rc;
-  }
+  const int n = (int)b->probs.size();
+  for (int i = 0; i < n; ++i)
+    if (!rho[i] || !w[i]) return invalid("null design pointer in batch");
+  std::vector<int> rc(n, TMX_OK);
+  std::vector<std::string> msg(n);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= n) return;
+      const int i = b->order[k];
+      if (hipSetDevice(b->probs[i]->device) != hipSuccess) {
+        rc[i] = TMX_ERR_HIP;
+        msg[i] = "hipSetDevice failed in a batch worker";
+        continue;
+      }
+      rc[i] = tmx_objective_and_gradient(b->probs[i], rho[i], &J[i], nullptr, nullptr, w[i],
+                                         nullptr, opts, stats ? &stats[i] : nullptr);
+      if (rc[i] == TMX_ERR_NONFINITE) J[i] = NAN;
+      else if (rc[i] != TMX_OK) msg[i] = last_error_cstr();
+    }
+  };
+  const int nw = std::min(b->workers, n);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nw; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  for (int i = 0; i < n; ++i)
+    if (rc[i] != TMX_OK && rc[i] != TMX_ERR_NONFINITE) {
+      set_error("design " + std::to_string(i) + ": " + msg[i]);
+      return rc[i];
+    }
   return TMX_OK;
 }

@@ -446,3 +446,58 @@ def problem_from_design(design_file_or_dict, N: int, *, precision="fp64", rtol=1
     space = ControlSpace(mesh, device)
     return ElasticityProblem(mesh, space, domain, params, precision=precision, rtol=rtol,
                              device=device), domain
+
+
+class ProblemBatch:
+    """Objective + gradient of many independent designs in one call (``tmx_batch_*``).
+
+    The problems may differ in size, loads, supports and penalty; each keeps its own
+    hierarchy.  The library runs them concurrently, one host worker per design at a time
+    (``workers``, default 4), largest first, so the latency-bound coarse-grid work of
+    different designs overlaps on the GPU.  Results are identical to calling each problem's
+    :meth:`ElasticityProblem.objective_and_gradient` alone."""
+
+    def __init__(self, problems, workers: int = 4):
+        self.problems = list(problems)
+        if not self.problems:
+            raise ValueError("a batch needs at least one problem")
+        self.lib = _lib.load()
+        ptrs = (ctypes.c_void_p * len(self.problems))(*[p._h.ptr.value for p in self.problems])
+        self.ptr = ctypes.c_void_p()
+        _lib.check(self.lib.tmx_batch_create(ptrs, len(self.problems), ctypes.byref(self.ptr)))
+        _lib.check(self.lib.tmx_batch_set_workers(self.ptr, int(workers)))
+        self.last_stats = []
+
+    def objective_and_gradient(self, rhos, *, rtol=None, precision=None):
+        """Returns (J list, w list); J is NaN for a design with non-finite entries."""
+        n = len(self.problems)
+        if len(rhos) != n:
+            raise ValueError(f"expected {n} designs, got {len(rhos)}")
+        rs, ws = [], []
+        for p, r in zip(self.problems, rhos):
+            p.penalizer.assert_has_penalization()
+            h = p._h
+            rs.append(_as_field(r, h.n_vertices, h.device, "rho"))
+            ws.append(torch.empty_like(rs[-1]))
+            h.stream()
+        for p in self.problems:
+            torch.cuda.current_stream(p._h.device).synchronize()
+        J = (ctypes.c_double * n)()
+        stats = (_lib.SolveStats * n)()
+        rp = (ctypes.c_void_p * n)(*[r.data_ptr() for r in rs])
+        wp = (ctypes.c_void_p * n)(*[w.data_ptr() for w in ws])
+        rtols = {p.rtol for p in self.problems}
+        if rtol is None and len(rtols) != 1:
+            raise ValueError("problems of a batch with different rtol need an explicit rtol")
+        opts = solve_opts(rtol=rtol or rtols.pop(), precision=precision)
+        _lib.check(self.lib.tmx_batch_objective_and_gradient(self.ptr, rp, J, wp,
+                                                             ctypes.byref(opts), stats))
+        self.last_stats = [s.as_dict() for s in stats]
+        return [float(j) for j in J], ws
+
+    def __del__(self):
+        lib = getattr(self, "lib", None)
+        ptr = getattr(self, "ptr", None)
+        if lib is not None and ptr is not None and ptr.value:
+            lib.tmx_batch_destroy(ptr)
+            self.ptr = ctypes.c_void_p()
