@@ -43,7 +43,12 @@ enum tmx_status {
 
 enum tmx_side { TMX_LEFT = 0, TMX_RIGHT = 1, TMX_TOP = 2, TMX_BOTTOM = 3 };
 enum tmx_side_bits { TMX_FIX_LEFT = 1, TMX_FIX_RIGHT = 2, TMX_FIX_TOP = 4, TMX_FIX_BOTTOM = 8 };
-enum tmx_precision { TMX_FP64 = 0, TMX_MIXED = 1, TMX_FP32 = 2 };
+/* Solve schemes.  TMX_FP64: FP64 multigrid-PCG.  TMX_MIXED: FP64 PCG (FP64 operator, residual
+ * and iterates) preconditioned by an FP32 V-cycle.  TMX_MIXED_IR: FP32 multigrid-PCG inner
+ * solves inside FP64 iterative refinement.  TMX_FP32 names single-precision kernels
+ * (tmx_matvec, tmx_precondition, tmx_bench_kernel); as a solve precision it means TMX_MIXED.
+ * Filter solves run in FP64 PCG with an FP32 V-cycle unless TMX_FP64 is asked for. */
+enum tmx_precision { TMX_FP64 = 0, TMX_MIXED = 1, TMX_FP32 = 2, TMX_MIXED_IR = 3 };
 
 #define TMX_MAX_TRACTIONS 16
 

@@ -4,9 +4,9 @@ differences and the oracle gradient; plus the reference's error semantics on the
 dJ/drho = M w (w = filtered sensitivity, the reference's calculate_objective_gradient), so the
 autograd gradient is checked along 32 seeded unit directions d against
 (J(rho + h d) - J(rho - h d)) / 2h with h = 1e-6.  FD noise is ~ |J| * (solve tolerance) / h,
-so the bound is relative to |grad|: 1e-6 for FP64 (solves to 1e-13, h = 1e-6).  The mixed
-path stops at the FP64 residual floor (~1e-11 here), so its check uses rtol 1e-10, h = 1e-4 and
-a 1e-4 bound.
+so the bound is relative to |grad|: 1e-6 for FP64 PCG (solves to 1e-13, h = 1e-6), with an
+FP64 or an FP32 V-cycle.  FP32 inner solves inside FP64 iterative refinement stop at the FP64
+residual floor (~1e-11 here), so that check uses rtol 1e-10, h = 1e-4 and a 1e-4 bound.
 """
 
 import numpy as np
@@ -21,7 +21,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("precision,rtol,h,tol", [("fp64", 1e-13, 1e-6, 1e-6),
-                                                  ("mixed", 1e-10, 1e-4, 1e-4)])
+                                                  ("mixed", 1e-13, 1e-6, 1e-6),
+                                                  ("mixed-ir", 1e-10, 1e-4, 1e-4)])
 def test_autograd_matches_central_differences_n400(precision, rtol, h, tol):
     N = 400
     nx, ny = 3 * N, N

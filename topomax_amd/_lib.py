@@ -22,7 +22,7 @@ TMX_ERR_UNSUPPORTED = -6
 
 TMX_LEFT, TMX_RIGHT, TMX_TOP, TMX_BOTTOM = 0, 1, 2, 3
 TMX_FIX_LEFT, TMX_FIX_RIGHT, TMX_FIX_TOP, TMX_FIX_BOTTOM = 1, 2, 4, 8
-TMX_FP64, TMX_MIXED, TMX_FP32 = 0, 1, 2
+TMX_FP64, TMX_MIXED, TMX_FP32, TMX_MIXED_IR = 0, 1, 2, 3
 TMX_MAX_TRACTIONS = 16
 TMX_OPT_FUSED, TMX_OPT_TAIL, TMX_OPT_NU_PRE, TMX_OPT_NU_POST = 0, 1, 2, 3
 TMX_OPT_OMEGA_ELAST, TMX_OPT_OMEGA_FILTER = 4, 5

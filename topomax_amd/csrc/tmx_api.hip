@@ -68,6 +68,13 @@ static int invalid(const std::string& m) {
 extern "C" const char* tmx_last_error(void) { return last_error_cstr(); }
 extern "C" int tmx_abi_version(void) { return TMX_ABI_VERSION; }
 
+// tmx_precision -> elasticity solve scheme (TMX_FP32 solves run as TMX_MIXED)
+static int solve_precision(int p) {
+  return p == TMX_FP64 ? kSolveFP64 : p == TMX_MIXED_IR ? kSolveMixedIR : kSolveMixed;
+}
+// the filter solves to 1e-12 in FP64 PCG, with an FP32 V-cycle unless FP64 is asked for
+static int filter_precision(int p) { return p == TMX_FP64 ? kSolveFP64 : kSolveMixed; }
+
 static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   if (d->nx < 1 || d->ny < 1) return invalid("nx and ny must be >= 1");
   if (!(d->width > 0) || !(d->height > 0)) return invalid("width and height must be > 0");
@@ -83,7 +90,7 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
     if (d->tractions[i].side < 0 || d->tractions[i].side > 3)
       return invalid("Malformed side: " + std::to_string(d->tractions[i].side));
   if ((d->fixed_sides & ~15u) != 0) return invalid("Malformed side in fixed_sides");
-  if (d->precision < 0 || d->precision > 2) return invalid("unknown precision");
+  if (d->precision < 0 || d->precision > 3) return invalid("unknown precision");
   h->desc = *d;
   h->nx = d->nx;
   h->ny = d->ny;
@@ -159,10 +166,10 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   TMX_HIP(hipStreamSynchronize(s));
   h->el_prm.rtol = 1e-10;
   h->el_prm.max_iter = 1000;
-  h->el_prm.precision = d->precision == TMX_FP64 ? 0 : 1;
+  h->el_prm.precision = solve_precision(d->precision);
   h->fl_prm.rtol = 1e-12;
   h->fl_prm.max_iter = 500;
-  h->fl_prm.precision = 0;
+  h->fl_prm.precision = filter_precision(d->precision);
   TMX_HIP(hipStreamSynchronize(s));
   return TMX_OK;
 }
@@ -232,7 +239,7 @@ static SolveParams merge(const SolveParams& base, const tmx_solve_opts* o) {
   if (!o) return p;
   if (o->rtol > 0) p.rtol = o->rtol;
   if (o->max_iter > 0) p.max_iter = o->max_iter;
-  if (o->precision >= 0) p.precision = o->precision == TMX_FP64 ? 0 : 1;
+  if (o->precision >= 0) p.precision = solve_precision(o->precision);
   if (o->inner_rtol > 0) p.inner_rtol = o->inner_rtol;
   if (o->max_outer > 0) p.max_outer = o->max_outer;
   return p;
@@ -250,9 +257,12 @@ static void export_stats(const SolveStats& s, tmx_solve_stats* o) {
 
 static size_t nverts(const tmx_problem* h) { return (size_t)(h->nx + 1) * (h->ny + 1); }
 
-static int filter_rhs_solve(tmx_problem* h, const double* b, double* w) {
+static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
+                            const tmx_solve_opts* o = nullptr) {
   SolveStats st;
-  const int rc = h->fl.solve(h->ws, b, w, h->fl_prm, &st, h->stream);
+  SolveParams prm = h->fl_prm;
+  if (o && o->precision >= 0) prm.precision = filter_precision(o->precision);
+  const int rc = h->fl.solve(h->ws, b, w, prm, &st, h->stream);
   if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(nverts(h), NAN, w, h->stream));
   return rc;
 }
@@ -340,7 +350,8 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   Scope scope(h);
   double* xiv = xi ? xi : h->xi_s.get();
   double* uv = u ? u : h->u_s.get();
-  int rc = tmx_filter_apply(h, rho, xiv);
+  TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
+  int rc = filter_rhs_solve(h, h->b_s.get(), xiv, opts);
   if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
   SolveStats st;
   if (rc == TMX_OK) rc = elasticity_solve(h, xiv, uv, opts, &st);
@@ -353,7 +364,7 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   if (w || Mw) {
     double* wv = w ? w : h->w_s.get();
     TMX_TRY(tmx_sensitivity(h, xiv, uv, h->b_s.get()));
-    const int rc2 = filter_rhs_solve(h, h->b_s.get(), wv);
+    const int rc2 = filter_rhs_solve(h, h->b_s.get(), wv, opts);
     if (rc2 != TMX_OK && rc2 != TMX_ERR_NONFINITE) return rc2;
     if (Mw) TMX_TRY(tmx_mass_apply(h, wv, Mw));
     if (rc == TMX_OK) rc = rc2;

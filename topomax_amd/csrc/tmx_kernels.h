@@ -197,10 +197,11 @@ template <typename T>
 hipError_t pcg_update_xr(size_t n, const double* sc, int i_alpha, T* x, T* r, const T* p,
                          const T* q, RedSlot& rr, hipStream_t s);
 // as pcg_update_xr, with alpha = rz / sum(pq partials) formed by every block (0 once stopped
-// or if p.Ap <= 0)
+// or if p.Ap <= 0); rlow (optional) receives a float copy of the new r
 template <typename T>
 hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, const RedSlot& pq,
-                               T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s);
+                               T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s,
+                               float* rlow = nullptr);
 // end of a fused iteration: pq, rr, rz from partials; flags; beta = rz_new / rz; rz = rz_new
 hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
                         const PcgSlots& q, hipStream_t s);

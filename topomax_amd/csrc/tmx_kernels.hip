@@ -1029,62 +1029,80 @@ __device__ __forceinline__ double interp1_weight(int f, int nf, int nc, int I) {
   return w;
 }
 
+// Weights and clamped fine indices of the three fine positions around coarse node I along one
+// axis (zero weight where a position is outside the grid or outside the coarse hat function).
+__device__ __forceinline__ void restrict_axis(int I, int nf, int nc, int f[3], double w[3]) {
+  const int p = cpos(I, nf, nc);
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int g = p - 1 + t;
+    const bool in = g >= 0 && g <= nf;
+    w[t] = in ? interp1_weight(g, nf, nc, I) : 0.0;
+    f[t] = min(max(g, 0), nf);
+  }
+}
+
+// Branch-free: all 9 (x BS) loads are issued before any is used, so a launch costs one memory
+// round trip rather than one per contributing fine node.
 template <typename T, int BS>
 __global__ void k_restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                                     const T* __restrict__ rf, T* __restrict__ bc) {
-  const size_t nn = (size_t)(nxc + 1) * (nyc + 1);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nn = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn) return;
-  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  const int I = (int)(i % (unsigned)(nxc + 1)), J = (int)(i / (unsigned)(nxc + 1));
+  int fx[3], fy[3];
+  double wx[3], wy[3];
+  restrict_axis(I, nxf, nxc, fx, wx);
+  restrict_axis(J, nyf, nyc, fy, wy);
   double acc[BS];
 #pragma unroll
   for (int q = 0; q < BS; ++q) acc[q] = 0.0;
-  if (!node_fixed(fixed, I, J, nxc, nyc)) {
-    const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
-    for (int fy = py - 1; fy <= py + 1; ++fy) {
-      if (fy < 0 || fy > nyf) continue;
-      const double wy = interp1_weight(fy, nyf, nyc, J);
-      if (wy == 0.0) continue;
-      for (int fx = px - 1; fx <= px + 1; ++fx) {
-        if (fx < 0 || fx > nxf) continue;
-        const double w = wy * interp1_weight(fx, nxf, nxc, I);
-        if (w == 0.0) continue;
-        const size_t g = (size_t)fy * (nxf + 1) + fx;
 #pragma unroll
-        for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
-      }
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const size_t g = (size_t)fy[b] * (nxf + 1) + fx[a];
+      const double w = wy[b] * wx[a];
+#pragma unroll
+      for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
     }
-  }
+  const bool fx0 = node_fixed(fixed, I, J, nxc, nyc);
 #pragma unroll
-  for (int q = 0; q < BS; ++q) bc[BS * i + q] = (T)acc[q];
+  for (int q = 0; q < BS; ++q) bc[BS * (size_t)i + q] = fx0 ? T(0) : (T)acc[q];
 }
 
 template <typename T, int BS>
 __global__ void k_prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                                    const T* __restrict__ ec, T* __restrict__ xf) {
-  const size_t nn = (size_t)(nxf + 1) * (nyf + 1);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nn = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn) return;
-  const int fx = (int)(i % (nxf + 1)), fy = (int)(i / (nxf + 1));
+  const int fx = (int)(i % (unsigned)(nxf + 1)), fy = (int)(i / (unsigned)(nxf + 1));
   if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
-  const Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
+  Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
+  if (ix.n == 1) { ix.c[1] = ix.c[0]; ix.w[1] = 0.0; }
+  if (iy.n == 1) { iy.c[1] = iy.c[0]; iy.w[1] = 0.0; }
   T add[BS];
 #pragma unroll
   for (int q = 0; q < BS; ++q) add[q] = T(0);
-  for (int b = 0; b < iy.n; ++b)
-    for (int a = 0; a < ix.n; ++a) {
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
       const T w = (T)(ix.w[a] * iy.w[b]);
       const size_t c = (size_t)iy.c[b] * (nxc + 1) + ix.c[a];
 #pragma unroll
       for (int q = 0; q < BS; ++q) add[q] += w * ec[BS * c + q];
     }
 #pragma unroll
-  for (int q = 0; q < BS; ++q) xf[BS * i + q] += add[q];
+  for (int q = 0; q < BS; ++q) xf[BS * (size_t)i + q] += add[q];
 }
 
 template <typename T, int BS>
 hipError_t restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* rf,
                              T* bc, hipStream_t s) {
+  if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
   k_restrict_bilinear<T, BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1)), kBlock, 0, s>>>(
       nxf, nyf, nxc, nyc, fixed, rf, bc);
   return hipGetLastError();
@@ -1092,6 +1110,7 @@ hipError_t restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
 template <typename T, int BS>
 hipError_t prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* ec,
                             T* xf, hipStream_t s) {
+  if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
   k_prolong_bilinear<T, BS><<<grid1d((size_t)(nxf + 1) * (nyf + 1)), kBlock, 0, s>>>(
       nxf, nyf, nxc, nyc, fixed, ec, xf);
   return hipGetLastError();
@@ -1791,7 +1810,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
     size_t n, const double* __restrict__ sc, PcgSlots q, const double* __restrict__ pqpart, int npq,
     T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ qv,
-    double* __restrict__ part) {
+    double* __restrict__ part, float* __restrict__ rlow) {
   __shared__ double alpha_sh;
   const double pq = sum_partials(pqpart, npq);
   if (threadIdx.x == 0) {
@@ -1805,6 +1824,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
     x[i] += al * p[i];
     const T ri = r[i] - al * qv[i];
     r[i] = ri;
+    if (rlow) rlow[i] = (float)ri;
     s += (double)ri * (double)ri;
   }
   const double v[1] = {s};
@@ -1814,19 +1834,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
 
 template <typename T>
 hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, const RedSlot& pq,
-                               T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s) {
+                               T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s,
+                               float* rlow) {
   const unsigned g = vec_grid(n);
   rr.nblocks = (int)g;
   k_pcg_update_xr_fused<T><<<g, kBlock, 0, s>>>(n, sc, q, pq.partials, pq.nblocks, x, r, p, qv,
-                                               rr.partials);
+                                               rr.partials, rlow);
   return hipGetLastError();
 }
 template hipError_t pcg_update_xr_fused<double>(size_t, const double*, const PcgSlots&,
                                                 const RedSlot&, double*, double*, const double*,
-                                                const double*, RedSlot&, hipStream_t);
+                                                const double*, RedSlot&, hipStream_t, float*);
 template hipError_t pcg_update_xr_fused<float>(size_t, const double*, const PcgSlots&,
                                                const RedSlot&, float*, float*, const float*,
-                                               const float*, RedSlot&, hipStream_t);
+                                               const float*, RedSlot&, hipStream_t, float*);
 
 __global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
                                                          const double* __restrict__ rrp, int nrr,

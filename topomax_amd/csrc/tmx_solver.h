@@ -110,10 +110,14 @@ struct Level {
   DevBuf<float> b32, x32, t32, r32;
 };
 
+// solve precision: FP64 PCG with an FP64 V-cycle, FP64 PCG with an FP32 V-cycle, or FP32 PCG
+// inside FP64 iterative refinement
+enum SolvePrecision { kSolveFP64 = 0, kSolveMixed = 1, kSolveMixedIR = 2 };
+
 struct SolveParams {
   double rtol = 1e-10;
   int max_iter = 500;
-  int precision = 0;
+  int precision = kSolveFP64;
   double inner_rtol = 1e-5;
   int max_outer = 30;
 };
@@ -149,8 +153,9 @@ class Hierarchy {
   int apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta, RedSlot* dot,
                    hipStream_t s);
 
-  // PCG from x0 = 0 on A x = b; result left in the hierarchy's Krylov x buffer (x_buf<T>())
-  template <typename T>
+  // PCG from x0 = 0 on A x = b with Krylov vectors in T and the V-cycle preconditioner in PC;
+  // result left in the hierarchy's Krylov x buffer (x_buf<T>())
+  template <typename T, typename PC = T>
   int pcg(Workspace& ws, const T* b, double rtol, int max_iter, double bnorm2, int* iters,
           double* rel, hipStream_t s);
   template <typename T>
@@ -192,8 +197,9 @@ class Hierarchy {
   DevBuf<double> x64_, r64_, z64_, p64_, q64_, pb64_;
   DevBuf<float> r32_, z32_, p32_, q32_, d32_, pb32_;
   DevBuf<double> rr64_;         // refinement residual
-  // captured PCG graphs: [precision][0 = init, 1 = even iteration, 2 = odd iteration]
-  hipGraphExec_t graphs_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  // captured PCG graphs: [fp64 / fp32 / fp64 with fp32 V-cycle][0 = init, 1 = even
+  // iteration, 2 = odd iteration]
+  hipGraphExec_t graphs_[3][3] = {};
   hipEvent_t ev_[2] = {nullptr, nullptr};
   void drop_graphs();
   ~Hierarchy();

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace tmx {
 
@@ -511,14 +512,30 @@ T* Hierarchy::x_buf() {
   return KrylovBufs<T>::x(*this);
 }
 
+// z = M r for Krylov vectors in T and the V-cycle in PC (partials of r.z to rz).  With an FP32
+// V-cycle under FP64 Krylov vectors, r arrives already rounded in the float buffer r32_
+// (written by the update kernel, or converted here at the start) and z is widened after.
+template <typename T, typename PC>
+static int enqueue_precondition(Hierarchy& h, RedSlot* rz, bool convert_r, hipStream_t s) {
+  if constexpr (std::is_same<T, PC>::value) {
+    TMX_TRY(h.vcycle<T>(0, KrylovBufs<T>::r(h), KrylovBufs<T>::z(h), s, rz));
+  } else {
+    static_assert(std::is_same<T, double>::value && std::is_same<PC, float>::value,
+                  "mixed PCG is FP64 Krylov with an FP32 V-cycle");
+    const size_t nv = h.n();
+    if (convert_r) TMX_HIP((convert<double, float>(nv, h.r64_.get(), h.r32_.get(), s)));
+    TMX_TRY(h.vcycle<float>(0, h.r32_.get(), h.z32_.get(), s, rz));
+    TMX_HIP((convert<float, double>(nv, h.z32_.get(), h.z64_.get(), s)));
+  }
+  return 0;
+}
+
 // PCG start, given r = b already in place: z = M r, rz = r.z, beta = 0, p_old = 0
-template <typename T>
+template <typename T, typename PC>
 static int enqueue_pcg_init(Hierarchy& h, Workspace& ws, hipStream_t s) {
   const size_t nv = h.n();
-  T* r = KrylovBufs<T>::r(h);
-  T* z = KrylovBufs<T>::z(h);
   RedSlot rz = ws.region(2);
-  TMX_TRY(h.vcycle<T>(0, r, z, s, &rz));
+  TMX_TRY((enqueue_precondition<T, PC>(h, &rz, true, s)));
   TMX_HIP(fill_zero_bytes(KrylovBufs<T>::p(h), sizeof(T) * nv, s));
   TMX_HIP(pcg_start(rz, ws.sc(), kPcg, s));
   return 0;
@@ -528,7 +545,7 @@ static int enqueue_pcg_init(Hierarchy& h, Workspace& ws, hipStream_t s) {
 //   p = z + beta p_old and q = A p (partials p.q); alpha; x += alpha p, r -= alpha q
 //   (partials r.r); z = M r (partials r.z from the last sweep); rr, beta, stop flags.
 // parity selects which of the two direction buffers holds p_old.
-template <typename T>
+template <typename T, typename PC>
 static int enqueue_pcg_iteration(Hierarchy& h, Workspace& ws, int parity, hipStream_t s) {
   const size_t nv = h.n();
   T* x = KrylovBufs<T>::x(h);
@@ -540,18 +557,20 @@ static int enqueue_pcg_iteration(Hierarchy& h, Workspace& ws, int parity, hipStr
   double* sc = ws.sc();
   RedSlot pq = ws.region(0), rr = ws.region(1), rz = ws.region(2);
   TMX_TRY(h.apply_update<T>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s));
-  TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, x, r, p_new, q, rr, s));
-  TMX_TRY(h.vcycle<T>(0, r, z, s, &rz));
+  constexpr bool split = !std::is_same<T, PC>::value;
+  TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, x, r, p_new, q, rr, s,
+                                 split ? h.r32_.get() : nullptr));
+  TMX_TRY((enqueue_precondition<T, PC>(h, &rz, false, s)));
   TMX_HIP(pcg_finalize(pq, rr, rz, sc, kPcg, s));
   return 0;
 }
 
-template <typename T>
+template <typename T, typename PC>
 static int capture(Hierarchy& h, Workspace& ws, int which, hipGraphExec_t* out, hipStream_t s) {
   hipGraph_t g = nullptr;
   TMX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  const int rc = which == 0 ? enqueue_pcg_init<T>(h, ws, s)
-                            : enqueue_pcg_iteration<T>(h, ws, which - 1, s);
+  const int rc = which == 0 ? enqueue_pcg_init<T, PC>(h, ws, s)
+                            : enqueue_pcg_iteration<T, PC>(h, ws, which - 1, s);
   const hipError_t e = hipStreamEndCapture(s, &g);
   if (rc != 0) {
     if (g) (void)hipGraphDestroy(g);
@@ -564,13 +583,13 @@ static int capture(Hierarchy& h, Workspace& ws, int which, hipGraphExec_t* out, 
   return 0;
 }
 
-template <typename T>
+template <typename T, typename PC>
 int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double bnorm2,
                    int* iters, double* rel, hipStream_t s) {
   const size_t nv = n();
-  const int prec = sizeof(T) == 8 ? 0 : 1;
+  const int prec = !std::is_same<T, PC>::value ? 2 : sizeof(T) == 8 ? 0 : 1;
   for (int w = 0; w < 3; ++w)
-    if (!graphs_[prec][w]) TMX_TRY(capture<T>(*this, ws, w, &graphs_[prec][w], s));
+    if (!graphs_[prec][w]) TMX_TRY((capture<T, PC>(*this, ws, w, &graphs_[prec][w], s)));
   double* sc = ws.sc();
   *iters = 0;
   *rel = 1.0;
@@ -603,7 +622,7 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
   *rel = std::sqrt(std::max(rrv, 0.0) / bnorm2);
   if (g_trace)
     std::fprintf(stderr, "[tmx] pcg<%s> %s n=%zu: %d its, rel %.3e, flag %g\n",
-                 sizeof(T) == 8 ? "f64" : "f32", phys_ == kElasticity ? "elast" : "filter", nv,
+                 prec == 2 ? "f64/f32" : prec == 0 ? "f64" : "f32", phys_ == kElasticity ? "elast" : "filter", nv,
                  *iters, *rel, flag);
   if (flag == 1.0) return 0;
   if (flag == 2.0) {
@@ -645,10 +664,12 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     return 0;
   }
   int rc = 0;
-  if (prm.precision == 0) {
+  if (prm.precision == kSolveFP64 || prm.precision == kSolveMixed) {
     int it = 0;
     double rel = 0;
-    rc = pcg<double>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s);
+    rc = prm.precision == kSolveFP64
+             ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s)
+             : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s);
     S.iterations = it;
     S.outer = 1;
     if (rc == 0 || rc == -3) {  // report the true residual

@@ -26,6 +26,11 @@ from . import _lib
 from .designs import DomainParameters, ElasticityParameters
 
 
+# solve schemes: FP64 multigrid-PCG; FP64 PCG with an FP32 V-cycle preconditioner; FP32
+# multigrid-PCG inner solves inside FP64 iterative refinement
+PRECISIONS = {"fp64": _lib.TMX_FP64, "mixed": _lib.TMX_MIXED, "mixed-ir": _lib.TMX_MIXED_IR}
+
+
 def _device(device=None) -> torch.device:
     if device is None:
         device = torch.device("cuda", torch.cuda.current_device())
@@ -168,7 +173,7 @@ def _make_desc(mesh: StructuredMesh, params: ElasticityParameters | None, filter
         d.body_force.center[0], d.body_force.center[1] = bf.region.center
         d.body_force.radius = bf.region.radius
         d.body_force.value[0], d.body_force.value[1] = bf.value
-    d.precision = {"fp64": _lib.TMX_FP64, "mixed": _lib.TMX_MIXED}[precision]
+    d.precision = PRECISIONS[precision]
     return d
 
 
@@ -184,7 +189,7 @@ def solve_opts(rtol=None, max_iter=None, precision=None, inner_rtol=None,
     o = _lib.SolveOpts()
     o.rtol = rtol or 0.0
     o.max_iter = max_iter or 0
-    o.precision = {None: -1, "fp64": _lib.TMX_FP64, "mixed": _lib.TMX_MIXED}[precision]
+    o.precision = -1 if precision is None else PRECISIONS[precision]
     o.inner_rtol = inner_rtol or 0.0
     o.max_outer = max_outer or 0
     return o
