@@ -152,15 +152,20 @@ hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned 
 template <int BS>
 hipError_t stencil_to_dense(int nx, int ny, const double* st, double* dense, hipStream_t s);
 
-// Bottom of a V-cycle in one workgroup: 9-point levels lev[0..nlev-2] (bilinear transfers)
-// and the dense coarsest lev[nlev-1] (x = Ainv b).  lev[0].b is the input, lev[0].x the output.
+// Bottom of a V-cycle in one workgroup of kTailThreads: 9-point levels lev[0..nlev-2]
+// (bilinear transfers, at most kTailThreads nodes each) and the dense coarsest lev[nlev-1]
+// (x = Ainv b).  Every level's vectors live in LDS (lev[l].off, in elements of T: iterate,
+// ping-pong and right-hand side, BS*nn each; the dense level holds b and x); b_in (global)
+// is the top level's right-hand side and x_out (global) receives its result.
 constexpr int kMaxTail = 8;
-constexpr int kTailNodes = 1024;
+constexpr int kTailThreads = 1024;
+constexpr int kTailNodes = kTailThreads;
+constexpr size_t kTailLdsBytes = 64 * 1024;
 template <typename T>
 struct TailLevel {
   int nx, ny;
-  const T* st;
-  T *x, *t, *b, *r;
+  const T* st;  // 9-point stencil (global, SoA as for stencil_op)
+  int off;      // LDS offset of this level's vectors, in elements of T
 };
 template <typename T>
 struct TailDesc {
@@ -168,10 +173,14 @@ struct TailDesc {
   unsigned fixed;
   T omega;
   const double* dense;
+  const T* b_in;
+  T* x_out;
   TailLevel<T> lev[kMaxTail];
 };
+// LDS bytes a tail over these levels needs (9-point levels nn[0..nlev-2], dense level nn[nlev-1])
+size_t tail_lds_bytes(const size_t* nn, int nlev, int bs, size_t elem);
 template <typename T, int BS>
-hipError_t tail_vcycle(const TailDesc<T>& d, hipStream_t s);
+hipError_t tail_vcycle(const TailDesc<T>& d, size_t lds_bytes, hipStream_t s);
 hipError_t dense_invert(int n, double* a, double* work, hipStream_t s);
 template <typename T>
 hipError_t dense_apply(int n, const double* ainv, const T* b, T* x, hipStream_t s);

@@ -1023,53 +1023,24 @@ __device__ __forceinline__ int cpos(int I, int nf, int nc) { return nf == nc ? I
 
 __device__ __forceinline__ double interp1_weight(int f, int nf, int nc, int I) {
   const Interp1 r = interp1(f, nf, nc);
-  double w = 0.0;
-  for (int t = 0; t < r.n; ++t)
-    if (r.c[t] == I) w += r.w[t];
-  return w;
+  return (r.c[0] == I ? r.w[0] : 0.0) + (r.n > 1 && r.c[1] == I ? r.w[1] : 0.0);
 }
 
-// Weights and clamped fine indices of the three fine positions around coarse node I along one
-// axis (zero weight where a position is outside the grid or outside the coarse hat function).
-__device__ __forceinline__ void restrict_axis(int I, int nf, int nc, int f[3], double w[3]) {
-  const int p = cpos(I, nf, nc);
-#pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int g = p - 1 + t;
-    const bool in = g >= 0 && g <= nf;
-    w[t] = in ? interp1_weight(g, nf, nc, I) : 0.0;
-    f[t] = min(max(g, 0), nf);
-  }
-}
+template <typename T, int BS>
+__device__ __forceinline__ void restrict_bilinear_node(int nxf, int nyf, int nxc, int nyc,
+                                                       unsigned fixed, const T* __restrict__ rf,
+                                                       T* __restrict__ bc, size_t i);
+template <typename T, int BS>
+__device__ __forceinline__ void prolong_bilinear_node(int nxf, int nyf, int nxc, int nyc,
+                                                      unsigned fixed, const T* __restrict__ ec,
+                                                      T* __restrict__ xf, size_t i);
 
-// Branch-free: all 9 (x BS) loads are issued before any is used, so a launch costs one memory
-// round trip rather than one per contributing fine node.
 template <typename T, int BS>
 __global__ void k_restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                                     const T* __restrict__ rf, T* __restrict__ bc) {
   const unsigned nn = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  const int I = (int)(i % (unsigned)(nxc + 1)), J = (int)(i / (unsigned)(nxc + 1));
-  int fx[3], fy[3];
-  double wx[3], wy[3];
-  restrict_axis(I, nxf, nxc, fx, wx);
-  restrict_axis(J, nyf, nyc, fy, wy);
-  double acc[BS];
-#pragma unroll
-  for (int q = 0; q < BS; ++q) acc[q] = 0.0;
-#pragma unroll
-  for (int b = 0; b < 3; ++b)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const size_t g = (size_t)fy[b] * (nxf + 1) + fx[a];
-      const double w = wy[b] * wx[a];
-#pragma unroll
-      for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
-    }
-  const bool fx0 = node_fixed(fixed, I, J, nxc, nyc);
-#pragma unroll
-  for (int q = 0; q < BS; ++q) bc[BS * (size_t)i + q] = fx0 ? T(0) : (T)acc[q];
+  if (i < nn) restrict_bilinear_node<T, BS>(nxf, nyf, nxc, nyc, fixed, rf, bc, i);
 }
 
 template <typename T, int BS>
@@ -1077,26 +1048,7 @@ __global__ void k_prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned 
                                    const T* __restrict__ ec, T* __restrict__ xf) {
   const unsigned nn = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  const int fx = (int)(i % (unsigned)(nxf + 1)), fy = (int)(i / (unsigned)(nxf + 1));
-  if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
-  Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
-  if (ix.n == 1) { ix.c[1] = ix.c[0]; ix.w[1] = 0.0; }
-  if (iy.n == 1) { iy.c[1] = iy.c[0]; iy.w[1] = 0.0; }
-  T add[BS];
-#pragma unroll
-  for (int q = 0; q < BS; ++q) add[q] = T(0);
-#pragma unroll
-  for (int b = 0; b < 2; ++b)
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const T w = (T)(ix.w[a] * iy.w[b]);
-      const size_t c = (size_t)iy.c[b] * (nxc + 1) + ix.c[a];
-#pragma unroll
-      for (int q = 0; q < BS; ++q) add[q] += w * ec[BS * c + q];
-    }
-#pragma unroll
-  for (int q = 0; q < BS; ++q) xf[BS * (size_t)i + q] += add[q];
+  if (i < nn) prolong_bilinear_node<T, BS>(nxf, nyf, nxc, nyc, fixed, ec, xf, i);
 }
 
 template <typename T, int BS>
@@ -1211,52 +1163,54 @@ template hipError_t stencil_op<float, 1>(int, int, const float*, int, const floa
 template hipError_t stencil_op<float, 2>(int, int, const float*, int, const float*, const float*,
                                          float*, float, hipStream_t);
 
-// per-node bilinear transfers (bodies of k_restrict_bilinear / k_prolong_bilinear)
+// per-node bilinear transfers (bodies of k_restrict_bilinear / k_prolong_bilinear and of the
+// tail kernel).  Branch-free: every load is issued before any is used (out-of-range and
+// non-contributing positions get zero weight at a clamped address), so a launch costs one
+// memory round trip rather than one per contributing node.
 template <typename T, int BS>
 __device__ __forceinline__ void restrict_bilinear_node(int nxf, int nyf, int nxc, int nyc,
                                                        unsigned fixed, const T* __restrict__ rf,
                                                        T* __restrict__ bc, size_t i) {
-  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  const int I = (int)((unsigned)i % (unsigned)(nxc + 1)), J = (int)((unsigned)i / (unsigned)(nxc + 1));
   double acc[BS];
 #pragma unroll
   for (int q = 0; q < BS; ++q) acc[q] = 0.0;
-  if (!node_fixed(fixed, I, J, nxc, nyc)) {
-    const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
+  const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int fy = py + dy;
-      const double wy = (fy < 0 || fy > nyf) ? 0.0 : interp1_weight(fy, nyf, nyc, J);
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int fy = py + dy;
+    const double wy = (fy < 0 || fy > nyf) ? 0.0 : interp1_weight(fy, nyf, nyc, J);
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int fx = px + dx;
-        const double w = (fx < 0 || fx > nxf) ? 0.0 : wy * interp1_weight(fx, nxf, nxc, I);
-        const size_t g = (size_t)min(max(fy, 0), nyf) * (nxf + 1) + min(max(fx, 0), nxf);
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int fx = px + dx;
+      const double w = (fx < 0 || fx > nxf) ? 0.0 : wy * interp1_weight(fx, nxf, nxc, I);
+      const size_t g = (size_t)min(max(fy, 0), nyf) * (nxf + 1) + min(max(fx, 0), nxf);
 #pragma unroll
-        for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
-      }
+      for (int q = 0; q < BS; ++q) acc[q] += w * (double)rf[BS * g + q];
     }
   }
+  const bool fx0 = node_fixed(fixed, I, J, nxc, nyc);
 #pragma unroll
-  for (int q = 0; q < BS; ++q) bc[BS * i + q] = (T)acc[q];
+  for (int q = 0; q < BS; ++q) bc[BS * i + q] = fx0 ? T(0) : (T)acc[q];
 }
 
 template <typename T, int BS>
 __device__ __forceinline__ void prolong_bilinear_node(int nxf, int nyf, int nxc, int nyc,
                                                       unsigned fixed, const T* __restrict__ ec,
                                                       T* __restrict__ xf, size_t i) {
-  const int fx = (int)(i % (nxf + 1)), fy = (int)(i / (nxf + 1));
+  const int fx = (int)((unsigned)i % (unsigned)(nxf + 1)), fy = (int)((unsigned)i / (unsigned)(nxf + 1));
   if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
   const Interp1 ix = interp1(fx, nxf, nxc), iy = interp1(fy, nyf, nyc);
+  const int cx0 = ix.c[0], cx1 = ix.n > 1 ? ix.c[1] : ix.c[0];
+  const int cy0 = iy.c[0], cy1 = iy.n > 1 ? iy.c[1] : iy.c[0];
+  const double wx0 = ix.w[0], wx1 = ix.n > 1 ? ix.w[1] : 0.0;
+  const double wy0 = iy.w[0], wy1 = iy.n > 1 ? iy.w[1] : 0.0;
+  const size_t r0 = (size_t)cy0 * (nxc + 1), r1 = (size_t)cy1 * (nxc + 1);
   T add[BS];
 #pragma unroll
-  for (int q = 0; q < BS; ++q) add[q] = T(0);
-  for (int b = 0; b < iy.n; ++b)
-    for (int a = 0; a < ix.n; ++a) {
-      const T w = (T)(ix.w[a] * iy.w[b]);
-      const size_t c = (size_t)iy.c[b] * (nxc + 1) + ix.c[a];
-#pragma unroll
-      for (int q = 0; q < BS; ++q) add[q] += w * ec[BS * c + q];
-    }
+  for (int q = 0; q < BS; ++q)
+    add[q] = (T)(wx0 * wy0) * ec[BS * (r0 + cx0) + q] + (T)(wx1 * wy0) * ec[BS * (r0 + cx1) + q] +
+             (T)(wx0 * wy1) * ec[BS * (r1 + cx0) + q] + (T)(wx1 * wy1) * ec[BS * (r1 + cx1) + q];
 #pragma unroll
   for (int q = 0; q < BS; ++q) xf[BS * i + q] += add[q];
 }
@@ -1415,82 +1369,194 @@ TMX_INST_S9F(float, 1)
 TMX_INST_S9F(float, 2)
 
 // ---------------------------------------------------------------- tail V-cycle
-// The bottom of the hierarchy (9-point levels small enough for one workgroup, then the
-// dense coarsest solve) runs as a single workgroup that steps through the whole V-cycle,
-// with a barrier between steps: one launch instead of ~7 per level.
-template <typename T, int BS, int OP>
-__device__ __forceinline__ void tail_sweep(const TailLevel<T>& L, const T* x, const T* b, T* y,
-                                           T omega) {
-  const size_t nn = (size_t)(L.nx + 1) * (L.ny + 1);
-  for (size_t v = threadIdx.x; v < nn; v += blockDim.x)
-    s9_node<T, BS, OP>(L.nx, L.ny, L.st, x, b, y, omega, v);
-  __syncthreads();
+// The bottom of the hierarchy (9-point levels of at most kTailThreads nodes, then the dense
+// coarsest solve) runs as a single workgroup that steps through the whole V-cycle with a
+// barrier between steps: one launch instead of ~7 per level.  Vectors stay in LDS; each
+// thread owns one node and holds that node's stencil row in registers while its level is
+// being smoothed, so a sweep costs LDS traffic only.
+
+size_t tail_lds_bytes(const size_t* nn, int nlev, int bs, size_t elem) {
+  size_t n = 0;
+  for (int l = 0; l + 1 < nlev; ++l) n += 3 * (size_t)bs * nn[l];
+  n += 2 * (size_t)bs * nn[nlev - 1];
+  return n * elem;
 }
 
 template <typename T, int BS>
-__global__ __launch_bounds__(1024) void k_tail_vcycle(TailDesc<T> d) {
+struct TailRow {
+  T a[9 * BS * BS];  // a[k*BS*BS + e], k = (dy+1)*3 + (dx+1)
+};
+
+template <typename T, int BS>
+__device__ __forceinline__ void tail_load_row(const TailLevel<T>& L, int v, TailRow<T, BS>& r) {
+  const int nn = (L.nx + 1) * (L.ny + 1);
+  if (v < nn) {
+#pragma unroll
+    for (int e = 0; e < 9 * BS * BS; ++e) r.a[e] = L.st[(size_t)e * nn + v];
+  }
+}
+
+// one 9-point operation at node v of a level held in LDS (x, b, y LDS pointers)
+template <typename T, int BS, int OP>
+__device__ __forceinline__ void tail_node(const TailLevel<T>& L, const TailRow<T, BS>& r,
+                                          const T* x, const T* b, T* y, T omega, int v) {
+  const int nx = L.nx, ny = L.ny;
+  if (v >= (nx + 1) * (ny + 1)) return;
+  const int ix = v % (nx + 1), iy = v / (nx + 1);
+  T acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = T(0);
+  if (OP != kSmoothZero) {
+#pragma unroll
+    for (int kk = 0; kk < 9; ++kk) {
+      const int jx = min(max(ix + kk % 3 - 1, 0), nx), jy = min(max(iy + kk / 3 - 1, 0), ny);
+      const int j = jy * (nx + 1) + jx;
+#pragma unroll
+      for (int a = 0; a < BS; ++a)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) acc[a] += r.a[kk * BS * BS + a * BS + c] * x[BS * j + c];
+    }
+  }
+  T res[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) res[q] = b[BS * v + q] - acc[q];
+  if (OP == kResidual) {
+#pragma unroll
+    for (int q = 0; q < BS; ++q) y[BS * v + q] = res[q];
+    return;
+  }
+  T d[BS];
+  if constexpr (BS == 1) {
+    d[0] = res[0] / r.a[4];
+  } else {
+    const T a00 = r.a[16], a01 = r.a[17], a10 = r.a[18], a11 = r.a[19];
+    const T inv = T(1) / (a00 * a11 - a01 * a10);
+    d[0] = (a11 * res[0] - a01 * res[1]) * inv;
+    d[1] = (a00 * res[1] - a10 * res[0]) * inv;
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) {
+    const T xo = (OP == kSmoothZero) ? T(0) : x[BS * v + q];
+    y[BS * v + q] = xo + omega * d[q];
+  }
+}
+
+// A node's stencil row stays in registers across a level's sweeps when it is small enough
+// (36 floats, 9 doubles); the FP64 2x2-block row (36 doubles) is re-read from global memory
+// (an L2 hit) for each sweep instead, which keeps the kernel within 128 VGPRs without spills.
+template <typename T, int BS>
+struct TailRowHolder {
+  static constexpr bool kKeep = sizeof(T) * 9 * BS * BS <= 144;
+  TailRow<T, BS> row;
+  __device__ __forceinline__ void level(const TailLevel<T>& L, int v) {
+    if constexpr (kKeep) tail_load_row<T, BS>(L, v, row);
+  }
+  template <int OP>
+  __device__ __forceinline__ void sweep(const TailLevel<T>& L, const T* x, const T* b, T* y,
+                                        T omega, int v) {
+    if constexpr (!kKeep) tail_load_row<T, BS>(L, v, row);
+    tail_node<T, BS, OP>(L, row, x, b, y, omega, v);
+  }
+};
+
+template <typename T, int BS>
+__global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
+  extern __shared__ __align__(16) unsigned char tail_lds[];
+  T* lds = reinterpret_cast<T*>(tail_lds);
+  const int tid = threadIdx.x;
   const int nl = d.nlev;  // levels 0..nl-2 are 9-point, level nl-1 is the dense coarsest
-  const int S = d.nu_pre + d.nu_post;
-  int wbuf[kMaxTail];
+  const T om = d.omega;
+  // ping-pong parity: the first sweep writes buffer (S-1)%2 so that after S sweeps the
+  // iterate is in buffer 0; after the pre-sweeps it is in buffer nu_post%2 on every level
+  const int w_first = (d.nu_pre + d.nu_post - 1) & 1;
+  const int w_mid = d.nu_post & 1;
+  {  // top-level right-hand side into LDS
+    const TailLevel<T>& L = d.lev[0];
+    const int n = BS * (L.nx + 1) * (L.ny + 1);
+    T* b = lds + L.off + 2 * n;
+    for (int i = tid; i < n; i += kTailThreads) b[i] = d.b_in[i];
+  }
+  __syncthreads();
   for (int l = 0; l + 1 < nl; ++l) {
     const TailLevel<T>& L = d.lev[l];
-    T* buf[2] = {L.x, L.t};
-    int w = (S - 1) % 2;
-    tail_sweep<T, BS, kSmoothZero>(L, nullptr, L.b, buf[w], d.omega);
+    const int nn = (L.nx + 1) * (L.ny + 1);
+    T* const b0 = lds + L.off;
+    T* const b1 = b0 + BS * nn;
+    const T* b = b0 + 2 * BS * nn;
+    TailRowHolder<T, BS> rh;
+    rh.level(L, tid);
+    int w = w_first;
+    rh.template sweep<kSmoothZero>(L, nullptr, b, w ? b1 : b0, om, tid);
+    __syncthreads();
     for (int k = 1; k < d.nu_pre; ++k) {
-      tail_sweep<T, BS, kSmooth>(L, buf[w], L.b, buf[w ^ 1], d.omega);
+      rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, om, tid);
+      __syncthreads();
       w ^= 1;
     }
-    wbuf[l] = w;
-    tail_sweep<T, BS, kResidual>(L, buf[w], L.b, L.r, d.omega);
+    T* const spare = w ? b0 : b1;  // the residual goes to the free ping-pong buffer
+    rh.template sweep<kResidual>(L, w ? b1 : b0, b, spare, om, tid);
+    __syncthreads();
     const TailLevel<T>& C = d.lev[l + 1];
-    const size_t nnc = (size_t)(C.nx + 1) * (C.ny + 1);
-    for (size_t i = threadIdx.x; i < nnc; i += blockDim.x)
-      restrict_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, L.r, C.b, i);
+    const int nnc = (C.nx + 1) * (C.ny + 1);
+    T* cb = lds + C.off + (l + 2 == nl ? 0 : 2 * BS * nnc);
+    if (tid < nnc) restrict_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, spare, cb, tid);
     __syncthreads();
   }
-  {  // dense coarsest: x = Ainv b, 8 lanes per row
+  {  // dense coarsest: x = Ainv b, 8 lanes per row (cn <= 128 rows)
     const TailLevel<T>& C = d.lev[nl - 1];
     const int n = d.cn;
-    for (int base = 0; base < n * 8; base += blockDim.x) {
-      const int t = base + threadIdx.x;
-      const int row = t >> 3, part = t & 7;
-      double s = 0.0;
-      if (row < n)
-        for (int j = part; j < n; j += 8) s += d.dense[(size_t)row * n + j] * (double)C.b[j];
-      s += __shfl_down(s, 4, 8);
-      s += __shfl_down(s, 2, 8);
-      s += __shfl_down(s, 1, 8);
-      if (row < n && part == 0) C.x[row] = (T)s;
+    const T* cb = lds + C.off;
+    T* cx = lds + C.off + n;
+    const int row = tid >> 3, part = tid & 7;
+    double s = 0.0;
+    if (row < n) {
+      const double* arow = d.dense + (size_t)row * n;
+      for (int j = part; j < n; j += 8) s += arow[j] * (double)cb[j];
     }
+    s += __shfl_down(s, 4, 8);
+    s += __shfl_down(s, 2, 8);
+    s += __shfl_down(s, 1, 8);
+    if (row < n && part == 0) cx[row] = (T)s;
     __syncthreads();
   }
   for (int l = nl - 2; l >= 0; --l) {
     const TailLevel<T>& L = d.lev[l];
     const TailLevel<T>& C = d.lev[l + 1];
-    T* buf[2] = {L.x, L.t};
-    int w = wbuf[l];
-    const size_t nn = (size_t)(L.nx + 1) * (L.ny + 1);
-    for (size_t i = threadIdx.x; i < nn; i += blockDim.x)
-      prolong_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, C.x, buf[w], i);
+    const int nn = (L.nx + 1) * (L.ny + 1);
+    T* const b0 = lds + L.off;
+    T* const b1 = b0 + BS * nn;
+    const T* b = b0 + 2 * BS * nn;
+    const T* cx = lds + C.off + (l + 2 == nl ? d.cn : 0);
+    TailRowHolder<T, BS> rh;
+    rh.level(L, tid);  // issued before the prolongation to overlap its latency
+    int w = w_mid;
+    if (tid < nn) prolong_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, cx, w ? b1 : b0, tid);
     __syncthreads();
     for (int k = 0; k < d.nu_post; ++k) {
-      tail_sweep<T, BS, kSmooth>(L, buf[w], L.b, buf[w ^ 1], d.omega);
+      rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, om, tid);
+      __syncthreads();
       w ^= 1;
+    }
+    if (l == 0) {
+      const int n = BS * nn;
+      for (int i = tid; i < n; i += kTailThreads) d.x_out[i] = b0[i];
     }
   }
 }
 
 template <typename T, int BS>
-hipError_t tail_vcycle(const TailDesc<T>& d, hipStream_t s) {
-  if (d.nlev < 1 || d.nlev > kMaxTail) return hipErrorInvalidValue;
-  k_tail_vcycle<T, BS><<<1, 1024, 0, s>>>(d);
+hipError_t tail_vcycle(const TailDesc<T>& d, size_t lds_bytes, hipStream_t s) {
+  if (d.nlev < 2 || d.nlev > kMaxTail || d.cn > 128 || lds_bytes > kTailLdsBytes)
+    return hipErrorInvalidValue;
+  for (int l = 0; l + 1 < d.nlev; ++l)
+    if ((d.lev[l].nx + 1) * (d.lev[l].ny + 1) > kTailThreads) return hipErrorInvalidValue;
+  k_tail_vcycle<T, BS><<<1, kTailThreads, lds_bytes, s>>>(d);
   return hipGetLastError();
 }
-template hipError_t tail_vcycle<double, 1>(const TailDesc<double>&, hipStream_t);
-template hipError_t tail_vcycle<double, 2>(const TailDesc<double>&, hipStream_t);
-template hipError_t tail_vcycle<float, 1>(const TailDesc<float>&, hipStream_t);
-template hipError_t tail_vcycle<float, 2>(const TailDesc<float>&, hipStream_t);
+template hipError_t tail_vcycle<double, 1>(const TailDesc<double>&, size_t, hipStream_t);
+template hipError_t tail_vcycle<double, 2>(const TailDesc<double>&, size_t, hipStream_t);
+template hipError_t tail_vcycle<float, 1>(const TailDesc<float>&, size_t, hipStream_t);
+template hipError_t tail_vcycle<float, 2>(const TailDesc<float>&, size_t, hipStream_t);
 
 // Galerkin product, one thread per coarse node.  Interpolation from coarse to fine node g:
 // nested P1 (x-, y- and diagonal midpoints) or tensor-product 1-D linear (bilinear).

@@ -96,15 +96,21 @@ int Hierarchy::init(Phys phys, int nx, int ny, double hx, double hy, unsigned fi
   return 0;
 }
 
-// tail: the longest suffix of small 9-point levels ending at the dense coarsest
+// tail: the longest suffix of small 9-point levels ending at the dense coarsest whose
+// vectors fit the tail kernel's LDS budget (sized for FP64, so both precisions fit)
 void Hierarchy::find_tail() {
   const int lastl = (int)levels_.size() - 1;
   tail_ = -1;
-  if (lastl >= 1) {
+  if (lastl >= 1 && bs_ * levels_[lastl].nn <= 128) {
+    const size_t cap = (size_t)std::min(tail_nodes_, kTailThreads);
     int t = lastl;
     while (t - 1 >= 1 && lastl - (t - 1) + 1 <= kMaxTail && levels_[t - 1].kind == kStencil &&
-           levels_[t - 1].nn <= (size_t)tail_nodes_ && !levels_[t].nested_from_prev)
+           levels_[t - 1].nn <= cap && !levels_[t].nested_from_prev) {
+      size_t nn[kMaxTail];
+      for (int l = t - 1; l <= lastl; ++l) nn[l - (t - 1)] = levels_[l].nn;
+      if (tail_lds_bytes(nn, lastl - (t - 1) + 1, bs_, sizeof(double)) > kTailLdsBytes) break;
       --t;
+    }
     tail_ = t;
   }
 }
@@ -385,7 +391,7 @@ static int prolong_to(Hierarchy& h, int l, const T* ec, T* xf, hipStream_t s) {
 }
 
 template <typename T>
-static int tail_launch(Hierarchy& h, hipStream_t s) {
+static int tail_launch(Hierarchy& h, const T* b, T* x, hipStream_t s) {
   const int last = (int)h.levels_.size() - 1;
   TailDesc<T> d;
   std::memset(&d, 0, sizeof(d));
@@ -396,19 +402,23 @@ static int tail_launch(Hierarchy& h, hipStream_t s) {
   d.fixed = h.fixed_;
   d.omega = (T)h.omega_;
   d.dense = h.dense_.get();
+  d.b_in = b;
+  d.x_out = x;
+  size_t nn[kMaxTail];
+  int off = 0;
   for (int l = h.tail_; l <= last; ++l) {
     Level& L = h.levels_[l];
     TailLevel<T>& t = d.lev[l - h.tail_];
     t.nx = L.nx;
     t.ny = L.ny;
     t.st = PrecBufs<T>::st(L);
-    t.x = PrecBufs<T>::x(L);
-    t.t = PrecBufs<T>::t(L);
-    t.b = PrecBufs<T>::b(L);
-    t.r = PrecBufs<T>::r(L);
+    t.off = off;
+    off += (l < last ? 3 : 2) * h.bs_ * (int)L.nn;
+    nn[l - h.tail_] = L.nn;
   }
-  if (h.bs_ == 2) TMX_HIP((tail_vcycle<T, 2>(d, s)));
-  else TMX_HIP((tail_vcycle<T, 1>(d, s)));
+  const size_t lds = tail_lds_bytes(nn, d.nlev, h.bs_, sizeof(T));
+  if (h.bs_ == 2) TMX_HIP((tail_vcycle<T, 2>(d, lds, s)));
+  else TMX_HIP((tail_vcycle<T, 1>(d, lds, s)));
   return 0;
 }
 
@@ -420,9 +430,7 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
     if (dot_bz) TMX_HIP(dot_t<T>((size_t)cn_, b, out, *dot_bz, s));
     return 0;
   }
-  if (use_tail_ && l == tail_ && l > 0) {  // b and out are this level's b / x buffers
-    return tail_launch<T>(*this, s);
-  }
+  if (use_tail_ && l == tail_ && l > 0) return tail_launch<T>(*this, b, out, s);
   Level& L = levels_[l];
   Level& C = levels_[l + 1];
   T* buf[2] = {out, PrecBufs<T>::t(L)};
