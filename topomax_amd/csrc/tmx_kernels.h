@@ -47,8 +47,9 @@ struct SweepShape {
 
 // kResidualRestrict: y = P^T (b - A x) on the nested coarse grid (nx, ny even);
 // kProlongSmooth: one Jacobi sweep applied to x + P e (e = nested coarse correction).
-// kApplyUpdate (PCG direction update fused with the operator): reads p_old = x and z = b,
-// forms p = z + beta p_old with beta = *fused.beta, writes p to fused.pnew and y = A p.
+// kApplyUpdate (PCG direction update fused with the operator): reads p_old = x and z = b
+// (or, when ex.zlow is set, the float z = ex.zlow), forms p = z + beta p_old with
+// beta = *ex.beta, writes p to ex.pnew and y = A p.
 // A dot slot receives partials of x.y (kApply / kApplyUpdate: p.Ap) or b.y (sweeps: r.z).
 enum SweepOp : int {
   kApply = 0, kResidual = 1, kSmooth = 2, kSmoothZero = 3, kResidualRestrict = 4,
@@ -60,6 +61,7 @@ struct SweepExtra {
   const T* ec = nullptr;         // kProlongSmooth: coarse correction
   T* pnew = nullptr;             // kApplyUpdate: updated direction
   const double* beta = nullptr;  // kApplyUpdate: device scalar
+  const float* zlow = nullptr;   // kApplyUpdate: z in float (FP32 preconditioner output)
 };
 
 template <typename T>
@@ -213,7 +215,7 @@ hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, co
                                float* rlow = nullptr);
 // end of a fused iteration: pq, rr, rz from partials; flags; beta = rz_new / rz; rz = rz_new
 hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
-                        const PcgSlots& q, hipStream_t s);
+                        const PcgSlots& q, hipStream_t s, double* host_out = nullptr);
 // start of a fused run: rz from partials, beta = 0, flag = 0, iters = 0
 hipError_t pcg_start(const RedSlot& rz, double* sc, const PcgSlots& q, hipStream_t s);
 // p = z + beta p, beta = sc[i_beta]

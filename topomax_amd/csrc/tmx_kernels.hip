@@ -146,6 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   const V* CW = reinterpret_cast<const V*>(a.cellw);
   const V* EC = reinterpret_cast<const V*>(ec);
   V* PN = reinterpret_cast<V*>(ex.pnew);
+  const float2* ZL = reinterpret_cast<const float2*>(ex.zlow);
   const T beta = AU ? (T)*ex.beta : T(0);
   const ElastConst<T> k = a.k;
 
@@ -158,7 +159,13 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     if (OP != kSmoothZero && ok) v = X[(size_t)iy * nvx + c];
     if constexpr (AU) {
       if (ok) {
-        const V z = B[(size_t)iy * nvx + c];
+        V z;
+        if (ZL) {
+          const float2 zl = ZL[(size_t)iy * nvx + c];
+          z = {(T)zl.x, (T)zl.y};
+        } else {
+          z = B[(size_t)iy * nvx + c];
+        }
         v = {z.x + beta * v.x, z.y + beta * v.y};
       }
     }
@@ -312,7 +319,7 @@ hipError_t elast_sweep(const ElastOpArgs<T>& a0, int op, const T* x, const T* b,
     a.sh = SweepShape::make(a.sh.nx, a.sh.ny, std::max(2, a.sh.rows & ~1), kStripCols - 2);
   }
   if (op == kProlongSmooth && (!ec || (a.sh.nx & 1) || (a.sh.ny & 1))) return hipErrorInvalidValue;
-  if (op == kApplyUpdate && (!ex.pnew || !ex.beta || !b)) return hipErrorInvalidValue;
+  if (op == kApplyUpdate && (!ex.pnew || !ex.beta || (!b && !ex.zlow))) return hipErrorInvalidValue;
   if (op == kResidualRestrict && dot_xy) return hipErrorInvalidValue;
   const int nb = a.sh.blocks();
   double* dp = nullptr;
@@ -752,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
     const bool ok = colv && iy >= 0 && iy <= ny;
     if (OP != kSmoothZero && ok) v = x[(size_t)iy * nvx + c];
     if constexpr (AU) {
-      if (ok) v = b[(size_t)iy * nvx + c] + beta * v;
+      if (ok) v = (ex.zlow ? (T)ex.zlow[(size_t)iy * nvx + c] : b[(size_t)iy * nvx + c]) + beta * v;
     }
     if constexpr (PS) {
       if (ok) {
@@ -847,7 +854,7 @@ hipError_t filter_sweep(const SweepShape& sh0, const FilterConst<T>& k, int op, 
     sh = SweepShape::make(sh.nx, sh.ny, std::max(2, sh.rows & ~1), kStripCols - 2);
   }
   if (op == kProlongSmooth && (!ec || (sh.nx & 1) || (sh.ny & 1))) return hipErrorInvalidValue;
-  if (op == kApplyUpdate && (!ex.pnew || !ex.beta || !b)) return hipErrorInvalidValue;
+  if (op == kApplyUpdate && (!ex.pnew || !ex.beta || (!b && !ex.zlow))) return hipErrorInvalidValue;
   if (op == kResidualRestrict && dot_xy) return hipErrorInvalidValue;
   const int nb = sh.blocks();
   double* dp = nullptr;
@@ -1918,7 +1925,8 @@ template hipError_t pcg_update_xr_fused<float>(size_t, const double*, const PcgS
 __global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
                                                          const double* __restrict__ rrp, int nrr,
                                                          const double* __restrict__ rzp, int nrz,
-                                                         double* __restrict__ sc, PcgSlots q) {
+                                                         double* __restrict__ sc, PcgSlots q,
+                                                         double* __restrict__ hout) {
   const double pq = sum_partials(pqp, npq);
   const double rr = sum_partials(rrp, nrr);
   const double rz = sum_partials(rzp, nrz);
@@ -1934,13 +1942,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restric
     const bool stop = sc[q.flag] != 0.0;
     sc[q.beta] = stop ? 0.0 : rz / sc[q.rz];
     if (!stop) sc[q.rz] = rz;
+    if (hout) {  // stop flag, r.r and iteration count straight to pinned host memory
+      hout[0] = sc[q.flag];
+      hout[1] = sc[q.rr];
+      hout[2] = sc[q.iters];
+    }
   }
 }
 
 hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
-                        const PcgSlots& q, hipStream_t s) {
+                        const PcgSlots& q, hipStream_t s, double* host_out) {
   k_pcg_finalize<<<1, kBlock, 0, s>>>(pq.partials, pq.nblocks, rr.partials, rr.nblocks,
-                                      rz.partials, rz.nblocks, sc, q);
+                                      rz.partials, rz.nblocks, sc, q, host_out);
   return hipGetLastError();
 }
 

@@ -77,6 +77,7 @@ struct Workspace {
   DevBuf<double> partials;  // kRegions x kMaxPartials
   DevBuf<double> scalars;
   double* host = nullptr;   // pinned: [0, kScalars) mirror + ring for lagged reads
+  double* host_dev = nullptr;  // device address of `host`, for kernels writing the ring
   int init(hipStream_t s);
   ~Workspace();
   RedSlot region(int k) const { return RedSlot{partials.get() + (size_t)k * kMaxPartials, 0}; }
@@ -148,10 +149,11 @@ class Hierarchy {
   // z = M^-1 b; at level 0 the last smoothing sweep also writes partials of b.z to dot_bz
   template <typename T>
   int vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz = nullptr);
-  // q = A p with p = z + beta p_old written to p_new; partials of p.q to dot
+  // q = A p with p = z + beta p_old written to p_new; partials of p.q to dot.  z is read from
+  // zlow (float) when that is given.
   template <typename T>
   int apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta, RedSlot* dot,
-                   hipStream_t s);
+                   hipStream_t s, const float* zlow = nullptr);
 
   // PCG from x0 = 0 on A x = b with Krylov vectors in T and the V-cycle preconditioner in PC;
   // result left in the hierarchy's Krylov x buffer (x_buf<T>())

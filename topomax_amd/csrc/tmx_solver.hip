@@ -25,8 +25,9 @@ const char* last_error_cstr() { return g_last_error.c_str(); }
 int Workspace::init(hipStream_t s) {
   TMX_TRY(partials.alloc((size_t)kRegions * kMaxPartials));
   TMX_TRY(scalars.alloc(kScalars));
-  TMX_HIP(hipHostMalloc(&host, sizeof(double) * 2 * kScalars, hipHostMallocDefault));
+  TMX_HIP(hipHostMalloc(&host, sizeof(double) * 2 * kScalars, hipHostMallocMapped));
   std::memset(host, 0, sizeof(double) * 2 * kScalars);
+  TMX_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_dev), host, 0));
   TMX_HIP(fill_zero_bytes(scalars.get(), sizeof(double) * kScalars, s));
   return 0;
 }
@@ -350,11 +351,12 @@ int Hierarchy::residual(const T* x, const T* b, T* r, hipStream_t s) {
 
 template <typename T>
 int Hierarchy::apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta,
-                            RedSlot* dot, hipStream_t s) {
+                            RedSlot* dot, hipStream_t s, const float* zlow) {
   SweepExtra<T> ex;
   ex.pnew = p_new;
   ex.beta = beta;
-  return level_op<T>(*this, 0, kApplyUpdate, p_old, z, q, dot, s, ex);
+  ex.zlow = zlow;
+  return level_op<T>(*this, 0, kApplyUpdate, p_old, zlow ? nullptr : z, q, dot, s, ex);
 }
 
 template <typename T>
@@ -522,7 +524,8 @@ T* Hierarchy::x_buf() {
 
 // z = M r for Krylov vectors in T and the V-cycle in PC (partials of r.z to rz).  With an FP32
 // V-cycle under FP64 Krylov vectors, r arrives already rounded in the float buffer r32_
-// (written by the update kernel, or converted here at the start) and z is widened after.
+// (written by the update kernel, or converted here at the start) and z stays in float (z32_),
+// which the direction update reads directly.
 template <typename T, typename PC>
 static int enqueue_precondition(Hierarchy& h, RedSlot* rz, bool convert_r, hipStream_t s) {
   if constexpr (std::is_same<T, PC>::value) {
@@ -533,7 +536,6 @@ static int enqueue_precondition(Hierarchy& h, RedSlot* rz, bool convert_r, hipSt
     const size_t nv = h.n();
     if (convert_r) TMX_HIP((convert<double, float>(nv, h.r64_.get(), h.r32_.get(), s)));
     TMX_TRY(h.vcycle<float>(0, h.r32_.get(), h.z32_.get(), s, rz));
-    TMX_HIP((convert<float, double>(nv, h.z32_.get(), h.z64_.get(), s)));
   }
   return 0;
 }
@@ -564,12 +566,14 @@ static int enqueue_pcg_iteration(Hierarchy& h, Workspace& ws, int parity, hipStr
   T* p_new = parity ? KrylovBufs<T>::p(h) : KrylovBufs<T>::pb(h);
   double* sc = ws.sc();
   RedSlot pq = ws.region(0), rr = ws.region(1), rz = ws.region(2);
-  TMX_TRY(h.apply_update<T>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s));
   constexpr bool split = !std::is_same<T, PC>::value;
+  TMX_TRY(h.apply_update<T>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s,
+                            split ? h.z32_.get() : nullptr));
   TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, x, r, p_new, q, rr, s,
                                  split ? h.r32_.get() : nullptr));
   TMX_TRY((enqueue_precondition<T, PC>(h, &rz, false, s)));
-  TMX_HIP(pcg_finalize(pq, rr, rz, sc, kPcg, s));
+  // the iteration graph of this parity reports into its own ring slot
+  TMX_HIP(pcg_finalize(pq, rr, rz, sc, kPcg, s, ws.host_dev + Workspace::kScalars + 4 * parity));
   return 0;
 }
 
@@ -607,26 +611,29 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
   TMX_HIP(set_scalar(sc, kPcg.tol2, rtol * rtol * bnorm2, s));
   TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
   // iteration k is launched before the flag of iteration k-1 is read, so the device always
-  // has queued work; once a stop flag is set the remaining queued iteration changes nothing
-  double* ring = ws.host + Workspace::kScalars;
+  // has queued work; once a stop flag is set the remaining queued iteration changes nothing.
+  // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) to
+  // ring slot (k-1)%2 in pinned host memory.  The host reads that slot after the event
+  // recorded behind iteration k has completed, and iteration k+2, which rewrites the slot, is
+  // launched only after that read.
+  volatile double* ring = ws.host + Workspace::kScalars;
+  auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
   double flag = 0.0;
   int launched = 0;
   for (int it = 1; it <= max_iter; ++it) {
     TMX_HIP(hipGraphLaunch(graphs_[prec][1 + ((it - 1) & 1)], s));
-    TMX_HIP(hipMemcpyAsync(ring + 4 * (it & 1), sc + kPcg.flag, 3 * sizeof(double),
-                           hipMemcpyDeviceToHost, s));
     TMX_HIP(hipEventRecord(ev_[it & 1], s));
     launched = it;
     if (it > 1) {
       TMX_HIP(hipEventSynchronize(ev_[(it - 1) & 1]));
-      flag = ring[4 * ((it - 1) & 1)];
+      flag = slot(it - 1)[0];
       if (flag != 0.0) break;
     }
   }
   TMX_HIP(hipEventSynchronize(ev_[launched & 1]));
-  flag = ring[4 * (launched & 1)];
-  const double rrv = ring[4 * (launched & 1) + 1];
-  *iters = (int)ring[4 * (launched & 1) + 2];
+  flag = slot(launched)[0];
+  const double rrv = slot(launched)[1];
+  *iters = (int)slot(launched)[2];
   *rel = std::sqrt(std::max(rrv, 0.0) / bnorm2);
   if (g_trace)
     std::fprintf(stderr, "[tmx] pcg<%s> %s n=%zu: %d its, rel %.3e, flag %g\n",
@@ -745,6 +752,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
 }
 
 template int Hierarchy::apply<double>(const double*, double*, RedSlot*, hipStream_t);
+template int Hierarchy::apply_update<double>(const double*, const double*, double*, double*,
+                                             const double*, RedSlot*, hipStream_t, const float*);
 template int Hierarchy::apply<float>(const float*, float*, RedSlot*, hipStream_t);
 template double* Hierarchy::x_buf<double>();
 template float* Hierarchy::x_buf<float>();
