@@ -1565,95 +1565,84 @@ template hipError_t tail_vcycle<double, 2>(const TailDesc<double>&, size_t, hipS
 template hipError_t tail_vcycle<float, 1>(const TailDesc<float>&, size_t, hipStream_t);
 template hipError_t tail_vcycle<float, 2>(const TailDesc<float>&, size_t, hipStream_t);
 
-// Galerkin product, one thread per coarse node.  Interpolation from coarse to fine node g:
-// nested P1 (x-, y- and diagonal midpoints) or tensor-product 1-D linear (bilinear).
-struct Interp2 {
-  int n;
-  int cx[4], cy[4];
-  double w[4];
-};
-
-__device__ Interp2 interp2(int gx, int gy, int nxf, int nyf, int nxc, int nyc, int nested) {
-  Interp2 r;
-  if (nested) {
-    const int ox = gx & 1, oy = gy & 1;
-    if (!ox && !oy) {
-      r.n = 1; r.cx[0] = gx >> 1; r.cy[0] = gy >> 1; r.w[0] = 1.0;
-    } else {
-      r.n = 2;
-      r.cx[0] = gx >> 1; r.cy[0] = gy >> 1;
-      r.cx[1] = (gx >> 1) + ox; r.cy[1] = (gy >> 1) + oy;
-      r.w[0] = r.w[1] = 0.5;
-    }
-    return r;
-  }
-  const Interp1 ix = interp1(gx, nxf, nxc), iy = interp1(gy, nyf, nyc);
-  r.n = 0;
-  for (int b = 0; b < iy.n; ++b)
-    for (int a = 0; a < ix.n; ++a) {
-      r.cx[r.n] = ix.c[a]; r.cy[r.n] = iy.c[b]; r.w[r.n] = ix.w[a] * iy.w[b];
-      ++r.n;
-    }
-  return r;
-}
-
+// Galerkin product R A P for tensor-product linear (bilinear) coarsening, one thread per
+// coarse node (I, J).  The fine nodes f that interpolate from (I, J) lie within one node of
+// its fine position p, and their stencil neighbours g within two; the 1-D interpolation
+// weights from those five fine positions per axis to the three coarse positions I-1..I+1
+// (and J-1..J+1) are tabulated once, so every loop below has compile-time trip counts and
+// indices: the 9 x 9 stencil entries of the 3 x 3 fine rows are loaded unconditionally
+// (rows clamped into the grid; their weight is zero then) and accumulated in registers.
 template <int BS>
-__global__ void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
+__global__ __launch_bounds__(64) void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                               const double* __restrict__ stf, double* __restrict__ stc) {
-  const size_t nnf = (size_t)(nxf + 1) * (nyf + 1);
-  const size_t nnc = (size_t)(nxc + 1) * (nyc + 1);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nnf = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
+  const unsigned nnc = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnc) return;
-  const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
+  const int I = (int)(i % (unsigned)(nxc + 1)), J = (int)(i / (unsigned)(nxc + 1));
   constexpr int B2 = BS * BS;
   double acc[9][B2];
+#pragma unroll
   for (int k = 0; k < 9; ++k)
+#pragma unroll
     for (int e = 0; e < B2; ++e) acc[k][e] = 0.0;
-  if (node_fixed(fixed, I, J, nxc, nyc)) {
-    for (int a = 0; a < BS; ++a) acc[4][a * BS + a] = 1.0;
-  } else {
-    const int px = nested ? 2 * I : cpos(I, nxf, nxc);
-    const int py = nested ? 2 * J : cpos(J, nyf, nyc);
-    for (int fy = py - 1; fy <= py + 1; ++fy)
-      for (int fx = px - 1; fx <= px + 1; ++fx) {
-        if (fx < 0 || fy < 0 || fx > nxf || fy > nyf) continue;
-        const Interp2 pf = interp2(fx, fy, nxf, nyf, nxc, nyc, nested);
-        double wf = 0.0;
-        for (int t = 0; t < pf.n; ++t)
-          if (pf.cx[t] == I && pf.cy[t] == J) wf += pf.w[t];
-        if (wf == 0.0) continue;
-        const size_t f = (size_t)fy * (nxf + 1) + fx;
-        for (int k = 0; k < 9; ++k) {
-          const int gx = fx + k % 3 - 1, gy = fy + k / 3 - 1;
-          if (gx < 0 || gy < 0 || gx > nxf || gy > nyf) continue;
-          double A[B2];
-          bool nz = false;
-          for (int e = 0; e < B2; ++e) {
-            A[e] = stf[(size_t)(k * B2 + e) * nnf + f];
-            nz |= (A[e] != 0.0);
-          }
-          if (!nz) continue;
-          const Interp2 pg = interp2(gx, gy, nxf, nyf, nxc, nyc, nested);
-          for (int t = 0; t < pg.n; ++t) {
-            const int dx = pg.cx[t] - I, dy = pg.cy[t] - J;
-            if (dx < -1 || dx > 1 || dy < -1 || dy > 1) continue;  // cannot happen for these P
-            if (node_fixed(fixed, pg.cx[t], pg.cy[t], nxc, nyc)) continue;
-            const double w = wf * pg.w[t];
-            const int kc = st_index(dx, dy);
-            for (int e = 0; e < B2; ++e) acc[kc][e] += w * A[e];
-          }
+  const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
+  // WX[a][d]: weight of coarse column I-1+d at fine column px-2+a (zero outside the grid)
+  float WX[5][3], WY[5][3];
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int gx = px - 2 + a, gy = py - 2 + a;
+      WX[a][d] = (gx < 0 || gx > nxf) ? 0.f : (float)interp1_weight(gx, nxf, nxc, I - 1 + d);
+      WY[a][d] = (gy < 0 || gy > nyf) ? 0.f : (float)interp1_weight(gy, nyf, nyc, J - 1 + d);
+    }
+  float live[9];  // coarse neighbour not fixed
+#pragma unroll
+  for (int kc = 0; kc < 9; ++kc) {
+    const int cx = I + kc % 3 - 1, cy = J + kc / 3 - 1;
+    live[kc] = node_fixed(fixed, cx, cy, nxc, nyc) ? 0.f : 1.f;
+  }
+  const bool self_fixed = node_fixed(fixed, I, J, nxc, nyc);
+#pragma unroll
+  for (int fyi = 0; fyi < 3; ++fyi)
+#pragma unroll
+    for (int fxi = 0; fxi < 3; ++fxi) {
+      // fine row f = (px-1+fxi, py-1+fyi): table index fxi+1, fyi+1
+      const double wf = self_fixed ? 0.0 : (double)(WX[fxi + 1][1] * WY[fyi + 1][1]);
+      const int fx = min(max(px - 1 + fxi, 0), nxf), fy = min(max(py - 1 + fyi, 0), nyf);
+      const unsigned f = (unsigned)fy * (unsigned)(nxf + 1) + (unsigned)fx;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int ax = fxi + k % 3, ay = fyi + k / 3;  // neighbour g at table index (ax, ay)
+        double A[B2];
+#pragma unroll
+        for (int e = 0; e < B2; ++e) A[e] = stf[(size_t)(k * B2 + e) * nnf + f];
+#pragma unroll
+        for (int kc = 0; kc < 9; ++kc) {
+          const double w = wf * (double)(WX[ax][kc % 3] * WY[ay][kc / 3] * live[kc]);
+#pragma unroll
+          for (int e = 0; e < B2; ++e) acc[kc][e] += w * A[e];
         }
       }
+    }
+  if (self_fixed) {
+#pragma unroll
+    for (int a = 0; a < BS; ++a) acc[4][a * BS + a] = 1.0;
   }
+#pragma unroll
   for (int k = 0; k < 9; ++k)
+#pragma unroll
     for (int e = 0; e < B2; ++e) stc[(size_t)(k * B2 + e) * nnc + i] = acc[k][e];
 }
 
 template <int BS>
 hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
                        const double* stf, double* stc, hipStream_t s) {
+  if (nested) return hipErrorInvalidValue;  // 9-point levels are always bilinear coarsenings
+  if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
   k_stencil_rap<BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1), 64), 64, 0, s>>>(
-      nxf, nyf, nxc, nyc, nested, fixed, stf, stc);
+      nxf, nyf, nxc, nyc, fixed, stf, stc);
   return hipGetLastError();
 }
 template hipError_t stencil_rap<1>(int, int, int, int, int, unsigned, const double*, double*,
@@ -1712,8 +1701,34 @@ __global__ __launch_bounds__(1024) void k_dense_invert(int n, double* __restrict
   }
 }
 
+// The same elimination with the whole matrix in LDS (n^2 + n doubles <= 64 KB, n <= 90).
+__global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __restrict__ a) {
+  extern __shared__ double lds_m[];
+  double* m = lds_m;
+  double* col = lds_m + n * n;
+  const int nn = n * n;
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) m[e] = a[e];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = m[i * n + k];
+    __syncthreads();
+    const double rp = 1.0 / col[k];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) m[k * n + j] = (j == k) ? rp : m[k * n + j] * rp;
+    __syncthreads();
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+      const int i = (int)((unsigned)e / (unsigned)n), j = e - i * n;
+      if (i == k) continue;
+      m[e] = (j == k) ? -col[i] * rp : m[e] - col[i] * m[k * n + j];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) a[e] = m[e];
+}
+
 hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
-  k_dense_invert<<<1, 1024, sizeof(double) * n, s>>>(n, a);
+  const size_t lds = sizeof(double) * ((size_t)n * n + n);
+  if (lds <= 64 * 1024) k_dense_invert_lds<<<1, 1024, lds, s>>>(n, a);
+  else k_dense_invert<<<1, 1024, sizeof(double) * n, s>>>(n, a);
   return hipGetLastError();
 }
 
