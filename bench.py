@@ -155,7 +155,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(independent designs shard with zero communication, see DESIGN.md)",
+              "(independent designs would shard across GPUs with zero communication)",
               file=sys.stderr)
         sys.exit(2)
 
