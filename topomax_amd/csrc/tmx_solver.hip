@@ -610,30 +610,51 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
   TMX_HIP(set_scalar(sc, kPcg.rr, bnorm2, s));
   TMX_HIP(set_scalar(sc, kPcg.tol2, rtol * rtol * bnorm2, s));
   TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
-  // iteration k is launched before the flag of iteration k-1 is read, so the device always
-  // has queued work; once a stop flag is set the remaining queued iteration changes nothing.
   // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) to
-  // ring slot (k-1)%2 in pinned host memory.  The host reads that slot after the event
-  // recorded behind iteration k has completed, and iteration k+2, which rewrites the slot, is
-  // launched only after that read.
+  // ring slot (k-1)%2 in pinned host memory; event ev_[k%2] is recorded behind it.  While the
+  // host waits for iteration k it normally has iteration k+1 queued already, so the device
+  // never idles.  An iteration queued behind the one that converges would run in full for
+  // nothing (its stop flag makes it change nothing), so once the residual history predicts
+  // convergence within the next iteration the host stops queueing ahead and pays one host
+  // round trip instead.  At most two iterations are in flight, so a ring slot is rewritten
+  // (by iteration k+2) only after the host has read it.
   volatile double* ring = ws.host + Workspace::kScalars;
   auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
-  double flag = 0.0;
-  int launched = 0;
-  for (int it = 1; it <= max_iter; ++it) {
-    TMX_HIP(hipGraphLaunch(graphs_[prec][1 + ((it - 1) & 1)], s));
-    TMX_HIP(hipEventRecord(ev_[it & 1], s));
-    launched = it;
-    if (it > 1) {
-      TMX_HIP(hipEventSynchronize(ev_[(it - 1) & 1]));
-      flag = slot(it - 1)[0];
-      if (flag != 0.0) break;
+  auto launch = [&](int k) -> hipError_t {
+    hipError_t e = hipGraphLaunch(graphs_[prec][1 + ((k - 1) & 1)], s);
+    if (e == hipSuccess) e = hipEventRecord(ev_[k & 1], s);
+    return e;
+  };
+  const double tol2 = rtol * rtol * bnorm2;
+  double flag = 0.0, rr_last = bnorm2, ratio = 1.0;
+  bool near = false;
+  int launched = 0, done = 0;
+  if (max_iter >= 1) {
+    TMX_HIP(launch(1));
+    launched = 1;
+  }
+  while (done < launched) {
+    if (launched == done + 1 && launched < max_iter && !near) {
+      TMX_HIP(launch(launched + 1));
+      ++launched;
+    }
+    TMX_HIP(hipEventSynchronize(ev_[(done + 1) & 1]));
+    ++done;
+    flag = slot(done)[0];
+    if (flag != 0.0) break;
+    const double rr = slot(done)[1];
+    if (rr_last > 0.0) ratio = rr / rr_last;
+    rr_last = rr;
+    near = rr * std::min(ratio, 1.0) <= 4.0 * tol2;
+    if (launched == done && done < max_iter) {
+      TMX_HIP(launch(done + 1));
+      ++launched;
     }
   }
-  TMX_HIP(hipEventSynchronize(ev_[launched & 1]));
-  flag = slot(launched)[0];
-  const double rrv = slot(launched)[1];
-  *iters = (int)slot(launched)[2];
+  if (launched > done) TMX_HIP(hipEventSynchronize(ev_[launched & 1]));
+  if (launched > 0) flag = slot(launched)[0];
+  const double rrv = launched > 0 ? slot(launched)[1] : bnorm2;
+  *iters = launched > 0 ? (int)slot(launched)[2] : 0;
   *rel = std::sqrt(std::max(rrv, 0.0) / bnorm2);
   if (g_trace)
     std::fprintf(stderr, "[tmx] pcg<%s> %s n=%zu: %d its, rel %.3e, flag %g\n",
