@@ -95,7 +95,7 @@ typedef struct tmx_solve_stats {
   int iterations;             /* PCG iterations (all inner solves) */
   int outer_iterations;       /* refinement steps (mixed), 1 for FP64 */
   double rel_residual;        /* final |b - A u| / |b| */
-  double setup_ms, solve_ms;  /* operator hierarchy update / iterations (host clock) */
+  double setup_ms, solve_ms;  /* operator rebuild (device time) / solve (host clock) */
   int levels;                 /* multigrid levels */
 } tmx_solve_stats;
 
