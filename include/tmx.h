@@ -194,6 +194,22 @@ int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho, dou
                                      double* const* w, const tmx_solve_opts* opts,
                                      tmx_solve_stats* stats);
 
+/* Entropic mirror descent, design-space half (src/solver.py:149-302), on n_v nodal device
+ * vectors; integrals are of the CG1 interpolant (lumped mass, as FEMSolver.integrate).
+ *   tmx_emd_half_step: half = psi - step * w                          (Solver.step, :188-193)
+ *   tmx_emd_volume:    vol = int expit(half + c), dvol = int expit'(half + c)
+ *                      (the function and derivative of Solver.project's Newton solve, :158-162)
+ *   tmx_emd_apply:     psi = half + c, rho = expit(psi)               (:176, :262)
+ *   tmx_emd_change:    out = sqrt(int (rho - expit(psi_prev))^2)      (:286-288)
+ *   tmx_logit:         psi = log(rho / (1 - rho))                     (:227) */
+int tmx_emd_half_step(tmx_problem* h, const double* psi, const double* w, double step,
+                      double* half);
+int tmx_emd_volume(tmx_problem* h, const double* half, double c, double* vol_host,
+                   double* dvol_host);
+int tmx_emd_apply(tmx_problem* h, const double* half, double c, double* psi, double* rho);
+int tmx_emd_change(tmx_problem* h, const double* rho, const double* psi_prev, double* out_host);
+int tmx_logit(tmx_problem* h, const double* rho, double* psi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -156,3 +156,29 @@ def test_smoothed_density_high_contrast(p):
     um = host(gp.forward(cuda(xi), precision="mixed-ir", rtol=1e-10))
     print("p", p, "mixed-ir u err", rel(um, u_ref), gp.last_stats, flush=True)
     assert rel(um, u_ref) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-ir"])
+def test_indefinite_stiffness_negative_filtered_density(precision):
+    """A filtered design that dips below zero (the P1 Helmholtz filter has no maximum
+    principle; EMD designs reach xi ~ -0.045) makes odd-penalty triangle factors negative and
+    K(xi) indefinite.  The reference's LU solves it; here PCG breaks down and the solve falls
+    back to MINRES preconditioned by the V-cycle of the factors floored at m."""
+    nx, ny = 60, 20
+    orc = oracle_problem(CANTILEVER, nx, ny)
+    gp = gpu_problem(CANTILEVER, nx, ny, precision=precision, rtol=1e-12)
+    orc.set_penalization(3)
+    gp.set_penalization(3)
+    # a disc of xi = -0.1 (181 triangles with factor -1e-3) in a [0.3, 1] design: K(xi) has
+    # eigenvalues near -0.01 (checked with the oracle's assembled matrix)
+    X, Y = np.meshgrid(np.linspace(0, 3, nx + 1), np.linspace(0, 1, ny + 1))
+    hole = ((X - 2.0) ** 2 + (Y - 0.5) ** 2 < 0.3 ** 2).ravel()
+    xi = np.where(hole, -0.1, 0.3 + 0.7 * smoothed_density(nx, ny, seed=4, sigma=2.0, lo=0.0))
+    assert np.min(orc.element_weights(xi)) < 0
+    u_ref = orc.forward(xi)
+    u = host(gp.forward(cuda(xi)))
+    st = gp.last_stats
+    print(precision, "u err", rel(u, u_ref), st, flush=True)
+    assert st["outer_iterations"] >= 2  # the MINRES fallback ran
+    assert rel(u, u_ref) < 1e-8
+    assert st["rel_residual"] < 1e-11

@@ -152,3 +152,23 @@ def test_product_does_not_import_oracle():
         if name.endswith(".py"):
             src = open(os.path.join(pkg, name)).read()
             assert "oracle" not in src.replace("no CPU fallback", ""), name
+
+
+def test_reference_record_pickle_layout(tmp_path):
+    """The EMD output records are written as the reference's src.utils dataclasses without
+    importing the reference, and the import table is restored afterwards."""
+    import pickletools
+    import sys
+    from topomax_amd.emd import _dump_reference_record, _penalty_formatter
+    before = {k: sys.modules.get(k) for k in ("src", "src.utils")}
+    path = tmp_path / "r.dat"
+    _dump_reference_record(str(path), "SolverResult",
+                           {"exit_condition": "x", "min_objective": 1.0, "objectives": [2.0, 1.0],
+                            "iterations": 2, "min_index": 1, "times": [0.1, 0.2]})
+    assert {k: sys.modules.get(k) for k in ("src", "src.utils")} == before
+    ops = [(op.name, arg) for op, arg, _ in pickletools.genops(path.read_bytes())]
+    names = [a for _, a in ops if isinstance(a, str)]
+    assert names[:2] == ["src.utils", "SolverResult"]
+    assert any(n == "STACK_GLOBAL" for n, _ in ops) and any(n == "NEWOBJ" for n, _ in ops)
+    fmt = _penalty_formatter([1.0, 2.5, 10.25])
+    assert [fmt(p) for p in (1.0, 2.5, 10.25)] == ["01.00", "02.50", "10.25"]

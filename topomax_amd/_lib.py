@@ -109,6 +109,11 @@ SIGNATURES = {
     "tmx_batch_objective_and_gradient": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D),
                                               ctypes.POINTER(_P), ctypes.POINTER(SolveOpts),
                                               ctypes.POINTER(SolveStats)]),
+    "tmx_emd_half_step": (_I, [_P, _P, _P, _D, _P]),
+    "tmx_emd_volume": (_I, [_P, _P, _D, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "tmx_emd_apply": (_I, [_P, _P, _D, _P, _P]),
+    "tmx_emd_change": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
+    "tmx_logit": (_I, [_P, _P, _P]),
 }
 
 _lib = None

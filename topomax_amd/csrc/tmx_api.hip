@@ -668,3 +668,55 @@ extern "C" int tmx_batch_objective_and_gradient(tmx_batch* b, const double* cons
     }
   return TMX_OK;
 }
+
+// ------------------------------------------------------------------ entropic mirror descent
+// The design-space half of src/solver.py:149-302 on the device: the latent step, the volume
+// integrals of the projection and the design change.  The scalar root-finding around
+// tmx_emd_volume stays on the host.
+extern "C" int tmx_emd_half_step(tmx_problem* h, const double* psi, const double* w, double step,
+                                 double* half) {
+  if (!h || !psi || !w || !half) return invalid("null argument");
+  Scope scope(h);
+  TMX_HIP(emd_half_step(nverts(h), psi, w, step, half, h->stream));
+  return TMX_OK;
+}
+
+extern "C" int tmx_emd_volume(tmx_problem* h, const double* half, double c, double* vol,
+                              double* dvol) {
+  if (!h || !half || !vol || !dvol) return invalid("null argument");
+  Scope scope(h);
+  RedSlot r0 = h->ws.region(2), r1 = h->ws.region(3);
+  TMX_HIP(emd_volume(nverts(h), half, c, h->lumped.get(), r0, r1, h->ws.sc() + kScInt,
+                     h->ws.sc() + kScTmp, h->stream));
+  TMX_TRY(h->ws.read(h->stream, kScInt, 2));
+  *vol = h->ws.host[kScInt];
+  *dvol = h->ws.host[kScTmp];
+  return TMX_OK;
+}
+
+extern "C" int tmx_emd_apply(tmx_problem* h, const double* half, double c, double* psi,
+                             double* rho) {
+  if (!h || !half || !psi || !rho) return invalid("null argument");
+  Scope scope(h);
+  TMX_HIP(emd_apply(nverts(h), half, c, psi, rho, h->stream));
+  return TMX_OK;
+}
+
+extern "C" int tmx_emd_change(tmx_problem* h, const double* rho, const double* psi_prev,
+                              double* out) {
+  if (!h || !rho || !psi_prev || !out) return invalid("null argument");
+  Scope scope(h);
+  RedSlot r = h->ws.region(3);
+  TMX_HIP(emd_change(nverts(h), rho, psi_prev, h->lumped.get(), r, h->ws.sc() + kScInt,
+                     h->stream));
+  TMX_TRY(h->ws.read(h->stream, kScInt, 1));
+  *out = std::sqrt(h->ws.host[kScInt]);
+  return TMX_OK;
+}
+
+extern "C" int tmx_logit(tmx_problem* h, const double* rho, double* psi) {
+  if (!h || !rho || !psi) return invalid("null argument");
+  Scope scope(h);
+  TMX_HIP(logit(nverts(h), rho, psi, h->stream));
+  return TMX_OK;
+}

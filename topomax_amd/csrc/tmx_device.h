@@ -58,6 +58,14 @@ __device__ __forceinline__ T hsym3(T a, T b, T c, int p) {
   return h;
 }
 
+// Triangle factor used by an operator: as is when wmin <= 0 (the operator being solved);
+// |w| raised to at least wmin when wmin > 0 (the preconditioner's SPD stand-in for a factor
+// that an odd penalty of a negative filtered density made negative).
+template <typename T>
+__device__ __forceinline__ T pc_weight(T w, T wmin) {
+  return wmin > T(0) ? max(w < T(0) ? -w : w, wmin) : w;
+}
+
 // ---------------------------------------------------------------- reductions
 // Each block writes its partial sum with a plain store; a separate one-block kernel adds the
 // partials in block order (deterministic).  Kernel boundaries order the two, so no fences

@@ -72,6 +72,7 @@ struct ElastOpArgs {
   const T* cellw;       // per-cell (wA, wB) means of r (coarser levels) or nullptr
   int p;
   T m, cp;              // rbar = m + cp * h_p(xi_a, xi_b, xi_c)
+  T wmin;               // rbar is raised to at least wmin (preconditioner operators use m)
   ElastConst<T> k;
 };
 
@@ -81,7 +82,7 @@ hipError_t elast_sweep(const ElastOpArgs<T>& a, int op, const T* x, const T* b, 
 
 // per-cell mean r from nodal xi (for restriction of level-0 weights)
 template <typename T>
-hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T* cellw,
+hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,
                               hipStream_t s);
 // coarse cell means = mean of the 4 nested fine triangles of each coarse triangle
 template <typename T>
@@ -235,5 +236,35 @@ hipError_t fill(size_t n, double val, double* p, hipStream_t s);
 hipError_t set_scalar(double* sc, int i, double v, hipStream_t s);
 // zero both components of the Dirichlet nodes of an interleaved vector
 hipError_t mask_fixed(int nx, int ny, unsigned fixed, double* v, hipStream_t s);
+
+// entropic mirror descent (nodal designs, lumped-mass integrals)
+hipError_t emd_half_step(size_t n, const double* psi, const double* w, double step, double* half,
+                         hipStream_t s);
+// out0 = int expit(half + c), out1 = int expit'(half + c) (device scalars)
+hipError_t emd_volume(size_t n, const double* half, double c, const double* m, RedSlot& r0,
+                      RedSlot& r1, double* out0, double* out1, hipStream_t s);
+hipError_t emd_apply(size_t n, const double* half, double c, double* psi, double* rho,
+                     hipStream_t s);
+// out = int (rho - expit(psi_prev))^2 (device scalar)
+hipError_t emd_change(size_t n, const double* rho, const double* psi_prev, const double* m,
+                      RedSlot& r, double* out, hipStream_t s);
+hipError_t logit(size_t n, const double* rho, double* psi, hipStream_t s);
+
+// MINRES (indefinite K): scalar slots and steps, see tmx_kernels.hip
+struct MrSlots {
+  int beta1, beta, oldb, alfa, dbar, epsln, oldeps, delta, phibar, phi, cs, sn, denom, iters,
+      tol, flag;
+};
+hipError_t mr_start(const RedSlot& rz, double* sc, const MrSlots& q, double tol, hipStream_t s);
+hipError_t mr_scale(size_t n, const double* sc, const MrSlots& q, const double* z, double* v,
+                    hipStream_t s);
+// y -= (beta/oldb) r1; alfa = v.y; y -= (alfa/beta) r2
+hipError_t mr_lanczos(size_t n, double* sc, const MrSlots& q, const double* v, const double* r1,
+                      const double* r2, double* y, RedSlot& part, hipStream_t s);
+hipError_t mr_step(const RedSlot& rz, double* sc, const MrSlots& q, double* host_out,
+                   hipStream_t s);
+hipError_t mr_update(size_t n, const double* sc, const MrSlots& q, double* v, const double* z,
+                     double* w_out, const double* w_older, const double* w_newer, double* x,
+                     hipStream_t s);
 
 }  // namespace tmx

@@ -211,8 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     T wA = T(0), wB = T(0);
     if (cell_ok) {
       if (XI) {
-        wA = a.m + a.cp * hsym3(zb, zbr, ztr, a.p);
-        wB = a.m + a.cp * hsym3(zb, zt, ztr, a.p);
+        wA = pc_weight(a.m + a.cp * hsym3(zb, zbr, ztr, a.p), a.wmin);
+        wB = pc_weight(a.m + a.cp * hsym3(zb, zt, ztr, a.p), a.wmin);
       } else {
         const V w = CW[(size_t)cy * nx + c];
         wA = w.x;
@@ -356,26 +356,27 @@ template hipError_t elast_sweep<float>(const ElastOpArgs<float>&, int, const flo
 
 template <typename T>
 __global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
-                                     V2<T>* __restrict__ cw) {
+                                     T wmin, V2<T>* __restrict__ cw) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)nx * ny) return;
   const int cx = (int)(i % nx), cy = (int)(i / nx);
   const size_t v0 = (size_t)cy * (nx + 1) + cx;
   const T a0 = xi[v0], a1 = xi[v0 + 1], a2 = xi[v0 + nx + 1], a3 = xi[v0 + nx + 2];
-  cw[i] = {m + cp * hsym3(a0, a1, a3, p), m + cp * hsym3(a0, a2, a3, p)};
+  cw[i] = {pc_weight(m + cp * hsym3(a0, a1, a3, p), wmin),
+           pc_weight(m + cp * hsym3(a0, a2, a3, p), wmin)};
 }
 
 template <typename T>
-hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T* cellw,
+hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,
                               hipStream_t s) {
   k_elast_cell_weights<T><<<grid1d((size_t)nx * ny), kBlock, 0, s>>>(
-      nx, ny, xi, p, m, cp, reinterpret_cast<V2<T>*>(cellw));
+      nx, ny, xi, p, m, cp, wmin, reinterpret_cast<V2<T>*>(cellw));
   return hipGetLastError();
 }
 template hipError_t elast_cell_weights<double>(int, int, const double*, int, double, double,
-                                               double*, hipStream_t);
-template hipError_t elast_cell_weights<float>(int, int, const float*, int, float, float, float*,
-                                              hipStream_t);
+                                               double, double*, hipStream_t);
+template hipError_t elast_cell_weights<float>(int, int, const float*, int, float, float, float,
+                                              float*, hipStream_t);
 
 // coarse triangle A (lower right) = fine (0,0).A + (1,0).A + (1,0).B + (1,1).A
 // coarse triangle B (upper left)  = fine (0,0).B + (0,1).A + (0,1).B + (1,1).B
@@ -435,8 +436,8 @@ __global__ void k_elast_to_stencil(ElastOpArgs<T> a, T* __restrict__ st,
           const size_t v0 = (size_t)cy * (nx + 1) + cx;
           const double x0 = a.xi[v0], x1 = a.xi[v0 + 1], x2 = a.xi[v0 + nx + 1],
                        x3 = a.xi[v0 + nx + 2];
-          w[0] = (double)a.m + (double)a.cp * hsym3(x0, x1, x3, a.p);
-          w[1] = (double)a.m + (double)a.cp * hsym3(x0, x2, x3, a.p);
+          w[0] = pc_weight((double)a.m + (double)a.cp * hsym3(x0, x1, x3, a.p), (double)a.wmin);
+          w[1] = pc_weight((double)a.m + (double)a.cp * hsym3(x0, x2, x3, a.p), (double)a.wmin);
         } else {
           w[0] = a.cellw[2 * ((size_t)cy * nx + cx)];
           w[1] = a.cellw[2 * ((size_t)cy * nx + cx) + 1];
@@ -2086,6 +2087,267 @@ __global__ void k_fill(size_t n, double val, double* __restrict__ p) {
 }
 hipError_t fill(size_t n, double val, double* p, hipStream_t s) {
   k_fill<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, val, p);
+  return hipGetLastError();
+}
+
+
+// ============================================================================ EMD
+// Entropic mirror descent on nodal designs (vertex order); integrals use the lumped CG1 mass
+// m_i = int phi_i, so sum_i m_i f_i is the integral of the CG1 interpolant of f.
+__device__ __forceinline__ double expit_d(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+__global__ __launch_bounds__(kBlock) void k_emd_half_step(size_t n, const double* __restrict__ psi,
+                                                          const double* __restrict__ w, double step,
+                                                          double* __restrict__ half) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    half[i] = psi[i] - step * w[i];
+}
+
+// partials of int expit(half + c) and int expit'(half + c)
+__global__ __launch_bounds__(kBlock) void k_emd_volume(size_t n, const double* __restrict__ half,
+                                                       double c, const double* __restrict__ m,
+                                                       double* __restrict__ p0,
+                                                       double* __restrict__ p1) {
+  double s0 = 0.0, s1 = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double e = expit_d(half[i] + c);
+    s0 += m[i] * e;
+    s1 += m[i] * (e * (1.0 - e));
+  }
+  const double v[2] = {s0, s1};
+  double* const o[2] = {p0, p1};
+  block_partials<2>(v, o);
+}
+
+__global__ __launch_bounds__(kBlock) void k_emd_apply(size_t n, const double* __restrict__ half,
+                                                      double c, double* __restrict__ psi,
+                                                      double* __restrict__ rho) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double p = half[i] + c;
+    psi[i] = p;
+    rho[i] = expit_d(p);
+  }
+}
+
+// partials of int (rho - expit(psi_prev))^2
+__global__ __launch_bounds__(kBlock) void k_emd_change(size_t n, const double* __restrict__ rho,
+                                                       const double* __restrict__ psi_prev,
+                                                       const double* __restrict__ m,
+                                                       double* __restrict__ p0) {
+  double s0 = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double d = rho[i] - expit_d(psi_prev[i]);
+    s0 += m[i] * d * d;
+  }
+  const double v[1] = {s0};
+  double* const o[1] = {p0};
+  block_partials<1>(v, o);
+}
+
+__global__ __launch_bounds__(kBlock) void k_logit(size_t n, const double* __restrict__ rho,
+                                                  double* __restrict__ psi) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    psi[i] = log(rho[i] / (1.0 - rho[i]));
+}
+
+// two partial arrays -> sc[i0], sc[i1]
+__global__ __launch_bounds__(kBlock) void k_reduce_pair(const double* __restrict__ p0,
+                                                        const double* __restrict__ p1, int n,
+                                                        double* __restrict__ o0,
+                                                        double* __restrict__ o1) {
+  const double a = sum_partials(p0, n);
+  const double b = sum_partials(p1, n);
+  if (threadIdx.x == 0) {
+    *o0 = a;
+    *o1 = b;
+  }
+}
+
+hipError_t emd_half_step(size_t n, const double* psi, const double* w, double step, double* half,
+                         hipStream_t s) {
+  k_emd_half_step<<<vec_grid(n), kBlock, 0, s>>>(n, psi, w, step, half);
+  return hipGetLastError();
+}
+
+hipError_t emd_volume(size_t n, const double* half, double c, const double* m, RedSlot& r0,
+                      RedSlot& r1, double* out0, double* out1, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  r0.nblocks = r1.nblocks = (int)g;
+  k_emd_volume<<<g, kBlock, 0, s>>>(n, half, c, m, r0.partials, r1.partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_reduce_pair<<<1, kBlock, 0, s>>>(r0.partials, r1.partials, (int)g, out0, out1);
+  return hipGetLastError();
+}
+
+hipError_t emd_apply(size_t n, const double* half, double c, double* psi, double* rho,
+                     hipStream_t s) {
+  k_emd_apply<<<vec_grid(n), kBlock, 0, s>>>(n, half, c, psi, rho);
+  return hipGetLastError();
+}
+
+hipError_t emd_change(size_t n, const double* rho, const double* psi_prev, const double* m,
+                      RedSlot& r, double* out, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  r.nblocks = (int)g;
+  k_emd_change<<<g, kBlock, 0, s>>>(n, rho, psi_prev, m, r.partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return reduce_to(r, out, s);
+}
+
+hipError_t logit(size_t n, const double* rho, double* psi, hipStream_t s) {
+  k_logit<<<vec_grid(n), kBlock, 0, s>>>(n, rho, psi);
+  return hipGetLastError();
+}
+
+// ============================================================================ MINRES
+// Preconditioned MINRES (Paige & Saunders 1975, the Lanczos form with Givens rotations) for
+// K(xi) when the filtered design dips below zero and an odd penalty makes K indefinite.
+// Scalars live in sc[q.base + k] (MrSlots); every kernel reads what the previous ones wrote.
+__global__ __launch_bounds__(kBlock) void k_mr_lanczos1(size_t n, const double* __restrict__ sc,
+                                                        MrSlots q, const double* __restrict__ v,
+                                                        const double* __restrict__ r1,
+                                                        double* __restrict__ y,
+                                                        double* __restrict__ part) {
+  const double beta = sc[q.beta], oldb = sc[q.oldb];
+  const double c = oldb != 0.0 ? beta / oldb : 0.0;
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double yi = y[i] - c * r1[i];
+    y[i] = yi;
+    s += v[i] * yi;
+  }
+  const double vv[1] = {s};
+  double* const o[1] = {part};
+  block_partials<1>(vv, o);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mr_lanczos2(size_t n, double* __restrict__ sc,
+                                                        MrSlots q, const double* __restrict__ part,
+                                                        int npart, const double* __restrict__ r2,
+                                                        double* __restrict__ y) {
+  const double alfa = sum_partials(part, npart);
+  const double c = alfa / sc[q.beta];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] -= c * r2[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[q.alfa] = alfa;
+}
+
+// start: beta1 = sqrt(r1 . M r1) from partials; Lanczos and rotation state reset
+__global__ __launch_bounds__(kBlock) void k_mr_start(const double* __restrict__ part, int npart,
+                                                     double* __restrict__ sc, MrSlots q,
+                                                     double tol) {
+  const double rz = sum_partials(part, npart);
+  if (threadIdx.x == 0) {
+    const double b1 = rz > 0.0 ? sqrt(rz) : 0.0;
+    sc[q.beta1] = b1;
+    sc[q.beta] = b1;
+    sc[q.oldb] = 0.0;
+    sc[q.dbar] = 0.0;
+    sc[q.epsln] = 0.0;
+    sc[q.phibar] = b1;
+    sc[q.cs] = -1.0;
+    sc[q.sn] = 0.0;
+    sc[q.iters] = 0.0;
+    sc[q.tol] = tol;
+    sc[q.flag] = !(rz == rz) ? 3.0 : rz < 0.0 ? 2.0 : rz == 0.0 ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mr_step(const double* __restrict__ part, int npart,
+                                                    double* __restrict__ sc, MrSlots q,
+                                                    double* __restrict__ hout) {
+  const double rz = sum_partials(part, npart);
+  if (threadIdx.x == 0) {
+    const double alfa = sc[q.alfa];
+    const double beta = rz > 0.0 ? sqrt(rz) : 0.0;
+    const double cs = sc[q.cs], sn = sc[q.sn], dbar = sc[q.dbar];
+    const double oldeps = sc[q.epsln];
+    const double delta = cs * dbar + sn * alfa;
+    const double gbar = sn * dbar - cs * alfa;
+    const double gamma = fmax(sqrt(gbar * gbar + beta * beta), 1e-300);
+    const double cs2 = gbar / gamma, sn2 = beta / gamma;
+    const double phibar = sc[q.phibar];
+    sc[q.oldb] = sc[q.beta];
+    sc[q.beta] = beta;
+    sc[q.oldeps] = oldeps;
+    sc[q.delta] = delta;
+    sc[q.epsln] = sn * beta;
+    sc[q.dbar] = -cs * beta;
+    sc[q.cs] = cs2;
+    sc[q.sn] = sn2;
+    sc[q.phi] = cs2 * phibar;
+    sc[q.phibar] = sn2 * phibar;
+    sc[q.denom] = 1.0 / gamma;
+    sc[q.iters] += 1.0;
+    double flag = 0.0;
+    if (!(rz == rz) || !(alfa == alfa)) flag = 3.0;
+    else if (rz < 0.0) flag = 2.0;  // preconditioner not positive definite
+    else if (sn2 * phibar <= sc[q.tol] * sc[q.beta1] || rz == 0.0) flag = 1.0;
+    sc[q.flag] = flag;
+    hout[0] = flag;
+    hout[1] = sc[q.phibar] / sc[q.beta1];
+    hout[2] = sc[q.iters];
+  }
+}
+
+// w = (v - oldeps w_older - delta w_newer) / gamma into w_out; x += phi w; v = z / beta
+__global__ __launch_bounds__(kBlock) void k_mr_update(size_t n, const double* __restrict__ sc,
+                                                      MrSlots q, double* __restrict__ v,
+                                                      const double* __restrict__ z,
+                                                      double* __restrict__ w_out,
+                                                      const double* __restrict__ w_older,
+                                                      const double* __restrict__ w_newer,
+                                                      double* __restrict__ x) {
+  const double oldeps = sc[q.oldeps], delta = sc[q.delta], denom = sc[q.denom];
+  const double phi = sc[q.phi], beta = sc[q.beta];
+  const double rb = beta > 0.0 ? 1.0 / beta : 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double w = (v[i] - oldeps * w_older[i] - delta * w_newer[i]) * denom;
+    w_out[i] = w;
+    x[i] += phi * w;
+    v[i] = z[i] * rb;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mr_scale(size_t n, const double* __restrict__ sc,
+                                                     MrSlots q, const double* __restrict__ z,
+                                                     double* __restrict__ v) {
+  const double beta = sc[q.beta];
+  const double rb = beta > 0.0 ? 1.0 / beta : 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v[i] = z[i] * rb;
+}
+
+hipError_t mr_start(const RedSlot& rz, double* sc, const MrSlots& q, double tol, hipStream_t s) {
+  k_mr_start<<<1, kBlock, 0, s>>>(rz.partials, rz.nblocks, sc, q, tol);
+  return hipGetLastError();
+}
+hipError_t mr_scale(size_t n, const double* sc, const MrSlots& q, const double* z, double* v,
+                    hipStream_t s) {
+  k_mr_scale<<<vec_grid(n), kBlock, 0, s>>>(n, sc, q, z, v);
+  return hipGetLastError();
+}
+hipError_t mr_lanczos(size_t n, double* sc, const MrSlots& q, const double* v, const double* r1,
+                      const double* r2, double* y, RedSlot& part, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  part.nblocks = (int)g;
+  k_mr_lanczos1<<<g, kBlock, 0, s>>>(n, sc, q, v, r1, y, part.partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_mr_lanczos2<<<g, kBlock, 0, s>>>(n, sc, q, part.partials, part.nblocks, r2, y);
+  return hipGetLastError();
+}
+hipError_t mr_step(const RedSlot& rz, double* sc, const MrSlots& q, double* host_out,
+                   hipStream_t s) {
+  k_mr_step<<<1, kBlock, 0, s>>>(rz.partials, rz.nblocks, sc, q, host_out);
+  return hipGetLastError();
+}
+hipError_t mr_update(size_t n, const double* sc, const MrSlots& q, double* v, const double* z,
+                     double* w_out, const double* w_older, const double* w_newer, double* x,
+                     hipStream_t s) {
+  k_mr_update<<<vec_grid(n), kBlock, 0, s>>>(n, sc, q, v, z, w_out, w_older, w_newer, x);
   return hipGetLastError();
 }
 

@@ -162,6 +162,11 @@ class Hierarchy {
           double* rel, hipStream_t s);
   template <typename T>
   T* x_buf();
+  // preconditioned MINRES (FP64, FP64 V-cycle) on A x = b from x0 = 0, restarted on the true
+  // residual until |b - A x| <= rtol |b|; result in x_buf<double>().  For operators that are
+  // not positive definite (PCG breakdown).
+  int minres(Workspace& ws, const double* b, double rtol, int max_iter, double bnorm2,
+             int* iters, double* rel, hipStream_t s);
   int solve(Workspace& ws, const double* b, double* x, const SolveParams& p, SolveStats* st,
             hipStream_t s);
 
@@ -199,6 +204,7 @@ class Hierarchy {
   DevBuf<double> x64_, r64_, z64_, p64_, q64_, pb64_;
   DevBuf<float> r32_, z32_, p32_, q32_, d32_, pb32_;
   DevBuf<double> rr64_;         // refinement residual
+  DevBuf<double> mrw_[3];       // MINRES direction vectors (elasticity only)
   // captured PCG graphs: [fp64 / fp32 / fp64 with fp32 V-cycle][0 = init, 1 = even
   // iteration, 2 = odd iteration]
   hipGraphExec_t graphs_[3][3] = {};

@@ -7,11 +7,13 @@
 // coarsest grid solved by a dense inverse.  Smoother: damped block Jacobi.
 #include "tmx_solver.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <type_traits>
 
 namespace tmx {
@@ -187,12 +189,19 @@ int Hierarchy::alloc_vectors() {
   TMX_TRY(q32_.alloc(n0));
   TMX_TRY(d32_.alloc(n0));
   TMX_TRY(rr64_.alloc(n0));
+  if (phys_ == kElasticity)
+    for (auto& w : mrw_) TMX_TRY(w.alloc(n0));
   return 0;
 }
 
+// Operator arguments of element level l.  The preconditioner's operators (V-cycle sweeps,
+// transfers and the coarse hierarchy) use |factor| raised to at least m, so they stay SPD
+// when a filtered design dips below zero and an odd penalty makes some factors negative (and
+// the preconditioned operator then has eigenvalues near +-1 there); the operator that PCG /
+// MINRES solve with (apply, residual) keeps the factors as they are.
 template <typename T>
 static ElastOpArgs<T> elast_args(const Hierarchy& h, int l, const T* xi, const T* cw,
-                                 bool fp64) {
+                                 bool fp64, bool precond) {
   const Level& L = h.levels_[l];
   ElastOpArgs<T> a;
   a.sh = fp64 ? L.sh64 : L.sh32;
@@ -202,6 +211,7 @@ static ElastOpArgs<T> elast_args(const Hierarchy& h, int l, const T* xi, const T
   a.p = h.p_;
   a.m = (T)h.m_;
   a.cp = (T)h.cp_;
+  a.wmin = precond ? (T)h.m_ : -std::numeric_limits<T>::max();
   a.k = ElastConst<T>::make(h.lam_, h.mu_, L.hx, L.hy);
   return a;
 }
@@ -210,7 +220,7 @@ static ElastOpArgs<T> elast_args(const Hierarchy& h, int l, const T* xi, const T
 static int element_stencil(const Hierarchy& h, int l, double* dst, hipStream_t s) {
   const Level& L = h.levels_[l];
   if (h.phys_ == kElasticity) {
-    ElastOpArgs<double> a = elast_args<double>(h, l, h.xi64_, L.cw64.get(), true);
+    ElastOpArgs<double> a = elast_args<double>(h, l, h.xi64_, L.cw64.get(), true, true);
     TMX_HIP(elast_to_stencil<double>(a, dst, s));
   } else {
     TMX_HIP(filter_to_stencil<double>(L.nx, L.ny, FilterConst<double>::make(h.eps_, L.hx, L.hy),
@@ -268,7 +278,7 @@ int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t 
   const Level& L0 = levels_[0];
   const int last = (int)levels_.size() - 1;
   if (last >= 1 && levels_[1].kind == kElem) {
-    TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p, m, cp_, cw0_.get(), s));
+    TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p, m, cp_, m, cw0_.get(), s));
   }
   for (int l = 1; l <= last; ++l) {
     Level& L = levels_[l];
@@ -315,12 +325,13 @@ const float* xi_for<float>(const Hierarchy& h) { return h.xi32_.get(); }
 
 template <typename T>
 static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, RedSlot* dot,
-                    hipStream_t s, const SweepExtra<T>& ex = SweepExtra<T>()) {
+                    hipStream_t s, const SweepExtra<T>& ex = SweepExtra<T>(),
+                    bool precond = true) {
   Level& L = h.levels_[l];
   const bool fp64 = sizeof(T) == 8;
   if (L.kind == kElem) {
     if (h.phys_ == kElasticity) {
-      ElastOpArgs<T> a = elast_args<T>(h, l, xi_for<T>(h), PrecBufs<T>::cw(L), fp64);
+      ElastOpArgs<T> a = elast_args<T>(h, l, xi_for<T>(h), PrecBufs<T>::cw(L), fp64, precond);
       TMX_HIP(elast_sweep<T>(a, op, x, b, y, (T)h.omega_, dot, s, ex));
     } else {
       TMX_HIP(filter_sweep<T>(PrecBufs<T>::sh(L), FilterConst<T>::make(h.eps_, L.hx, L.hy), op, x,
@@ -341,12 +352,12 @@ static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, R
 
 template <typename T>
 int Hierarchy::apply(const T* x, T* y, RedSlot* dot, hipStream_t s) {
-  return level_op<T>(*this, 0, kApply, x, nullptr, y, dot, s);
+  return level_op<T>(*this, 0, kApply, x, nullptr, y, dot, s, SweepExtra<T>(), false);
 }
 
 template <typename T>
 int Hierarchy::residual(const T* x, const T* b, T* r, hipStream_t s) {
-  return level_op<T>(*this, 0, kResidual, x, b, r, nullptr, s);
+  return level_op<T>(*this, 0, kResidual, x, b, r, nullptr, s, SweepExtra<T>(), false);
 }
 
 template <typename T>
@@ -356,7 +367,7 @@ int Hierarchy::apply_update(const T* p_old, const T* z, T* q, T* p_new, const do
   ex.pnew = p_new;
   ex.beta = beta;
   ex.zlow = zlow;
-  return level_op<T>(*this, 0, kApplyUpdate, p_old, zlow ? nullptr : z, q, dot, s, ex);
+  return level_op<T>(*this, 0, kApplyUpdate, p_old, zlow ? nullptr : z, q, dot, s, ex, false);
 }
 
 template <typename T>
@@ -673,6 +684,86 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
   return -3;
 }
 
+// MINRES scalar slots (after the PCG slots)
+constexpr MrSlots kMr = {30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45};
+
+int Hierarchy::minres(Workspace& ws, const double* b, double rtol, int max_iter, double bnorm2,
+                      int* iters, double* rel, hipStream_t s) {
+  constexpr int kMaxRestarts = 8;
+  const size_t nv = n();
+  double* sc = ws.sc();
+  double* X = x64_.get();
+  double* R1 = r64_.get();
+  double* R2 = pb64_.get();
+  double* Y = q64_.get();
+  double* Z = z64_.get();
+  double* V = p64_.get();
+  double* W[3] = {mrw_[0].get(), mrw_[1].get(), mrw_[2].get()};
+  if (!W[0]) {
+    set_error("internal: MINRES buffers exist for elasticity only");
+    return -1;
+  }
+  RedSlot part = ws.region(0), rz = ws.region(2), gen = ws.region(3);
+  volatile double* ring = ws.host + Workspace::kScalars + 8;
+  double* ring_dev = ws.host_dev + Workspace::kScalars + 8;
+  TMX_HIP(fill_zero_bytes(X, sizeof(double) * nv, s));
+  int total = 0, rc = -3;
+  double relres = 1.0;
+  for (int restart = 0;; ++restart) {
+    TMX_TRY(residual<double>(X, b, R1, s));
+    TMX_HIP(dot(nv, R1, R1, gen, sc + kScTmp, s));
+    TMX_TRY(ws.read(s, kScTmp, 1));
+    relres = std::sqrt(std::max(ws.host[kScTmp], 0.0) / bnorm2);
+    if (!std::isfinite(relres)) {
+      set_error("non-finite residual in MINRES");
+      return -5;
+    }
+    if (relres <= rtol) {
+      rc = 0;
+      break;
+    }
+    if (restart >= kMaxRestarts || total >= max_iter) break;
+    // reduction asked of this run, measured in the preconditioner's norm
+    const double tol = 0.5 * rtol / relres;
+    TMX_TRY(vcycle<double>(0, R1, Z, s, &rz));
+    TMX_HIP(mr_start(rz, sc, kMr, tol, s));
+    TMX_HIP(copy_vec<double>(nv, R1, R2, s));
+    TMX_HIP(fill_zero_bytes(W[1], sizeof(double) * nv, s));
+    TMX_HIP(fill_zero_bytes(W[2], sizeof(double) * nv, s));
+    TMX_HIP(mr_scale(nv, sc, kMr, Z, V, s));
+    for (; total < max_iter; ++total) {
+      TMX_TRY(apply<double>(V, Y, nullptr, s));
+      TMX_HIP(mr_lanczos(nv, sc, kMr, V, R1, R2, Y, part, s));
+      std::swap(R1, R2);  // r1 <- r2
+      std::swap(R2, Y);   // r2 <- y (the new Lanczos vector); y <- the old r1 (free)
+      TMX_TRY(vcycle<double>(0, R2, Z, s, &rz));
+      TMX_HIP(mr_step(rz, sc, kMr, ring_dev, s));
+      TMX_HIP(mr_update(nv, sc, kMr, V, Z, W[0], W[1], W[2], X, s));
+      std::rotate(W, W + 1, W + 3);
+      TMX_HIP(hipStreamSynchronize(s));
+      const double flag = ring[0];
+      if (flag == 2.0) {
+        set_error("MINRES breakdown: preconditioner not positive definite");
+        return -4;
+      }
+      if (flag == 3.0) {
+        set_error("non-finite value in MINRES");
+        return -5;
+      }
+      if (flag == 1.0) {
+        ++total;
+        break;
+      }
+    }
+  }
+  *iters = total;
+  *rel = relres;
+  if (g_trace)
+    std::fprintf(stderr, "[tmx] minres n=%zu: %d its, rel %.3e, rc %d\n", nv, total, relres, rc);
+  if (rc != 0) set_error("MINRES did not reach the tolerance within the iteration limit");
+  return rc;
+}
+
 static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -708,6 +799,12 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
              : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s);
     S.iterations = it;
     S.outer = 1;
+    if (rc == -4 && phys_ == kElasticity) {  // K(xi) not positive definite: MINRES
+      int it2 = 0;
+      rc = minres(ws, b, prm.rtol, std::max(1, prm.max_iter - it), bb, &it2, &rel, s);
+      S.iterations += it2;
+      S.outer = 2;
+    }
     if (rc == 0 || rc == -3) {  // report the true residual
       TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
       TMX_TRY(residual<double>(x, b, rr64_.get(), s));
@@ -749,6 +846,19 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       S.iterations += it;
       budget -= it;
       S.outer = outer + 1;
+      if (irc == -4 && phys_ == kElasticity) {  // K(xi) not positive definite: MINRES
+        int it2 = 0;
+        double rel2 = 0;
+        rc = minres(ws, b, prm.rtol, std::max(1, budget), bb, &it2, &rel2, s);
+        S.iterations += it2;
+        S.outer = outer + 2;
+        if (rc != 0 && rc != -3) return rc;
+        TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
+        S.rel_residual = rel2;
+        TMX_HIP(hipStreamSynchronize(s));
+        S.solve_ms = ms_since(t0);
+        return rc;
+      }
       if (irc != 0 && irc != -3) return irc;
       TMX_HIP(axpy_scaled<float>(nv, ws.sc(), kScRR64, d32_.get(), x, s));
       TMX_TRY(residual<double>(x, b, rr64_.get(), s));
