@@ -30,11 +30,28 @@ __device__ __forceinline__ V2<T>& operator+=(V2<T>& a, V2<T> b) {
 
 // Cross-lane moves inside one wavefront.  down(v): lane i receives lane i+1's value;
 // up(v): lane i receives lane i-1's value.  Edge lanes receive their own value, which
-// the sweep kernels never use for an output node.
-template <typename T>
-__device__ __forceinline__ T lane_down(T v) { return __shfl_down(v, 1, kWave); }
-template <typename T>
-__device__ __forceinline__ T lane_up(T v) { return __shfl_up(v, 1, kWave); }
+// the sweep kernels never use for an output node.  Doubles move as two DPP moves (GFX9
+// wavefront shifts wave_shl:1 = 0x130 and wave_shr:1 = 0x138, with the lane's own value kept
+// where the source lane is out of the wave); floats use the LDS permute, which measured
+// faster for the FP32 sweeps (the DPP form costs hazard wait states there).
+__device__ __forceinline__ int dpp_down_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_up_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_down(float v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ double lane_down(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_down_i((int)(b & 0xffffffffll)), hi = dpp_down_i((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_up(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_up_i((int)(b & 0xffffffffll)), hi = dpp_up_i((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 template <typename T>
 __device__ __forceinline__ V2<T> lane_down(V2<T> v) { return {lane_down(v.x), lane_down(v.y)}; }
 template <typename T>
