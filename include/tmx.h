@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TMX_ABI_VERSION 1
+#define TMX_ABI_VERSION 2
 
 enum tmx_status {
   TMX_OK = 0,
@@ -81,6 +81,11 @@ typedef struct tmx_problem_desc {
   int has_body_force;
   tmx_body_force body_force;
   int precision;              /* default tmx_precision for solves */
+  int degree;                 /* displacement element: 1 (P1, default when 0) or 2 (P2, the
+                                 reference's VectorElement("CG", triangle, 2),
+                                 FEM_src/elasisity_problem.py:194-196); P2 solves run in FP64
+                                 (MINRES preconditioned by P2 block Jacobi over the P1
+                                 multigrid) and u has 2 (2nx+1)(2ny+1) entries */
 } tmx_problem_desc;
 
 typedef struct tmx_solve_opts {

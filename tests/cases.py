@@ -36,7 +36,8 @@ def oracle_problem(case: dict, nx: int, ny: int, degree: int = 1):
                             body_force=case.get("body_force"), degree=degree)
 
 
-def gpu_problem(case: dict, nx: int, ny: int, precision: str = "fp64", rtol: float = 1e-12):
+def gpu_problem(case: dict, nx: int, ny: int, precision: str = "fp64", rtol: float = 1e-12,
+                degree: int = 1):
     from topomax_amd import (CircularRegion, ControlSpace, DomainParameters,
                              ElasticityParameters, ElasticityProblem, Force, Side,
                              StructuredMesh, Traction)
@@ -56,7 +57,8 @@ def gpu_problem(case: dict, nx: int, ny: int, precision: str = "fp64", rtol: flo
                                   case["poisson_ratio"])
     domain = DomainParameters(case["width"], case["height"], ProblemType.ELASTICITY, 25.0, 25.0,
                               [3.0], 0.5)
-    return ElasticityProblem(mesh, space, domain, params, precision=precision, rtol=rtol)
+    return ElasticityProblem(mesh, space, domain, params, precision=precision, rtol=rtol,
+                             degree=degree)
 
 
 def case_from_design(design: dict) -> dict:

@@ -1,0 +1,113 @@
+"""The P2 displacement element (the reference's VectorElement("CG", triangle, 2)) on the GPU
+against the P2 oracle, and the reference's own golden run reproduced on the GPU.
+
+Tolerances: load vectors and operator applications to 1e-12 relative (exact integration on
+both sides); solves to 1e-11 relative residual (the FP64 residual floor of the P2 system is
+about 3e-12 here) must match the oracle's SuperLU solve to 1e-8;
+the triangle golden (tests/test_data/FEM/triangle of the reference: objective after 24 EMD
+iterations at N=10, p=3) to 1e-10 relative in the objective, with the same iteration count,
+and 1e-8 in the sorted design (the golden design is stored in dolfin's dof order).
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests.cases import CANTILEVER, case_from_design, gpu_problem, oracle_problem, smoothed_density
+from tests.conftest import GOLDEN, design_path
+from topomax_amd.emd import EMDSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def cuda(a):
+    return torch.as_tensor(a, dtype=torch.float64, device="cuda:0")
+
+
+@pytest.mark.parametrize("name,N", [("triangle", 10), ("cantilever", 8), ("bridge", 4),
+                                    ("short_cantilever", 2)])
+def test_p2_load_vector(name, N):
+    case = case_from_design(json.load(open(design_path(name))))
+    nx, ny = round(case["width"] * N), round(case["height"] * N)
+    orc = oracle_problem(case, nx, ny, degree=2)
+    gp = gpu_problem(case, nx, ny, degree=2)
+    F = host(gp.load_vector())
+    print(name, F.shape, orc.load.shape, rel(F, orc.load), flush=True)
+    assert F.shape == orc.load.shape
+    assert rel(F, orc.load) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_p2_matvec(p):
+    nx, ny = 24, 8
+    orc = oracle_problem(CANTILEVER, nx, ny, degree=2)
+    gp = gpu_problem(CANTILEVER, nx, ny, degree=2)
+    orc.set_penalization(p)
+    gp.set_penalization(p)
+    xi = smoothed_density(nx, ny, seed=2, sigma=2.0)
+    u = np.random.default_rng(3).standard_normal(orc.n_dofs)
+    y_ref = orc.masked_operator(xi) @ u
+    y = host(gp.matvec(cuda(xi), cuda(u)))
+    print("p", p, rel(y, y_ref), flush=True)
+    assert rel(y, y_ref) < 1e-12
+
+
+@pytest.mark.parametrize("p", [3, 5])
+def test_p2_objective_and_gradient(p):
+    nx, ny = 60, 20
+    orc = oracle_problem(CANTILEVER, nx, ny, degree=2)
+    gp = gpu_problem(CANTILEVER, nx, ny, rtol=1e-11, degree=2)
+    orc.set_penalization(p)
+    gp.set_penalization(p)
+    rho = smoothed_density(nx, ny, seed=7, sigma=3.0)
+    J_ref = orc.calculate_objective(rho)
+    w_ref = orc.calculate_objective_gradient()
+    J, w, mw = gp.objective_and_gradient(cuda(rho))
+    print("p", p, J, J_ref, rel(host(gp.u), orc.u), rel(host(w), w_ref), gp.last_stats, flush=True)
+    assert abs(J - J_ref) < 1e-8 * abs(J_ref)
+    assert rel(host(gp.u), orc.u) < 1e-8
+    assert rel(host(w), w_ref) < 1e-8
+
+
+def test_p2_indefinite_negative_density():
+    nx, ny = 30, 10
+    orc = oracle_problem(CANTILEVER, nx, ny, degree=2)
+    gp = gpu_problem(CANTILEVER, nx, ny, rtol=1e-11, degree=2)
+    orc.set_penalization(3)
+    gp.set_penalization(3)
+    X, Y = np.meshgrid(np.linspace(0, 3, nx + 1), np.linspace(0, 1, ny + 1))
+    hole = ((X - 2.0) ** 2 + (Y - 0.5) ** 2 < 0.3 ** 2).ravel()
+    xi = np.where(hole, -0.1, 0.3 + 0.7 * smoothed_density(nx, ny, seed=4, sigma=2.0, lo=0.0))
+    u_ref = orc.forward(xi)
+    u = host(gp.forward(cuda(xi)))
+    print("u err", rel(u, u_ref), gp.last_stats, flush=True)
+    assert rel(u, u_ref) < 1e-8
+
+
+def test_triangle_golden_on_gpu():
+    """The reference's known answer (tests/test_elasticity_solver.py:30-55): the triangle
+    design at N=10 with the P2 element, EMD to convergence; objective 0.0018191324070894702
+    at iteration 24."""
+    golden = json.load(open(os.path.join(GOLDEN, "triangle_golden.json")))
+    solver = EMDSolver(golden["N"], design_path("triangle"), degree=2, rtol=1e-11,
+                       filter_rtol=1e-13)
+    res = solver.solve()[0]
+    rho = np.sort(host(solver.rho))
+    print("iteration", res.iteration, golden["iteration"], "objective", res.objectives[-1],
+          golden["objective"], "rho", np.abs(rho - np.array(golden["rho_sorted"])).max(),
+          flush=True)
+    assert res.iteration == golden["iteration"]
+    assert abs(res.objectives[-1] - golden["objective"]) < 1e-10 * golden["objective"]
+    assert np.abs(rho - np.array(golden["rho_sorted"])).max() < 1e-8

@@ -36,7 +36,7 @@ def test_ctypes_binding_covers_header():
     from topomax_amd import _lib
     assert sorted(_lib.SIGNATURES) == header_functions()
     lib = _lib.load()
-    assert lib.tmx_abi_version() == 1
+    assert lib.tmx_abi_version() == 2
 
 
 def test_desc_layout_matches_c_struct():

@@ -53,7 +53,7 @@ class ProblemDesc(ctypes.Structure):
         ("fixed_sides", ctypes.c_uint),
         ("n_tractions", ctypes.c_int), ("tractions", Traction * TMX_MAX_TRACTIONS),
         ("has_body_force", ctypes.c_int), ("body_force", BodyForce),
-        ("precision", ctypes.c_int),
+        ("precision", ctypes.c_int), ("degree", ctypes.c_int),
     ]
 
 

@@ -5,11 +5,13 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "tmx_p2.h"
 #include "tmx_solver.h"
 
 namespace tmx {
@@ -37,6 +39,11 @@ struct tmx_problem {
   DevBuf<double> tmp64;
   DevBuf<float> xi32, u32, y32;
   SolveParams el_prm, fl_prm;
+  // P2 element (degree 2): lattice geometry, element weights of the operator (W2) and of the
+  // preconditioner (W2pc, from max(xi, 0)), P2 block diagonal, loads and solver vectors
+  int degree = 1;
+  P2Geom g2{};
+  DevBuf<double> W2, W2pc, D2, F2, F2m, p2v[11], r1c, e1c;
   ~tmx_problem() {
     if (ev_in) (void)hipEventDestroy(ev_in);
     for (auto e : ev_setup)
@@ -94,6 +101,7 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
       return invalid("Malformed side: " + std::to_string(d->tractions[i].side));
   if ((d->fixed_sides & ~15u) != 0) return invalid("Malformed side in fixed_sides");
   if (d->precision < 0 || d->precision > 3) return invalid("unknown precision");
+  if (d->degree < 0 || d->degree > 2) return invalid("degree must be 1 or 2");
   h->desc = *d;
   h->nx = d->nx;
   h->ny = d->ny;
@@ -169,6 +177,24 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, ab, cd),
                                kApply, h->tmp64.get(), nullptr, h->lumped.get(), 0.0, nullptr, s));
   TMX_HIP(hipStreamSynchronize(s));
+  h->degree = d->degree == 2 ? 2 : 1;
+  if (h->degree == 2) {
+    h->g2 = P2Geom{h->nx, h->ny, ab, cd, h->lam, h->mu, fx};
+    const size_t n2 = h->g2.nodes(), ncell = (size_t)h->nx * h->ny;
+    TMX_TRY(h->W2.alloc(12 * ncell));
+    TMX_TRY(h->W2pc.alloc(12 * ncell));
+    TMX_TRY(h->D2.alloc(3 * n2));
+    TMX_TRY(h->F2.alloc(2 * n2));
+    TMX_TRY(h->F2m.alloc(2 * n2));
+    for (auto& v : h->p2v) TMX_TRY(v.alloc(2 * n2));
+    TMX_TRY(h->r1c.alloc(2 * nv));
+    TMX_TRY(h->e1c.alloc(2 * nv));
+    TMX_TRY(h->u_s.alloc(2 * n2));
+    TMX_HIP(p2_assemble_load(ld, h->F2.get(), s));
+    TMX_HIP(copy_vec<double>(2 * n2, h->F2.get(), h->F2m.get(), s));
+    TMX_HIP(p2_mask_fixed(h->g2, h->F2m.get(), s));
+    TMX_HIP(hipStreamSynchronize(s));
+  }
   h->el_prm.rtol = 1e-10;
   h->el_prm.max_iter = 1000;
   h->el_prm.precision = solve_precision(d->precision);
@@ -210,7 +236,7 @@ extern "C" int tmx_problem_info(const tmx_problem* h, int64_t* n_vertices, int64
   if (!h) return invalid("null handle");
   const int64_t nv = (int64_t)(h->nx + 1) * (h->ny + 1);
   if (n_vertices) *n_vertices = nv;
-  if (n_dofs) *n_dofs = 2 * nv;
+  if (n_dofs) *n_dofs = h->degree == 2 ? 2 * (int64_t)h->g2.nodes() : 2 * nv;
   if (fixed_sides_applied) {
     unsigned f = 0;
     if (h->fixed & kLeft) f |= TMX_FIX_LEFT;
@@ -261,6 +287,11 @@ static void export_stats(const SolveStats& s, tmx_solve_stats* o) {
 }
 
 static size_t nverts(const tmx_problem* h) { return (size_t)(h->nx + 1) * (h->ny + 1); }
+// displacement unknowns: 2 per P1 vertex, or 2 per P2 lattice node
+static size_t ndofs(const tmx_problem* h) {
+  return h->degree == 2 ? 2 * h->g2.nodes() : 2 * nverts(h);
+}
+static const double* load_F(const tmx_problem* h) { return h->degree == 2 ? h->F2.get() : h->F.get(); }
 
 static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
                             const tmx_solve_opts* o = nullptr) {
@@ -293,18 +324,99 @@ extern "C" int tmx_filter_solve(tmx_problem* h, const double* b, double* w) {
   return filter_rhs_solve(h, b, w);
 }
 
+// P2 solve of K2(xi) u = F2: MINRES (symmetric; K2 may be indefinite when the filtered design
+// dips below zero) preconditioned by a symmetric two-level cycle: nu damped block-Jacobi
+// sweeps on the P2 operator of max(xi, 0), the P1 multigrid V-cycle on P1 (the P1 space of
+// the same mesh is a subspace of P2, and the P1 operator is the Galerkin restriction of the
+// P2 one), nu sweeps back.  FP64 throughout.
+static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolveParams& prm,
+                    SolveStats* st) {
+  hipStream_t s = h->stream;
+  const P2Geom& g = h->g2;
+  const size_t n = 2 * g.nodes();
+  constexpr int kNu = 2;
+  constexpr double kOmega = 0.6;  // lambda_max(D^-1 K) of P2 block Jacobi is about 2.7
+  TMX_HIP(p2_weights(g, xi, h->p, h->m, false, h->W2.get(), s));
+  TMX_HIP(p2_weights(g, xi, h->p, h->m, true, h->W2pc.get(), s));
+  TMX_HIP(p2_diag(g, h->W2pc.get(), h->D2.get(), s));
+  DevBuf<double>* P = h->p2v;
+  double* X = P[8].get();
+  double* ta = P[9].get();
+  double* tb = P[10].get();
+  MinresBufs bf{P[0].get(), P[1].get(), P[2].get(), P[3].get(), P[4].get(), P[5].get(),
+                P[6].get(), P[7].get()};
+  auto apply_fn = [&](const double* x, double* y) -> int {
+    TMX_HIP(p2_apply(g, h->W2.get(), x, y, s));
+    return 0;
+  };
+  auto pc_fn = [&](const double* r, double* z, RedSlot* rz) -> int {
+    double* cur = ta;
+    double* nxt = tb;
+    TMX_HIP(p2_smooth(g, h->W2pc.get(), h->D2.get(), nullptr, r, cur, kOmega, s));
+    for (int k = 1; k < kNu; ++k) {
+      TMX_HIP(p2_smooth(g, h->W2pc.get(), h->D2.get(), cur, r, nxt, kOmega, s));
+      std::swap(cur, nxt);
+    }
+    TMX_HIP(p2_residual(g, h->W2pc.get(), cur, r, nxt, s));
+    TMX_HIP(p2_restrict(g, nxt, h->r1c.get(), s));
+    TMX_TRY(h->el.vcycle<double>(0, h->r1c.get(), h->e1c.get(), s));
+    TMX_HIP(p2_prolong_add(g, h->e1c.get(), cur, s));
+    for (int k = 0; k < kNu; ++k) {
+      double* out = k == kNu - 1 ? z : nxt;
+      TMX_HIP(p2_smooth(g, h->W2pc.get(), h->D2.get(), cur, r, out, kOmega, s));
+      std::swap(cur, nxt);
+    }
+    TMX_HIP(dot_t<double>(n, r, z, *rz, s));
+    return 0;
+  };
+  RedSlot gen = h->ws.region(3);
+  TMX_HIP(dot(n, h->F2m.get(), h->F2m.get(), gen, h->ws.sc() + kScBB, s));
+  TMX_TRY(h->ws.read(s, kScBB, 1));
+  const double bb = h->ws.host[kScBB];
+  st->levels = h->el.levels() + 1;
+  st->outer = 1;
+  if (!std::isfinite(bb)) {
+    set_error("non-finite right-hand side");
+    return TMX_ERR_NONFINITE;
+  }
+  if (bb == 0.0) {
+    TMX_HIP(fill_zero_bytes(u, sizeof(double) * n, s));
+    st->rel_residual = 0.0;
+    return TMX_OK;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int it = 0;
+  double rel = 0.0;
+  const int rc = minres_core(h->ws, n, h->F2m.get(), X, bf, apply_fn, pc_fn, prm.rtol,
+                             prm.max_iter, bb, &it, &rel, s, std::getenv("TMX_TRACE") != nullptr);
+  st->iterations = it;
+  st->rel_residual = rel;
+  if (rc == TMX_OK || rc == TMX_ERR_NOT_CONVERGED) TMX_HIP(copy_vec<double>(n, X, u, s));
+  TMX_HIP(hipStreamSynchronize(s));
+  st->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
 static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const tmx_solve_opts* o,
                             SolveStats* st) {
   TMX_TRY(need_penalty(h));
   TMX_HIP(hipEventRecord(h->ev_setup[0], h->stream));
   TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, h->stream));
   TMX_HIP(hipEventRecord(h->ev_setup[1], h->stream));
+  if (h->degree == 2) {
+    const int rc = p2_solve(h, xi, u, merge(h->el_prm, o), st);
+    float setup = 0.f;
+    if (hipEventElapsedTime(&setup, h->ev_setup[0], h->ev_setup[1]) != hipSuccess) setup = -1.f;
+    st->setup_ms = setup;
+    if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(ndofs(h), NAN, u, h->stream));
+    return rc;
+  }
   const SolveParams prm = merge(h->el_prm, o);
   const int rc = h->el.solve(h->ws, h->Fm.get(), u, prm, st, h->stream);
   float setup = 0.f;  // the solve has synchronised the stream past both events
   if (hipEventElapsedTime(&setup, h->ev_setup[0], h->ev_setup[1]) != hipSuccess) setup = -1.f;
   st->setup_ms = setup;
-  if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(2 * nverts(h), NAN, u, h->stream));
+  if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(ndofs(h), NAN, u, h->stream));
   return rc;
 }
 
@@ -330,7 +442,7 @@ static int read_dot(tmx_problem* h, size_t n, const double* a, const double* b, 
 extern "C" int tmx_compliance(tmx_problem* h, const double* u, double* out) {
   if (!h || !u || !out) return invalid("null argument");
   Scope scope(h);
-  return read_dot(h, 2 * nverts(h), h->F.get(), u, kScJ, out);
+  return read_dot(h, ndofs(h), load_F(h), u, kScJ, out);
 }
 
 extern "C" int tmx_integrate(tmx_problem* h, const double* values, double* out) {
@@ -343,6 +455,10 @@ extern "C" int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u
   if (!h || !xi || !u || !b) return invalid("null argument");
   Scope scope(h);
   TMX_TRY(need_penalty(h));
+  if (h->degree == 2) {
+    TMX_HIP(p2_sensitivity(h->g2, xi, u, h->p, h->m, b, h->stream));
+    return TMX_OK;
+  }
   TMX_HIP(elast_sensitivity(h->el.levels_[0].sh64, xi, u, h->p, h->m, h->lam, h->mu, h->hx, h->hy,
                             b, h->stream));
   return TMX_OK;
@@ -361,7 +477,7 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
   SolveStats st;
   if (rc == TMX_OK) rc = elasticity_solve(h, xiv, uv, opts, &st);
-  else TMX_HIP(fill(2 * nverts(h), NAN, uv, h->stream));
+  else TMX_HIP(fill(ndofs(h), NAN, uv, h->stream));
   export_stats(st, stats);
   if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
   double Jv = 0.0;
@@ -387,6 +503,15 @@ extern "C" int tmx_matvec(tmx_problem* h, const double* xi, const double* u, dou
     return invalid("precision must be TMX_FP64 or TMX_FP32");
   Scope scope(h);
   const size_t nv = nverts(h);
+  if (h->degree == 2) {
+    if (precision != TMX_FP64) {
+      set_error("the P2 operator is applied in FP64 only");
+      return TMX_ERR_UNSUPPORTED;
+    }
+    TMX_HIP(p2_weights(h->g2, xi, h->p, h->m, false, h->W2.get(), h->stream));
+    TMX_HIP(p2_apply(h->g2, h->W2.get(), u, y, h->stream));
+    return TMX_OK;
+  }
   TMX_TRY(h->el.set_fine_operator(xi, h->p, h->m, h->stream));
   if (precision == TMX_FP64) {
     TMX_TRY(h->el.apply<double>(u, y, nullptr, h->stream));
@@ -401,7 +526,7 @@ extern "C" int tmx_matvec(tmx_problem* h, const double* xi, const double* u, dou
 extern "C" int tmx_load_vector(tmx_problem* h, double* F) {
   if (!h || !F) return invalid("null argument");
   Scope scope(h);
-  TMX_HIP(copy_vec<double>(2 * nverts(h), h->F.get(), F, h->stream));
+  TMX_HIP(copy_vec<double>(ndofs(h), load_F(h), F, h->stream));
   return TMX_OK;
 }
 
@@ -455,6 +580,10 @@ extern "C" int tmx_precondition(tmx_problem* h, int which, int precision, const 
   if (which != 0 && which != 1) return invalid("which must be 0 (elasticity) or 1 (filter)");
   if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
   if (which == 0 && !xi) return invalid("xi is required for the elasticity preconditioner");
+  if (which == 0 && h->degree == 2) {
+    set_error("tmx_precondition applies the P1 elasticity V-cycle; not available for P2");
+    return TMX_ERR_UNSUPPORTED;
+  }
   if (which == 0) TMX_TRY(need_penalty(h));
   Scope scope(h);
   hipStream_t s = h->stream;
@@ -543,6 +672,10 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
   if (kernel < 0 || kernel > 2) return invalid("unknown kernel id");
   if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
   if (kernel < 2 && !h->el.has_operator_) return invalid("no elasticity operator yet: solve first");
+  if (kernel < 2 && h->degree == 2) {
+    set_error("tmx_bench_kernel times the P1 elasticity kernels; not available for P2");
+    return TMX_ERR_UNSUPPORTED;
+  }
   Scope scope(h);
   const size_t nv = nverts(h);
   hipStream_t s = h->stream;

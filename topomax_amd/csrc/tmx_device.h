@@ -128,4 +128,50 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
   return t;
 }
 
+// ---------------------------------------------------------------- load rules
+// Body force / traction rules of the reference (FEM_src/elasisity_problem.py:20-73) at a point,
+// with IEEE semantics (no contraction): the body force is value inside the open disc
+// |x - c| < r; a traction adds its value on the named side, |coordinate - centre| within
+// length/2 + DOLFIN_EPS (df.between).  Vertex and midpoint coordinates as dolfin computes them.
+static __device__ __noinline__ void body_value(const LoadDesc& d, double x, double y, double& fx,
+                                        double& fy) {
+#pragma clang fp contract(off)
+  fx = 0.0;
+  fy = 0.0;
+  if (!d.has_body) return;
+  const double dx = x - d.bc[0], dy = y - d.bc[1];
+  const double dx2 = __dmul_rn(dx, dx), dy2 = __dmul_rn(dy, dy);
+  const double dist = __dsqrt_rn(__dadd_rn(dx2, dy2));
+  if (dist < d.br) {
+    fx = d.bv[0];
+    fy = d.bv[1];
+  }
+}
+
+static __device__ __noinline__ void traction_value(const LoadDesc& d, double x, double y, double& fx,
+                                            double& fy) {
+#pragma clang fp contract(off)
+  const double eps = 3.0e-16;
+  fx = 0.0;
+  fy = 0.0;
+  for (int i = 0; i < d.ntr; ++i) {
+    bool on = false;
+    switch (d.tside[i]) {
+      case 0: on = (x == 0.0) && (y >= d.tlo[i] - eps) && (y <= d.thi[i] + eps); break;
+      case 1: on = (x == d.width) && (y >= d.tlo[i] - eps) && (y <= d.thi[i] + eps); break;
+      case 2: on = (y == d.height) && (x >= d.tlo[i] - eps) && (x <= d.thi[i] + eps); break;
+      default: on = (y == 0.0) && (x >= d.tlo[i] - eps) && (x <= d.thi[i] + eps); break;
+    }
+    if (on) {
+      fx = __dadd_rn(fx, d.tv[i][0]);
+      fy = __dadd_rn(fy, d.tv[i][1]);
+    }
+  }
+}
+
+static __device__ __forceinline__ double vx(const LoadDesc& d, int ix) { return __dadd_rn(0.0, __dmul_rn(d.ab, (double)ix)); }
+static __device__ __forceinline__ double vy(const LoadDesc& d, int iy) { return __dadd_rn(0.0, __dmul_rn(d.cd, (double)iy)); }
+static __device__ __forceinline__ double mid(double a, double b) { return __dadd_rn(__dmul_rn(0.5, a), __dmul_rn(0.5, b)); }
+
+
 }  // namespace tmx

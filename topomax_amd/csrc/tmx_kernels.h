@@ -267,4 +267,7 @@ hipError_t mr_update(size_t n, const double* sc, const MrSlots& q, double* v, co
                      double* w_out, const double* w_older, const double* w_newer, double* x,
                      hipStream_t s);
 
+// out = a - b
+hipError_t sub_vec(size_t n, const double* a, const double* b, double* out, hipStream_t s);
+
 }  // namespace tmx

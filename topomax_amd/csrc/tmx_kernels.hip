@@ -608,46 +608,6 @@ hipError_t elast_sensitivity(const SweepShape& sh, const double* xi, const doubl
 //   int_T N_vertex(j) l_i = |T|/30 (j = i), -|T|/60 (j != i)
 //   int_T N_edge(jk) l_i  = 2|T|/15 (i in {j,k}), |T|/15 (otherwise)
 //   int_e N_a phi_a = L/6, int_e N_b phi_a = 0, int_e N_mid phi_a = L/3 (edge e = (a,b))
-__device__ __noinline__ void body_value(const LoadDesc& d, double x, double y, double& fx,
-                                        double& fy) {
-#pragma clang fp contract(off)
-  fx = 0.0;
-  fy = 0.0;
-  if (!d.has_body) return;
-  const double dx = x - d.bc[0], dy = y - d.bc[1];
-  const double dx2 = __dmul_rn(dx, dx), dy2 = __dmul_rn(dy, dy);
-  const double dist = __dsqrt_rn(__dadd_rn(dx2, dy2));
-  if (dist < d.br) {
-    fx = d.bv[0];
-    fy = d.bv[1];
-  }
-}
-
-__device__ __noinline__ void traction_value(const LoadDesc& d, double x, double y, double& fx,
-                                            double& fy) {
-#pragma clang fp contract(off)
-  const double eps = 3.0e-16;
-  fx = 0.0;
-  fy = 0.0;
-  for (int i = 0; i < d.ntr; ++i) {
-    bool on = false;
-    switch (d.tside[i]) {
-      case 0: on = (x == 0.0) && (y >= d.tlo[i] - eps) && (y <= d.thi[i] + eps); break;
-      case 1: on = (x == d.width) && (y >= d.tlo[i] - eps) && (y <= d.thi[i] + eps); break;
-      case 2: on = (y == d.height) && (x >= d.tlo[i] - eps) && (x <= d.thi[i] + eps); break;
-      default: on = (y == 0.0) && (x >= d.tlo[i] - eps) && (x <= d.thi[i] + eps); break;
-    }
-    if (on) {
-      fx = __dadd_rn(fx, d.tv[i][0]);
-      fy = __dadd_rn(fy, d.tv[i][1]);
-    }
-  }
-}
-
-__device__ __forceinline__ double vx(const LoadDesc& d, int ix) { return __dadd_rn(0.0, __dmul_rn(d.ab, (double)ix)); }
-__device__ __forceinline__ double vy(const LoadDesc& d, int iy) { return __dadd_rn(0.0, __dmul_rn(d.cd, (double)iy)); }
-__device__ __forceinline__ double mid(double a, double b) { return __dadd_rn(__dmul_rn(0.5, a), __dmul_rn(0.5, b)); }
-
 __global__ void k_assemble_load(LoadDesc d, double* __restrict__ F) {
 #pragma clang fp contract(off)
   const int nx = d.nx, ny = d.ny;
@@ -2348,6 +2308,16 @@ hipError_t mr_update(size_t n, const double* sc, const MrSlots& q, double* v, co
                      double* w_out, const double* w_older, const double* w_newer, double* x,
                      hipStream_t s) {
   k_mr_update<<<vec_grid(n), kBlock, 0, s>>>(n, sc, q, v, z, w_out, w_older, w_newer, x);
+  return hipGetLastError();
+}
+
+__global__ void k_sub_vec(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                          double* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+hipError_t sub_vec(size_t n, const double* a, const double* b, double* out, hipStream_t s) {
+  k_sub_vec<<<vec_grid(n), kBlock, 0, s>>>(n, a, b, out);
   return hipGetLastError();
 }
 

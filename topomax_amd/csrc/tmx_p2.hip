@@ -1,0 +1,553 @@
+// P2 displacement element kernels (see tmx_p2.h for the lattice layout).
+//
+// The element operator is written in barycentric moments.  With l_k the barycentric
+// coordinates of a triangle and d_k = grad l_k, every P2 basis gradient is linear in l:
+// grad phi_a = sum_l l_l g_al with g_kl = (4 delta_kl - 1) d_k for vertex k and
+// g_ml = 4 (delta_li d_j + delta_lj d_i) for the midpoint m of edge (i, j).  So
+// grad u = sum_l l_l G_l, sigma(u) = sum_l l_l S_l, and the element force on node a is
+// y_a = int r(xi) sigma(u) grad phi_a = sum_k T_k g_ak with T_k = sum_l W_lk S_l and
+// W_lk = int_T r(xi_h) l_l l_k.  For integer p the moments are exact:
+//   int_T (a.l)^p l^beta = 2|T| beta! p! / (|beta| + p + 2)! h_p(a with a_i repeated beta_i
+//   more times),
+// h_p the complete homogeneous symmetric polynomial (the Dirichlet moment identity).
+// Every kernel gathers per node over the triangles that contain it, so results are
+// deterministic.
+#include <cmath>
+
+#include "tmx_device.h"
+#include "tmx_p2.h"
+
+namespace tmx {
+
+namespace {
+
+constexpr int kMaxP = 8;
+
+// complete homogeneous symmetric polynomial of degree p (0 <= p <= kMaxP) in N values
+template <int N>
+__device__ __forceinline__ double hsym_multi(const double (&x)[N], int p) {
+  double h[kMaxP + 1];
+#pragma unroll
+  for (int k = 0; k <= kMaxP; ++k) h[k] = k == 0 ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int k = 1; k <= kMaxP; ++k) h[k] += x[i] * h[k - 1];
+  double r = h[0];
+#pragma unroll
+  for (int k = 1; k <= kMaxP; ++k)
+    if (k == p) r = h[k];
+  return r;
+}
+
+__device__ __forceinline__ double factorial(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return f;
+}
+
+// local vertex offsets (cells) of triangle t: A = (v0, v1, v3), B = (v0, v2, v3)
+__device__ __forceinline__ void tri_vertex(int t, int k, int& ox, int& oy) {
+  // A: (0,0) (1,0) (1,1); B: (0,0) (0,1) (1,1)
+  ox = k == 0 ? 0 : (k == 2 ? 1 : (t == 0 ? 1 : 0));
+  oy = k == 0 ? 0 : (k == 2 ? 1 : (t == 0 ? 0 : 1));
+}
+
+// lattice coordinates of the 6 local nodes of triangle t in cell (cx, cy)
+__device__ __forceinline__ void tri_nodes(int t, int cx, int cy, int (&I)[6], int (&J)[6]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int ox, oy;
+    tri_vertex(t, k, ox, oy);
+    I[k] = 2 * (cx + ox);
+    J[k] = 2 * (cy + oy);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {  // midpoint opposite vertex k
+    const int i = (k + 1) % 3, j = (k + 2) % 3;
+    I[3 + k] = (I[i] + I[j]) / 2;
+    J[3 + k] = (J[i] + J[j]) / 2;
+  }
+}
+
+// barycentric gradients d_k of triangle t (units 1/hx, 1/hy applied)
+__device__ __forceinline__ void tri_grads(int t, double hx, double hy, double (&d)[3][2]) {
+  if (t == 0) {
+    d[0][0] = -1.0 / hx; d[0][1] = 0.0;
+    d[1][0] = 1.0 / hx;  d[1][1] = -1.0 / hy;
+    d[2][0] = 0.0;       d[2][1] = 1.0 / hy;
+  } else {
+    d[0][0] = 0.0;       d[0][1] = -1.0 / hy;
+    d[1][0] = -1.0 / hx; d[1][1] = 1.0 / hy;
+    d[2][0] = 1.0 / hx;  d[2][1] = 0.0;
+  }
+}
+
+__device__ __forceinline__ double wlk(const double (&W)[6], int l, int k) {
+  const int a = l < k ? l : k, b = l < k ? k : l;
+  // packed 00 01 02 11 12 22
+  const int idx = a == 0 ? b : (a == 1 ? 2 + b : 5);
+  return W[idx];
+}
+
+// strain-stress coefficients S_l (xx, xy, yy) of sigma(u) = sum_l l_l S_l, and eps_l
+__device__ __forceinline__ void stress_coeffs(const double (&d)[3][2], const double (&u)[6][2],
+                                              double lam, double mu, double (&S)[3][3],
+                                              double (&E)[3][3]) {
+  double U0[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) U0[c][j] += u[k][c] * d[k][j];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    double G[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) G[c][j] = 4.0 * u[l][c] * d[l][j] - U0[c][j];
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      if (jj == l) continue;
+      const int m = 3 + (3 - l - jj);  // midpoint of edge (l, jj)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) G[c][j] += 4.0 * u[m][c] * d[jj][j];
+    }
+    const double exx = G[0][0], eyy = G[1][1], exy = 0.5 * (G[0][1] + G[1][0]);
+    const double tr = exx + eyy;
+    E[l][0] = exx;
+    E[l][1] = exy;
+    E[l][2] = eyy;
+    S[l][0] = lam * tr + 2.0 * mu * exx;
+    S[l][1] = 2.0 * mu * exy;
+    S[l][2] = lam * tr + 2.0 * mu * eyy;
+  }
+}
+
+// force of one triangle on its local node a (0..5) for local displacements u
+__device__ __forceinline__ void tri_force(const double (&W)[6], const double (&d)[3][2],
+                                          const double (&u)[6][2], double lam, double mu, int a,
+                                          double (&y)[2]) {
+  double S[3][3], E[3][3];
+  stress_coeffs(d, u, lam, mu, S, E);
+  double T[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      T[k][e] = wlk(W, 0, k) * S[0][e] + wlk(W, 1, k) * S[1][e] + wlk(W, 2, k) * S[2][e];
+  auto tv = [&](int k, int dk, double& ox, double& oy) {  // T_k d_dk
+    ox = T[k][0] * d[dk][0] + T[k][1] * d[dk][1];
+    oy = T[k][1] * d[dk][0] + T[k][2] * d[dk][1];
+  };
+  if (a < 3) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double ox, oy;
+      tv(k, a, ox, oy);
+      const double c = k == a ? 3.0 : -1.0;  // 4 delta - 1
+      sx += c * ox;
+      sy += c * oy;
+    }
+    y[0] = sx;
+    y[1] = sy;
+  } else {
+    const int k = a - 3, i = (k + 1) % 3, j = (k + 2) % 3;
+    double ax, ay, bx, by;
+    tv(i, j, ax, ay);
+    tv(j, i, bx, by);
+    y[0] = 4.0 * (ax + bx);
+    y[1] = 4.0 * (ay + by);
+  }
+}
+
+__device__ __forceinline__ bool p2_fixed(unsigned fixed, int I, int J, int nX, int nY) {
+  return ((fixed & kLeft) && I == 0) || ((fixed & kRight) && I == nX - 1) ||
+         ((fixed & kBottom) && J == 0) || ((fixed & kTop) && J == nY - 1);
+}
+
+__device__ __forceinline__ void load_W(const double* __restrict__ W, int nx, int cx, int cy, int t,
+                                       double (&w)[6]) {
+  const double* src = W + 12 * ((size_t)cy * nx + cx) + 6 * t;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) w[e] = src[e];
+}
+
+// visit every triangle containing lattice node (I, J): f(t, cx, cy, local index, nodes I, J)
+template <class F>
+__device__ __forceinline__ void for_each_triangle(const P2Geom& g, int I, int J, F&& f) {
+  const int cx0 = (I & 1) ? (I - 1) / 2 : I / 2 - 1, cx1 = (I & 1) ? (I - 1) / 2 : I / 2;
+  const int cy0 = (J & 1) ? (J - 1) / 2 : J / 2 - 1, cy1 = (J & 1) ? (J - 1) / 2 : J / 2;
+  for (int cy = cy0; cy <= cy1; ++cy) {
+    if (cy < 0 || cy >= g.ny) continue;
+    for (int cx = cx0; cx <= cx1; ++cx) {
+      if (cx < 0 || cx >= g.nx) continue;
+      for (int t = 0; t < 2; ++t) {
+        int NI[6], NJ[6];
+        tri_nodes(t, cx, cy, NI, NJ);
+        int a = -1;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+          if (NI[q] == I && NJ[q] == J) a = q;
+        if (a >= 0) f(t, cx, cy, a, NI, NJ);
+      }
+    }
+  }
+}
+
+// (K u)_a at lattice node (I, J) with fixed nodes of u taken as zero
+__device__ __forceinline__ void node_force(const P2Geom& g, const double* __restrict__ W,
+                                           const double* __restrict__ u, int I, int J,
+                                           double (&acc)[2]) {
+  const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
+  acc[0] = acc[1] = 0.0;
+  for_each_triangle(g, I, J, [&](int t, int cx, int cy, int a, const int (&NI)[6],
+                                 const int (&NJ)[6]) {
+    double w[6], d[3][2], ul[6][2], y[2];
+    load_W(W, g.nx, cx, cy, t, w);
+    tri_grads(t, g.hx, g.hy, d);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const size_t n = (size_t)NJ[q] * nX + NI[q];
+      const bool fx = p2_fixed(g.fixed, NI[q], NJ[q], nX, nY);
+      ul[q][0] = fx ? 0.0 : u[2 * n];
+      ul[q][1] = fx ? 0.0 : u[2 * n + 1];
+    }
+    tri_force(w, d, ul, g.lam, g.mu, a, y);
+    acc[0] += y[0];
+    acc[1] += y[1];
+  });
+}
+
+__global__ void k_p2_weights(P2Geom g, const double* __restrict__ xi, int p, double m,
+                             int positive, double* __restrict__ W) {
+  const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= (size_t)g.nx * g.ny) return;
+  const int cx = (int)(cell % g.nx), cy = (int)(cell / g.nx);
+  const double area = 0.5 * g.hx * g.hy;
+  const double cp = (1.0 - m) * 2.0 * area * factorial(p) / factorial(p + 4);
+  for (int t = 0; t < 2; ++t) {
+    double a[3];
+    for (int k = 0; k < 3; ++k) {
+      int ox, oy;
+      tri_vertex(t, k, ox, oy);
+      const double v = xi[(size_t)(cy + oy) * (g.nx + 1) + cx + ox];
+      a[k] = positive ? fmax(v, 0.0) : v;
+    }
+    double* out = W + 12 * cell + 6 * t;
+    int e = 0;
+    for (int l = 0; l < 3; ++l)
+      for (int k = l; k < 3; ++k, ++e) {
+        const double beta_f = l == k ? 2.0 : 1.0;
+        const double xs[5] = {a[0], a[1], a[2], a[l], a[k]};
+        out[e] = m * area * beta_f / 12.0 + cp * beta_f * hsym_multi<5>(xs, p);
+      }
+  }
+}
+
+__global__ void k_p2_apply(P2Geom g, const double* __restrict__ W, const double* __restrict__ u,
+                           const double* __restrict__ b, double* __restrict__ y, int residual) {
+  const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)nX * nY) return;
+  const int I = (int)(n % nX), J = (int)(n / nX);
+  double f[2];
+  if (p2_fixed(g.fixed, I, J, nX, nY)) {
+    f[0] = u[2 * n];
+    f[1] = u[2 * n + 1];
+  } else {
+    node_force(g, W, u, I, J, f);
+  }
+  if (residual) {
+    y[2 * n] = b[2 * n] - f[0];
+    y[2 * n + 1] = b[2 * n + 1] - f[1];
+  } else {
+    y[2 * n] = f[0];
+    y[2 * n + 1] = f[1];
+  }
+}
+
+__global__ void k_p2_diag(P2Geom g, const double* __restrict__ W, double* __restrict__ D) {
+  const int nX = 2 * g.nx + 1;
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)nX * (2 * g.ny + 1)) return;
+  const int I = (int)(n % nX), J = (int)(n / nX);
+  double dxx = 0.0, dxy = 0.0, dyy = 0.0;
+  for_each_triangle(g, I, J, [&](int t, int cx, int cy, int a, const int (&)[6],
+                                 const int (&)[6]) {
+    double w[6], d[3][2], ul[6][2], y[2];
+    load_W(W, g.nx, cx, cy, t, w);
+    tri_grads(t, g.hx, g.hy, d);
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ul[q][0] = ul[q][1] = 0.0;
+      ul[a][c] = 1.0;
+      tri_force(w, d, ul, g.lam, g.mu, a, y);
+      if (c == 0) {
+        dxx += y[0];
+        dxy += y[1];
+      } else {
+        dyy += y[1];
+      }
+    }
+  });
+  D[3 * n] = dxx;
+  D[3 * n + 1] = dxy;
+  D[3 * n + 2] = dyy;
+}
+
+__global__ void k_p2_smooth(P2Geom g, const double* __restrict__ W, const double* __restrict__ D,
+                            const double* __restrict__ x, const double* __restrict__ b,
+                            double* __restrict__ y, double omega) {
+  const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)nX * nY) return;
+  const int I = (int)(n % nX), J = (int)(n / nX);
+  if (p2_fixed(g.fixed, I, J, nX, nY)) {
+    y[2 * n] = y[2 * n + 1] = 0.0;
+    return;
+  }
+  double f[2] = {0.0, 0.0};
+  if (x) node_force(g, W, x, I, J, f);
+  const double rx = b[2 * n] - f[0], ry = b[2 * n + 1] - f[1];
+  const double a = D[3 * n], c = D[3 * n + 1], e = D[3 * n + 2];
+  const double inv = 1.0 / (a * e - c * c);
+  const double x0 = x ? x[2 * n] : 0.0, x1 = x ? x[2 * n + 1] : 0.0;
+  y[2 * n] = x0 + omega * (e * rx - c * ry) * inv;
+  y[2 * n + 1] = x1 + omega * (a * ry - c * rx) * inv;
+}
+
+// r1 at P1 vertex (i, j) = r2(2i, 2j) + 1/2 (the six adjacent midpoints)
+__global__ void k_p2_restrict(P2Geom g, const double* __restrict__ r2, double* __restrict__ r1) {
+  const int nx = g.nx, ny = g.ny, nX = 2 * nx + 1;
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)(nx + 1) * (ny + 1)) return;
+  const int i = (int)(v % (nx + 1)), j = (int)(v / (nx + 1));
+  double sx = 0.0, sy = 0.0;
+  if (!node_fixed(g.fixed, i, j, nx, ny)) {
+    const int I = 2 * i, J = 2 * j;
+    sx = r2[2 * ((size_t)J * nX + I)];
+    sy = r2[2 * ((size_t)J * nX + I) + 1];
+    const int off[6][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, -1}};
+    for (int q = 0; q < 6; ++q) {
+      const int mI = I + off[q][0], mJ = J + off[q][1];
+      if (mI < 0 || mJ < 0 || mI > 2 * nx || mJ > 2 * ny) continue;
+      const size_t m = (size_t)mJ * nX + mI;
+      sx += 0.5 * r2[2 * m];
+      sy += 0.5 * r2[2 * m + 1];
+    }
+  }
+  r1[2 * v] = sx;
+  r1[2 * v + 1] = sy;
+}
+
+__global__ void k_p2_prolong_add(P2Geom g, const double* __restrict__ e1, double* __restrict__ x2) {
+  const int nx = g.nx, nX = 2 * nx + 1, nY = 2 * g.ny + 1;
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)nX * nY) return;
+  const int I = (int)(n % nX), J = (int)(n / nX);
+  if (p2_fixed(g.fixed, I, J, nX, nY)) return;
+  // the two P1 vertices whose average this node is (the same vertex twice at a vertex)
+  const int i0 = I / 2, j0 = J / 2;
+  const int i1 = (I + 1) / 2, j1 = (J + 1) / 2;
+  const size_t a = (size_t)j0 * (nx + 1) + i0, b = (size_t)j1 * (nx + 1) + i1;
+  x2[2 * n] += 0.5 * (e1[2 * a] + e1[2 * b]);
+  x2[2 * n + 1] += 0.5 * (e1[2 * a + 1] + e1[2 * b + 1]);
+}
+
+// sensitivity at P1 vertex v: sum over its triangles of
+//   -p (1-m) sum_lk Q_lk int (a.l)^(p-1) l_l l_k l_i,  Q_lk = S_l : eps_k
+__global__ void k_p2_sensitivity(P2Geom g, const double* __restrict__ xi,
+                                 const double* __restrict__ u, int p, double m,
+                                 double* __restrict__ out) {
+  const int nx = g.nx, ny = g.ny, nX = 2 * nx + 1;
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)(nx + 1) * (ny + 1)) return;
+  const int iv = (int)(v % (nx + 1)), jv = (int)(v / (nx + 1));
+  const double area = 0.5 * g.hx * g.hy;
+  const double coef = -(double)p * (1.0 - m) * 2.0 * area * factorial(p - 1) / factorial(p + 4);
+  double acc = 0.0;
+  for_each_triangle(g, 2 * iv, 2 * jv, [&](int t, int cx, int cy, int li, const int (&NI)[6],
+                                           const int (&NJ)[6]) {
+    double d[3][2], ul[6][2], a[3];
+    tri_grads(t, g.hx, g.hy, d);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const size_t n = (size_t)NJ[q] * nX + NI[q];
+      ul[q][0] = u[2 * n];
+      ul[q][1] = u[2 * n + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[k] = xi[(size_t)(NJ[k] / 2) * (nx + 1) + NI[k] / 2];
+    double S[3][3], E[3][3];
+    stress_coeffs(d, ul, g.lam, g.mu, S, E);
+    double s = 0.0;
+    for (int l = 0; l < 3; ++l)
+      for (int k = 0; k < 3; ++k) {
+        const double Q = S[l][0] * E[k][0] + 2.0 * S[l][1] * E[k][1] + S[l][2] * E[k][2];
+        const int c0 = (l == 0) + (k == 0) + (li == 0), c1 = (l == 1) + (k == 1) + (li == 1),
+                  c2 = (l == 2) + (k == 2) + (li == 2);
+        const double bf = factorial(c0) * factorial(c1) * factorial(c2);
+        const double xs[6] = {a[0], a[1], a[2], a[l], a[k], a[li]};
+        s += Q * bf * hsym_multi<6>(xs, p - 1);
+      }
+    acc += coef * s;
+  });
+  out[v] = acc;
+}
+
+// ---------------------------------------------------------------- loads
+__device__ __forceinline__ double lat_x(const LoadDesc& d, int I) {
+  return (I & 1) ? mid(vx(d, I / 2), vx(d, I / 2 + 1)) : vx(d, I / 2);
+}
+__device__ __forceinline__ double lat_y(const LoadDesc& d, int J) {
+  return (J & 1) ? mid(vy(d, J / 2), vy(d, J / 2 + 1)) : vy(d, J / 2);
+}
+
+// P2 mass matrix * 180 / |T| (local order: vertices, then midpoints opposite vertex 0, 1, 2)
+__constant__ double c_p2_mass[6][6] = {{6, -1, -1, -4, 0, 0},   {-1, 6, -1, 0, -4, 0},
+                                       {-1, -1, 6, 0, 0, -4},   {-4, 0, 0, 32, 16, 16},
+                                       {0, -4, 0, 16, 32, 16},  {0, 0, -4, 16, 16, 32}};
+
+__global__ void k_p2_load(LoadDesc d, double* __restrict__ F) {
+  const int nx = d.nx, ny = d.ny, nX = 2 * nx + 1, nY = 2 * ny + 1;
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)nX * nY) return;
+  const int I = (int)(n % nX), J = (int)(n / nX);
+  double sx = 0.0, sy = 0.0;
+  if (d.has_body) {
+    P2Geom g{nx, ny, d.ab, d.cd, 0.0, 0.0, 0u};
+    for_each_triangle(g, I, J, [&](int t, int cx, int cy, int a, const int (&NI)[6],
+                                   const int (&NJ)[6]) {
+      int px[3], py[3];
+      for (int k = 0; k < 3; ++k) {
+        px[k] = NI[k] / 2;
+        py[k] = NJ[k] / 2;
+      }
+      const double area = 0.5 * fabs((vx(d, px[1]) - vx(d, px[0])) * (vy(d, py[2]) - vy(d, py[0])) -
+                                     (vx(d, px[2]) - vx(d, px[0])) * (vy(d, py[1]) - vy(d, py[0])));
+      for (int q = 0; q < 6; ++q) {
+        double fx, fy;
+        body_value(d, lat_x(d, NI[q]), lat_y(d, NJ[q]), fx, fy);
+        const double w = c_p2_mass[a][q] * area / 180.0;
+        sx += w * fx;
+        sy += w * fy;
+      }
+    });
+  }
+  if (d.ntr > 0 && (I == 0 || I == nX - 1 || J == 0 || J == nY - 1)) {
+    // boundary edges (lattice end nodes e0, e1 even; midpoint between) containing this node
+    auto edge = [&](int I0, int J0, int I1, int J1) {
+      const int IM = (I0 + I1) / 2, JM = (J0 + J1) / 2;
+      double t0x, t0y, t1x, t1y, tmx, tmy;
+      traction_value(d, lat_x(d, I0), lat_y(d, J0), t0x, t0y);
+      traction_value(d, lat_x(d, I1), lat_y(d, J1), t1x, t1y);
+      traction_value(d, lat_x(d, IM), lat_y(d, JM), tmx, tmy);
+      const double len = (I0 != I1) ? fabs(lat_x(d, I1) - lat_x(d, I0))
+                                    : fabs(lat_y(d, J1) - lat_y(d, J0));
+      const double s = len / 30.0;
+      if (I == I0 && J == J0) {
+        sx += s * (4 * t0x - t1x + 2 * tmx);
+        sy += s * (4 * t0y - t1y + 2 * tmy);
+      } else if (I == I1 && J == J1) {
+        sx += s * (-t0x + 4 * t1x + 2 * tmx);
+        sy += s * (-t0y + 4 * t1y + 2 * tmy);
+      } else {
+        sx += s * (2 * t0x + 2 * t1x + 16 * tmx);
+        sy += s * (2 * t0y + 2 * t1y + 16 * tmy);
+      }
+    };
+    if (J == 0 || J == nY - 1) {  // horizontal boundary: edges [2k, 2k+2] along I
+      if (I & 1) {
+        edge(I - 1, J, I + 1, J);
+      } else {
+        if (I > 0) edge(I - 2, J, I, J);
+        if (I < nX - 1) edge(I, J, I + 2, J);
+      }
+    }
+    if (I == 0 || I == nX - 1) {  // vertical boundary
+      if (J & 1) {
+        edge(I, J - 1, I, J + 1);
+      } else {
+        if (J > 0) edge(I, J - 2, I, J);
+        if (J < nY - 1) edge(I, J, I, J + 2);
+      }
+    }
+  }
+  F[2 * n] = sx;
+  F[2 * n + 1] = sy;
+}
+
+__global__ void k_p2_mask(P2Geom g, double* __restrict__ v) {
+  const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)nX * nY) return;
+  if (p2_fixed(g.fixed, (int)(n % nX), (int)(n / nX), nX, nY)) v[2 * n] = v[2 * n + 1] = 0.0;
+}
+
+unsigned grid_for(size_t n) { return (unsigned)std::max<size_t>(1, (n + 255) / 256); }
+
+}  // namespace
+
+hipError_t p2_weights(const P2Geom& g, const double* xi, int p, double m, bool positive_part,
+                      double* W, hipStream_t s) {
+  if (p < 1 || p > kMaxP) return hipErrorInvalidValue;
+  k_p2_weights<<<grid_for((size_t)g.nx * g.ny), 256, 0, s>>>(g, xi, p, m, positive_part ? 1 : 0, W);
+  return hipGetLastError();
+}
+
+hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s) {
+  k_p2_apply<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, u, nullptr, y, 0);
+  return hipGetLastError();
+}
+
+hipError_t p2_residual(const P2Geom& g, const double* W, const double* x, const double* b,
+                       double* r, hipStream_t s) {
+  k_p2_apply<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, x, b, r, 1);
+  return hipGetLastError();
+}
+
+hipError_t p2_diag(const P2Geom& g, const double* W, double* D, hipStream_t s) {
+  k_p2_diag<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, D);
+  return hipGetLastError();
+}
+
+hipError_t p2_smooth(const P2Geom& g, const double* W, const double* D, const double* x,
+                     const double* b, double* y, double omega, hipStream_t s) {
+  k_p2_smooth<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, D, x, b, y, omega);
+  return hipGetLastError();
+}
+
+hipError_t p2_restrict(const P2Geom& g, const double* r2, double* r1, hipStream_t s) {
+  k_p2_restrict<<<grid_for((size_t)(g.nx + 1) * (g.ny + 1)), 256, 0, s>>>(g, r2, r1);
+  return hipGetLastError();
+}
+
+hipError_t p2_prolong_add(const P2Geom& g, const double* e1, double* x2, hipStream_t s) {
+  k_p2_prolong_add<<<grid_for(g.nodes()), 256, 0, s>>>(g, e1, x2);
+  return hipGetLastError();
+}
+
+hipError_t p2_assemble_load(const LoadDesc& d, double* F, hipStream_t s) {
+  k_p2_load<<<grid_for((size_t)(2 * d.nx + 1) * (2 * d.ny + 1)), 256, 0, s>>>(d, F);
+  return hipGetLastError();
+}
+
+hipError_t p2_mask_fixed(const P2Geom& g, double* v, hipStream_t s) {
+  k_p2_mask<<<grid_for(g.nodes()), 256, 0, s>>>(g, v);
+  return hipGetLastError();
+}
+
+hipError_t p2_sensitivity(const P2Geom& g, const double* xi, const double* u, int p, double m,
+                          double* b, hipStream_t s) {
+  if (p < 1 || p > kMaxP) return hipErrorInvalidValue;
+  k_p2_sensitivity<<<grid_for((size_t)(g.nx + 1) * (g.ny + 1)), 256, 0, s>>>(g, xi, u, p, m, b);
+  return hipGetLastError();
+}
+
+}  // namespace tmx

@@ -5,6 +5,7 @@
 
 #include <memory>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "tmx_kernels.h"
@@ -127,6 +128,19 @@ struct SolveStats {
   int iterations = 0, outer = 0, levels = 0;
   double rel_residual = 0, setup_ms = 0, solve_ms = 0;
 };
+
+// Preconditioned MINRES (FP64) on A x = b from x0 = 0, restarted on the true residual until
+// |b - A x| <= rtol |b|.  apply(x, y): y = A x.  precond(r, z, rz): z = M^-1 r with M SPD and
+// the partials of r.z in rz.  bufs: eight vectors of n doubles (two residuals, two work
+// vectors, the Lanczos vector and three directions); the result is left in x.
+struct MinresBufs {
+  double *r1, *r2, *y, *z, *v, *w0, *w1, *w2;
+};
+int minres_core(Workspace& ws, size_t n, const double* b, double* x, MinresBufs bufs,
+                const std::function<int(const double*, double*)>& apply,
+                const std::function<int(const double*, double*, RedSlot*)>& precond,
+                double rtol, int max_iter, double bnorm2, int* iters, double* rel,
+                hipStream_t s, bool trace);
 
 // Geometric multigrid + PCG for one operator family on one grid.
 class Hierarchy {

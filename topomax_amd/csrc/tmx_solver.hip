@@ -687,30 +687,24 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
 // MINRES scalar slots (after the PCG slots)
 constexpr MrSlots kMr = {30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45};
 
-int Hierarchy::minres(Workspace& ws, const double* b, double rtol, int max_iter, double bnorm2,
-                      int* iters, double* rel, hipStream_t s) {
+int minres_core(Workspace& ws, size_t nv, const double* b, double* X, MinresBufs bf,
+                const std::function<int(const double*, double*)>& apply,
+                const std::function<int(const double*, double*, RedSlot*)>& precond,
+                double rtol, int max_iter, double bnorm2, int* iters, double* rel,
+                hipStream_t s, bool trace) {
   constexpr int kMaxRestarts = 8;
-  const size_t nv = n();
   double* sc = ws.sc();
-  double* X = x64_.get();
-  double* R1 = r64_.get();
-  double* R2 = pb64_.get();
-  double* Y = q64_.get();
-  double* Z = z64_.get();
-  double* V = p64_.get();
-  double* W[3] = {mrw_[0].get(), mrw_[1].get(), mrw_[2].get()};
-  if (!W[0]) {
-    set_error("internal: MINRES buffers exist for elasticity only");
-    return -1;
-  }
+  double *R1 = bf.r1, *R2 = bf.r2, *Y = bf.y, *Z = bf.z, *V = bf.v;
+  double* W[3] = {bf.w0, bf.w1, bf.w2};
   RedSlot part = ws.region(0), rz = ws.region(2), gen = ws.region(3);
   volatile double* ring = ws.host + Workspace::kScalars + 8;
   double* ring_dev = ws.host_dev + Workspace::kScalars + 8;
   TMX_HIP(fill_zero_bytes(X, sizeof(double) * nv, s));
-  int total = 0, rc = -3;
-  double relres = 1.0;
+  int total = 0, rc = -3, stalled = 0;
+  double relres = 1.0, best = 1.0;
   for (int restart = 0;; ++restart) {
-    TMX_TRY(residual<double>(X, b, R1, s));
+    TMX_TRY(apply(X, Y));
+    TMX_HIP(sub_vec(nv, b, Y, R1, s));  // r1 = b - A x
     TMX_HIP(dot(nv, R1, R1, gen, sc + kScTmp, s));
     TMX_TRY(ws.read(s, kScTmp, 1));
     relres = std::sqrt(std::max(ws.host[kScTmp], 0.0) / bnorm2);
@@ -722,21 +716,29 @@ int Hierarchy::minres(Workspace& ws, const double* b, double rtol, int max_iter,
       rc = 0;
       break;
     }
+    if (trace) std::fprintf(stderr, "[tmx] minres restart %d: |r|/|b| = %.3e after %d its\n",
+                            restart, relres, total);
+    // the true residual has a round-off floor: stop once two restarts gain less than 2x
+    if (restart > 0) {
+      stalled = relres > 0.5 * best ? stalled + 1 : 0;
+      best = std::min(best, relres);
+      if (stalled >= 2) break;
+    }
     if (restart >= kMaxRestarts || total >= max_iter) break;
     // reduction asked of this run, measured in the preconditioner's norm
     const double tol = 0.5 * rtol / relres;
-    TMX_TRY(vcycle<double>(0, R1, Z, s, &rz));
+    TMX_TRY(precond(R1, Z, &rz));
     TMX_HIP(mr_start(rz, sc, kMr, tol, s));
     TMX_HIP(copy_vec<double>(nv, R1, R2, s));
     TMX_HIP(fill_zero_bytes(W[1], sizeof(double) * nv, s));
     TMX_HIP(fill_zero_bytes(W[2], sizeof(double) * nv, s));
     TMX_HIP(mr_scale(nv, sc, kMr, Z, V, s));
     for (; total < max_iter; ++total) {
-      TMX_TRY(apply<double>(V, Y, nullptr, s));
+      TMX_TRY(apply(V, Y));
       TMX_HIP(mr_lanczos(nv, sc, kMr, V, R1, R2, Y, part, s));
       std::swap(R1, R2);  // r1 <- r2
       std::swap(R2, Y);   // r2 <- y (the new Lanczos vector); y <- the old r1 (free)
-      TMX_TRY(vcycle<double>(0, R2, Z, s, &rz));
+      TMX_TRY(precond(R2, Z, &rz));
       TMX_HIP(mr_step(rz, sc, kMr, ring_dev, s));
       TMX_HIP(mr_update(nv, sc, kMr, V, Z, W[0], W[1], W[2], X, s));
       std::rotate(W, W + 1, W + 3);
@@ -758,10 +760,26 @@ int Hierarchy::minres(Workspace& ws, const double* b, double rtol, int max_iter,
   }
   *iters = total;
   *rel = relres;
-  if (g_trace)
+  if (trace)
     std::fprintf(stderr, "[tmx] minres n=%zu: %d its, rel %.3e, rc %d\n", nv, total, relres, rc);
-  if (rc != 0) set_error("MINRES did not reach the tolerance within the iteration limit");
+  if (rc != 0)
+    set_error("MINRES stopped at relative residual " + std::to_string(relres) +
+              " above the tolerance (iteration limit or round-off floor)");
   return rc;
+}
+
+int Hierarchy::minres(Workspace& ws, const double* b, double rtol, int max_iter, double bnorm2,
+                      int* iters, double* rel, hipStream_t s) {
+  if (!mrw_[0].get()) {
+    set_error("internal: MINRES buffers exist for elasticity only");
+    return -1;
+  }
+  MinresBufs bf{r64_.get(), pb64_.get(), q64_.get(), z64_.get(), p64_.get(),
+                mrw_[0].get(), mrw_[1].get(), mrw_[2].get()};
+  auto apply_fn = [&](const double* x, double* y) { return apply<double>(x, y, nullptr, s); };
+  auto pc_fn = [&](const double* r, double* z, RedSlot* rz) { return vcycle<double>(0, r, z, s, rz); };
+  return minres_core(ws, n(), b, x64_.get(), bf, apply_fn, pc_fn, rtol, max_iter, bnorm2, iters,
+                     rel, s, g_trace);
 }
 
 static double ms_since(std::chrono::steady_clock::time_point t0) {

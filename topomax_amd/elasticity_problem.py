@@ -145,7 +145,7 @@ class _Handle:
 
 
 def _make_desc(mesh: StructuredMesh, params: ElasticityParameters | None, filter_radius: float,
-               precision: str) -> _lib.ProblemDesc:
+               precision: str, degree: int = 1) -> _lib.ProblemDesc:
     d = _lib.ProblemDesc()
     d.nx, d.ny = mesh.nx, mesh.ny
     d.width, d.height = mesh.width, mesh.height
@@ -174,6 +174,9 @@ def _make_desc(mesh: StructuredMesh, params: ElasticityParameters | None, filter
         d.body_force.radius = bf.region.radius
         d.body_force.value[0], d.body_force.value[1] = bf.value
     d.precision = PRECISIONS[precision]
+    if degree not in (1, 2):
+        raise ValueError("degree must be 1 (P1) or 2 (P2)")
+    d.degree = degree
     return d
 
 
@@ -244,14 +247,16 @@ class HelmholtzFilter:
 class ElasticityProblem:
     """Elastic compliance topology optimization problem (``FEM_src/elasisity_problem.py:76``).
 
-    Extra keyword arguments (not in the reference): ``precision`` ("fp64" or "mixed": FP32
-    multigrid-PCG inside FP64 iterative refinement), ``rtol`` (relative residual of the
-    state solve; the reference solves directly), ``device``.
+    Extra keyword arguments (not in the reference): ``precision`` ("fp64"; "mixed": FP64 PCG
+    with an FP32 multigrid preconditioner; "mixed-ir": FP32 multigrid-PCG inside FP64
+    iterative refinement), ``rtol`` (relative residual of the state solve; the reference
+    solves directly), ``degree`` (displacement element: 1 = P1, the default, or 2 = P2, the
+    reference's element, solved in FP64), ``device``.
     """
 
     def __init__(self, mesh: StructuredMesh, control_space: ControlSpace,
                  domain_parameters: DomainParameters, elasticity_parameters: ElasticityParameters,
-                 *, precision: str = "fp64", rtol: float = 1e-10, device=None):
+                 *, precision: str = "fp64", rtol: float = 1e-10, degree: int = 1, device=None):
         self.parameters = elasticity_parameters
         self.domain_size = (domain_parameters.width, domain_parameters.height)
         if (mesh.width, mesh.height) != (float(domain_parameters.width),
@@ -262,8 +267,9 @@ class ElasticityProblem:
         self.device = _device(device if device is not None else control_space.device)
         self.precision = precision
         self.rtol = rtol
+        self.degree = degree
         desc = _make_desc(mesh, elasticity_parameters, elasticity_parameters.filter_radius,
-                          precision)
+                          precision, degree)
         self._h = _Handle(desc, self.device)
         self.filter = HelmholtzFilter(elasticity_parameters.filter_radius, control_space,
                                       _handle=self._h)
@@ -439,7 +445,7 @@ class ElasticityProblem:
 
 
 def problem_from_design(design_file_or_dict, N: int, *, precision="fp64", rtol=1e-10,
-                        device=None):
+                        degree: int = 1, device=None):
     """Mesh + control space + ElasticityProblem the way ``FEMSolver`` builds them
     (``FEM_src/solver.py:16-30,38-55``)."""
     from .designs import parse_design, parse_design_dict
@@ -450,7 +456,7 @@ def problem_from_design(design_file_or_dict, N: int, *, precision="fp64", rtol=1
     mesh = StructuredMesh.from_N(domain.width, domain.height, N)
     space = ControlSpace(mesh, device)
     return ElasticityProblem(mesh, space, domain, params, precision=precision, rtol=rtol,
-                             device=device), domain
+                             degree=degree, device=device), domain
 
 
 class ProblemBatch:

@@ -72,7 +72,8 @@ class EMDSolver:
 
     def __init__(self, N: int, design, *, data_path: str | None = None, skip_multiple: int = 1,
                  precision: str = "fp64", rtol: float = 1e-12, filter_rtol: float | None = None,
-                 max_iterations: int = 1000, device=None, verbose: bool = False):
+                 max_iterations: int = 1000, degree: int = 1, device=None,
+                 verbose: bool = False):
         if isinstance(design, dict):
             self.parameters, problem_parameters = parse_design_dict(design)
             self.design_str = "design"
@@ -95,7 +96,8 @@ class EMDSolver:
                                    int(self.height * self.N))
         self.control_space = ControlSpace(self.mesh, device)
         self.problem = ElasticityProblem(self.mesh, self.control_space, self.parameters,
-                                         problem_parameters, precision=precision, rtol=rtol)
+                                         problem_parameters, precision=precision, rtol=rtol,
+                                         degree=degree)
         if filter_rtol is not None:
             self.problem.set_option("filter_rtol", filter_rtol)
         self.rho = self.control_space.function(self.parameters.volume_fraction)
