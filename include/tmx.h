@@ -215,6 +215,13 @@ int tmx_emd_apply(tmx_problem* h, const double* half, double c, double* psi, dou
 int tmx_emd_change(tmx_problem* h, const double* rho, const double* psi_prev, double* out_host);
 int tmx_logit(tmx_problem* h, const double* rho, double* psi);
 
+/* The design-consistency re-evaluation (misc/get_consistent_objectives.py:110-156) moves a
+ * design found on a coarser mesh to this problem's mesh: df.interpolate of a CG1 function on
+ * the coarse 'right' mesh (coarse_nx x coarse_ny cells, same rectangle) onto this mesh's
+ * CG1 space.  rho_coarse: (coarse_nx+1)(coarse_ny+1) device values; rho: n_v. */
+int tmx_interpolate_design(tmx_problem* h, int coarse_nx, int coarse_ny,
+                           const double* rho_coarse, double* rho);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,3 +77,35 @@ def case_from_design(design: dict) -> dict:
     return dict(width=dom["width"], height=dom["height"], young_modulus=par["young_modulus"],
                 poisson_ratio=par["poisson_ratio"], filter_radius=par["filter_radius"],
                 fixed_sides=par["fixed_sides"], tractions=tractions, body_force=body)
+
+
+def cfg5_cases(n_designs: int = 64, seed: int = 2024):
+    """The batched workload: designs with N uniform in {40, 80, ..., 320}, aspect W in {1, 2, 3}
+    (H = 1), penalty p in {1..5}, 1-3 tractions on random sides, an optional circular body force
+    (radius 0.1, random centre) and a random non-empty set of fixed sides.  Returns a list of
+    (case, N, p) with seeded parameters."""
+    rng = np.random.default_rng(seed)
+    sides = ["Left", "Right", "Top", "Bottom"]
+    out = []
+    for _ in range(n_designs):
+        N = int(rng.choice(np.arange(40, 321, 40)))
+        W = float(rng.choice([1.0, 2.0, 3.0]))
+        p = int(rng.integers(1, 6))
+        tractions = []
+        for _ in range(int(rng.integers(1, 4))):
+            side = str(rng.choice(sides))
+            extent = W if side in ("Top", "Bottom") else 1.0
+            length = float(rng.uniform(0.05, 0.3)) * extent
+            center = float(rng.uniform(length / 2, extent - length / 2))
+            tractions.append({"side": side, "center": center, "length": length,
+                              "value": tuple(float(v) for v in rng.uniform(-1, 1, 2))})
+        body = None
+        if rng.uniform() < 0.5:
+            body = {"center": (float(rng.uniform(0.1, W - 0.1)), float(rng.uniform(0.1, 0.9))),
+                    "radius": 0.1, "value": tuple(float(v) for v in rng.uniform(-1, 1, 2))}
+        mask = int(rng.integers(1, 16))
+        fixed = [s for i, s in enumerate(sides) if mask & (1 << i)]
+        case = dict(width=W, height=1.0, young_modulus=1.0, poisson_ratio=0.3,
+                    filter_radius=0.05, fixed_sides=fixed, tractions=tractions, body_force=body)
+        out.append((case, N, p))
+    return out

@@ -84,3 +84,36 @@ def test_batch_rejects_duplicates_and_size_mismatch():
     batch = ProblemBatch(probs[:2])
     with pytest.raises(ValueError):
         batch.objective_and_gradient([rhos[1], rhos[0]])
+
+
+def test_batch_64_random_designs():
+    """The batched workload: 64 seeded designs (N in {40..320}, aspect 1-3, p 1-5, 1-3 random
+    tractions, optional body force, random fixed sides), mixed precision with autotuned tiles,
+    in one batch call; every result bitwise equal to its solo call, and the two smallest
+    designs against the oracle."""
+    from tests.cases import cfg5_cases
+    cases = cfg5_cases(64)
+    probs, rhos = [], []
+    for k, (case, N, p) in enumerate(cases):
+        nx, ny = int(case["width"] * N), N
+        gp = gpu_problem(case, nx, ny, precision="mixed", rtol=1e-10)
+        gp.set_penalization(p)
+        probs.append(gp)
+        rhos.append(torch.as_tensor(smoothed_density(nx, ny, seed=k), device="cuda:0"))
+    for gp in probs:
+        gp.autotune()
+    batch = ProblemBatch(probs, workers=4)
+    J, w = batch.objective_and_gradient(rhos)
+    assert all(np.isfinite(J))
+    for k in range(0, 64, 7):  # solo spot checks, bitwise
+        Js, ws, _ = probs[k].objective_and_gradient(rhos[k])
+        assert J[k] == Js and torch.equal(w[k], ws)
+    small = sorted(range(64), key=lambda k: cases[k][1] * cases[k][0]["width"])[:2]
+    for k in small:
+        case, N, p = cases[k]
+        nx, ny = int(case["width"] * N), N
+        orc = oracle_problem(case, nx, ny)
+        orc.set_penalization(p)
+        Jo = orc.calculate_objective(rhos[k].cpu().numpy())
+        print("design", k, case["fixed_sides"], N, p, J[k], Jo, flush=True)
+        assert abs(J[k] - Jo) <= 1e-6 * abs(Jo)  # 0 when every load sits on a fixed side

@@ -114,6 +114,7 @@ SIGNATURES = {
     "tmx_emd_apply": (_I, [_P, _P, _D, _P, _P]),
     "tmx_emd_change": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
     "tmx_logit": (_I, [_P, _P, _P]),
+    "tmx_interpolate_design": (_I, [_P, _I, _I, _P, _P]),
 }
 
 _lib = None

@@ -853,3 +853,14 @@ extern "C" int tmx_logit(tmx_problem* h, const double* rho, double* psi) {
   TMX_HIP(logit(nverts(h), rho, psi, h->stream));
   return TMX_OK;
 }
+
+// ------------------------------------------------------------------ design transfer
+extern "C" int tmx_interpolate_design(tmx_problem* h, int coarse_nx, int coarse_ny,
+                                      const double* rho_coarse, double* rho) {
+  if (!h || !rho_coarse || !rho) return invalid("null argument");
+  if (coarse_nx < 1 || coarse_ny < 1) return invalid("coarse nx and ny must be >= 1");
+  Scope scope(h);
+  TMX_HIP(interp_p1(coarse_nx, coarse_ny, h->desc.width / coarse_nx, h->desc.height / coarse_ny,
+                    h->nx, h->ny, h->hx, h->hy, rho_coarse, rho, h->stream));
+  return TMX_OK;
+}

@@ -270,4 +270,8 @@ hipError_t mr_update(size_t n, const double* sc, const MrSlots& q, double* v, co
 // out = a - b
 hipError_t sub_vec(size_t n, const double* a, const double* b, double* out, hipStream_t s);
 
+// fine CG1 values from a coarser CG1 function on the same rectangle ('right' meshes)
+hipError_t interp_p1(int nxc, int nyc, double hxc, double hyc, int nxf, int nyf, double hxf,
+                     double hyf, const double* rc, double* rf, hipStream_t s);
+
 }  // namespace tmx

@@ -2321,4 +2321,31 @@ hipError_t sub_vec(size_t n, const double* a, const double* b, double* out, hipS
   return hipGetLastError();
 }
 
+// ============================================================================ design transfer
+// Value at every fine vertex of the CG1 function given by nodal values on a coarser 'right'
+// mesh of the same rectangle (dolfin interpolate of a CG1 function onto another mesh's CG1
+// space): locate the coarse cell, then the triangle (A below the diagonal v0-v3, B above),
+// and evaluate the linear interpolant there.
+__global__ void k_interp_p1(int nxc, int nyc, double hxc, double hyc, int nxf, int nyf,
+                            double hxf, double hyf, const double* __restrict__ rc,
+                            double* __restrict__ rf) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)(nxf + 1) * (nyf + 1)) return;
+  const int ix = (int)(v % (nxf + 1)), iy = (int)(v / (nxf + 1));
+  const double x = 0.0 + hxf * ix, y = 0.0 + hyf * iy;
+  const double fx = x / hxc, fy = y / hyc;
+  const int cx = min(max((int)floor(fx), 0), nxc - 1), cy = min(max((int)floor(fy), 0), nyc - 1);
+  const double s = fmin(fmax(fx - cx, 0.0), 1.0), t = fmin(fmax(fy - cy, 0.0), 1.0);
+  const size_t v0 = (size_t)cy * (nxc + 1) + cx;
+  const double r0 = rc[v0], r1 = rc[v0 + 1], r2 = rc[v0 + nxc + 1], r3 = rc[v0 + nxc + 2];
+  rf[v] = s >= t ? r0 + s * (r1 - r0) + t * (r3 - r1) : r0 + t * (r2 - r0) + s * (r3 - r2);
+}
+
+hipError_t interp_p1(int nxc, int nyc, double hxc, double hyc, int nxf, int nyf, double hxf,
+                     double hyf, const double* rc, double* rf, hipStream_t s) {
+  const size_t n = (size_t)(nxf + 1) * (nyf + 1);
+  k_interp_p1<<<grid1d(n), kBlock, 0, s>>>(nxc, nyc, hxc, hyc, nxf, nyf, hxf, hyf, rc, rf);
+  return hipGetLastError();
+}
+
 }  // namespace tmx

@@ -386,6 +386,20 @@ class ElasticityProblem:
         _lib.check(h.lib.tmx_load_vector(h.ptr, _ptr(F)))
         return F
 
+    def interpolate_design(self, rho_coarse, coarse_mesh: StructuredMesh) -> torch.Tensor:
+        """This mesh's CG1 interpolant of a design given on a coarser mesh of the same domain
+        (``df.interpolate(small_rho, control_space)``, ``misc/get_consistent_objectives.py:
+        132-134``)."""
+        h = self._h
+        if (coarse_mesh.width, coarse_mesh.height) != (self.mesh.width, self.mesh.height):
+            raise ValueError("the coarse mesh covers a different domain")
+        rc = _as_field(rho_coarse, coarse_mesh.n_vertices, h.device, "rho_coarse")
+        out = torch.empty(h.n_vertices, dtype=torch.float64, device=h.device)
+        h.stream()
+        _lib.check(h.lib.tmx_interpolate_design(h.ptr, coarse_mesh.nx, coarse_mesh.ny, _ptr(rc),
+                                                _ptr(out)))
+        return out
+
     def integrate(self, values) -> float:
         """``FEMSolver.integrate`` (``FEM_src/solver.py:81-84``)."""
         h = self._h
