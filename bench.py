@@ -72,19 +72,24 @@ def bench_kernel(problem, kernel, precision, reps=50):
     return us.value, nbytes.value
 
 
-def batch_throughput(n_designs: int, N: int, steps: int, rtol: float):
-    """Many independent designs: n_designs cantilevers at N (penalties 1..5, seeds differ),
-    one objective+gradient each, run one after another vs in one tmx_batch call."""
+def batch_throughput(n_designs: int, steps: int, rtol: float):
+    """The batched workload: n_designs seeded designs (N in {40, ..., 320}, aspect 1-3, p 1-5,
+    1-3 random tractions, optional body force, random fixed sides; tests.cases.cfg5_cases),
+    mixed precision with autotuned tiles; one objective+gradient each, run one after another
+    vs in one tmx_batch call."""
     import torch
-    from tests.cases import CANTILEVER, gpu_problem, smoothed_density
+    from tests.cases import cfg5_cases, gpu_problem, smoothed_density
     from topomax_amd import ProblemBatch
-    nx, ny = 3 * N, N
     probs, rhos = [], []
-    for k in range(n_designs):
-        gp = gpu_problem(CANTILEVER, nx, ny, precision="mixed", rtol=rtol)
-        gp.set_penalization(1 + k % 5)
+    ndof = 0
+    for k, (case, N, p) in enumerate(cfg5_cases(n_designs)):
+        nx, ny = int(case["width"] * N), N
+        gp = gpu_problem(case, nx, ny, precision="mixed", rtol=rtol)
+        gp.set_penalization(p)
+        gp.autotune()
         probs.append(gp)
-        rhos.append(torch.as_tensor(smoothed_density(nx, ny, seed=100 + k), device="cuda:0"))
+        rhos.append(torch.as_tensor(smoothed_density(nx, ny, seed=k), device="cuda:0"))
+        ndof += 2 * (nx + 1) * (ny + 1)
     batch = ProblemBatch(probs, workers=4)
     for gp, r in zip(probs, rhos):
         gp.objective_and_gradient(r)
@@ -101,8 +106,9 @@ def batch_throughput(n_designs: int, N: int, steps: int, rtol: float):
         batch.objective_and_gradient(rhos)
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
-    ndof = n_designs * 2 * (nx + 1) * (ny + 1)
-    return {"designs": n_designs, "N": N, "workers": 4, "ms_serial": t_serial * 1e3,
+    return {"designs": n_designs, "workload": "seeded random designs, N 40-320, aspect 1-3, "
+                                              "p 1-5, mixed precision, autotuned tiles",
+            "dofs": ndof, "workers": 4, "ms_serial": t_serial * 1e3,
             "ms_batched": t_batch * 1e3, "GDOF_s_serial": ndof / t_serial / 1e9,
             "GDOF_s_batched": ndof / t_batch / 1e9, "speedup": t_serial / t_batch}
 
@@ -145,8 +151,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-designs", type=int, default=16)
-    ap.add_argument("--batch-N", type=int, default=250)
+    ap.add_argument("--batch-designs", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="set up the N operator, then launch only the roofline kernel (FP32 "
                          "fine-grid sweep, 20x) and a plain copy of the 2 n_v load vector "
@@ -215,8 +220,7 @@ def main():
 
     batch = None
     if not args.no_batch:
-        batch = batch_throughput(args.batch_designs, args.batch_N, max(2, args.steps // 2),
-                                 args.rtol)
+        batch = batch_throughput(args.batch_designs, max(2, args.steps // 2), args.rtol)
 
     cpu = None
     if not args.no_cpu_baseline:
