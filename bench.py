@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--N", type=int, default=1000)
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--cpu-N", type=int, default=200)
+    ap.add_argument("--cpu-N", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

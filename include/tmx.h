@@ -173,7 +173,11 @@ enum tmx_option {
   TMX_OPT_NU_COARSE = 6,    /* pre- and post-sweeps on the coarser levels (PRE/POST: finest) */
   TMX_OPT_FILTER_RTOL = 7,  /* relative residual of the Helmholtz filter solves (1e-12) */
   TMX_OPT_NU_FINE_FILTER = 8,   /* filter: pre- and post-sweeps on the finest level */
-  TMX_OPT_NU_COARSE_FILTER = 9  /* filter: pre- and post-sweeps on the coarser levels */
+  TMX_OPT_NU_COARSE_FILTER = 9, /* filter: pre- and post-sweeps on the coarser levels */
+  TMX_OPT_CHEB_ELAST = 10,      /* > 0: Chebyshev step sizes for an upper bound lambda of
+                                   lambda_max(D^-1 A) (elasticity); 0: damped Jacobi */
+  TMX_OPT_CHEB_FILTER = 11,     /* the same for the filter */
+  TMX_OPT_CHEB_RATIO = 12       /* smoothing band [lambda / ratio, lambda] (both; 4) */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

@@ -134,6 +134,7 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   TMX_TRY(h->el.init(kElasticity, h->nx, h->ny, ab, cd, fx, h->lam, h->mu, 0.0, 64, s));
   TMX_TRY(h->fl.init(kFilter, h->nx, h->ny, ab, cd, 0u, 0.0, 0.0, d->filter_radius, 128, s));
   TMX_TRY(h->fl.build_filter(s));
+  h->el.nu_coarse_ = 3;  // with Chebyshev steps a third coarse sweep pays for itself
   const size_t nv = (size_t)(h->nx + 1) * (h->ny + 1);
   TMX_TRY(h->F.alloc(2 * nv));
   TMX_TRY(h->Fm.alloc(2 * nv));
@@ -201,6 +202,8 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   h->fl_prm.rtol = 1e-12;
   h->fl_prm.max_iter = 500;
   h->fl_prm.precision = filter_precision(d->precision);
+  // the filter operator is fixed: bound lambda_max(D^-1 A) for its Chebyshev steps once
+  TMX_TRY(h->fl.estimate_lmax(h->ws, s));
   TMX_HIP(hipStreamSynchronize(s));
   return TMX_OK;
 }
@@ -402,6 +405,9 @@ static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const t
   TMX_TRY(need_penalty(h));
   TMX_HIP(hipEventRecord(h->ev_setup[0], h->stream));
   TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, h->stream));
+  // lambda_max(D^-1 A) of block Jacobi hardly depends on the design (the factors scale D and
+  // A alike), so the first operator's estimate (with a margin) serves the later ones
+  if (h->el.cheb_auto_ && !h->el.cheb_done_) TMX_TRY(h->el.estimate_lmax(h->ws, h->stream));
   TMX_HIP(hipEventRecord(h->ev_setup[1], h->stream));
   if (h->degree == 2) {
     const int rc = p2_solve(h, xi, u, merge(h->el_prm, o), st);
@@ -561,6 +567,20 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
     case TMX_OPT_FILTER_RTOL:
       if (!(value > 0 && value < 1)) return invalid("filter rtol must be in (0, 1)");
       h->fl_prm.rtol = value;
+      break;
+    case TMX_OPT_CHEB_ELAST:
+    case TMX_OPT_CHEB_FILTER:
+      if (!(value >= 0 && value < 100)) return invalid("Chebyshev bound must be in [0, 100)");
+      {
+        Hierarchy& x = option == TMX_OPT_CHEB_ELAST ? h->el : h->fl;
+        x.cheb_lmax_ = value;
+        x.cheb_auto_ = false;
+        x.cheb_done_ = true;
+      }
+      break;
+    case TMX_OPT_CHEB_RATIO:
+      if (!(value > 1 && value < 1000)) return invalid("Chebyshev ratio must be in (1, 1000)");
+      for (auto* x : hs) x->cheb_ratio_ = value;
       break;
     case TMX_OPT_OMEGA_ELAST:
     case TMX_OPT_OMEGA_FILTER:

@@ -170,11 +170,12 @@ struct TailLevel {
   const T* st;  // 9-point stencil (global, SoA as for stencil_op)
   int off;      // LDS offset of this level's vectors, in elements of T
 };
+constexpr int kMaxNu = 8;  // smoothing sweeps per pass in the tail kernel
 template <typename T>
 struct TailDesc {
   int nlev, cn, nu_pre, nu_post;
   unsigned fixed;
-  T omega;
+  T omega[kMaxNu];  // step size of sweep k of a smoothing pass
   const double* dense;
   const T* b_in;
   T* x_out;
@@ -273,5 +274,11 @@ hipError_t sub_vec(size_t n, const double* a, const double* b, double* out, hipS
 // fine CG1 values from a coarser CG1 function on the same rectangle ('right' meshes)
 hipError_t interp_p1(int nxc, int nyc, double hxc, double hyc, int nxf, int nyf, double hxf,
                      double hyf, const double* rc, double* rf, hipStream_t s);
+
+// v[i] = pseudo-random value in [-1, 1] from a hash of i (power-iteration start vector)
+hipError_t fill_hash(size_t n, double* v, hipStream_t s);
+
+// out = c * in
+hipError_t scale_copy(size_t n, double c, const double* in, double* out, hipStream_t s);
 
 }  // namespace tmx

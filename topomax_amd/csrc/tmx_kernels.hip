@@ -1433,7 +1433,6 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
   T* lds = reinterpret_cast<T*>(tail_lds);
   const int tid = threadIdx.x;
   const int nl = d.nlev;  // levels 0..nl-2 are 9-point, level nl-1 is the dense coarsest
-  const T om = d.omega;
   // ping-pong parity: the first sweep writes buffer (S-1)%2 so that after S sweeps the
   // iterate is in buffer 0; after the pre-sweeps it is in buffer nu_post%2 on every level
   const int w_first = (d.nu_pre + d.nu_post - 1) & 1;
@@ -1454,15 +1453,15 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
     TailRowHolder<T, BS> rh;
     rh.level(L, tid);
     int w = w_first;
-    rh.template sweep<kSmoothZero>(L, nullptr, b, w ? b1 : b0, om, tid);
+    rh.template sweep<kSmoothZero>(L, nullptr, b, w ? b1 : b0, d.omega[0], tid);
     __syncthreads();
     for (int k = 1; k < d.nu_pre; ++k) {
-      rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, om, tid);
+      rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, d.omega[k], tid);
       __syncthreads();
       w ^= 1;
     }
     T* const spare = w ? b0 : b1;  // the residual goes to the free ping-pong buffer
-    rh.template sweep<kResidual>(L, w ? b1 : b0, b, spare, om, tid);
+    rh.template sweep<kResidual>(L, w ? b1 : b0, b, spare, T(0), tid);
     __syncthreads();
     const TailLevel<T>& C = d.lev[l + 1];
     const int nnc = (C.nx + 1) * (C.ny + 1);
@@ -1501,7 +1500,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
     if (tid < nn) prolong_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, cx, w ? b1 : b0, tid);
     __syncthreads();
     for (int k = 0; k < d.nu_post; ++k) {
-      rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, om, tid);
+      rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, d.omega[k], tid);
       __syncthreads();
       w ^= 1;
     }
@@ -1514,7 +1513,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
 
 template <typename T, int BS>
 hipError_t tail_vcycle(const TailDesc<T>& d, size_t lds_bytes, hipStream_t s) {
-  if (d.nlev < 2 || d.nlev > kMaxTail || d.cn > 128 || lds_bytes > kTailLdsBytes)
+  if (d.nlev < 2 || d.nlev > kMaxTail || d.cn > 128 || lds_bytes > kTailLdsBytes ||
+      d.nu_pre < 1 || d.nu_pre > kMaxNu || d.nu_post < 0 || d.nu_post > kMaxNu)
     return hipErrorInvalidValue;
   for (int l = 0; l + 1 < d.nlev; ++l)
     if ((d.lev[l].nx + 1) * (d.lev[l].ny + 1) > kTailThreads) return hipErrorInvalidValue;
@@ -2345,6 +2345,31 @@ hipError_t interp_p1(int nxc, int nyc, double hxc, double hyc, int nxf, int nyf,
                      double hyf, const double* rc, double* rf, hipStream_t s) {
   const size_t n = (size_t)(nxf + 1) * (nyf + 1);
   k_interp_p1<<<grid1d(n), kBlock, 0, s>>>(nxc, nyc, hxc, hyc, nxf, nyf, hxf, hyf, rc, rf);
+  return hipGetLastError();
+}
+
+// deterministic pseudo-random start vector in [-1, 1] (integer hash of the index)
+__global__ void k_fill_hash(size_t n, double* __restrict__ v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    unsigned h = (unsigned)(i * 2654435761u) ^ 0x9e3779b9u;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    v[i] = (double)h * (2.0 / 4294967295.0) - 1.0;
+  }
+}
+hipError_t fill_hash(size_t n, double* v, hipStream_t s) {
+  k_fill_hash<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, v);
+  return hipGetLastError();
+}
+
+__global__ void k_scale_copy(size_t n, double c, const double* __restrict__ in,
+                             double* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = c * in[i];
+}
+hipError_t scale_copy(size_t n, double c, const double* in, double* out, hipStream_t s) {
+  k_scale_copy<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, c, in, out);
   return hipGetLastError();
 }
 

@@ -197,6 +197,18 @@ class Hierarchy {
   int p_ = 0;
   double m_ = 0, cp_ = 0;
   double omega_ = 0.7;
+  // Chebyshev step sizes: with cheb_lmax_ > 0 the nu Jacobi sweeps of one smoothing pass use
+  // omega_k = 1 / theta_k, theta_k the roots of the degree-nu Chebyshev polynomial on
+  // [cheb_lmax_ / cheb_ratio_, cheb_lmax_] (an upper bound of lambda_max(D^-1 A)), so one pass
+  // damps that whole band; the pass polynomial has commuting factors, so pre- and
+  // post-smoothing stay adjoint and the V-cycle symmetric.  0: damped Jacobi with omega_.
+  double cheb_lmax_ = 0.0, cheb_ratio_ = 8.0;
+  double sweep_omega(int k, int nu) const;
+  // power-iteration estimate of max over levels of lambda_max(D^-1 A) (preconditioner
+  // operators, FP64); sets cheb_lmax_ to it times the margin
+  int estimate_lmax(Workspace& ws, hipStream_t s, double margin = 1.1);
+  bool cheb_auto_ = true;     // estimate once, at the first operator
+  bool cheb_done_ = false;
   int nu_pre_ = 2, nu_post_ = 2;  // Jacobi sweeps on the finest level
   int nu_coarse_ = 2;              // pre- and post-sweeps on every coarser level
   bool fused_ = true, use_tail_ = true;

@@ -118,6 +118,13 @@ void Hierarchy::find_tail() {
   }
 }
 
+double Hierarchy::sweep_omega(int k, int nu) const {
+  if (!(cheb_lmax_ > 0.0) || nu < 1) return omega_;
+  const double b = cheb_lmax_, a = cheb_lmax_ / cheb_ratio_;
+  const double theta = 0.5 * (a + b) + 0.5 * (b - a) * std::cos((2 * k + 1) * M_PI / (2.0 * nu));
+  return 1.0 / theta;
+}
+
 void Hierarchy::drop_graphs() {
   for (auto& g : graphs_)
     for (auto& e : g)
@@ -292,6 +299,8 @@ int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t 
 
 int Hierarchy::build_filter(hipStream_t s) { return build_coarse_stencils(s); }
 
+
+
 // ------------------------------------------------------------------ level operators
 template <typename T>
 struct PrecBufs;
@@ -326,16 +335,18 @@ const float* xi_for<float>(const Hierarchy& h) { return h.xi32_.get(); }
 template <typename T>
 static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, RedSlot* dot,
                     hipStream_t s, const SweepExtra<T>& ex = SweepExtra<T>(),
-                    bool precond = true) {
+                    bool precond = true,
+                    double omega = std::numeric_limits<double>::quiet_NaN()) {
   Level& L = h.levels_[l];
   const bool fp64 = sizeof(T) == 8;
+  const T om = (T)(std::isnan(omega) ? h.omega_ : omega);  // NaN: the hierarchy's damping
   if (L.kind == kElem) {
     if (h.phys_ == kElasticity) {
       ElastOpArgs<T> a = elast_args<T>(h, l, xi_for<T>(h), PrecBufs<T>::cw(L), fp64, precond);
-      TMX_HIP(elast_sweep<T>(a, op, x, b, y, (T)h.omega_, dot, s, ex));
+      TMX_HIP(elast_sweep<T>(a, op, x, b, y, om, dot, s, ex));
     } else {
       TMX_HIP(filter_sweep<T>(PrecBufs<T>::sh(L), FilterConst<T>::make(h.eps_, L.hx, L.hy), op, x,
-                              b, y, (T)h.omega_, dot, s, ex));
+                              b, y, om, dot, s, ex));
     }
   } else {
     if (dot || op > kSmoothZero) {
@@ -343,9 +354,9 @@ static int level_op(Hierarchy& h, int l, int op, const T* x, const T* b, T* y, R
       return -1;
     }
     if (h.bs_ == 2)
-      TMX_HIP((stencil_op<T, 2>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, (T)h.omega_, s)));
+      TMX_HIP((stencil_op<T, 2>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, om, s)));
     else
-      TMX_HIP((stencil_op<T, 1>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, (T)h.omega_, s)));
+      TMX_HIP((stencil_op<T, 1>(L.nx, L.ny, PrecBufs<T>::st(L), op, x, b, y, om, s)));
   }
   return 0;
 }
@@ -413,7 +424,7 @@ static int tail_launch(Hierarchy& h, const T* b, T* x, hipStream_t s) {
   d.nu_pre = h.nu_coarse_;
   d.nu_post = h.nu_coarse_;
   d.fixed = h.fixed_;
-  d.omega = (T)h.omega_;
+  for (int k = 0; k < kMaxNu; ++k) d.omega[k] = (T)h.sweep_omega(k, h.nu_coarse_);
   d.dense = h.dense_.get();
   d.b_in = b;
   d.x_out = x;
@@ -451,9 +462,11 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   const int npost = l == 0 ? nu_post_ : nu_coarse_;
   const int S = npre + npost;
   int w = (S - 1) % 2;
-  TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s));
+  TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s, SweepExtra<T>(),
+                      true, sweep_omega(0, npre)));
   for (int k = 1; k < npre; ++k) {
-    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s));
+    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s, SweepExtra<T>(),
+                        true, sweep_omega(k, npre)));
     w ^= 1;
   }
   T* cb = PrecBufs<T>::b(C);
@@ -477,13 +490,13 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
     TMX_TRY(restrict_to<T>(*this, l, r, cb, s));
   }
   TMX_TRY(vcycle<T>(l + 1, cb, cx, s));
-  int post = npost;
+  int post = npost, j = 0;  // j: index of the next post-sweep in the smoothing pass
   RedSlot* last_dot = l == 0 ? dot_bz : nullptr;
   if (fused_el) {
     SweepExtra<T> ex;
     ex.ec = cx;
     TMX_TRY(level_op<T>(*this, l, kProlongSmooth, buf[w], b, buf[w ^ 1],
-                        post == 1 ? last_dot : nullptr, s, ex));
+                        post == 1 ? last_dot : nullptr, s, ex, true, sweep_omega(j++, npost)));
     w ^= 1;
     --post;
   } else if (fused_st) {
@@ -500,9 +513,45 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   }
   for (int k = 0; k < post; ++k) {
     TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1],
-                        k == post - 1 ? last_dot : nullptr, s));
+                        k == post - 1 ? last_dot : nullptr, s, SweepExtra<T>(), true,
+                        sweep_omega(j++, npost)));
     w ^= 1;
   }
+  return 0;
+}
+
+int Hierarchy::estimate_lmax(Workspace& ws, hipStream_t s, double margin) {
+  constexpr int kIters = 24;
+  RedSlot r = ws.region(3);
+  double lmax = 0.0;
+  const int last = (int)levels_.size() - 1;
+  for (int l = 0; l < last; ++l) {
+    Level& L = levels_[l];
+    const size_t n = (size_t)bs_ * L.nn;
+    double* x = L.x64.get();
+    double* y = L.t64.get();
+    double* zero = L.b64.get();
+    TMX_HIP(fill_zero_bytes(zero, sizeof(double) * n, s));
+    TMX_HIP(fill_hash(n, x, s));
+    double est = 0.0;
+    for (int it = 0; it < kIters; ++it) {
+      TMX_HIP(dot(n, x, x, r, ws.sc() + kScTmp, s));
+      // y = x + D^-1 A x (one Jacobi sweep on b = 0 with step -1): eigenvalues 1 + lambda
+      TMX_TRY(level_op<double>(*this, l, kSmooth, x, zero, y, nullptr, s, SweepExtra<double>(),
+                               true, -1.0));
+      TMX_HIP(dot(n, y, y, r, ws.sc() + kScTmp + 1, s));
+      TMX_TRY(ws.read(s, kScTmp, 2));
+      const double xx = ws.host[kScTmp], yy = ws.host[kScTmp + 1];
+      if (!(xx > 0) || !std::isfinite(yy)) break;
+      est = std::sqrt(yy / xx) - 1.0;
+      TMX_HIP(scale_copy(n, 1.0 / std::sqrt(yy), y, x, s));
+    }
+    lmax = std::max(lmax, est);
+    if (g_trace) std::fprintf(stderr, "[tmx] level %d lambda_max(D^-1 A) ~ %.4f\n", l, est);
+  }
+  cheb_lmax_ = lmax > 0.0 ? margin * lmax : 0.0;
+  cheb_done_ = true;
+  drop_graphs();
   return 0;
 }
 
@@ -817,6 +866,16 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
              : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s);
     S.iterations = it;
     S.outer = 1;
+    if (rc == -4 && cheb_lmax_ > 0.0) {  // Chebyshev bound too low for this operator: Jacobi
+      cheb_lmax_ = 0.0;
+      drop_graphs();
+      int it2 = 0;
+      rc = prm.precision == kSolveFP64
+               ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bb, &it2, &rel, s)
+               : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bb, &it2, &rel, s);
+      S.iterations += it2;
+      it += it2;
+    }
     if (rc == -4 && phys_ == kElasticity) {  // K(xi) not positive definite: MINRES
       int it2 = 0;
       rc = minres(ws, b, prm.rtol, std::max(1, prm.max_iter - it), bb, &it2, &rel, s);

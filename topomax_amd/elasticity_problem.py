@@ -418,7 +418,8 @@ class ElasticityProblem:
 
     def set_option(self, name: str, value):
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
-        nu_coarse_filter, omega_elast, omega_filter, filter_rtol."""
+        nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
+        cheb_ratio."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -426,7 +427,10 @@ class ElasticityProblem:
                "omega_filter": _lib.TMX_OPT_OMEGA_FILTER,
                "filter_rtol": _lib.TMX_OPT_FILTER_RTOL,
                "nu_fine_filter": _lib.TMX_OPT_NU_FINE_FILTER,
-               "nu_coarse_filter": _lib.TMX_OPT_NU_COARSE_FILTER}[name]
+               "nu_coarse_filter": _lib.TMX_OPT_NU_COARSE_FILTER,
+               "cheb_elast": _lib.TMX_OPT_CHEB_ELAST,
+               "cheb_filter": _lib.TMX_OPT_CHEB_FILTER,
+               "cheb_ratio": _lib.TMX_OPT_CHEB_RATIO}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
