@@ -83,6 +83,11 @@ __device__ __forceinline__ T pc_weight(T w, T wmin) {
   return wmin > T(0) ? max(w < T(0) ? -w : w, wmin) : w;
 }
 
+// 1/x for the smoothers' 2x2 block inverses: the hardware reciprocal (1 ulp) in FP32, where
+// the V-cycle is a preconditioner; correctly rounded division in FP64
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // ---------------------------------------------------------------- reductions
 // Each block writes its partial sum with a plain store; a separate one-block kernel adds the
 // partials in block order (deterministic).  Kernel boundaries order the two, so no fences

@@ -156,15 +156,15 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   auto ld_x = [&](int iy, V& raw) -> V {
     V v = {T(0), T(0)};
     const bool ok = colv && iy >= 0 && iy <= ny;
-    if (OP != kSmoothZero && ok) v = X[(size_t)iy * nvx + c];
+    if (OP != kSmoothZero && ok) v = X[(unsigned)(iy * nvx + c)];
     if constexpr (AU) {
       if (ok) {
         V z;
         if (ZL) {
-          const float2 zl = ZL[(size_t)iy * nvx + c];
+          const float2 zl = ZL[(unsigned)(iy * nvx + c)];
           z = {(T)zl.x, (T)zl.y};
         } else {
-          z = B[(size_t)iy * nvx + c];
+          z = B[(unsigned)(iy * nvx + c)];
         }
         v = {z.x + beta * v.x, z.y + beta * v.y};
       }
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     return v;
   };
   auto ld_xi = [&](int iy) -> T {
-    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(size_t)iy * nvx + c];
+    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(unsigned)(iy * nvx + c)];
     return T(0);
   };
 
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
         wA = pc_weight(a.m + a.cp * hsym3(zb, zbr, ztr, a.p), a.wmin);
         wB = pc_weight(a.m + a.cp * hsym3(zb, zt, ztr, a.p), a.wmin);
       } else {
-        const V w = CW[(size_t)cy * nx + c];
+        const V w = CW[(unsigned)(cy * nx + c)];
         wA = w.x;
         wB = w.y;
       }
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     }
     // node row cy is complete
     const bool rowv = colv && cy >= 0 && cy <= ny;
-    const size_t idx = (size_t)cy * nvx + c;
+    const unsigned idx = (unsigned)(cy * nvx + c);
     const bool fx = row_fixed(cy);
     if constexpr (RR) {
       const V Ax = fx ? ub_raw : acc_b;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
           d = res;
         } else {
           const T det = d_b.xx * d_b.yy - d_b.xy * d_b.xy;
-          const T inv = T(1) / det;
+          const T inv = recip(det);
           d = {(d_b.yy * res.x - d_b.xy * res.y) * inv, (d_b.xx * res.y - d_b.xy * res.x) * inv};
         }
         const V yo = {ub_raw.x + omega * d.x, ub_raw.y + omega * d.y};
@@ -321,6 +321,7 @@ hipError_t elast_sweep(const ElastOpArgs<T>& a0, int op, const T* x, const T* b,
   if (op == kProlongSmooth && (!ec || (a.sh.nx & 1) || (a.sh.ny & 1))) return hipErrorInvalidValue;
   if (op == kApplyUpdate && (!ex.pnew || !ex.beta || (!b && !ex.zlow))) return hipErrorInvalidValue;
   if (op == kResidualRestrict && dot_xy) return hipErrorInvalidValue;
+  if ((size_t)(a.sh.nx + 1) * (a.sh.ny + 1) >= (1ull << 31)) return hipErrorInvalidValue;
   const int nb = a.sh.blocks();
   double* dp = nullptr;
   if (dot_xy) {
