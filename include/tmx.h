@@ -171,13 +171,19 @@ enum tmx_option {
   TMX_OPT_FUSED = 0, TMX_OPT_TAIL = 1, TMX_OPT_NU_PRE = 2, TMX_OPT_NU_POST = 3,
   TMX_OPT_OMEGA_ELAST = 4, TMX_OPT_OMEGA_FILTER = 5,
   TMX_OPT_NU_COARSE = 6,    /* pre- and post-sweeps on the coarser levels (PRE/POST: finest) */
-  TMX_OPT_FILTER_RTOL = 7,  /* relative residual of the Helmholtz filter solves (1e-12) */
+  TMX_OPT_FILTER_RTOL = 7,  /* relative residual of the Helmholtz filter solves (default:
+                               1e-2 x the elasticity rtol, clamped to [1e-12, 1e-10]) */
   TMX_OPT_NU_FINE_FILTER = 8,   /* filter: pre- and post-sweeps on the finest level */
   TMX_OPT_NU_COARSE_FILTER = 9, /* filter: pre- and post-sweeps on the coarser levels */
   TMX_OPT_CHEB_ELAST = 10,      /* > 0: Chebyshev step sizes for an upper bound lambda of
                                    lambda_max(D^-1 A) (elasticity); 0: damped Jacobi */
   TMX_OPT_CHEB_FILTER = 11,     /* the same for the filter */
-  TMX_OPT_CHEB_RATIO = 12       /* smoothing band [lambda / ratio, lambda] (both; 4) */
+  TMX_OPT_CHEB_RATIO = 12,      /* smoothing band [lambda / ratio, lambda] (both; 8) */
+  TMX_OPT_BAND = 13,            /* first level (>= 1) of the tiled multi-sweep V-cycle kernels
+                                   (both families and precisions; default 1, FP64 elasticity
+                                   2), 0: one launch per sweep / transfer instead */
+  TMX_OPT_BAND_FINE_FILTER = 14 /* 1 (default): the filter's finest level in the tiled
+                                   kernels too */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

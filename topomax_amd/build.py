@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtmx.so")
-SOURCES = ["tmx_kernels.hip", "tmx_p2.hip", "tmx_solver.hip", "tmx_api.hip"]
+SOURCES = ["tmx_kernels.hip", "tmx_band.hip", "tmx_p2.hip", "tmx_solver.hip", "tmx_api.hip"]
 ARCH = os.environ.get("TMX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

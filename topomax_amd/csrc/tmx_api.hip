@@ -39,6 +39,7 @@ struct tmx_problem {
   DevBuf<double> tmp64;
   DevBuf<float> xi32, u32, y32;
   SolveParams el_prm, fl_prm;
+  bool fl_rtol_set = false;  // filter tolerance given explicitly (else derived per solve)
   // P2 element (degree 2): lattice geometry, element weights of the operator (W2) and of the
   // preconditioner (W2pc, from max(xi, 0)), P2 block diagonal, loads and solver vectors
   int degree = 1;
@@ -82,7 +83,7 @@ extern "C" int tmx_abi_version(void) { return TMX_ABI_VERSION; }
 static int solve_precision(int p) {
   return p == TMX_FP64 ? kSolveFP64 : p == TMX_MIXED_IR ? kSolveMixedIR : kSolveMixed;
 }
-// the filter solves to 1e-12 in FP64 PCG, with an FP32 V-cycle unless FP64 is asked for
+// the filter solves in FP64 PCG, with an FP32 V-cycle unless FP64 is asked for
 static int filter_precision(int p) { return p == TMX_FP64 ? kSolveFP64 : kSolveMixed; }
 
 static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
@@ -135,6 +136,8 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   TMX_TRY(h->fl.init(kFilter, h->nx, h->ny, ab, cd, 0u, 0.0, 0.0, d->filter_radius, 128, s));
   TMX_TRY(h->fl.build_filter(s));
   h->el.nu_coarse_ = 3;  // with Chebyshev steps a third coarse sweep pays for itself
+  // FP64 element level 1 is faster launch per sweep: its band kernels are VALU-bound there
+  h->el.band_first_fp64_ = 2;
   const size_t nv = (size_t)(h->nx + 1) * (h->ny + 1);
   TMX_TRY(h->F.alloc(2 * nv));
   TMX_TRY(h->Fm.alloc(2 * nv));
@@ -301,6 +304,12 @@ static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
   SolveStats st;
   SolveParams prm = h->fl_prm;
   if (o && o->precision >= 0) prm.precision = filter_precision(o->precision);
+  if (!h->fl_rtol_set) {
+    // two orders of magnitude below the elasticity solve's own tolerance, within [1e-12, 1e-10]:
+    // the filtered design then adds nothing visible to the solve's error
+    const double el = (o && o->rtol > 0) ? o->rtol : h->el_prm.rtol;
+    prm.rtol = std::min(1e-10, std::max(1e-12, 1e-2 * el));
+  }
   const int rc = h->fl.solve(h->ws, b, w, prm, &st, h->stream);
   if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(nverts(h), NAN, w, h->stream));
   return rc;
@@ -567,6 +576,7 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
     case TMX_OPT_FILTER_RTOL:
       if (!(value > 0 && value < 1)) return invalid("filter rtol must be in (0, 1)");
       h->fl_prm.rtol = value;
+      h->fl_rtol_set = true;
       break;
     case TMX_OPT_CHEB_ELAST:
     case TMX_OPT_CHEB_FILTER:
@@ -581,6 +591,13 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
     case TMX_OPT_CHEB_RATIO:
       if (!(value > 1 && value < 1000)) return invalid("Chebyshev ratio must be in (1, 1000)");
       for (auto* x : hs) x->cheb_ratio_ = value;
+      break;
+    case TMX_OPT_BAND:
+      if (value < 0 || value > 64 || value != std::floor(value)) return invalid("band level must be 0..64");
+      for (auto* x : hs) x->band_first_ = x->band_first_fp64_ = (int)value;
+      break;
+    case TMX_OPT_BAND_FINE_FILTER:
+      h->fl.band_fine_filter_ = value != 0.0;
       break;
     case TMX_OPT_OMEGA_ELAST:
     case TMX_OPT_OMEGA_FILTER:

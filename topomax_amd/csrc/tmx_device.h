@@ -179,4 +179,33 @@ static __device__ __forceinline__ double vy(const LoadDesc& d, int iy) { return 
 static __device__ __forceinline__ double mid(double a, double b) { return __dadd_rn(__dmul_rn(0.5, a), __dmul_rn(0.5, b)); }
 
 
+// ---------------------------------------------------------------- bilinear transfers
+// 1-D interpolation from coarse nodes to fine node f (nf fine cells, nc coarse cells).
+struct Interp1 {
+  int n;
+  int c[2];
+  double w[2];
+};
+
+__device__ __forceinline__ Interp1 interp1(int f, int nf, int nc) {
+  Interp1 r;
+  if (nf == nc) {  // not coarsened in this direction
+    r.n = 1; r.c[0] = f; r.w[0] = 1.0;
+  } else if ((f & 1) == 0) {
+    r.n = 1; r.c[0] = f >> 1; r.w[0] = 1.0;
+  } else if (f == nf) {  // odd fine count: last fine node is the last coarse node
+    r.n = 1; r.c[0] = nc; r.w[0] = 1.0;
+  } else {
+    r.n = 2; r.c[0] = (f - 1) >> 1; r.c[1] = (f + 1) >> 1; r.w[0] = r.w[1] = 0.5;
+  }
+  return r;
+}
+
+__device__ __forceinline__ int cpos(int I, int nf, int nc) { return nf == nc ? I : min(2 * I, nf); }
+
+__device__ __forceinline__ double interp1_weight(int f, int nf, int nc, int I) {
+  const Interp1 r = interp1(f, nf, nc);
+  return (r.c[0] == I ? r.w[0] : 0.0) + (r.n > 1 && r.c[1] == I ? r.w[1] : 0.0);
+}
+
 }  // namespace tmx

@@ -967,34 +967,6 @@ hipError_t prolong_nested(int nxc, int nyc, unsigned fixed, const T* ec, T* xf, 
   return hipGetLastError();
 }
 
-// 1-D interpolation from coarse nodes to fine node f (nf fine cells, nc coarse cells).
-struct Interp1 {
-  int n;
-  int c[2];
-  double w[2];
-};
-
-__device__ __forceinline__ Interp1 interp1(int f, int nf, int nc) {
-  Interp1 r;
-  if (nf == nc) {  // not coarsened in this direction
-    r.n = 1; r.c[0] = f; r.w[0] = 1.0;
-  } else if ((f & 1) == 0) {
-    r.n = 1; r.c[0] = f >> 1; r.w[0] = 1.0;
-  } else if (f == nf) {  // odd fine count: last fine node is the last coarse node
-    r.n = 1; r.c[0] = nc; r.w[0] = 1.0;
-  } else {
-    r.n = 2; r.c[0] = (f - 1) >> 1; r.c[1] = (f + 1) >> 1; r.w[0] = r.w[1] = 0.5;
-  }
-  return r;
-}
-
-__device__ __forceinline__ int cpos(int I, int nf, int nc) { return nf == nc ? I : min(2 * I, nf); }
-
-__device__ __forceinline__ double interp1_weight(int f, int nf, int nc, int I) {
-  const Interp1 r = interp1(f, nf, nc);
-  return (r.c[0] == I ? r.w[0] : 0.0) + (r.n > 1 && r.c[1] == I ? r.w[1] : 0.0);
-}
-
 template <typename T, int BS>
 __device__ __forceinline__ void restrict_bilinear_node(int nxf, int nyf, int nxc, int nyc,
                                                        unsigned fixed, const T* __restrict__ rf,

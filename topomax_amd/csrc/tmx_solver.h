@@ -212,6 +212,11 @@ class Hierarchy {
   int nu_pre_ = 2, nu_post_ = 2;  // Jacobi sweeps on the finest level
   int nu_coarse_ = 2;              // pre- and post-sweeps on every coarser level
   bool fused_ = true, use_tail_ = true;
+  // tiled multi-sweep kernels (one launch per half-visit of a level): on levels >= band_first_
+  // (FP32 V-cycles) or >= band_first_fp64_ (FP64) above the tail, and on the filter's finest
+  // level when band_fine_filter_ is set; a first level of 0 switches them off
+  int band_first_ = 1, band_first_fp64_ = 1;
+  bool band_fine_filter_ = true;
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
   void find_tail();
   int tail_ = -1;               // first level handled by the one-workgroup tail kernel

@@ -6,6 +6,7 @@
 // linear coarsenings whose Galerkin operators are explicit 9-point block stencils, down to a
 // coarsest grid solved by a dense inverse.  Smoother: damped block Jacobi.
 #include "tmx_solver.h"
+#include "tmx_band.h"
 
 #include <algorithm>
 #include <chrono>
@@ -446,6 +447,58 @@ static int tail_launch(Hierarchy& h, const T* b, T* x, hipStream_t s) {
   return 0;
 }
 
+// Band-kernel description of level l (below it level l + 1) with the largest tile that fits
+// the LDS budget and still gives about two workgroups per CU; small levels take the smallest
+// tile, where latency rather than the halo's extra work decides.  False when the level runs
+// launch per sweep instead.
+template <typename T>
+static bool band_level(const Hierarchy& h, int l, BandLevel<T>& B) {
+  const Level& L = h.levels_[l];
+  const Level& C = h.levels_[l + 1];
+  const int first = sizeof(T) == 8 ? h.band_first_fp64_ : h.band_first_;
+  if (first < 1) return false;
+  if (l == 0 ? !(h.phys_ == kFilter && h.band_fine_filter_) : l < first) return false;
+  std::memset(&B, 0, sizeof(B));
+  B.nx = L.nx;
+  B.ny = L.ny;
+  B.nxc = C.nx;
+  B.nyc = C.ny;
+  B.nested = C.nested_from_prev;
+  B.fixed = h.fixed_;
+  if (L.kind == kElem) {
+    if (h.phys_ == kElasticity) {
+      if (l == 0) return false;  // the finest elasticity level carries the nodal SIMP factor
+      B.src = kSrcCells;
+      B.cw = PrecBufs<T>::cw(L);
+      band_elastic_table<T>(B, L.hx, L.hy, h.lam_, h.mu_);
+    } else {
+      B.src = kSrcUnit;
+      band_filter_table<T>(B, L.hx, L.hy, h.eps_, h.bs_);
+    }
+  } else {
+    B.src = kSrcStencil;
+    B.st = PrecBufs<T>::st(L);
+  }
+  B.nu_down = l == 0 ? h.nu_pre_ : h.nu_coarse_;
+  B.nu_up = l == 0 ? h.nu_post_ : h.nu_coarse_;
+  if (B.nu_down < 1 || B.nu_down > kMaxNu || B.nu_up < 1 || B.nu_up > kMaxNu) return false;
+  for (int k = 0; k < B.nu_down; ++k) B.omega_down[k] = (T)h.sweep_omega(k, B.nu_down);
+  for (int k = 0; k < B.nu_up; ++k) B.omega_up[k] = (T)h.sweep_omega(k, B.nu_up);
+  static const int cand[][2] = {{32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8}};
+  bool found = false;
+  for (const auto& c : cand) {
+    const int nu = std::max(B.nu_down, B.nu_up);
+    if (band_lds_bytes(c[0], c[1], nu, h.bs_, sizeof(T), B.src, true) > kBandLdsBytes) continue;
+    const int tiles = band_tiles(L.nx, L.ny, c[0], c[1]);
+    if (l == 0 && tiles > kMaxPartials) continue;  // one r.z partial per tile
+    B.tx = c[0];
+    B.ty = c[1];
+    found = true;
+    if (tiles >= 480) break;
+  }
+  return found;
+}
+
 template <typename T>
 int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz) {
   const int last = (int)levels_.size() - 1;
@@ -457,6 +510,19 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   if (use_tail_ && l == tail_ && l > 0) return tail_launch<T>(*this, b, out, s);
   Level& L = levels_[l];
   Level& C = levels_[l + 1];
+  BandLevel<T> B;
+  if (band_level<T>(*this, l, B)) {
+    T* t = PrecBufs<T>::t(L);
+    T* cb = PrecBufs<T>::b(C);
+    T* cx = PrecBufs<T>::x(C);
+    if (bs_ == 2) TMX_HIP((band_down<T, 2>(B, b, t, cb, s)));
+    else TMX_HIP((band_down<T, 1>(B, b, t, cb, s)));
+    TMX_TRY(vcycle<T>(l + 1, cb, cx, s));
+    RedSlot* dot = l == 0 ? dot_bz : nullptr;
+    if (bs_ == 2) TMX_HIP((band_up<T, 2>(B, t, cx, b, out, dot, s)));
+    else TMX_HIP((band_up<T, 1>(B, t, cx, b, out, dot, s)));
+    return 0;
+  }
   T* buf[2] = {out, PrecBufs<T>::t(L)};
   const int npre = l == 0 ? nu_pre_ : nu_coarse_;
   const int npost = l == 0 ? nu_post_ : nu_coarse_;

@@ -419,7 +419,7 @@ class ElasticityProblem:
     def set_option(self, name: str, value):
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
-        cheb_ratio."""
+        cheb_ratio, band, band_fine_filter."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -430,7 +430,9 @@ class ElasticityProblem:
                "nu_coarse_filter": _lib.TMX_OPT_NU_COARSE_FILTER,
                "cheb_elast": _lib.TMX_OPT_CHEB_ELAST,
                "cheb_filter": _lib.TMX_OPT_CHEB_FILTER,
-               "cheb_ratio": _lib.TMX_OPT_CHEB_RATIO}[name]
+               "cheb_ratio": _lib.TMX_OPT_CHEB_RATIO,
+               "band": _lib.TMX_OPT_BAND,
+               "band_fine_filter": _lib.TMX_OPT_BAND_FINE_FILTER}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
