@@ -304,6 +304,7 @@ static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
   SolveStats st;
   SolveParams prm = h->fl_prm;
   if (o && o->precision >= 0) prm.precision = filter_precision(o->precision);
+  prm.verify = false;  // filter statistics are not reported
   if (!h->fl_rtol_set) {
     // two orders of magnitude below the elasticity solve's own tolerance, within [1e-12, 1e-10]:
     // the filtered design then adds nothing visible to the solve's error

@@ -123,7 +123,18 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
   __shared__ double sh[kBlock / kWave];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  // eight loads in flight per thread (the same left-to-right order of additions as one at a
+  // time), so a few thousand partials cost one memory latency rather than one per stride
+  const int bd = (int)blockDim.x;
+  int i = (int)threadIdx.x;
+  for (; i + 7 * bd < n; i += 8 * bd) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[i + k * bd];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; i < n; i += bd) s += part[i];
   s = wave_sum(s);
   __syncthreads();
   if (lane == 0) sh[w] = s;

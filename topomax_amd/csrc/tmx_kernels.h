@@ -26,6 +26,7 @@ constexpr int kMaxPartials = 16384;
 // Device scalar slots of one PCG run (indices into a double array).
 struct PcgSlots {
   int rz, pq, rr, alpha, beta, flag, tol2, iters;
+  int seq;  // iterations finished, stopped or not (reported to the host with the ring)
 };
 
 // ------------------------------------------------------------------ elasticity

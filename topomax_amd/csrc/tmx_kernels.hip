@@ -1891,10 +1891,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restric
     const bool stop = sc[q.flag] != 0.0;
     sc[q.beta] = stop ? 0.0 : rz / sc[q.rz];
     if (!stop) sc[q.rz] = rz;
-    if (hout) {  // stop flag, r.r and iteration count straight to pinned host memory
+    const double seq = sc[q.seq] + 1.0;
+    sc[q.seq] = seq;
+    if (hout) {  // stop flag, r.r and iteration count straight to pinned host memory, then
+                 // the sequence number the host polls for, once the rest is visible
       hout[0] = sc[q.flag];
       hout[1] = sc[q.rr];
       hout[2] = sc[q.iters];
+      __threadfence_system();
+      hout[3] = seq;
     }
   }
 }
@@ -1914,6 +1919,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__
     sc[q.beta] = 0.0;
     sc[q.flag] = (rz == rz) ? 0.0 : 3.0;
     sc[q.iters] = 0.0;
+    sc[q.seq] = 0.0;
   }
 }
 

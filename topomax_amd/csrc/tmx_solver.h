@@ -89,7 +89,7 @@ struct Workspace {
 // scalar slots used outside PCG
 enum : int { kScBB = 0, kScRR64 = 1, kScJ = 2, kScInt = 3, kScTmp = 4 };
 // PCG slots (flag, rr, iters contiguous for one async read)
-constexpr PcgSlots kPcg = {10, 11, 16, 12, 13, 15, 14, 17};
+constexpr PcgSlots kPcg = {10, 11, 16, 12, 13, 15, 14, 17, 18};
 
 enum LevelKind : int { kElem = 0, kStencil = 1, kDense = 2 };
 enum Phys : int { kElasticity = 0, kFilter = 1 };
@@ -122,6 +122,9 @@ struct SolveParams {
   int precision = kSolveFP64;
   double inner_rtol = 1e-5;
   int max_outer = 30;
+  // FP64 / mixed: report the true residual |b - A x| / |b| (one more operator application and
+  // a host round trip) rather than PCG's recursively updated one
+  bool verify = true;
 };
 
 struct SolveStats {
@@ -239,7 +242,6 @@ class Hierarchy {
   // captured PCG graphs: [fp64 / fp32 / fp64 with fp32 V-cycle][0 = init, 1 = even
   // iteration, 2 = odd iteration]
   hipGraphExec_t graphs_[3][3] = {};
-  hipEvent_t ev_[2] = {nullptr, nullptr};
   void drop_graphs();
   ~Hierarchy();
 

@@ -9,6 +9,7 @@
 #include "tmx_band.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -135,11 +136,7 @@ void Hierarchy::drop_graphs() {
       }
 }
 
-Hierarchy::~Hierarchy() {
-  drop_graphs();
-  for (auto& e : ev_)
-    if (e) (void)hipEventDestroy(e);
-}
+Hierarchy::~Hierarchy() { drop_graphs(); }
 
 void Hierarchy::set_rows(int rows64, int rows32) {
   drop_graphs();
@@ -181,8 +178,6 @@ int Hierarchy::alloc_vectors() {
     TMX_TRY(xi64buf_.alloc(levels_[0].nn));
     xi64_ = xi64buf_.get();
   }
-  TMX_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
-  TMX_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
   const size_t n0 = n();
   TMX_TRY(x64_.alloc(n0));
   TMX_TRY(r64_.alloc(n0));
@@ -736,20 +731,36 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
   TMX_HIP(set_scalar(sc, kPcg.rr, bnorm2, s));
   TMX_HIP(set_scalar(sc, kPcg.tol2, rtol * rtol * bnorm2, s));
   TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
-  // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) to
-  // ring slot (k-1)%2 in pinned host memory; event ev_[k%2] is recorded behind it.  While the
-  // host waits for iteration k it normally has iteration k+1 queued already, so the device
-  // never idles.  An iteration queued behind the one that converges would run in full for
-  // nothing (its stop flag makes it change nothing), so once the residual history predicts
-  // convergence within the next iteration the host stops queueing ahead and pays one host
-  // round trip instead.  At most two iterations are in flight, so a ring slot is rewritten
-  // (by iteration k+2) only after the host has read it.
+  // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) and
+  // then the sequence number k to ring slot (k-1)%2 in pinned host memory; the host polls that
+  // number (no event packets between the graphs).  While the host waits for iteration k it
+  // normally has iteration k+1 queued already, so the device never idles.  An iteration queued
+  // behind the one that converges would run in full for nothing (its stop flag makes it change
+  // nothing), so once the residual history predicts convergence within the next iteration the
+  // host stops queueing ahead and pays one host round trip instead.  At most two iterations are
+  // in flight, so a ring slot is rewritten (by iteration k+2) only after the host has read it.
   volatile double* ring = ws.host + Workspace::kScalars;
   auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
+  ring[3] = ring[7] = 0.0;  // no PCG work of this stream is in flight (solves end synchronized)
   auto launch = [&](int k) -> hipError_t {
-    hipError_t e = hipGraphLaunch(graphs_[prec][1 + ((k - 1) & 1)], s);
-    if (e == hipSuccess) e = hipEventRecord(ev_[k & 1], s);
-    return e;
+    return hipGraphLaunch(graphs_[prec][1 + ((k - 1) & 1)], s);
+  };
+  // wait until iteration k has reported; a stream that drains without the report is an error,
+  // never a hang
+  auto wait_iter = [&](int k) -> int {
+    volatile double* sl = slot(k);
+    for (long spins = 1; sl[3] < (double)k; ++spins) {
+      if ((spins & 1023) == 0) {
+        const hipError_t q = hipStreamQuery(s);
+        if (q == hipSuccess && sl[3] < (double)k) {
+          set_error("internal: PCG iteration " + std::to_string(k) + " did not report");
+          return -2;
+        }
+        if (q != hipSuccess && q != hipErrorNotReady) TMX_HIP(q);
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return 0;
   };
   const double tol2 = rtol * rtol * bnorm2;
   double flag = 0.0, rr_last = bnorm2, ratio = 1.0;
@@ -764,7 +775,7 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
       TMX_HIP(launch(launched + 1));
       ++launched;
     }
-    TMX_HIP(hipEventSynchronize(ev_[(done + 1) & 1]));
+    TMX_TRY(wait_iter(done + 1));
     ++done;
     flag = slot(done)[0];
     if (flag != 0.0) break;
@@ -777,7 +788,7 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
       ++launched;
     }
   }
-  if (launched > done) TMX_HIP(hipEventSynchronize(ev_[launched & 1]));
+  if (launched > done) TMX_TRY(wait_iter(launched));
   if (launched > 0) flag = slot(launched)[0];
   const double rrv = launched > 0 ? slot(launched)[1] : bnorm2;
   *iters = launched > 0 ? (int)slot(launched)[2] : 0;
@@ -948,12 +959,15 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       S.iterations += it2;
       S.outer = 2;
     }
-    if (rc == 0 || rc == -3) {  // report the true residual
+    if (rc == 0 || rc == -3) {
       TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
-      TMX_TRY(residual<double>(x, b, rr64_.get(), s));
-      TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScRR64, s));
-      TMX_TRY(ws.read(s, kScRR64, 1));
-      S.rel_residual = std::sqrt(ws.host[kScRR64] / bb);
+      S.rel_residual = rel;
+      if (prm.verify) {  // the true residual
+        TMX_TRY(residual<double>(x, b, rr64_.get(), s));
+        TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScRR64, s));
+        TMX_TRY(ws.read(s, kScRR64, 1));
+        S.rel_residual = std::sqrt(ws.host[kScRR64] / bb);
+      }
     }
   } else {
     TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
