@@ -213,7 +213,8 @@ def main():
 
     # dominant kernel of the mixed step: the FP32 fine-grid block-Jacobi sweep
     dom = kernels["smooth_fp32"]
-    roofline = {"kernel": "k_elast_sweep<float,true,kSmooth> (fine-grid FP32 block-Jacobi sweep)",
+    roofline = {"kernel": "k_elast_sweep<float,false,kSmooth> (finest-level FP32 block-Jacobi sweep "
+                          "of the V-cycle, per-cell preconditioner weights)",
                 "bound": "hbm", "achieved": dom["GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": dom["GBs"] / PEAK_HBM_GBS, "traffic": PMC_TRAFFIC_BYTES.get(N),
                 "algorithmic_bytes_per_launch": dom["bytes"], "avg_launch_us": dom["us"]}

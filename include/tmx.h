@@ -182,8 +182,11 @@ enum tmx_option {
   TMX_OPT_BAND = 13,            /* first level (>= 1) of the tiled multi-sweep V-cycle kernels
                                    (both families and precisions; default 1, FP64 elasticity
                                    2), 0: one launch per sweep / transfer instead */
-  TMX_OPT_BAND_FINE_FILTER = 14 /* 1 (default): the filter's finest level in the tiled
+  TMX_OPT_BAND_FINE_FILTER = 14, /* 1 (default): the filter's finest level in the tiled
                                    kernels too */
+  TMX_OPT_FINE_CELL_WEIGHTS = 15 /* elasticity finest-level V-cycle sweeps read per-cell
+                                   weights formed once per operator instead of the nodal xi:
+                                   bit 0 FP32 V-cycles (default on), bit 1 FP64 */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

@@ -2,8 +2,9 @@
 against the P2 oracle, and the reference's own golden run reproduced on the GPU.
 
 Tolerances: load vectors and operator applications to 1e-12 relative (exact integration on
-both sides); solves to 1e-11 relative residual (the FP64 residual floor of the P2 system is
-about 3e-12 here) must match the oracle's SuperLU solve to 1e-8;
+both sides); solves to 3e-11 relative residual (the FP64 residual floor of the P2 system is
+about 3e-12 at p=3 and close to 1e-11 at p=5, where the SIMP contrast is 1e-15) must match the
+oracle's SuperLU solve to 1e-8;
 the triangle golden (tests/test_data/FEM/triangle of the reference: objective after 24 EMD
 iterations at N=10, p=3) to 1e-10 relative in the objective, with the same iteration count,
 and 1e-8 in the sorted design (the golden design is stored in dolfin's dof order).
@@ -68,7 +69,7 @@ def test_p2_matvec(p):
 def test_p2_objective_and_gradient(p):
     nx, ny = 60, 20
     orc = oracle_problem(CANTILEVER, nx, ny, degree=2)
-    gp = gpu_problem(CANTILEVER, nx, ny, rtol=1e-11, degree=2)
+    gp = gpu_problem(CANTILEVER, nx, ny, rtol=3e-11, degree=2)
     orc.set_penalization(p)
     gp.set_penalization(p)
     rho = smoothed_density(nx, ny, seed=7, sigma=3.0)

@@ -136,6 +136,8 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   TMX_TRY(h->fl.init(kFilter, h->nx, h->ny, ab, cd, 0u, 0.0, 0.0, d->filter_radius, 128, s));
   TMX_TRY(h->fl.build_filter(s));
   h->el.nu_coarse_ = 3;  // with Chebyshev steps a third coarse sweep pays for itself
+  // the filter's band kernels make extra sweeps cheap: three on every level measured fastest
+  h->fl.nu_pre_ = h->fl.nu_post_ = h->fl.nu_coarse_ = 3;
   // FP64 element level 1 is faster launch per sweep: its band kernels are VALU-bound there
   h->el.band_first_fp64_ = 2;
   const size_t nv = (size_t)(h->nx + 1) * (h->ny + 1);
@@ -600,6 +602,10 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
     case TMX_OPT_BAND_FINE_FILTER:
       h->fl.band_fine_filter_ = value != 0.0;
       break;
+    case TMX_OPT_FINE_CELL_WEIGHTS:
+      if (value < 0 || value > 3 || value != std::floor(value)) return invalid("mask must be 0..3");
+      h->el.fine_cw_ = (unsigned)value;
+      break;
     case TMX_OPT_OMEGA_ELAST:
     case TMX_OPT_OMEGA_FILTER:
       if (!(value > 0 && value < 2)) return invalid("omega must be in (0, 2)");
@@ -749,9 +755,13 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
   (void)hipEventDestroy(e1);
   *avg_us = 1000.0 * ms / reps;
   if (bytes) {
-    // algorithmic bytes per node: apply u (2) + y (2) + xi (1); sweep adds b (2); filter x + y
+    // algorithmic bytes per node: apply u (2) + y (2) + xi (1); sweep adds b (2); filter x + y.
+    // A sweep reading the preconditioner's per-cell weights reads 2 values per cell instead of
+    // xi per node.
     const double per = kernel == 0 ? 5.0 : (kernel == 1 ? 7.0 : 2.0);
     *bytes = per * sz * (double)nv;
+    if (kernel == 1 && h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u)))
+      *bytes = 6.0 * sz * (double)nv + 2.0 * sz * (double)h->nx * (double)h->ny;
   }
   return TMX_OK;
 }

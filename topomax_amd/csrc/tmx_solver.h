@@ -230,6 +230,13 @@ class Hierarchy {
   DevBuf<double> cst64_;        // coarsest stencil (fp64)
   DevBuf<double> tmpst_;        // explicit stencil of the last element level
   DevBuf<double> cw0_;          // level-0 cell weights (restriction source)
+  DevBuf<float> cw0_32_;        // the same in FP32
+  // Finest-level V-cycle sweeps read the preconditioner's per-cell weights (cw0_ / cw0_32_,
+  // one pair per cell, formed once per operator) instead of re-forming them from the nodal xi
+  // in every sweep; bit 0: FP32 V-cycles, bit 1: FP64.  has_cw0_: the weights match the
+  // current operator.
+  unsigned fine_cw_ = 1u;
+  bool has_cw0_ = false, graphs_cw0_ = false;  // graphs_cw0_: has_cw0_ when graphs were captured
   const double* xi64_ = nullptr;  // = xi64buf_ once an operator is set
   DevBuf<double> xi64buf_;
   DevBuf<float> xi32_;

@@ -174,6 +174,7 @@ int Hierarchy::alloc_vectors() {
   if (tmp_nodes) TMX_TRY(tmpst_.alloc(9 * bs * bs * tmp_nodes));
   if (phys_ == kElasticity) {
     TMX_TRY(cw0_.alloc(2 * (size_t)levels_[0].nx * levels_[0].ny));
+    TMX_TRY(cw0_32_.alloc(2 * (size_t)levels_[0].nx * levels_[0].ny));
     TMX_TRY(xi32_.alloc(levels_[0].nn));
     TMX_TRY(xi64buf_.alloc(levels_[0].nn));
     xi64_ = xi64buf_.get();
@@ -211,6 +212,11 @@ static ElastOpArgs<T> elast_args(const Hierarchy& h, int l, const T* xi, const T
   a.fixed = h.fixed_;
   a.xi = l == 0 ? xi : nullptr;
   a.cellw = l == 0 ? nullptr : cw;
+  if (l == 0 && precond && h.has_cw0_ && (h.fine_cw_ & (fp64 ? 2u : 1u))) {
+    a.xi = nullptr;  // the preconditioner's weights of the finest cells, formed once
+    a.cellw = fp64 ? reinterpret_cast<const T*>(h.cw0_.get())
+                   : reinterpret_cast<const T*>(h.cw0_32_.get());
+  }
   a.p = h.p_;
   a.m = (T)h.m_;
   a.cp = (T)h.cp_;
@@ -269,6 +275,7 @@ int Hierarchy::set_fine_operator(const double* xi, int p, double m, hipStream_t 
   p_ = p;
   m_ = m;
   cp_ = (1.0 - m) * 2.0 / ((p + 1.0) * (p + 2.0));
+  has_cw0_ = false;
   const Level& L0 = levels_[0];
   if (xi != xi64buf_.get()) TMX_HIP(copy_vec<double>(L0.nn, xi, xi64buf_.get(), s));
   TMX_HIP((convert<double, float>(L0.nn, xi64buf_.get(), xi32_.get(), s)));
@@ -280,9 +287,9 @@ int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t 
   TMX_TRY(set_fine_operator(xi, p, m, s));
   const Level& L0 = levels_[0];
   const int last = (int)levels_.size() - 1;
-  if (last >= 1 && levels_[1].kind == kElem) {
-    TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p, m, cp_, m, cw0_.get(), s));
-  }
+  TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p, m, cp_, m, cw0_.get(), s));
+  TMX_HIP((convert<double, float>(cw0_.size(), cw0_.get(), cw0_32_.get(), s)));
+  has_cw0_ = true;
   for (int l = 1; l <= last; ++l) {
     Level& L = levels_[l];
     if (L.kind != kElem) break;
@@ -721,6 +728,8 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
                    int* iters, double* rel, hipStream_t s) {
   const size_t nv = n();
   const int prec = !std::is_same<T, PC>::value ? 2 : sizeof(T) == 8 ? 0 : 1;
+  if (graphs_cw0_ != has_cw0_) drop_graphs();  // captured with the other finest-level source
+  graphs_cw0_ = has_cw0_;
   for (int w = 0; w < 3; ++w)
     if (!graphs_[prec][w]) TMX_TRY((capture<T, PC>(*this, ws, w, &graphs_[prec][w], s)));
   double* sc = ws.sc();
@@ -888,9 +897,13 @@ int minres_core(Workspace& ws, size_t nv, const double* b, double* X, MinresBufs
   *rel = relres;
   if (trace)
     std::fprintf(stderr, "[tmx] minres n=%zu: %d its, rel %.3e, rc %d\n", nv, total, relres, rc);
-  if (rc != 0)
-    set_error("MINRES stopped at relative residual " + std::to_string(relres) +
-              " above the tolerance (iteration limit or round-off floor)");
+  if (rc != 0) {
+    char msg[160];
+    std::snprintf(msg, sizeof(msg),
+                  "MINRES stopped at relative residual %.3e above the tolerance %.3e (iteration "
+                  "limit or round-off floor)", relres, rtol);
+    set_error(msg);
+  }
   return rc;
 }
 
@@ -1030,8 +1043,12 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     if (rc == -3) {
       S.rel_residual = std::sqrt(rr / bb);
       if (S.rel_residual < prm.rtol) rc = 0;
-      else set_error("mixed-precision refinement stopped at relative residual " +
-                     std::to_string(S.rel_residual) + " above the tolerance");
+      else {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "mixed-precision refinement stopped at relative residual "
+                      "%.3e above the tolerance %.3e", S.rel_residual, prm.rtol);
+        set_error(msg);
+      }
     }
   }
   TMX_HIP(hipStreamSynchronize(s));

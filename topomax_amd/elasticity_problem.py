@@ -419,7 +419,7 @@ class ElasticityProblem:
     def set_option(self, name: str, value):
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
-        cheb_ratio, band, band_fine_filter."""
+        cheb_ratio, band, band_fine_filter, fine_cell_weights."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -432,7 +432,8 @@ class ElasticityProblem:
                "cheb_filter": _lib.TMX_OPT_CHEB_FILTER,
                "cheb_ratio": _lib.TMX_OPT_CHEB_RATIO,
                "band": _lib.TMX_OPT_BAND,
-               "band_fine_filter": _lib.TMX_OPT_BAND_FINE_FILTER}[name]
+               "band_fine_filter": _lib.TMX_OPT_BAND_FINE_FILTER,
+               "fine_cell_weights": _lib.TMX_OPT_FINE_CELL_WEIGHTS}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
