@@ -53,12 +53,21 @@ __host__ __device__ constexpr bool band_nz(int bs, int t, int i, int j, int e) {
   return lam || mu || dot;
 }
 
+// floor(t / d) for 0 <= t <= 256 and 1 <= d <= 256: a reciprocal estimate and one correction
+// step, a handful of instructions instead of a full integer division
+__device__ __forceinline__ int div_small(int t, int d) {
+  int q = (int)((float)t * __builtin_amdgcn_rcpf((float)d));
+  if ((q + 1) * d <= t) ++q;
+  if (q * d > t) --q;
+  return q;
+}
+
 // f(rx, ry) for every point of an RX x RY region, points spread over the workgroup's threads
 // in row-major order; coordinates are stepped by kBandThreads without a division per point
 template <typename F>
 __device__ __forceinline__ void for_region(int RX, int RY, F&& f) {
-  const int sy = kBandThreads / RX, sx = kBandThreads - sy * RX;
-  int ry = (int)threadIdx.x / RX, rx = (int)threadIdx.x - ry * RX;
+  const int sy = div_small(kBandThreads, RX), sx = kBandThreads - sy * RX;
+  int ry = div_small((int)threadIdx.x, RX), rx = (int)threadIdx.x - ry * RX;
   while (ry < RY) {
     f(rx, ry);
     rx += sx;
@@ -76,8 +85,8 @@ __device__ __forceinline__ void for_region(int RX, int RY, F&& f) {
 // slot u of the batch holds a point.
 template <int NB, typename FL, typename FS>
 __device__ __forceinline__ void for_region_batched(int RX, int RY, FL&& load, FS&& store) {
-  const int sy = kBandThreads / RX, sx = kBandThreads - sy * RX;
-  int ry = (int)threadIdx.x / RX, rx = (int)threadIdx.x - ry * RX;
+  const int sy = div_small(kBandThreads, RX), sx = kBandThreads - sy * RX;
+  int ry = div_small((int)threadIdx.x, RX), rx = (int)threadIdx.x - ry * RX;
   while (ry < RY) {
     int px[NB], py[NB];
 #pragma unroll
@@ -216,7 +225,7 @@ struct BandNode {
                                        int OX, int OY) {
     if constexpr (ONE) {
       const int t = (int)threadIdx.x;
-      ly = t / EX;
+      ly = div_small(t, EX);
       lx = t - ly * EX;
       gx = OX + lx;
       gy = OY + ly;
@@ -230,6 +239,47 @@ struct BandNode {
     }
   }
 };
+
+// Interior pass of a scalar level whose tile lies strictly inside the grid (every node has
+// the precomputed row, none is fixed): each thread updates two horizontally adjacent nodes
+// per step, walking down a column pair, so the two nodes share their neighbour loads and the
+// address arithmetic (one multiply per pass instead of one per node).
+template <typename T, bool ZERO, bool RES>
+__device__ __forceinline__ void band_pass_unit_pairs(const BandLevel<T>& L, const T* x,
+                                                     const T* bl, T* y, T omega, int EX, int EY,
+                                                     int s) {
+  const int RX = EX - 2 * s, RY = EY - 2 * s;
+  const int NP = (RX + 1) >> 1;          // column pairs of the region (<= kBandThreads)
+  const int rows = div_small(kBandThreads, NP);  // rows in flight across the workgroup
+  const int t = (int)threadIdx.x;
+  const int r0 = div_small(t, NP), cp = t - r0 * NP;
+  if (r0 >= rows) return;
+  const bool two = 2 * cp + 1 < RX;      // the pair's second node is in the region
+  const T a0 = L.urow[0], a1 = L.urow[1], a3 = L.urow[3], a4 = L.urow[4], a5 = L.urow[5];
+  const T a7 = L.urow[7], a8 = L.urow[8], dv = L.udinv[0];
+  const int step = rows * EX;
+  int i = (s + r0) * EX + s + 2 * cp;
+  for (int ry = r0; ry < RY; ry += rows, i += step) {
+    const int j2 = two ? i + 2 : i;      // (the second node's right column, kept in the tile)
+    T ra = bl[i], rb = two ? bl[i + 1] : T(0);
+    if constexpr (!ZERO) {
+      const T xs = x[i - EX - 1], xt = x[i - EX], xu = x[i - EX + 1];
+      const T xl = x[i - 1], xa = x[i], xb = x[i + 1], xr = x[j2];
+      const T xv = x[i + EX], xw = x[i + EX + 1], xz = x[j2 + EX];
+      ra -= a0 * xs + a1 * xt + a3 * xl + a4 * xa + a5 * xb + a7 * xv + a8 * xw;
+      rb -= a0 * xt + a1 * xu + a3 * xa + a4 * xb + a5 * xr + a7 * xw + a8 * xz;
+      if constexpr (!RES) {
+        ra = xa + omega * dv * ra;
+        rb = xb + omega * dv * rb;
+      }
+    } else {
+      ra = omega * dv * ra;
+      rb = omega * dv * rb;
+    }
+    y[i] = ra;
+    if (two) y[i + 1] = rb;
+  }
+}
 
 // one pass over the extended-tile nodes at distance >= s from its edge
 template <typename T, int BS, int SRC, bool ONE, bool ZERO, bool RES>
@@ -251,6 +301,10 @@ __device__ __forceinline__ void band_pass(const BandLevel<T>& L, BandNode<T, BS,
       // extended tile strictly inside the grid: every node has the interior row and none is
       // fixed, so the pass needs no per-node tests
       if (OX >= 1 && OY >= 1 && OX + EX <= L.nx && OY + EY <= L.ny) {
+        if constexpr (BS == 1) {
+          band_pass_unit_pairs<T, ZERO, RES>(L, x, bl, y, omega, EX, EY, s);
+          return;
+        }
         BandRow<T, BS> U;
 #pragma unroll
         for (int e = 0; e < 9 * BS * BS; ++e) U.a[e] = L.urow[e];

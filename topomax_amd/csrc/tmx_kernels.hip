@@ -1871,15 +1871,43 @@ template hipError_t pcg_update_xr_fused<float>(size_t, const double*, const PcgS
                                                const RedSlot&, float*, float*, const float*,
                                                const float*, RedSlot&, hipStream_t, float*);
 
-__global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
-                                                         const double* __restrict__ rrp, int nrr,
-                                                         const double* __restrict__ rzp, int nrz,
-                                                         double* __restrict__ sc, PcgSlots q,
-                                                         double* __restrict__ hout) {
-  const double pq = sum_partials(pqp, npq);
-  const double rr = sum_partials(rrp, nrr);
-  const double rz = sum_partials(rzp, nrz);
-  if (threadIdx.x == 0) {
+// One workgroup of kFinThreads: the three reductions share one pass over the partials (each
+// thread keeps its loads in flight) and one barrier, so the kernel costs about one memory
+// round trip plus the pinned-memory report.
+constexpr int kFinThreads = 1024;
+__global__ __launch_bounds__(kFinThreads) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
+                                                              const double* __restrict__ rrp, int nrr,
+                                                              const double* __restrict__ rzp, int nrz,
+                                                              double* __restrict__ sc, PcgSlots q,
+                                                              double* __restrict__ hout) {
+  __shared__ double sh[3][kFinThreads / kWave];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  const int n = max(npq, max(nrr, nrz));
+  for (int i = tid; i < n; i += kFinThreads) {
+    const double a = i < npq ? pqp[i] : 0.0;
+    const double b = i < nrr ? rrp[i] : 0.0;
+    const double c = i < nrz ? rzp[i] : 0.0;
+    v0 += a;
+    v1 += b;
+    v2 += c;
+  }
+  v0 = wave_sum(v0);
+  v1 = wave_sum(v1);
+  v2 = wave_sum(v2);
+  if (lane == 0) {
+    sh[0][w] = v0;
+    sh[1][w] = v1;
+    sh[2][w] = v2;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double pq = 0.0, rr = 0.0, rz = 0.0;
+    for (int k = 0; k < kFinThreads / kWave; ++k) {
+      pq += sh[0][k];
+      rr += sh[1][k];
+      rz += sh[2][k];
+    }
     if (sc[q.flag] == 0.0) {
       sc[q.pq] = pq;
       sc[q.rr] = rr;
@@ -1906,8 +1934,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_finalize(const double* __restric
 
 hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
                         const PcgSlots& q, hipStream_t s, double* host_out) {
-  k_pcg_finalize<<<1, kBlock, 0, s>>>(pq.partials, pq.nblocks, rr.partials, rr.nblocks,
-                                      rz.partials, rz.nblocks, sc, q, host_out);
+  k_pcg_finalize<<<1, kFinThreads, 0, s>>>(pq.partials, pq.nblocks, rr.partials, rr.nblocks,
+                                           rz.partials, rz.nblocks, sc, q, host_out);
   return hipGetLastError();
 }
 

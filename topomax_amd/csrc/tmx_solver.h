@@ -120,7 +120,7 @@ struct SolveParams {
   double rtol = 1e-10;
   int max_iter = 500;
   int precision = kSolveFP64;
-  double inner_rtol = 1e-5;
+  double inner_rtol = 1e-4;  // measured fastest at N=1000 (fewer inner iterations in total)
   int max_outer = 30;
   // FP64 / mixed: report the true residual |b - A x| / |b| (one more operator application and
   // a host round trip) rather than PCG's recursively updated one
