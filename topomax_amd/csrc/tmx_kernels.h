@@ -63,6 +63,10 @@ struct SweepExtra {
   T* pnew = nullptr;             // kApplyUpdate: updated direction
   const double* beta = nullptr;  // kApplyUpdate: device scalar
   const float* zlow = nullptr;   // kApplyUpdate: z in float (FP32 preconditioner output)
+  // kApplyUpdate: the previous iteration's iterate update, deferred to here because this
+  // kernel reads p_old anyway: xk += (*alpha) p_old
+  T* xk = nullptr;
+  const double* alpha = nullptr;
 };
 
 template <typename T>
@@ -195,6 +199,9 @@ hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s);
 // ------------------------------------------------------------------ vectors
 template <typename T>
 hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s);
+// y += (*a) x with a device scalar
+template <typename T>
+hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s);
 // out = sum of r's partials (one block)
 hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s);
 // dot_t + reduce_to
@@ -211,9 +218,10 @@ template <typename T>
 hipError_t pcg_update_xr(size_t n, const double* sc, int i_alpha, T* x, T* r, const T* p,
                          const T* q, RedSlot& rr, hipStream_t s);
 // as pcg_update_xr, with alpha = rz / sum(pq partials) formed by every block (0 once stopped
-// or if p.Ap <= 0); rlow (optional) receives a float copy of the new r
+// or if p.Ap <= 0); rlow (optional) receives a float copy of the new r.  x = nullptr: the x
+// update is deferred (alpha is stored in sc[q.alpha] for the next direction update)
 template <typename T>
-hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, const RedSlot& pq,
+hipError_t pcg_update_xr_fused(size_t n, double* sc, const PcgSlots& q, const RedSlot& pq,
                                T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s,
                                float* rlow = nullptr);
 // end of a fused iteration: pq, rr, rz from partials; flags; beta = rz_new / rz; rz = rz_new

@@ -148,6 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   V* PN = reinterpret_cast<V*>(ex.pnew);
   const float2* ZL = reinterpret_cast<const float2*>(ex.zlow);
   const T beta = AU ? (T)*ex.beta : T(0);
+  V* XK = reinterpret_cast<V*>(ex.xk);
+  const T alpha = (AU && ex.xk) ? (T)*ex.alpha : T(0);
   const ElastConst<T> k = a.k;
 
   auto row_fixed = [&](int iy) {
@@ -272,7 +274,16 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
       const V Ax = fx ? ub_raw : acc_b;
       if constexpr (OP == kApply || AU) {
         Y[idx] = Ax;
-        if constexpr (AU) PN[idx] = ub_raw;
+        if constexpr (AU) {
+          PN[idx] = ub_raw;
+          if (XK) {  // deferred x update of the previous iteration (p_old is an L2 hit here)
+            const V po = X[idx];
+            V xo = XK[idx];
+            xo.x += alpha * po.x;
+            xo.y += alpha * po.y;
+            XK[idx] = xo;
+          }
+        }
         if (want_dot) dot += (double)ub_raw.x * (double)Ax.x + (double)ub_raw.y * (double)Ax.y;
       } else if constexpr (OP == kResidual) {
         Y[idx] = B[idx] - Ax;
@@ -699,6 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
   constexpr bool AU = (OP == kApplyUpdate);
   const T* __restrict__ ec = ex.ec;
   const T beta = AU ? (T)*ex.beta : T(0);
+  const T alpha = (AU && ex.xk) ? (T)*ex.alpha : T(0);
   constexpr bool SMOOTH = (OP == kSmooth || OP == kSmoothZero || PS);
   constexpr int H = RR ? 2 : 1;
   const bool want_dot = dpart != nullptr;
@@ -780,7 +792,10 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
     } else if (cy >= r0 && outl) {
       if constexpr (OP == kApply || AU) {
         y[idx] = acc_b;
-        if constexpr (AU) ex.pnew[idx] = ub;
+        if constexpr (AU) {
+          ex.pnew[idx] = ub;
+          if (ex.xk) ex.xk[idx] += alpha * x[idx];  // deferred x update (see SweepExtra)
+        }
         if (want_dot) dot += (double)ub * (double)acc_b;
       } else if constexpr (OP == kResidual) {
         y[idx] = b[idx] - acc_b;
@@ -1736,6 +1751,23 @@ __global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__
   if (threadIdx.x == 0) *out = t;
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpy_dev(size_t n, const double* __restrict__ a,
+                                                     const T* __restrict__ x, T* __restrict__ y) {
+  const T av = (T)*a;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] += av * x[i];
+}
+
+template <typename T>
+hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s) {
+  k_axpy_dev<T><<<vec_grid(n), kBlock, 0, s>>>(n, a, x, y);
+  return hipGetLastError();
+}
+template hipError_t axpy_dev<double>(size_t, const double*, const double*, double*, hipStream_t);
+template hipError_t axpy_dev<float>(size_t, const double*, const float*, float*, hipStream_t);
+
 hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s) {
   k_reduce_to<<<1, kBlock, 0, s>>>(r.partials, r.nblocks, out);
   return hipGetLastError();
@@ -1830,7 +1862,7 @@ template hipError_t pcg_update_xr<float>(size_t, const double*, int, float*, flo
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
-    size_t n, const double* __restrict__ sc, PcgSlots q, const double* __restrict__ pqpart, int npq,
+    size_t n, double* __restrict__ sc, PcgSlots q, const double* __restrict__ pqpart, int npq,
     T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ qv,
     double* __restrict__ part, float* __restrict__ rlow) {
   __shared__ double alpha_sh;
@@ -1841,9 +1873,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
   }
   __syncthreads();
   const T al = (T)alpha_sh;
+  if (!x && blockIdx.x == 0 && threadIdx.x == 0) sc[q.alpha] = alpha_sh;  // for the deferred x
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    x[i] += al * p[i];
+    if (x) x[i] += al * p[i];
     const T ri = r[i] - al * qv[i];
     r[i] = ri;
     if (rlow) rlow[i] = (float)ri;
@@ -1855,7 +1888,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
 }
 
 template <typename T>
-hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, const RedSlot& pq,
+hipError_t pcg_update_xr_fused(size_t n, double* sc, const PcgSlots& q, const RedSlot& pq,
                                T* x, T* r, const T* p, const T* qv, RedSlot& rr, hipStream_t s,
                                float* rlow) {
   const unsigned g = vec_grid(n);
@@ -1864,10 +1897,10 @@ hipError_t pcg_update_xr_fused(size_t n, const double* sc, const PcgSlots& q, co
                                                rr.partials, rlow);
   return hipGetLastError();
 }
-template hipError_t pcg_update_xr_fused<double>(size_t, const double*, const PcgSlots&,
+template hipError_t pcg_update_xr_fused<double>(size_t, double*, const PcgSlots&,
                                                 const RedSlot&, double*, double*, const double*,
                                                 const double*, RedSlot&, hipStream_t, float*);
-template hipError_t pcg_update_xr_fused<float>(size_t, const double*, const PcgSlots&,
+template hipError_t pcg_update_xr_fused<float>(size_t, double*, const PcgSlots&,
                                                const RedSlot&, float*, float*, const float*,
                                                const float*, RedSlot&, hipStream_t, float*);
 
@@ -1948,6 +1981,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__
     sc[q.flag] = (rz == rz) ? 0.0 : 3.0;
     sc[q.iters] = 0.0;
     sc[q.seq] = 0.0;
+    sc[q.alpha] = 0.0;  // no deferred x update before the first direction
   }
 }
 

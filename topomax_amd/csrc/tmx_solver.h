@@ -168,9 +168,11 @@ class Hierarchy {
   int vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz = nullptr);
   // q = A p with p = z + beta p_old written to p_new; partials of p.q to dot.  z is read from
   // zlow (float) when that is given.
+  // With xk, also xk += (*alpha) p_old (the previous iteration's deferred iterate update).
   template <typename T>
   int apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta, RedSlot* dot,
-                   hipStream_t s, const float* zlow = nullptr);
+                   hipStream_t s, const float* zlow = nullptr, T* xk = nullptr,
+                   const double* alpha = nullptr);
 
   // PCG from x0 = 0 on A x = b with Krylov vectors in T and the V-cycle preconditioner in PC;
   // result left in the hierarchy's Krylov x buffer (x_buf<T>())

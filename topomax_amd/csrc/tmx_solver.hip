@@ -376,11 +376,14 @@ int Hierarchy::residual(const T* x, const T* b, T* r, hipStream_t s) {
 
 template <typename T>
 int Hierarchy::apply_update(const T* p_old, const T* z, T* q, T* p_new, const double* beta,
-                            RedSlot* dot, hipStream_t s, const float* zlow) {
+                            RedSlot* dot, hipStream_t s, const float* zlow, T* xk,
+                            const double* alpha) {
   SweepExtra<T> ex;
   ex.pnew = p_new;
   ex.beta = beta;
   ex.zlow = zlow;
+  ex.xk = xk;
+  ex.alpha = alpha;
   return level_op<T>(*this, 0, kApplyUpdate, p_old, zlow ? nullptr : z, q, dot, s, ex, false);
 }
 
@@ -695,9 +698,12 @@ static int enqueue_pcg_iteration(Hierarchy& h, Workspace& ws, int parity, hipStr
   double* sc = ws.sc();
   RedSlot pq = ws.region(0), rr = ws.region(1), rz = ws.region(2);
   constexpr bool split = !std::is_same<T, PC>::value;
+  // x += alpha p of the previous iteration rides along with the direction update (which reads
+  // p_old anyway); this iteration's alpha is left in sc[alpha] for the next one, and the last
+  // one is applied after the loop (pcg's flush)
   TMX_TRY(h.apply_update<T>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s,
-                            split ? h.z32_.get() : nullptr));
-  TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, x, r, p_new, q, rr, s,
+                            split ? h.z32_.get() : nullptr, x, sc + kPcg.alpha));
+  TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, nullptr, r, p_new, q, rr, s,
                                  split ? h.r32_.get() : nullptr));
   TMX_TRY((enqueue_precondition<T, PC>(h, &rz, false, s)));
   // the iteration graph of this parity reports into its own ring slot
@@ -798,6 +804,10 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
     }
   }
   if (launched > done) TMX_TRY(wait_iter(launched));
+  if (launched > 0) {  // the last iteration's deferred x += alpha p (alpha is 0 once stopped)
+    T* p_last = ((launched - 1) & 1) ? KrylovBufs<T>::p(*this) : KrylovBufs<T>::pb(*this);
+    TMX_HIP(axpy_dev<T>(nv, sc + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s));
+  }
   if (launched > 0) flag = slot(launched)[0];
   const double rrv = launched > 0 ? slot(launched)[1] : bnorm2;
   *iters = launched > 0 ? (int)slot(launched)[2] : 0;
@@ -1058,7 +1068,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
 
 template int Hierarchy::apply<double>(const double*, double*, RedSlot*, hipStream_t);
 template int Hierarchy::apply_update<double>(const double*, const double*, double*, double*,
-                                             const double*, RedSlot*, hipStream_t, const float*);
+                                             const double*, RedSlot*, hipStream_t, const float*,
+                                             double*, const double*);
 template int Hierarchy::apply<float>(const float*, float*, RedSlot*, hipStream_t);
 template double* Hierarchy::x_buf<double>();
 template float* Hierarchy::x_buf<float>();
