@@ -498,9 +498,11 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   else TMX_HIP(fill(ndofs(h), NAN, uv, h->stream));
   export_stats(st, stats);
   if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
-  double Jv = 0.0;
-  TMX_TRY(tmx_compliance(h, uv, &Jv));
-  if (J) *J = Jv;
+  // J = F.u reduced into a device scalar now, read after the step's final synchronisation
+  RedSlot gen = h->ws.region(3);
+  TMX_HIP(dot(ndofs(h), load_F(h), uv, gen, h->ws.sc() + kScJ, h->stream));
+  TMX_HIP(hipMemcpyAsync(h->ws.host + kScJ, h->ws.sc() + kScJ, sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
   if (w || Mw) {
     double* wv = w ? w : h->w_s.get();
     TMX_TRY(tmx_sensitivity(h, xiv, uv, h->b_s.get()));
@@ -510,6 +512,7 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
     if (rc == TMX_OK) rc = rc2;
   }
   TMX_HIP(hipStreamSynchronize(h->stream));
+  if (J) *J = h->ws.host[kScJ];
   return rc;
 }
 

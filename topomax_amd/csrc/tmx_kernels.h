@@ -229,6 +229,14 @@ hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz,
                         const PcgSlots& q, hipStream_t s, double* host_out = nullptr);
 // start of a fused run: rz from partials, beta = 0, flag = 0, iters = 0
 hipError_t pcg_start(const RedSlot& rz, double* sc, const PcgSlots& q, hipStream_t s);
+// PCG set-up from x0 = 0 without a host round trip: r = b, x = 0, rlow = float(b) (optional)
+// and partials of b.b in one pass; then one block forms bb, r.r = bb, tol2 = rtol2 * bb and the
+// start flag (1: b = 0, nothing to solve; 3: non-finite b; else 0), stores bb in sc[bb_slot]
+// and in pinned host memory (host_bb[0])
+template <typename T>
+hipError_t pcg_begin(size_t n, const T* b, T* x, T* r, float* rlow, RedSlot& bb, hipStream_t s);
+hipError_t pcg_begin_final(const RedSlot& bb, double* sc, const PcgSlots& q, double rtol2,
+                           int bb_slot, double* host_bb, hipStream_t s);
 // p = z + beta p, beta = sc[i_beta]
 template <typename T>
 hipError_t pcg_update_p(size_t n, const double* sc, int i_beta, const T* z, T* p, hipStream_t s);

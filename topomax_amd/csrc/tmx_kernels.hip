@@ -1978,11 +1978,66 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__
   if (threadIdx.x == 0) {
     sc[q.rz] = rz;
     sc[q.beta] = 0.0;
-    sc[q.flag] = (rz == rz) ? 0.0 : 3.0;
+    // keep a start flag set by pcg_begin_final (b = 0 or non-finite)
+    if (sc[q.flag] == 0.0 && !(rz == rz)) sc[q.flag] = 3.0;
     sc[q.iters] = 0.0;
     sc[q.seq] = 0.0;
     sc[q.alpha] = 0.0;  // no deferred x update before the first direction
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_begin(size_t n, const T* __restrict__ b,
+                                                      T* __restrict__ x, T* __restrict__ r,
+                                                      float* __restrict__ rlow,
+                                                      double* __restrict__ part) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const T v = b[i];
+    r[i] = v;
+    x[i] = T(0);
+    if (rlow) rlow[i] = (float)v;
+    s += (double)v * (double)v;
+  }
+  const double v1[1] = {s};
+  double* const o[1] = {part};
+  block_partials<1>(v1, o);
+}
+
+template <typename T>
+hipError_t pcg_begin(size_t n, const T* b, T* x, T* r, float* rlow, RedSlot& bb, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  bb.nblocks = (int)g;
+  k_pcg_begin<T><<<g, kBlock, 0, s>>>(n, b, x, r, rlow, bb.partials);
+  return hipGetLastError();
+}
+template hipError_t pcg_begin<double>(size_t, const double*, double*, double*, float*, RedSlot&,
+                                      hipStream_t);
+template hipError_t pcg_begin<float>(size_t, const float*, float*, float*, float*, RedSlot&,
+                                     hipStream_t);
+
+__global__ __launch_bounds__(kBlock) void k_pcg_begin_final(const double* __restrict__ part, int n,
+                                                            double* __restrict__ sc, PcgSlots q,
+                                                            double rtol2, int bb_slot,
+                                                            double* __restrict__ host_bb) {
+  const double bb = sum_partials(part, n);
+  if (threadIdx.x == 0) {
+    sc[bb_slot] = bb;
+    sc[q.rr] = bb;
+    sc[q.tol2] = rtol2 * bb;
+    sc[q.flag] = !(bb == bb) || bb > 1e300 ? 3.0 : (bb == 0.0 ? 1.0 : 0.0);
+    if (host_bb) {
+      host_bb[0] = bb;
+      __threadfence_system();
+    }
+  }
+}
+
+hipError_t pcg_begin_final(const RedSlot& bb, double* sc, const PcgSlots& q, double rtol2,
+                           int bb_slot, double* host_bb, hipStream_t s) {
+  k_pcg_begin_final<<<1, kBlock, 0, s>>>(bb.partials, bb.nblocks, sc, q, rtol2, bb_slot, host_bb);
+  return hipGetLastError();
 }
 
 hipError_t pcg_start(const RedSlot& rz, double* sc, const PcgSlots& q, hipStream_t s) {

@@ -88,6 +88,9 @@ struct Workspace {
 
 // scalar slots used outside PCG
 enum : int { kScBB = 0, kScRR64 = 1, kScJ = 2, kScInt = 3, kScTmp = 4 };
+// pinned ring (after the kScalars mirror): PCG report slots at 0 and 4, MINRES at 8, |b|^2 of
+// the current PCG solve at kRingBB
+constexpr int kRingBB = 24;
 // PCG slots (flag, rr, iters contiguous for one async read)
 constexpr PcgSlots kPcg = {10, 11, 16, 12, 13, 15, 14, 17, 18};
 
@@ -176,8 +179,9 @@ class Hierarchy {
 
   // PCG from x0 = 0 on A x = b with Krylov vectors in T and the V-cycle preconditioner in PC;
   // result left in the hierarchy's Krylov x buffer (x_buf<T>())
+  // (|b|^2 is formed on the device and returned in *bnorm2)
   template <typename T, typename PC = T>
-  int pcg(Workspace& ws, const T* b, double rtol, int max_iter, double bnorm2, int* iters,
+  int pcg(Workspace& ws, const T* b, double rtol, int max_iter, double* bnorm2, int* iters,
           double* rel, hipStream_t s);
   template <typename T>
   T* x_buf();

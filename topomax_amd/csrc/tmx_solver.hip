@@ -676,7 +676,7 @@ template <typename T, typename PC>
 static int enqueue_pcg_init(Hierarchy& h, Workspace& ws, hipStream_t s) {
   const size_t nv = h.n();
   RedSlot rz = ws.region(2);
-  TMX_TRY((enqueue_precondition<T, PC>(h, &rz, true, s)));
+  TMX_TRY((enqueue_precondition<T, PC>(h, &rz, false, s)));  // float r written by pcg_begin
   TMX_HIP(fill_zero_bytes(KrylovBufs<T>::p(h), sizeof(T) * nv, s));
   TMX_HIP(pcg_start(rz, ws.sc(), kPcg, s));
   return 0;
@@ -730,7 +730,7 @@ static int capture(Hierarchy& h, Workspace& ws, int which, hipGraphExec_t* out, 
 }
 
 template <typename T, typename PC>
-int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double bnorm2,
+int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double* bnorm2,
                    int* iters, double* rel, hipStream_t s) {
   const size_t nv = n();
   const int prec = !std::is_same<T, PC>::value ? 2 : sizeof(T) == 8 ? 0 : 1;
@@ -741,10 +741,16 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
   double* sc = ws.sc();
   *iters = 0;
   *rel = 1.0;
-  TMX_HIP(fill_zero_bytes(KrylovBufs<T>::x(*this), sizeof(T) * nv, s));
-  TMX_HIP(copy_vec<T>(nv, b, KrylovBufs<T>::r(*this), s));
-  TMX_HIP(set_scalar(sc, kPcg.rr, bnorm2, s));
-  TMX_HIP(set_scalar(sc, kPcg.tol2, rtol * rtol * bnorm2, s));
+  // r = b, x = 0 (and the float r of a mixed V-cycle) with |b|^2, the tolerance and the start
+  // flag formed on the device: no host round trip before the first iteration
+  volatile double* host_bb = ws.host + Workspace::kScalars + kRingBB;
+  host_bb[0] = std::numeric_limits<double>::quiet_NaN();
+  RedSlot bbs = ws.region(3);
+  constexpr bool split = !std::is_same<T, PC>::value;
+  TMX_HIP(pcg_begin<T>(nv, b, KrylovBufs<T>::x(*this), KrylovBufs<T>::r(*this),
+                       split ? r32_.get() : nullptr, bbs, s));
+  TMX_HIP(pcg_begin_final(bbs, sc, kPcg, rtol * rtol, kScBB,
+                          ws.host_dev + Workspace::kScalars + kRingBB, s));
   TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
   // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) and
   // then the sequence number k to ring slot (k-1)%2 in pinned host memory; the host polls that
@@ -777,8 +783,8 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
     std::atomic_thread_fence(std::memory_order_acquire);
     return 0;
   };
-  const double tol2 = rtol * rtol * bnorm2;
-  double flag = 0.0, rr_last = bnorm2, ratio = 1.0;
+  double bb = -1.0, tol2 = 0.0;  // |b|^2 from the ring, read with the first report
+  double flag = 0.0, rr_last = 0.0, ratio = 1.0;
   bool near = false;
   int launched = 0, done = 0;
   if (max_iter >= 1) {
@@ -792,6 +798,11 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
     }
     TMX_TRY(wait_iter(done + 1));
     ++done;
+    if (bb < 0.0) {
+      bb = host_bb[0];
+      tol2 = rtol * rtol * bb;
+      rr_last = bb;
+    }
     flag = slot(done)[0];
     if (flag != 0.0) break;
     const double rr = slot(done)[1];
@@ -808,10 +819,17 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
     T* p_last = ((launched - 1) & 1) ? KrylovBufs<T>::p(*this) : KrylovBufs<T>::pb(*this);
     TMX_HIP(axpy_dev<T>(nv, sc + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s));
   }
+  if (launched == 0) {  // no iteration reported |b|^2: read it
+    TMX_TRY(ws.read(s, kScBB, 1));
+    bb = ws.host[kScBB];
+  } else if (bb < 0.0) {
+    bb = host_bb[0];
+  }
+  *bnorm2 = bb;
   if (launched > 0) flag = slot(launched)[0];
-  const double rrv = launched > 0 ? slot(launched)[1] : bnorm2;
+  const double rrv = launched > 0 ? slot(launched)[1] : bb;
   *iters = launched > 0 ? (int)slot(launched)[2] : 0;
-  *rel = std::sqrt(std::max(rrv, 0.0) / bnorm2);
+  *rel = bb > 0.0 ? std::sqrt(std::max(rrv, 0.0) / bb) : 0.0;
   if (g_trace)
     std::fprintf(stderr, "[tmx] pcg<%s> %s n=%zu: %d its, rel %.3e, flag %g\n",
                  prec == 2 ? "f64/f32" : prec == 0 ? "f64" : "f32", phys_ == kElasticity ? "elast" : "filter", nv,
@@ -822,7 +840,7 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double 
     return -4;
   }
   if (flag == 3.0) {
-    set_error("non-finite residual in PCG");
+    set_error("non-finite right-hand side or residual in PCG");
     return -5;
   }
   set_error("PCG did not reach the tolerance within the iteration limit");
@@ -940,30 +958,20 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   const size_t nv = n();
   auto t0 = std::chrono::steady_clock::now();
   RedSlot gen = ws.region(3);
-  TMX_HIP(dot(nv, b, b, gen, ws.sc() + kScBB, s));
-  TMX_TRY(ws.read(s, kScBB, 1));
-  const double bb = ws.host[kScBB];
   SolveStats local;
   SolveStats& S = st ? *st : local;
   S.levels = levels();
   S.iterations = 0;
   S.outer = 0;
-  if (!std::isfinite(bb)) {
-    set_error("non-finite right-hand side");
-    return -5;
-  }
-  if (bb == 0.0) {
-    TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
-    S.rel_residual = 0.0;
-    return 0;
-  }
   int rc = 0;
   if (prm.precision == kSolveFP64 || prm.precision == kSolveMixed) {
+    // |b|^2 is formed on the device inside pcg (b = 0 converges at once with x = 0; a
+    // non-finite b stops with the non-finite code)
     int it = 0;
-    double rel = 0;
+    double rel = 0, bb = 0;
     rc = prm.precision == kSolveFP64
-             ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s)
-             : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bb, &it, &rel, s);
+             ? pcg<double>(ws, b, prm.rtol, prm.max_iter, &bb, &it, &rel, s)
+             : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, &bb, &it, &rel, s);
     S.iterations = it;
     S.outer = 1;
     if (rc == -4 && cheb_lmax_ > 0.0) {  // Chebyshev bound too low for this operator: Jacobi
@@ -971,8 +979,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       drop_graphs();
       int it2 = 0;
       rc = prm.precision == kSolveFP64
-               ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bb, &it2, &rel, s)
-               : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bb, &it2, &rel, s);
+               ? pcg<double>(ws, b, prm.rtol, prm.max_iter, &bb, &it2, &rel, s)
+               : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, &bb, &it2, &rel, s);
       S.iterations += it2;
       it += it2;
     }
@@ -985,7 +993,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     if (rc == 0 || rc == -3) {
       TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
       S.rel_residual = rel;
-      if (prm.verify) {  // the true residual
+      if (prm.verify && bb > 0.0) {  // the true residual
         TMX_TRY(residual<double>(x, b, rr64_.get(), s));
         TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScRR64, s));
         TMX_TRY(ws.read(s, kScRR64, 1));
@@ -993,6 +1001,18 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       }
     }
   } else {
+    TMX_HIP(dot(nv, b, b, gen, ws.sc() + kScBB, s));
+    TMX_TRY(ws.read(s, kScBB, 1));
+    const double bb = ws.host[kScBB];
+    if (!std::isfinite(bb)) {
+      set_error("non-finite right-hand side");
+      return -5;
+    }
+    if (bb == 0.0) {
+      TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
+      S.rel_residual = 0.0;
+      return 0;
+    }
     TMX_HIP(fill_zero_bytes(x, sizeof(double) * nv, s));
     TMX_HIP(copy_vec<double>(nv, b, rr64_.get(), s));
     double rr = bb;
@@ -1019,7 +1039,8 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       int it = 0;
       double rel = 0;
       const double inner = std::max(prm.inner_rtol, 0.5 * prm.rtol / S.rel_residual);
-      int irc = pcg<float>(ws, rin, inner, budget, 1.0, &it, &rel, s);
+      double bbf = 0.0;
+      int irc = pcg<float>(ws, rin, inner, budget, &bbf, &it, &rel, s);
       if (g_trace)
         std::fprintf(stderr, "[tmx] refine %d: |r|/|b| = %.3e, inner its %d rel %.3e rc %d\n",
                      outer, S.rel_residual, it, rel, irc);
