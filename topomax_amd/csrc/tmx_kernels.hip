@@ -155,11 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   auto row_fixed = [&](int iy) {
     return colfix || ((fixed & kBottom) && iy == 0) || ((fixed & kTop) && iy == ny);
   };
+  V po_next = {T(0), T(0)};  // AU: raw p_old of the row ld_x last loaded (for the deferred x)
   auto ld_x = [&](int iy, V& raw) -> V {
     V v = {T(0), T(0)};
     const bool ok = colv && iy >= 0 && iy <= ny;
     if (OP != kSmoothZero && ok) v = X[(unsigned)(iy * nvx + c)];
     if constexpr (AU) {
+      po_next = v;
       if (ok) {
         V z;
         if (ZL) {
@@ -195,6 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   int cy = cy_begin;
   V ub_raw;
   V ub = ld_x(cy, ub_raw);
+  V po_b = po_next;
   T zb = ld_xi(cy);
   V ubr = lane_down(ub);
   T zbr = lane_down(zb);
@@ -206,6 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   for (; cy < cy_end; ++cy) {
     V ut_raw;
     const V ut = ld_x(cy + 1, ut_raw);
+    const V po_t = po_next;
     const T zt = ld_xi(cy + 1);
     const V utr = lane_down(ut);
     const T ztr = lane_down(zt);
@@ -276,11 +280,10 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
         Y[idx] = Ax;
         if constexpr (AU) {
           PN[idx] = ub_raw;
-          if (XK) {  // deferred x update of the previous iteration (p_old is an L2 hit here)
-            const V po = X[idx];
+          if (XK) {  // deferred x update of the previous iteration (p_old kept from the load)
             V xo = XK[idx];
-            xo.x += alpha * po.x;
-            xo.y += alpha * po.y;
+            xo.x += alpha * po_b.x;
+            xo.y += alpha * po_b.y;
             XK[idx] = xo;
           }
         }
@@ -307,6 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     d_b = d_t;
     ub = ut;
     ub_raw = ut_raw;
+    po_b = po_t;
     ubr = utr;
     zb = zt;
     zbr = ztr;
@@ -728,11 +732,13 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
   const int rend = min(r0 + sh.rows, ny + 1);
   const int cy_begin = RR ? r0 - 2 : r0 - 1;
   const int cy_end = !wave_ok ? cy_begin : (RR ? min(rend, ny) + 1 : rend);
+  T po_next = T(0);  // AU: raw p_old of the row ld last loaded (for the deferred x)
   auto ld = [&](int iy) -> T {
     T v = T(0);
     const bool ok = colv && iy >= 0 && iy <= ny;
     if (OP != kSmoothZero && ok) v = x[(size_t)iy * nvx + c];
     if constexpr (AU) {
+      po_next = v;
       if (ok) v = (ex.zlow ? (T)ex.zlow[(size_t)iy * nvx + c] : b[(size_t)iy * nvx + c]) + beta * v;
     }
     if constexpr (PS) {
@@ -749,11 +755,13 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
   const T dfull = ka + kb + T(4) * mA, dhalf = ka + kb + T(2) * mA;
   int cy = cy_begin;
   T ub = ld(cy), ubr = lane_down(ub);
+  T po_b = po_next;
   T acc_b = T(0), d_b = T(0);
   T rA = T(0), rB = T(0);
   double dot = 0.0;
   for (; cy < cy_end; ++cy) {
     const T ut = ld(cy + 1), utr = lane_down(ut);
+    const T po_t = po_next;
     const bool cell_ok = cellc && cy >= 0 && cy < ny;
     const T w = cell_ok ? T(1) : T(0);
     T acc_t = T(0);
@@ -794,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
         y[idx] = acc_b;
         if constexpr (AU) {
           ex.pnew[idx] = ub;
-          if (ex.xk) ex.xk[idx] += alpha * x[idx];  // deferred x update (see SweepExtra)
+          if (ex.xk) ex.xk[idx] += alpha * po_b;  // deferred x update (see SweepExtra)
         }
         if (want_dot) dot += (double)ub * (double)acc_b;
       } else if constexpr (OP == kResidual) {
@@ -810,6 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
     d_b = d_t;
     ub = ut;
     ubr = utr;
+    po_b = po_t;
   }
   if constexpr (!RR) {
     if (want_dot) {
