@@ -1407,11 +1407,13 @@ __device__ __forceinline__ void tail_node(const TailLevel<T>& L, const TailRow<T
 }
 
 // A node's stencil row stays in registers across a level's sweeps when it is small enough
-// (36 floats, 9 doubles); the FP64 2x2-block row (36 doubles) is re-read from global memory
-// (an L2 hit) for each sweep instead, which keeps the kernel within 128 VGPRs without spills.
+// (36 floats, 9 doubles) or the workgroup is at most 256 threads; otherwise (the FP64 2x2-block
+// row, 36 doubles, at 1024 threads) it is re-read from global memory (an L2 hit) for each
+// sweep, which keeps the kernel within 128 VGPRs without spills.  (A 256-thread tail with the
+// 47x16 level moved to the band kernels measured slower at N=1000.)
 template <typename T, int BS>
 struct TailRowHolder {
-  static constexpr bool kKeep = sizeof(T) * 9 * BS * BS <= 144;
+  static constexpr bool kKeep = sizeof(T) * 9 * BS * BS <= 144 || kTailThreads <= 256;
   TailRow<T, BS> row;
   __device__ __forceinline__ void level(const TailLevel<T>& L, int v) {
     if constexpr (kKeep) tail_load_row<T, BS>(L, v, row);
@@ -1471,16 +1473,19 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
     const int n = d.cn;
     const T* cb = lds + C.off;
     T* cx = lds + C.off + n;
-    const int row = tid >> 3, part = tid & 7;
-    double s = 0.0;
-    if (row < n) {
-      const double* arow = d.dense + (size_t)row * n;
-      for (int j = part; j < n; j += 8) s += arow[j] * (double)cb[j];
+    const int part = tid & 7;
+    for (int row0 = 0; row0 < n; row0 += kTailThreads / 8) {  // (all lanes loop alike)
+      const int row = row0 + (tid >> 3);
+      double s = 0.0;
+      if (row < n) {
+        const double* arow = d.dense + (size_t)row * n;
+        for (int j = part; j < n; j += 8) s += arow[j] * (double)cb[j];
+      }
+      s += __shfl_down(s, 4, 8);
+      s += __shfl_down(s, 2, 8);
+      s += __shfl_down(s, 1, 8);
+      if (row < n && part == 0) cx[row] = (T)s;
     }
-    s += __shfl_down(s, 4, 8);
-    s += __shfl_down(s, 2, 8);
-    s += __shfl_down(s, 1, 8);
-    if (row < n && part == 0) cx[row] = (T)s;
     __syncthreads();
   }
   for (int l = nl - 2; l >= 0; --l) {
