@@ -199,7 +199,7 @@ int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
 
 /* Batched objective + gradient over many independent problems (uneven sizes, penalties and
  * loads; each problem keeps its own penalty, tiles and operators).  Designs run concurrently,
- * each on its own stream, driven by `workers` host threads (default 4), largest first, so the
+ * each on its own stream, driven by `workers` host threads (default 8), largest first, so the
  * latency-bound coarse-grid kernels of different designs overlap.  The handles must not be
  * used by other threads during the call.  rho[i], w[i] are device pointers (n_v of design i);
  * J_host[i] receives the compliance (NaN for a non-finite design).  stats may be NULL or

@@ -489,7 +489,7 @@ class ProblemBatch:
     different designs overlaps on the GPU.  Results are identical to calling each problem's
     :meth:`ElasticityProblem.objective_and_gradient` alone."""
 
-    def __init__(self, problems, workers: int = 4):
+    def __init__(self, problems, workers: int = 8):
         self.problems = list(problems)
         if not self.problems:
             raise ValueError("a batch needs at least one problem")
