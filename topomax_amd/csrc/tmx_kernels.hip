@@ -1529,74 +1529,69 @@ template hipError_t tail_vcycle<float, 1>(const TailDesc<float>&, size_t, hipStr
 template hipError_t tail_vcycle<float, 2>(const TailDesc<float>&, size_t, hipStream_t);
 
 // Galerkin product R A P for tensor-product linear (bilinear) coarsening, one thread per
-// coarse node (I, J).  The fine nodes f that interpolate from (I, J) lie within one node of
-// its fine position p, and their stencil neighbours g within two; the 1-D interpolation
-// weights from those five fine positions per axis to the three coarse positions I-1..I+1
-// (and J-1..J+1) are tabulated once, so every loop below has compile-time trip counts and
-// indices: the 9 x 9 stencil entries of the 3 x 3 fine rows are loaded unconditionally
-// (rows clamped into the grid; their weight is zero then) and accumulated in registers.
+// coarse node (I, J) and coarse stencil direction kc (nine threads per node): the thread forms
+// the coarse block A_c(I,J)[kc] = sum over fine rows f near (I, J) and their stencil neighbours
+// g of P(f,I) A_f(f,g) P(g, (I,J)+kc).  The fine nodes f that interpolate from (I, J) lie
+// within one node of its fine position p, and their stencil neighbours g within two; the 1-D
+// interpolation weights from those five fine positions per axis to the three coarse positions
+// I-1..I+1 (and J-1..J+1) are tabulated, so every loop has compile-time trip counts and
+// indices: the 9 x 9 stencil entries of the 3 x 3 fine rows are loaded unconditionally (rows
+// clamped into the grid; their weight is zero then).  The nine threads of a node read the
+// same fine entries (cache hits), and the per-thread work is a ninth of a node's.
 template <int BS>
 __global__ __launch_bounds__(64) void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                               const double* __restrict__ stf, double* __restrict__ stc) {
   const unsigned nnf = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
   const unsigned nnc = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned i = t / 9u;
+  const int kc = (int)(t - 9u * i);
   if (i >= nnc) return;
   const int I = (int)(i % (unsigned)(nxc + 1)), J = (int)(i / (unsigned)(nxc + 1));
   constexpr int B2 = BS * BS;
-  double acc[9][B2];
+  double acc[B2];
 #pragma unroll
-  for (int k = 0; k < 9; ++k)
-#pragma unroll
-    for (int e = 0; e < B2; ++e) acc[k][e] = 0.0;
+  for (int e = 0; e < B2; ++e) acc[e] = 0.0;
   const int px = cpos(I, nxf, nxc), py = cpos(J, nyf, nyc);
-  // WX[a][d]: weight of coarse column I-1+d at fine column px-2+a (zero outside the grid)
-  float WX[5][3], WY[5][3];
+  const int dxc = kc % 3, dyc = kc / 3;  // coarse neighbour (I-1+dxc, J-1+dyc)
+  // wx[a]: weight of coarse column I-1+dxc at fine column px-2+a (zero outside the grid);
+  // wx1[a]: weight of coarse column I at it (the row side of R)
+  float wx[5], wy[5], wx1[5], wy1[5];
 #pragma unroll
-  for (int a = 0; a < 5; ++a)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int gx = px - 2 + a, gy = py - 2 + a;
-      WX[a][d] = (gx < 0 || gx > nxf) ? 0.f : (float)interp1_weight(gx, nxf, nxc, I - 1 + d);
-      WY[a][d] = (gy < 0 || gy > nyf) ? 0.f : (float)interp1_weight(gy, nyf, nyc, J - 1 + d);
-    }
-  float live[9];  // coarse neighbour not fixed
-#pragma unroll
-  for (int kc = 0; kc < 9; ++kc) {
-    const int cx = I + kc % 3 - 1, cy = J + kc / 3 - 1;
-    live[kc] = node_fixed(fixed, cx, cy, nxc, nyc) ? 0.f : 1.f;
+  for (int a = 0; a < 5; ++a) {
+    const int gx = px - 2 + a, gy = py - 2 + a;
+    const bool ox = gx < 0 || gx > nxf, oy = gy < 0 || gy > nyf;
+    wx[a] = ox ? 0.f : (float)interp1_weight(gx, nxf, nxc, I - 1 + dxc);
+    wy[a] = oy ? 0.f : (float)interp1_weight(gy, nyf, nyc, J - 1 + dyc);
+    wx1[a] = ox ? 0.f : (float)interp1_weight(gx, nxf, nxc, I);
+    wy1[a] = oy ? 0.f : (float)interp1_weight(gy, nyf, nyc, J);
   }
   const bool self_fixed = node_fixed(fixed, I, J, nxc, nyc);
+  const bool live = !node_fixed(fixed, I - 1 + dxc, J - 1 + dyc, nxc, nyc);
+  if (!self_fixed && live) {
 #pragma unroll
-  for (int fyi = 0; fyi < 3; ++fyi)
+    for (int fyi = 0; fyi < 3; ++fyi)
 #pragma unroll
-    for (int fxi = 0; fxi < 3; ++fxi) {
-      // fine row f = (px-1+fxi, py-1+fyi): table index fxi+1, fyi+1
-      const double wf = self_fixed ? 0.0 : (double)(WX[fxi + 1][1] * WY[fyi + 1][1]);
-      const int fx = min(max(px - 1 + fxi, 0), nxf), fy = min(max(py - 1 + fyi, 0), nyf);
-      const unsigned f = (unsigned)fy * (unsigned)(nxf + 1) + (unsigned)fx;
+      for (int fxi = 0; fxi < 3; ++fxi) {
+        // fine row f = (px-1+fxi, py-1+fyi): table index fxi+1, fyi+1
+        const double wf = (double)(wx1[fxi + 1] * wy1[fyi + 1]);
+        const int fx = min(max(px - 1 + fxi, 0), nxf), fy = min(max(py - 1 + fyi, 0), nyf);
+        const unsigned f = (unsigned)fy * (unsigned)(nxf + 1) + (unsigned)fx;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        const int ax = fxi + k % 3, ay = fyi + k / 3;  // neighbour g at table index (ax, ay)
-        double A[B2];
+        for (int k = 0; k < 9; ++k) {
+          const int ax = fxi + k % 3, ay = fyi + k / 3;  // neighbour g at table index (ax, ay)
+          const double w = wf * (double)(wx[ax] * wy[ay]);
 #pragma unroll
-        for (int e = 0; e < B2; ++e) A[e] = stf[(size_t)(k * B2 + e) * nnf + f];
-#pragma unroll
-        for (int kc = 0; kc < 9; ++kc) {
-          const double w = wf * (double)(WX[ax][kc % 3] * WY[ay][kc / 3] * live[kc]);
-#pragma unroll
-          for (int e = 0; e < B2; ++e) acc[kc][e] += w * A[e];
+          for (int e = 0; e < B2; ++e) acc[e] += w * stf[(size_t)(k * B2 + e) * nnf + f];
         }
       }
-    }
-  if (self_fixed) {
+  }
+  if (self_fixed && kc == 4) {
 #pragma unroll
-    for (int a = 0; a < BS; ++a) acc[4][a * BS + a] = 1.0;
+    for (int a = 0; a < BS; ++a) acc[a * BS + a] = 1.0;
   }
 #pragma unroll
-  for (int k = 0; k < 9; ++k)
-#pragma unroll
-    for (int e = 0; e < B2; ++e) stc[(size_t)(k * B2 + e) * nnc + i] = acc[k][e];
+  for (int e = 0; e < B2; ++e) stc[(size_t)(kc * B2 + e) * nnc + i] = acc[e];
 }
 
 template <int BS>
@@ -1604,7 +1599,7 @@ hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned 
                        const double* stf, double* stc, hipStream_t s) {
   if (nested) return hipErrorInvalidValue;  // 9-point levels are always bilinear coarsenings
   if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
-  k_stencil_rap<BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1), 64), 64, 0, s>>>(
+  k_stencil_rap<BS><<<grid1d(9 * (size_t)(nxc + 1) * (nyc + 1), 64), 64, 0, s>>>(
       nxf, nyf, nxc, nyc, fixed, stf, stc);
   return hipGetLastError();
 }
