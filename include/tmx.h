@@ -184,9 +184,11 @@ enum tmx_option {
                                    2), 0: one launch per sweep / transfer instead */
   TMX_OPT_BAND_FINE_FILTER = 14, /* 1 (default): the filter's finest level in the tiled
                                    kernels too */
-  TMX_OPT_FINE_CELL_WEIGHTS = 15 /* elasticity finest-level V-cycle sweeps read per-cell
+  TMX_OPT_FINE_CELL_WEIGHTS = 15, /* elasticity finest-level V-cycle sweeps read per-cell
                                    weights formed once per operator instead of the nodal xi:
                                    bit 0 FP32 V-cycles (default on), bit 1 FP64 */
+  TMX_OPT_GRAPHS = 16           /* 1: PCG iterations replay captured hipGraphs; 0 (default):
+                                   their kernels are launched one by one (measured faster) */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the
@@ -199,8 +201,8 @@ int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
 
 /* Batched objective + gradient over many independent problems (uneven sizes, penalties and
  * loads; each problem keeps its own penalty, tiles and operators).  Designs run concurrently,
- * each on its own stream, driven by `workers` host threads (default 8), largest first, so the
- * latency-bound coarse-grid kernels of different designs overlap.  The handles must not be
+ * each on its own stream, driven by `workers` host threads (default min(16, cores), at least
+ * 4), largest first, so the latency-bound coarse-grid kernels of different designs overlap.  The handles must not be
  * used by other threads during the call.  rho[i], w[i] are device pointers (n_v of design i);
  * J_host[i] receives the compliance (NaN for a non-finite design).  stats may be NULL or
  * point to `count` entries.  No counterpart in the reference (it solves one design). */

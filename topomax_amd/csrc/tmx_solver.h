@@ -225,6 +225,10 @@ class Hierarchy {
   // (FP32 V-cycles) or >= band_first_fp64_ (FP64) above the tail, and on the filter's finest
   // level when band_fine_filter_ is set; a first level of 0 switches them off
   int band_first_ = 1, band_first_fp64_ = 1;
+  // PCG iterations replayed as captured graphs, or launched one by one (the default: on
+  // ROCm 7.2 / MI355X direct launches measured 2-3% faster per solve and 5% faster in batches,
+  // the host keeping an iteration ahead either way)
+  bool use_graphs_ = false;
   bool band_fine_filter_ = true;
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
   void find_tail();

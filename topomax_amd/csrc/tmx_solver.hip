@@ -736,8 +736,9 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   const int prec = !std::is_same<T, PC>::value ? 2 : sizeof(T) == 8 ? 0 : 1;
   if (graphs_cw0_ != has_cw0_) drop_graphs();  // captured with the other finest-level source
   graphs_cw0_ = has_cw0_;
-  for (int w = 0; w < 3; ++w)
-    if (!graphs_[prec][w]) TMX_TRY((capture<T, PC>(*this, ws, w, &graphs_[prec][w], s)));
+  if (use_graphs_)
+    for (int w = 0; w < 3; ++w)
+      if (!graphs_[prec][w]) TMX_TRY((capture<T, PC>(*this, ws, w, &graphs_[prec][w], s)));
   double* sc = ws.sc();
   *iters = 0;
   *rel = 1.0;
@@ -751,7 +752,8 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
                        split ? r32_.get() : nullptr, bbs, s));
   TMX_HIP(pcg_begin_final(bbs, sc, kPcg, rtol * rtol, kScBB,
                           ws.host_dev + Workspace::kScalars + kRingBB, s));
-  TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
+  if (use_graphs_) TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
+  else TMX_TRY((enqueue_pcg_init<T, PC>(*this, ws, s)));
   // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) and
   // then the sequence number k to ring slot (k-1)%2 in pinned host memory; the host polls that
   // number (no event packets between the graphs).  While the host waits for iteration k it
@@ -764,7 +766,9 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
   ring[3] = ring[7] = 0.0;  // no PCG work of this stream is in flight (solves end synchronized)
   auto launch = [&](int k) -> hipError_t {
-    return hipGraphLaunch(graphs_[prec][1 + ((k - 1) & 1)], s);
+    if (use_graphs_) return hipGraphLaunch(graphs_[prec][1 + ((k - 1) & 1)], s);
+    if (enqueue_pcg_iteration<T, PC>(*this, ws, (k - 1) & 1, s) != 0) return hipErrorUnknown;
+    return hipSuccess;
   };
   // wait until iteration k has reported; a stream that drains without the report is an error,
   // never a hang

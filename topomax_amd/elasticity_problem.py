@@ -419,7 +419,7 @@ class ElasticityProblem:
     def set_option(self, name: str, value):
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
-        cheb_ratio, band, band_fine_filter, fine_cell_weights."""
+        cheb_ratio, band, band_fine_filter, fine_cell_weights, graphs."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -433,7 +433,8 @@ class ElasticityProblem:
                "cheb_ratio": _lib.TMX_OPT_CHEB_RATIO,
                "band": _lib.TMX_OPT_BAND,
                "band_fine_filter": _lib.TMX_OPT_BAND_FINE_FILTER,
-               "fine_cell_weights": _lib.TMX_OPT_FINE_CELL_WEIGHTS}[name]
+               "fine_cell_weights": _lib.TMX_OPT_FINE_CELL_WEIGHTS,
+               "graphs": _lib.TMX_OPT_GRAPHS}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
@@ -485,11 +486,11 @@ class ProblemBatch:
 
     The problems may differ in size, loads, supports and penalty; each keeps its own
     hierarchy.  The library runs them concurrently, one host worker per design at a time
-    (``workers``, default 4), largest first, so the latency-bound coarse-grid work of
-    different designs overlaps on the GPU.  Results are identical to calling each problem's
-    :meth:`ElasticityProblem.objective_and_gradient` alone."""
+    (``workers``, default min(16, cores), at least 4), largest first, so the latency-bound
+    coarse-grid work of different designs overlaps on the GPU.  Results are identical to
+    calling each problem's :meth:`ElasticityProblem.objective_and_gradient` alone."""
 
-    def __init__(self, problems, workers: int = 8):
+    def __init__(self, problems, workers: int | None = None):
         self.problems = list(problems)
         if not self.problems:
             raise ValueError("a batch needs at least one problem")
@@ -497,7 +498,8 @@ class ProblemBatch:
         ptrs = (ctypes.c_void_p * len(self.problems))(*[p._h.ptr.value for p in self.problems])
         self.ptr = ctypes.c_void_p()
         _lib.check(self.lib.tmx_batch_create(ptrs, len(self.problems), ctypes.byref(self.ptr)))
-        _lib.check(self.lib.tmx_batch_set_workers(self.ptr, int(workers)))
+        if workers is not None:
+            _lib.check(self.lib.tmx_batch_set_workers(self.ptr, int(workers)))
         self.last_stats = []
 
     def objective_and_gradient(self, rhos, *, rtol=None, precision=None):
