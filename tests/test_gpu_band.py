@@ -86,3 +86,29 @@ def test_band_solve_matches(precision):
     print(precision, J0, J1, s0, gp.last_stats, flush=True)
     assert abs(J1 - J0) < 1e-9 * abs(J0)
     assert rel(w1, w0) < 1e-8
+
+
+@pytest.mark.parametrize("nu_coarse,nu_filter,nu_fine_filter", [(1, 1, 1), (2, 2, 4), (5, 4, 2),
+                                                                (8, 8, 8)])
+def test_band_sweep_counts(nu_coarse, nu_filter, nu_fine_filter):
+    """Halo widths follow the sweep counts (nu + 1 down, nu up): other counts than the
+    defaults, up to the maximum, against the per-sweep launches."""
+    nx, ny = 250, 90
+    case = dict(CANTILEVER, fixed_sides=["Left", "Bottom"])
+    gp = gpu_problem(case, nx, ny)
+    gp.set_penalization(3)
+    gp.set_option("nu_coarse", nu_coarse)
+    gp.set_option("nu_coarse_filter", nu_filter)
+    gp.set_option("nu_fine_filter", nu_fine_filter)
+    xi = torch.as_tensor(smoothed_density(nx, ny, 4, sigma=2.0), device="cuda:0")
+    r, rf = vectors(nx, ny, 9)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        gp.set_option("band", 0)
+        z_ref = gp.precondition(r, "elasticity", xi, precision=prec)
+        f_ref = gp.precondition(rf, "filter", precision=prec)
+        gp.set_option("band", 1)
+        z = gp.precondition(r, "elasticity", xi, precision=prec)
+        f = gp.precondition(rf, "filter", precision=prec)
+        print(nu_coarse, nu_filter, nu_fine_filter, prec, rel(z, z_ref), rel(f, f_ref), flush=True)
+        assert rel(z, z_ref) < tol
+        assert rel(f, f_ref) < tol

@@ -21,7 +21,7 @@ struct RedSlot {
   int nblocks;
 };
 
-constexpr int kMaxPartials = 16384;
+constexpr int kMaxPartials = 65536;
 
 // Device scalar slots of one PCG run (indices into a double array).
 struct PcgSlots {
