@@ -157,7 +157,9 @@ int tmx_integrate(tmx_problem* h, const double* values, double* out_host);
 int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
 /* Average duration (microseconds, HIP events on the handle's stream) of `reps` back-to-back
  * launches of one fine-grid kernel with the operator of the last solve:
- * kernel 0 = elasticity K(xi)u apply, 1 = elasticity block-Jacobi sweep, 2 = filter apply.
+ * kernel 0 = elasticity K(xi)u apply, 1 = elasticity block-Jacobi sweep, 2 = filter apply,
+ * 3 = two elasticity sweeps from zero in one launch, 4 = prolongation and two sweeps in one
+ * launch (3 and 4 with the handle's sweep-pair chunk heights).
  * precision TMX_FP64 or TMX_FP32.  Writes the per-launch algorithmic bytes to *bytes. */
 int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps, double* avg_us,
                      double* bytes);
@@ -187,8 +189,13 @@ enum tmx_option {
   TMX_OPT_FINE_CELL_WEIGHTS = 15, /* elasticity finest-level V-cycle sweeps read per-cell
                                    weights formed once per operator instead of the nodal xi:
                                    bit 0 FP32 V-cycles (default on), bit 1 FP64 */
-  TMX_OPT_GRAPHS = 16           /* 1: PCG iterations replay captured hipGraphs; 0 (default):
+  TMX_OPT_GRAPHS = 16,          /* 1: PCG iterations replay captured hipGraphs; 0 (default):
                                    their kernels are launched one by one (measured faster) */
+  TMX_OPT_SWEEP_PAIRS = 17,     /* > 0: the elasticity finest level runs its first two
+                                   pre-sweeps as one launch, in node-row chunks of this height
+                                   (default 4); 0: one launch per sweep */
+  TMX_OPT_SWEEP_PAIRS_POST = 18 /* the same for prolongation with the first two post-sweeps
+                                   (default 16) */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

@@ -608,6 +608,14 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
     case TMX_OPT_GRAPHS:
       for (auto* x : hs) x->use_graphs_ = value != 0.0;
       break;
+    case TMX_OPT_SWEEP_PAIRS:
+    case TMX_OPT_SWEEP_PAIRS_POST:
+      if (value < 0 || value > 256 || value != std::floor(value))
+        return invalid("sweep-pair rows must be 0..256");
+      h->el.drop_graphs();
+      for (int i = 0; i < 2; ++i)
+        (option == TMX_OPT_SWEEP_PAIRS ? h->el.pair_rows_ : h->el.pair_rows_post_)[i] = (int)value;
+      break;
     case TMX_OPT_FINE_CELL_WEIGHTS:
       if (value < 0 || value > 3 || value != std::floor(value)) return invalid("mask must be 0..3");
       h->el.fine_cw_ = (unsigned)value;
@@ -701,6 +709,41 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
       }
     }
   }
+  // paired sweeps of the elasticity V-cycle: chunk height per kernel and precision (the launch
+  // is latency-bound, so the best height is the smallest whose workgroups still fit the
+  // device at once, which depends on the grid)
+  TMX_TRY(h->el.fine_cell_weights(s));
+  const int pair_cands[] = {2, 3, 4, 6, 8, 12, 16, 24};
+  for (int prec = 0; prec < 2; ++prec) {
+    for (int kind = 0; kind < 2; ++kind) {
+      if (kind == 1 && !h->el.pair_post_ok()) continue;
+      int& slot = (kind == 0 ? h->el.pair_rows_ : h->el.pair_rows_post_)[prec == 0 ? 1 : 0];
+      if (slot <= 0) continue;  // switched off by the caller
+      const int op1 = kind == 0 ? kSmoothZero : kProlongSmooth;
+      auto launch = [&](int rows) -> int {
+        if (prec == 0)
+          return h->el.sweep_pair0<double>(op1, rows, h->u_s.get(), h->Fm.get(), h->tmp64.get(),
+                                            0.5, 0.5, nullptr, s);
+        return h->el.sweep_pair0<float>(op1, rows, h->u32.get(), h->y32.get(), h->el.q32_.get(),
+                                        0.5, 0.5, nullptr, s);
+      };
+      double best_pair = 1e30;
+      for (int rows : pair_cands) {
+        const int reps = 20;
+        for (int warm = 0; warm < 2; ++warm) TMX_TRY(launch(rows));
+        TMX_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) TMX_TRY(launch(rows));
+        TMX_HIP(hipEventRecord(e1, s));
+        TMX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_pair) {
+          best_pair = ms;
+          slot = rows;
+        }
+      }
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   h->el.set_rows(best[0], best[1]);
@@ -719,10 +762,11 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
 extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps,
                                 double* avg_us, double* bytes) {
   if (!h || !avg_us || reps < 1) return invalid("bad arguments");
-  if (kernel < 0 || kernel > 2) return invalid("unknown kernel id");
+  if (kernel < 0 || kernel > 4) return invalid("unknown kernel id");
   if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
-  if (kernel < 2 && !h->el.has_operator_) return invalid("no elasticity operator yet: solve first");
-  if (kernel < 2 && h->degree == 2) {
+  const bool elastic = kernel != 2;
+  if (elastic && !h->el.has_operator_) return invalid("no elasticity operator yet: solve first");
+  if (elastic && h->degree == 2) {
     set_error("tmx_bench_kernel times the P1 elasticity kernels; not available for P2");
     return TMX_ERR_UNSUPPORTED;
   }
@@ -744,6 +788,15 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
       if (f64) return h->el.smooth0<double>(h->u_s.get(), h->Fm.get(), h->tmp64.get(), s);
       return h->el.smooth0<float>(h->u32.get(), h->y32.get(), h->el.q32_.get(), s);
     }
+    if (kernel >= 3) {  // paired sweeps with the handle's chunk heights
+      const int op1 = kernel == 3 ? kSmoothZero : kProlongSmooth;
+      const int rows = (kernel == 3 ? h->el.pair_rows_ : h->el.pair_rows_post_)[f64 ? 1 : 0];
+      if (f64)
+        return h->el.sweep_pair0<double>(op1, rows, h->u_s.get(), h->Fm.get(), h->tmp64.get(),
+                                         0.5, 0.5, nullptr, s);
+      return h->el.sweep_pair0<float>(op1, rows, h->u32.get(), h->y32.get(), h->el.q32_.get(),
+                                      0.5, 0.5, nullptr, s);
+    }
     if (f64) return h->fl.apply<double>(h->u_s.get(), h->tmp64.get(), nullptr, s);
     return h->fl.apply<float>(h->u32.get(), h->el.q32_.get(), nullptr, s);
   };
@@ -764,10 +817,15 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
     // algorithmic bytes per node: apply u (2) + y (2) + xi (1); sweep adds b (2); filter x + y.
     // A sweep reading the preconditioner's per-cell weights reads 2 values per cell instead of
     // xi per node.
-    const double per = kernel == 0 ? 5.0 : (kernel == 1 ? 7.0 : 2.0);
+    // A pair reads b and writes y (zero start) or also reads x (prolongation; the coarse
+    // correction is a quarter-size stream).
+    const double per = kernel == 0 ? 5.0 : (kernel == 1 ? 7.0 : (kernel == 2 ? 2.0 :
+                       (kernel == 3 ? 5.0 : 7.0)));
     *bytes = per * sz * (double)nv;
-    if (kernel == 1 && h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u)))
-      *bytes = 6.0 * sz * (double)nv + 2.0 * sz * (double)h->nx * (double)h->ny;
+    const bool cw = h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u));
+    if (cw && (kernel == 1 || kernel >= 3))
+      *bytes += 2.0 * sz * (double)h->nx * (double)h->ny - sz * (double)nv;
+    if (kernel == 4) *bytes += 2.0 * sz * (double)(h->nx / 2 + 1) * (double)(h->ny / 2 + 1);
   }
   return TMX_OK;
 }

@@ -85,6 +85,13 @@ template <typename T>
 hipError_t elast_sweep(const ElastOpArgs<T>& a, int op, const T* x, const T* b, T* y, T omega,
                        RedSlot* dot, hipStream_t s, const SweepExtra<T>& ex = SweepExtra<T>());
 
+// Two block-Jacobi sweeps in one launch: op1 = kSmoothZero (y = S2 S1 0) or kProlongSmooth
+// (y = S2 S1 (x + P ec)), with damping om1, om2 and node-row chunks of `rows`; a dot slot
+// receives partials of b.y.  Same results as the two launches up to round-off.
+template <typename T>
+hipError_t elast_sweep_pair(const ElastOpArgs<T>& a, int op1, int rows, const T* x, const T* b,
+                            T* y, T om1, T om2, RedSlot* dot, hipStream_t s, const T* ec);
+
 // per-cell mean r from nodal xi (for restriction of level-0 weights)
 template <typename T>
 hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,

@@ -370,6 +370,239 @@ template hipError_t elast_sweep<float>(const ElastOpArgs<float>&, int, const flo
                                        float*, float, RedSlot*, hipStream_t,
                                        const SweepExtra<float>&);
 
+// Two consecutive block-Jacobi sweeps in one pass over the grid: the first is kSmoothZero
+// (from x = 0) or kProlongSmooth (from x + P e), the second a plain sweep of its result.  The
+// wave's 64 lanes carry two halo columns on each side (60 output columns: the first sweep's
+// result is right in lanes 1..63, the second's in lanes 2..61).  The second sweep trails the
+// first by one cell row, so the intermediate iterate never leaves registers: per cell row cy
+// the first sweep forms the cell's weights, diagonal blocks and (kProlongSmooth) forces and
+// completes its node row cy; the second sweep then takes cell row cy - 1 between the first
+// sweep's node rows cy - 1 and cy and completes its node row cy - 1.  A chunk of R node rows
+// sweeps R + 3 cell rows (node rows r0 - 1 .. rend of the first sweep).
+template <typename T, bool XI, int OP1>
+__global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
+                                                             const T* __restrict__ x,
+                                                             const T* __restrict__ b,
+                                                             T* __restrict__ y, T om1, T om2,
+                                                             double* __restrict__ dpart,
+                                                             const T* __restrict__ ec) {
+  using V = V2<T>;
+  constexpr bool PS = (OP1 == kProlongSmooth);
+  constexpr int H = 2;
+  const SweepShape sh = a.sh;
+  const bool want_dot = dpart != nullptr;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
+  const int nxc = nx >> 1;
+  const int c = strip * sh.stripw - H + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= H && lane < kWave - H;
+  const unsigned fixed = a.fixed;
+  const bool colfix = ((fixed & kLeft) && c == 0) || ((fixed & kRight) && c == nx);
+  const int r0 = chunk * sh.rows;
+  const int rend = min(r0 + sh.rows, ny + 1);  // owned node rows [r0, rend)
+  const int cy_begin = r0 - 2;
+  const int cy_end = wave_ok ? rend + 1 : cy_begin;
+  const int first_row = max(0, r0 - 1);  // first node row of the first sweep with a full diagonal
+  const V* X = reinterpret_cast<const V*>(x);
+  const V* B = reinterpret_cast<const V*>(b);
+  V* Y = reinterpret_cast<V*>(y);
+  const V* CW = reinterpret_cast<const V*>(a.cellw);
+  const V* EC = reinterpret_cast<const V*>(ec);
+  const ElastConst<T> k = a.k;
+  const V zero = {T(0), T(0)};
+
+  auto row_fixed = [&](int iy) {
+    return colfix || ((fixed & kBottom) && iy == 0) || ((fixed & kTop) && iy == ny);
+  };
+  auto ld_x = [&](int iy, V& raw) -> V {  // x + P e (kProlongSmooth)
+    V v = zero;
+    const bool ok = colv && iy >= 0 && iy <= ny;
+    if (ok) v = X[(unsigned)(iy * nvx + c)];
+    if (ok && !row_fixed(iy)) {
+      const int I0 = c >> 1, J0 = iy >> 1, ox = c & 1, oy = iy & 1;
+      V e = EC[(size_t)J0 * (nxc + 1) + I0];
+      if (ox | oy) {
+        const V e1 = EC[(size_t)(J0 + oy) * (nxc + 1) + I0 + ox];
+        e = {T(0.5) * (e.x + e1.x), T(0.5) * (e.y + e1.y)};
+      }
+      v += e;
+    }
+    raw = v;
+    if (row_fixed(iy)) v = zero;
+    return v;
+  };
+  auto ld_xi = [&](int iy) -> T {
+    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(unsigned)(iy * nvx + c)];
+    return T(0);
+  };
+
+  int cy = cy_begin;
+  // first sweep: cell row cy between node rows cy (u_b) and cy + 1
+  V ub = zero, ub_raw = zero, ubr = zero;
+  if constexpr (PS) {
+    ub = ld_x(cy, ub_raw);
+    ubr = lane_down(ub);
+  }
+  T zb = ld_xi(cy);
+  T zbr = lane_down(zb);
+  V acc1 = zero;                            // forces on node row cy from cell row cy - 1
+  Diag3<T> d_b = {T(0), T(0), T(0)};        // its diagonal block from cell row cy - 1
+  // second sweep: cell row cy - 1 between node rows cy - 1 (v_b) and cy
+  V vb = zero, vb_raw = zero, vbr = zero;   // first-sweep iterate at node row cy - 1
+  V acc2 = zero;                            // forces on node row cy - 1 from cell row cy - 2
+  V bp = zero;                              // b at node row cy - 1
+  Diag3<T> dp = {T(0), T(0), T(0)};         // diagonal block of node row cy - 1
+  T invp = T(0);
+  T wAp = T(0), wBp = T(0);                 // weights of cell row cy - 1
+  double dot = 0.0;
+
+  for (; cy < cy_end; ++cy) {
+    // ---- first sweep, cell row cy
+    V ut = zero, ut_raw = zero, utr = zero;
+    if constexpr (PS) {
+      ut = ld_x(cy + 1, ut_raw);
+      utr = lane_down(ut);
+    }
+    const T zt = ld_xi(cy + 1);
+    const T ztr = lane_down(zt);
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    T wA = T(0), wB = T(0);
+    if (cell_ok) {
+      if (XI) {
+        wA = pc_weight(a.m + a.cp * hsym3(zb, zbr, ztr, a.p), a.wmin);
+        wB = pc_weight(a.m + a.cp * hsym3(zb, zt, ztr, a.p), a.wmin);
+      } else {
+        const V w = CW[(unsigned)(cy * nx + c)];
+        wA = w.x;
+        wB = w.y;
+      }
+    }
+    V acc1t = zero;
+    if constexpr (PS) {
+      V f0, f1, f2, f3;
+      cell_forces(k, wA, wB, ub, ubr, ut, utr, f0, f1, f2, f3);
+      acc1 += f0;
+      acc1t = f2;
+      acc1 += lane_up(f1);
+      acc1t += lane_up(f3);
+    }
+    const T wAl = lane_up(wA), wBl = lane_up(wB);
+    d_b.xx += wA * k.Pxx + wB * k.Qxx + wAl * k.Rxx;
+    d_b.xy += wAl * k.Rxy;
+    d_b.yy += wA * k.Pyy + wB * k.Qyy + wAl * k.Ryy;
+    Diag3<T> d_t;
+    d_t.xx = wB * k.Rxx + wAl * k.Qxx + wBl * k.Pxx;
+    d_t.xy = wB * k.Rxy;
+    d_t.yy = wB * k.Ryy + wAl * k.Qyy + wBl * k.Pyy;
+    // node row cy of the first sweep is complete
+    const bool rowv = colv && cy >= first_row && cy <= ny;
+    const bool fx = row_fixed(cy);
+    V bb = zero, v_raw = zero;
+    T inv = T(0);
+    if (rowv) {
+      bb = B[(unsigned)(cy * nvx + c)];
+      const V Ax = PS ? (fx ? ub_raw : acc1) : zero;
+      const V res = bb - Ax;
+      V d;
+      if (fx) {
+        d = res;
+      } else {
+        inv = recip(d_b.xx * d_b.yy - d_b.xy * d_b.xy);
+        d = {(d_b.yy * res.x - d_b.xy * res.y) * inv, (d_b.xx * res.y - d_b.xy * res.x) * inv};
+      }
+      v_raw = {ub_raw.x + om1 * d.x, ub_raw.y + om1 * d.y};
+    }
+    const V vt = fx ? zero : v_raw;
+    const V vtr = lane_down(vt);
+    // ---- second sweep, cell row cy - 1
+    {
+      V f0, f1, f2, f3;
+      cell_forces(k, wAp, wBp, vb, vbr, vt, vtr, f0, f1, f2, f3);
+      acc2 += f0;
+      acc2 += lane_up(f1);
+      const int ry = cy - 1;  // node row ry of the second sweep is complete
+      if (ry >= r0 && ry < rend && outl) {
+        const bool fx2 = row_fixed(ry);
+        const V Ax = fx2 ? vb_raw : acc2;
+        const V res = bp - Ax;
+        V d;
+        if (fx2) {
+          d = res;
+        } else {
+          d = {(dp.yy * res.x - dp.xy * res.y) * invp, (dp.xx * res.y - dp.xy * res.x) * invp};
+        }
+        const V yo = {vb_raw.x + om2 * d.x, vb_raw.y + om2 * d.y};
+        Y[(unsigned)(ry * nvx + c)] = yo;
+        if (want_dot) dot += (double)bp.x * (double)yo.x + (double)bp.y * (double)yo.y;
+      }
+      acc2 = f2;
+      acc2 += lane_up(f3);
+    }
+    vb = vt;
+    vb_raw = v_raw;
+    vbr = vtr;
+    bp = bb;
+    dp = d_b;
+    invp = inv;
+    wAp = wA;
+    wBp = wB;
+    acc1 = acc1t;
+    d_b = d_t;
+    ub = ut;
+    ub_raw = ut_raw;
+    ubr = utr;
+    zb = zt;
+    zbr = ztr;
+  }
+  if (want_dot) {
+    const double v[1] = {dot};
+    double* const o[1] = {dpart};
+    block_partials<1>(v, o);
+  }
+}
+
+template <typename T>
+hipError_t elast_sweep_pair(const ElastOpArgs<T>& a0, int op1, int rows, const T* x, const T* b,
+                            T* y, T om1, T om2, RedSlot* dot_by, hipStream_t s, const T* ec) {
+  if (op1 != kSmoothZero && op1 != kProlongSmooth) return hipErrorInvalidValue;
+  if (op1 == kProlongSmooth && (!ec || !x || (a0.sh.nx & 1) || (a0.sh.ny & 1)))
+    return hipErrorInvalidValue;
+  if (!b || !y || (!a0.xi && !a0.cellw)) return hipErrorInvalidValue;
+  if ((size_t)(a0.sh.nx + 1) * (a0.sh.ny + 1) >= (1ull << 31)) return hipErrorInvalidValue;
+  ElastOpArgs<T> a = a0;
+  a.sh = SweepShape::make(a0.sh.nx, a0.sh.ny, std::max(1, rows), kWave - 2 * 2);
+  const int nb = a.sh.blocks();
+  double* dp = nullptr;
+  if (dot_by) {
+    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    dp = dot_by->partials;
+    dot_by->nblocks = nb;
+  }
+  const bool xi = a.xi != nullptr;
+#define TMX_LAUNCH(XIV, OPV) \
+  k_elast_sweep_pair<T, XIV, OPV><<<nb, kBlock, 0, s>>>(a, x, b, y, om1, om2, dp, ec)
+  if (op1 == kSmoothZero) {
+    if (xi) TMX_LAUNCH(true, kSmoothZero);
+    else TMX_LAUNCH(false, kSmoothZero);
+  } else {
+    if (xi) TMX_LAUNCH(true, kProlongSmooth);
+    else TMX_LAUNCH(false, kProlongSmooth);
+  }
+#undef TMX_LAUNCH
+  return hipGetLastError();
+}
+template hipError_t elast_sweep_pair<double>(const ElastOpArgs<double>&, int, int, const double*,
+                                             const double*, double*, double, double, RedSlot*,
+                                             hipStream_t, const double*);
+template hipError_t elast_sweep_pair<float>(const ElastOpArgs<float>&, int, int, const float*,
+                                            const float*, float*, float, float, RedSlot*,
+                                            hipStream_t, const float*);
+
 template <typename T>
 __global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
                                      T wmin, V2<T>* __restrict__ cw) {

@@ -155,6 +155,8 @@ class Hierarchy {
            double mu, double eps, int coarse_max_nodes, hipStream_t s);
   // elasticity: copy nodal xi (device, fp64) and rebuild the coarse operators, penalty p
   int update_elasticity(const double* xi, int p, double m, hipStream_t s);
+  // the preconditioner's per-cell weights of the finest cells from the current xi, p, m
+  int fine_cell_weights(hipStream_t s);
   // elasticity fine operator only (copies xi; no coarse rebuild)
   int set_fine_operator(const double* xi, int p, double m, hipStream_t s);
   // filter: build the constant hierarchy once
@@ -166,6 +168,11 @@ class Hierarchy {
   int residual(const T* x, const T* b, T* r, hipStream_t s);
   template <typename T>
   int smooth0(const T* x, const T* b, T* y, hipStream_t s);
+  // two finest-level sweeps in one launch (elasticity): op1 = kSmoothZero (x unused) or
+  // kProlongSmooth (x + P e with e = level 1's x), chunk height rows
+  template <typename T>
+  int sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, double om1, double om2,
+                  RedSlot* dot, hipStream_t s);
   // z = M^-1 b; at level 0 the last smoothing sweep also writes partials of b.z to dot_bz
   template <typename T>
   int vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz = nullptr);
@@ -230,6 +237,11 @@ class Hierarchy {
   // the host keeping an iteration ahead either way)
   bool use_graphs_ = false;
   bool band_fine_filter_ = true;
+  // > 0: the finest elasticity level's first two pre-sweeps (pair_rows_), and its prolongation
+  // with the first two post-sweeps (pair_rows_post_), run as one launch each with node-row
+  // chunks of that height; 0: one launch per sweep.  Index 0: FP32 V-cycles, 1: FP64.
+  int pair_rows_[2] = {8, 8}, pair_rows_post_[2] = {12, 16};
+  bool pair_post_ok() const;  // the finest level has a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
   void find_tail();
   int tail_ = -1;               // first level handled by the one-workgroup tail kernel

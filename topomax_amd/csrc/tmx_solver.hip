@@ -283,13 +283,23 @@ int Hierarchy::set_fine_operator(const double* xi, int p, double m, hipStream_t 
   return 0;
 }
 
-int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
-  TMX_TRY(set_fine_operator(xi, p, m, s));
+int Hierarchy::fine_cell_weights(hipStream_t s) {
   const Level& L0 = levels_[0];
-  const int last = (int)levels_.size() - 1;
-  TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p, m, cp_, m, cw0_.get(), s));
+  TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p_, m_, cp_, m_, cw0_.get(), s));
   TMX_HIP((convert<double, float>(cw0_.size(), cw0_.get(), cw0_32_.get(), s)));
   has_cw0_ = true;
+  return 0;
+}
+
+bool Hierarchy::pair_post_ok() const {
+  return levels_.size() >= 2 && levels_[1].nested_from_prev && !(levels_[0].nx & 1) &&
+         !(levels_[0].ny & 1);
+}
+
+int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
+  TMX_TRY(set_fine_operator(xi, p, m, s));
+  const int last = (int)levels_.size() - 1;
+  TMX_TRY(fine_cell_weights(s));
   for (int l = 1; l <= last; ++l) {
     Level& L = levels_[l];
     if (L.kind != kElem) break;
@@ -390,6 +400,27 @@ int Hierarchy::apply_update(const T* p_old, const T* z, T* q, T* p_new, const do
 template <typename T>
 int Hierarchy::smooth0(const T* x, const T* b, T* y, hipStream_t s) {
   return level_op<T>(*this, 0, kSmooth, x, b, y, nullptr, s);
+}
+
+template <typename T>
+int Hierarchy::sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, double om1, double om2,
+                           RedSlot* dot, hipStream_t s) {
+  if (phys_ != kElasticity || levels_[0].kind != kElem || rows < 1) {
+    set_error("internal: paired sweeps need the finest elasticity level");
+    return -1;
+  }
+  const T* ec = nullptr;
+  if (op1 == kProlongSmooth) {
+    if (!pair_post_ok()) {
+      set_error("internal: paired prolongation needs a nested level below the finest");
+      return -1;
+    }
+    ec = PrecBufs<T>::x(levels_[1]);
+  }
+  const ElastOpArgs<T> a = elast_args<T>(*this, 0, xi_for<T>(*this), PrecBufs<T>::cw(levels_[0]),
+                                         sizeof(T) == 8, true);
+  TMX_HIP(elast_sweep_pair<T>(a, op1, rows, x, b, y, (T)om1, (T)om2, dot, s, ec));
+  return 0;
 }
 
 template <typename T>
@@ -531,18 +562,31 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   T* buf[2] = {out, PrecBufs<T>::t(L)};
   const int npre = l == 0 ? nu_pre_ : nu_coarse_;
   const int npost = l == 0 ? nu_post_ : nu_coarse_;
-  const int S = npre + npost;
-  int w = (S - 1) % 2;
-  TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s, SweepExtra<T>(),
-                      true, sweep_omega(0, npre)));
-  for (int k = 1; k < npre; ++k) {
-    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s, SweepExtra<T>(),
-                        true, sweep_omega(k, npre)));
-    w ^= 1;
-  }
   T* cb = PrecBufs<T>::b(C);
   T* cx = PrecBufs<T>::x(C);
   const bool fused_el = fused_ && L.kind == kElem && C.nested_from_prev;  // nested transfer
+  // the finest elasticity level runs its first two pre-sweeps, and prolongation with its first
+  // two post-sweeps, as one launch each
+  const bool pairs = l == 0 && phys_ == kElasticity && L.kind == kElem;
+  const int pi = sizeof(T) == 8 ? 1 : 0;
+  const bool pair_pre = pairs && pair_rows_[pi] > 0 && npre >= 2;
+  const bool pair_post = pairs && pair_rows_post_[pi] > 0 && fused_el && npost >= 2;
+  const int launches = npre - (pair_pre ? 1 : 0) + npost - (pair_post ? 1 : 0);
+  int w = (launches - 1) % 2;
+  int kpre = 1;
+  if (pair_pre) {
+    TMX_TRY(sweep_pair0<T>(kSmoothZero, pair_rows_[pi], nullptr, b, buf[w], sweep_omega(0, npre),
+                           sweep_omega(1, npre), nullptr, s));
+    kpre = 2;
+  } else {
+    TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s, SweepExtra<T>(),
+                        true, sweep_omega(0, npre)));
+  }
+  for (; kpre < npre; ++kpre) {
+    TMX_TRY(level_op<T>(*this, l, kSmooth, buf[w], b, buf[w ^ 1], nullptr, s, SweepExtra<T>(),
+                        true, sweep_omega(kpre, npre)));
+    w ^= 1;
+  }
   // (9-point levels keep separate residual / restriction / prolongation launches: recomputing
   // the fine residuals per coarse node costs more than the extra launch there)
   const bool fused_st = false;
@@ -563,7 +607,14 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   TMX_TRY(vcycle<T>(l + 1, cb, cx, s));
   int post = npost, j = 0;  // j: index of the next post-sweep in the smoothing pass
   RedSlot* last_dot = l == 0 ? dot_bz : nullptr;
-  if (fused_el) {
+  if (pair_post) {
+    TMX_TRY(sweep_pair0<T>(kProlongSmooth, pair_rows_post_[pi], buf[w], b, buf[w ^ 1],
+                           sweep_omega(0, npost), sweep_omega(1, npost),
+                           post == 2 ? last_dot : nullptr, s));
+    j = 2;
+    w ^= 1;
+    post -= 2;
+  } else if (fused_el) {
     SweepExtra<T> ex;
     ex.ec = cx;
     TMX_TRY(level_op<T>(*this, l, kProlongSmooth, buf[w], b, buf[w ^ 1],
@@ -1101,6 +1152,10 @@ template float* Hierarchy::x_buf<float>();
 template int Hierarchy::residual<double>(const double*, const double*, double*, hipStream_t);
 template int Hierarchy::smooth0<double>(const double*, const double*, double*, hipStream_t);
 template int Hierarchy::smooth0<float>(const float*, const float*, float*, hipStream_t);
+template int Hierarchy::sweep_pair0<double>(int, int, const double*, const double*, double*,
+                                            double, double, RedSlot*, hipStream_t);
+template int Hierarchy::sweep_pair0<float>(int, int, const float*, const float*, float*, double,
+                                           double, RedSlot*, hipStream_t);
 template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t, RedSlot*);
 template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t, RedSlot*);
 

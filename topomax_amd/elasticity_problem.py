@@ -419,7 +419,8 @@ class ElasticityProblem:
     def set_option(self, name: str, value):
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
-        cheb_ratio, band, band_fine_filter, fine_cell_weights, graphs."""
+        cheb_ratio, band, band_fine_filter, fine_cell_weights, graphs, sweep_pairs,
+        sweep_pairs_post."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -434,7 +435,9 @@ class ElasticityProblem:
                "band": _lib.TMX_OPT_BAND,
                "band_fine_filter": _lib.TMX_OPT_BAND_FINE_FILTER,
                "fine_cell_weights": _lib.TMX_OPT_FINE_CELL_WEIGHTS,
-               "graphs": _lib.TMX_OPT_GRAPHS}[name]
+               "graphs": _lib.TMX_OPT_GRAPHS,
+               "sweep_pairs": _lib.TMX_OPT_SWEEP_PAIRS,
+               "sweep_pairs_post": _lib.TMX_OPT_SWEEP_PAIRS_POST}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
