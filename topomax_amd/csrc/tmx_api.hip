@@ -302,11 +302,12 @@ static size_t ndofs(const tmx_problem* h) {
 static const double* load_F(const tmx_problem* h) { return h->degree == 2 ? h->F2.get() : h->F.get(); }
 
 static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
-                            const tmx_solve_opts* o = nullptr) {
+                            const tmx_solve_opts* o = nullptr, bool sync = true) {
   SolveStats st;
   SolveParams prm = h->fl_prm;
   if (o && o->precision >= 0) prm.precision = filter_precision(o->precision);
   prm.verify = false;  // filter statistics are not reported
+  prm.sync = sync;
   if (!h->fl_rtol_set) {
     // two orders of magnitude below the elasticity solve's own tolerance, within [1e-12, 1e-10]:
     // the filtered design then adds nothing visible to the solve's error
@@ -412,8 +413,10 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
   return rc;
 }
 
+// sync = false (P1): returns with the solve's work queued; st->complete() and setup_ms (from
+// ev_setup) are then the caller's, after it synchronises the stream
 static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const tmx_solve_opts* o,
-                            SolveStats* st) {
+                            SolveStats* st, bool sync = true) {
   TMX_TRY(need_penalty(h));
   TMX_HIP(hipEventRecord(h->ev_setup[0], h->stream));
   TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, h->stream));
@@ -429,11 +432,14 @@ static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const t
     if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(ndofs(h), NAN, u, h->stream));
     return rc;
   }
-  const SolveParams prm = merge(h->el_prm, o);
+  SolveParams prm = merge(h->el_prm, o);
+  prm.sync = sync;
   const int rc = h->el.solve(h->ws, h->Fm.get(), u, prm, st, h->stream);
-  float setup = 0.f;  // the solve has synchronised the stream past both events
-  if (hipEventElapsedTime(&setup, h->ev_setup[0], h->ev_setup[1]) != hipSuccess) setup = -1.f;
-  st->setup_ms = setup;
+  if (sync) {
+    float setup = 0.f;  // the solve has synchronised the stream past both events
+    if (hipEventElapsedTime(&setup, h->ev_setup[0], h->ev_setup[1]) != hipSuccess) setup = -1.f;
+    st->setup_ms = setup;
+  }
   if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(ndofs(h), NAN, u, h->stream));
   return rc;
 }
@@ -490,14 +496,33 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   Scope scope(h);
   double* xiv = xi ? xi : h->xi_s.get();
   double* uv = u ? u : h->u_s.get();
-  TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
-  int rc = filter_rhs_solve(h, h->b_s.get(), xiv, opts);
-  if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
+  // The three solves and the reductions between them are queued back to back: no solve waits
+  // for its last kernels, so the device does not idle at the phase changes.  The statistics
+  // (true residual, timings) and J are read after the one synchronisation at the end.
   SolveStats st;
-  if (rc == TMX_OK) rc = elasticity_solve(h, xiv, uv, opts, &st);
-  else TMX_HIP(fill(ndofs(h), NAN, uv, h->stream));
-  export_stats(st, stats);
-  if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return rc;
+  bool st_pending = false;
+  auto finish = [&](int code) -> int {
+    const hipError_t e = hipStreamSynchronize(h->stream);
+    if (st_pending) {
+      st.complete();
+      float setup = 0.f;
+      if (hipEventElapsedTime(&setup, h->ev_setup[0], h->ev_setup[1]) != hipSuccess) setup = -1.f;
+      st.setup_ms = setup;
+    }
+    export_stats(st, stats);
+    if (e != hipSuccess) TMX_HIP(e);
+    return code;
+  };
+  TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
+  int rc = filter_rhs_solve(h, h->b_s.get(), xiv, opts, false);
+  if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return finish(rc);
+  if (rc == TMX_OK) {
+    st_pending = h->degree != 2;
+    rc = elasticity_solve(h, xiv, uv, opts, &st, false);
+  } else {
+    TMX_HIP(fill(ndofs(h), NAN, uv, h->stream));
+  }
+  if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return finish(rc);
   // J = F.u reduced into a device scalar now, read after the step's final synchronisation
   RedSlot gen = h->ws.region(3);
   TMX_HIP(dot(ndofs(h), load_F(h), uv, gen, h->ws.sc() + kScJ, h->stream));
@@ -506,12 +531,13 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   if (w || Mw) {
     double* wv = w ? w : h->w_s.get();
     TMX_TRY(tmx_sensitivity(h, xiv, uv, h->b_s.get()));
-    const int rc2 = filter_rhs_solve(h, h->b_s.get(), wv, opts);
-    if (rc2 != TMX_OK && rc2 != TMX_ERR_NONFINITE) return rc2;
+    const int rc2 = filter_rhs_solve(h, h->b_s.get(), wv, opts, false);
+    if (rc2 != TMX_OK && rc2 != TMX_ERR_NONFINITE) return finish(rc2);
     if (Mw) TMX_TRY(tmx_mass_apply(h, wv, Mw));
     if (rc == TMX_OK) rc = rc2;
   }
-  TMX_HIP(hipStreamSynchronize(h->stream));
+  const int frc = finish(rc);
+  if (frc != rc) return frc;
   if (J) *J = h->ws.host[kScJ];
   return rc;
 }

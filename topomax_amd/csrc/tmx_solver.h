@@ -87,7 +87,7 @@ struct Workspace {
 };
 
 // scalar slots used outside PCG
-enum : int { kScBB = 0, kScRR64 = 1, kScJ = 2, kScInt = 3, kScTmp = 4 };
+enum : int { kScBB = 0, kScRR64 = 1, kScJ = 2, kScInt = 3, kScTmp = 4, kScVerify = 6 };
 // pinned ring (after the kScalars mirror): PCG report slots at 0 and 4, MINRES at 8, |b|^2 of
 // the current PCG solve at kRingBB
 constexpr int kRingBB = 24;
@@ -128,11 +128,21 @@ struct SolveParams {
   // FP64 / mixed: report the true residual |b - A x| / |b| (one more operator application and
   // a host round trip) rather than PCG's recursively updated one
   bool verify = true;
+  // FP64 / mixed: false returns with the solution's last kernels (and the true residual)
+  // still queued, so the caller's next work follows without a host round trip; the caller
+  // synchronises the stream and then calls SolveStats::complete()
+  bool sync = true;
 };
 
 struct SolveStats {
   int iterations = 0, outer = 0, levels = 0;
   double rel_residual = 0, setup_ms = 0, solve_ms = 0;
+  // left by a solve with sync = false: the pinned |b - A x|^2 (with |b|^2) and the events
+  // around the solve's device work, read by complete() once the stream has drained
+  const double* pending_rr = nullptr;
+  double pending_bb = 0;
+  hipEvent_t pending_ev[2] = {nullptr, nullptr};
+  void complete();
 };
 
 // Preconditioned MINRES (FP64) on A x = b from x0 = 0, restarted on the true residual until
@@ -241,6 +251,7 @@ class Hierarchy {
   // with the first two post-sweeps (pair_rows_post_), run as one launch each with node-row
   // chunks of that height; 0: one launch per sweep.  Index 0: FP32 V-cycles, 1: FP64.
   int pair_rows_[2] = {8, 8}, pair_rows_post_[2] = {12, 16};
+  hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
   bool pair_post_ok() const;  // the finest level has a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
   void find_tail();

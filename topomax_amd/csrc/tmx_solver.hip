@@ -136,7 +136,23 @@ void Hierarchy::drop_graphs() {
       }
 }
 
-Hierarchy::~Hierarchy() { drop_graphs(); }
+Hierarchy::~Hierarchy() {
+  drop_graphs();
+  for (auto& e : solve_ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void SolveStats::complete() {
+  if (pending_rr) {
+    rel_residual = pending_bb > 0.0 ? std::sqrt(std::max(*pending_rr, 0.0) / pending_bb) : 0.0;
+    pending_rr = nullptr;
+  }
+  if (pending_ev[0] && pending_ev[1]) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pending_ev[0], pending_ev[1]) == hipSuccess) solve_ms = ms;
+    pending_ev[0] = pending_ev[1] = nullptr;
+  }
+}
 
 void Hierarchy::set_rows(int rows64, int rows32) {
   drop_graphs();
@@ -815,7 +831,9 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   // in flight, so a ring slot is rewritten (by iteration k+2) only after the host has read it.
   volatile double* ring = ws.host + Workspace::kScalars;
   auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
-  ring[3] = ring[7] = 0.0;  // no PCG work of this stream is in flight (solves end synchronized)
+  // no PCG iteration of this stream is in flight: an earlier solve returned only after every
+  // iteration it queued had reported
+  ring[3] = ring[7] = 0.0;
   auto launch = [&](int k) -> hipError_t {
     if (use_graphs_) return hipGraphLaunch(graphs_[prec][1 + ((k - 1) & 1)], s);
     if (enqueue_pcg_iteration<T, PC>(*this, ws, (k - 1) & 1, s) != 0) return hipErrorUnknown;
@@ -1020,6 +1038,12 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   S.outer = 0;
   int rc = 0;
   if (prm.precision == kSolveFP64 || prm.precision == kSolveMixed) {
+    const bool defer = !prm.sync;
+    if (defer) {
+      for (auto& e : solve_ev_)
+        if (!e) TMX_HIP(hipEventCreate(&e));
+      TMX_HIP(hipEventRecord(solve_ev_[0], s));
+    }
     // |b|^2 is formed on the device inside pcg (b = 0 converges at once with x = 0; a
     // non-finite b stops with the non-finite code)
     int it = 0;
@@ -1050,10 +1074,23 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       S.rel_residual = rel;
       if (prm.verify && bb > 0.0) {  // the true residual
         TMX_TRY(residual<double>(x, b, rr64_.get(), s));
-        TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScRR64, s));
-        TMX_TRY(ws.read(s, kScRR64, 1));
-        S.rel_residual = std::sqrt(ws.host[kScRR64] / bb);
+        TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScVerify, s));
+        if (defer) {
+          TMX_HIP(hipMemcpyAsync(ws.host + kScVerify, ws.sc() + kScVerify, sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+          S.pending_rr = ws.host + kScVerify;
+          S.pending_bb = bb;
+        } else {
+          TMX_TRY(ws.read(s, kScVerify, 1));
+          S.rel_residual = std::sqrt(ws.host[kScVerify] / bb);
+        }
       }
+    }
+    if (defer) {
+      TMX_HIP(hipEventRecord(solve_ev_[1], s));
+      S.pending_ev[0] = solve_ev_[0];
+      S.pending_ev[1] = solve_ev_[1];
+      return rc;
     }
   } else {
     TMX_HIP(dot(nv, b, b, gen, ws.sc() + kScBB, s));
