@@ -174,7 +174,7 @@ enum tmx_option {
   TMX_OPT_OMEGA_ELAST = 4, TMX_OPT_OMEGA_FILTER = 5,
   TMX_OPT_NU_COARSE = 6,    /* pre- and post-sweeps on the coarser levels (PRE/POST: finest) */
   TMX_OPT_FILTER_RTOL = 7,  /* relative residual of the Helmholtz filter solves (default:
-                               1e-2 x the elasticity rtol, clamped to [1e-12, 1e-10]) */
+                               1e-1 x the elasticity rtol, clamped to [1e-12, 1e-9]) */
   TMX_OPT_NU_FINE_FILTER = 8,   /* filter: pre- and post-sweeps on the finest level */
   TMX_OPT_NU_COARSE_FILTER = 9, /* filter: pre- and post-sweeps on the coarser levels */
   TMX_OPT_CHEB_ELAST = 10,      /* > 0: Chebyshev step sizes for an upper bound lambda of
