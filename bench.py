@@ -29,8 +29,10 @@ if ROOT not in sys.path:
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel at N=1000, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --roofline-only`, each scaled by the same counter's
-# reading for a plain copy of 48,064,016 bytes (FETCH x1.9989, WRITE x1.0000).
-PMC_TRAFFIC_BYTES = {1000: 139.80e6}
+# reading for a plain copy of 48,064,016 bytes (FETCH x2.0, WRITE x1.0).
+PMC_TRAFFIC_BYTES = {1000: 308.49e6}
+# tmx_bench_kernel id of the roofline kernel: the mixed PCG's direction update + FP64 operator
+ROOFLINE_KERNEL = 5
 
 
 def cpu_model() -> str:
@@ -153,8 +155,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-designs", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
-                    help="set up the N operator, then launch only the roofline kernel (FP32 "
-                         "fine-grid sweep, 20x) and a plain copy of the 2 n_v load vector "
+                    help="set up the N operator, then launch only the roofline kernel (PCG "
+                         "direction update + FP64 operator, 20x) and a plain copy of the 2 n_v "
+                         "load vector "
                          "(calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
 
@@ -177,7 +180,7 @@ def main():
         problem.autotune()  # the same tiles as the timed run
         problem.objective_and_gradient(rho)
         torch.cuda.synchronize()
-        us, nbytes = bench_kernel(problem, 1, "fp32", reps=20)
+        us, nbytes = bench_kernel(problem, ROOFLINE_KERNEL, "fp64", reps=20)
         F = problem.load_vector()  # one k_copy<double> of 2 n_v doubles
         torch.cuda.synchronize()
         print(json.dumps({"roofline_kernel_us": us, "algorithmic_bytes": nbytes,
@@ -189,7 +192,8 @@ def main():
                                                      "mixed", args.rtol)
     kernels = {}
     for name, kid, prec in (("apply_fp32", 0, "fp32"), ("smooth_fp32", 1, "fp32"),
-                            ("apply_fp64", 0, "fp64"), ("smooth_fp64", 1, "fp64")):
+                            ("apply_fp64", 0, "fp64"), ("smooth_fp64", 1, "fp64"),
+                            ("update_apply_fp64", ROOFLINE_KERNEL, "fp64")):
         us, nbytes = bench_kernel(problem, kid, prec)
         kernels[name] = {"us": us, "bytes": nbytes, "GBs": nbytes / us / 1e3}
 
@@ -212,10 +216,12 @@ def main():
                      "gradient_rel_err_vs_fp64":
                          float(torch.linalg.norm(wir - w64) / torch.linalg.norm(w64))}
 
-    # dominant kernel of the mixed step: the FP32 fine-grid block-Jacobi sweep
-    dom = kernels["smooth_fp32"]
-    roofline = {"kernel": "k_elast_sweep<float,false,kSmooth> (finest-level FP32 block-Jacobi sweep "
-                          "of the V-cycle, per-cell preconditioner weights)",
+    # the largest elasticity kernel of the mixed step: the PCG direction update fused with the
+    # FP64 operator application (one launch per iteration)
+    dom = kernels["update_apply_fp64"]
+    roofline = {"kernel": "k_elast_sweep<double,true,kApplyUpdate> (PCG direction update "
+                          "p = z + beta p_old with q = K(xi) p in FP64, p.q partials and the "
+                          "deferred x += alpha p_old)",
                 "bound": "hbm", "achieved": dom["GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": dom["GBs"] / PEAK_HBM_GBS, "traffic": PMC_TRAFFIC_BYTES.get(N),
                 "algorithmic_bytes_per_launch": dom["bytes"], "avg_launch_us": dom["us"]}

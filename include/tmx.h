@@ -159,7 +159,9 @@ int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
  * launches of one fine-grid kernel with the operator of the last solve:
  * kernel 0 = elasticity K(xi)u apply, 1 = elasticity block-Jacobi sweep, 2 = filter apply,
  * 3 = two elasticity sweeps from zero in one launch, 4 = prolongation and two sweeps in one
- * launch (3 and 4 with the handle's sweep-pair chunk heights).
+ * launch (3 and 4 with the handle's sweep-pair chunk heights), 5 = the mixed PCG's direction
+ * update with the FP64 operator application (p = z + beta p_old, q = K(xi) p, x += alpha p_old;
+ * TMX_FP64 only; runs on the PCG's own vectors).
  * precision TMX_FP64 or TMX_FP32.  Writes the per-launch algorithmic bytes to *bytes. */
 int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps, double* avg_us,
                      double* bytes);

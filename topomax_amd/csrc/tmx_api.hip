@@ -789,7 +789,8 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
 extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps,
                                 double* avg_us, double* bytes) {
   if (!h || !avg_us || reps < 1) return invalid("bad arguments");
-  if (kernel < 0 || kernel > 4) return invalid("unknown kernel id");
+  if (kernel < 0 || kernel > 5) return invalid("unknown kernel id");
+  if (kernel == 5 && precision != TMX_FP64) return invalid("kernel 5 runs with FP64 vectors");
   if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
   const bool elastic = kernel != 2;
   if (elastic && !h->el.has_operator_) return invalid("no elasticity operator yet: solve first");
@@ -814,6 +815,12 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
     if (kernel == 1) {
       if (f64) return h->el.smooth0<double>(h->u_s.get(), h->Fm.get(), h->tmp64.get(), s);
       return h->el.smooth0<float>(h->u32.get(), h->y32.get(), h->el.q32_.get(), s);
+    }
+    if (kernel == 5) {  // the mixed PCG's direction update + operator, on its own buffers
+      Hierarchy& e = h->el;
+      return e.apply_update<double>(e.p64_.get(), nullptr, e.q64_.get(), e.pb64_.get(),
+                                    h->ws.sc() + kPcg.beta, nullptr, s, e.z32_.get(),
+                                    e.x64_.get(), h->ws.sc() + kPcg.alpha);
     }
     if (kernel >= 3) {  // paired sweeps with the handle's chunk heights
       const int op1 = kernel == 3 ? kSmoothZero : kProlongSmooth;
@@ -849,6 +856,9 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
     const double per = kernel == 0 ? 5.0 : (kernel == 1 ? 7.0 : (kernel == 2 ? 2.0 :
                        (kernel == 3 ? 5.0 : 7.0)));
     *bytes = per * sz * (double)nv;
+    // direction update + operator per vertex: p_old (2 doubles), z (2 floats), xi, x read and
+    // written, p and q written: 96 B
+    if (kernel == 5) *bytes = 96.0 * (double)nv;
     const bool cw = h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u));
     if (cw && (kernel == 1 || kernel >= 3))
       *bytes += 2.0 * sz * (double)h->nx * (double)h->ny - sz * (double)nv;
