@@ -320,18 +320,23 @@ static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
   return rc;
 }
 
-extern "C" int tmx_mass_apply(tmx_problem* h, const double* x, double* y) {
-  if (!h || !x || !y) return invalid("null argument");
-  Scope scope(h);
+// y = M x on the handle's stream (no Scope: for use inside entry points)
+static int mass_apply(tmx_problem* h, const double* x, double* y) {
   TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, h->hx, h->hy),
                                kApply, x, nullptr, y, 0.0, nullptr, h->stream));
   return TMX_OK;
 }
 
+extern "C" int tmx_mass_apply(tmx_problem* h, const double* x, double* y) {
+  if (!h || !x || !y) return invalid("null argument");
+  Scope scope(h);
+  return mass_apply(h, x, y);
+}
+
 extern "C" int tmx_filter_apply(tmx_problem* h, const double* rho, double* xi) {
   if (!h || !rho || !xi) return invalid("null argument");
   Scope scope(h);
-  TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
+  TMX_TRY(mass_apply(h, rho, h->b_s.get()));
   return filter_rhs_solve(h, h->b_s.get(), xi);
 }
 
@@ -476,9 +481,7 @@ extern "C" int tmx_integrate(tmx_problem* h, const double* values, double* out) 
   return read_dot(h, nverts(h), h->lumped.get(), values, kScInt, out);
 }
 
-extern "C" int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
-  if (!h || !xi || !u || !b) return invalid("null argument");
-  Scope scope(h);
+static int sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
   TMX_TRY(need_penalty(h));
   if (h->degree == 2) {
     TMX_HIP(p2_sensitivity(h->g2, xi, u, h->p, h->m, b, h->stream));
@@ -487,6 +490,12 @@ extern "C" int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u
   TMX_HIP(elast_sensitivity(h->el.levels_[0].sh64, xi, u, h->p, h->m, h->lam, h->mu, h->hx, h->hy,
                             b, h->stream));
   return TMX_OK;
+}
+
+extern "C" int tmx_sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
+  if (!h || !xi || !u || !b) return invalid("null argument");
+  Scope scope(h);
+  return sensitivity(h, xi, u, b);
 }
 
 extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, double* J,
@@ -503,7 +512,7 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   SolveStats st;
   bool st_pending = false;
   auto finish = [&](int code) -> int {
-    const hipError_t e = hipStreamSynchronize(h->stream);
+    const hipError_t e = stream_wait(h->stream);
     if (st_pending) {
       st.complete();
       float setup = 0.f;
@@ -514,7 +523,7 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
     if (e != hipSuccess) TMX_HIP(e);
     return code;
   };
-  TMX_TRY(tmx_mass_apply(h, rho, h->b_s.get()));
+  TMX_TRY(mass_apply(h, rho, h->b_s.get()));
   int rc = filter_rhs_solve(h, h->b_s.get(), xiv, opts, false);
   if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return finish(rc);
   if (rc == TMX_OK) {
@@ -531,10 +540,10 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
                          hipMemcpyDeviceToHost, h->stream));
   if (w || Mw) {
     double* wv = w ? w : h->w_s.get();
-    TMX_TRY(tmx_sensitivity(h, xiv, uv, h->b_s.get()));
+    TMX_TRY(sensitivity(h, xiv, uv, h->b_s.get()));
     const int rc2 = filter_rhs_solve(h, h->b_s.get(), wv, opts, false);
     if (rc2 != TMX_OK && rc2 != TMX_ERR_NONFINITE) return finish(rc2);
-    if (Mw) TMX_TRY(tmx_mass_apply(h, wv, Mw));
+    if (Mw) TMX_TRY(mass_apply(h, wv, Mw));
     if (rc == TMX_OK) rc = rc2;
   }
   const int frc = finish(rc);

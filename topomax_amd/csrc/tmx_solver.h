@@ -119,6 +119,11 @@ struct Level {
 // inside FP64 iterative refinement
 enum SolvePrecision { kSolveFP64 = 0, kSolveMixed = 1, kSolveMixedIR = 2 };
 
+// Waits for the stream's work by polling it: the host is back within microseconds of the last
+// kernel (a blocking synchronisation wakes the thread up tens of microseconds later).  Returns
+// the stream's error, if any.
+hipError_t stream_wait(hipStream_t s);
+
 struct SolveParams {
   double rtol = 1e-10;
   int max_iter = 500;

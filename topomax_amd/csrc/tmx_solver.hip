@@ -40,10 +40,17 @@ Workspace::~Workspace() {
   if (host) (void)hipHostFree(host);
 }
 
+hipError_t stream_wait(hipStream_t s) {
+  for (;;) {
+    const hipError_t q = hipStreamQuery(s);
+    if (q != hipErrorNotReady) return q;
+  }
+}
+
 int Workspace::read(hipStream_t s, int first, int count) {
   TMX_HIP(hipMemcpyAsync(host + first, scalars.get() + first, sizeof(double) * count,
                          hipMemcpyDeviceToHost, s));
-  TMX_HIP(hipStreamSynchronize(s));
+  TMX_HIP(stream_wait(s));
   return 0;
 }
 
