@@ -159,7 +159,8 @@ int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
  * launches of one fine-grid kernel with the operator of the last solve:
  * kernel 0 = elasticity K(xi)u apply, 1 = elasticity block-Jacobi sweep, 2 = filter apply,
  * 3 = two elasticity sweeps from zero in one launch, 4 = prolongation and two sweeps in one
- * launch (3 and 4 with the handle's sweep-pair chunk heights), 5 = the mixed PCG's direction
+ * launch (3 and 4 with the handle's sweep-pair chunk heights), 6 = two sweeps from zero with
+ * the residual restricted to the next level in one launch, 5 = the mixed PCG's direction
  * update with the FP64 operator application (p = z + beta p_old, q = K(xi) p, x += alpha p_old;
  * TMX_FP64 only; runs on the PCG's own vectors).
  * precision TMX_FP64 or TMX_FP32.  Writes the per-launch algorithmic bytes to *bytes. */
@@ -196,8 +197,11 @@ enum tmx_option {
   TMX_OPT_SWEEP_PAIRS = 17,     /* > 0: the elasticity finest level runs its first two
                                    pre-sweeps as one launch, in node-row chunks of this height
                                    (default 4); 0: one launch per sweep */
-  TMX_OPT_SWEEP_PAIRS_POST = 18 /* the same for prolongation with the first two post-sweeps
+  TMX_OPT_SWEEP_PAIRS_POST = 18, /* the same for prolongation with the first two post-sweeps
                                    (default 16) */
+  TMX_OPT_SWEEP_PAIR_RESTRICT = 19 /* > 0: with two pre-sweeps, the zero-start pair also forms
+                                   the residual and restricts it to the next level, in chunks
+                                   of this many rows (default 12); 0: a separate launch */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

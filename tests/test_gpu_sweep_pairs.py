@@ -1,5 +1,6 @@
 """The paired-sweep launches of the finest elasticity level (two block-Jacobi sweeps per launch:
-zero start + sweep, and prolongation + sweep + sweep) against the launch-per-sweep V-cycle.
+zero start + sweep, optionally with the residual and its restriction fused behind them, and
+prolongation + sweep + sweep) against the launch-per-sweep V-cycle.
 
 Both compute the same sweeps in the same order, only without storing the intermediate iterate,
 so the preconditioner output agrees to round-off: 1e-12 relative in FP64 and 1e-5 in FP32.
@@ -39,11 +40,14 @@ def test_pairs_equal_per_sweep(nx, ny, fixed):
         gp.set_option("fine_cell_weights", cw)
         for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
             gp.set_option("sweep_pairs", 0)
+            gp.set_option("sweep_pairs_post", 0)
             z_ref = gp.precondition(r, "elasticity", xi, precision=prec)
-            for rows in (16, 1, 5):
+            for rows, rr in ((16, 12), (1, 0), (5, 2), (12, 3), (8, 64)):
                 gp.set_option("sweep_pairs", rows)
+                gp.set_option("sweep_pairs_post", rows)
+                gp.set_option("sweep_pair_restrict", rr)
                 z = gp.precondition(r, "elasticity", xi, precision=prec)
-                print(nx, ny, fixed, cw, prec, rows, rel(z, z_ref), flush=True)
+                print(nx, ny, fixed, cw, prec, rows, rr, rel(z, z_ref), flush=True)
                 assert torch.isfinite(z).all()
                 assert rel(z, z_ref) < tol
 
@@ -60,8 +64,10 @@ def test_pairs_sweep_counts(nu_pre, nu_post):
     r = residual_vector(nx, ny, 9)
     for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
         gp.set_option("sweep_pairs", 0)
+        gp.set_option("sweep_pairs_post", 0)
         z_ref = gp.precondition(r, "elasticity", xi, precision=prec)
         gp.set_option("sweep_pairs", 16)
+        gp.set_option("sweep_pairs_post", 16)
         z = gp.precondition(r, "elasticity", xi, precision=prec)
         print(nu_pre, nu_post, prec, rel(z, z_ref), flush=True)
         assert rel(z, z_ref) < tol
@@ -89,9 +95,11 @@ def test_pairs_solve_matches(precision):
     gp.set_penalization(3)
     rho = torch.as_tensor(smoothed_density(nx, ny, 3), device="cuda:0")
     gp.set_option("sweep_pairs", 0)
+    gp.set_option("sweep_pairs_post", 0)
     J0, w0, _ = gp.objective_and_gradient(rho)
     s0 = dict(gp.last_stats)
     gp.set_option("sweep_pairs", 16)
+    gp.set_option("sweep_pairs_post", 16)
     J1, w1, _ = gp.objective_and_gradient(rho)
     print(precision, J0, J1, s0, gp.last_stats, flush=True)
     assert abs(J1 - J0) < 1e-9 * abs(J0)

@@ -646,11 +646,14 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       break;
     case TMX_OPT_SWEEP_PAIRS:
     case TMX_OPT_SWEEP_PAIRS_POST:
+    case TMX_OPT_SWEEP_PAIR_RESTRICT:
       if (value < 0 || value > 256 || value != std::floor(value))
         return invalid("sweep-pair rows must be 0..256");
       h->el.drop_graphs();
       for (int i = 0; i < 2; ++i)
-        (option == TMX_OPT_SWEEP_PAIRS ? h->el.pair_rows_ : h->el.pair_rows_post_)[i] = (int)value;
+        (option == TMX_OPT_SWEEP_PAIRS        ? h->el.pair_rows_
+         : option == TMX_OPT_SWEEP_PAIRS_POST ? h->el.pair_rows_post_
+                                              : h->el.pair_rr_rows_)[i] = (int)value;
       break;
     case TMX_OPT_FINE_CELL_WEIGHTS:
       if (value < 0 || value > 3 || value != std::floor(value)) return invalid("mask must be 0..3");
@@ -751,12 +754,19 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
   TMX_TRY(h->el.fine_cell_weights(s));
   const int pair_cands[] = {2, 3, 4, 6, 8, 12, 16, 24};
   for (int prec = 0; prec < 2; ++prec) {
-    for (int kind = 0; kind < 2; ++kind) {
-      if (kind == 1 && !h->el.pair_post_ok()) continue;
-      int& slot = (kind == 0 ? h->el.pair_rows_ : h->el.pair_rows_post_)[prec == 0 ? 1 : 0];
+    for (int kind = 0; kind < 3; ++kind) {
+      if (kind >= 1 && !h->el.pair_post_ok()) continue;
+      int& slot = (kind == 0   ? h->el.pair_rows_
+                   : kind == 1 ? h->el.pair_rows_post_
+                               : h->el.pair_rr_rows_)[prec == 0 ? 1 : 0];
       if (slot <= 0) continue;  // switched off by the caller
       const int op1 = kind == 0 ? kSmoothZero : kProlongSmooth;
       auto launch = [&](int rows) -> int {
+        if (kind == 2) {
+          if (prec == 0)
+            return h->el.sweep_pair_rr0<double>(rows, h->Fm.get(), h->tmp64.get(), 0.5, 0.5, s);
+          return h->el.sweep_pair_rr0<float>(rows, h->y32.get(), h->el.q32_.get(), 0.5, 0.5, s);
+        }
         if (prec == 0)
           return h->el.sweep_pair0<double>(op1, rows, h->u_s.get(), h->Fm.get(), h->tmp64.get(),
                                             0.5, 0.5, nullptr, s);
@@ -798,7 +808,7 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
 extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps,
                                 double* avg_us, double* bytes) {
   if (!h || !avg_us || reps < 1) return invalid("bad arguments");
-  if (kernel < 0 || kernel > 5) return invalid("unknown kernel id");
+  if (kernel < 0 || kernel > 6) return invalid("unknown kernel id");
   if (kernel == 5 && precision != TMX_FP64) return invalid("kernel 5 runs with FP64 vectors");
   if (precision != TMX_FP64 && precision != TMX_FP32) return invalid("bad precision");
   const bool elastic = kernel != 2;
@@ -830,6 +840,13 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
       return e.apply_update<double>(e.p64_.get(), nullptr, e.q64_.get(), e.pb64_.get(),
                                     h->ws.sc() + kPcg.beta, nullptr, s, e.z32_.get(),
                                     e.x64_.get(), h->ws.sc() + kPcg.alpha);
+    }
+    if (kernel == 6) {  // zero-start pair with residual + restriction
+      if (!h->el.pair_post_ok()) return invalid("kernel 6 needs even nx and ny");
+      const int rows = h->el.pair_rr_rows_[f64 ? 1 : 0];
+      if (f64)
+        return h->el.sweep_pair_rr0<double>(rows, h->Fm.get(), h->tmp64.get(), 0.5, 0.5, s);
+      return h->el.sweep_pair_rr0<float>(rows, h->y32.get(), h->el.q32_.get(), 0.5, 0.5, s);
     }
     if (kernel >= 3) {  // paired sweeps with the handle's chunk heights
       const int op1 = kernel == 3 ? kSmoothZero : kProlongSmooth;
@@ -868,6 +885,7 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
     // direction update + operator per vertex: p_old (2 doubles), z (2 floats), xi, x read and
     // written, p and q written: 96 B
     if (kernel == 5) *bytes = 96.0 * (double)nv;
+    if (kernel == 6) *bytes += 2.0 * sz * (double)(h->nx / 2 + 1) * (double)(h->ny / 2 + 1);
     const bool cw = h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u));
     if (cw && (kernel == 1 || kernel >= 3))
       *bytes += 2.0 * sz * (double)h->nx * (double)h->ny - sz * (double)nv;

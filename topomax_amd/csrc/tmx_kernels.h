@@ -92,6 +92,12 @@ template <typename T>
 hipError_t elast_sweep_pair(const ElastOpArgs<T>& a, int op1, int rows, const T* x, const T* b,
                             T* y, T om1, T om2, RedSlot* dot, hipStream_t s, const T* ec);
 
+// The zero-start pair (y = S2 S1 0) with the residual b - A y restricted to the nested coarse
+// grid in the same launch (yc); nx, ny even, node-row chunks of `rows` (rounded down to even).
+template <typename T>
+hipError_t elast_sweep_pair_rr(const ElastOpArgs<T>& a, int rows, const T* b, T* y, T* yc,
+                               T om1, T om2, hipStream_t s);
+
 // per-cell mean r from nodal xi (for restriction of level-0 weights)
 template <typename T>
 hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,

@@ -603,6 +603,214 @@ template hipError_t elast_sweep_pair<float>(const ElastOpArgs<float>&, int, int,
                                             const float*, float*, float, float, RedSlot*,
                                             hipStream_t, const float*);
 
+// The zero-start pair with the residual and its nested restriction fused behind it: three
+// stages, each trailing the one before by a cell row.  At cell row cy the first sweep completes
+// its node row cy (from zero: the diagonal blocks only), the second sweep takes cell row cy - 1
+// and completes node row cy - 1 (written out: the pre-smoothed iterate), and the residual stage
+// takes cell row cy - 2 of the second sweep's iterate and completes the residual of node row
+// cy - 2; once the residual rows 2J - 1 .. 2J + 1 exist, the lanes of even columns write
+// coarse node row J.  The lanes carry four halo columns on each side (56 output columns); a
+// chunk of R (even) node rows sweeps R + 6 cell rows.
+template <typename T, bool XI>
+__global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a,
+                                                                const T* __restrict__ b,
+                                                                T* __restrict__ y,
+                                                                T* __restrict__ yc, T om1,
+                                                                T om2) {
+  using V = V2<T>;
+  constexpr int H = 4;
+  const SweepShape sh = a.sh;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
+  const int nxc = nx >> 1, nyc = ny >> 1;
+  const int c = strip * sh.stripw - H + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= H && lane < kWave - H;
+  const unsigned fixed = a.fixed;
+  const bool colfix = ((fixed & kLeft) && c == 0) || ((fixed & kRight) && c == nx);
+  const int r0 = chunk * sh.rows;
+  const int rend = min(r0 + sh.rows, ny + 1);  // owned node rows [r0, rend)
+  const int cy_begin = r0 - 4;
+  const int cy_end = wave_ok ? rend + 2 : cy_begin;
+  const int first_row = max(0, r0 - 3);  // first node row of the first sweep with a full diagonal
+  const V* B = reinterpret_cast<const V*>(b);
+  V* Y = reinterpret_cast<V*>(y);
+  V* YC = reinterpret_cast<V*>(yc);
+  const V* CW = reinterpret_cast<const V*>(a.cellw);
+  const ElastConst<T> k = a.k;
+  const V zero = {T(0), T(0)};
+
+  auto row_fixed = [&](int iy) {
+    return colfix || ((fixed & kBottom) && iy == 0) || ((fixed & kTop) && iy == ny);
+  };
+  auto ld_xi = [&](int iy) -> T {
+    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(unsigned)(iy * nvx + c)];
+    return T(0);
+  };
+
+  int cy = cy_begin;
+  T zb = ld_xi(cy);
+  T zbr = lane_down(zb);
+  Diag3<T> d_b = {T(0), T(0), T(0)};  // first sweep: diagonal block of node row cy so far
+  // second sweep: cell row cy - 1 between node rows cy - 1 (v_b) and cy of the first sweep
+  V vb = zero, vb_raw = zero, vbr = zero;
+  V acc2 = zero;
+  V bp = zero;  // b at node row cy - 1
+  Diag3<T> dp = {T(0), T(0), T(0)};
+  T invp = T(0);
+  T wAp = T(0), wBp = T(0);
+  // residual stage: cell row cy - 2 between node rows cy - 2 (s_b) and cy - 1 of the second sweep
+  V sb = zero, sb_raw = zero, sbr = zero;
+  V acc3 = zero;
+  V bpp = zero;  // b at node row cy - 2
+  T wApp = T(0), wBpp = T(0);
+  V rA = zero, rB = zero;  // residual rows cy - 4, cy - 3
+
+  for (; cy < cy_end; ++cy) {
+    // ---- first sweep, cell row cy
+    const T zt = ld_xi(cy + 1);
+    const T ztr = lane_down(zt);
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    T wA = T(0), wB = T(0);
+    if (cell_ok) {
+      if (XI) {
+        wA = pc_weight(a.m + a.cp * hsym3(zb, zbr, ztr, a.p), a.wmin);
+        wB = pc_weight(a.m + a.cp * hsym3(zb, zt, ztr, a.p), a.wmin);
+      } else {
+        const V w = CW[(unsigned)(cy * nx + c)];
+        wA = w.x;
+        wB = w.y;
+      }
+    }
+    const T wAl = lane_up(wA), wBl = lane_up(wB);
+    d_b.xx += wA * k.Pxx + wB * k.Qxx + wAl * k.Rxx;
+    d_b.xy += wAl * k.Rxy;
+    d_b.yy += wA * k.Pyy + wB * k.Qyy + wAl * k.Ryy;
+    Diag3<T> d_t;
+    d_t.xx = wB * k.Rxx + wAl * k.Qxx + wBl * k.Pxx;
+    d_t.xy = wB * k.Rxy;
+    d_t.yy = wB * k.Ryy + wAl * k.Qyy + wBl * k.Pyy;
+    const bool rowv = colv && cy >= first_row && cy <= ny;
+    const bool fx = row_fixed(cy);
+    V bb = zero, v_raw = zero;
+    T inv = T(0);
+    if (rowv) {
+      bb = B[(unsigned)(cy * nvx + c)];
+      V d;
+      if (fx) {
+        d = bb;
+      } else {
+        inv = recip(d_b.xx * d_b.yy - d_b.xy * d_b.xy);
+        d = {(d_b.yy * bb.x - d_b.xy * bb.y) * inv, (d_b.xx * bb.y - d_b.xy * bb.x) * inv};
+      }
+      v_raw = {om1 * d.x, om1 * d.y};
+    }
+    const V vt = fx ? zero : v_raw;
+    const V vtr = lane_down(vt);
+    // ---- second sweep, cell row cy - 1: node row cy - 1 complete
+    V y2_raw = zero;
+    const int ry = cy - 1;
+    const bool fx2 = row_fixed(ry);
+    {
+      V f0, f1, f2, f3;
+      cell_forces(k, wAp, wBp, vb, vbr, vt, vtr, f0, f1, f2, f3);
+      acc2 += f0;
+      acc2 += lane_up(f1);
+      if (colv && ry >= 0 && ry <= ny) {
+        const V Ax = fx2 ? vb_raw : acc2;
+        const V res = bp - Ax;
+        V d;
+        if (fx2) {
+          d = res;
+        } else {
+          d = {(dp.yy * res.x - dp.xy * res.y) * invp, (dp.xx * res.y - dp.xy * res.x) * invp};
+        }
+        y2_raw = {vb_raw.x + om2 * d.x, vb_raw.y + om2 * d.y};
+        if (ry >= r0 && ry < rend && outl) Y[(unsigned)(ry * nvx + c)] = y2_raw;
+      }
+      acc2 = f2;
+      acc2 += lane_up(f3);
+    }
+    const V y2 = fx2 ? zero : y2_raw;
+    const V y2r = lane_down(y2);
+    // ---- residual stage, cell row cy - 2: residual of node row cy - 2 complete
+    V rC = zero;
+    const int rk = cy - 2;
+    {
+      V g0, g1, g2, g3;
+      cell_forces(k, wApp, wBpp, sb, sbr, y2, y2r, g0, g1, g2, g3);
+      acc3 += g0;
+      acc3 += lane_up(g1);
+      if (colv && rk >= 0 && rk <= ny) {
+        const V Ax = row_fixed(rk) ? sb_raw : acc3;
+        rC = bpp - Ax;
+      }
+      acc3 = g2;
+      acc3 += lane_up(g3);
+    }
+    // coarse row J: centre fine row 2J = rk - 1 (rk odd), or 2J = ny at the top (ny even)
+    const bool odd_emit = (rk & 1) && rk - 1 >= r0 && rk - 1 < rend;
+    const bool top_emit = rk == ny && ny >= r0 && ny < rend;
+    if (odd_emit || top_emit) {
+      const V lo = odd_emit ? rA : rB, mid = odd_emit ? rB : rC;
+      const V hi = odd_emit ? rC : zero;
+      const V mr = lane_down(mid), ml = lane_up(mid), hr = lane_down(hi), ll = lane_up(lo);
+      const int J = odd_emit ? (rk - 1) >> 1 : ny >> 1;
+      if (outl && !(c & 1)) {
+        const int I = c >> 1;
+        V o = zero;
+        if (!node_fixed(fixed, I, J, nxc, nyc)) {
+          o.x = mid.x + T(0.5) * (mr.x + ml.x + hi.x + lo.x + hr.x + ll.x);
+          o.y = mid.y + T(0.5) * (mr.y + ml.y + hi.y + lo.y + hr.y + ll.y);
+        }
+        YC[(size_t)J * (nxc + 1) + I] = o;
+      }
+    }
+    rA = rB;
+    rB = rC;
+    // ---- shifts
+    sb = y2;
+    sb_raw = y2_raw;
+    sbr = y2r;
+    bpp = bp;
+    wApp = wAp;
+    wBpp = wBp;
+    vb = vt;
+    vb_raw = v_raw;
+    vbr = vtr;
+    bp = bb;
+    dp = d_b;
+    invp = inv;
+    wAp = wA;
+    wBp = wB;
+    d_b = d_t;
+    zb = zt;
+    zbr = ztr;
+  }
+}
+
+template <typename T>
+hipError_t elast_sweep_pair_rr(const ElastOpArgs<T>& a0, int rows, const T* b, T* y, T* yc,
+                               T om1, T om2, hipStream_t s) {
+  if (!b || !y || !yc || (!a0.xi && !a0.cellw)) return hipErrorInvalidValue;
+  if ((a0.sh.nx & 1) || (a0.sh.ny & 1)) return hipErrorInvalidValue;
+  if ((size_t)(a0.sh.nx + 1) * (a0.sh.ny + 1) >= (1ull << 31)) return hipErrorInvalidValue;
+  ElastOpArgs<T> a = a0;
+  a.sh = SweepShape::make(a0.sh.nx, a0.sh.ny, std::max(2, rows & ~1), kWave - 2 * 4);
+  const int nb = a.sh.blocks();
+  if (a.xi) k_elast_sweep_pair_rr<T, true><<<nb, kBlock, 0, s>>>(a, b, y, yc, om1, om2);
+  else k_elast_sweep_pair_rr<T, false><<<nb, kBlock, 0, s>>>(a, b, y, yc, om1, om2);
+  return hipGetLastError();
+}
+template hipError_t elast_sweep_pair_rr<double>(const ElastOpArgs<double>&, int, const double*,
+                                                double*, double*, double, double, hipStream_t);
+template hipError_t elast_sweep_pair_rr<float>(const ElastOpArgs<float>&, int, const float*,
+                                               float*, float*, float, float, hipStream_t);
+
 template <typename T>
 __global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
                                      T wmin, V2<T>* __restrict__ cw) {

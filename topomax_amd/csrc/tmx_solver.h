@@ -188,6 +188,9 @@ class Hierarchy {
   template <typename T>
   int sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, double om1, double om2,
                   RedSlot* dot, hipStream_t s);
+  // the zero-start pair with residual + restriction to level 1's right-hand side fused
+  template <typename T>
+  int sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s);
   // z = M^-1 b; at level 0 the last smoothing sweep also writes partials of b.z to dot_bz
   template <typename T>
   int vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz = nullptr);
@@ -256,6 +259,9 @@ class Hierarchy {
   // with the first two post-sweeps (pair_rows_post_), run as one launch each with node-row
   // chunks of that height; 0: one launch per sweep.  Index 0: FP32 V-cycles, 1: FP64.
   int pair_rows_[2] = {8, 8}, pair_rows_post_[2] = {12, 16};
+  // > 0: with two pre-sweeps, the zero-start pair also forms the residual and restricts it
+  // (chunks of this many rows); 0: a separate residual + restriction launch
+  int pair_rr_rows_[2] = {12, 12};
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
   bool pair_post_ok() const;  // the finest level has a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel

@@ -447,6 +447,18 @@ int Hierarchy::sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, doub
 }
 
 template <typename T>
+int Hierarchy::sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s) {
+  if (phys_ != kElasticity || !pair_post_ok() || rows < 1) {
+    set_error("internal: fused restriction needs the finest elasticity level over a nested one");
+    return -1;
+  }
+  const ElastOpArgs<T> a = elast_args<T>(*this, 0, xi_for<T>(*this), PrecBufs<T>::cw(levels_[0]),
+                                         sizeof(T) == 8, true);
+  TMX_HIP(elast_sweep_pair_rr<T>(a, rows, b, y, PrecBufs<T>::b(levels_[1]), (T)om1, (T)om2, s));
+  return 0;
+}
+
+template <typename T>
 static int restrict_to(Hierarchy& h, int l, const T* rf, T* bc, hipStream_t s) {
   const Level& F = h.levels_[l];
   const Level& C = h.levels_[l + 1];
@@ -594,10 +606,15 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   const int pi = sizeof(T) == 8 ? 1 : 0;
   const bool pair_pre = pairs && pair_rows_[pi] > 0 && npre >= 2;
   const bool pair_post = pairs && pair_rows_post_[pi] > 0 && fused_el && npost >= 2;
+  const bool pair_rr = pair_pre && npre == 2 && fused_el && pair_rr_rows_[pi] > 0;
   const int launches = npre - (pair_pre ? 1 : 0) + npost - (pair_post ? 1 : 0);
   int w = (launches - 1) % 2;
   int kpre = 1;
-  if (pair_pre) {
+  if (pair_rr) {  // writes the pre-smoothed iterate and the coarse right-hand side cb
+    TMX_TRY(sweep_pair_rr0<T>(pair_rr_rows_[pi], b, buf[w], sweep_omega(0, npre),
+                              sweep_omega(1, npre), s));
+    kpre = 2;
+  } else if (pair_pre) {
     TMX_TRY(sweep_pair0<T>(kSmoothZero, pair_rows_[pi], nullptr, b, buf[w], sweep_omega(0, npre),
                            sweep_omega(1, npre), nullptr, s));
     kpre = 2;
@@ -613,7 +630,9 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   // (9-point levels keep separate residual / restriction / prolongation launches: recomputing
   // the fine residuals per coarse node costs more than the extra launch there)
   const bool fused_st = false;
-  if (fused_el) {
+  if (pair_rr) {
+    // done by the pair
+  } else if (fused_el) {
     TMX_TRY(level_op<T>(*this, l, kResidualRestrict, buf[w], b, cb, nullptr, s));
   } else if (fused_st) {
     if (bs_ == 2)
@@ -1200,6 +1219,10 @@ template int Hierarchy::sweep_pair0<double>(int, int, const double*, const doubl
                                             double, double, RedSlot*, hipStream_t);
 template int Hierarchy::sweep_pair0<float>(int, int, const float*, const float*, float*, double,
                                            double, RedSlot*, hipStream_t);
+template int Hierarchy::sweep_pair_rr0<double>(int, const double*, double*, double, double,
+                                               hipStream_t);
+template int Hierarchy::sweep_pair_rr0<float>(int, const float*, float*, double, double,
+                                              hipStream_t);
 template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t, RedSlot*);
 template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t, RedSlot*);
 
