@@ -153,7 +153,8 @@ int tmx_integrate(tmx_problem* h, const double* values, double* out_host);
 
 /* Autotuner: times the FP64 and FP32 elasticity operator sweeps on this problem's fine grid
  * for each candidate row-chunk size and keeps the fastest; results are written to
- * rows_out[2] (fp64, fp32) and best_us_out[2]. */
+ * rows_out[2] (fp64, fp32) and best_us_out[2].  It also picks the chunk heights of the paired
+ * sweeps (TMX_OPT_SWEEP_PAIRS, _POST, _RESTRICT) per precision. */
 int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
 /* Average duration (microseconds, HIP events on the handle's stream) of `reps` back-to-back
  * launches of one fine-grid kernel with the operator of the last solve:
@@ -196,12 +197,14 @@ enum tmx_option {
                                    their kernels are launched one by one (measured faster) */
   TMX_OPT_SWEEP_PAIRS = 17,     /* > 0: the elasticity finest level runs its first two
                                    pre-sweeps as one launch, in node-row chunks of this height
-                                   (default 4); 0: one launch per sweep */
+                                   (default 8); 0: one launch per sweep */
   TMX_OPT_SWEEP_PAIRS_POST = 18, /* the same for prolongation with the first two post-sweeps
-                                   (default 16) */
+                                   (default 12 for FP32 V-cycles, 16 for FP64) */
   TMX_OPT_SWEEP_PAIR_RESTRICT = 19 /* > 0: with two pre-sweeps, the zero-start pair also forms
                                    the residual and restricts it to the next level, in chunks
-                                   of this many rows (default 12); 0: a separate launch */
+                                   of this many rows (default 12); 0: a separate launch.
+                                   Setting any of 17-19 applies to both precisions;
+                                   tmx_autotune re-picks the heights of those left > 0 */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the
