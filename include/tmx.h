@@ -200,11 +200,16 @@ enum tmx_option {
                                    (default 8); 0: one launch per sweep */
   TMX_OPT_SWEEP_PAIRS_POST = 18, /* the same for prolongation with the first two post-sweeps
                                    (default 12 for FP32 V-cycles, 16 for FP64) */
-  TMX_OPT_SWEEP_PAIR_RESTRICT = 19 /* > 0: with two pre-sweeps, the zero-start pair also forms
+  TMX_OPT_SWEEP_PAIR_RESTRICT = 19, /* > 0: with two pre-sweeps, the zero-start pair also forms
                                    the residual and restricts it to the next level, in chunks
                                    of this many rows (default 12); 0: a separate launch.
                                    Setting any of 17-19 applies to both precisions;
                                    tmx_autotune re-picks the heights of those left > 0 */
+  TMX_OPT_FILTER_PIPE = 20      /* > 0: the filter's finest level (even nx, ny; at most 4
+                                   sweeps per pass) runs as two register column pipelines in
+                                   node-row chunks of this height (default 12; tmx_autotune
+                                   re-picks it per direction and precision); 0: the band
+                                   kernels */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

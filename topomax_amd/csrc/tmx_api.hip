@@ -644,6 +644,12 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
     case TMX_OPT_GRAPHS:
       for (auto* x : hs) x->use_graphs_ = value != 0.0;
       break;
+    case TMX_OPT_FILTER_PIPE:
+      if (value < 0 || value > 256 || value != std::floor(value))
+        return invalid("filter pipeline rows must be 0..256");
+      h->fl.drop_graphs();
+      for (auto& r : h->fl.fpipe_rows_) r[0] = r[1] = (int)value;
+      break;
     case TMX_OPT_SWEEP_PAIRS:
     case TMX_OPT_SWEEP_PAIRS_POST:
     case TMX_OPT_SWEEP_PAIR_RESTRICT:
@@ -785,6 +791,36 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
         TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
         if (ms < best_pair) {
           best_pair = ms;
+          slot = rows;
+        }
+      }
+    }
+  }
+  // the filter's finest-level pipelines: chunk height per direction and precision
+  const Hierarchy& fl = h->fl;
+  const bool fl_pipes = fl.levels_.size() >= 2 && fl.levels_[1].nested_from_prev &&
+                        fl.nu_pre_ <= kMaxPipeNu && fl.nu_post_ <= kMaxPipeNu;
+  const int pipe_cands[] = {4, 6, 8, 10, 12, 16, 20, 24};
+  for (int prec = 0; prec < 2 && fl_pipes; ++prec) {
+    for (int dir = 0; dir < 2; ++dir) {
+      int& slot = h->fl.fpipe_rows_[prec == 0 ? 1 : 0][dir];
+      if (slot <= 0) continue;  // switched off by the caller
+      double best_pipe = 1e30;
+      for (int rows : pipe_cands) {
+        auto launch = [&]() -> int {
+          return prec == 0 ? h->fl.filter_pipe0<double>(dir == 0, rows, s)
+                           : h->fl.filter_pipe0<float>(dir == 0, rows, s);
+        };
+        const int reps = 20;
+        for (int warm = 0; warm < 2; ++warm) TMX_TRY(launch());
+        TMX_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) TMX_TRY(launch());
+        TMX_HIP(hipEventRecord(e1, s));
+        TMX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_pipe) {
+          best_pipe = ms;
           slot = rows;
         }
       }

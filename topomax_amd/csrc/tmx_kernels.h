@@ -137,6 +137,22 @@ template <typename T>
 hipError_t filter_sweep(const SweepShape& sh, const FilterConst<T>& k, int op, const T* x,
                         const T* b, T* y, T omega, RedSlot* dot, hipStream_t s,
                         const SweepExtra<T>& ex = SweepExtra<T>());
+
+// The filter's finest level as two column pipelines (nx, ny even): down = nu block-Jacobi
+// sweeps from zero (step sizes omega[0..nu)), y the smoothed iterate, yc the restricted
+// residual; up = x + P ec, then nu sweeps into y (partials of b.y to dot).  nu <= kMaxPipeNu.
+constexpr int kMaxPipeNu = 4;
+template <typename T>
+struct PipeOmega {
+  T w[kMaxPipeNu];
+};
+template <typename T>
+hipError_t filter_pipe_down(int nx, int ny, const FilterConst<T>& k, int nu, const T* omega,
+                            int rows, const T* b, T* y, T* yc, hipStream_t s);
+template <typename T>
+hipError_t filter_pipe_up(int nx, int ny, const FilterConst<T>& k, int nu, const T* omega,
+                          int rows, const T* x, const T* ec, const T* b, T* y, RedSlot* dot,
+                          hipStream_t s);
 template <typename T>
 hipError_t filter_to_stencil(int nx, int ny, const FilterConst<T>& k, T* st, hipStream_t s);
 

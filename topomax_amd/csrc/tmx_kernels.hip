@@ -1312,6 +1312,292 @@ template hipError_t filter_sweep<float>(const SweepShape&, const FilterConst<flo
                                         const float*, const float*, float*, float, RedSlot*,
                                         hipStream_t, const SweepExtra<float>&);
 
+// ---------------------------------------------------------------------------- filter pipelines
+// The filter's finest level as two column pipelines (no LDS): NU block-Jacobi sweeps stacked
+// in one pass over the grid, sweep s trailing sweep s - 1 by one cell row so that the
+// intermediate iterates stay in registers.  Each stage loses one lane of its neighbours on
+// each side, hence the halo columns: NU + 2 for the down pass (NU sweeps from zero, the
+// residual, the nested restriction), NU for the up pass (x + P e, NU sweeps, b.y).  The
+// right-hand side, the inverse diagonal and the cell mask of the rows a stage still needs
+// ride along in short register histories.  The filter operator has natural boundary
+// conditions only, so no node is fixed.
+template <typename T>
+__device__ __forceinline__ void filter_cell(T ka, T kb, T mA, T w, T u0, T u1, T u2, T u3, T& f0,
+                                            T& f1, T& f2, T& f3) {
+  const T SA = u0 + u1 + u3, SB = u0 + u2 + u3;
+  f0 = w * (ka * (u0 - u1) + mA * (u0 + SA) + kb * (u0 - u2) + mA * (u0 + SB));
+  f1 = w * (ka * (u1 - u0) + kb * (u1 - u3) + mA * (u1 + SA));
+  f2 = w * (kb * (u2 - u0) + ka * (u2 - u3) + mA * (u2 + SB));
+  f3 = w * (kb * (u3 - u1) + mA * (u3 + SA) + ka * (u3 - u2) + mA * (u3 + SB));
+}
+
+template <typename T, int NU>
+__global__ __launch_bounds__(kBlock) void k_filter_pipe_down(SweepShape sh, FilterConst<T> k,
+                                                             PipeOmega<T> om,
+                                                             const T* __restrict__ b,
+                                                             T* __restrict__ y,
+                                                             T* __restrict__ yc) {
+  constexpr int H = NU + 2;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1, nxc = nx >> 1;
+  const int c = strip * sh.stripw - H + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= H && lane < kWave - H;
+  const int r0 = chunk * sh.rows;
+  const int rend = min(r0 + sh.rows, ny + 1);  // owned node rows [r0, rend)
+  const int first_row = max(0, r0 - NU - 1);    // first sweep: first row with a full diagonal
+  const int cy_begin = r0 - NU - 2;
+  const int cy_end = wave_ok ? rend + NU : cy_begin;
+  const T ka = k.ka, kb = k.kb, mA = k.mA;
+  const T dfull = ka + kb + T(4) * mA, dhalf = ka + kb + T(2) * mA;
+  T bh[NU + 1], dh[NU + 1], wh[NU + 2];  // rows / cell rows cy, cy - 1, ...
+  T prev[NU + 2], prevr[NU + 2], acc[NU + 2];
+#pragma unroll
+  for (int i = 0; i < NU + 2; ++i) {
+    if (i < NU + 1) bh[i] = dh[i] = T(0);
+    wh[i] = prev[i] = prevr[i] = acc[i] = T(0);
+  }
+  T d_b = T(0), rA = T(0), rB = T(0);
+  for (int cy = cy_begin; cy < cy_end; ++cy) {
+#pragma unroll
+    for (int i = NU + 1; i > 0; --i) {
+      if (i <= NU) {
+        bh[i] = bh[i - 1];
+        dh[i] = dh[i - 1];
+      }
+      wh[i] = wh[i - 1];
+    }
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    const T w = cell_ok ? T(1) : T(0);
+    wh[0] = w;
+    const T wl = lane_up(w);
+    d_b += w * dfull + wl * dhalf;
+    const T d_t = w * dhalf + wl * dfull;
+    const bool rowv = colv && cy >= first_row && cy <= ny;
+    T bb = T(0), di = T(0);
+    if (rowv) {
+      bb = b[(size_t)cy * nvx + c];
+      di = T(1) / d_b;
+    }
+    bh[0] = bb;
+    dh[0] = di;
+    T cur = om.w[0] * bb * di;  // first sweep from zero, node row cy
+    if (NU == 1 && cy >= r0 && cy < rend && outl) y[(size_t)cy * nvx + c] = cur;
+    d_b = d_t;
+#pragma unroll
+    for (int s = 2; s <= NU + 1; ++s) {  // sweeps 2..NU, then the residual
+      const int row = cy - s + 1;
+      const T curr = lane_down(cur);
+      T f0, f1, f2, f3;
+      filter_cell(ka, kb, mA, wh[s - 1], prev[s], prevr[s], cur, curr, f0, f1, f2, f3);
+      acc[s] += f0;
+      acc[s] += lane_up(f1);
+      const bool ok = colv && row >= 0 && row <= ny;
+      T out = T(0);
+      if (s <= NU) {
+        if (ok) out = prev[s] + om.w[s - 1] * (bh[s - 1] - acc[s]) * dh[s - 1];
+        if (s == NU && row >= r0 && row < rend && outl) y[(size_t)row * nvx + c] = out;
+      } else if (ok) {
+        out = bh[s - 1] - acc[s];
+      }
+      acc[s] = f2;
+      acc[s] += lane_up(f3);
+      prev[s] = cur;
+      prevr[s] = curr;
+      cur = out;
+    }
+    // cur: residual of node row rk; coarse row J from fine rows 2J - 1 .. 2J + 1
+    const int rk = cy - NU;
+    const bool odd_emit = (rk & 1) && rk - 1 >= r0 && rk - 1 < rend;
+    const bool top_emit = rk == ny && ny >= r0 && ny < rend;
+    if (odd_emit || top_emit) {
+      const T lo = odd_emit ? rA : rB, mid = odd_emit ? rB : cur, hi = odd_emit ? cur : T(0);
+      const T mr = lane_down(mid), ml = lane_up(mid), hr = lane_down(hi), ll = lane_up(lo);
+      const int J = odd_emit ? (rk - 1) >> 1 : ny >> 1;
+      if (outl && !(c & 1))
+        yc[(size_t)J * (nxc + 1) + (c >> 1)] = mid + T(0.5) * (mr + ml + hi + lo + hr + ll);
+    }
+    rA = rB;
+    rB = cur;
+  }
+}
+
+template <typename T, int NU>
+__global__ __launch_bounds__(kBlock) void k_filter_pipe_up(SweepShape sh, FilterConst<T> k,
+                                                           PipeOmega<T> om,
+                                                           const T* __restrict__ x,
+                                                           const T* __restrict__ ec,
+                                                           const T* __restrict__ b,
+                                                           T* __restrict__ y,
+                                                           double* __restrict__ dpart) {
+  constexpr int H = NU;
+  const bool want_dot = dpart != nullptr;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
+  const bool wave_ok = chunk < sh.nchunks;
+  const int nx = sh.nx, ny = sh.ny, nvx = nx + 1, nxc = nx >> 1;
+  const int c = strip * sh.stripw - H + lane;
+  const bool colv = wave_ok && c >= 0 && c <= nx;
+  const bool cellc = wave_ok && c >= 0 && c < nx;
+  const bool outl = colv && lane >= H && lane < kWave - H;
+  const int r0 = chunk * sh.rows;
+  const int rend = min(r0 + sh.rows, ny + 1);
+  const int first_row = max(0, r0 - NU + 1);
+  const int cy_begin = r0 - NU;
+  const int cy_end = wave_ok ? rend + NU - 1 : cy_begin;
+  const T ka = k.ka, kb = k.kb, mA = k.mA;
+  const T dfull = ka + kb + T(4) * mA, dhalf = ka + kb + T(2) * mA;
+  auto ldx = [&](int iy) -> T {  // x + P e
+    T v = T(0);
+    if (colv && iy >= 0 && iy <= ny) {
+      const int I0 = c >> 1, J0 = iy >> 1, ox = c & 1, oy = iy & 1;
+      T e = ec[(size_t)J0 * (nxc + 1) + I0];
+      if (ox | oy) e = T(0.5) * (e + ec[(size_t)(J0 + oy) * (nxc + 1) + I0 + ox]);
+      v = x[(size_t)iy * nvx + c] + e;
+    }
+    return v;
+  };
+  T bh[NU], dh[NU], wh[NU + 1];
+  T prev[NU + 1], prevr[NU + 1], acc[NU + 1];
+#pragma unroll
+  for (int i = 0; i < NU + 1; ++i) {
+    if (i < NU) bh[i] = dh[i] = T(0);
+    wh[i] = prev[i] = prevr[i] = acc[i] = T(0);
+  }
+  T ub = ldx(cy_begin);
+  T ubr = lane_down(ub);
+  T d_b = T(0), acc1 = T(0);
+  double dot = 0.0;
+  for (int cy = cy_begin; cy < cy_end; ++cy) {
+#pragma unroll
+    for (int i = NU; i > 0; --i) {
+      if (i < NU) {
+        bh[i] = bh[i - 1];
+        dh[i] = dh[i - 1];
+      }
+      wh[i] = wh[i - 1];
+    }
+    const T ut = ldx(cy + 1);
+    const T utr = lane_down(ut);
+    const bool cell_ok = cellc && cy >= 0 && cy < ny;
+    const T w = cell_ok ? T(1) : T(0);
+    wh[0] = w;
+    T f0, f1, f2, f3;
+    filter_cell(ka, kb, mA, w, ub, ubr, ut, utr, f0, f1, f2, f3);
+    acc1 += f0;
+    acc1 += lane_up(f1);
+    const T wl = lane_up(w);
+    d_b += w * dfull + wl * dhalf;
+    const T d_t = w * dhalf + wl * dfull;
+    const bool rowv = colv && cy >= first_row && cy <= ny;
+    T bb = T(0), di = T(0), cur = T(0);
+    if (rowv) {
+      bb = b[(size_t)cy * nvx + c];
+      di = T(1) / d_b;
+      cur = ub + om.w[0] * (bb - acc1) * di;
+    }
+    bh[0] = bb;
+    dh[0] = di;
+    if (NU == 1 && cy >= r0 && cy < rend && outl) {
+      y[(size_t)cy * nvx + c] = cur;
+      if (want_dot) dot += (double)bb * (double)cur;
+    }
+    acc1 = f2;
+    acc1 += lane_up(f3);
+    d_b = d_t;
+    ub = ut;
+    ubr = utr;
+#pragma unroll
+    for (int s = 2; s <= NU; ++s) {
+      const int row = cy - s + 1;
+      const T curr = lane_down(cur);
+      T g0, g1, g2, g3;
+      filter_cell(ka, kb, mA, wh[s - 1], prev[s], prevr[s], cur, curr, g0, g1, g2, g3);
+      acc[s] += g0;
+      acc[s] += lane_up(g1);
+      T out = T(0);
+      if (colv && row >= 0 && row <= ny)
+        out = prev[s] + om.w[s - 1] * (bh[s - 1] - acc[s]) * dh[s - 1];
+      if (s == NU && row >= r0 && row < rend && outl) {
+        y[(size_t)row * nvx + c] = out;
+        if (want_dot) dot += (double)bh[s - 1] * (double)out;
+      }
+      acc[s] = g2;
+      acc[s] += lane_up(g3);
+      prev[s] = cur;
+      prevr[s] = curr;
+      cur = out;
+    }
+  }
+  if (want_dot) {
+    const double v[1] = {dot};
+    double* const o[1] = {dpart};
+    block_partials<1>(v, o);
+  }
+}
+
+template <typename T>
+hipError_t filter_pipe_down(int nx, int ny, const FilterConst<T>& k, int nu, const T* omega,
+                            int rows, const T* b, T* y, T* yc, hipStream_t s) {
+  if ((nx & 1) || (ny & 1) || nu < 1 || nu > kMaxPipeNu || !b || !y || !yc)
+    return hipErrorInvalidValue;
+  if ((size_t)(nx + 1) * (ny + 1) >= (1ull << 31)) return hipErrorInvalidValue;
+  PipeOmega<T> om{};
+  for (int i = 0; i < nu; ++i) om.w[i] = omega[i];
+  const SweepShape sh = SweepShape::make(nx, ny, std::max(2, rows & ~1), kWave - 2 * (nu + 2));
+  const int nb = sh.blocks();
+  switch (nu) {
+    case 1: k_filter_pipe_down<T, 1><<<nb, kBlock, 0, s>>>(sh, k, om, b, y, yc); break;
+    case 2: k_filter_pipe_down<T, 2><<<nb, kBlock, 0, s>>>(sh, k, om, b, y, yc); break;
+    case 3: k_filter_pipe_down<T, 3><<<nb, kBlock, 0, s>>>(sh, k, om, b, y, yc); break;
+    default: k_filter_pipe_down<T, 4><<<nb, kBlock, 0, s>>>(sh, k, om, b, y, yc); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t filter_pipe_up(int nx, int ny, const FilterConst<T>& k, int nu, const T* omega,
+                          int rows, const T* x, const T* ec, const T* b, T* y, RedSlot* dot,
+                          hipStream_t s) {
+  if ((nx & 1) || (ny & 1) || nu < 1 || nu > kMaxPipeNu || !x || !ec || !b || !y)
+    return hipErrorInvalidValue;
+  if ((size_t)(nx + 1) * (ny + 1) >= (1ull << 31)) return hipErrorInvalidValue;
+  PipeOmega<T> om{};
+  for (int i = 0; i < nu; ++i) om.w[i] = omega[i];
+  const SweepShape sh = SweepShape::make(nx, ny, std::max(1, rows), kWave - 2 * nu);
+  const int nb = sh.blocks();
+  double* dp = nullptr;
+  if (dot) {
+    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    dp = dot->partials;
+    dot->nblocks = nb;
+  }
+  switch (nu) {
+    case 1: k_filter_pipe_up<T, 1><<<nb, kBlock, 0, s>>>(sh, k, om, x, ec, b, y, dp); break;
+    case 2: k_filter_pipe_up<T, 2><<<nb, kBlock, 0, s>>>(sh, k, om, x, ec, b, y, dp); break;
+    case 3: k_filter_pipe_up<T, 3><<<nb, kBlock, 0, s>>>(sh, k, om, x, ec, b, y, dp); break;
+    default: k_filter_pipe_up<T, 4><<<nb, kBlock, 0, s>>>(sh, k, om, x, ec, b, y, dp); break;
+  }
+  return hipGetLastError();
+}
+template hipError_t filter_pipe_down<double>(int, int, const FilterConst<double>&, int,
+                                             const double*, int, const double*, double*, double*,
+                                             hipStream_t);
+template hipError_t filter_pipe_down<float>(int, int, const FilterConst<float>&, int,
+                                            const float*, int, const float*, float*, float*,
+                                            hipStream_t);
+template hipError_t filter_pipe_up<double>(int, int, const FilterConst<double>&, int,
+                                           const double*, int, const double*, const double*,
+                                           const double*, double*, RedSlot*, hipStream_t);
+template hipError_t filter_pipe_up<float>(int, int, const FilterConst<float>&, int, const float*,
+                                          int, const float*, const float*, const float*, float*,
+                                          RedSlot*, hipStream_t);
+
 template <typename T>
 __global__ void k_filter_to_stencil(int nx, int ny, FilterConst<T> k, T* __restrict__ st) {
   const size_t nn = (size_t)(nx + 1) * (ny + 1);

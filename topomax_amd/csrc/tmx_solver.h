@@ -188,6 +188,10 @@ class Hierarchy {
   template <typename T>
   int sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, double om1, double om2,
                   RedSlot* dot, hipStream_t s);
+  // one launch of the filter's finest-level pipeline (down or up) on the hierarchy's own
+  // level buffers, for timing (tmx_autotune)
+  template <typename T>
+  int filter_pipe0(bool down, int rows, hipStream_t s);
   // the zero-start pair with residual + restriction to level 1's right-hand side fused
   template <typename T>
   int sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s);
@@ -262,6 +266,10 @@ class Hierarchy {
   // > 0: with two pre-sweeps, the zero-start pair also forms the residual and restricts it
   // (chunks of this many rows); 0: a separate residual + restriction launch
   int pair_rr_rows_[2] = {12, 12};
+  // > 0: the filter's finest level runs as two column pipelines (tmx_kernels.h
+  // filter_pipe_down / _up) with node-row chunks of these heights ([FP32, FP64][down, up]);
+  // 0: band kernels
+  int fpipe_rows_[2][2] = {{12, 12}, {12, 12}};
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
   bool pair_post_ok() const;  // the finest level has a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel

@@ -447,6 +447,31 @@ int Hierarchy::sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, doub
 }
 
 template <typename T>
+int Hierarchy::filter_pipe0(bool down, int rows, hipStream_t s) {
+  if (phys_ != kFilter || levels_.size() < 2 || !levels_[1].nested_from_prev) {
+    set_error("internal: the filter pipelines need a nested level below the finest");
+    return -1;
+  }
+  Level& L = levels_[0];
+  Level& C = levels_[1];
+  const int nu = down ? nu_pre_ : nu_post_;
+  if (nu > kMaxPipeNu) {
+    set_error("internal: too many sweeps for the filter pipelines");
+    return -1;
+  }
+  T om[kMaxPipeNu];
+  for (int k = 0; k < nu; ++k) om[k] = (T)sweep_omega(k, nu);
+  const FilterConst<T> fk = FilterConst<T>::make(eps_, L.hx, L.hy);
+  if (down)
+    TMX_HIP(filter_pipe_down<T>(L.nx, L.ny, fk, nu, om, rows, PrecBufs<T>::b(L),
+                                PrecBufs<T>::t(L), PrecBufs<T>::b(C), s));
+  else
+    TMX_HIP(filter_pipe_up<T>(L.nx, L.ny, fk, nu, om, rows, PrecBufs<T>::t(L), PrecBufs<T>::x(C),
+                              PrecBufs<T>::b(L), PrecBufs<T>::x(L), nullptr, s));
+  return 0;
+}
+
+template <typename T>
 int Hierarchy::sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s) {
   if (phys_ != kElasticity || !pair_post_ok() || rows < 1) {
     set_error("internal: fused restriction needs the finest elasticity level over a nested one");
@@ -581,6 +606,23 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   if (use_tail_ && l == tail_ && l > 0) return tail_launch<T>(*this, b, out, s);
   Level& L = levels_[l];
   Level& C = levels_[l + 1];
+  const int pk = sizeof(T) == 8 ? 1 : 0;
+  if (l == 0 && phys_ == kFilter && L.kind == kElem && C.nested_from_prev &&
+      fpipe_rows_[pk][0] > 0 && fpipe_rows_[pk][1] > 0 && nu_pre_ <= kMaxPipeNu &&
+      nu_post_ <= kMaxPipeNu) {
+    T* t = PrecBufs<T>::t(L);
+    T* cb = PrecBufs<T>::b(C);
+    T* cx = PrecBufs<T>::x(C);
+    T om_d[kMaxPipeNu], om_u[kMaxPipeNu];
+    for (int k = 0; k < nu_pre_; ++k) om_d[k] = (T)sweep_omega(k, nu_pre_);
+    for (int k = 0; k < nu_post_; ++k) om_u[k] = (T)sweep_omega(k, nu_post_);
+    const FilterConst<T> fk = FilterConst<T>::make(eps_, L.hx, L.hy);
+    TMX_HIP(filter_pipe_down<T>(L.nx, L.ny, fk, nu_pre_, om_d, fpipe_rows_[pk][0], b, t, cb, s));
+    TMX_TRY(vcycle<T>(1, cb, cx, s));
+    TMX_HIP(filter_pipe_up<T>(L.nx, L.ny, fk, nu_post_, om_u, fpipe_rows_[pk][1], t, cx, b, out,
+                              dot_bz, s));
+    return 0;
+  }
   BandLevel<T> B;
   if (band_level<T>(*this, l, B)) {
     T* t = PrecBufs<T>::t(L);
@@ -1219,6 +1261,8 @@ template int Hierarchy::sweep_pair0<double>(int, int, const double*, const doubl
                                             double, double, RedSlot*, hipStream_t);
 template int Hierarchy::sweep_pair0<float>(int, int, const float*, const float*, float*, double,
                                            double, RedSlot*, hipStream_t);
+template int Hierarchy::filter_pipe0<double>(bool, int, hipStream_t);
+template int Hierarchy::filter_pipe0<float>(bool, int, hipStream_t);
 template int Hierarchy::sweep_pair_rr0<double>(int, const double*, double*, double, double,
                                                hipStream_t);
 template int Hierarchy::sweep_pair_rr0<float>(int, const float*, float*, double, double,
