@@ -10,6 +10,10 @@
 // the right lane the same way.  Every HBM value is read once per sweep (plus one halo row
 // per chunk and two halo columns per 64), no LDS is used, and a node row is written as soon
 // as the two cell rows around it are done.
+//
+// Several sweeps stack in one pass the same way (k_elast_sweep_pair, k_elast_sweep_pair_rr,
+// k_filter_pipe_*): each stage trails the one before by one cell row, so intermediate iterates
+// never leave registers, and each stage costs one more halo column on each side of the strip.
 
 #include "tmx_device.h"
 #include "tmx_kernels.h"
