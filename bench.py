@@ -193,6 +193,8 @@ def main():
     kernels = {}
     for name, kid, prec in (("apply_fp32", 0, "fp32"), ("smooth_fp32", 1, "fp32"),
                             ("apply_fp64", 0, "fp64"), ("smooth_fp64", 1, "fp64"),
+                            ("sweep_pair_restrict_fp32", 6, "fp32"),
+                            ("sweep_pair_prolong_fp32", 4, "fp32"),
                             ("update_apply_fp64", ROOFLINE_KERNEL, "fp64")):
         us, nbytes = bench_kernel(problem, kid, prec)
         kernels[name] = {"us": us, "bytes": nbytes, "GBs": nbytes / us / 1e3}

@@ -915,17 +915,18 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
     // xi per node.
     // A pair reads b and writes y (zero start) or also reads x (prolongation; the coarse
     // correction is a quarter-size stream).
-    const double per = kernel == 0 ? 5.0 : (kernel == 1 ? 7.0 : (kernel == 2 ? 2.0 :
-                       (kernel == 3 ? 5.0 : 7.0)));
+    const double per = kernel == 0 ? 5.0 : kernel == 1 ? 7.0 : kernel == 2 ? 2.0 :
+                       (kernel == 3 || kernel == 6) ? 5.0 : 7.0;
     *bytes = per * sz * (double)nv;
+    const bool cw = h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u));
+    if (cw && (kernel == 1 || kernel == 3 || kernel == 4 || kernel == 6))
+      *bytes += 2.0 * sz * (double)h->nx * (double)h->ny - sz * (double)nv;
+    // the coarse stream: correction read (4) or restricted residual written (6)
+    if (kernel == 4 || kernel == 6)
+      *bytes += 2.0 * sz * (double)(h->nx / 2 + 1) * (double)(h->ny / 2 + 1);
     // direction update + operator per vertex: p_old (2 doubles), z (2 floats), xi, x read and
     // written, p and q written: 96 B
     if (kernel == 5) *bytes = 96.0 * (double)nv;
-    if (kernel == 6) *bytes += 2.0 * sz * (double)(h->nx / 2 + 1) * (double)(h->ny / 2 + 1);
-    const bool cw = h->el.has_cw0_ && (h->el.fine_cw_ & (f64 ? 2u : 1u));
-    if (cw && (kernel == 1 || kernel >= 3))
-      *bytes += 2.0 * sz * (double)h->nx * (double)h->ny - sz * (double)nv;
-    if (kernel == 4) *bytes += 2.0 * sz * (double)(h->nx / 2 + 1) * (double)(h->ny / 2 + 1);
   }
   return TMX_OK;
 }
