@@ -172,3 +172,29 @@ def test_reference_record_pickle_layout(tmp_path):
     assert any(n == "STACK_GLOBAL" for n, _ in ops) and any(n == "NEWOBJ" for n, _ in ops)
     fmt = _penalty_formatter([1.0, 2.5, 10.25])
     assert [fmt(p) for p in (1.0, 2.5, 10.25)] == ["01.00", "02.50", "10.25"]
+
+
+def test_option_constants_match_header():
+    """Every TMX_OPT_* of include/tmx.h has the same value in the ctypes binding (as the C
+    compiler evaluates the enum), and a name in ElasticityProblem.set_option."""
+    import inspect
+    import tempfile
+    from topomax_amd import _lib
+    from topomax_amd.elasticity_problem import ElasticityProblem
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(TMX_OPT_[A-Z0-9_]+)\s*=", text)))
+    assert len(names) >= 20
+    body = "".join(f'  printf("%d\\n", (int){n});\n' for n in names)
+    src = '#include <stdio.h>\n#include "tmx.h"\nint main(void) {\n' + body + "  return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "o.c")
+        exe = os.path.join(d, "o")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                               check=True).stdout.split()]
+    for n, v in zip(names, vals):
+        assert getattr(_lib, n) == v, n
+    src_py = inspect.getsource(ElasticityProblem.set_option)
+    for n in names:
+        assert f"_lib.{n}" in src_py, f"{n} has no set_option name"
