@@ -205,11 +205,15 @@ enum tmx_option {
                                    of this many rows (default 12); 0: a separate launch.
                                    Setting any of 17-19 applies to both precisions;
                                    tmx_autotune re-picks the heights of those left > 0 */
-  TMX_OPT_FILTER_PIPE = 20      /* > 0: the filter's finest level (even nx, ny; at most 4
+  TMX_OPT_FILTER_PIPE = 20,     /* > 0: the filter's finest level (even nx, ny; at most 4
                                    sweeps per pass) runs as two register column pipelines in
                                    node-row chunks of this height (default 12; tmx_autotune
                                    re-picks it per direction and precision); 0: the band
                                    kernels */
+  TMX_OPT_FILTER_DCT = 21       /* 1 (default): the filter's FP32 preconditioner is the exact
+                                   DCT-I / cyclic-reduction solver of eps^2 K + (lumped mass)
+                                   when ny is even and nx + 1 <= 3072 (3-5 PCG iterations);
+                                   0: the multigrid V-cycle */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

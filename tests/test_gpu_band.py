@@ -37,6 +37,7 @@ def test_band_equals_per_sweep(nx, ny, fixed):
     gp = gpu_problem(case, nx, ny)
     gp.set_penalization(3)
     gp.set_option("filter_pipe", 0)  # the filter's finest level in the band kernels too
+    gp.set_option("filter_dct", 0)  # and its FP32 preconditioner the V-cycle
     xi = torch.as_tensor(smoothed_density(nx, ny, 1, sigma=2.0), device="cuda:0")
     r, rf = vectors(nx, ny, 5)
     for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
@@ -61,6 +62,7 @@ def test_band_repeated_calls_identical():
     gp = gpu_problem(CANTILEVER, nx, ny)
     gp.set_penalization(3)
     gp.set_option("filter_pipe", 0)
+    gp.set_option("filter_dct", 0)
     xi = torch.as_tensor(smoothed_density(nx, ny, 2, sigma=2.0), device="cuda:0")
     r, rf = vectors(nx, ny, 6)
     for prec in ("fp64", "fp32"):
@@ -80,6 +82,7 @@ def test_band_solve_matches(precision):
     gp = gpu_problem(CANTILEVER, nx, ny, precision=precision, rtol=1e-11)
     gp.set_penalization(3)
     gp.set_option("filter_pipe", 0)
+    gp.set_option("filter_dct", 0)
     rho = torch.as_tensor(smoothed_density(nx, ny, 3), device="cuda:0")
     gp.set_option("band", 0)
     J0, w0, _ = gp.objective_and_gradient(rho)
@@ -101,6 +104,7 @@ def test_band_sweep_counts(nu_coarse, nu_filter, nu_fine_filter):
     gp = gpu_problem(case, nx, ny)
     gp.set_penalization(3)
     gp.set_option("filter_pipe", 0)
+    gp.set_option("filter_dct", 0)
     gp.set_option("nu_coarse", nu_coarse)
     gp.set_option("nu_coarse_filter", nu_filter)
     gp.set_option("nu_fine_filter", nu_fine_filter)

@@ -30,7 +30,7 @@ TMX_OPT_NU_COARSE, TMX_OPT_FILTER_RTOL = 6, 7
 TMX_OPT_CHEB_ELAST, TMX_OPT_CHEB_FILTER, TMX_OPT_CHEB_RATIO, TMX_OPT_BAND = 10, 11, 12, 13
 TMX_OPT_BAND_FINE_FILTER, TMX_OPT_FINE_CELL_WEIGHTS, TMX_OPT_GRAPHS = 14, 15, 16
 TMX_OPT_SWEEP_PAIRS, TMX_OPT_SWEEP_PAIRS_POST, TMX_OPT_SWEEP_PAIR_RESTRICT = 17, 18, 19
-TMX_OPT_FILTER_PIPE = 20
+TMX_OPT_FILTER_PIPE, TMX_OPT_FILTER_DCT = 20, 21
 TMX_OPT_NU_FINE_FILTER, TMX_OPT_NU_COARSE_FILTER = 8, 9
 
 SIDE_CODES = {"Left": TMX_LEFT, "Right": TMX_RIGHT, "Top": TMX_TOP, "Bottom": TMX_BOTTOM}

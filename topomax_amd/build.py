@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtmx.so")
-SOURCES = ["tmx_kernels.hip", "tmx_band.hip", "tmx_p2.hip", "tmx_solver.hip", "tmx_api.hip"]
+SOURCES = ["tmx_kernels.hip", "tmx_band.hip", "tmx_p2.hip", "tmx_dct.hip", "tmx_solver.hip",
+           "tmx_api.hip"]
 ARCH = os.environ.get("TMX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -59,7 +60,8 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
     while procs:
         _wait(procs.pop(0))
     tmp = LIB + ".tmp"
-    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
+    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs,
+            "-lrocblas"]
     if verbose:
         print(" ".join(link), flush=True)
     _run(link)

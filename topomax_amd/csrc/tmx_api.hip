@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "tmx_p2.h"
+#include "tmx_dct.h"
 #include "tmx_solver.h"
 
 namespace tmx {
@@ -643,6 +644,10 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       break;
     case TMX_OPT_GRAPHS:
       for (auto* x : hs) x->use_graphs_ = value != 0.0;
+      break;
+    case TMX_OPT_FILTER_DCT:
+      h->fl.drop_graphs();
+      h->fl.use_dct_ = value != 0.0;
       break;
     case TMX_OPT_FILTER_PIPE:
       if (value < 0 || value > 256 || value != std::floor(value))

@@ -1,0 +1,222 @@
+// DCT-I / cyclic-reduction solver of the filter's tensor-lumped operator (see tmx_dct.h).
+//
+// Data layout: vectors are node-major rows, v = iy (nx + 1) + ix, so a y-column is strided and a
+// mode row (fixed y-mode, all x) is contiguous.  Seen column-major, a block of rows is an
+// (nx + 1) x rows matrix with leading dimension nx + 1, and the y transforms are plain GEMMs
+// with the folded DCT-I matrices on the right.
+#include "tmx_dct.h"
+
+#include <cmath>
+#include <vector>
+
+#include "tmx_device.h"
+
+namespace tmx {
+
+// s[j] = r[j] + r[n - j] (j < h), s[h] = r[h]; d[j] = r[j] - r[n - j] (j < h); rows of mx values
+__global__ void k_dct_fold(int mx, int n, const float* __restrict__ r, float* __restrict__ s,
+                           float* __restrict__ d) {
+  const int h = n / 2;
+  const size_t total = (size_t)(h + 1) * mx;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t / mx), ix = (int)(t - (size_t)j * mx);
+    const float a = r[(size_t)j * mx + ix];
+    if (j < h) {
+      const float b = r[(size_t)(n - j) * mx + ix];
+      s[t] = a + b;
+      d[t] = a - b;
+    } else {
+      s[t] = a;
+    }
+  }
+}
+
+// z[j] = e[j] + o[j], z[n - j] = e[j] - o[j] (j < h), z[h] = e[h]; partials of r.z
+__global__ __launch_bounds__(kBlock) void k_dct_unfold(int mx, int n, const float* __restrict__ e,
+                                                       const float* __restrict__ o,
+                                                       const float* __restrict__ r,
+                                                       float* __restrict__ z,
+                                                       double* __restrict__ part) {
+  const int h = n / 2;
+  const size_t total = (size_t)(h + 1) * mx;
+  double acc = 0.0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t / mx), ix = (int)(t - (size_t)j * mx);
+    const size_t lo = (size_t)j * mx + ix;
+    if (j < h) {
+      const size_t hi = (size_t)(n - j) * mx + ix;
+      const float zl = e[t] + o[t], zh = e[t] - o[t];
+      z[lo] = zl;
+      z[hi] = zh;
+      acc += (double)r[lo] * (double)zl + (double)r[hi] * (double)zh;
+    } else {
+      const float zl = e[t];
+      z[lo] = zl;
+      acc += (double)r[lo] * (double)zl;
+    }
+  }
+  if (part) {
+    const double v[1] = {acc};
+    double* const out[1] = {part};
+    block_partials<1>(v, out);
+  }
+}
+
+// One workgroup per mode row (mode = 2 * blockIdx.x + parity): solves
+// (a T_x + c_m W_x) u = f / D_m in place by parallel cyclic reduction in LDS.
+constexpr int kPcrThreads = 1024;
+constexpr int kPcrPer = DctPrecond::kMaxRow / kPcrThreads;  // equations per thread
+
+__global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, int parity, float a_off,
+                                                         double hxhy, double cy,
+                                                         float* __restrict__ rows) {
+  __shared__ float la[DctPrecond::kMaxRow], lb[DctPrecond::kMaxRow], lc[DctPrecond::kMaxRow],
+      lf[DctPrecond::kMaxRow];
+  const int mode = 2 * (int)blockIdx.x + parity;
+  const double lam = 2.0 - 2.0 * std::cos(M_PI * (double)mode / (double)n);
+  const float c = (float)(hxhy + cy * lam);
+  const float inv_d = (mode == 0 || mode == n) ? 1.0f / (float)n : 2.0f / (float)n;
+  float* row = rows + (size_t)blockIdx.x * mx;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < mx; i += kPcrThreads) {
+    const bool edge = i == 0 || i == mx - 1;
+    la[i] = i > 0 ? -a_off : 0.0f;
+    lc[i] = i < mx - 1 ? -a_off : 0.0f;
+    lb[i] = a_off * (edge ? 1.0f : 2.0f) + c * (edge ? 0.5f : 1.0f);
+    lf[i] = row[i] * inv_d;
+  }
+  __syncthreads();
+  for (int s = 1; s < mx; s *= 2) {
+    float na[kPcrPer], nb[kPcrPer], nc[kPcrPer], nf[kPcrPer];
+#pragma unroll
+    for (int q = 0; q < kPcrPer; ++q) {
+      const int i = tid + q * kPcrThreads;
+      if (i >= mx) continue;
+      const int im = i - s, ip = i + s;
+      float k1 = 0.0f, k2 = 0.0f, ai = 0.0f, ci = 0.0f;
+      float bi = lb[i], fi = lf[i];
+      if (im >= 0) {
+        k1 = la[i] / lb[im];
+        ai = -la[im] * k1;
+        bi -= lc[im] * k1;
+        fi -= lf[im] * k1;
+      }
+      if (ip < mx) {
+        k2 = lc[i] / lb[ip];
+        ci = -lc[ip] * k2;
+        bi -= la[ip] * k2;
+        fi -= lf[ip] * k2;
+      }
+      na[q] = ai;
+      nb[q] = bi;
+      nc[q] = ci;
+      nf[q] = fi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPcrPer; ++q) {
+      const int i = tid + q * kPcrThreads;
+      if (i >= mx) continue;
+      la[i] = na[q];
+      lb[i] = nb[q];
+      lc[i] = nc[q];
+      lf[i] = nf[q];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < mx; i += kPcrThreads) row[i] = lf[i] / lb[i];
+}
+
+bool DctPrecond::supported(int nx, int ny) {
+  return ny >= 2 && !(ny & 1) && nx >= 1 && nx + 1 <= kMaxRow;
+}
+
+DctPrecond::~DctPrecond() {
+  if (blas_) (void)rocblas_destroy_handle(blas_);
+}
+
+static int blas_check(rocblas_status st, const char* what) {
+  if (st == rocblas_status_success) return 0;
+  set_error(std::string("rocBLAS ") + what + " failed: " + rocblas_status_to_string(st));
+  return -1;
+}
+
+int DctPrecond::init(int nx, int ny, double hx, double hy, double eps, hipStream_t s) {
+  if (!supported(nx, ny)) {
+    set_error("internal: DCT preconditioner needs an even ny and nx + 1 <= 3072");
+    return -1;
+  }
+  nx_ = nx;
+  ny_ = ny;
+  h_ = ny / 2;
+  mx_ = nx + 1;
+  hx_ = hx;
+  hy_ = hy;
+  eps_ = eps;
+  const int h = h_, n = ny;
+  std::vector<float> ce((size_t)(h + 1) * (h + 1)), co((size_t)h * h);
+  // column-major: ce[j + m (h + 1)] = cos(pi j 2m / n); co[j + m h] = cos(pi j (2m + 1) / n)
+  for (int m = 0; m <= h; ++m)
+    for (int j = 0; j <= h; ++j)
+      ce[(size_t)j + (size_t)m * (h + 1)] =
+          (float)std::cos(M_PI * (double)((long)j * 2 * m % (2 * n)) / (double)n);
+  for (int m = 0; m < h; ++m)
+    for (int j = 0; j < h; ++j)
+      co[(size_t)j + (size_t)m * h] =
+          (float)std::cos(M_PI * (double)((long)j * (2 * m + 1) % (2 * n)) / (double)n);
+  TMX_TRY(ce_.alloc(ce.size()));
+  TMX_TRY(co_.alloc(co.size()));
+  TMX_TRY(fs_.alloc((size_t)(h + 1) * mx_));
+  TMX_TRY(fd_.alloc((size_t)(h + 1) * mx_));
+  TMX_TRY(ts_.alloc((size_t)(h + 1) * mx_));
+  TMX_TRY(td_.alloc((size_t)(h + 1) * mx_));
+  TMX_HIP(hipMemcpyAsync(ce_.get(), ce.data(), ce.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  TMX_HIP(hipMemcpyAsync(co_.get(), co.data(), co.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  TMX_HIP(hipStreamSynchronize(s));
+  if (!blas_) TMX_TRY(blas_check(rocblas_create_handle(&blas_), "create_handle"));
+  return 0;
+}
+
+int DctPrecond::apply(const float* r, float* z, RedSlot* rz, hipStream_t s) {
+  const int h = h_, n = ny_, mx = mx_;
+  const size_t total = (size_t)(h + 1) * mx;
+  const unsigned g = (unsigned)std::min<size_t>(4096, (total + kBlock - 1) / kBlock);
+  k_dct_fold<<<g, kBlock, 0, s>>>(mx, n, r, fs_.get(), fd_.get());
+  TMX_HIP(hipGetLastError());
+  TMX_TRY(blas_check(rocblas_set_stream(blas_, s), "set_stream"));
+  const float one = 1.0f, zero = 0.0f;
+  // forward: T_e (mx x h+1) = S (mx x h+1) Ce; T_o (mx x h) = D (mx x h) Co
+  TMX_TRY(blas_check(rocblas_sgemm(blas_, rocblas_operation_none, rocblas_operation_none, mx, h + 1,
+                                   h + 1, &one, fs_.get(), mx, ce_.get(), h + 1, &zero, ts_.get(),
+                                   mx), "sgemm"));
+  TMX_TRY(blas_check(rocblas_sgemm(blas_, rocblas_operation_none, rocblas_operation_none, mx, h, h,
+                                   &one, fd_.get(), mx, co_.get(), h, &zero, td_.get(), mx),
+                     "sgemm"));
+  // tridiagonal solves along x, one workgroup per mode
+  const float a_off = (float)(eps_ * eps_ * hy_ / hx_);
+  const double cy = eps_ * eps_ * hx_ / hy_;
+  k_dct_pcr<<<h + 1, kPcrThreads, 0, s>>>(mx, n, 0, a_off, hx_ * hy_, cy, ts_.get());
+  TMX_HIP(hipGetLastError());
+  k_dct_pcr<<<h, kPcrThreads, 0, s>>>(mx, n, 1, a_off, hx_ * hy_, cy, td_.get());
+  TMX_HIP(hipGetLastError());
+  // inverse: E (mx x h+1) = T_e Ce^T; O (mx x h) = T_o Co^T
+  TMX_TRY(blas_check(rocblas_sgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, mx,
+                                   h + 1, h + 1, &one, ts_.get(), mx, ce_.get(), h + 1, &zero,
+                                   fs_.get(), mx), "sgemm"));
+  TMX_TRY(blas_check(rocblas_sgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, mx,
+                                   h, h, &one, td_.get(), mx, co_.get(), h, &zero, fd_.get(), mx),
+                     "sgemm"));
+  const unsigned gu = (unsigned)std::min<size_t>(2048, (total + kBlock - 1) / kBlock);
+  double* part = nullptr;
+  if (rz) {
+    rz->nblocks = (int)gu;
+    part = rz->partials;
+  }
+  k_dct_unfold<<<gu, kBlock, 0, s>>>(mx, n, fs_.get(), fd_.get(), r, z, part);
+  TMX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace tmx

@@ -164,6 +164,8 @@ int minres_core(Workspace& ws, size_t n, const double* b, double* x, MinresBufs 
                 hipStream_t s, bool trace);
 
 // Geometric multigrid + PCG for one operator family on one grid.
+class DctPrecond;
+
 class Hierarchy {
  public:
   int init(Phys phys, int nx, int ny, double hx, double hy, unsigned fixed, double lam,
@@ -270,6 +272,10 @@ class Hierarchy {
   // filter_pipe_down / _up) with node-row chunks of these heights ([FP32, FP64][down, up]);
   // 0: band kernels
   int fpipe_rows_[2][2] = {{12, 12}, {12, 12}};
+  // filter, FP32 preconditioner: the exact DCT-I / cyclic-reduction solver of the
+  // tensor-lumped operator (tmx_dct.h) instead of the V-cycle, where the grid allows it
+  bool use_dct_ = true;
+  std::unique_ptr<DctPrecond> dct_;
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
   bool pair_post_ok() const;  // the finest level has a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel

@@ -420,7 +420,7 @@ class ElasticityProblem:
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
         cheb_ratio, band, band_fine_filter, fine_cell_weights, graphs, sweep_pairs,
-        sweep_pairs_post, sweep_pair_restrict, filter_pipe."""
+        sweep_pairs_post, sweep_pair_restrict, filter_pipe, filter_dct."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -439,7 +439,8 @@ class ElasticityProblem:
                "sweep_pairs": _lib.TMX_OPT_SWEEP_PAIRS,
                "sweep_pairs_post": _lib.TMX_OPT_SWEEP_PAIRS_POST,
                "sweep_pair_restrict": _lib.TMX_OPT_SWEEP_PAIR_RESTRICT,
-               "filter_pipe": _lib.TMX_OPT_FILTER_PIPE}[name]
+               "filter_pipe": _lib.TMX_OPT_FILTER_PIPE,
+               "filter_dct": _lib.TMX_OPT_FILTER_DCT}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
