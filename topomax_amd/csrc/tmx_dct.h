@@ -32,7 +32,7 @@ class DctPrecond {
   int apply(const float* r, float* z, RedSlot* rz, hipStream_t s);
   ~DctPrecond();
 
-  static constexpr int kMaxRow = 3072;  // one mode row in LDS: 4 arrays of floats (48 KB)
+  static constexpr int kMaxRow = 3072;  // one mode row in LDS: 5 arrays of floats (60 KB)
 
  private:
   int nx_ = 0, ny_ = 0, h_ = 0, mx_ = 0;

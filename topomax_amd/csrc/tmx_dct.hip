@@ -64,27 +64,33 @@ __global__ __launch_bounds__(kBlock) void k_dct_unfold(int mx, int n, const floa
   }
 }
 
-// One workgroup per mode row (mode = 2 * blockIdx.x + parity): solves
-// (a T_x + c_m W_x) u = f / D_m in place by parallel cyclic reduction in LDS.
+// One workgroup per mode row: rows 0..h of `even` are the modes 0, 2, .., n, rows 0..h-1 of
+// `odd` the modes 1, 3, .., n-1 (blocks h+1 ..).  Each solves (a T_x + c_m W_x) u = f / D_m in
+// place by parallel cyclic reduction in LDS, keeping 1/b beside b so that each equation costs
+// one division per step (five arrays of floats: 60 KB of LDS for the longest row).
 constexpr int kPcrThreads = 1024;
 constexpr int kPcrPer = DctPrecond::kMaxRow / kPcrThreads;  // equations per thread
 
-__global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, int parity, float a_off,
-                                                         double hxhy, double cy,
-                                                         float* __restrict__ rows) {
-  __shared__ float la[DctPrecond::kMaxRow], lb[DctPrecond::kMaxRow], lc[DctPrecond::kMaxRow],
-      lf[DctPrecond::kMaxRow];
-  const int mode = 2 * (int)blockIdx.x + parity;
+__global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, float a_off, double hxhy,
+                                                         double cy, float* __restrict__ even,
+                                                         float* __restrict__ odd) {
+  __shared__ float la[DctPrecond::kMaxRow], lb[DctPrecond::kMaxRow], lr[DctPrecond::kMaxRow],
+      lc[DctPrecond::kMaxRow], lf[DctPrecond::kMaxRow];  // sub, diagonal, 1 / diagonal, super, rhs
+  const int h = n / 2;
+  const int blk = (int)blockIdx.x;
+  const int mode = blk <= h ? 2 * blk : 2 * (blk - h - 1) + 1;
+  float* row = blk <= h ? even + (size_t)blk * mx : odd + (size_t)(blk - h - 1) * mx;
   const double lam = 2.0 - 2.0 * std::cos(M_PI * (double)mode / (double)n);
   const float c = (float)(hxhy + cy * lam);
   const float inv_d = (mode == 0 || mode == n) ? 1.0f / (float)n : 2.0f / (float)n;
-  float* row = rows + (size_t)blockIdx.x * mx;
   const int tid = threadIdx.x;
   for (int i = tid; i < mx; i += kPcrThreads) {
     const bool edge = i == 0 || i == mx - 1;
     la[i] = i > 0 ? -a_off : 0.0f;
     lc[i] = i < mx - 1 ? -a_off : 0.0f;
-    lb[i] = a_off * (edge ? 1.0f : 2.0f) + c * (edge ? 0.5f : 1.0f);
+    const float bi = a_off * (edge ? 1.0f : 2.0f) + c * (edge ? 0.5f : 1.0f);
+    lb[i] = bi;
+    lr[i] = 1.0f / bi;
     lf[i] = row[i] * inv_d;
   }
   __syncthreads();
@@ -95,16 +101,16 @@ __global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, int pari
       const int i = tid + q * kPcrThreads;
       if (i >= mx) continue;
       const int im = i - s, ip = i + s;
-      float k1 = 0.0f, k2 = 0.0f, ai = 0.0f, ci = 0.0f;
+      float ai = 0.0f, ci = 0.0f;
       float bi = lb[i], fi = lf[i];
       if (im >= 0) {
-        k1 = la[i] / lb[im];
+        const float k1 = la[i] * lr[im];
         ai = -la[im] * k1;
         bi -= lc[im] * k1;
         fi -= lf[im] * k1;
       }
       if (ip < mx) {
-        k2 = lc[i] / lb[ip];
+        const float k2 = lc[i] * lr[ip];
         ci = -lc[ip] * k2;
         bi -= la[ip] * k2;
         fi -= lf[ip] * k2;
@@ -121,12 +127,13 @@ __global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, int pari
       if (i >= mx) continue;
       la[i] = na[q];
       lb[i] = nb[q];
+      lr[i] = 1.0f / nb[q];
       lc[i] = nc[q];
       lf[i] = nf[q];
     }
     __syncthreads();
   }
-  for (int i = tid; i < mx; i += kPcrThreads) row[i] = lf[i] / lb[i];
+  for (int i = tid; i < mx; i += kPcrThreads) row[i] = lf[i] * lr[i];
 }
 
 bool DctPrecond::supported(int nx, int ny) {
@@ -197,9 +204,7 @@ int DctPrecond::apply(const float* r, float* z, RedSlot* rz, hipStream_t s) {
   // tridiagonal solves along x, one workgroup per mode
   const float a_off = (float)(eps_ * eps_ * hy_ / hx_);
   const double cy = eps_ * eps_ * hx_ / hy_;
-  k_dct_pcr<<<h + 1, kPcrThreads, 0, s>>>(mx, n, 0, a_off, hx_ * hy_, cy, ts_.get());
-  TMX_HIP(hipGetLastError());
-  k_dct_pcr<<<h, kPcrThreads, 0, s>>>(mx, n, 1, a_off, hx_ * hy_, cy, td_.get());
+  k_dct_pcr<<<2 * h + 1, kPcrThreads, 0, s>>>(mx, n, a_off, hx_ * hy_, cy, ts_.get(), td_.get());
   TMX_HIP(hipGetLastError());
   // inverse: E (mx x h+1) = T_e Ce^T; O (mx x h) = T_o Co^T
   TMX_TRY(blas_check(rocblas_sgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, mx,
