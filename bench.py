@@ -204,21 +204,13 @@ def main():
         t64, J64, w64, st64 = time_steps(problem, rho, max(2, args.steps // 2), 1, "fp64",
                                          args.rtol)
         dw = float(torch.linalg.norm(w_mixed - w64) / torch.linalg.norm(w64))
-        # the mixed step's filter preconditioner (FP32 DCT-I / cyclic reduction) has no FP64
-        # counterpart at this size; the mixed step with the filter's multigrid V-cycle instead
-        # isolates the precision from that algorithmic difference
-        problem.set_option("filter_dct", 0)
-        tmg, _, _, _ = time_steps(problem, rho, max(2, args.steps // 2), 1, "mixed", args.rtol)
-        problem.set_option("filter_dct", 1)
         result_fp64 = {"ms_per_step": t64 * 1e3, "GDOF_s": ndof / t64 / 1e9,
                        "iterations": st64[-1]["iterations"],
                        "compliance_rel_err_mixed_vs_fp64": abs(J_mixed - J64) / abs(J64),
                        "gradient_rel_err_mixed_vs_fp64": dw,
                        "speedup_mixed_over_fp64": t64 / t_mixed,
-                       "filter_preconditioner": "multigrid V-cycle (FP64)",
-                       "mixed_filter_preconditioner": "DCT-I / cyclic reduction (FP32)",
-                       "mixed_ms_per_step_with_multigrid_filter": tmg * 1e3,
-                       "speedup_same_filter_preconditioner": t64 / tmg}
+                       "filter_preconditioner": "DCT-I / cyclic reduction (FP64)",
+                       "mixed_filter_preconditioner": "DCT-I / cyclic reduction (FP32)"}
         tir, Jir, wir, stir = time_steps(problem, rho, max(2, args.steps // 2), 1, "mixed-ir",
                                          args.rtol)
         result_ir = {"ms_per_step": tir * 1e3, "GDOF_s": ndof / tir / 1e9,

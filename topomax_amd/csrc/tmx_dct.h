@@ -1,5 +1,5 @@
 // Exact fast solver of the filter's spectrally equivalent operator, used as its PCG
-// preconditioner (FP32).
+// preconditioner (FP32 in the mixed scheme, FP64 in the FP64 scheme).
 //
 // With natural boundary conditions on the structured 'right' mesh the filter stiffness is a
 // sum of Kronecker products of 1D Neumann Laplacians T (tridiag(-1, 2, -1), boundary diagonal
@@ -23,23 +23,25 @@
 
 namespace tmx {
 
+template <typename T>
 class DctPrecond {
  public:
   // nx, ny: cells; ny even, nx + 1 <= kMaxRow; eps: the filter's length scale
   static bool supported(int nx, int ny);
   int init(int nx, int ny, double hx, double hy, double eps, hipStream_t s);
-  // z = A'^-1 r (FP32 vectors of (nx + 1)(ny + 1) values), partials of r.z into rz if given
-  int apply(const float* r, float* z, RedSlot* rz, hipStream_t s);
+  // z = A'^-1 r (vectors of (nx + 1)(ny + 1) values), partials of r.z into rz if given
+  int apply(const T* r, T* z, RedSlot* rz, hipStream_t s);
   ~DctPrecond();
 
-  static constexpr int kMaxRow = 3072;  // one mode row in LDS: 5 arrays of floats (60 KB)
+  // one mode row in LDS: five arrays of T (60 KB in FP32, 120 KB in FP64; 160 KB per CU)
+  static constexpr int kMaxRow = 3072;
 
  private:
   int nx_ = 0, ny_ = 0, h_ = 0, mx_ = 0;
   double hx_ = 0, hy_ = 0, eps_ = 0;
-  DevBuf<float> ce_, co_;  // folded DCT-I matrices, column-major (h+1)^2 and h^2
-  DevBuf<float> fs_, fd_;  // folded sums / differences, then the inverse transforms
-  DevBuf<float> ts_, td_;  // transformed even / odd modes
+  DevBuf<T> ce_, co_;  // folded DCT-I matrices, column-major (h+1)^2 and h^2
+  DevBuf<T> fs_, fd_;  // folded sums / differences, then the inverse transforms
+  DevBuf<T> ts_, td_;  // transformed even / odd modes
   rocblas_handle blas_ = nullptr;
 };
 

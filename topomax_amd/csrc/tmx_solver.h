@@ -164,6 +164,7 @@ int minres_core(Workspace& ws, size_t n, const double* b, double* x, MinresBufs 
                 hipStream_t s, bool trace);
 
 // Geometric multigrid + PCG for one operator family on one grid.
+template <typename T>
 class DctPrecond;
 
 class Hierarchy {
@@ -272,10 +273,16 @@ class Hierarchy {
   // filter_pipe_down / _up) with node-row chunks of these heights ([FP32, FP64][down, up]);
   // 0: band kernels
   int fpipe_rows_[2][2] = {{12, 12}, {12, 12}};
-  // filter, FP32 preconditioner: the exact DCT-I / cyclic-reduction solver of the
-  // tensor-lumped operator (tmx_dct.h) instead of the V-cycle, where the grid allows it
+  // filter preconditioner: the exact DCT-I / cyclic-reduction solver of the tensor-lumped
+  // operator (tmx_dct.h) instead of the V-cycle, where the grid allows it
   bool use_dct_ = true;
-  std::unique_ptr<DctPrecond> dct_;
+  std::unique_ptr<DctPrecond<float>> dct32_;
+  std::unique_ptr<DctPrecond<double>> dct64_;
+  template <typename T>
+  std::unique_ptr<DctPrecond<T>>& dct_slot() {
+    if constexpr (sizeof(T) == 8) return dct64_;
+    else return dct32_;
+  }
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
   bool pair_post_ok() const;  // the finest level has a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel

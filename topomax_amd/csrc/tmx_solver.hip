@@ -605,16 +605,15 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
     return 0;
   }
   if (use_tail_ && l == tail_ && l > 0) return tail_launch<T>(*this, b, out, s);
-  if constexpr (std::is_same<T, float>::value) {
-    if (l == 0 && phys_ == kFilter && use_dct_ && !use_graphs_ &&
-        DctPrecond::supported(levels_[0].nx, levels_[0].ny)) {
-      if (!dct_) {
-        std::unique_ptr<DctPrecond> d(new DctPrecond);
-        TMX_TRY(d->init(levels_[0].nx, levels_[0].ny, levels_[0].hx, levels_[0].hy, eps_, s));
-        dct_ = std::move(d);
-      }
-      return dct_->apply(b, out, dot_bz, s);
+  if (l == 0 && phys_ == kFilter && use_dct_ && !use_graphs_ &&
+      DctPrecond<T>::supported(levels_[0].nx, levels_[0].ny)) {
+    auto& dct = dct_slot<T>();
+    if (!dct) {
+      std::unique_ptr<DctPrecond<T>> d(new DctPrecond<T>);
+      TMX_TRY(d->init(levels_[0].nx, levels_[0].ny, levels_[0].hx, levels_[0].hy, eps_, s));
+      dct = std::move(d);
     }
+    return dct->apply(b, out, dot_bz, s);
   }
   Level& L = levels_[l];
   Level& C = levels_[l + 1];
