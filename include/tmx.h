@@ -210,10 +210,17 @@ enum tmx_option {
                                    node-row chunks of this height (default 12; tmx_autotune
                                    re-picks it per direction and precision); 0: the band
                                    kernels */
-  TMX_OPT_FILTER_DCT = 21       /* 1 (default): the filter's FP32 preconditioner is the exact
-                                   DCT-I / cyclic-reduction solver of eps^2 K + (lumped mass)
-                                   when ny is even and nx + 1 <= 3072 (3-5 PCG iterations);
-                                   0: the multigrid V-cycle */
+  TMX_OPT_FILTER_DCT = 21,      /* 1 (default): the filter's preconditioner (FP32 and FP64) is
+                                   the exact DCT-I / cyclic-reduction solver of
+                                   eps^2 K + (lumped mass) when ny is even and nx + 1 <= 3072
+                                   (3-5 PCG iterations); 0: the multigrid V-cycle */
+  TMX_OPT_PAIR_LEVELS = 22,     /* elasticity element levels 0 .. value-1 run the paired
+                                   column-strip kernels, the levels below them the band kernels
+                                   (default 1: the finest only); 1..8 */
+  TMX_OPT_PAIR_ROWS_COARSE = 23, /* node-row chunk height of the paired kernels on the coarser
+                                   paired levels (0: one launch per sweep there) */
+  TMX_OPT_PAIR_NU_COARSE = 24   /* sweeps each way on the coarser paired levels (0: the
+                                   coarse-level count, TMX_OPT_NU_COARSE) */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

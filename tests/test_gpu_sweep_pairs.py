@@ -104,3 +104,40 @@ def test_pairs_solve_matches(precision):
     print(precision, J0, J1, s0, gp.last_stats, flush=True)
     assert abs(J1 - J0) < 1e-9 * abs(J0)
     assert rel(w1, w0) < 1e-8
+
+
+@pytest.mark.parametrize("nx,ny", [(120, 40), (250, 90), (300, 100)])
+@pytest.mark.parametrize("levels,nu", [(2, 0), (2, 2), (3, 2)])
+def test_paired_coarse_levels(nx, ny, levels, nu):
+    """Paired kernels on the coarser element levels (option pair_levels) against one launch
+    per sweep on the same levels (pair_rows_coarse 0): the same sweeps in the same order."""
+    gp = gpu_problem(CANTILEVER, nx, ny)
+    gp.set_penalization(3)
+    xi = torch.as_tensor(smoothed_density(nx, ny, 2, sigma=2.0), device="cuda:0")
+    r = residual_vector(nx, ny, 6)
+    gp.set_option("pair_levels", levels)
+    gp.set_option("pair_nu_coarse", nu)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        gp.set_option("pair_rows_coarse", 0)
+        z_ref = gp.precondition(r, "elasticity", xi, precision=prec)
+        for rows in (1, 4, 8, 64):
+            gp.set_option("pair_rows_coarse", rows)
+            z = gp.precondition(r, "elasticity", xi, precision=prec)
+            print(nx, ny, levels, nu, prec, rows, rel(z, z_ref), flush=True)
+            assert torch.isfinite(z).all()
+            assert rel(z, z_ref) < tol
+
+
+def test_paired_coarse_levels_solve():
+    """The solve with paired level-1 kernels reaches the band kernels' objective and
+    gradient (both to rtol 1e-10)."""
+    nx, ny = 300, 100
+    gp = gpu_problem(CANTILEVER, nx, ny, rtol=1e-10)
+    gp.set_penalization(3)
+    rho = torch.as_tensor(smoothed_density(nx, ny, 4), device="cuda:0")
+    J0, w0, _ = gp.objective_and_gradient(rho)
+    gp.set_option("pair_levels", 2)
+    J1, w1, _ = gp.objective_and_gradient(rho)
+    print(J0, J1, flush=True)
+    assert abs(J1 - J0) < 1e-9 * abs(J0)
+    assert rel(w1, w0) < 1e-8

@@ -649,6 +649,24 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       h->fl.drop_graphs();
       h->fl.use_dct_ = value != 0.0;
       break;
+    case TMX_OPT_PAIR_LEVELS:
+      if (value < 1 || value > 8 || value != std::floor(value))
+        return invalid("paired levels must be 1..8");
+      h->el.drop_graphs();
+      h->el.pair_levels_ = (int)value;
+      break;
+    case TMX_OPT_PAIR_ROWS_COARSE:
+      if (value < 0 || value > 256 || value != std::floor(value))
+        return invalid("sweep-pair rows must be 0..256");
+      h->el.drop_graphs();
+      for (int& r : h->el.pair_rows_coarse_) r = (int)value;
+      break;
+    case TMX_OPT_PAIR_NU_COARSE:
+      if (value < 0 || value > 16 || value != std::floor(value))
+        return invalid("sweeps must be 0..16");
+      h->el.drop_graphs();
+      h->el.pair_nu_coarse_ = (int)value;
+      break;
     case TMX_OPT_FILTER_PIPE:
       if (value < 0 || value > 256 || value != std::floor(value))
         return invalid("filter pipeline rows must be 0..256");

@@ -186,18 +186,20 @@ class Hierarchy {
   int residual(const T* x, const T* b, T* r, hipStream_t s);
   template <typename T>
   int smooth0(const T* x, const T* b, T* y, hipStream_t s);
-  // two finest-level sweeps in one launch (elasticity): op1 = kSmoothZero (x unused) or
-  // kProlongSmooth (x + P e with e = level 1's x), chunk height rows
+  // two sweeps of elasticity element level l (default the finest) in one launch: op1 =
+  // kSmoothZero (x unused) or kProlongSmooth (x + P e with e = level l + 1's x), chunk height
+  // rows
   template <typename T>
   int sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, double om1, double om2,
-                  RedSlot* dot, hipStream_t s);
+                  RedSlot* dot, hipStream_t s, int l = 0);
   // one launch of the filter's finest-level pipeline (down or up) on the hierarchy's own
   // level buffers, for timing (tmx_autotune)
   template <typename T>
   int filter_pipe0(bool down, int rows, hipStream_t s);
-  // the zero-start pair with residual + restriction to level 1's right-hand side fused
+  // the zero-start pair with residual + restriction to level l + 1's right-hand side fused
   template <typename T>
-  int sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s);
+  int sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s,
+                     int l = 0);
   // z = M^-1 b; at level 0 the last smoothing sweep also writes partials of b.z to dot_bz
   template <typename T>
   int vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz = nullptr);
@@ -269,6 +271,12 @@ class Hierarchy {
   // > 0: with two pre-sweeps, the zero-start pair also forms the residual and restricts it
   // (chunks of this many rows); 0: a separate residual + restriction launch
   int pair_rr_rows_[2] = {12, 12};
+  // elasticity element levels 0 .. pair_levels_ - 1 use the paired column-strip kernels (the
+  // coarser ones with chunk heights pair_rows_coarse_ [pre, post, fused restriction] and
+  // pair_nu_coarse_ sweeps each way, 0: nu_coarse_); the levels below them the band kernels
+  int pair_levels_ = 1;
+  int pair_rows_coarse_[3] = {8, 8, 8};
+  int pair_nu_coarse_ = 0;
   // > 0: the filter's finest level runs as two column pipelines (tmx_kernels.h
   // filter_pipe_down / _up) with node-row chunks of these heights ([FP32, FP64][down, up]);
   // 0: band kernels
@@ -284,7 +292,7 @@ class Hierarchy {
     else return dct32_;
   }
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
-  bool pair_post_ok() const;  // the finest level has a nested level below it
+  bool pair_post_ok(int l = 0) const;  // level l has even sides and a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
   void find_tail();
   int tail_ = -1;               // first level handled by the one-workgroup tail kernel

@@ -315,9 +315,9 @@ int Hierarchy::fine_cell_weights(hipStream_t s) {
   return 0;
 }
 
-bool Hierarchy::pair_post_ok() const {
-  return levels_.size() >= 2 && levels_[1].nested_from_prev && !(levels_[0].nx & 1) &&
-         !(levels_[0].ny & 1);
+bool Hierarchy::pair_post_ok(int l) const {
+  return l >= 0 && (int)levels_.size() >= l + 2 && levels_[l + 1].nested_from_prev &&
+         !(levels_[l].nx & 1) && !(levels_[l].ny & 1);
 }
 
 int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
@@ -428,20 +428,21 @@ int Hierarchy::smooth0(const T* x, const T* b, T* y, hipStream_t s) {
 
 template <typename T>
 int Hierarchy::sweep_pair0(int op1, int rows, const T* x, const T* b, T* y, double om1, double om2,
-                           RedSlot* dot, hipStream_t s) {
-  if (phys_ != kElasticity || levels_[0].kind != kElem || rows < 1) {
-    set_error("internal: paired sweeps need the finest elasticity level");
+                           RedSlot* dot, hipStream_t s, int l) {
+  if (phys_ != kElasticity || l < 0 || l >= (int)levels_.size() || levels_[l].kind != kElem ||
+      rows < 1) {
+    set_error("internal: paired sweeps need an elasticity element level");
     return -1;
   }
   const T* ec = nullptr;
   if (op1 == kProlongSmooth) {
-    if (!pair_post_ok()) {
-      set_error("internal: paired prolongation needs a nested level below the finest");
+    if (!pair_post_ok(l)) {
+      set_error("internal: paired prolongation needs a nested level below");
       return -1;
     }
-    ec = PrecBufs<T>::x(levels_[1]);
+    ec = PrecBufs<T>::x(levels_[l + 1]);
   }
-  const ElastOpArgs<T> a = elast_args<T>(*this, 0, xi_for<T>(*this), PrecBufs<T>::cw(levels_[0]),
+  const ElastOpArgs<T> a = elast_args<T>(*this, l, xi_for<T>(*this), PrecBufs<T>::cw(levels_[l]),
                                          sizeof(T) == 8, true);
   TMX_HIP(elast_sweep_pair<T>(a, op1, rows, x, b, y, (T)om1, (T)om2, dot, s, ec));
   return 0;
@@ -473,14 +474,16 @@ int Hierarchy::filter_pipe0(bool down, int rows, hipStream_t s) {
 }
 
 template <typename T>
-int Hierarchy::sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s) {
-  if (phys_ != kElasticity || !pair_post_ok() || rows < 1) {
-    set_error("internal: fused restriction needs the finest elasticity level over a nested one");
+int Hierarchy::sweep_pair_rr0(int rows, const T* b, T* y, double om1, double om2, hipStream_t s,
+                              int l) {
+  if (phys_ != kElasticity || !pair_post_ok(l) || levels_[l].kind != kElem || rows < 1) {
+    set_error("internal: fused restriction needs an elasticity element level over a nested one");
     return -1;
   }
-  const ElastOpArgs<T> a = elast_args<T>(*this, 0, xi_for<T>(*this), PrecBufs<T>::cw(levels_[0]),
+  const ElastOpArgs<T> a = elast_args<T>(*this, l, xi_for<T>(*this), PrecBufs<T>::cw(levels_[l]),
                                          sizeof(T) == 8, true);
-  TMX_HIP(elast_sweep_pair_rr<T>(a, rows, b, y, PrecBufs<T>::b(levels_[1]), (T)om1, (T)om2, s));
+  TMX_HIP(elast_sweep_pair_rr<T>(a, rows, b, y, PrecBufs<T>::b(levels_[l + 1]), (T)om1, (T)om2,
+                                 s));
   return 0;
 }
 
@@ -634,8 +637,10 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
                               dot_bz, s));
     return 0;
   }
+  // elasticity element levels above pair_levels_ run the paired column-strip kernels
+  const bool pair_level = phys_ == kElasticity && L.kind == kElem && l < pair_levels_;
   BandLevel<T> B;
-  if (band_level<T>(*this, l, B)) {
+  if (!pair_level && band_level<T>(*this, l, B)) {
     T* t = PrecBufs<T>::t(L);
     T* cb = PrecBufs<T>::b(C);
     T* cx = PrecBufs<T>::x(C);
@@ -648,28 +653,31 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
     return 0;
   }
   T* buf[2] = {out, PrecBufs<T>::t(L)};
-  const int npre = l == 0 ? nu_pre_ : nu_coarse_;
-  const int npost = l == 0 ? nu_post_ : nu_coarse_;
+  const int nuc = pair_level && pair_nu_coarse_ > 0 ? pair_nu_coarse_ : nu_coarse_;
+  const int npre = l == 0 ? nu_pre_ : nuc;
+  const int npost = l == 0 ? nu_post_ : nuc;
   T* cb = PrecBufs<T>::b(C);
   T* cx = PrecBufs<T>::x(C);
   const bool fused_el = fused_ && L.kind == kElem && C.nested_from_prev;  // nested transfer
-  // the finest elasticity level runs its first two pre-sweeps, and prolongation with its first
+  // paired elasticity levels run their first two pre-sweeps, and prolongation with their first
   // two post-sweeps, as one launch each
-  const bool pairs = l == 0 && phys_ == kElasticity && L.kind == kElem;
   const int pi = sizeof(T) == 8 ? 1 : 0;
-  const bool pair_pre = pairs && pair_rows_[pi] > 0 && npre >= 2;
-  const bool pair_post = pairs && pair_rows_post_[pi] > 0 && fused_el && npost >= 2;
-  const bool pair_rr = pair_pre && npre == 2 && fused_el && pair_rr_rows_[pi] > 0;
+  const int rows_pre = l == 0 ? pair_rows_[pi] : pair_rows_coarse_[0];
+  const int rows_post = l == 0 ? pair_rows_post_[pi] : pair_rows_coarse_[1];
+  const int rows_rr = l == 0 ? pair_rr_rows_[pi] : pair_rows_coarse_[2];
+  const bool pair_pre = pair_level && rows_pre > 0 && npre >= 2;
+  const bool pair_post = pair_level && rows_post > 0 && fused_el && npost >= 2;
+  const bool pair_rr = pair_pre && npre == 2 && fused_el && rows_rr > 0;
   const int launches = npre - (pair_pre ? 1 : 0) + npost - (pair_post ? 1 : 0);
   int w = (launches - 1) % 2;
   int kpre = 1;
   if (pair_rr) {  // writes the pre-smoothed iterate and the coarse right-hand side cb
-    TMX_TRY(sweep_pair_rr0<T>(pair_rr_rows_[pi], b, buf[w], sweep_omega(0, npre),
-                              sweep_omega(1, npre), s));
+    TMX_TRY(sweep_pair_rr0<T>(rows_rr, b, buf[w], sweep_omega(0, npre), sweep_omega(1, npre), s,
+                              l));
     kpre = 2;
   } else if (pair_pre) {
-    TMX_TRY(sweep_pair0<T>(kSmoothZero, pair_rows_[pi], nullptr, b, buf[w], sweep_omega(0, npre),
-                           sweep_omega(1, npre), nullptr, s));
+    TMX_TRY(sweep_pair0<T>(kSmoothZero, rows_pre, nullptr, b, buf[w], sweep_omega(0, npre),
+                           sweep_omega(1, npre), nullptr, s, l));
     kpre = 2;
   } else {
     TMX_TRY(level_op<T>(*this, l, kSmoothZero, nullptr, b, buf[w], nullptr, s, SweepExtra<T>(),
@@ -703,9 +711,9 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
   int post = npost, j = 0;  // j: index of the next post-sweep in the smoothing pass
   RedSlot* last_dot = l == 0 ? dot_bz : nullptr;
   if (pair_post) {
-    TMX_TRY(sweep_pair0<T>(kProlongSmooth, pair_rows_post_[pi], buf[w], b, buf[w ^ 1],
+    TMX_TRY(sweep_pair0<T>(kProlongSmooth, rows_post, buf[w], b, buf[w ^ 1],
                            sweep_omega(0, npost), sweep_omega(1, npost),
-                           post == 2 ? last_dot : nullptr, s));
+                           post == 2 ? last_dot : nullptr, s, l));
     j = 2;
     w ^= 1;
     post -= 2;
@@ -1269,15 +1277,15 @@ template int Hierarchy::residual<double>(const double*, const double*, double*, 
 template int Hierarchy::smooth0<double>(const double*, const double*, double*, hipStream_t);
 template int Hierarchy::smooth0<float>(const float*, const float*, float*, hipStream_t);
 template int Hierarchy::sweep_pair0<double>(int, int, const double*, const double*, double*,
-                                            double, double, RedSlot*, hipStream_t);
+                                            double, double, RedSlot*, hipStream_t, int);
 template int Hierarchy::sweep_pair0<float>(int, int, const float*, const float*, float*, double,
-                                           double, RedSlot*, hipStream_t);
+                                           double, RedSlot*, hipStream_t, int);
 template int Hierarchy::filter_pipe0<double>(bool, int, hipStream_t);
 template int Hierarchy::filter_pipe0<float>(bool, int, hipStream_t);
 template int Hierarchy::sweep_pair_rr0<double>(int, const double*, double*, double, double,
-                                               hipStream_t);
+                                               hipStream_t, int);
 template int Hierarchy::sweep_pair_rr0<float>(int, const float*, float*, double, double,
-                                              hipStream_t);
+                                              hipStream_t, int);
 template int Hierarchy::vcycle<double>(int, const double*, double*, hipStream_t, RedSlot*);
 template int Hierarchy::vcycle<float>(int, const float*, float*, hipStream_t, RedSlot*);
 

@@ -440,7 +440,10 @@ class ElasticityProblem:
                "sweep_pairs_post": _lib.TMX_OPT_SWEEP_PAIRS_POST,
                "sweep_pair_restrict": _lib.TMX_OPT_SWEEP_PAIR_RESTRICT,
                "filter_pipe": _lib.TMX_OPT_FILTER_PIPE,
-               "filter_dct": _lib.TMX_OPT_FILTER_DCT}[name]
+               "filter_dct": _lib.TMX_OPT_FILTER_DCT,
+               "pair_levels": _lib.TMX_OPT_PAIR_LEVELS,
+               "pair_rows_coarse": _lib.TMX_OPT_PAIR_ROWS_COARSE,
+               "pair_nu_coarse": _lib.TMX_OPT_PAIR_NU_COARSE}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
