@@ -11,7 +11,8 @@
 // lambda_m = 2 - 2 cos(pi m / n), v_m' W_y v_k = D_m delta_mk (D_m = n / 2, or n for m = 0, n).
 // So A'^-1 r = sum_m v_m (x) (B_x + lambda_m C_x)^-1 (v_m' r) / D_m: a DCT-I along y (folded
 // into even and odd modes, two GEMMs of half size each way), one tridiagonal solve along x per
-// mode (parallel cyclic reduction in LDS), and the inverse DCT-I.  A'^-1 A has its spectrum in
+// mode (partition method: block eliminations, cyclic reduction of the blocks' interface
+// system in LDS), and the inverse DCT-I.  A'^-1 A has its spectrum in
 // [0.95, 1.003] for eps/h = 2 and closer to 1 as eps/h grows (the mass is a small part of the
 // operator there), so PCG needs 3-5 iterations to 1e-9 whatever the grid.
 #pragma once
@@ -33,15 +34,16 @@ class DctPrecond {
   int apply(const T* r, T* z, RedSlot* rz, hipStream_t s);
   ~DctPrecond();
 
-  // one mode row in LDS: five arrays of T (60 KB in FP32, 120 KB in FP64; 160 KB per CU)
+  // longest x row: one workgroup of 256 threads solves it, 12 rows per thread, with the row
+  // and two coefficient arrays in LDS (36 KB in FP32, 72 KB in FP64)
   static constexpr int kMaxRow = 3072;
 
  private:
   int nx_ = 0, ny_ = 0, h_ = 0, mx_ = 0;
   double hx_ = 0, hy_ = 0, eps_ = 0;
-  DevBuf<T> ce_, co_;  // folded DCT-I matrices, column-major (h+1)^2 and h^2
-  DevBuf<T> fs_, fd_;  // folded sums / differences, then the inverse transforms
-  DevBuf<T> ts_, td_;  // transformed even / odd modes
+  DevBuf<T> c_;  // folded DCT-I matrices, even then odd, column-major (h+1)^2 each
+  DevBuf<T> f_;  // folded sums then differences (mx x h+1 each), then the inverse transforms
+  DevBuf<T> t_;  // transformed even then odd modes (mx x h+1 each)
   rocblas_handle blas_ = nullptr;
 };
 

@@ -1,4 +1,4 @@
-// DCT-I / cyclic-reduction solver of the filter's tensor-lumped operator (see tmx_dct.h).
+// DCT-I / tridiagonal solver of the filter's tensor-lumped operator (see tmx_dct.h).
 //
 // Data layout: vectors are node-major rows, v = iy (nx + 1) + ix, so a y-column is strided and a
 // mode row (fixed y-mode, all x) is contiguous.  Seen column-major, a block of rows is an
@@ -66,22 +66,32 @@ __global__ __launch_bounds__(kBlock) void k_dct_unfold(int mx, int n, const T* _
   }
 }
 
-// One workgroup per mode row: rows 0..h of `even` are the modes 0, 2, .., n, rows 0..h-1 of
-// `odd` the modes 1, 3, .., n-1 (blocks h+1 ..).  Each solves (a T_x + c_m W_x) u = f / D_m in
-// place by parallel cyclic reduction in LDS, keeping 1/b beside b so that each equation costs
-// one division per step (five arrays of floats: 60 KB of LDS for the longest row).
-constexpr int kPcrThreads = 1024;
+// One tridiagonal solve along x per mode row: rows 0..h of `even` are the modes 0, 2, .., n,
+// rows 0..h-1 of `odd` the modes 1, 3, .., n-1 (workgroups h+1 ..).  Each solves
+// (a T_x + c_m W_x) u = f / D_m in place, one workgroup of kTriThreads per mode, by the
+// partition method: thread t owns the k = ceil(mx / kTriThreads) consecutive rows s..e of one
+// block.  A downward elimination writes each of its rows as
+//   x_i + al_i x_{s-1} + ga_i x_{i+1} = ph_i,
+// an upward one as x_i = P_i - A_i x_{s-1} - G_i x_e (P_e = A_e = 0, G_e = -1).  Substituting
+// the next block's first row into this block's last one leaves a tridiagonal system in the
+// blocks' last unknowns y_t = x_{e_t}:
+//   al_e y_{t-1} + (1 - ga_e A'_s) y_t - ga_e G'_s y_{t+1} = ph_e - ga_e P'_s   (' : block t+1),
+// solved by parallel cyclic reduction in LDS (log2(kTriThreads) steps, one equation per
+// thread); then x_i = P_i - A_i y_{t-1} - G_i y_t.  The operator is diagonally dominant, so the
+// eliminations need no pivoting.  ph / P stay in the staged row; al / A and ga / G sit in LDS
+// transposed (index j * kTriThreads + t), so a wave's accesses fall in distinct banks.
+constexpr int kTriThreads = 256;
 
 template <typename T>
-constexpr int kDctMaxRow = DctPrecond<T>::kMaxRow;
-
-template <typename T>
-__global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, T a_off, double hxhy,
+__global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off, double hxhy,
                                                          double cy, T* __restrict__ even,
                                                          T* __restrict__ odd) {
-  constexpr int kRow = kDctMaxRow<T>;
-  constexpr int kPcrPer = kRow / kPcrThreads;  // equations per thread
-  __shared__ T la[kRow], lb[kRow], lr[kRow], lc[kRow], lf[kRow];  // sub, diag, 1/diag, super, rhs
+  constexpr int kRow = DctPrecond<T>::kMaxRow;
+  static_assert(kRow % kTriThreads == 0, "row capacity");
+  __shared__ T lrow[kRow];               // right-hand side, then ph, P and x
+  __shared__ T lal[kRow], lga[kRow];     // al then A, ga then G (transposed)
+  __shared__ T ra[kTriThreads], rb[kTriThreads], rc[kTriThreads], rf[kTriThreads];
+  __shared__ T sp[kTriThreads], sa[kTriThreads], sg[kTriThreads];  // P, A, G of first rows
   const int h = n / 2;
   const int blk = (int)blockIdx.x;
   const int mode = blk <= h ? 2 * blk : 2 * (blk - h - 1) + 1;
@@ -89,57 +99,101 @@ __global__ __launch_bounds__(kPcrThreads) void k_dct_pcr(int mx, int n, T a_off,
   const double lam = 2.0 - 2.0 * std::cos(M_PI * (double)mode / (double)n);
   const T c = (T)(hxhy + cy * lam);
   const T inv_d = (mode == 0 || mode == n) ? T(1) / (T)n : T(2) / (T)n;
+  const T b_in = a_off * T(2) + c, b_edge = a_off + c * T(0.5);
   const int tid = threadIdx.x;
-  for (int i = tid; i < mx; i += kPcrThreads) {
-    const bool edge = i == 0 || i == mx - 1;
-    la[i] = i > 0 ? -a_off : T(0);
-    lc[i] = i < mx - 1 ? -a_off : T(0);
-    const T bi = a_off * (edge ? T(1) : T(2)) + c * (edge ? T(0.5) : T(1));
-    lb[i] = bi;
-    lr[i] = T(1) / bi;
-    lf[i] = row[i] * inv_d;
+  for (int i = tid; i < mx; i += kTriThreads) lrow[i] = row[i] * inv_d;
+  const int k = (mx + kTriThreads - 1) / kTriThreads;
+  const int nblk = (mx + k - 1) / k;
+  const int s0 = tid * k;
+  const int jl = tid < nblk ? min(s0 + k, mx) - 1 - s0 : -1;  // last row of the block, local
+  __syncthreads();
+  // downward over rows s0 .. s0 + jl
+  T al = T(0), ga = T(0), ph = T(0);
+  for (int j = 0; j <= jl; ++j) {
+    const int i = s0 + j;
+    const T ai = i > 0 ? -a_off : T(0);
+    const T ci = i < mx - 1 ? -a_off : T(0);
+    const T bi = (i == 0 || i == mx - 1) ? b_edge : b_in;
+    const T inv = T(1) / (bi - ai * ga);  // ga = 0 at j = 0
+    al = j == 0 ? ai * inv : -ai * al * inv;
+    ph = (lrow[i] - ai * ph) * inv;
+    ga = ci * inv;
+    lrow[i] = ph;
+    lal[j * kTriThreads + tid] = al;
+    lga[j * kTriThreads + tid] = ga;
+  }
+  const T ale = al, gae = ga, phe = ph;  // the block's last row
+  // upward: (ph, al, ga) -> (P, A, G) in place
+  T P = T(0), A = T(0), G = T(-1);
+  if (jl >= 0) {
+    lrow[s0 + jl] = P;
+    lal[jl * kTriThreads + tid] = A;
+    lga[jl * kTriThreads + tid] = G;
+  }
+  for (int j = jl - 1; j >= 0; --j) {
+    const int q = j * kTriThreads + tid;
+    const T g = lga[q];
+    P = lrow[s0 + j] - g * P;
+    A = lal[q] - g * A;
+    G = -g * G;
+    lrow[s0 + j] = P;
+    lal[q] = A;
+    lga[q] = G;
+  }
+  sp[tid] = P;
+  sa[tid] = A;
+  sg[tid] = G;
+  __syncthreads();
+  // interface system in the blocks' last unknowns (identity rows past the last block)
+  {
+    T ai = T(0), bi = T(1), ci = T(0), fi = T(0);
+    if (tid < nblk) {
+      ai = ale;
+      fi = phe;
+      if (tid + 1 < nblk) {
+        bi = T(1) - gae * sa[tid + 1];
+        ci = -gae * sg[tid + 1];
+        fi = phe - gae * sp[tid + 1];
+      }
+    }
+    ra[tid] = ai;
+    rb[tid] = bi;
+    rc[tid] = ci;
+    rf[tid] = fi;
   }
   __syncthreads();
-  for (int s = 1; s < mx; s *= 2) {
-    T na[kPcrPer], nb[kPcrPer], nc[kPcrPer], nf[kPcrPer];
-#pragma unroll
-    for (int q = 0; q < kPcrPer; ++q) {
-      const int i = tid + q * kPcrThreads;
-      if (i >= mx) continue;
-      const int im = i - s, ip = i + s;
-      T ai = T(0), ci = T(0);
-      T bi = lb[i], fi = lf[i];
-      if (im >= 0) {
-        const T k1 = la[i] * lr[im];
-        ai = -la[im] * k1;
-        bi -= lc[im] * k1;
-        fi -= lf[im] * k1;
-      }
-      if (ip < mx) {
-        const T k2 = lc[i] * lr[ip];
-        ci = -lc[ip] * k2;
-        bi -= la[ip] * k2;
-        fi -= lf[ip] * k2;
-      }
-      na[q] = ai;
-      nb[q] = bi;
-      nc[q] = ci;
-      nf[q] = fi;
+  for (int st = 1; st < kTriThreads; st *= 2) {
+    const int im = tid - st, ip = tid + st;
+    T ai = T(0), bi = rb[tid], ci = T(0), fi = rf[tid];
+    if (im >= 0) {
+      const T k1 = ra[tid] / rb[im];
+      ai = -ra[im] * k1;
+      bi -= rc[im] * k1;
+      fi -= rf[im] * k1;
+    }
+    if (ip < kTriThreads) {
+      const T k2 = rc[tid] / rb[ip];
+      ci = -rc[ip] * k2;
+      bi -= ra[ip] * k2;
+      fi -= rf[ip] * k2;
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPcrPer; ++q) {
-      const int i = tid + q * kPcrThreads;
-      if (i >= mx) continue;
-      la[i] = na[q];
-      lb[i] = nb[q];
-      lr[i] = T(1) / nb[q];
-      lc[i] = nc[q];
-      lf[i] = nf[q];
-    }
+    ra[tid] = ai;
+    rb[tid] = bi;
+    rc[tid] = ci;
+    rf[tid] = fi;
     __syncthreads();
   }
-  for (int i = tid; i < mx; i += kPcrThreads) row[i] = lf[i] * lr[i];
+  sp[tid] = rf[tid] / rb[tid];  // y_t
+  __syncthreads();
+  const T y = sp[tid];
+  const T ym = tid > 0 ? sp[tid - 1] : T(0);
+  for (int j = 0; j <= jl; ++j) {
+    const int q = j * kTriThreads + tid;
+    lrow[s0 + j] = lrow[s0 + j] - lal[q] * ym - lga[q] * y;
+  }
+  __syncthreads();
+  for (int i = tid; i < mx; i += kTriThreads) row[i] = lrow[i];
 }
 
 template <typename T>
@@ -152,17 +206,21 @@ DctPrecond<T>::~DctPrecond() {
   if (blas_) (void)rocblas_destroy_handle(blas_);
 }
 
-// plain GEMM in the element type: C (m x n) = op(A) op(B), column-major
-static rocblas_status gemm(rocblas_handle b, rocblas_operation tb, int m, int n, int k,
-                           const float* A, int lda, const float* B, int ldb, float* C, int ldc) {
+// two plain GEMMs in the element type in one call: C_i (m x n) = A_i op(B_i), column-major,
+// the second operands at the given strides
+static rocblas_status gemm2(rocblas_handle b, rocblas_operation tb, int m, int n, int k,
+                            const float* A, int lda, long sa, const float* B, int ldb, long sb,
+                            float* C, int ldc, long sc) {
   const float one = 1.0f, zero = 0.0f;
-  return rocblas_sgemm(b, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+  return rocblas_sgemm_strided_batched(b, rocblas_operation_none, tb, m, n, k, &one, A, lda, sa,
+                                       B, ldb, sb, &zero, C, ldc, sc, 2);
 }
-static rocblas_status gemm(rocblas_handle b, rocblas_operation tb, int m, int n, int k,
-                           const double* A, int lda, const double* B, int ldb, double* C,
-                           int ldc) {
+static rocblas_status gemm2(rocblas_handle b, rocblas_operation tb, int m, int n, int k,
+                            const double* A, int lda, long sa, const double* B, int ldb, long sb,
+                            double* C, int ldc, long sc) {
   const double one = 1.0, zero = 0.0;
-  return rocblas_dgemm(b, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+  return rocblas_dgemm_strided_batched(b, rocblas_operation_none, tb, m, n, k, &one, A, lda, sa,
+                                       B, ldb, sb, &zero, C, ldc, sc, 2);
 }
 
 static int blas_check(rocblas_status st, const char* what) {
@@ -186,24 +244,26 @@ int DctPrecond<T>::init(int nx, int ny, double hx, double hy, double eps, hipStr
   hy_ = hy;
   eps_ = eps;
   const int h = h_, n = ny;
-  std::vector<T> ce((size_t)(h + 1) * (h + 1)), co((size_t)h * h);
-  // column-major: ce[j + m (h + 1)] = cos(pi j 2m / n); co[j + m h] = cos(pi j (2m + 1) / n)
+  const size_t hh = (size_t)(h + 1) * (h + 1);
+  // column-major (h+1) x (h+1) blocks: even block cmat[j + m (h+1)] = cos(pi j 2m / n); odd
+  // block cmat[hh + j + m (h+1)] = cos(pi j (2m + 1) / n) for j, m < h, zero in row and
+  // column h, so that the odd half runs as a GEMM of the even half's shape
+  std::vector<T> cmat(2 * hh, T(0));
   for (int m = 0; m <= h; ++m)
     for (int j = 0; j <= h; ++j)
-      ce[(size_t)j + (size_t)m * (h + 1)] =
+      cmat[(size_t)j + (size_t)m * (h + 1)] =
           (T)std::cos(M_PI * (double)((long)j * 2 * m % (2 * n)) / (double)n);
   for (int m = 0; m < h; ++m)
     for (int j = 0; j < h; ++j)
-      co[(size_t)j + (size_t)m * h] =
+      cmat[hh + (size_t)j + (size_t)m * (h + 1)] =
           (T)std::cos(M_PI * (double)((long)j * (2 * m + 1) % (2 * n)) / (double)n);
-  TMX_TRY(ce_.alloc(ce.size()));
-  TMX_TRY(co_.alloc(co.size()));
-  TMX_TRY(fs_.alloc((size_t)(h + 1) * mx_));
-  TMX_TRY(fd_.alloc((size_t)(h + 1) * mx_));
-  TMX_TRY(ts_.alloc((size_t)(h + 1) * mx_));
-  TMX_TRY(td_.alloc((size_t)(h + 1) * mx_));
-  TMX_HIP(hipMemcpyAsync(ce_.get(), ce.data(), ce.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  TMX_HIP(hipMemcpyAsync(co_.get(), co.data(), co.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  TMX_TRY(c_.alloc(cmat.size()));
+  TMX_TRY(f_.alloc(2 * (size_t)(h + 1) * mx_));
+  TMX_TRY(t_.alloc(2 * (size_t)(h + 1) * mx_));
+  TMX_HIP(hipMemcpyAsync(c_.get(), cmat.data(), cmat.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  // the odd halves' column h is never written and must stay zero (0 * garbage may be NaN)
+  TMX_HIP(hipMemsetAsync(f_.get(), 0, f_.size() * sizeof(T), s));
+  TMX_HIP(hipMemsetAsync(t_.get(), 0, t_.size() * sizeof(T), s));
   TMX_HIP(hipStreamSynchronize(s));
   if (!blas_) TMX_TRY(blas_check(rocblas_create_handle(&blas_), "create_handle"));
   return 0;
@@ -214,32 +274,32 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
   const int h = h_, n = ny_, mx = mx_;
   const size_t total = (size_t)(h + 1) * mx;
   const unsigned g = (unsigned)std::min<size_t>(4096, (total + kBlock - 1) / kBlock);
-  k_dct_fold<T><<<g, kBlock, 0, s>>>(mx, n, r, fs_.get(), fd_.get());
+  const long half = (long)(h + 1) * mx, hh = (long)(h + 1) * (h + 1);
+  T* fe = f_.get();
+  T* fo = fe + half;
+  T* te = t_.get();
+  T* to = te + half;
+  k_dct_fold<T><<<g, kBlock, 0, s>>>(mx, n, r, fe, fo);
   TMX_HIP(hipGetLastError());
   TMX_TRY(blas_check(rocblas_set_stream(blas_, s), "set_stream"));
-  // forward: T_e (mx x h+1) = S (mx x h+1) Ce; T_o (mx x h) = D (mx x h) Co
-  TMX_TRY(blas_check(gemm(blas_, rocblas_operation_none, mx, h + 1, h + 1, fs_.get(), mx,
-                          ce_.get(), h + 1, ts_.get(), mx), "gemm"));
-  TMX_TRY(blas_check(gemm(blas_, rocblas_operation_none, mx, h, h, fd_.get(), mx, co_.get(), h,
-                          td_.get(), mx), "gemm"));
+  // forward: T_e = S Ce, T_o = D Co (mx x h+1 each, one batched call)
+  TMX_TRY(blas_check(gemm2(blas_, rocblas_operation_none, mx, h + 1, h + 1, fe, mx, half,
+                           c_.get(), h + 1, hh, te, mx, half), "gemm"));
   // tridiagonal solves along x, one workgroup per mode
   const T a_off = (T)(eps_ * eps_ * hy_ / hx_);
   const double cy = eps_ * eps_ * hx_ / hy_;
-  k_dct_pcr<T><<<2 * h + 1, kPcrThreads, 0, s>>>(mx, n, a_off, hx_ * hy_, cy, ts_.get(),
-                                                 td_.get());
+  k_dct_tri<T><<<2 * h + 1, kTriThreads, 0, s>>>(mx, n, a_off, hx_ * hy_, cy, te, to);
   TMX_HIP(hipGetLastError());
-  // inverse: E (mx x h+1) = T_e Ce^T; O (mx x h) = T_o Co^T
-  TMX_TRY(blas_check(gemm(blas_, rocblas_operation_transpose, mx, h + 1, h + 1, ts_.get(), mx,
-                          ce_.get(), h + 1, fs_.get(), mx), "gemm"));
-  TMX_TRY(blas_check(gemm(blas_, rocblas_operation_transpose, mx, h, h, td_.get(), mx, co_.get(),
-                          h, fd_.get(), mx), "gemm"));
+  // inverse: E = T_e Ce^T, O = T_o Co^T
+  TMX_TRY(blas_check(gemm2(blas_, rocblas_operation_transpose, mx, h + 1, h + 1, te, mx, half,
+                           c_.get(), h + 1, hh, fe, mx, half), "gemm"));
   const unsigned gu = (unsigned)std::min<size_t>(2048, (total + kBlock - 1) / kBlock);
   double* part = nullptr;
   if (rz) {
     rz->nblocks = (int)gu;
     part = rz->partials;
   }
-  k_dct_unfold<T><<<gu, kBlock, 0, s>>>(mx, n, fs_.get(), fd_.get(), r, z, part);
+  k_dct_unfold<T><<<gu, kBlock, 0, s>>>(mx, n, fe, fo, r, z, part);
   TMX_HIP(hipGetLastError());
   return 0;
 }
