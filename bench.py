@@ -209,8 +209,8 @@ def main():
                        "compliance_rel_err_mixed_vs_fp64": abs(J_mixed - J64) / abs(J64),
                        "gradient_rel_err_mixed_vs_fp64": dw,
                        "speedup_mixed_over_fp64": t64 / t_mixed,
-                       "filter_preconditioner": "DCT-I / cyclic reduction (FP64)",
-                       "mixed_filter_preconditioner": "DCT-I / cyclic reduction (FP32)"}
+                       "filter_preconditioner": "DCT-I / tridiagonal partition solver (FP64)",
+                       "mixed_filter_preconditioner": "DCT-I / tridiagonal partition solver (FP32)"}
         tir, Jir, wir, stir = time_steps(problem, rho, max(2, args.steps // 2), 1, "mixed-ir",
                                          args.rtol)
         result_ir = {"ms_per_step": tir * 1e3, "GDOF_s": ndof / tir / 1e9,

@@ -211,7 +211,7 @@ enum tmx_option {
                                    re-picks it per direction and precision); 0: the band
                                    kernels */
   TMX_OPT_FILTER_DCT = 21,      /* 1 (default): the filter's preconditioner (FP32 and FP64) is
-                                   the exact DCT-I / cyclic-reduction solver of
+                                   the exact DCT-I / tridiagonal solver of
                                    eps^2 K + (lumped mass) when ny is even and nx + 1 <= 3072
                                    (3-5 PCG iterations); 0: the multigrid V-cycle */
   TMX_OPT_PAIR_LEVELS = 22,     /* elasticity element levels 0 .. value-1 run the paired

@@ -281,7 +281,7 @@ class Hierarchy {
   // filter_pipe_down / _up) with node-row chunks of these heights ([FP32, FP64][down, up]);
   // 0: band kernels
   int fpipe_rows_[2][2] = {{12, 12}, {12, 12}};
-  // filter preconditioner: the exact DCT-I / cyclic-reduction solver of the tensor-lumped
+  // filter preconditioner: the exact DCT-I / tridiagonal solver of the tensor-lumped
   // operator (tmx_dct.h) instead of the V-cycle, where the grid allows it
   bool use_dct_ = true;
   std::unique_ptr<DctPrecond<float>> dct32_;
