@@ -1,5 +1,5 @@
 """The filter's exact fast preconditioner: z = A'^-1 r for A' = eps^2 K + hx hy (W_y (x) W_x) (the
-DCT-I in y folded into even and odd modes, cyclic reduction in x).  Checked by applying A' to z
+DCT-I in y folded into even and odd modes, tridiagonal solves in x by the partition method).  Checked by applying A' to z
 in FP64 (torch, written out from its stencil) and comparing with r; and at the solve level
 against the multigrid-preconditioned filter."""
 
@@ -50,9 +50,12 @@ TOL = {"fp32": 1e-4, "fp64": 1e-11}
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-@pytest.mark.parametrize("nx,ny", [(60, 20), (120, 40), (250, 90), (20, 10), (300, 100), (15, 8)])
+@pytest.mark.parametrize("nx,ny", [(60, 20), (120, 40), (250, 90), (20, 10), (300, 100), (15, 8),
+                                   (1, 200), (3071, 4), (2047, 6)])
 def test_dct_solves_lumped_operator(nx, ny, precision):
-    """Grids of more than 128 nodes (smaller filter grids are solved densely, exactly)."""
+    """Grids of more than 128 nodes (smaller filter grids are solved densely, exactly).  x rows
+    of 2 nodes (one row per thread block), 3072 (the longest: 12 rows in each of the 256
+    blocks) and 2048 (8 rows per block), and rows whose last block is short."""
     gp = gpu_problem(CANTILEVER, nx, ny)
     gp.set_penalization(3)
     gp.set_option("filter_dct", 1)
@@ -68,9 +71,10 @@ def test_dct_solves_lumped_operator(nx, ny, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_odd_grid_keeps_the_v_cycle(precision):
-    """ny odd: the preconditioner stays the V-cycle (the same output with the option on or off)."""
-    nx, ny = 45, 15
+@pytest.mark.parametrize("nx,ny", [(45, 15), (3072, 2)])
+def test_unsupported_grid_keeps_the_v_cycle(precision, nx, ny):
+    """ny odd, or x rows longer than 3072 nodes: the preconditioner stays the V-cycle (the same
+    output with the option on or off)."""
     gp = gpu_problem(CANTILEVER, nx, ny)
     gp.set_penalization(3)
     r = torch.randn((nx + 1) * (ny + 1), dtype=torch.float64).cuda()
