@@ -41,7 +41,7 @@ class DctPrecond {
  private:
   int nx_ = 0, ny_ = 0, h_ = 0, mx_ = 0;
   double hx_ = 0, hy_ = 0, eps_ = 0;
-  DevBuf<T> c_;  // folded DCT-I matrices, even then odd, column-major (h+1)^2 each
+  DevBuf<T> c_;  // folded DCT-I matrices Ce, Co, Ce, Co^T, column-major (h+1)^2 each
   DevBuf<T> f_;  // folded sums then differences (mx x h+1 each), then the inverse transforms
   DevBuf<T> t_;  // transformed even then odd modes (mx x h+1 each)
   rocblas_handle blas_ = nullptr;
