@@ -35,7 +35,7 @@ class DctPrecond {
   ~DctPrecond();
 
   // longest x row: one workgroup of 256 threads solves it, 12 rows per thread, with the row
-  // and two coefficient arrays in LDS (36 KB in FP32, 72 KB in FP64)
+  // and two coefficient arrays in LDS (40 KB in FP32, 80 KB in FP64 at this length)
   static constexpr int kMaxRow = 3072;
 
  private:

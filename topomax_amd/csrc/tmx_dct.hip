@@ -79,19 +79,33 @@ __global__ __launch_bounds__(kBlock) void k_dct_unfold(int mx, int n, const T* _
 // solved by parallel cyclic reduction in LDS (log2(kTriThreads) steps, one equation per
 // thread); then x_i = P_i - A_i y_{t-1} - G_i y_t.  The operator is diagonally dominant, so the
 // eliminations need no pivoting.  ph / P stay in the staged row; al / A and ga / G sit in LDS
-// transposed (index j * kTriThreads + t), so a wave's accesses fall in distinct banks.
+// transposed (index j * nblk + t), so a wave's accesses fall in distinct banks.  LDS is sized
+// to the row (40 KB in FP32 at 3001 nodes: four workgroups per CU).
 constexpr int kTriThreads = 256;
+
+// LDS of k_dct_tri for rows of mx nodes: the row, two coefficient arrays of k * nblk values
+// (k rows per block, nblk blocks) and four arrays of the interface system
+template <typename T>
+static size_t dct_tri_lds(int mx) {
+  const int k = (mx + kTriThreads - 1) / kTriThreads;
+  const int nblk = (mx + k - 1) / k;
+  return sizeof(T) * ((size_t)mx + 2 * (size_t)k * nblk + 4 * (size_t)kTriThreads);
+}
 
 template <typename T>
 __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off, double hxhy,
                                                          double cy, T* __restrict__ even,
                                                          T* __restrict__ odd) {
-  constexpr int kRow = DctPrecond<T>::kMaxRow;
-  static_assert(kRow % kTriThreads == 0, "row capacity");
-  __shared__ T lrow[kRow];               // right-hand side, then ph, P and x
-  __shared__ T lal[kRow], lga[kRow];     // al then A, ga then G (transposed)
-  __shared__ T ra[kTriThreads], rb[kTriThreads], rc[kTriThreads], rf[kTriThreads];
-  __shared__ T sp[kTriThreads], sa[kTriThreads], sg[kTriThreads];  // P, A, G of first rows
+  extern __shared__ __align__(16) unsigned char tri_lds[];
+  const int k = (mx + kTriThreads - 1) / kTriThreads;
+  const int nblk = (mx + k - 1) / k;
+  T* lrow = reinterpret_cast<T*>(tri_lds);  // right-hand side, then ph, P and x
+  T* lal = lrow + mx;                       // al then A, at j * nblk + t
+  T* lga = lal + (size_t)k * nblk;          // ga then G, at j * nblk + t
+  T* ra = lga + (size_t)k * nblk;           // interface system; ra..rc first hold the first
+  T* rb = ra + kTriThreads;                 // rows' P, A, G
+  T* rc = rb + kTriThreads;
+  T* rf = rc + kTriThreads;
   const int h = n / 2;
   const int blk = (int)blockIdx.x;
   const int mode = blk <= h ? 2 * blk : 2 * (blk - h - 1) + 1;
@@ -102,8 +116,6 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
   const T b_in = a_off * T(2) + c, b_edge = a_off + c * T(0.5);
   const int tid = threadIdx.x;
   for (int i = tid; i < mx; i += kTriThreads) lrow[i] = row[i] * inv_d;
-  const int k = (mx + kTriThreads - 1) / kTriThreads;
-  const int nblk = (mx + k - 1) / k;
   const int s0 = tid * k;
   const int jl = tid < nblk ? min(s0 + k, mx) - 1 - s0 : -1;  // last row of the block, local
   __syncthreads();
@@ -119,19 +131,19 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
     ph = (lrow[i] - ai * ph) * inv;
     ga = ci * inv;
     lrow[i] = ph;
-    lal[j * kTriThreads + tid] = al;
-    lga[j * kTriThreads + tid] = ga;
+    lal[j * nblk + tid] = al;
+    lga[j * nblk + tid] = ga;
   }
   const T ale = al, gae = ga, phe = ph;  // the block's last row
   // upward: (ph, al, ga) -> (P, A, G) in place
   T P = T(0), A = T(0), G = T(-1);
   if (jl >= 0) {
     lrow[s0 + jl] = P;
-    lal[jl * kTriThreads + tid] = A;
-    lga[jl * kTriThreads + tid] = G;
+    lal[jl * nblk + tid] = A;
+    lga[jl * nblk + tid] = G;
   }
   for (int j = jl - 1; j >= 0; --j) {
-    const int q = j * kTriThreads + tid;
+    const int q = j * nblk + tid;
     const T g = lga[q];
     P = lrow[s0 + j] - g * P;
     A = lal[q] - g * A;
@@ -140,9 +152,9 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
     lal[q] = A;
     lga[q] = G;
   }
-  sp[tid] = P;
-  sa[tid] = A;
-  sg[tid] = G;
+  ra[tid] = P;
+  rb[tid] = A;
+  rc[tid] = G;
   __syncthreads();
   // interface system in the blocks' last unknowns (identity rows past the last block)
   {
@@ -151,11 +163,12 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
       ai = ale;
       fi = phe;
       if (tid + 1 < nblk) {
-        bi = T(1) - gae * sa[tid + 1];
-        ci = -gae * sg[tid + 1];
-        fi = phe - gae * sp[tid + 1];
+        bi = T(1) - gae * rb[tid + 1];
+        ci = -gae * rc[tid + 1];
+        fi = phe - gae * ra[tid + 1];
       }
     }
+    __syncthreads();  // ra..rc are overwritten below
     ra[tid] = ai;
     rb[tid] = bi;
     rc[tid] = ci;
@@ -184,12 +197,13 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
     rf[tid] = fi;
     __syncthreads();
   }
-  sp[tid] = rf[tid] / rb[tid];  // y_t
+  const T y = rf[tid] / rb[tid];  // y_t
   __syncthreads();
-  const T y = sp[tid];
-  const T ym = tid > 0 ? sp[tid - 1] : T(0);
+  ra[tid] = y;
+  __syncthreads();
+  const T ym = tid > 0 ? ra[tid - 1] : T(0);
   for (int j = 0; j <= jl; ++j) {
-    const int q = j * kTriThreads + tid;
+    const int q = j * nblk + tid;
     lrow[s0 + j] = lrow[s0 + j] - lal[q] * ym - lga[q] * y;
   }
   __syncthreads();
@@ -271,6 +285,10 @@ int DctPrecond<T>::init(int nx, int ny, double hx, double hy, double eps, hipStr
   TMX_HIP(hipMemsetAsync(f_.get(), 0, f_.size() * sizeof(T), s));
   TMX_HIP(hipMemsetAsync(t_.get(), 0, t_.size() * sizeof(T), s));
   TMX_HIP(hipStreamSynchronize(s));
+  // rows up to 3072 nodes need more than 64 KB of dynamic LDS in FP64
+  TMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dct_tri<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)dct_tri_lds<T>(kMaxRow)));
   if (!blas_) TMX_TRY(blas_check(rocblas_create_handle(&blas_), "create_handle"));
   return 0;
 }
@@ -300,7 +318,8 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
   // tridiagonal solves along x, one workgroup per mode
   const T a_off = (T)(eps_ * eps_ * hy_ / hx_);
   const double cy = eps_ * eps_ * hx_ / hy_;
-  k_dct_tri<T><<<2 * h + 1, kTriThreads, 0, s>>>(mx, n, a_off, hx_ * hy_, cy, te, to);
+  k_dct_tri<T><<<2 * h + 1, kTriThreads, dct_tri_lds<T>(mx), s>>>(mx, n, a_off, hx_ * hy_, cy,
+                                                                   te, to);
   TMX_HIP(hipGetLastError());
   // inverse: E = T_e Ce^T, O = T_o Co^T
   TMX_TRY(blas_check(gemm2(blas_, op, mx, h + 1, h + 1, te, mx, half, c_inv, h + 1, hh, fe, mx,
