@@ -219,8 +219,11 @@ enum tmx_option {
                                    (default 1: the finest only); 1..8 */
   TMX_OPT_PAIR_ROWS_COARSE = 23, /* node-row chunk height of the paired kernels on the coarser
                                    paired levels (0: one launch per sweep there) */
-  TMX_OPT_PAIR_NU_COARSE = 24   /* sweeps each way on the coarser paired levels (0: the
+  TMX_OPT_PAIR_NU_COARSE = 24,  /* sweeps each way on the coarser paired levels (0: the
                                    coarse-level count, TMX_OPT_NU_COARSE) */
+  TMX_OPT_BAND_WIDE = 25        /* 1: the elasticity's band levels run 1024-thread workgroups
+                                   with one node per thread (square tiles, e.g. 24 x 24), each
+                                   node's row built once per launch; 0: 256 threads */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

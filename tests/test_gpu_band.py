@@ -120,3 +120,23 @@ def test_band_sweep_counts(nu_coarse, nu_filter, nu_fine_filter):
         print(nu_coarse, nu_filter, nu_fine_filter, prec, rel(z, z_ref), rel(f, f_ref), flush=True)
         assert rel(z, z_ref) < tol
         assert rel(f, f_ref) < tol
+
+
+@pytest.mark.parametrize("nx,ny", [(120, 40), (250, 90), (300, 100), (45, 15)])
+def test_band_wide_equals_band(nx, ny):
+    """The 1024-thread one-node-per-thread band kernels (option band_wide) on the elasticity's
+    cell-weight levels against the 256-thread kernels: the same sweeps in the same order."""
+    gp = gpu_problem(CANTILEVER, nx, ny)
+    gp.set_penalization(3)
+    xi = torch.as_tensor(smoothed_density(nx, ny, 3, sigma=2.0), device="cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(9)
+    r = torch.randn(2 * (nx + 1) * (ny + 1), generator=g, dtype=torch.float64).cuda()
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        gp.set_option("band_wide", 0)
+        z0 = gp.precondition(r, "elasticity", xi, precision=prec)
+        gp.set_option("band_wide", 1)
+        z1 = gp.precondition(r, "elasticity", xi, precision=prec)
+        err = float(torch.linalg.norm(z1 - z0) / torch.linalg.norm(z0))
+        print(nx, ny, prec, err, flush=True)
+        assert torch.isfinite(z1).all()
+        assert err < tol

@@ -32,6 +32,7 @@ TMX_OPT_BAND_FINE_FILTER, TMX_OPT_FINE_CELL_WEIGHTS, TMX_OPT_GRAPHS = 14, 15, 16
 TMX_OPT_SWEEP_PAIRS, TMX_OPT_SWEEP_PAIRS_POST, TMX_OPT_SWEEP_PAIR_RESTRICT = 17, 18, 19
 TMX_OPT_FILTER_PIPE, TMX_OPT_FILTER_DCT = 20, 21
 TMX_OPT_PAIR_LEVELS, TMX_OPT_PAIR_ROWS_COARSE, TMX_OPT_PAIR_NU_COARSE = 22, 23, 24
+TMX_OPT_BAND_WIDE = 25
 TMX_OPT_NU_FINE_FILTER, TMX_OPT_NU_COARSE_FILTER = 8, 9
 
 SIDE_CODES = {"Left": TMX_LEFT, "Right": TMX_RIGHT, "Top": TMX_TOP, "Bottom": TMX_BOTTOM}

@@ -661,6 +661,10 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       h->el.drop_graphs();
       for (int& r : h->el.pair_rows_coarse_) r = (int)value;
       break;
+    case TMX_OPT_BAND_WIDE:
+      h->el.drop_graphs();
+      h->el.band_wide_ = value != 0.0;
+      break;
     case TMX_OPT_PAIR_NU_COARSE:
       if (value < 0 || value > 16 || value != std::floor(value))
         return invalid("sweeps must be 0..16");

@@ -27,6 +27,10 @@
 namespace tmx {
 
 constexpr int kBandThreads = 256;
+// workgroup size of the wide variant: elasticity cell-weight levels whose extended tile has
+// more nodes than kBandThreads but at most this many keep one node per thread (rows built
+// once per launch instead of once per pass)
+constexpr int kBandThreadsWide = 1024;
 constexpr size_t kBandLdsBytes = 64 * 1024;
 
 // where a level's operator comes from
@@ -52,6 +56,7 @@ struct BandLevel {
   T omega_down[kMaxNu];  // step size of each sweep
   T omega_up[kMaxNu];
   int tx, ty;            // owned fine tile (nodes), even
+  int wide;              // 1: kSrcCells levels may use the kBandThreadsWide variant
 };
 
 // element blocks of the two triangle types: elasticity (lam, mu) or the filter (eps)
@@ -62,6 +67,8 @@ void band_filter_table(BandLevel<T>& L, double hx, double hy, double eps, int bs
 
 // LDS bytes of the down (down = true) or up launch of a level with tile (tx, ty)
 size_t band_lds_bytes(int tx, int ty, int nu, int bs, size_t elem, int src, bool down);
+// nodes along one side of the extended tile of a t-node tile
+size_t band_ext(int t, int nu, bool down);
 // tiles (workgroups) of a launch
 int band_tiles(int nx, int ny, int tx, int ty);
 

@@ -53,7 +53,7 @@ __host__ __device__ constexpr bool band_nz(int bs, int t, int i, int j, int e) {
   return lam || mu || dot;
 }
 
-// floor(t / d) for 0 <= t <= 256 and 1 <= d <= 256: a reciprocal estimate and one correction
+// floor(t / d) for 0 <= t <= 1024 and 1 <= d <= 1024: a reciprocal estimate and one correction
 // step, a handful of instructions instead of a full integer division
 __device__ __forceinline__ int div_small(int t, int d) {
   int q = (int)((float)t * __builtin_amdgcn_rcpf((float)d));
@@ -63,10 +63,11 @@ __device__ __forceinline__ int div_small(int t, int d) {
 }
 
 // f(rx, ry) for every point of an RX x RY region, points spread over the workgroup's threads
-// in row-major order; coordinates are stepped by kBandThreads without a division per point
-template <typename F>
+// in row-major order; coordinates are stepped by NT (the workgroup size) without a division
+// per point
+template <int NT, typename F>
 __device__ __forceinline__ void for_region(int RX, int RY, F&& f) {
-  const int sy = div_small(kBandThreads, RX), sx = kBandThreads - sy * RX;
+  const int sy = div_small(NT, RX), sx = NT - sy * RX;
   int ry = div_small((int)threadIdx.x, RX), rx = (int)threadIdx.x - ry * RX;
   while (ry < RY) {
     f(rx, ry);
@@ -83,9 +84,9 @@ __device__ __forceinline__ void for_region(int RX, int RY, F&& f) {
 // the batch's NB points first, then store(u, rx, ry), so each thread has NB global loads in
 // flight rather than waiting out the memory latency once per point.  valid(u) says whether
 // slot u of the batch holds a point.
-template <int NB, typename FL, typename FS>
+template <int NT, int NB, typename FL, typename FS>
 __device__ __forceinline__ void for_region_batched(int RX, int RY, FL&& load, FS&& store) {
-  const int sy = div_small(kBandThreads, RX), sx = kBandThreads - sy * RX;
+  const int sy = div_small(NT, RX), sx = NT - sy * RX;
   int ry = div_small((int)threadIdx.x, RX), rx = (int)threadIdx.x - ry * RX;
   while (ry < RY) {
     int px[NB], py[NB];
@@ -213,7 +214,8 @@ __device__ __forceinline__ void band_update(const BandRow<T, BS>& R, const T* x,
 }
 
 // Per-thread node of a launch.  ONE: thread t owns extended-tile node t for the whole launch
-// (the tile has at most kBandThreads nodes) and keeps its row in registers; otherwise the
+// (the tile has at most as many nodes as the workgroup threads) and keeps its row in
+// registers; otherwise the
 // nodes of each pass are spread over the threads and rows are rebuilt per pass.
 template <typename T, int BS, int SRC, bool ONE>
 struct BandNode {
@@ -244,13 +246,13 @@ struct BandNode {
 // the precomputed row, none is fixed): each thread updates two horizontally adjacent nodes
 // per step, walking down a column pair, so the two nodes share their neighbour loads and the
 // address arithmetic (one multiply per pass instead of one per node).
-template <typename T, bool ZERO, bool RES>
+template <typename T, bool ZERO, bool RES, int NT>
 __device__ __forceinline__ void band_pass_unit_pairs(const BandLevel<T>& L, const T* x,
                                                      const T* bl, T* y, T omega, int EX, int EY,
                                                      int s) {
   const int RX = EX - 2 * s, RY = EY - 2 * s;
-  const int NP = (RX + 1) >> 1;          // column pairs of the region (<= kBandThreads)
-  const int rows = div_small(kBandThreads, NP);  // rows in flight across the workgroup
+  const int NP = (RX + 1) >> 1;          // column pairs of the region (<= NT)
+  const int rows = div_small(NT, NP);    // rows in flight across the workgroup
   const int t = (int)threadIdx.x;
   const int r0 = div_small(t, NP), cp = t - r0 * NP;
   if (r0 >= rows) return;
@@ -282,7 +284,7 @@ __device__ __forceinline__ void band_pass_unit_pairs(const BandLevel<T>& L, cons
 }
 
 // one pass over the extended-tile nodes at distance >= s from its edge
-template <typename T, int BS, int SRC, bool ONE, bool ZERO, bool RES>
+template <typename T, int BS, int SRC, bool ONE, bool ZERO, bool RES, int NT>
 __device__ __forceinline__ void band_pass(const BandLevel<T>& L, BandNode<T, BS, SRC, ONE>& me,
                                           const T* x, const T* bl, const T* cwl, T* y, T omega,
                                           int EX, int EY, int OX, int OY, int s) {
@@ -302,7 +304,7 @@ __device__ __forceinline__ void band_pass(const BandLevel<T>& L, BandNode<T, BS,
       // fixed, so the pass needs no per-node tests
       if (OX >= 1 && OY >= 1 && OX + EX <= L.nx && OY + EY <= L.ny) {
         if constexpr (BS == 1) {
-          band_pass_unit_pairs<T, ZERO, RES>(L, x, bl, y, omega, EX, EY, s);
+          band_pass_unit_pairs<T, ZERO, RES, NT>(L, x, bl, y, omega, EX, EY, s);
           return;
         }
         BandRow<T, BS> U;
@@ -310,7 +312,7 @@ __device__ __forceinline__ void band_pass(const BandLevel<T>& L, BandNode<T, BS,
         for (int e = 0; e < 9 * BS * BS; ++e) U.a[e] = L.urow[e];
 #pragma unroll
         for (int e = 0; e < BS * BS; ++e) U.dinv[e] = L.udinv[e];
-        for_region(EX - 2 * s, EY - 2 * s, [&](int rx, int ry) {
+        for_region<NT>(EX - 2 * s, EY - 2 * s, [&](int rx, int ry) {
           band_update<T, BS, SRC, ZERO, RES>(U, x, bl, y, omega, EX, (s + ry) * EX + (s + rx));
         });
         return;
@@ -318,7 +320,7 @@ __device__ __forceinline__ void band_pass(const BandLevel<T>& L, BandNode<T, BS,
     }
     const unsigned nvx = (unsigned)L.nx + 1u;
     const unsigned nn = nvx * ((unsigned)L.ny + 1u);
-    for_region(EX - 2 * s, EY - 2 * s, [&](int rx, int ry) {
+    for_region<NT>(EX - 2 * s, EY - 2 * s, [&](int rx, int ry) {
       const int lx = s + rx, ly = s + ry;
       const int gx = OX + lx, gy = OY + ly;
       if (gx < 0 || gy < 0 || gx > L.nx || gy > L.ny) return;
@@ -347,13 +349,13 @@ __device__ __forceinline__ void band_coarse_range(int X0, int X1, int nf, int nc
   }
 }
 
-template <typename T, int SRC>
+template <typename T, int SRC, int NT>
 __device__ __forceinline__ void band_stage_cells(const BandLevel<T>& L, T* cwl, int EX, int EY,
                                                  int OX, int OY) {
   if constexpr (SRC == kSrcCells) {
     const int CX = EX + 1, CY = EY + 1;
     T w[kStageBatch][2];
-    for_region_batched<kStageBatch>(
+    for_region_batched<NT, kStageBatch>(
         CX, CY,
         [&](int u, int rx, int ry) {
           const int cx = OX - 1 + rx, cy = OY - 1 + ry;
@@ -380,8 +382,8 @@ __device__ __forceinline__ void band_tile(const BandLevel<T>& L, int& X0, int& Y
   Y0 = (int)(blockIdx.x / ntx) * L.ty;
 }
 
-template <typename T, int BS, int SRC, bool ONE>
-__global__ __launch_bounds__(kBandThreads) void k_band_down(BandLevel<T> L,
+template <typename T, int BS, int SRC, bool ONE, int NT>
+__global__ __launch_bounds__(NT) void k_band_down(BandLevel<T> L,
                                                             const T* __restrict__ b,
                                                             T* __restrict__ xo,
                                                             T* __restrict__ bc) {
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(kBandThreads) void k_band_down(BandLevel<T> L,
   T* cwl = xb + BS * NE;
   const unsigned nvx = (unsigned)L.nx + 1u;
   T bv[kStageBatch][BS];
-  for_region_batched<kStageBatch>(
+  for_region_batched<NT, kStageBatch>(
       EX, EY,
       [&](int u, int rx, int ry) {
         const int gx = OX + rx, gy = OY + ry;
@@ -418,17 +420,17 @@ __global__ __launch_bounds__(kBandThreads) void k_band_down(BandLevel<T> L,
           xb[BS * k + q] = T(0);
         }
       });
-  band_stage_cells<T, SRC>(L, cwl, EX, EY, OX, OY);
+  band_stage_cells<T, SRC, NT>(L, cwl, EX, EY, OX, OY);
   __syncthreads();
   BandNode<T, BS, SRC, ONE> me;
   me.init(L, cwl, EX, EY, OX, OY);
   T* cur = xa;
   T* nxt = xb;
-  band_pass<T, BS, SRC, ONE, true, false>(L, me, nullptr, bl, cwl, cur, L.omega_down[0], EX, EY,
+  band_pass<T, BS, SRC, ONE, true, false, NT>(L, me, nullptr, bl, cwl, cur, L.omega_down[0], EX, EY,
                                           OX, OY, 0);
   __syncthreads();
   for (int k = 1; k < nu; ++k) {
-    band_pass<T, BS, SRC, ONE, false, false>(L, me, cur, bl, cwl, nxt, L.omega_down[k], EX, EY,
+    band_pass<T, BS, SRC, ONE, false, false, NT>(L, me, cur, bl, cwl, nxt, L.omega_down[k], EX, EY,
                                              OX, OY, k);
     __syncthreads();
     T* t = cur;
@@ -436,10 +438,10 @@ __global__ __launch_bounds__(kBandThreads) void k_band_down(BandLevel<T> L,
     nxt = t;
   }
   // residual of the pre-smoothed iterate into the spare buffer
-  band_pass<T, BS, SRC, ONE, false, true>(L, me, cur, bl, cwl, nxt, T(0), EX, EY, OX, OY, nu);
+  band_pass<T, BS, SRC, ONE, false, true, NT>(L, me, cur, bl, cwl, nxt, T(0), EX, EY, OX, OY, nu);
   // owned fine nodes of the pre-smoothed iterate
   const int X1 = min(X0 + L.tx, L.nx + 1), Y1 = min(Y0 + L.ty, L.ny + 1);
-  for_region(X1 - X0, Y1 - Y0, [&](int rx, int ry) {
+  for_region<NT>(X1 - X0, Y1 - Y0, [&](int rx, int ry) {
     const int gx = X0 + rx, gy = Y0 + ry;
     const int i = (gy - OY) * EX + (gx - OX);
     const unsigned g = (unsigned)gy * nvx + (unsigned)gx;
@@ -452,7 +454,7 @@ __global__ __launch_bounds__(kBandThreads) void k_band_down(BandLevel<T> L,
   band_coarse_range(X0, X0 + L.tx, L.nx, L.nxc, I0, I1);
   band_coarse_range(Y0, Y0 + L.ty, L.ny, L.nyc, J0, J1);
   const T* r = nxt;
-  if (I1 > I0) for_region(I1 - I0, J1 - J0, [&](int rx, int ry) {
+  if (I1 > I0) for_region<NT>(I1 - I0, J1 - J0, [&](int rx, int ry) {
     const int I = I0 + rx, J = J0 + ry;
     T acc[BS];
 #pragma unroll
@@ -491,8 +493,8 @@ __global__ __launch_bounds__(kBandThreads) void k_band_down(BandLevel<T> L,
   });
 }
 
-template <typename T, int BS, int SRC, bool ONE>
-__global__ __launch_bounds__(kBandThreads) void k_band_up(BandLevel<T> L, const T* __restrict__ x,
+template <typename T, int BS, int SRC, bool ONE, int NT>
+__global__ __launch_bounds__(NT) void k_band_up(BandLevel<T> L, const T* __restrict__ x,
                                                           const T* __restrict__ ec,
                                                           const T* __restrict__ b,
                                                           T* __restrict__ y,
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(kBandThreads) void k_band_up(BandLevel<T> L, const 
   T* cwl = xb + BS * NE;
   const unsigned nvx = (unsigned)L.nx + 1u, nvc = (unsigned)L.nxc + 1u;
   T bv[kStageBatch][BS], xv[kStageBatch][BS];
-  for_region_batched<kStageBatch>(
+  for_region_batched<NT, kStageBatch>(
       EX, EY,
       [&](int u, int rx, int ry) {
         const int gx = OX + rx, gy = OY + ry;
@@ -556,14 +558,14 @@ __global__ __launch_bounds__(kBandThreads) void k_band_up(BandLevel<T> L, const 
           xb[BS * k + q] = T(0);
         }
       });
-  band_stage_cells<T, SRC>(L, cwl, EX, EY, OX, OY);
+  band_stage_cells<T, SRC, NT>(L, cwl, EX, EY, OX, OY);
   __syncthreads();
   BandNode<T, BS, SRC, ONE> me;
   me.init(L, cwl, EX, EY, OX, OY);
   T* cur = xa;
   T* nxt = xb;
   for (int k = 0; k < nu; ++k) {
-    band_pass<T, BS, SRC, ONE, false, false>(L, me, cur, bl, cwl, nxt, L.omega_up[k], EX, EY, OX,
+    band_pass<T, BS, SRC, ONE, false, false, NT>(L, me, cur, bl, cwl, nxt, L.omega_up[k], EX, EY, OX,
                                              OY, k + 1);
     __syncthreads();
     T* t = cur;
@@ -572,7 +574,7 @@ __global__ __launch_bounds__(kBandThreads) void k_band_up(BandLevel<T> L, const 
   }
   const int X1 = min(X0 + L.tx, L.nx + 1), Y1 = min(Y0 + L.ty, L.ny + 1);
   double part = 0.0;
-  for_region(X1 - X0, Y1 - Y0, [&](int rx, int ry) {
+  for_region<NT>(X1 - X0, Y1 - Y0, [&](int rx, int ry) {
     const int gx = X0 + rx, gy = Y0 + ry;
     const int i = (gy - OY) * EX + (gx - OX);
     const unsigned g = (unsigned)gy * nvx + (unsigned)gx;
@@ -582,10 +584,12 @@ __global__ __launch_bounds__(kBandThreads) void k_band_up(BandLevel<T> L, const 
       part += (double)bl[BS * i + q] * (double)cur[BS * i + q];
     }
   });
-  if (dot) {
-    const double v[1] = {part};
-    double* const o[1] = {dot};
-    block_partials<1>(v, o);
+  if constexpr (NT <= kBlock) {  // (block_partials holds one slot per wave of kBlock)
+    if (dot) {
+      const double v[1] = {part};
+      double* const o[1] = {dot};
+      block_partials<1>(v, o);
+    }
   }
 }
 
@@ -648,7 +652,7 @@ template void band_elastic_table<float>(BandLevel<float>&, double, double, doubl
 template void band_filter_table<double>(BandLevel<double>&, double, double, double, int);
 template void band_filter_table<float>(BandLevel<float>&, double, double, double, int);
 
-static size_t band_ext(int t, int nu, bool down) {
+size_t band_ext(int t, int nu, bool down) {
   return (size_t)t + 2 * ((size_t)nu + (down ? 1 : 0));
 }
 
@@ -677,18 +681,18 @@ static bool band_valid(const BandLevel<T>& L) {
   return okx && oky;
 }
 
-template <typename T, int BS, int SRC, bool ONE>
+template <typename T, int BS, int SRC, bool ONE, int NT = kBandThreads>
 static void launch_down(const BandLevel<T>& L, const T* b, T* x, T* bc, size_t lds,
                         hipStream_t s) {
-  k_band_down<T, BS, SRC, ONE><<<band_tiles(L.nx, L.ny, L.tx, L.ty), kBandThreads, lds, s>>>(
-      L, b, x, bc);
+  k_band_down<T, BS, SRC, ONE, NT><<<band_tiles(L.nx, L.ny, L.tx, L.ty), NT, lds, s>>>(L, b, x,
+                                                                                        bc);
 }
 
-template <typename T, int BS, int SRC, bool ONE>
+template <typename T, int BS, int SRC, bool ONE, int NT = kBandThreads>
 static void launch_up(const BandLevel<T>& L, const T* x, const T* ec, const T* b, T* y,
                       double* dot, size_t lds, hipStream_t s) {
-  k_band_up<T, BS, SRC, ONE><<<band_tiles(L.nx, L.ny, L.tx, L.ty), kBandThreads, lds, s>>>(
-      L, x, ec, b, y, dot);
+  k_band_up<T, BS, SRC, ONE, NT><<<band_tiles(L.nx, L.ny, L.tx, L.ty), NT, lds, s>>>(L, x, ec, b,
+                                                                                      y, dot);
 }
 
 template <typename T, int BS>
@@ -696,9 +700,11 @@ hipError_t band_down(const BandLevel<T>& L, const T* b, T* x, T* bc, hipStream_t
   if (!band_valid(L)) return hipErrorInvalidValue;
   const size_t lds = band_lds_bytes(L.tx, L.ty, L.nu_down, BS, sizeof(T), L.src, true);
   if (lds > kBandLdsBytes) return hipErrorInvalidValue;
-  const bool one =
-      band_ext(L.tx, L.nu_down, true) * band_ext(L.ty, L.nu_down, true) <= (size_t)kBandThreads;
-  if (L.src == kSrcStencil) {
+  const size_t ne = band_ext(L.tx, L.nu_down, true) * band_ext(L.ty, L.nu_down, true);
+  const bool one = ne <= (size_t)kBandThreads;
+  if (L.wide && L.src == kSrcCells && !one && ne <= (size_t)kBandThreadsWide) {
+    launch_down<T, BS, kSrcCells, true, kBandThreadsWide>(L, b, x, bc, lds, s);
+  } else if (L.src == kSrcStencil) {
     if (one) launch_down<T, BS, kSrcStencil, true>(L, b, x, bc, lds, s);
     else launch_down<T, BS, kSrcStencil, false>(L, b, x, bc, lds, s);
   } else if (L.src == kSrcCells) {
@@ -724,9 +730,11 @@ hipError_t band_up(const BandLevel<T>& L, const T* x, const T* ec, const T* b, T
     dot->nblocks = tiles;
     part = dot->partials;
   }
-  const bool one =
-      band_ext(L.tx, L.nu_up, false) * band_ext(L.ty, L.nu_up, false) <= (size_t)kBandThreads;
-  if (L.src == kSrcStencil) {
+  const size_t ne = band_ext(L.tx, L.nu_up, false) * band_ext(L.ty, L.nu_up, false);
+  const bool one = ne <= (size_t)kBandThreads;
+  if (L.wide && L.src == kSrcCells && !one && ne <= (size_t)kBandThreadsWide && !part) {
+    launch_up<T, BS, kSrcCells, true, kBandThreadsWide>(L, x, ec, b, y, part, lds, s);
+  } else if (L.src == kSrcStencil) {
     if (one) launch_up<T, BS, kSrcStencil, true>(L, x, ec, b, y, part, lds, s);
     else launch_up<T, BS, kSrcStencil, false>(L, x, ec, b, y, part, lds, s);
   } else if (L.src == kSrcCells) {

@@ -264,6 +264,8 @@ class Hierarchy {
   // the host keeping an iteration ahead either way)
   bool use_graphs_ = false;
   bool band_fine_filter_ = true;
+  // elasticity cell-weight band levels use the 1024-thread one-node-per-thread kernels
+  bool band_wide_ = false;
   // > 0: the finest elasticity level's first two pre-sweeps (pair_rows_), and its prolongation
   // with the first two post-sweeps (pair_rows_post_), run as one launch each with node-row
   // chunks of that height; 0: one launch per sweep.  Index 0: FP32 V-cycles, 1: FP64.

@@ -584,6 +584,20 @@ static bool band_level(const Hierarchy& h, int l, BandLevel<T>& B) {
   if (B.nu_down < 1 || B.nu_down > kMaxNu || B.nu_up < 1 || B.nu_up > kMaxNu) return false;
   for (int k = 0; k < B.nu_down; ++k) B.omega_down[k] = (T)h.sweep_omega(k, B.nu_down);
   for (int k = 0; k < B.nu_up; ++k) B.omega_up[k] = (T)h.sweep_omega(k, B.nu_up);
+  if (h.band_wide_ && B.src == kSrcCells) {
+    // one node per thread of a kBandThreadsWide workgroup: the largest even square tile whose
+    // extended tile fits it (24 x 24 for three sweeps)
+    const int nu = std::max(B.nu_down, B.nu_up);
+    int t = 2;
+    while (band_ext(t + 2, nu, true) * band_ext(t + 2, nu, true) <= (size_t)kBandThreadsWide) t += 2;
+    if (band_ext(t, nu, true) * band_ext(t, nu, true) <= (size_t)kBandThreadsWide &&
+        band_lds_bytes(t, t, nu, h.bs_, sizeof(T), B.src, true) <= kBandLdsBytes &&
+        !(l == 0 && band_tiles(L.nx, L.ny, t, t) > kMaxPartials)) {
+      B.tx = B.ty = t;
+      B.wide = 1;
+      return true;
+    }
+  }
   static const int cand[][2] = {{32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8}};
   bool found = false;
   for (const auto& c : cand) {

@@ -443,7 +443,8 @@ class ElasticityProblem:
                "filter_dct": _lib.TMX_OPT_FILTER_DCT,
                "pair_levels": _lib.TMX_OPT_PAIR_LEVELS,
                "pair_rows_coarse": _lib.TMX_OPT_PAIR_ROWS_COARSE,
-               "pair_nu_coarse": _lib.TMX_OPT_PAIR_NU_COARSE}[name]
+               "pair_nu_coarse": _lib.TMX_OPT_PAIR_NU_COARSE,
+               "band_wide": _lib.TMX_OPT_BAND_WIDE}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
