@@ -76,17 +76,18 @@ def bench_kernel(problem, kernel, precision, reps=50):
 
 def batch_throughput(n_designs: int, steps: int, rtol: float):
     """The batched workload: n_designs seeded designs (N in {40, ..., 320}, aspect 1-3, p 1-5,
-    1-3 random tractions, optional body force, random fixed sides; tests.cases.cfg5_cases),
+    1-3 random tractions, optional body force, random fixed sides;
+    topomax_amd.workloads.batch_cases),
     mixed precision with autotuned tiles; one objective+gradient each, run one after another
     vs in one tmx_batch call."""
     import torch
-    from tests.cases import cfg5_cases, gpu_problem, smoothed_density
     from topomax_amd import ProblemBatch
+    from topomax_amd.workloads import batch_cases, problem_from_case, smoothed_density
     probs, rhos = [], []
     ndof = 0
-    for k, (case, N, p) in enumerate(cfg5_cases(n_designs)):
+    for k, (case, N, p) in enumerate(batch_cases(n_designs)):
         nx, ny = int(case["width"] * N), N
-        gp = gpu_problem(case, nx, ny, precision="mixed", rtol=rtol)
+        gp = problem_from_case(case, nx, ny, precision="mixed", rtol=rtol)
         gp.set_penalization(p)
         gp.autotune()
         probs.append(gp)
@@ -118,9 +119,13 @@ def batch_throughput(n_designs: int, steps: int, rtol: float):
 def cpu_baseline(n_cpu: int, seconds_budget: float = 30.0):
     """The CPU oracle (numpy/scipy, SuperLU direct solves, single thread) on a bounded sample
     of the same workload: one objective + gradient of the same cantilever at N=n_cpu."""
-    from tests.cases import CANTILEVER, oracle_problem, smoothed_density
+    from oracle.fem import ElasticityOracle, StructuredMesh
+    from topomax_amd.workloads import CANTILEVER as C, smoothed_density
     nx, ny = 3 * n_cpu, n_cpu
-    orc = oracle_problem(CANTILEVER, nx, ny)
+    orc = ElasticityOracle(StructuredMesh(C["width"], C["height"], nx, ny),
+                           young_modulus=C["young_modulus"], poisson_ratio=C["poisson_ratio"],
+                           filter_radius=C["filter_radius"], fixed_sides=C["fixed_sides"],
+                           tractions=C["tractions"], body_force=None)
     orc.set_penalization(3)
     rho = smoothed_density(nx, ny, seed=0, sigma=5.0)
     t0 = time.perf_counter()
@@ -168,11 +173,11 @@ def main():
         sys.exit(2)
 
     import torch
-    from tests.cases import CANTILEVER, gpu_problem, smoothed_density
+    from topomax_amd.workloads import CANTILEVER, problem_from_case, smoothed_density
 
     N = args.N
     nx, ny = 3 * N, N
-    problem = gpu_problem(CANTILEVER, nx, ny, precision="mixed", rtol=args.rtol)
+    problem = problem_from_case(CANTILEVER, nx, ny, precision="mixed", rtol=args.rtol)
     problem.set_penalization(args.p)
     rho = torch.as_tensor(smoothed_density(nx, ny, seed=0, sigma=5.0), device="cuda:0")
     ndof = 2 * (nx + 1) * (ny + 1)

@@ -94,6 +94,28 @@ def test_nan_density_gives_nan_objective():
     assert np.isnan(J2)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_handle_recovers_after_nan_design(precision):
+    """After a non-finite design, a finite design on the same handle gives exactly what a
+    fresh handle gives: no NaN survives in the solvers' buffers (even ny, so the filter solves
+    run the DCT-I preconditioner, whose odd half carries a zero-padding row)."""
+    nx, ny = 60, 20
+    rho_ok = torch.as_tensor(smoothed_density(nx, ny, seed=5, sigma=2.0), device="cuda:0")
+    bad = rho_ok.clone()
+    bad[7] = float("nan")
+    used = gpu_problem(CANTILEVER, nx, ny, precision=precision)
+    used.set_penalization(3)
+    J_bad, _, _ = used.objective_and_gradient(bad)
+    assert np.isnan(J_bad)
+    J1, w1, _ = used.objective_and_gradient(rho_ok)
+    fresh = gpu_problem(CANTILEVER, nx, ny, precision=precision)
+    fresh.set_penalization(3)
+    J2, w2, _ = fresh.objective_and_gradient(rho_ok)
+    print(precision, J1, J2, flush=True)
+    assert np.isfinite(J1) and J1 == J2
+    assert torch.equal(w1, w2)
+
+
 def test_iteration_limit_reports_not_converged():
     gp = gpu_problem(CANTILEVER, 60, 20)
     gp.set_penalization(3)

@@ -97,7 +97,7 @@ def test_p2_indefinite_negative_density():
     assert rel(u, u_ref) < 1e-8
 
 
-def test_triangle_golden_on_gpu():
+def test_triangle_golden_on_gpu(tmp_path):
     """The reference's known answer (tests/test_elasticity_solver.py:30-55): the triangle
     design at N=10 with the P2 element, EMD to convergence; objective 0.0018191324070894702
     at iteration 24."""
@@ -112,3 +112,19 @@ def test_triangle_golden_on_gpu():
     assert res.iteration == golden["iteration"]
     assert abs(res.objectives[-1] - golden["objective"]) < 1e-10 * golden["objective"]
     assert np.abs(rho - np.array(golden["rho_sorted"])).max() < 1e-8
+    # the design written in the reference's dof order: the permutation is recovered by
+    # matching values (rank by rank; mirror-image vertices hold equal values, so either
+    # choice of a pair gives the same vector) and save_rho must then write the reference's
+    # correct_rho.dat vector, which load_design maps back to vertex order
+    from topomax_amd.consistency import load_design
+    vertex = host(solver.rho)
+    golden_dof = np.array(golden["rho_dof_order"])
+    perm = np.empty(vertex.size, dtype=np.int64)
+    perm[np.argsort(golden_dof, kind="stable")] = np.argsort(vertex, kind="stable")
+    path = solver.save_rho(str(tmp_path / "triangle"), dof_order=perm)
+    N, size, back = load_design(str(tmp_path / path), dof_order=perm)
+    from topomax_amd.consistency import read_design_dict
+    written = read_design_dict(str(tmp_path / path))
+    assert N == golden["N"] and size == (1.0, 1.0) and "order" not in written
+    assert np.abs(written["vector"] - golden_dof).max() < 1e-8
+    assert np.array_equal(back, vertex)

@@ -182,3 +182,24 @@ def test_indefinite_stiffness_negative_filtered_density(precision):
     assert st["outer_iterations"] >= 2  # the MINRES fallback ran
     assert rel(u, u_ref) < 1e-8
     assert st["rel_residual"] < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_indefinite_design_leaves_smoother_unchanged(precision):
+    """The breakdown of an indefinite design retries its own solve with Jacobi smoothing and
+    then MINRES; the next positive-definite design on the handle solves exactly as before it
+    (same Chebyshev smoother, same iteration count, bitwise equal displacement)."""
+    nx, ny = 60, 20
+    gp = gpu_problem(CANTILEVER, nx, ny, precision=precision, rtol=1e-12)
+    gp.set_penalization(3)
+    X, Y = np.meshgrid(np.linspace(0, 3, nx + 1), np.linspace(0, 1, ny + 1))
+    hole = ((X - 2.0) ** 2 + (Y - 0.5) ** 2 < 0.3 ** 2).ravel()
+    good = 0.3 + 0.7 * smoothed_density(nx, ny, seed=4, sigma=2.0, lo=0.0)
+    u_a = gp.forward(cuda(good)).clone()
+    it_a = gp.last_stats["iterations"]
+    gp.forward(cuda(np.where(hole, -0.1, good)))
+    assert gp.last_stats["outer_iterations"] >= 2  # the fallback ran
+    u_b = gp.forward(cuda(good))
+    print(precision, it_a, gp.last_stats, flush=True)
+    assert gp.last_stats["iterations"] == it_a and gp.last_stats["outer_iterations"] == 1
+    assert torch.equal(u_a, u_b)

@@ -198,3 +198,66 @@ def test_option_constants_match_header():
     src_py = inspect.getsource(ElasticityProblem.set_option)
     for n in names:
         assert f"_lib.{n}" in src_py, f"{n} has no set_option name"
+
+
+def test_design_record_N_is_per_unit_length(tmp_path):
+    """save_function stores mesh_to_N(mesh), the elements per unit length
+    (FEM_src/utils.py:29-37, 47-69), and load_function rebuilds int(w N) x int(h N) cells:
+    for the 12 x 2 bridge at solver N = 40 that is N = 20 and the solver's own 240 x 40 mesh."""
+    import pickle
+    import numpy as np
+    from topomax_amd.consistency import design_mesh, load_design
+    from topomax_amd.emd import design_record, mesh_parameter
+    n_unit, full_N = mesh_parameter(40, 12.0, 2.0)
+    assert (n_unit, full_N) == (20, 40)
+    mesh = design_mesh(n_unit, (12.0, 2.0))
+    assert (mesh.nx, mesh.ny) == (int(12.0 * n_unit), int(2.0 * n_unit)) == (240, 40)
+    vec = np.linspace(0.0, 1.0, mesh.n_vertices)
+    path = tmp_path / "bridge_rho.dat"
+    with open(path, "wb") as fh:
+        pickle.dump(design_record(vec, n_unit, 12.0, 2.0), fh)
+    N, size, got = load_design(str(path))
+    assert N == 20 and size == (12.0, 2.0) and np.array_equal(got, vec)
+
+
+def test_design_record_dof_order_round_trip(tmp_path):
+    """With a dof order the vector is written as the reference writes it (no marker) and
+    load_design maps it back to vertex order; without the order such a file is refused."""
+    import pickle
+    import numpy as np
+    from topomax_amd.consistency import load_design
+    from topomax_amd.emd import design_record
+    rng = np.random.default_rng(3)
+    vec = rng.uniform(size=121)
+    perm = rng.permutation(121)
+    rec = design_record(vec, 10, 1.0, 1.0, dof_order=perm)
+    assert "order" not in rec and np.array_equal(rec["vector"], vec[perm])
+    path = tmp_path / "d_rho.dat"
+    with open(path, "wb") as fh:
+        pickle.dump(rec, fh)
+    with pytest.raises(ValueError, match="dof order"):
+        load_design(str(path))
+    _, _, back = load_design(str(path), dof_order=perm)
+    assert np.array_equal(back, vec)
+    with pytest.raises(ValueError, match="permutation"):
+        design_record(vec, 10, 1.0, 1.0, dof_order=np.zeros(121, dtype=int))
+
+
+def test_load_design_refuses_foreign_globals(tmp_path):
+    """A design file that names anything but numpy's array reconstructors is refused before
+    the named object is imported or called."""
+    import pickle
+    from topomax_amd.consistency import load_design
+    marker = tmp_path / "ran"
+
+    class Payload:
+        def __reduce__(self):
+            return (open, (str(marker), "w"))
+
+    path = tmp_path / "evil_rho.dat"
+    with open(path, "wb") as fh:
+        pickle.dump({"N": 10, "domain_size": (1.0, 1.0), "problem": "design",
+                     "vector": Payload()}, fh)
+    with pytest.raises(pickle.UnpicklingError, match="io.open|builtins.open"):
+        load_design(str(path))
+    assert not marker.exists()

@@ -13,7 +13,9 @@
 
 namespace tmx {
 
-// s[j] = r[j] + r[n - j] (j < h), s[h] = r[h]; d[j] = r[j] - r[n - j] (j < h); rows of mx values
+// s[j] = r[j] + r[n - j] (j < h), s[h] = r[h]; d[j] = r[j] - r[n - j] (j < h), d[h] = 0; rows
+// of mx values.  Row h of d is the odd half's zero padding: it is rewritten on every call, so a
+// non-finite value that an earlier call left in the buffer cannot reach this one's transforms
 template <typename T>
 __global__ void k_dct_fold(int mx, int n, const T* __restrict__ r, T* __restrict__ s,
                            T* __restrict__ d) {
@@ -29,6 +31,7 @@ __global__ void k_dct_fold(int mx, int n, const T* __restrict__ r, T* __restrict
       d[t] = a - b;
     } else {
       s[t] = a;
+      d[t] = T(0);
     }
   }
 }
@@ -281,7 +284,7 @@ int DctPrecond<T>::init(int nx, int ny, double hx, double hy, double eps, hipStr
   TMX_TRY(f_.alloc(2 * (size_t)(h + 1) * mx_));
   TMX_TRY(t_.alloc(2 * (size_t)(h + 1) * mx_));
   TMX_HIP(hipMemcpyAsync(c_.get(), cmat.data(), cmat.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  // the odd halves' column h is never written and must stay zero (0 * garbage may be NaN)
+  // (k_dct_fold rewrites the odd half's zero padding on every application)
   TMX_HIP(hipMemsetAsync(f_.get(), 0, f_.size() * sizeof(T), s));
   TMX_HIP(hipMemsetAsync(t_.get(), 0, t_.size() * sizeof(T), s));
   TMX_HIP(hipStreamSynchronize(s));

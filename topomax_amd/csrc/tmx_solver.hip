@@ -789,7 +789,7 @@ int Hierarchy::estimate_lmax(Workspace& ws, hipStream_t s, double margin) {
     if (g_trace) std::fprintf(stderr, "[tmx] level %d lambda_max(D^-1 A) ~ %.4f\n", l, est);
   }
   cheb_lmax_ = lmax > 0.0 ? margin * lmax : 0.0;
-  cheb_done_ = true;
+  cheb_done_ = lmax > 0.0;  // (no estimate from a non-finite operator: the next one tries)
   drop_graphs();
   return 0;
 }
@@ -1154,7 +1154,12 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
              : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, &bb, &it, &rel, s);
     S.iterations = it;
     S.outer = 1;
-    if (rc == -4 && cheb_lmax_ > 0.0) {  // Chebyshev bound too low for this operator: Jacobi
+    if (rc == -4 && cheb_lmax_ > 0.0) {
+      // Breakdown with Chebyshev smoothing: either the bound is too low for this operator or
+      // the operator is indefinite.  This solve retries with damped Jacobi; the bound comes
+      // back afterwards, and is re-estimated at the next operator when Jacobi succeeded (a
+      // bound that was too low), so one design does not change the smoother of later ones.
+      const double keep = cheb_lmax_;
       cheb_lmax_ = 0.0;
       drop_graphs();
       int it2 = 0;
@@ -1163,6 +1168,9 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
                : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, &bb, &it2, &rel, s);
       S.iterations += it2;
       it += it2;
+      cheb_lmax_ = keep;
+      if (rc != -4) cheb_done_ = false;
+      drop_graphs();
     }
     if (rc == -4 && phys_ == kElasticity) {  // K(xi) not positive definite: MINRES
       int it2 = 0;

@@ -68,12 +68,13 @@ class EMDSolver:
 
     ``design`` is a design file path or its parsed JSON dict; ``N`` is the reference's mesh
     parameter.  ``data_path`` enables the reference's per-iteration and per-penalty output
-    files under ``{data_path}/GPU/{design}/data``."""
+    files under ``{data_path}/GPU/{design}/data``; ``dof_order`` (the vertex of each CG1 dof)
+    makes the saved design vectors dof-ordered like the reference's (``save_rho``)."""
 
     def __init__(self, N: int, design, *, data_path: str | None = None, skip_multiple: int = 1,
                  precision: str = "fp64", rtol: float = 1e-12, filter_rtol: float | None = None,
                  max_iterations: int = 1000, degree: int = 1, device=None,
-                 verbose: bool = False):
+                 verbose: bool = False, dof_order=None):
         if isinstance(design, dict):
             self.parameters, problem_parameters = parse_design_dict(design)
             self.design_str = "design"
@@ -81,14 +82,14 @@ class EMDSolver:
             self.parameters, problem_parameters = parse_design(design)
             self.design_str = os.path.splitext(os.path.basename(design))[0]
         self.width, self.height = self.parameters.width, self.parameters.height
-        # N elements per unit length of the shorter side (src/solver.py:65-67)
-        self.N = int(N / min(self.width, self.height))
-        self.full_N = int(self.N * min(self.width, self.height))
+        self.N, self.full_N = mesh_parameter(N, self.width, self.height)
         self.volume = self.width * self.height * self.parameters.volume_fraction
         self.step_size = self.parameters.fem_step_size
         self.max_iterations = max_iterations
         self.skip_multiple = skip_multiple
         self.verbose = verbose
+        # vertex of each dof of the saved design vectors (None: vertex order), see save_rho
+        self.dof_order = dof_order
         self.output_folder = (None if data_path is None
                               else f"{data_path}/GPU/{self.design_str}/data")
 
@@ -232,14 +233,18 @@ class EMDSolver:
     def _file_root(self, penalty, suffix):
         return f"{self.output_folder}/N={self.full_N}_p={self.penalty_formatter(penalty)}_{suffix}"
 
-    def save_rho(self, file_root: str) -> str:
-        """``FEM_src/utils.py:47-69`` layout: a pickled dict with N, domain_size, problem and
-        the nodal vector.  The vector is in vertex order (v = iy*(nx+1)+ix), marked by
-        ``"order": "vertex"``; dolfin's CG1 dof order is not reproduced."""
+    def save_rho(self, file_root: str, dof_order=None) -> str:
+        """``FEM_src/utils.py:47-69`` layout: a pickled dict with N (elements per unit length,
+        as ``mesh_to_N`` gives it), domain_size, problem and the nodal vector.
+
+        Without ``dof_order`` the vector is in vertex order (v = iy*(nx+1)+ix), marked by
+        ``"order": "vertex"``.  With it (dof_order[k] = the vertex of dof k, e.g. dolfin's
+        ``dof_to_vertex_map`` of the CG1 space) the vector is written in that dof order with
+        no marker, as the reference's ``save_function`` writes it, so ``load_function`` and
+        ``plot.py`` read it unchanged.  ``self.dof_order`` is the default."""
         rho_file = file_root + "_rho.dat"
-        data = {"N": self.full_N, "domain_size": (float(self.width), float(self.height)),
-                "problem": "design", "vector": self.rho.detach().cpu().numpy(),
-                "order": "vertex"}
+        data = design_record(self.rho.detach().cpu().numpy(), self.N, self.width, self.height,
+                             self.dof_order if dof_order is None else dof_order)
         with open(rho_file, "wb") as fh:
             pickle.dump(data, fh)
         return os.path.basename(rho_file)
@@ -264,6 +269,30 @@ class EMDSolver:
                  "objectives": list(objectives), "iterations": len(objectives),
                  "min_index": min_index, "times": list(times)}
         _dump_reference_record(f"{file_root}.dat", "SolverResult", state)
+
+
+def mesh_parameter(N: int, width: float, height: float):
+    """(elements per unit length, the N of the file names) for the solver's mesh parameter N:
+    N elements along the shorter side (``src/solver.py:65-67``)."""
+    n_unit = int(N / min(width, height))
+    return n_unit, int(n_unit * min(width, height))
+
+
+def design_record(vec, n_unit: int, width: float, height: float, dof_order=None) -> dict:
+    """The dict ``save_function`` pickles for a CG1 design (``FEM_src/utils.py:47-69``): N is
+    the elements per unit length (``mesh_to_N``).  ``vec`` is in vertex order; with
+    ``dof_order`` (dof_order[k] = the vertex of dof k) the vector is written in that order,
+    otherwise it stays in vertex order and the dict is marked ``"order": "vertex"``."""
+    vec = np.asarray(vec, dtype=np.float64)
+    data = {"N": int(n_unit), "domain_size": (float(width), float(height)), "problem": "design"}
+    if dof_order is None:
+        data.update(vector=vec, order="vertex")
+        return data
+    perm = np.asarray(dof_order, dtype=np.int64)
+    if perm.shape != vec.shape or not np.array_equal(np.sort(perm), np.arange(vec.size)):
+        raise ValueError("dof_order is not a permutation of the mesh vertices")
+    data["vector"] = vec[perm]
+    return data
 
 
 def _penalty_formatter(penalties):
