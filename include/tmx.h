@@ -150,6 +150,17 @@ int tmx_matvec(tmx_problem* h, const double* xi, const double* u, double* y, int
 int tmx_load_vector(tmx_problem* h, double* F);
 /* FEMSolver.integrate (FEM_src/solver.py:81-84): int of the CG1 interpolant of values */
 int tmx_integrate(tmx_problem* h, const double* values, double* out_host);
+/* tmx_integrate without a handle and without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops).  The nx x ny 'right' mesh of the width x height rectangle is given
+ * directly; the lumped mass is formed inside the kernel and the block partial sums use a
+ * static 32 KiB device buffer that each GPU holds until the process ends, so at most
+ * 8,388,608 vertices ((nx+1)(ny+1)) per call; larger meshes return TMX_ERR_INVALID.
+ * values: n_v device doubles (vertex order); out: ONE device double, written on `stream`
+ * (a hipStream_t, NULL for the default stream); the call does not synchronise.  Calls that
+ * overlap on one GPU race on the shared buffer. */
+int tmx_integrate_noalloc(int nx, int ny, double width, double height, const double* values,
+                          double* out, void* stream);
 
 /* Autotuner: times the FP64 and FP32 elasticity operator sweeps on this problem's fine grid
  * for each candidate row-chunk size and keeps the fastest; results are written to

@@ -104,6 +104,7 @@ SIGNATURES = {
     "tmx_matvec": (_I, [_P, _P, _P, _P, _I]),
     "tmx_load_vector": (_I, [_P, _P]),
     "tmx_integrate": (_I, [_P, _P, ctypes.POINTER(_D)]),
+    "tmx_integrate_noalloc": (_I, [_I, _I, _D, _D, _P, _P, _P]),
     "tmx_autotune": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "tmx_set_tiles": (_I, [_P, _I, _I]),
     "tmx_set_option": (_I, [_P, _I, _D]),
