@@ -8,7 +8,11 @@ cfg2_n400.json: the same cantilever at N=400 (1200x400 cells, 963,202 DOFs), rho
 Gaussian-smoothed (sigma 5 cells) seeded noise mapped to [1e-3, 1], p=3: J, norms of u and w,
 and 2000 seeded samples of u and w (the full vectors would be 11 MB).
 
-    python tests/golden/make_cfg_golden.py
+cfg3_n1000.json: the same at N=1000 (3000x1000 cells, 6,008,002 DOFs), p=3 (the benchmark's
+workload), solved with SuperLU in the grid's nested-dissection order (the generic orderings
+run out of memory there; the process is capped at 56 GiB of address space).
+
+    python tests/golden/make_cfg_golden.py [cfg1 cfg2 cfg3]
 """
 
 import json
@@ -41,10 +45,12 @@ def cfg1():
     np.savez_compressed(os.path.join(HERE, "cfg1_n20.npz"), **out)
 
 
-def cfg2():
-    N = 400
+def cfg2(N=400, name="cfg2_n400.json", permc_spec=None):
     nx, ny = 3 * N, N
     t0 = time.time()
+    if permc_spec:
+        from oracle.fem import ElasticityOracle
+        ElasticityOracle.permc_spec = permc_spec
     prob = oracle_problem(CANTILEVER, nx, ny)
     prob.set_penalization(3)
     rho = smoothed_density(nx, ny, seed=0, sigma=5.0)
@@ -55,19 +61,28 @@ def cfg2():
     iu = np.sort(rng.choice(u.size, 2000, replace=False))
     iw = np.sort(rng.choice(w.size, 2000, replace=False))
     data = {
-        "workload": "cantilever 3x1, N=400, smoothed density seed 0 sigma 5 in [1e-3,1], p=3",
-        "solver": "oracle P1 FP64, SuperLU direct",
+        "workload": f"cantilever 3x1, N={N}, smoothed density seed 0 sigma 5 in [1e-3,1], p=3",
+        "solver": "oracle P1 FP64, SuperLU direct" + (f" ({permc_spec} order)" if permc_spec
+                                                      else ""),
         "seconds": time.time() - t0,
         "J": J, "u_norm": float(np.linalg.norm(u)), "u_max": float(np.abs(u).max()),
         "w_norm": float(np.linalg.norm(w)), "xi_sum": float(xi.sum()),
         "u_index": iu.tolist(), "u_values": u[iu].tolist(),
         "w_index": iw.tolist(), "w_values": w[iw].tolist(),
     }
-    with open(os.path.join(HERE, "cfg2_n400.json"), "w") as fh:
+    with open(os.path.join(HERE, name), "w") as fh:
         json.dump(data, fh)
-    print(f"cfg2: J={J!r} in {data['seconds']:.1f}s")
+    print(f"{name}: J={J!r} in {data['seconds']:.1f}s")
+
+
+def cfg3():
+    import resource
+    cap = 56 << 30
+    resource.setrlimit(resource.RLIMIT_AS, (cap, cap))
+    cfg2(N=1000, name="cfg3_n1000.json", permc_spec="ND")
 
 
 if __name__ == "__main__":
-    cfg1()
-    cfg2()
+    which = sys.argv[1:] or ["cfg1", "cfg2"]
+    for name in which:
+        {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}[name]()

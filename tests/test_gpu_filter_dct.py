@@ -45,17 +45,30 @@ def a_prime(z, nx, ny, hx, hy, eps):
 
 
 # relative residual of A' z = r: FP32 round-off of the transforms and the reduction, and the
-# same in FP64
+# same in FP64.  The residual bound is the round-off times the condition of A' along y, about
+# 4 (eps / hy)^2 (the y Laplacian against the mass), so FP32 gets at least 6e-8 times that
+# (1e-4 up to ny = 1000, 2.5e-3 at ny = 2048 with eps = 0.05)
 TOL = {"fp32": 1e-4, "fp64": 1e-11}
+
+
+def tolerance(precision, ny):
+    if precision == "fp64":
+        return TOL["fp64"]
+    kappa = 4.0 * (CANTILEVER["filter_radius"] * ny / CANTILEVER["height"]) ** 2
+    return max(TOL["fp32"], 6e-8 * kappa)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("nx,ny", [(60, 20), (120, 40), (250, 90), (20, 10), (300, 100), (15, 8),
-                                   (1, 200), (3071, 4), (2047, 6)])
+                                   (1, 200), (3071, 4), (2047, 6), (45, 15), (63, 21), (140, 70),
+                                   (3, 2048), (96, 175)])
 def test_dct_solves_lumped_operator(nx, ny, precision):
     """Grids of more than 128 nodes (smaller filter grids are solved densely, exactly).  x rows
     of 2 nodes (one row per thread block), 3072 (the longest: 12 rows in each of the 256
-    blocks) and 2048 (8 rows per block), and rows whose last block is short."""
+    blocks) and 2048 (8 rows per block), and rows whose last block is short.  The y transform
+    is an FFT of 2 ny points: radix plans 4-4-5 (ny = 20) .. with radix 2, 3 (ny = 90, 15),
+    5 and 7 (ny = 21, 70, 175), odd ny, the longest column (ny = 2048), and odd column counts
+    (nx + 1 odd: a workgroup's second column is outside the grid)."""
     gp = gpu_problem(CANTILEVER, nx, ny)
     gp.set_penalization(3)
     gp.set_option("filter_dct", 1)
@@ -67,14 +80,15 @@ def test_dct_solves_lumped_operator(nx, ny, precision):
     err = float(torch.linalg.norm(back - r) / torch.linalg.norm(r))
     print(nx, ny, precision, err, flush=True)
     assert torch.isfinite(z).all()
-    assert err < TOL[precision]
+    assert err < tolerance(precision, ny)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-@pytest.mark.parametrize("nx,ny", [(45, 15), (3072, 2)])
+@pytest.mark.parametrize("nx,ny", [(44, 22), (3072, 2), (8, 2100)])
 def test_unsupported_grid_keeps_the_v_cycle(precision, nx, ny):
-    """ny odd, or x rows longer than 3072 nodes: the preconditioner stays the V-cycle (the same
-    output with the option on or off)."""
+    """2 ny with a prime factor above 7, x rows longer than 3072 nodes, or columns longer than
+    2049 nodes: the preconditioner stays the V-cycle (the same output with the option on or
+    off)."""
     gp = gpu_problem(CANTILEVER, nx, ny)
     gp.set_penalization(3)
     r = torch.randn((nx + 1) * (ny + 1), dtype=torch.float64).cuda()

@@ -60,8 +60,7 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
     while procs:
         _wait(procs.pop(0))
     tmp = LIB + ".tmp"
-    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs,
-            "-lrocblas"]
+    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     if verbose:
         print(" ".join(link), flush=True)
     _run(link)

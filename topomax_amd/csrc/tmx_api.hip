@@ -785,7 +785,7 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
   // is latency-bound, so the best height is the smallest whose workgroups still fit the
   // device at once, which depends on the grid)
   TMX_TRY(h->el.fine_cell_weights(s));
-  const int pair_cands[] = {2, 3, 4, 6, 8, 12, 16, 24};
+  const int pair_cands[] = {2, 3, 4, 6, 8, 12, 16, 24, 32, 48};
   for (int prec = 0; prec < 2; ++prec) {
     for (int kind = 0; kind < 3; ++kind) {
       if (kind >= 1 && !h->el.pair_post_ok()) continue;

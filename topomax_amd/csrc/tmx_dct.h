@@ -9,42 +9,60 @@
 //   A' = W_y (x) B_x + T_y (x) C_x,  B_x = eps^2 (hy/hx) T_x + hx hy W_x,  C_x = eps^2 (hx/hy) W_x,
 // which the DCT-I in y diagonalises: T_y v_m = lambda_m W_y v_m with v_m[j] = cos(pi j m / n),
 // lambda_m = 2 - 2 cos(pi m / n), v_m' W_y v_k = D_m delta_mk (D_m = n / 2, or n for m = 0, n).
-// So A'^-1 r = sum_m v_m (x) (B_x + lambda_m C_x)^-1 (v_m' r) / D_m: a DCT-I along y (folded
-// into even and odd modes, two GEMMs of half size each way), one tridiagonal solve along x per
-// mode (partition method: block eliminations, cyclic reduction of the blocks' interface
-// system in LDS), and the inverse DCT-I.  A'^-1 A has its spectrum in
-// [0.95, 1.003] for eps/h = 2 and closer to 1 as eps/h grows (the mass is a small part of the
-// operator there), so PCG needs 3-5 iterations to 1e-9 whatever the grid.
+// So A'^-1 r = sum_m v_m (x) (B_x + lambda_m C_x)^-1 (v_m' r) / D_m: a DCT-I along y, one
+// tridiagonal solve along x per mode (partition method: block eliminations, cyclic reduction
+// of the blocks' interface system in LDS), and the same DCT-I again (the matrix
+// C[j, m] = cos(pi j m / n) is symmetric).  A'^-1 A has its spectrum in [0.95, 1.003] for
+// eps/h = 2 and closer to 1 as eps/h grows (the mass is a small part of the operator there),
+// so PCG needs 3-5 iterations to 1e-9 whatever the grid.
+//
+// The DCT-I of the n + 1 values of a y-column is computed from a complex FFT of length 2n in
+// LDS (k_dct1_cols): two columns a, b are packed as a + i b and extended evenly
+// (z[2n - j] = z[j]); the DFT of a real even sequence is real, so the real and imaginary parts
+// of the FFT are the two columns' transforms, up to the end-point terms
+//   T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2,   Y = DFT_2n of the even extension.
+// The FFT is a mixed-radix Stockham transform (radices 2, 3, 4, 5, 7), so 2n must have no
+// prime factor above 7; other grids keep the multigrid V-cycle.
 #pragma once
-
-#include <rocblas/rocblas.h>
 
 #include "tmx_kernels.h"
 #include "tmx_solver.h"
 
 namespace tmx {
 
+// radix plan of one FFT length (stages applied in order)
+struct FftPlan {
+  static constexpr int kMaxStages = 16;
+  int L = 0;
+  int nstages = 0;
+  int radix[kMaxStages] = {};
+};
+
+// the radix plan of length L from radices {4, 2, 3, 5, 7}; false when L has a larger prime
+bool fft_plan(int L, FftPlan& plan);
+
 template <typename T>
 class DctPrecond {
  public:
-  // nx, ny: cells; ny even, nx + 1 <= kMaxRow; eps: the filter's length scale
+  // nx, ny: cells; 2 ny factors into 2, 3, 5, 7, nx + 1 <= kMaxRow, and the FFT of 2 ny
+  // points fits the LDS budget; eps: the filter's length scale
   static bool supported(int nx, int ny);
   int init(int nx, int ny, double hx, double hy, double eps, hipStream_t s);
   // z = A'^-1 r (vectors of (nx + 1)(ny + 1) values), partials of r.z into rz if given
   int apply(const T* r, T* z, RedSlot* rz, hipStream_t s);
-  ~DctPrecond();
 
   // longest x row: one workgroup of 256 threads solves it, 12 rows per thread, with the row
   // and two coefficient arrays in LDS (40 KB in FP32, 80 KB in FP64 at this length)
   static constexpr int kMaxRow = 3072;
+  // y-columns per FFT workgroup (two per complex sequence of 2 ny points; ny <= 2048)
+  static constexpr int kCols = 2;
 
  private:
-  int nx_ = 0, ny_ = 0, h_ = 0, mx_ = 0;
+  int nx_ = 0, ny_ = 0, mx_ = 0;
   double hx_ = 0, hy_ = 0, eps_ = 0;
-  DevBuf<T> c_;  // folded DCT-I matrices Ce, Co, Ce, Co^T, column-major (h+1)^2 each
-  DevBuf<T> f_;  // folded sums then differences (mx x h+1 each), then the inverse transforms
-  DevBuf<T> t_;  // transformed even then odd modes (mx x h+1 each)
-  rocblas_handle blas_ = nullptr;
+  FftPlan plan_;
+  DevBuf<T> tw_;  // twiddles exp(-2 pi i k / 2n), k < 2n, as (re, im) pairs
+  DevBuf<T> t_;   // transformed modes: row m = mode m (n + 1 rows of nx + 1 values)
 };
 
 }  // namespace tmx

@@ -1,9 +1,9 @@
 // DCT-I / tridiagonal solver of the filter's tensor-lumped operator (see tmx_dct.h).
 //
 // Data layout: vectors are node-major rows, v = iy (nx + 1) + ix, so a y-column is strided and a
-// mode row (fixed y-mode, all x) is contiguous.  Seen column-major, a block of rows is an
-// (nx + 1) x rows matrix with leading dimension nx + 1, and the y transforms are plain GEMMs
-// with the folded DCT-I matrices on the right.
+// mode row (fixed y-mode, all x) is contiguous.  The transforms run on y-columns held in LDS
+// (k_dct1_cols: a group of kCols adjacent columns per workgroup, every row read and written as
+// one short contiguous segment); the tridiagonal solves run on mode rows (k_dct_tri).
 #include "tmx_dct.h"
 
 #include <cmath>
@@ -13,64 +13,245 @@
 
 namespace tmx {
 
-// s[j] = r[j] + r[n - j] (j < h), s[h] = r[h]; d[j] = r[j] - r[n - j] (j < h), d[h] = 0; rows
-// of mx values.  Row h of d is the odd half's zero padding: it is rewritten on every call, so a
-// non-finite value that an earlier call left in the buffer cannot reach this one's transforms
+bool fft_plan(int L, FftPlan& plan) {
+  plan = FftPlan();
+  plan.L = L;
+  if (L < 2) return false;
+  int m = L;
+  auto push = [&](int r) {
+    if (plan.nstages >= FftPlan::kMaxStages) return false;
+    plan.radix[plan.nstages++] = r;
+    m /= r;
+    return true;
+  };
+  while (m % 4 == 0)
+    if (!push(4)) return false;
+  while (m % 2 == 0)
+    if (!push(2)) return false;
+  for (int r : {3, 5, 7})
+    while (m % r == 0)
+      if (!push(r)) return false;
+  return m == 1;
+}
+
+// ---------------------------------------------------------------- complex FFT in LDS
+constexpr int kFftThreads = 256;
+constexpr int kFftMaxPoints = 4096;  // points of the complex sequences of one workgroup
+
 template <typename T>
-__global__ void k_dct_fold(int mx, int n, const T* __restrict__ r, T* __restrict__ s,
-                           T* __restrict__ d) {
-  const int h = n / 2;
-  const size_t total = (size_t)(h + 1) * mx;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
-    const int j = (int)(t / mx), ix = (int)(t - (size_t)j * mx);
-    const T a = r[(size_t)j * mx + ix];
-    if (j < h) {
-      const T b = r[(size_t)(n - j) * mx + ix];
-      s[t] = a + b;
-      d[t] = a - b;
-    } else {
-      s[t] = a;
-      d[t] = T(0);
+__device__ __forceinline__ V2<T> cmul(V2<T> a, V2<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// cos and sin of 2 pi q / R, q = 0 .. R-1, for the odd radices
+template <int R>
+struct RootsOdd;
+template <>
+struct RootsOdd<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+};
+template <>
+struct RootsOdd<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
+                                  -0.80901699437494742410, 0.30901699437494742410};
+  static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
+                                  -0.58778525229247312917, -0.95105651629515357212};
+};
+template <>
+struct RootsOdd<7> {
+  static constexpr double c[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429,
+                                  -0.90096886790241912624, -0.90096886790241912624,
+                                  -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                                  0.43388373911755812048, -0.43388373911755812048,
+                                  -0.97492791218182360702, -0.78183148246802980871};
+};
+
+// forward DFT of R points in place: y_k = sum_j v_j exp(-2 pi i j k / R)
+template <typename T, int R>
+__device__ __forceinline__ void dft(V2<T> (&v)[R]) {
+  if constexpr (R == 2) {
+    const V2<T> a = v[0];
+    v[0] = a + v[1];
+    v[1] = a - v[1];
+  } else if constexpr (R == 4) {
+    const V2<T> s02 = v[0] + v[2], d02 = v[0] - v[2], s13 = v[1] + v[3], d13 = v[1] - v[3];
+    v[0] = s02 + s13;
+    v[2] = s02 - s13;
+    v[1] = {d02.x + d13.y, d02.y - d13.x};  // d02 - i d13
+    v[3] = {d02.x - d13.y, d02.y + d13.x};  // d02 + i d13
+  } else {
+    // odd R: pair v_k with v_{R-k}; y_m = M_m - i N_m, y_{R-m} = M_m + i N_m with
+    // M_m = v_0 + sum_k cos(2 pi m k / R) (v_k + v_{R-k}), N_m = sum_k sin(2 pi m k / R) (v_k - v_{R-k})
+    constexpr int H = R / 2;
+    V2<T> t[H + 1], u[H + 1];
+    V2<T> y0 = v[0];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      t[k] = v[k] + v[R - k];
+      u[k] = v[k] - v[R - k];
+      y0 = y0 + t[k];
     }
+    V2<T> out[R];
+    out[0] = y0;
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      V2<T> M = v[0], N = {T(0), T(0)};
+#pragma unroll
+      for (int k = 1; k <= H; ++k) {
+        const T c = (T)RootsOdd<R>::c[(m * k) % R], sn = (T)RootsOdd<R>::s[(m * k) % R];
+        M = {M.x + c * t[k].x, M.y + c * t[k].y};
+        N = {N.x + sn * u[k].x, N.y + sn * u[k].y};
+      }
+      out[m] = {M.x + N.y, M.y - N.x};
+      out[R - m] = {M.x - N.y, M.y + N.x};
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
   }
 }
 
-// z[j] = e[j] + o[j], z[n - j] = e[j] - o[j] (j < h), z[h] = e[h]; partials of r.z
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dct_unfold(int mx, int n, const T* __restrict__ e,
-                                                       const T* __restrict__ o,
-                                                       const T* __restrict__ r,
-                                                       T* __restrict__ z,
-                                                       double* __restrict__ part) {
-  const int h = n / 2;
-  const size_t total = (size_t)(h + 1) * mx;
+// q = floor(j / d) for 0 <= j, d <= 8192 from a float reciprocal and one correction each way
+__device__ __forceinline__ int div_fft(int j, int d, float dinv) {
+  int q = (int)((float)j * dinv);
+  if ((q + 1) * d <= j) ++q;
+  if (q * d > j) --q;
+  return q;
+}
+
+// One Stockham radix-R stage over nseq sequences of L points in LDS, Ns = product of the
+// radices of the stages before it: butterfly j of a sequence takes the points j + r L/R,
+// twiddles them by exp(-2 pi i (j mod Ns) r / (Ns R)), transforms them and writes them to
+// (j div Ns) Ns R + (j mod Ns) + r Ns.  Every thread first reads and transforms all its
+// butterflies into registers, then (after a barrier) writes them, so the stage runs in place.
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
+                                          const V2<T>* __restrict__ tw) {
+  constexpr int MAXB = (kFftMaxPoints / R + kFftThreads - 1) / kFftThreads;
+  const int nb = L / R;
+  const int total = nseq * nb;
+  const int span = L / (Ns * R);
+  const float ns_inv = 1.0f / (float)Ns;
+  V2<T> v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = (int)threadIdx.x + q * kFftThreads;
+    dst[q] = -1;
+    if (b < total) {
+      const int f = b >= nb ? 1 : 0;  // at most two sequences per workgroup
+      const int j = b - f * nb;
+      const int jm = j - div_fft(j, Ns, ns_inv) * Ns;
+      const V2<T>* src = buf + f * L + j;
+      v[q][0] = src[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(src[r * nb], tw[jm * r * span]);
+      dft<T, R>(v[q]);
+      dst[q] = f * L + (j - jm) * R + jm;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q)
+    if (dst[q] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[dst[q] + r * Ns] = v[q][r];
+    }
+  __syncthreads();
+}
+
+// DCT-I of the y-columns x0 .. x0 + kCols - 1 (x0 = kCols * group), all n + 1 rows:
+//   dst[m][x] = sum_j src[j][x] cos(pi j m / n),  m = 0 .. n,
+// with (INV) partials of sum rdot . dst per workgroup.  Workgroups b, b + 8, b + 16, .. (one
+// XCD each, as workgroups are dealt out to the XCDs in turn) take consecutive column groups,
+// so the workgroups that share a row's cache line share that XCD's L2.
+template <typename T, bool INV>
+__global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int groups, FftPlan plan,
+                                                          const V2<T>* __restrict__ tw,
+                                                          const T* __restrict__ src,
+                                                          T* __restrict__ dst,
+                                                          const T* __restrict__ rdot,
+                                                          double* __restrict__ part) {
+  constexpr int C = DctPrecond<T>::kCols, NS = C / 2;
+  extern __shared__ __align__(16) unsigned char dct_lds[];
+  V2<T>* z = reinterpret_cast<V2<T>*>(dct_lds);
+  __shared__ V2<T> ends[NS][2];  // each sequence's points 0 and n before the transform
+  const int L = 2 * n;
+  const int per = (int)gridDim.x / 8;
+  const int g = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
   double acc = 0.0;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
-    const int j = (int)(t / mx), ix = (int)(t - (size_t)j * mx);
-    const size_t lo = (size_t)j * mx + ix;
-    if (j < h) {
-      const size_t hi = (size_t)(n - j) * mx + ix;
-      const T zl = e[t] + o[t], zh = e[t] - o[t];
-      z[lo] = zl;
-      z[hi] = zh;
-      acc += (double)r[lo] * (double)zl + (double)r[hi] * (double)zh;
-    } else {
-      const T zl = e[t];
-      z[lo] = zl;
-      acc += (double)r[lo] * (double)zl;
+  if (g < groups) {
+    const int x0 = g * C;
+    const int total = (n + 1) * C;
+    T* zs = reinterpret_cast<T*>(z);
+    // column c is the real (c even) or imaginary part of sequence c / 2; point j is stored
+    // at j and (0 < j < n) at its mirror L - j.  Eight loads in flight per thread.
+    constexpr int NB = 8;
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads) {
+      T v[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = i0 + u * kFftThreads;
+        const int j = i / C, c = i % C;
+        v[u] = (i < total && x0 + c < mx) ? src[(size_t)j * mx + x0 + c] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = i0 + u * kFftThreads;
+        if (i < total) {
+          const int j = i / C, c = i % C;
+          const int base = 2 * (c / 2) * L + (c & 1);
+          zs[base + 2 * j] = v[u];
+          if (j > 0 && j < n) zs[base + 2 * (L - j)] = v[u];
+        }
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NS) {
+      ends[threadIdx.x][0] = z[threadIdx.x * L];
+      ends[threadIdx.x][1] = z[threadIdx.x * L + n];
+    }
+    // (the first stage's barrier orders these reads before any write of the transform)
+    int Ns = 1;
+    for (int st = 0; st < plan.nstages; ++st) {
+      const int R = plan.radix[st];
+      switch (R) {
+        case 2: fft_stage<T, 2>(z, L, NS, Ns, tw); break;
+        case 3: fft_stage<T, 3>(z, L, NS, Ns, tw); break;
+        case 4: fft_stage<T, 4>(z, L, NS, Ns, tw); break;
+        case 5: fft_stage<T, 5>(z, L, NS, Ns, tw); break;
+        default: fft_stage<T, 7>(z, L, NS, Ns, tw); break;
+      }
+      Ns *= R;
+    }
+    // T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2
+    for (int i = (int)threadIdx.x; i < total; i += kFftThreads) {
+      const int m = i / C, c = i % C, x = x0 + c;
+      if (x < mx) {
+        const int f = c / 2;
+        const V2<T> Y = z[f * L + m];
+        const T y = (c & 1) ? Y.y : Y.x;
+        const T r0 = (c & 1) ? ends[f][0].y : ends[f][0].x;
+        const T rn = (c & 1) ? ends[f][1].y : ends[f][1].x;
+        const T val = T(0.5) * (y + r0 + ((m & 1) ? -rn : rn));
+        const size_t o = (size_t)m * mx + x;
+        dst[o] = val;
+        if constexpr (INV) acc += (double)rdot[o] * (double)val;
+      }
     }
   }
-  if (part) {
-    const double v[1] = {acc};
-    double* const out[1] = {part};
-    block_partials<1>(v, out);
+  if constexpr (INV) {
+    if (part) {
+      const double v[1] = {acc};
+      double* const out[1] = {part};
+      block_partials<1>(v, out);
+    }
   }
 }
 
-// One tridiagonal solve along x per mode row: rows 0..h of `even` are the modes 0, 2, .., n,
-// rows 0..h-1 of `odd` the modes 1, 3, .., n-1 (workgroups h+1 ..).  Each solves
+// One tridiagonal solve along x per mode row: row m of `modes` is mode m (workgroup m).  Each
+// solves
 // (a T_x + c_m W_x) u = f / D_m in place, one workgroup of kTriThreads per mode, by the
 // partition method: thread t owns the k = ceil(mx / kTriThreads) consecutive rows s..e of one
 // block.  A downward elimination writes each of its rows as
@@ -97,8 +278,7 @@ static size_t dct_tri_lds(int mx) {
 
 template <typename T>
 __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off, double hxhy,
-                                                         double cy, T* __restrict__ even,
-                                                         T* __restrict__ odd) {
+                                                         double cy, T* __restrict__ modes) {
   extern __shared__ __align__(16) unsigned char tri_lds[];
   const int k = (mx + kTriThreads - 1) / kTriThreads;
   const int nblk = (mx + k - 1) / k;
@@ -109,10 +289,8 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
   T* rb = ra + kTriThreads;                 // rows' P, A, G
   T* rc = rb + kTriThreads;
   T* rf = rc + kTriThreads;
-  const int h = n / 2;
-  const int blk = (int)blockIdx.x;
-  const int mode = blk <= h ? 2 * blk : 2 * (blk - h - 1) + 1;
-  T* row = blk <= h ? even + (size_t)blk * mx : odd + (size_t)(blk - h - 1) * mx;
+  const int mode = (int)blockIdx.x;
+  T* row = modes + (size_t)mode * mx;
   const double lam = 2.0 - 2.0 * std::cos(M_PI * (double)mode / (double)n);
   const T c = (T)(hxhy + cy * lam);
   const T inv_d = (mode == 0 || mode == n) ? T(1) / (T)n : T(2) / (T)n;
@@ -215,125 +393,77 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
 
 template <typename T>
 bool DctPrecond<T>::supported(int nx, int ny) {
-  return ny >= 2 && !(ny & 1) && nx >= 1 && nx + 1 <= kMaxRow;
-}
-
-template <typename T>
-DctPrecond<T>::~DctPrecond() {
-  if (blas_) (void)rocblas_destroy_handle(blas_);
-}
-
-// two plain GEMMs in the element type in one call: C_i (m x n) = A_i op(B_i), column-major,
-// the second operands at the given strides
-static rocblas_status gemm2(rocblas_handle b, rocblas_operation tb, int m, int n, int k,
-                            const float* A, int lda, long sa, const float* B, int ldb, long sb,
-                            float* C, int ldc, long sc) {
-  const float one = 1.0f, zero = 0.0f;
-  return rocblas_sgemm_strided_batched(b, rocblas_operation_none, tb, m, n, k, &one, A, lda, sa,
-                                       B, ldb, sb, &zero, C, ldc, sc, 2);
-}
-static rocblas_status gemm2(rocblas_handle b, rocblas_operation tb, int m, int n, int k,
-                            const double* A, int lda, long sa, const double* B, int ldb, long sb,
-                            double* C, int ldc, long sc) {
-  const double one = 1.0, zero = 0.0;
-  return rocblas_dgemm_strided_batched(b, rocblas_operation_none, tb, m, n, k, &one, A, lda, sa,
-                                       B, ldb, sb, &zero, C, ldc, sc, 2);
-}
-
-static int blas_check(rocblas_status st, const char* what) {
-  if (st == rocblas_status_success) return 0;
-  set_error(std::string("rocBLAS ") + what + " failed: " + rocblas_status_to_string(st));
-  return -1;
+  FftPlan p;
+  return ny >= 1 && nx >= 1 && nx + 1 <= kMaxRow && fft_plan(2 * ny, p) &&
+         (kCols / 2) * 2 * ny <= kFftMaxPoints;
 }
 
 template <typename T>
 int DctPrecond<T>::init(int nx, int ny, double hx, double hy, double eps, hipStream_t s) {
   if (!supported(nx, ny)) {
-    set_error("internal: DCT preconditioner needs an even ny and nx + 1 <= " +
+    set_error("internal: DCT preconditioner needs 2 ny with prime factors <= 7, ny <= " +
+              std::to_string(kFftMaxPoints / kCols) + " and nx + 1 <= " +
               std::to_string(kMaxRow));
     return -1;
   }
   nx_ = nx;
   ny_ = ny;
-  h_ = ny / 2;
   mx_ = nx + 1;
   hx_ = hx;
   hy_ = hy;
   eps_ = eps;
-  const int h = h_, n = ny;
-  const size_t hh = (size_t)(h + 1) * (h + 1);
-  // column-major (h+1) x (h+1) blocks: even block Ce[j + m (h+1)] = cos(pi j 2m / n)
-  // (symmetric); odd block Co[j + m (h+1)] = cos(pi j (2m + 1) / n) for j, m < h, zero in row
-  // and column h, so that the odd half runs as a GEMM of the even half's shape.  Stored as
-  // [Ce, Co, Ce, Co^T]: the first pair serves the forward transform with B untransposed and
-  // the inverse with B transposed, the second pair the other way round
-  std::vector<T> cmat(4 * hh, T(0));
-  for (int m = 0; m <= h; ++m)
-    for (int j = 0; j <= h; ++j) {
-      const T v = (T)std::cos(M_PI * (double)((long)j * 2 * m % (2 * n)) / (double)n);
-      cmat[(size_t)j + (size_t)m * (h + 1)] = v;
-      cmat[2 * hh + (size_t)j + (size_t)m * (h + 1)] = v;
-    }
-  for (int m = 0; m < h; ++m)
-    for (int j = 0; j < h; ++j) {
-      const T v = (T)std::cos(M_PI * (double)((long)j * (2 * m + 1) % (2 * n)) / (double)n);
-      cmat[hh + (size_t)j + (size_t)m * (h + 1)] = v;
-      cmat[3 * hh + (size_t)m + (size_t)j * (h + 1)] = v;
-    }
-  TMX_TRY(c_.alloc(cmat.size()));
-  TMX_TRY(f_.alloc(2 * (size_t)(h + 1) * mx_));
-  TMX_TRY(t_.alloc(2 * (size_t)(h + 1) * mx_));
-  TMX_HIP(hipMemcpyAsync(c_.get(), cmat.data(), cmat.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  // (k_dct_fold rewrites the odd half's zero padding on every application)
-  TMX_HIP(hipMemsetAsync(f_.get(), 0, f_.size() * sizeof(T), s));
-  TMX_HIP(hipMemsetAsync(t_.get(), 0, t_.size() * sizeof(T), s));
+  const int L = 2 * ny;
+  (void)fft_plan(L, plan_);
+  // twiddles exp(-2 pi i k / L) from the exact integer angle, evaluated in double
+  std::vector<T> tw(2 * (size_t)L);
+  for (int k = 0; k < L; ++k) {
+    const double a = 2.0 * M_PI * (double)k / (double)L;
+    tw[2 * (size_t)k] = (T)std::cos(a);
+    tw[2 * (size_t)k + 1] = (T)-std::sin(a);
+  }
+  TMX_TRY(tw_.alloc(tw.size()));
+  TMX_TRY(t_.alloc((size_t)(ny + 1) * mx_));
+  TMX_HIP(hipMemcpyAsync(tw_.get(), tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice, s));
   TMX_HIP(hipStreamSynchronize(s));
   // rows up to 3072 nodes need more than 64 KB of dynamic LDS in FP64
   TMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dct_tri<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)dct_tri_lds<T>(kMaxRow)));
-  if (!blas_) TMX_TRY(blas_check(rocblas_create_handle(&blas_), "create_handle"));
+  for (const void* f : {reinterpret_cast<const void*>(&k_dct1_cols<T, false>),
+                        reinterpret_cast<const void*>(&k_dct1_cols<T, true>)})
+    TMX_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(kFftMaxPoints * sizeof(V2<T>))));
   return 0;
 }
 
 template <typename T>
 int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
-  const int h = h_, n = ny_, mx = mx_;
-  const size_t total = (size_t)(h + 1) * mx;
-  const unsigned g = (unsigned)std::min<size_t>(4096, (total + kBlock - 1) / kBlock);
-  const long half = (long)(h + 1) * mx, hh = (long)(h + 1) * (h + 1);
-  T* fe = f_.get();
-  T* fo = fe + half;
-  T* te = t_.get();
-  T* to = te + half;
-  k_dct_fold<T><<<g, kBlock, 0, s>>>(mx, n, r, fe, fo);
+  const int n = ny_, mx = mx_;
+  const int groups = (mx + kCols - 1) / kCols;
+  const int nblk = 8 * ((groups + 7) / 8);
+  const size_t lds = (size_t)(kCols / 2) * 2 * (size_t)n * sizeof(V2<T>);
+  const V2<T>* tw = reinterpret_cast<const V2<T>*>(tw_.get());
+  T* t = t_.get();
+  // forward DCT-I along y: t[m] = sum_j r[j] cos(pi j m / n)
+  k_dct1_cols<T, false><<<nblk, kFftThreads, lds, s>>>(n, mx, groups, plan_, tw, r, t, nullptr,
+                                                       nullptr);
   TMX_HIP(hipGetLastError());
-  TMX_TRY(blas_check(rocblas_set_stream(blas_, s), "set_stream"));
-  // rocBLAS's kernels for these shapes on gfx950 (3001 x 501 x 501 at N=1000): with B
-  // transposed 34 us against 44 us in FP32, untransposed 62 us against 72 us in FP64
-  const rocblas_operation op = sizeof(T) == 4 ? rocblas_operation_transpose
-                                              : rocblas_operation_none;
-  const T* c_fwd = op == rocblas_operation_none ? c_.get() : c_.get() + 2 * hh;
-  const T* c_inv = op == rocblas_operation_none ? c_.get() + 2 * hh : c_.get();
-  // forward: T_e = S Ce, T_o = D Co (mx x h+1 each, one batched call)
-  TMX_TRY(blas_check(gemm2(blas_, op, mx, h + 1, h + 1, fe, mx, half, c_fwd, h + 1, hh, te, mx,
-                           half), "gemm"));
-  // tridiagonal solves along x, one workgroup per mode
+  // tridiagonal solves along x, one workgroup per mode (scaled by 1 / D_m there)
   const T a_off = (T)(eps_ * eps_ * hy_ / hx_);
   const double cy = eps_ * eps_ * hx_ / hy_;
-  k_dct_tri<T><<<2 * h + 1, kTriThreads, dct_tri_lds<T>(mx), s>>>(mx, n, a_off, hx_ * hy_, cy,
-                                                                   te, to);
+  k_dct_tri<T><<<n + 1, kTriThreads, dct_tri_lds<T>(mx), s>>>(mx, n, a_off, hx_ * hy_, cy, t);
   TMX_HIP(hipGetLastError());
-  // inverse: E = T_e Ce^T, O = T_o Co^T
-  TMX_TRY(blas_check(gemm2(blas_, op, mx, h + 1, h + 1, te, mx, half, c_inv, h + 1, hh, fe, mx,
-                           half), "gemm"));
-  const unsigned gu = (unsigned)std::min<size_t>(2048, (total + kBlock - 1) / kBlock);
+  // inverse: z[j] = sum_m t[m] cos(pi j m / n), with the partials of r.z
   double* part = nullptr;
   if (rz) {
-    rz->nblocks = (int)gu;
+    if (nblk > kMaxPartials) {
+      set_error("internal: too many DCT workgroups for the reduction slots");
+      return -1;
+    }
+    rz->nblocks = nblk;
     part = rz->partials;
   }
-  k_dct_unfold<T><<<gu, kBlock, 0, s>>>(mx, n, fe, fo, r, z, part);
+  k_dct1_cols<T, true><<<nblk, kFftThreads, lds, s>>>(n, mx, groups, plan_, tw, t, z, r, part);
   TMX_HIP(hipGetLastError());
   return 0;
 }

@@ -113,6 +113,17 @@ struct Diag3 {
   T xx, xy, yy;
 };
 
+// One row's inputs of a column sweep, loaded ahead of the row's turn: nodal xi (or a cell's
+// weights) and a right-hand side / iterate value.  Rows in flight per wave:
+constexpr int kPrefetchRows = 4;
+template <typename T>
+struct RowIn {
+  T z = T(0);
+  V2<T> w = {T(0), T(0)};
+  V2<T> b = {T(0), T(0)};
+  V2<T> x = {T(0), T(0)}, e = {T(0), T(0)}, e1 = {T(0), T(0)};  // iterate, coarse correction
+};
+
 template <typename T, bool XI, int OP>
 __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const T* __restrict__ x,
                                                         const T* __restrict__ b, T* __restrict__ y,
@@ -465,14 +476,58 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
   T wAp = T(0), wBp = T(0);                 // weights of cell row cy - 1
   double dot = 0.0;
 
-  for (; cy < cy_end; ++cy) {
+  // Loads of cell row cy (x + P e and xi of node row cy + 1, the cell weights, b of node row
+  // cy) are issued kPrefetchRows rows ahead from clamped in-grid addresses; validity, fixed
+  // nodes and the interpolation are applied at the row's turn (see k_elast_sweep_pair_rr)
+  const int cn = min(max(c, 0), nx), cc = min(max(c, 0), nx - 1);
+  const int I0 = cn >> 1, ox = cn & 1;
+  auto fetch = [&](int row) {
+    RowIn<T> r;
+    const int nt = min(max(row + 1, 0), ny), nb = min(max(row, 0), ny);
+    const int cr = min(max(row, 0), ny - 1);
+    if constexpr (PS) {
+      const int J0 = nt >> 1, oy = nt & 1;
+      r.x = X[(unsigned)(nt * nvx + cn)];
+      r.e = EC[(size_t)J0 * (nxc + 1) + I0];
+      r.e1 = EC[(size_t)(J0 + oy) * (nxc + 1) + I0 + ox];
+    }
+    if constexpr (XI) r.z = a.xi[(unsigned)(nt * nvx + cn)];
+    else r.w = CW[(unsigned)(cr * nx + cc)];
+    r.b = B[(unsigned)(nb * nvx + cn)];
+    return r;
+  };
+  // x + P e at node row iy from its prefetched values (as ld_x)
+  auto form_x = [&](int iy, const RowIn<T>& in, V& raw) -> V {
+    V v = zero;
+    const bool ok = colv && iy >= 0 && iy <= ny;
+    if (ok) v = in.x;
+    if (ok && !row_fixed(iy)) {
+      V e = in.e;
+      if (ox | (iy & 1)) e = {T(0.5) * (e.x + in.e1.x), T(0.5) * (e.y + in.e1.y)};
+      v += e;
+    }
+    raw = v;
+    if (row_fixed(iy)) v = zero;
+    return v;
+  };
+  RowIn<T> pf[kPrefetchRows];
+  if (wave_ok) {
+#pragma unroll
+    for (int q = 0; q < kPrefetchRows; ++q) pf[q] = fetch(cy_begin + q);
+  }
+  for (int cy0 = cy_begin; cy0 < cy_end; cy0 += kPrefetchRows) {
+#pragma unroll
+   for (int q = 0; q < kPrefetchRows; ++q) {
+    cy = cy0 + q;
+    if (cy >= cy_end) break;
+    const RowIn<T> in = pf[q];
     // ---- first sweep, cell row cy
     V ut = zero, ut_raw = zero, utr = zero;
     if constexpr (PS) {
-      ut = ld_x(cy + 1, ut_raw);
+      ut = form_x(cy + 1, in, ut_raw);
       utr = lane_down(ut);
     }
-    const T zt = ld_xi(cy + 1);
+    const T zt = (XI && colv && cy + 1 >= 0 && cy + 1 <= ny) ? in.z : T(0);
     const T ztr = lane_down(zt);
     const bool cell_ok = cellc && cy >= 0 && cy < ny;
     T wA = T(0), wB = T(0);
@@ -481,9 +536,8 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
         wA = pc_weight(a.m + a.cp * hsym3(zb, zbr, ztr, a.p), a.wmin);
         wB = pc_weight(a.m + a.cp * hsym3(zb, zt, ztr, a.p), a.wmin);
       } else {
-        const V w = CW[(unsigned)(cy * nx + c)];
-        wA = w.x;
-        wB = w.y;
+        wA = in.w.x;
+        wB = in.w.y;
       }
     }
     V acc1t = zero;
@@ -509,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
     V bb = zero, v_raw = zero;
     T inv = T(0);
     if (rowv) {
-      bb = B[(unsigned)(cy * nvx + c)];
+      bb = in.b;
       const V Ax = PS ? (fx ? ub_raw : acc1) : zero;
       const V res = bb - Ax;
       V d;
@@ -562,6 +616,8 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
     ubr = utr;
     zb = zt;
     zbr = ztr;
+    pf[q] = fetch(cy + kPrefetchRows);
+   }
   }
   if (want_dot) {
     const double v[1] = {dot};
@@ -674,9 +730,34 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
   T wApp = T(0), wBpp = T(0);
   V rA = zero, rB = zero;  // residual rows cy - 4, cy - 3
 
-  for (; cy < cy_end; ++cy) {
+  // The loads of cell row cy (xi of node row cy + 1 or the cell weights, b of node row cy) are
+  // issued kPrefetchRows rows before the row is swept, from clamped in-grid addresses (validity
+  // is applied where the values are used), so a wave waits out the memory latency once per
+  // chunk rather than once per row.  The row loop is unrolled by kPrefetchRows so that each
+  // ring slot keeps its registers (a rotating ring would force a wait at every move).
+  const int cn = min(max(c, 0), nx), cc = min(max(c, 0), nx - 1);
+  auto fetch = [&](int row) {
+    RowIn<T> r;
+    const int nt = min(max(row + 1, 0), ny), nb = min(max(row, 0), ny);
+    const int cr = min(max(row, 0), ny - 1);
+    if constexpr (XI) r.z = a.xi[(unsigned)(nt * nvx + cn)];
+    else r.w = CW[(unsigned)(cr * nx + cc)];
+    r.b = B[(unsigned)(nb * nvx + cn)];
+    return r;
+  };
+  RowIn<T> pf[kPrefetchRows];
+  if (wave_ok) {
+#pragma unroll
+    for (int q = 0; q < kPrefetchRows; ++q) pf[q] = fetch(cy_begin + q);
+  }
+  for (int cy0 = cy_begin; cy0 < cy_end; cy0 += kPrefetchRows) {
+#pragma unroll
+   for (int q = 0; q < kPrefetchRows; ++q) {
+    cy = cy0 + q;
+    if (cy >= cy_end) break;
+    const RowIn<T> in = pf[q];
     // ---- first sweep, cell row cy
-    const T zt = ld_xi(cy + 1);
+    const T zt = (XI && colv && cy + 1 >= 0 && cy + 1 <= ny) ? in.z : T(0);
     const T ztr = lane_down(zt);
     const bool cell_ok = cellc && cy >= 0 && cy < ny;
     T wA = T(0), wB = T(0);
@@ -685,9 +766,8 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
         wA = pc_weight(a.m + a.cp * hsym3(zb, zbr, ztr, a.p), a.wmin);
         wB = pc_weight(a.m + a.cp * hsym3(zb, zt, ztr, a.p), a.wmin);
       } else {
-        const V w = CW[(unsigned)(cy * nx + c)];
-        wA = w.x;
-        wB = w.y;
+        wA = in.w.x;
+        wB = in.w.y;
       }
     }
     const T wAl = lane_up(wA), wBl = lane_up(wB);
@@ -703,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
     V bb = zero, v_raw = zero;
     T inv = T(0);
     if (rowv) {
-      bb = B[(unsigned)(cy * nvx + c)];
+      bb = in.b;
       V d;
       if (fx) {
         d = bb;
@@ -794,6 +874,8 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
     d_b = d_t;
     zb = zt;
     zbr = ztr;
+    pf[q] = fetch(cy + kPrefetchRows);
+   }
   }
 }
 
