@@ -11,6 +11,12 @@ by an FP32 multigrid V-cycle, for the elasticity and both filter solves); the FP
 FP32 inner solves inside FP64 iterative refinement ("mixed-ir") are timed beside it.
 
 Prints one JSON line (rank 0, N=1).  `value` = GDOF/s = DOFs / (solve+grad seconds) / 1e9.
+
+Beside it: `roofline` (the PCG direction update + FP64 operator, timed with HIP events around
+each of its launches inside the timed steps), `kernels` (fine-grid kernels, each launch timed
+alone after 512 MiB of other writes), `step_bytes` (the elasticity PCG's algorithmic streams
+per step over the step time), `p2` (the reference's P2 element at N=400, FP64), `batch` (the
+64-design workload) and `cpu_baseline` (the CPU oracle at N=400).
 """
 
 from __future__ import annotations
@@ -116,12 +122,61 @@ def batch_throughput(n_designs: int, steps: int, rtol: float):
             "GDOF_s_batched": ndof / t_batch / 1e9, "speedup": t_serial / t_batch}
 
 
-def cpu_baseline(n_cpu: int, seconds_budget: float = 30.0):
-    """The CPU oracle (numpy/scipy, SuperLU direct solves, single thread) on a bounded sample
-    of the same workload: one objective + gradient of the same cantilever at N=n_cpu."""
+def instep_kernel_us(problem, rho, steps, precision, rtol, kernel=ROOFLINE_KERNEL):
+    """Average duration of `kernel`'s launches inside `steps` full steps, from HIP events the
+    library records around each launch on the handle's stream."""
+    import ctypes
+    from topomax_amd import _lib
+    h = problem._h
+    _lib.check(h.lib.tmx_kernel_timing(h.ptr, kernel, 1))
+    total, count = ctypes.c_double(), ctypes.c_int()
+    try:
+        _lib.check(h.lib.tmx_kernel_timing_read(h.ptr, kernel, ctypes.byref(total),
+                                                ctypes.byref(count)))
+        for _ in range(steps):
+            problem.objective_and_gradient(rho, rtol=rtol, precision=precision)
+        _lib.check(h.lib.tmx_kernel_timing_read(h.ptr, kernel, ctypes.byref(total),
+                                                ctypes.byref(count)))
+    finally:
+        _lib.check(h.lib.tmx_kernel_timing(h.ptr, kernel, 0))
+    return total.value / max(1, count.value), count.value
+
+
+def p2_throughput(N: int, steps: int, rtol: float):
+    """The reference's own element (P2 displacement, FEM_src/elasisity_problem.py:194-196) on
+    the same cantilever at N: FP64 MINRES preconditioned by P2 block Jacobi over the P1
+    multigrid, one objective + gradient per step."""
+    import torch
+    from topomax_amd.workloads import CANTILEVER, problem_from_case, smoothed_density
+    nx, ny = 3 * N, N
+    gp = problem_from_case(CANTILEVER, nx, ny, precision="fp64", rtol=rtol, degree=2)
+    gp.set_penalization(3)
+    rho = torch.as_tensor(smoothed_density(nx, ny, seed=0, sigma=5.0), device="cuda:0")
+    gp.objective_and_gradient(rho)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        J, _, _ = gp.objective_and_gradient(rho)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ndof = 2 * (2 * nx + 1) * (2 * ny + 1)
+    st = gp.last_stats
+    return {"workload": f"cantilever 3x1, N={N}, P2 displacement ({ndof} DOFs), p=3, FP64, "
+                        f"rtol {rtol:g}", "dofs": ndof, "ms_per_step": dt * 1e3,
+            "GDOF_s": ndof / dt / 1e9, "iterations": st["iterations"],
+            "rel_residual": st["rel_residual"], "compliance": J}
+
+
+def cpu_baseline(n_cpu: int):
+    """The CPU oracle (numpy/scipy, SuperLU direct solves in nested-dissection order, single
+    thread) on the same workload at N=n_cpu: one objective + gradient (filter solve, elasticity
+    solve, compliance, sensitivity, filter adjoint solve, M w) of the same cantilever; the
+    assembly of the filter and its factorisation (done once, as the oracle caches it) are not
+    timed, so the CPU figure is favourable to the CPU."""
     from oracle.fem import ElasticityOracle, StructuredMesh
     from topomax_amd.workloads import CANTILEVER as C, smoothed_density
     nx, ny = 3 * n_cpu, n_cpu
+    ElasticityOracle.permc_spec = "ND"
     orc = ElasticityOracle(StructuredMesh(C["width"], C["height"], nx, ny),
                            young_modulus=C["young_modulus"], poisson_ratio=C["poisson_ratio"],
                            filter_radius=C["filter_radius"], fixed_sides=C["fixed_sides"],
@@ -139,9 +194,10 @@ def cpu_baseline(n_cpu: int, seconds_budget: float = 30.0):
         "unit": "GDOF/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"CPU oracle (numpy/scipy SuperLU, single-threaded, not FEniCS) P1 FP64 "
-                   f"filter+solve+compliance+gradient of the same cantilever at N={n_cpu} "
-                   f"({ndof} DOFs): {dt:.2f} s per step on {cpu_model()} "
+        "sample": (f"CPU oracle (numpy/scipy SuperLU in nested-dissection order, "
+                   f"single-threaded, not FEniCS) P1 FP64 filter+solve+compliance+gradient of "
+                   f"the same cantilever at N={n_cpu} ({ndof} DOFs; the N=400 configuration of "
+                   f"the workload): {dt:.2f} s per step on {cpu_model()} "
                    f"({len(os.sched_getaffinity(0))} cores visible)"),
     }
 
@@ -154,10 +210,12 @@ def main():
     ap.add_argument("--N", type=int, default=1000)
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--cpu-N", type=int, default=300)
+    ap.add_argument("--cpu-N", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-p2", action="store_true")
+    ap.add_argument("--p2-N", type=int, default=400)
     ap.add_argument("--batch-designs", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="set up the N operator, then launch only the roofline kernel (PCG "
@@ -195,16 +253,22 @@ def main():
 
     t_mixed, J_mixed, w_mixed, st_mixed = time_steps(problem, rho, args.steps, args.warmup,
                                                      "mixed", args.rtol)
+    # the roofline kernel's launches inside the timed steps (events around each launch)
+    au_us, au_launches = instep_kernel_us(problem, rho, args.steps, "mixed", args.rtol)
+    # fine-grid kernels, each launch alone after 512 MiB of other writes (cold caches)
     kernels = {}
+    problem.set_option("bench_cold", 1)
     for name, kid, prec in (("apply_fp32", 0, "fp32"), ("smooth_fp32", 1, "fp32"),
                             ("apply_fp64", 0, "fp64"), ("smooth_fp64", 1, "fp64"),
                             ("sweep_pair_restrict_fp32", 6, "fp32"),
                             ("sweep_pair_prolong_fp32", 4, "fp32"),
                             ("update_apply_fp64", ROOFLINE_KERNEL, "fp64")):
-        us, nbytes = bench_kernel(problem, kid, prec)
+        us, nbytes = bench_kernel(problem, kid, prec, reps=20)
         kernels[name] = {"us": us, "bytes": nbytes, "GBs": nbytes / us / 1e3}
+    problem.set_option("bench_cold", 0)
+    b2b_us, _ = bench_kernel(problem, ROOFLINE_KERNEL, "fp64")
 
-    result_fp64 = result_ir = None
+    result_fp64 = result_ir = result_ru = None
     if not args.no_fp64:
         t64, J64, w64, st64 = time_steps(problem, rho, max(2, args.steps // 2), 1, "fp64",
                                          args.rtol)
@@ -224,16 +288,46 @@ def main():
                      "compliance_rel_err_vs_fp64": abs(Jir - J64) / abs(J64),
                      "gradient_rel_err_vs_fp64":
                          float(torch.linalg.norm(wir - w64) / torch.linalg.norm(w64))}
+        tru, Jru, wru, stru = time_steps(problem, rho, max(2, args.steps // 2), 1, "mixed-ru",
+                                         args.rtol)
+        result_ru = {"ms_per_step": tru * 1e3, "GDOF_s": ndof / tru / 1e9,
+                     "iterations": stru[-1]["iterations"],
+                     "outer_iterations": stru[-1]["outer_iterations"],
+                     "rel_residual": stru[-1]["rel_residual"],
+                     "speedup_over_fp64": t64 / tru,
+                     "compliance_rel_err_vs_fp64": abs(Jru - J64) / abs(J64),
+                     "gradient_rel_err_vs_fp64":
+                         float(torch.linalg.norm(wru - w64) / torch.linalg.norm(w64))}
 
     # the largest elasticity kernel of the mixed step: the PCG direction update fused with the
-    # FP64 operator application (one launch per iteration)
-    dom = kernels["update_apply_fp64"]
+    # FP64 operator application (one launch per iteration), timed inside the steps
+    au_bytes = kernels["update_apply_fp64"]["bytes"]
+    achieved = au_bytes / au_us / 1e3
     roofline = {"kernel": "k_elast_sweep<double,true,kApplyUpdate> (PCG direction update "
                           "p = z + beta p_old with q = K(xi) p in FP64, p.q partials and the "
                           "deferred x += alpha p_old)",
-                "bound": "hbm", "achieved": dom["GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": dom["GBs"] / PEAK_HBM_GBS, "traffic": PMC_TRAFFIC_BYTES.get(N),
-                "algorithmic_bytes_per_launch": dom["bytes"], "avg_launch_us": dom["us"]}
+                "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": PMC_TRAFFIC_BYTES.get(N),
+                "algorithmic_bytes_per_launch": au_bytes, "avg_launch_us": au_us,
+                "timing": f"HIP events around each of its {au_launches} launches inside "
+                          f"{args.steps} timed mixed steps",
+                "back_to_back_us": b2b_us,
+                "cold_us": kernels["update_apply_fp64"]["us"]}
+    # whole step: the elasticity PCG's algorithmic streams per iteration (direction update +
+    # operator 96 B, x/r update 56 B, the finest V-cycle level's two paired launches 26 + 34 B
+    # per vertex) over the step time; the filter solves and the coarse levels are left out, so
+    # this is a lower bound of the step's algorithmic bytes
+    nv = (nx + 1) * (ny + 1)
+    its = st_mixed[-1]["iterations"]
+    step_bytes = its * (96 + 56 + 26 + 34) * nv
+    step = {"bytes": step_bytes, "GBs": step_bytes / t_mixed / 1e9,
+            "frac": step_bytes / t_mixed / 1e9 / PEAK_HBM_GBS,
+            "streams": f"{its} PCG iterations x 212 B per vertex (direction update + operator, "
+                       "x/r update, finest-level paired V-cycle launches)"}
+
+    p2 = None
+    if not args.no_p2:
+        p2 = p2_throughput(args.p2_N, max(2, args.steps // 5), args.rtol)
 
     batch = None
     if not args.no_batch:
@@ -267,7 +361,10 @@ def main():
         "compliance": J_mixed,
         "fp64": result_fp64,
         "mixed_ir": result_ir,
+        "mixed_ru": result_ru,
         "batch": batch,
+        "p2": p2,
+        "step_bytes": step,
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

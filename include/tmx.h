@@ -45,10 +45,15 @@ enum tmx_side { TMX_LEFT = 0, TMX_RIGHT = 1, TMX_TOP = 2, TMX_BOTTOM = 3 };
 enum tmx_side_bits { TMX_FIX_LEFT = 1, TMX_FIX_RIGHT = 2, TMX_FIX_TOP = 4, TMX_FIX_BOTTOM = 8 };
 /* Solve schemes.  TMX_FP64: FP64 multigrid-PCG.  TMX_MIXED: FP64 PCG (FP64 operator, residual
  * and iterates) preconditioned by an FP32 V-cycle.  TMX_MIXED_IR: FP32 multigrid-PCG inner
- * solves inside FP64 iterative refinement.  TMX_FP32 names single-precision kernels
- * (tmx_matvec, tmx_precondition, tmx_bench_kernel); as a solve precision it means TMX_MIXED.
- * Filter solves run in FP64 PCG with an FP32 V-cycle unless TMX_FP64 is asked for. */
-enum tmx_precision { TMX_FP64 = 0, TMX_MIXED = 1, TMX_FP32 = 2, TMX_MIXED_IR = 3 };
+ * solves inside FP64 iterative refinement.  TMX_MIXED_RU: one FP32 multigrid-PCG (FP32
+ * operator, Krylov vectors and V-cycle) with the iterate accumulated in FP64 and the FP32
+ * residual replaced by the true FP64 one b - A x every few iterations (reliable updates; the
+ * search direction is kept), finished when the FP64 residual meets the tolerance.  TMX_FP32
+ * names single-precision kernels (tmx_matvec, tmx_precondition, tmx_bench_kernel); as a solve
+ * precision it means TMX_MIXED.  Filter solves run in FP64 PCG with an FP32 V-cycle unless
+ * TMX_FP64 is asked for. */
+enum tmx_precision { TMX_FP64 = 0, TMX_MIXED = 1, TMX_FP32 = 2, TMX_MIXED_IR = 3,
+                     TMX_MIXED_RU = 4 };
 
 #define TMX_MAX_TRACTIONS 16
 
@@ -178,6 +183,14 @@ int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out);
  * precision TMX_FP64 or TMX_FP32.  Writes the per-launch algorithmic bytes to *bytes. */
 int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps, double* avg_us,
                      double* bytes);
+/* In-step timing (measurement only; no counterpart in the reference).  With `enable` set,
+ * every launch of kernel 5 of tmx_bench_kernel (the PCG direction update with the FP64
+ * operator, one per elasticity PCG iteration) inside later solves is bracketed by HIP events
+ * on the handle's stream; tmx_kernel_timing_read synchronises them and returns the summed
+ * duration (microseconds) and the count of launches timed since the last read (at most 256;
+ * later ones are not timed until a read).  Timing stays off under TMX_OPT_GRAPHS. */
+int tmx_kernel_timing(tmx_problem* h, int kernel, int enable);
+int tmx_kernel_timing_read(tmx_problem* h, int kernel, double* total_us, int* launches);
 /* Solver options (both operator families).  TMX_OPT_FUSED: 1 (default) fuses residual with
  * restriction and prolongation with the next smoothing sweep on element levels; TMX_OPT_TAIL:
  * 0 off, 1 on (default), > 1 on with that many nodes as the largest level run by the
@@ -232,9 +245,13 @@ enum tmx_option {
                                    paired levels (0: one launch per sweep there) */
   TMX_OPT_PAIR_NU_COARSE = 24,  /* sweeps each way on the coarser paired levels (0: the
                                    coarse-level count, TMX_OPT_NU_COARSE) */
-  TMX_OPT_BAND_WIDE = 25        /* 1: the elasticity's band levels run 1024-thread workgroups
+  TMX_OPT_BAND_WIDE = 25,       /* 1: the elasticity's band levels run 1024-thread workgroups
                                    with one node per thread (square tiles, e.g. 24 x 24), each
                                    node's row built once per launch; 0: 256 threads */
+  TMX_OPT_BENCH_COLD = 26,      /* 1: tmx_bench_kernel times each launch alone after reading
+                                   512 MiB (no inputs left in the caches); 0: back to back */
+  TMX_OPT_RU_EVERY = 27         /* TMX_MIXED_RU: iterations between residual replacements
+                                   (default 4; none once the residual nears the tolerance) */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

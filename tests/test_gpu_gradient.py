@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("precision,rtol,h,tol", [("fp64", 1e-13, 1e-6, 1e-6),
                                                   ("mixed", 1e-13, 1e-6, 1e-6),
-                                                  ("mixed-ir", 1e-10, 1e-4, 1e-4)])
+                                                  ("mixed-ir", 1e-10, 1e-4, 1e-4),
+                                                  ("mixed-ru", 1e-10, 1e-4, 1e-4)])
 def test_autograd_matches_central_differences_n400(precision, rtol, h, tol):
     N = 400
     nx, ny = 3 * N, N

@@ -119,7 +119,7 @@ def test_cfg1_forward_objective_gradient_fp64(seed):
     assert rel(w, w_ref) < 1e-8
 
 
-@pytest.mark.parametrize("precision,tol", [("mixed", 1e-8), ("mixed-ir", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-8), ("mixed-ir", 1e-6), ("mixed-ru", 1e-8)])
 def test_cfg1_mixed_precision(precision, tol):
     nx, ny = 60, 20
     orc = oracle_problem(CANTILEVER, nx, ny)
@@ -156,9 +156,12 @@ def test_smoothed_density_high_contrast(p):
     um = host(gp.forward(cuda(xi), precision="mixed-ir", rtol=1e-10))
     print("p", p, "mixed-ir u err", rel(um, u_ref), gp.last_stats, flush=True)
     assert rel(um, u_ref) < 1e-6
+    um = host(gp.forward(cuda(xi), precision="mixed-ru", rtol=1e-12))
+    print("p", p, "mixed-ru u err", rel(um, u_ref), gp.last_stats, flush=True)
+    assert rel(um, u_ref) < 1e-8
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-ir"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-ir", "mixed-ru"])
 def test_indefinite_stiffness_negative_filtered_density(precision):
     """A filtered design that dips below zero (the P1 Helmholtz filter has no maximum
     principle; EMD designs reach xi ~ -0.045) makes odd-penalty triangle factors negative and
@@ -184,7 +187,7 @@ def test_indefinite_stiffness_negative_filtered_density(precision):
     assert st["rel_residual"] < 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-ru"])
 def test_indefinite_design_leaves_smoother_unchanged(precision):
     """The breakdown of an indefinite design retries its own solve with Jacobi smoothing and
     then MINRES; the next positive-definite design on the handle solves exactly as before it

@@ -22,7 +22,7 @@ TMX_ERR_UNSUPPORTED = -6
 
 TMX_LEFT, TMX_RIGHT, TMX_TOP, TMX_BOTTOM = 0, 1, 2, 3
 TMX_FIX_LEFT, TMX_FIX_RIGHT, TMX_FIX_TOP, TMX_FIX_BOTTOM = 1, 2, 4, 8
-TMX_FP64, TMX_MIXED, TMX_FP32, TMX_MIXED_IR = 0, 1, 2, 3
+TMX_FP64, TMX_MIXED, TMX_FP32, TMX_MIXED_IR, TMX_MIXED_RU = 0, 1, 2, 3, 4
 TMX_MAX_TRACTIONS = 16
 TMX_OPT_FUSED, TMX_OPT_TAIL, TMX_OPT_NU_PRE, TMX_OPT_NU_POST = 0, 1, 2, 3
 TMX_OPT_OMEGA_ELAST, TMX_OPT_OMEGA_FILTER = 4, 5
@@ -33,6 +33,8 @@ TMX_OPT_SWEEP_PAIRS, TMX_OPT_SWEEP_PAIRS_POST, TMX_OPT_SWEEP_PAIR_RESTRICT = 17,
 TMX_OPT_FILTER_PIPE, TMX_OPT_FILTER_DCT = 20, 21
 TMX_OPT_PAIR_LEVELS, TMX_OPT_PAIR_ROWS_COARSE, TMX_OPT_PAIR_NU_COARSE = 22, 23, 24
 TMX_OPT_BAND_WIDE = 25
+TMX_OPT_BENCH_COLD = 26
+TMX_OPT_RU_EVERY = 27
 TMX_OPT_NU_FINE_FILTER, TMX_OPT_NU_COARSE_FILTER = 8, 9
 
 SIDE_CODES = {"Left": TMX_LEFT, "Right": TMX_RIGHT, "Top": TMX_TOP, "Bottom": TMX_BOTTOM}
@@ -106,6 +108,8 @@ SIGNATURES = {
     "tmx_integrate": (_I, [_P, _P, ctypes.POINTER(_D)]),
     "tmx_integrate_noalloc": (_I, [_I, _I, _D, _D, _P, _P, _P]),
     "tmx_autotune": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
+    "tmx_kernel_timing": (_I, [_P, _I, _I]),
+    "tmx_kernel_timing_read": (_I, [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "tmx_set_tiles": (_I, [_P, _I, _I]),
     "tmx_set_option": (_I, [_P, _I, _D]),
     "tmx_precondition": (_I, [_P, _I, _I, _P, _P, _P]),

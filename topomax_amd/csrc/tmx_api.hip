@@ -41,6 +41,7 @@ struct tmx_problem {
   DevBuf<float> xi32, u32, y32;
   SolveParams el_prm, fl_prm;
   bool fl_rtol_set = false;  // filter tolerance given explicitly (else derived per solve)
+  bool bench_cold = false;   // tmx_bench_kernel: evict the caches before each timed launch
   // P2 element (degree 2): lattice geometry, element weights of the operator (W2) and of the
   // preconditioner (W2pc, from max(xi, 0)), P2 block diagonal, loads and solver vectors
   int degree = 1;
@@ -82,7 +83,8 @@ extern "C" int tmx_abi_version(void) { return TMX_ABI_VERSION; }
 
 // tmx_precision -> elasticity solve scheme (TMX_FP32 solves run as TMX_MIXED)
 static int solve_precision(int p) {
-  return p == TMX_FP64 ? kSolveFP64 : p == TMX_MIXED_IR ? kSolveMixedIR : kSolveMixed;
+  return p == TMX_FP64 ? kSolveFP64 : p == TMX_MIXED_IR ? kSolveMixedIR
+         : p == TMX_MIXED_RU ? kSolveMixedRU : kSolveMixed;
 }
 // the filter solves in FP64 PCG, with an FP32 V-cycle unless FP64 is asked for
 static int filter_precision(int p) { return p == TMX_FP64 ? kSolveFP64 : kSolveMixed; }
@@ -102,7 +104,7 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
     if (d->tractions[i].side < 0 || d->tractions[i].side > 3)
       return invalid("Malformed side: " + std::to_string(d->tractions[i].side));
   if ((d->fixed_sides & ~15u) != 0) return invalid("Malformed side in fixed_sides");
-  if (d->precision < 0 || d->precision > 3) return invalid("unknown precision");
+  if (d->precision < 0 || d->precision > 4) return invalid("unknown precision");
   if (d->degree < 0 || d->degree > 2) return invalid("degree must be 1 or 2");
   h->desc = *d;
   h->nx = d->nx;
@@ -665,6 +667,14 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       h->el.drop_graphs();
       h->el.band_wide_ = value != 0.0;
       break;
+    case TMX_OPT_BENCH_COLD:
+      h->bench_cold = value != 0.0;
+      break;
+    case TMX_OPT_RU_EVERY:
+      if (value < 1 || value > 1000 || value != std::floor(value))
+        return invalid("ru_every must be an integer in [1, 1000]");
+      h->el.ru_every_ = (int)value;
+      break;
     case TMX_OPT_PAIR_NU_COARSE:
       if (value < 0 || value > 16 || value != std::floor(value))
         return invalid("sweeps must be 0..16");
@@ -868,6 +878,22 @@ extern "C" int tmx_autotune(tmx_problem* h, int* rows_out, double* best_us_out) 
   return TMX_OK;
 }
 
+extern "C" int tmx_kernel_timing(tmx_problem* h, int kernel, int enable) {
+  if (!h) return invalid("null handle");
+  if (kernel != 5) return invalid("in-step timing covers kernel 5 (PCG direction update)");
+  h->el.au_timing_ = enable != 0;
+  h->el.au_used_ = 0;
+  return TMX_OK;
+}
+
+extern "C" int tmx_kernel_timing_read(tmx_problem* h, int kernel, double* total_us,
+                                      int* launches) {
+  if (!h || !total_us || !launches) return invalid("null argument");
+  if (kernel != 5) return invalid("in-step timing covers kernel 5 (PCG direction update)");
+  Scope scope(h);
+  return h->el.au_read(total_us, launches);
+}
+
 extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int reps,
                                 double* avg_us, double* bytes) {
   if (!h || !avg_us || reps < 1) return invalid("bad arguments");
@@ -927,12 +953,35 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
   hipEvent_t e0, e1;
   TMX_HIP(hipEventCreate(&e0));
   TMX_HIP(hipEventCreate(&e1));
-  TMX_HIP(hipEventRecord(e0, s));
-  for (int i = 0; i < reps; ++i) TMX_TRY(launch());
-  TMX_HIP(hipEventRecord(e1, s));
-  TMX_HIP(hipEventSynchronize(e1));
   float ms = 0;
-  TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  if (!h->bench_cold) {
+    TMX_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) TMX_TRY(launch());
+    TMX_HIP(hipEventRecord(e1, s));
+    TMX_HIP(hipEventSynchronize(e1));
+    TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  } else {
+    // each launch timed alone after reading 512 MiB, twice the 256 MB Infinity Cache, so
+    // the kernel finds none of its inputs in the caches (as inside a solve, where the
+    // V-cycle's traffic separates its launches); reads leave no dirty lines to write back
+    // during the timed launch
+    DevBuf<double> evict;
+    const size_t ne = (size_t)64 << 20;
+    TMX_TRY(evict.alloc(ne));
+    TMX_HIP(fill(ne, 1.0, evict.get(), s));
+    TMX_HIP(hipStreamSynchronize(s));
+    RedSlot er = h->ws.region(3);
+    for (int i = 0; i < reps; ++i) {
+      TMX_HIP(dot(ne, evict.get(), evict.get(), er, h->ws.sc() + kScTmp, s));
+      TMX_HIP(hipEventRecord(e0, s));
+      TMX_TRY(launch());
+      TMX_HIP(hipEventRecord(e1, s));
+      TMX_HIP(hipEventSynchronize(e1));
+      float one = 0;
+      TMX_HIP(hipEventElapsedTime(&one, e0, e1));
+      ms += one;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_us = 1000.0 * ms / reps;

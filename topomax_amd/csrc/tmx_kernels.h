@@ -233,6 +233,12 @@ template <typename T>
 hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s);
 // out = sum of r's partials (one block)
 hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s);
+// reliable-update PCG: x += xs + (*alpha) p, xs = 0; and r = (float) r64 with partials of |r64|^2
+hipError_t ru_flush(size_t n, const double* alpha, const float* p, float* xs, double* x,
+                    hipStream_t s);
+hipError_t ru_replace(size_t n, const double* r64, float* r, RedSlot& rr, hipStream_t s);
+// y += a
+hipError_t add_vec(size_t n, const double* a, double* y, hipStream_t s);
 // dot_t + reduce_to
 hipError_t dot(size_t n, const double* a, const double* b, RedSlot& r, double* out,
                hipStream_t s);

@@ -2567,6 +2567,62 @@ hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s) {
 template hipError_t dot_t<double>(size_t, const double*, const double*, RedSlot&, hipStream_t);
 template hipError_t dot_t<float>(size_t, const float*, const float*, RedSlot&, hipStream_t);
 
+// Reliable-update PCG (FP32 Krylov vectors, FP64 iterate).  Flush: x64 += xs + alpha p and
+// xs = 0 (xs: the FP32 iterate increments since the last flush, alpha p: the increment of the
+// current iteration, not yet added).  Replace: r = (float) r64 and partials of |r64|^2, so the
+// recursive FP32 residual restarts from the true FP64 one and the stop test sees the true norm.
+__global__ __launch_bounds__(kBlock) void k_ru_flush(size_t n, const double* __restrict__ alpha,
+                                                    const float* __restrict__ p,
+                                                    float* __restrict__ xs,
+                                                    double* __restrict__ x) {
+  const double a = *alpha;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    x[i] += (double)xs[i] + a * (double)p[i];
+    xs[i] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ru_replace(size_t n, const double* __restrict__ r64,
+                                                      float* __restrict__ r,
+                                                      double* __restrict__ part) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double v = r64[i];
+    r[i] = (float)v;
+    s += v * v;
+  }
+  const double v[1] = {s};
+  double* const o[1] = {part};
+  block_partials<1>(v, o);
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_vec(size_t n, const double* __restrict__ a,
+                                                    double* __restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] += a[i];
+}
+
+hipError_t add_vec(size_t n, const double* a, double* y, hipStream_t s) {
+  k_add_vec<<<vec_grid(n), kBlock, 0, s>>>(n, a, y);
+  return hipGetLastError();
+}
+
+hipError_t ru_flush(size_t n, const double* alpha, const float* p, float* xs, double* x,
+                    hipStream_t s) {
+  k_ru_flush<<<vec_grid(n), kBlock, 0, s>>>(n, alpha, p, xs, x);
+  return hipGetLastError();
+}
+
+hipError_t ru_replace(size_t n, const double* r64, float* r, RedSlot& rr, hipStream_t s) {
+  const unsigned g = vec_grid(n);
+  rr.nblocks = (int)g;
+  k_ru_replace<<<g, kBlock, 0, s>>>(n, r64, r, rr.partials);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__ part, int n,
                                                       double* __restrict__ out) {
   const double t = sum_partials(part, n);

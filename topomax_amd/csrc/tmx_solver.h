@@ -117,7 +117,7 @@ struct Level {
 
 // solve precision: FP64 PCG with an FP64 V-cycle, FP64 PCG with an FP32 V-cycle, or FP32 PCG
 // inside FP64 iterative refinement
-enum SolvePrecision { kSolveFP64 = 0, kSolveMixed = 1, kSolveMixedIR = 2 };
+enum SolvePrecision { kSolveFP64 = 0, kSolveMixed = 1, kSolveMixedIR = 2, kSolveMixedRU = 3 };
 
 // Waits for the stream's work by polling it: the host is back within microseconds of the last
 // kernel (a blocking synchronisation wakes the thread up tens of microseconds later).  Returns
@@ -219,6 +219,14 @@ class Hierarchy {
           double* rel, hipStream_t s);
   template <typename T>
   T* x_buf();
+  // Reliable-update PCG on A x = b from x0 = 0 (van der Vorst and Ye 2000, residual
+  // replacement): Krylov vectors, operator and V-cycle in FP32, the iterate accumulated in
+  // FP64 (x64_), and the recursive FP32 residual replaced by the true FP64 one b - A x64
+  // every ru_every_ iterations until it nears the tolerance; the search direction is kept,
+  // unlike a refinement restart.
+  int pcg_ru(Workspace& ws, const double* b, double rtol, int max_iter, double* bnorm2,
+             int* iters, double* rel, hipStream_t s);
+  int ru_every_ = 4;  // iterations between replacements
   // preconditioned MINRES (FP64, FP64 V-cycle) on A x = b from x0 = 0, restarted on the true
   // residual until |b - A x| <= rtol |b|; result in x_buf<double>().  For operators that are
   // not positive definite (PCG breakdown).
@@ -294,6 +302,16 @@ class Hierarchy {
     else return dct32_;
   }
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
+  // In-step timing of the PCG direction update + operator (apply_update): with au_timing_ on,
+  // each launch inside a PCG iteration is bracketed by a pair of events on the solve's stream
+  // (up to kAuEvents launches between two reads; direct launches only, not under graphs)
+  static constexpr int kAuEvents = 256;
+  bool au_timing_ = false;
+  std::vector<hipEvent_t> au_ev_;
+  int au_used_ = 0;
+  int au_record(int which, hipStream_t s);  // which: 0 before, 1 after the launch
+  // sum of the recorded launches' durations (us) and their count since the last read
+  int au_read(double* total_us, int* launches);
   bool pair_post_ok(int l = 0) const;  // level l has even sides and a nested level below it
   int tail_nodes_ = kTailNodes;  // largest 9-point level handled by the tail kernel
   void find_tail();

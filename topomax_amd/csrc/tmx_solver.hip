@@ -135,6 +135,31 @@ double Hierarchy::sweep_omega(int k, int nu) const {
   return 1.0 / theta;
 }
 
+int Hierarchy::au_record(int which, hipStream_t s) {
+  if (au_ev_.empty()) {
+    au_ev_.assign(2 * kAuEvents, nullptr);
+    for (auto& e : au_ev_) TMX_HIP(hipEventCreate(&e));
+  }
+  if (au_used_ >= kAuEvents) return 0;  // full: later launches are not timed until a read
+  TMX_HIP(hipEventRecord(au_ev_[2 * au_used_ + which], s));
+  if (which == 1) ++au_used_;
+  return 0;
+}
+
+int Hierarchy::au_read(double* total_us, int* launches) {
+  double t = 0.0;
+  for (int i = 0; i < au_used_; ++i) {
+    TMX_HIP(hipEventSynchronize(au_ev_[2 * i + 1]));
+    float ms = 0.f;
+    TMX_HIP(hipEventElapsedTime(&ms, au_ev_[2 * i], au_ev_[2 * i + 1]));
+    t += 1000.0 * ms;
+  }
+  *total_us = t;
+  *launches = au_used_;
+  au_used_ = 0;
+  return 0;
+}
+
 void Hierarchy::drop_graphs() {
   for (auto& g : graphs_)
     for (auto& e : g)
@@ -146,6 +171,8 @@ void Hierarchy::drop_graphs() {
 
 Hierarchy::~Hierarchy() {
   drop_graphs();
+  for (auto& e : au_ev_)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : solve_ev_)
     if (e) (void)hipEventDestroy(e);
 }
@@ -869,8 +896,11 @@ static int enqueue_pcg_iteration(Hierarchy& h, Workspace& ws, int parity, hipStr
   // x += alpha p of the previous iteration rides along with the direction update (which reads
   // p_old anyway); this iteration's alpha is left in sc[alpha] for the next one, and the last
   // one is applied after the loop (pcg's flush)
+  const bool timed = h.au_timing_ && !h.use_graphs_ && h.phys_ == kElasticity;
+  if (timed) TMX_TRY(h.au_record(0, s));
   TMX_TRY(h.apply_update<T>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s,
                             split ? h.z32_.get() : nullptr, x, sc + kPcg.alpha));
+  if (timed) TMX_TRY(h.au_record(1, s));
   TMX_HIP(pcg_update_xr_fused<T>(nv, sc, kPcg, pq, nullptr, r, p_new, q, rr, s,
                                  split ? h.r32_.get() : nullptr));
   TMX_TRY((enqueue_precondition<T, PC>(h, &rz, false, s)));
@@ -1021,6 +1051,178 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   return -3;
 }
 
+// Reliable-update PCG iteration: the FP32 Krylov iteration of enqueue_pcg_iteration<float>
+// with the increments of the FP64 solution gathered in the FP32 buffer d32_ (deferred_x: this
+// iteration adds the previous one's alpha p there; off after a replacement, which has added it
+// already).  With `replace`, after the x/r update: x64 += d32 + alpha p, d32 = 0, then the
+// recursive FP32 residual is replaced by the true one, r64 = b - A x64 with the FP64 operator
+// (r64_), before the V-cycle; the stop test of that iteration sees |r64|^2.
+static int enqueue_ru_iteration(Hierarchy& h, Workspace& ws, int parity, bool replace,
+                                bool deferred_x, const double* b64, hipStream_t s) {
+  const size_t nv = h.n();
+  float* xs = h.d32_.get();
+  float* r = h.r32_.get();
+  float* z = h.z32_.get();
+  float* q = h.q32_.get();
+  float* p_old = parity ? h.pb32_.get() : h.p32_.get();
+  float* p_new = parity ? h.p32_.get() : h.pb32_.get();
+  double* sc = ws.sc();
+  RedSlot pq = ws.region(0), rr = ws.region(1), rz = ws.region(2);
+  const bool timed = h.au_timing_ && h.phys_ == kElasticity;
+  if (timed) TMX_TRY(h.au_record(0, s));
+  TMX_TRY(h.apply_update<float>(p_old, z, q, p_new, sc + kPcg.beta, &pq, s, nullptr,
+                                deferred_x ? xs : nullptr, sc + kPcg.alpha));
+  if (timed) TMX_TRY(h.au_record(1, s));
+  TMX_HIP(pcg_update_xr_fused<float>(nv, sc, kPcg, pq, nullptr, r, p_new, q, rr, s, nullptr));
+  if (replace) {
+    TMX_HIP(ru_flush(nv, sc + kPcg.alpha, p_new, xs, h.x64_.get(), s));
+    TMX_TRY(h.residual<double>(h.x64_.get(), b64, h.r64_.get(), s));
+    TMX_HIP(ru_replace(nv, h.r64_.get(), r, rr, s));
+  }
+  TMX_TRY((enqueue_precondition<float, float>(h, &rz, false, s)));
+  TMX_HIP(pcg_finalize(pq, rr, rz, sc, kPcg, s, ws.host_dev + Workspace::kScalars + 4 * parity));
+  return 0;
+}
+
+int Hierarchy::pcg_ru(Workspace& ws, const double* b, double rtol, int max_iter, double* bnorm2,
+                      int* iters, double* rel, hipStream_t s) {
+  const size_t nv = n();
+  double* sc = ws.sc();
+  *iters = 0;
+  *rel = 1.0;
+  // Iteration k replaces its residual ru_every_ iterations after the last replacement, never
+  // while a replacement is still in flight, and not once the recursive residual is predicted
+  // (from the reported history: the last report times the recent reduction per iteration)
+  // within 10x of the tolerance: the recursion finishes from there.  Replacing close to the
+  // FP64 round-off floor of the true residual (about 5e-11 relative at N=400, 2e-10 at N=1000
+  // on these designs) would restart the recursion from that floor and stall it.
+  const int every = std::max(1, ru_every_);
+  std::vector<char> replaced((size_t)std::max(1, max_iter) + 2, 0);
+  auto replace_at = [&](int k) { return replaced[(size_t)k] != 0; };
+  int last_rep = 0;
+  bool rep_pending = false;
+  // x64 = 0; the FP32 iteration starts from r = (float) b, d32 = 0, p = 0
+  TMX_HIP(fill_zero_bytes(x64_.get(), sizeof(double) * nv, s));
+  float* b32 = levels_[0].b32.get();
+  TMX_HIP((convert<double, float>(nv, b, b32, s)));
+  volatile double* host_bb = ws.host + Workspace::kScalars + kRingBB;
+  host_bb[0] = std::numeric_limits<double>::quiet_NaN();
+  RedSlot bbs = ws.region(3);
+  TMX_HIP(pcg_begin<float>(nv, b32, d32_.get(), r32_.get(), nullptr, bbs, s));
+  TMX_HIP(pcg_begin_final(bbs, sc, kPcg, rtol * rtol, kScBB,
+                          ws.host_dev + Workspace::kScalars + kRingBB, s));
+  TMX_TRY((enqueue_pcg_init<float, float>(*this, ws, s)));
+  // the host loop of pcg(): iteration k reports into ring slot (k - 1) % 2, the host keeps
+  // one iteration queued ahead until the residual history predicts convergence
+  volatile double* ring = ws.host + Workspace::kScalars;
+  auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
+  ring[3] = ring[7] = 0.0;
+  double bb = -1.0, tol2 = 0.0, flag = 0.0, rr_last = 0.0, ratio = 1.0;
+  double rr_now = -1.0, per_it = 1.0;  // last reported |r|^2 and its reduction per iteration
+  int rr_at = 0;                         // the iteration it was reported by
+  auto launch = [&](int k) -> hipError_t {
+    bool rep = !rep_pending && k - last_rep >= every;
+    if (rep && rr_now >= 0.0 && bb > 0.0) {
+      const double pred = rr_now * std::pow(std::min(per_it, 1.0), (double)(k - rr_at));
+      rep = pred > 100.0 * tol2;
+    }
+    replaced[(size_t)k] = rep ? 1 : 0;
+    if (rep) {
+      last_rep = k;
+      rep_pending = true;
+    }
+    const bool deferred = k == 1 || !replace_at(k - 1);
+    if (enqueue_ru_iteration(*this, ws, (k - 1) & 1, replace_at(k), deferred, b, s) != 0)
+      return hipErrorUnknown;
+    return hipSuccess;
+  };
+  auto wait_iter = [&](int k) -> int {
+    volatile double* sl = slot(k);
+    for (long spins = 1; sl[3] < (double)k; ++spins) {
+      if ((spins & 1023) == 0) {
+        const hipError_t q = hipStreamQuery(s);
+        if (q == hipSuccess && sl[3] < (double)k) {
+          set_error("internal: PCG iteration " + std::to_string(k) + " did not report");
+          return -2;
+        }
+        if (q != hipSuccess && q != hipErrorNotReady) TMX_HIP(q);
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return 0;
+  };
+  bool near = false;
+  int launched = 0, done = 0;
+  if (max_iter >= 1) {
+    TMX_HIP(launch(1));
+    launched = 1;
+  }
+  while (done < launched) {
+    if (launched == done + 1 && launched < max_iter && !near) {
+      TMX_HIP(launch(launched + 1));
+      ++launched;
+    }
+    TMX_TRY(wait_iter(done + 1));
+    ++done;
+    if (bb < 0.0) {
+      bb = host_bb[0];
+      tol2 = rtol * rtol * bb;
+      rr_last = bb;
+    }
+    flag = slot(done)[0];
+    {
+      const double rrd = slot(done)[1];
+      if (rr_now > 0.0 && rrd > 0.0) per_it = std::pow(rrd / rr_now, 1.0 / (double)(done - rr_at));
+      rr_now = rrd;
+      rr_at = done;
+      if (replace_at(done)) rep_pending = false;
+    }
+    if (g_trace)
+      std::fprintf(stderr, "[tmx] pcg-ru it %d%s rel %.3e flag %g\n", done,
+                   replace_at(done) ? " (replaced)" : "",
+                   bb > 0.0 ? std::sqrt(std::max((double)slot(done)[1], 0.0) / bb) : 0.0, flag);
+    if (flag != 0.0) break;
+    const double rr = slot(done)[1];
+    if (rr_last > 0.0) ratio = rr / rr_last;
+    rr_last = rr;
+    near = rr * std::min(ratio, 1.0) <= 4.0 * tol2;
+    if (launched == done && done < max_iter) {
+      TMX_HIP(launch(done + 1));
+      ++launched;
+    }
+  }
+  if (launched > done) TMX_TRY(wait_iter(launched));
+  // the increments not yet in x64: d32 and, unless the last iteration replaced, its alpha p
+  if (launched > 0 && !replace_at(launched)) {
+    float* p_last = ((launched - 1) & 1) ? p32_.get() : pb32_.get();
+    TMX_HIP(ru_flush(nv, sc + kPcg.alpha, p_last, d32_.get(), x64_.get(), s));
+  }
+  if (launched == 0) {
+    TMX_TRY(ws.read(s, kScBB, 1));
+    bb = ws.host[kScBB];
+  } else if (bb < 0.0) {
+    bb = host_bb[0];
+  }
+  *bnorm2 = bb;
+  if (launched > 0) flag = slot(launched)[0];
+  const double rrv = launched > 0 ? slot(launched)[1] : bb;
+  *iters = launched > 0 ? (int)slot(launched)[2] : 0;
+  *rel = bb > 0.0 ? std::sqrt(std::max(rrv, 0.0) / bb) : 0.0;
+  if (g_trace)
+    std::fprintf(stderr, "[tmx] pcg-ru n=%zu: %d its, rel %.3e, flag %g\n", nv, *iters, *rel, flag);
+  if (flag == 1.0) return 0;
+  if (flag == 2.0) {
+    set_error("PCG breakdown: p.Ap <= 0 (operator not positive definite)");
+    return -4;
+  }
+  if (flag == 3.0) {
+    set_error("non-finite right-hand side or residual in PCG");
+    return -5;
+  }
+  set_error("PCG did not reach the tolerance within the iteration limit");
+  return -3;
+}
+
 // MINRES scalar slots (after the PCG slots)
 constexpr MrSlots kMr = {30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45};
 
@@ -1138,7 +1340,14 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   S.iterations = 0;
   S.outer = 0;
   int rc = 0;
-  if (prm.precision == kSolveFP64 || prm.precision == kSolveMixed) {
+  if (prm.precision == kSolveFP64 || prm.precision == kSolveMixed ||
+      prm.precision == kSolveMixedRU) {
+    auto krylov = [&](int* its, double* relv, double* bbv) {
+      return prm.precision == kSolveFP64 ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s)
+             : prm.precision == kSolveMixed
+                 ? pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s)
+                 : pcg_ru(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s);
+    };
     const bool defer = !prm.sync;
     if (defer) {
       for (auto& e : solve_ev_)
@@ -1149,9 +1358,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     // non-finite b stops with the non-finite code)
     int it = 0;
     double rel = 0, bb = 0;
-    rc = prm.precision == kSolveFP64
-             ? pcg<double>(ws, b, prm.rtol, prm.max_iter, &bb, &it, &rel, s)
-             : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, &bb, &it, &rel, s);
+    rc = krylov(&it, &rel, &bb);
     S.iterations = it;
     S.outer = 1;
     if (rc == -4 && cheb_lmax_ > 0.0) {
@@ -1163,9 +1370,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       cheb_lmax_ = 0.0;
       drop_graphs();
       int it2 = 0;
-      rc = prm.precision == kSolveFP64
-               ? pcg<double>(ws, b, prm.rtol, prm.max_iter, &bb, &it2, &rel, s)
-               : pcg<double, float>(ws, b, prm.rtol, prm.max_iter, &bb, &it2, &rel, s);
+      rc = krylov(&it2, &rel, &bb);
       S.iterations += it2;
       it += it2;
       cheb_lmax_ = keep;
@@ -1181,7 +1386,36 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
     if (rc == 0 || rc == -3) {
       TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
       S.rel_residual = rel;
-      if (prm.verify && bb > 0.0) {  // the true residual
+      if (prm.precision == kSolveMixedRU && rc == 0 && bb > 0.0) {
+        // the FP32 recursion stops on its own residual, which drifts from the true one after
+        // the last replacement: check the true residual and, while it is above the tolerance
+        // and the last round more than halved it, solve for the remaining residual
+        // (b' = b - A x, x += d) in another round
+        double prev = 0.0;
+        for (int round = 0; round < 3; ++round) {
+          TMX_TRY(residual<double>(x, b, rr64_.get(), s));
+          TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScVerify, s));
+          TMX_TRY(ws.read(s, kScVerify, 1));
+          const double true_rel = std::sqrt(std::max(ws.host[kScVerify], 0.0) / bb);
+          if (!std::isfinite(true_rel)) {
+            set_error("non-finite residual in the reliable-update solve");
+            return -5;
+          }
+          S.rel_residual = true_rel;
+          if (true_rel <= prm.rtol || (round > 0 && true_rel > 0.5 * prev) ||
+              S.iterations >= prm.max_iter)
+            break;
+          prev = true_rel;
+          int it2 = 0;
+          double rel2 = 0, bb2 = 0;
+          const int rc2 = pcg_ru(ws, rr64_.get(), prm.rtol / true_rel,
+                                 prm.max_iter - S.iterations, &bb2, &it2, &rel2, s);
+          S.iterations += it2;
+          S.outer += 1;
+          if (rc2 != 0 && rc2 != -3) return rc2;
+          TMX_HIP(add_vec(nv, x64_.get(), x, s));
+        }
+      } else if (prm.verify && bb > 0.0) {  // the true residual
         TMX_TRY(residual<double>(x, b, rr64_.get(), s));
         TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScVerify, s));
         if (defer) {

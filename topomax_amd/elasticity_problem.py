@@ -28,7 +28,8 @@ from .designs import DomainParameters, ElasticityParameters
 
 # solve schemes: FP64 multigrid-PCG; FP64 PCG with an FP32 V-cycle preconditioner; FP32
 # multigrid-PCG inner solves inside FP64 iterative refinement
-PRECISIONS = {"fp64": _lib.TMX_FP64, "mixed": _lib.TMX_MIXED, "mixed-ir": _lib.TMX_MIXED_IR}
+PRECISIONS = {"fp64": _lib.TMX_FP64, "mixed": _lib.TMX_MIXED, "mixed-ir": _lib.TMX_MIXED_IR,
+              "mixed-ru": _lib.TMX_MIXED_RU}
 
 
 def _device(device=None) -> torch.device:
@@ -420,7 +421,8 @@ class ElasticityProblem:
         """Solver knobs: fused, tail, nu_pre, nu_post, nu_coarse (elasticity), nu_fine_filter,
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
         cheb_ratio, band, band_fine_filter, fine_cell_weights, graphs, sweep_pairs,
-        sweep_pairs_post, sweep_pair_restrict, filter_pipe, filter_dct."""
+        sweep_pairs_post, sweep_pair_restrict, filter_pipe, filter_dct, pair_levels,
+        pair_rows_coarse, pair_nu_coarse, band_wide, bench_cold, ru_every."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -444,7 +446,9 @@ class ElasticityProblem:
                "pair_levels": _lib.TMX_OPT_PAIR_LEVELS,
                "pair_rows_coarse": _lib.TMX_OPT_PAIR_ROWS_COARSE,
                "pair_nu_coarse": _lib.TMX_OPT_PAIR_NU_COARSE,
-               "band_wide": _lib.TMX_OPT_BAND_WIDE}[name]
+               "band_wide": _lib.TMX_OPT_BAND_WIDE,
+               "bench_cold": _lib.TMX_OPT_BENCH_COLD,
+               "ru_every": _lib.TMX_OPT_RU_EVERY}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
