@@ -174,6 +174,55 @@ def test_total_loads():
         assert pb.load[1::2].sum() < 0
 
 
+@pytest.mark.parametrize("degree", [1, 2])
+@pytest.mark.parametrize("side,center,length", [("Right", 0.4, 0.3), ("Left", 0.1, 0.2),
+                                                ("Top", 1.35, 0.3), ("Bottom", 0.7, 0.45)])
+def test_traction_first_moments(degree, side, center, length):
+    """The split of a traction over the nodes: sum_i F_i x_i and sum_i F_i y_i must equal
+    int t_h x ds and int t_h y ds over the whole exterior boundary, where t_h is the P2 edge
+    interpolant of the pointwise rule (FEM_src/elasisity_problem.py:39-73).  t_h times a linear
+    function is a cubic on each boundary edge, so Simpson's rule over the edge's end points and
+    midpoint integrates it exactly; the rule is evaluated here independently (exact side test,
+    the interval widened by DOLFIN_EPS), on every boundary edge: a corner that the rule selects
+    also loads the first edge of the neighbouring side (the Left and Top cases, whose intervals
+    reach a corner).  A contribution booked on the wrong end node of an edge, or edges walked
+    in the wrong direction, changes a moment while keeping the total: the partially covered
+    edge at each end of an interval carries an asymmetry of +-t h / 6, which an interval with
+    two partial ends cancels and one that reaches its side's end exposes."""
+    nx, ny, width, height = 12, 8, 1.5, 1.0
+    value = (0.7, -1.3)
+    trac = [{"side": side, "center": center, "length": length, "value": value}]
+    prob = small_problem(nx, ny, degree=degree, fixed=(), tractions=trac, width=width,
+                         height=height)
+    lo, hi = center - length / 2, center + length / 2
+    eps = 3.0e-16
+
+    def rule(x, y):
+        on = {"Left": x == 0.0, "Right": x == width, "Top": y == height, "Bottom": y == 0.0}[side]
+        t = y if side in ("Left", "Right") else x
+        return 1.0 if on and lo - eps <= t <= hi + eps else 0.0
+
+    hx, hy = width / nx, height / ny
+    edges = [((i * hx, 0.0), ((i + 1) * hx, 0.0)) for i in range(nx)]
+    edges += [((i * hx, height), ((i + 1) * hx, height)) for i in range(nx)]
+    edges += [((0.0, j * hy), (0.0, (j + 1) * hy)) for j in range(ny)]
+    edges += [((width, j * hy), (width, (j + 1) * hy)) for j in range(ny)]
+    m0 = mx = my = 0.0
+    for (ax, ay), (bx, by) in edges:
+        L = abs(bx - ax) + abs(by - ay)
+        cx, cy = 0.5 * ax + 0.5 * bx, 0.5 * ay + 0.5 * by
+        ta, tc, tb = rule(ax, ay), rule(cx, cy), rule(bx, by)
+        m0 += L / 6 * (ta + 4 * tc + tb)
+        mx += L / 6 * (ta * ax + 4 * tc * cx + tb * bx)
+        my += L / 6 * (ta * ay + 4 * tc * cy + tb * by)
+    xs, ys = prob.node_coords[:, 0], prob.node_coords[:, 1]
+    for comp in range(2):
+        F = prob.load[comp::2]
+        assert abs(F.sum() - value[comp] * m0) < 1e-13
+        assert abs(F @ xs - value[comp] * mx) < 1e-13
+        assert abs(F @ ys - value[comp] * my) < 1e-13
+
+
 # ------------------------------------------------------------------ invariants
 @pytest.mark.parametrize("degree", [1, 2])
 def test_rigid_body_null_space(degree):
