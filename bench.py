@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-p2", action="store_true")
     ap.add_argument("--p2-N", type=int, default=400)
+    ap.add_argument("--p2-only", action="store_true",
+                    help="time only the P2 element's steps (for a kernel trace of them)")
     ap.add_argument("--batch-designs", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="set up the N operator, then launch only the roofline kernel (PCG "
@@ -229,6 +231,10 @@ def main():
               "(independent designs would shard across GPUs with zero communication)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.p2_only:
+        print(json.dumps({"p2": p2_throughput(args.p2_N, args.steps, args.rtol)}), flush=True)
+        return
 
     import torch
     from topomax_amd.workloads import CANTILEVER, problem_from_case, smoothed_density

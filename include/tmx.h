@@ -245,9 +245,10 @@ enum tmx_option {
                                    paired levels (0: one launch per sweep there) */
   TMX_OPT_PAIR_NU_COARSE = 24,  /* sweeps each way on the coarser paired levels (0: the
                                    coarse-level count, TMX_OPT_NU_COARSE) */
-  TMX_OPT_BAND_WIDE = 25,       /* 1: the elasticity's band levels run 1024-thread workgroups
-                                   with one node per thread (square tiles, e.g. 24 x 24), each
-                                   node's row built once per launch; 0: 256 threads */
+  TMX_OPT_BAND_WIDE = 25,       /* the elasticity's cell-weight band levels: 0: 256-thread
+                                   workgroups; 1: 1024 threads with one node per thread (square
+                                   tiles, e.g. 24 x 24), each node's row built once per launch;
+                                   2 (default): 512 threads on the standard tiles; 3: 1024 */
   TMX_OPT_BENCH_COLD = 26,      /* 1: tmx_bench_kernel times each launch alone after reading
                                    512 MiB (no inputs left in the caches); 0: back to back */
   TMX_OPT_RU_EVERY = 27         /* TMX_MIXED_RU: iterations between residual replacements

@@ -122,10 +122,12 @@ def test_band_sweep_counts(nu_coarse, nu_filter, nu_fine_filter):
         assert rel(f, f_ref) < tol
 
 
+@pytest.mark.parametrize("wide", [1, 2, 3])
 @pytest.mark.parametrize("nx,ny", [(120, 40), (250, 90), (300, 100), (45, 15)])
-def test_band_wide_equals_band(nx, ny):
-    """The 1024-thread one-node-per-thread band kernels (option band_wide) on the elasticity's
-    cell-weight levels against the 256-thread kernels: the same sweeps in the same order."""
+def test_band_wide_equals_band(nx, ny, wide):
+    """The 1024-thread one-node-per-thread band kernels (option band_wide 1) and the 512 and
+    1024-thread kernels on the standard tiles (2, 3) on the elasticity's cell-weight levels
+    against the 256-thread kernels: the same sweeps in the same order."""
     gp = gpu_problem(CANTILEVER, nx, ny)
     gp.set_penalization(3)
     xi = torch.as_tensor(smoothed_density(nx, ny, 3, sigma=2.0), device="cuda:0")
@@ -134,7 +136,7 @@ def test_band_wide_equals_band(nx, ny):
     for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
         gp.set_option("band_wide", 0)
         z0 = gp.precondition(r, "elasticity", xi, precision=prec)
-        gp.set_option("band_wide", 1)
+        gp.set_option("band_wide", wide)
         z1 = gp.precondition(r, "elasticity", xi, precision=prec)
         err = float(torch.linalg.norm(z1 - z0) / torch.linalg.norm(z0))
         print(nx, ny, prec, err, flush=True)

@@ -665,7 +665,9 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       break;
     case TMX_OPT_BAND_WIDE:
       h->el.drop_graphs();
-      h->el.band_wide_ = value != 0.0;
+      if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
+        return invalid("band_wide must be 0, 1, 2 or 3");
+      h->el.band_wide_ = (int)value;
       break;
     case TMX_OPT_BENCH_COLD:
       h->bench_cold = value != 0.0;

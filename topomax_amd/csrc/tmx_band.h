@@ -31,6 +31,8 @@ constexpr int kBandThreads = 256;
 // more nodes than kBandThreads but at most this many keep one node per thread (rows built
 // once per launch instead of once per pass)
 constexpr int kBandThreadsWide = 1024;
+// workgroup size of the mid variant (option): the same tiles as kBandThreads, twice the threads
+constexpr int kBandThreadsMid = 512;
 constexpr size_t kBandLdsBytes = 64 * 1024;
 
 // where a level's operator comes from
@@ -56,7 +58,9 @@ struct BandLevel {
   T omega_down[kMaxNu];  // step size of each sweep
   T omega_up[kMaxNu];
   int tx, ty;            // owned fine tile (nodes), even
-  int wide;              // 1: kSrcCells levels may use the kBandThreadsWide variant
+  int wide;              // 1: kSrcCells levels may use the kBandThreadsWide variant;
+                         // 2 / 3: kSrcCells levels run kBandThreadsMid / kBandThreadsWide
+                         // threads on the standard tiles
 };
 
 // element blocks of the two triangle types: elasticity (lam, mu) or the filter (eps)

@@ -702,7 +702,17 @@ hipError_t band_down(const BandLevel<T>& L, const T* b, T* x, T* bc, hipStream_t
   if (lds > kBandLdsBytes) return hipErrorInvalidValue;
   const size_t ne = band_ext(L.tx, L.nu_down, true) * band_ext(L.ty, L.nu_down, true);
   const bool one = ne <= (size_t)kBandThreads;
-  if (L.wide && L.src == kSrcCells && !one && ne <= (size_t)kBandThreadsWide) {
+  if (L.wide == 2 && L.src == kSrcCells) {
+    if (ne <= (size_t)kBandThreadsMid)
+      launch_down<T, BS, kSrcCells, true, kBandThreadsMid>(L, b, x, bc, lds, s);
+    else
+      launch_down<T, BS, kSrcCells, false, kBandThreadsMid>(L, b, x, bc, lds, s);
+  } else if (L.wide == 3 && L.src == kSrcCells) {
+    if (ne <= (size_t)kBandThreadsWide)
+      launch_down<T, BS, kSrcCells, true, kBandThreadsWide>(L, b, x, bc, lds, s);
+    else
+      launch_down<T, BS, kSrcCells, false, kBandThreadsWide>(L, b, x, bc, lds, s);
+  } else if (L.wide == 1 && L.src == kSrcCells && !one && ne <= (size_t)kBandThreadsWide) {
     launch_down<T, BS, kSrcCells, true, kBandThreadsWide>(L, b, x, bc, lds, s);
   } else if (L.src == kSrcStencil) {
     if (one) launch_down<T, BS, kSrcStencil, true>(L, b, x, bc, lds, s);
@@ -732,7 +742,18 @@ hipError_t band_up(const BandLevel<T>& L, const T* x, const T* ec, const T* b, T
   }
   const size_t ne = band_ext(L.tx, L.nu_up, false) * band_ext(L.ty, L.nu_up, false);
   const bool one = ne <= (size_t)kBandThreads;
-  if (L.wide && L.src == kSrcCells && !one && ne <= (size_t)kBandThreadsWide && !part) {
+  if (L.wide == 2 && L.src == kSrcCells && !part) {
+    if (ne <= (size_t)kBandThreadsMid)
+      launch_up<T, BS, kSrcCells, true, kBandThreadsMid>(L, x, ec, b, y, part, lds, s);
+    else
+      launch_up<T, BS, kSrcCells, false, kBandThreadsMid>(L, x, ec, b, y, part, lds, s);
+  } else if (L.wide == 3 && L.src == kSrcCells && !part) {
+    if (ne <= (size_t)kBandThreadsWide)
+      launch_up<T, BS, kSrcCells, true, kBandThreadsWide>(L, x, ec, b, y, part, lds, s);
+    else
+      launch_up<T, BS, kSrcCells, false, kBandThreadsWide>(L, x, ec, b, y, part, lds, s);
+  } else if (L.wide == 1 && L.src == kSrcCells && !one && ne <= (size_t)kBandThreadsWide &&
+             !part) {
     launch_up<T, BS, kSrcCells, true, kBandThreadsWide>(L, x, ec, b, y, part, lds, s);
   } else if (L.src == kSrcStencil) {
     if (one) launch_up<T, BS, kSrcStencil, true>(L, x, ec, b, y, part, lds, s);

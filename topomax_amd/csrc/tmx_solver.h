@@ -273,7 +273,10 @@ class Hierarchy {
   bool use_graphs_ = false;
   bool band_fine_filter_ = true;
   // elasticity cell-weight band levels use the 1024-thread one-node-per-thread kernels
-  bool band_wide_ = false;
+  // the elasticity's cell-weight band levels: 0: 256-thread workgroups; 1: 1024 threads, one
+  // node each (square tiles); 2 / 3: 512 / 1024 threads on the standard tiles.  2 (the
+  // default) measured 0.15 ms faster per mixed step at N=1000 than 0, 3 slower than both.
+  int band_wide_ = 2;
   // > 0: the finest elasticity level's first two pre-sweeps (pair_rows_), and its prolongation
   // with the first two post-sweeps (pair_rows_post_), run as one launch each with node-row
   // chunks of that height; 0: one launch per sweep.  Index 0: FP32 V-cycles, 1: FP64.

@@ -611,7 +611,7 @@ static bool band_level(const Hierarchy& h, int l, BandLevel<T>& B) {
   if (B.nu_down < 1 || B.nu_down > kMaxNu || B.nu_up < 1 || B.nu_up > kMaxNu) return false;
   for (int k = 0; k < B.nu_down; ++k) B.omega_down[k] = (T)h.sweep_omega(k, B.nu_down);
   for (int k = 0; k < B.nu_up; ++k) B.omega_up[k] = (T)h.sweep_omega(k, B.nu_up);
-  if (h.band_wide_ && B.src == kSrcCells) {
+  if (h.band_wide_ == 1 && B.src == kSrcCells) {
     // one node per thread of a kBandThreadsWide workgroup: the largest even square tile whose
     // extended tile fits it (24 x 24 for three sweeps)
     const int nu = std::max(B.nu_down, B.nu_up);
@@ -637,6 +637,7 @@ static bool band_level(const Hierarchy& h, int l, BandLevel<T>& B) {
     found = true;
     if (tiles >= 480) break;
   }
+  if (found && h.band_wide_ >= 2 && B.src == kSrcCells) B.wide = h.band_wide_;
   return found;
 }
 
@@ -1018,11 +1019,13 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
       ++launched;
     }
   }
-  if (launched > done) TMX_TRY(wait_iter(launched));
-  if (launched > 0) {  // the last iteration's deferred x += alpha p (alpha is 0 once stopped)
+  // the last iteration's deferred x += alpha p (alpha is 0 once stopped), queued before the
+  // host waits for the last queued iteration's report, so the device does not idle meanwhile
+  if (launched > 0) {
     T* p_last = ((launched - 1) & 1) ? KrylovBufs<T>::p(*this) : KrylovBufs<T>::pb(*this);
     TMX_HIP(axpy_dev<T>(nv, sc + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s));
   }
+  if (launched > done) TMX_TRY(wait_iter(launched));
   if (launched == 0) {  // no iteration reported |b|^2: read it
     TMX_TRY(ws.read(s, kScBB, 1));
     bb = ws.host[kScBB];
@@ -1191,12 +1194,13 @@ int Hierarchy::pcg_ru(Workspace& ws, const double* b, double rtol, int max_iter,
       ++launched;
     }
   }
-  if (launched > done) TMX_TRY(wait_iter(launched));
   // the increments not yet in x64: d32 and, unless the last iteration replaced, its alpha p
+  // (queued before the wait for the last report, as in pcg)
   if (launched > 0 && !replace_at(launched)) {
     float* p_last = ((launched - 1) & 1) ? p32_.get() : pb32_.get();
     TMX_HIP(ru_flush(nv, sc + kPcg.alpha, p_last, d32_.get(), x64_.get(), s));
   }
+  if (launched > done) TMX_TRY(wait_iter(launched));
   if (launched == 0) {
     TMX_TRY(ws.read(s, kScBB, 1));
     bb = ws.host[kScBB];
