@@ -35,8 +35,8 @@ if ROOT not in sys.path:
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel at N=1000, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --roofline-only`, each scaled by the same counter's
-# reading for a plain copy of 48,064,016 bytes (FETCH x2.0, WRITE x1.0).
-PMC_TRAFFIC_BYTES = {1000: 321.29e6}
+# reading for a plain copy of 48,064,016 bytes (FETCH x1.999, WRITE x1.0).
+PMC_TRAFFIC_BYTES = {1000: 308.42e6}
 # tmx_bench_kernel id of the roofline kernel: the mixed PCG's direction update + FP64 operator
 ROOFLINE_KERNEL = 5
 
