@@ -264,19 +264,40 @@ int tmx_precondition(tmx_problem* h, int which, int precision, const double* xi,
 int tmx_set_tiles(tmx_problem* h, int rows_fp64, int rows_fp32);
 
 /* Batched objective + gradient over many independent problems (uneven sizes, penalties and
- * loads; each problem keeps its own penalty, tiles and operators).  Designs run concurrently,
- * each on its own stream, driven by `workers` host threads (default min(16, cores), at least
- * 4), largest first, so the latency-bound coarse-grid kernels of different designs overlap.  The handles must not be
- * used by other threads during the call.  rho[i], w[i] are device pointers (n_v of design i);
- * J_host[i] receives the compliance (NaN for a non-finite design).  stats may be NULL or
- * point to `count` entries.  No counterpart in the reference (it solves one design). */
+ * loads; each problem keeps its own penalty, tiles and operators).  No counterpart in the
+ * reference (it solves one design).
+ *
+ * TMX_BATCH_MERGED (the default): the designs that qualify run as one merged step: one
+ * kernel launch per PCG phase and V-cycle stage covers every such design, the designs'
+ * iterations in lockstep, so the launches per step do not grow with the number of designs.
+ * A design qualifies once an objective step with the same precision has completed on its
+ * handle (P1 element, TMX_FP64 or TMX_MIXED / TMX_FP32, no graph replay, same device as the
+ * first problem).  The others, and designs whose merged solve stops without converging, run
+ * as single-design calls on worker threads.  Results are bitwise equal to single-design calls.
+ * TMX_BATCH_ALONE: every design runs as a single-design call, each on its own stream, driven
+ * by `workers` host threads (default min(16, cores), at least 4), largest first.
+ *
+ * The handles must not be used by other threads during the call.  rho[i], w[i] are device
+ * pointers (n_v of design i); J_host[i] receives the compliance (NaN for a non-finite design).
+ * stats may be NULL or point to `count` entries. */
 typedef struct tmx_batch tmx_batch;
+enum { TMX_BATCH_MERGED = 0, TMX_BATCH_ALONE = 1 };
+typedef struct tmx_batch_report {
+  int merged_designs;    /* designs whose results came from the merged step */
+  int solo_designs;      /* designs run as single-design calls */
+  int launches;          /* kernel launches of the merged step */
+  int iterations[3];     /* merged PCG iterations: filter, elasticity, adjoint filter */
+  double record_ms;      /* host time spent recording and merging (0: the plan was reused) */
+} tmx_batch_report;
 int tmx_batch_create(tmx_problem* const* problems, int count, tmx_batch** out);
 void tmx_batch_destroy(tmx_batch* b);
 int tmx_batch_set_workers(tmx_batch* b, int workers);
+int tmx_batch_set_mode(tmx_batch* b, int mode);
 int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho, double* J_host,
                                      double* const* w, const tmx_solve_opts* opts,
                                      tmx_solve_stats* stats);
+/* what the last tmx_batch_objective_and_gradient call did */
+int tmx_batch_last_report(const tmx_batch* b, tmx_batch_report* out);
 
 /* Entropic mirror descent, design-space half (src/solver.py:149-302), on n_v nodal device
  * vectors; integrals are of the CG1 interpolant (lumped mass, as FEMSolver.integrate).

@@ -261,3 +261,33 @@ def test_load_design_refuses_foreign_globals(tmp_path):
     with pytest.raises(pickle.UnpicklingError, match="io.open|builtins.open"):
         load_design(str(path))
     assert not marker.exists()
+
+
+def test_batch_report_layout_and_modes():
+    """tmx_batch_report and the TMX_BATCH_* modes as the C compiler sees them equal the ctypes
+    binding's."""
+    src = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "tmx.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %d %d\n", sizeof(tmx_batch_report),
+         offsetof(tmx_batch_report, solo_designs), offsetof(tmx_batch_report, launches),
+         offsetof(tmx_batch_report, iterations), offsetof(tmx_batch_report, record_ms),
+         sizeof(((tmx_batch_report*)0)->iterations), TMX_BATCH_MERGED, TMX_BATCH_ALONE);
+  return 0;
+}
+'''
+    import tempfile
+    from topomax_amd import _lib
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                               check=True).stdout.split()]
+    R = _lib.BatchReport
+    assert vals == [ctypes.sizeof(R), R.solo_designs.offset, R.launches.offset,
+                    R.iterations.offset, R.record_ms.offset, ctypes.sizeof(ctypes.c_int * 3),
+                    _lib.TMX_BATCH_MERGED, _lib.TMX_BATCH_ALONE]

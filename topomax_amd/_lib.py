@@ -36,6 +36,7 @@ TMX_OPT_BAND_WIDE = 25
 TMX_OPT_BENCH_COLD = 26
 TMX_OPT_RU_EVERY = 27
 TMX_OPT_NU_FINE_FILTER, TMX_OPT_NU_COARSE_FILTER = 8, 9
+TMX_BATCH_MERGED, TMX_BATCH_ALONE = 0, 1
 
 SIDE_CODES = {"Left": TMX_LEFT, "Right": TMX_RIGHT, "Top": TMX_TOP, "Bottom": TMX_BOTTOM}
 FIX_BITS = {"Left": TMX_FIX_LEFT, "Right": TMX_FIX_RIGHT, "Top": TMX_FIX_TOP,
@@ -80,6 +81,17 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchReport(ctypes.Structure):
+    _fields_ = [("merged_designs", ctypes.c_int), ("solo_designs", ctypes.c_int),
+                ("launches", ctypes.c_int), ("iterations", ctypes.c_int * 3),
+                ("record_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["iterations"] = list(self.iterations)
+        return d
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -117,6 +129,8 @@ SIGNATURES = {
     "tmx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
     "tmx_batch_destroy": (None, [_P]),
     "tmx_batch_set_workers": (_I, [_P, _I]),
+    "tmx_batch_set_mode": (_I, [_P, _I]),
+    "tmx_batch_last_report": (_I, [_P, ctypes.POINTER(BatchReport)]),
     "tmx_batch_objective_and_gradient": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D),
                                               ctypes.POINTER(_P), ctypes.POINTER(SolveOpts),
                                               ctypes.POINTER(SolveStats)]),

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtmx.so")
 SOURCES = ["tmx_kernels.hip", "tmx_band.hip", "tmx_p2.hip", "tmx_dct.hip", "tmx_solver.hip",
-           "tmx_api.hip", "tmx_noalloc.hip"]
+           "tmx_api.hip", "tmx_batch.hip", "tmx_noalloc.hip"]
 ARCH = os.environ.get("TMX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -11,69 +11,12 @@
 #include <thread>
 #include <vector>
 
-#include "tmx_p2.h"
 #include "tmx_dct.h"
-#include "tmx_solver.h"
-
-namespace tmx {
-const char* last_error_cstr();
-}
-
-using namespace tmx;
-
-struct tmx_problem {
-  tmx_problem_desc desc;
-  hipStream_t stream = nullptr;  // internal non-blocking stream (all work, graph capture)
-  hipStream_t user = nullptr;    // caller's stream (tmx_set_stream), joined by events
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipEvent_t ev_setup[2] = {nullptr, nullptr};  // brackets the operator rebuild (timing)
-  int device = 0;  // HIP device current at creation; all of the handle's memory lives there
-  int nx = 0, ny = 0;
-  double hx = 0, hy = 0, lam = 0, mu = 0, m = 1e-6;
-  unsigned fixed = 0;  // sides actually fixed after the near() test
-  int p = 0;
-  bool has_penalty = false;
-  Hierarchy el, fl;
-  Workspace ws;
-  DevBuf<double> F, Fm, lumped;
-  DevBuf<double> xi_s, u_s, b_s, w_s;     // scratch of the fused call
-  DevBuf<double> tmp64;
-  DevBuf<float> xi32, u32, y32;
-  SolveParams el_prm, fl_prm;
-  bool fl_rtol_set = false;  // filter tolerance given explicitly (else derived per solve)
-  bool bench_cold = false;   // tmx_bench_kernel: evict the caches before each timed launch
-  // P2 element (degree 2): lattice geometry, element weights of the operator (W2) and of the
-  // preconditioner (W2pc, from max(xi, 0)), P2 block diagonal, loads and solver vectors
-  int degree = 1;
-  P2Geom g2{};
-  DevBuf<double> W2, W2pc, D2, F2, F2m, p2v[11], r1c, e1c;
-  ~tmx_problem() {
-    if (ev_in) (void)hipEventDestroy(ev_in);
-    for (auto e : ev_setup)
-      if (e) (void)hipEventDestroy(e);
-    if (ev_out) (void)hipEventDestroy(ev_out);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-// Orders an entry point's work after the caller's stream and the caller's stream after it.
-class Scope {
- public:
-  explicit Scope(tmx_problem* h) : h_(h) {
-    if (hipEventRecord(h->ev_in, h->user) == hipSuccess) (void)hipStreamWaitEvent(h->stream, h->ev_in, 0);
-  }
-  ~Scope() {
-    if (hipEventRecord(h_->ev_out, h_->stream) == hipSuccess)
-      (void)hipStreamWaitEvent(h_->user, h_->ev_out, 0);
-  }
-
- private:
-  tmx_problem* h_;
-};
+#include "tmx_handle.h"
 
 static const double kDolfinEps = 3.0e-16;
 
-static int invalid(const std::string& m) {
+int invalid(const std::string& m) {
   set_error(m);
   return TMX_ERR_INVALID;
 }
@@ -81,13 +24,11 @@ static int invalid(const std::string& m) {
 extern "C" const char* tmx_last_error(void) { return last_error_cstr(); }
 extern "C" int tmx_abi_version(void) { return TMX_ABI_VERSION; }
 
-// tmx_precision -> elasticity solve scheme (TMX_FP32 solves run as TMX_MIXED)
-static int solve_precision(int p) {
+int solve_precision(int p) {
   return p == TMX_FP64 ? kSolveFP64 : p == TMX_MIXED_IR ? kSolveMixedIR
          : p == TMX_MIXED_RU ? kSolveMixedRU : kSolveMixed;
 }
-// the filter solves in FP64 PCG, with an FP32 V-cycle unless FP64 is asked for
-static int filter_precision(int p) { return p == TMX_FP64 ? kSolveFP64 : kSolveMixed; }
+int filter_precision(int p) { return p == TMX_FP64 ? kSolveFP64 : kSolveMixed; }
 
 static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
   if (d->nx < 1 || d->ny < 1) return invalid("nx and ny must be >= 1");
@@ -238,6 +179,7 @@ extern "C" void tmx_problem_destroy(tmx_problem* h) {
 
 extern "C" int tmx_set_stream(tmx_problem* h, void* stream) {
   if (!h) return invalid("null handle");
+  if (h->user != static_cast<hipStream_t>(stream)) ++h->version;
   h->user = static_cast<hipStream_t>(stream);
   return TMX_OK;
 }
@@ -268,15 +210,16 @@ extern "C" int tmx_set_penalization(tmx_problem* h, double p) {
   }
   h->p = (int)p;
   h->has_penalty = true;
+  ++h->version;
   return TMX_OK;
 }
 
-static int need_penalty(const tmx_problem* h) {
+int need_penalty(const tmx_problem* h) {
   if (!h->has_penalty) return invalid("You must set penalization before calling penalizer");
   return TMX_OK;
 }
 
-static SolveParams merge(const SolveParams& base, const tmx_solve_opts* o) {
+SolveParams merge(const SolveParams& base, const tmx_solve_opts* o) {
   SolveParams p = base;
   if (!o) return p;
   if (o->rtol > 0) p.rtol = o->rtol;
@@ -287,7 +230,7 @@ static SolveParams merge(const SolveParams& base, const tmx_solve_opts* o) {
   return p;
 }
 
-static void export_stats(const SolveStats& s, tmx_solve_stats* o) {
+void export_stats(const SolveStats& s, tmx_solve_stats* o) {
   if (!o) return;
   o->iterations = s.iterations;
   o->outer_iterations = s.outer;
@@ -297,20 +240,17 @@ static void export_stats(const SolveStats& s, tmx_solve_stats* o) {
   o->levels = s.levels;
 }
 
-static size_t nverts(const tmx_problem* h) { return (size_t)(h->nx + 1) * (h->ny + 1); }
+size_t nverts(const tmx_problem* h) { return (size_t)(h->nx + 1) * (h->ny + 1); }
 // displacement unknowns: 2 per P1 vertex, or 2 per P2 lattice node
-static size_t ndofs(const tmx_problem* h) {
+size_t ndofs(const tmx_problem* h) {
   return h->degree == 2 ? 2 * h->g2.nodes() : 2 * nverts(h);
 }
-static const double* load_F(const tmx_problem* h) { return h->degree == 2 ? h->F2.get() : h->F.get(); }
+const double* load_F(const tmx_problem* h) { return h->degree == 2 ? h->F2.get() : h->F.get(); }
 
-static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
-                            const tmx_solve_opts* o = nullptr, bool sync = true) {
-  SolveStats st;
+SolveParams filter_params(const tmx_problem* h, const tmx_solve_opts* o) {
   SolveParams prm = h->fl_prm;
   if (o && o->precision >= 0) prm.precision = filter_precision(o->precision);
   prm.verify = false;  // filter statistics are not reported
-  prm.sync = sync;
   if (!h->fl_rtol_set) {
     // an order of magnitude below the elasticity solve's own tolerance, within [1e-12, 1e-9]:
     // the filtered design then adds nothing visible to the solve's error (at N = 1000 and a
@@ -318,13 +258,20 @@ static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
     const double el = (o && o->rtol > 0) ? o->rtol : h->el_prm.rtol;
     prm.rtol = std::min(1e-9, std::max(1e-12, 1e-1 * el));
   }
+  return prm;
+}
+
+static int filter_rhs_solve(tmx_problem* h, const double* b, double* w,
+                            const tmx_solve_opts* o = nullptr, bool sync = true) {
+  SolveStats st;
+  SolveParams prm = filter_params(h, o);
+  prm.sync = sync;
   const int rc = h->fl.solve(h->ws, b, w, prm, &st, h->stream);
   if (rc == TMX_ERR_NONFINITE) TMX_HIP(fill(nverts(h), NAN, w, h->stream));
   return rc;
 }
 
-// y = M x on the handle's stream (no Scope: for use inside entry points)
-static int mass_apply(tmx_problem* h, const double* x, double* y) {
+int mass_apply(tmx_problem* h, const double* x, double* y) {
   TMX_HIP(filter_sweep<double>(h->fl.levels_[0].sh64, FilterConst<double>::make(0.0, h->hx, h->hy),
                                kApply, x, nullptr, y, 0.0, nullptr, h->stream));
   return TMX_OK;
@@ -484,7 +431,7 @@ extern "C" int tmx_integrate(tmx_problem* h, const double* values, double* out) 
   return read_dot(h, nverts(h), h->lumped.get(), values, kScInt, out);
 }
 
-static int sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
+int sensitivity(tmx_problem* h, const double* xi, const double* u, double* b) {
   TMX_TRY(need_penalty(h));
   if (h->degree == 2) {
     TMX_HIP(p2_sensitivity(h->g2, xi, u, h->p, h->m, b, h->stream));
@@ -552,6 +499,13 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   const int frc = finish(rc);
   if (frc != rc) return frc;
   if (J) *J = h->ws.host[kScJ];
+  if (rc == TMX_OK && h->degree == 1) {
+    // every lazily built part of this scheme pair exists now (merged batches rely on it)
+    const SolveParams ep = merge(h->el_prm, opts), fp = filter_params(h, opts);
+    const bool e_ok = ep.precision == kSolveFP64 || ep.precision == kSolveMixed;
+    const bool f_ok = fp.precision == kSolveFP64 || fp.precision == kSolveMixed;
+    if (e_ok && f_ok) h->merge_ready[ep.precision][fp.precision] = true;
+  }
   return rc;
 }
 
@@ -592,6 +546,9 @@ extern "C" int tmx_load_vector(tmx_problem* h, double* F) {
 
 extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
   if (!h) return invalid("null handle");
+  // an option may change which launches a step makes, or leave parts to build lazily
+  ++h->version;
+  for (auto& r : h->merge_ready) r[0] = r[1] = false;
   Hierarchy* hs[2] = {&h->el, &h->fl};
   switch (option) {
     case TMX_OPT_FUSED:
@@ -746,6 +703,7 @@ extern "C" int tmx_precondition(tmx_problem* h, int which, int precision, const 
 
 extern "C" int tmx_set_tiles(tmx_problem* h, int rows64, int rows32) {
   if (!h) return invalid("null handle");
+  ++h->version;
   h->el.set_rows(rows64, rows32);
   h->fl.set_rows(rows64, rows32);
   return TMX_OK;
@@ -1006,92 +964,6 @@ extern "C" int tmx_bench_kernel(tmx_problem* h, int kernel, int precision, int r
     // written, p and q written: 96 B
     if (kernel == 5) *bytes = 96.0 * (double)nv;
   }
-  return TMX_OK;
-}
-
-// ------------------------------------------------------------------ batch
-// Independent designs run concurrently: each worker thread drives whole objective+gradient
-// steps of one design at a time on that design's own stream, so the latency-bound coarse
-// multigrid kernels of different designs overlap on the device.  Workers take designs
-// largest first from a shared counter.
-struct tmx_batch {
-  std::vector<tmx_problem*> probs;
-  std::vector<int> order;  // largest first
-  // host threads: up to 16 (one per core of a one-GPU share); on the 64-design workload 16
-  // measured 5% faster than 8 and 30% faster than 4
-  int workers = (int)std::min(16u, std::max(4u, std::thread::hardware_concurrency()));
-};
-
-extern "C" int tmx_batch_create(tmx_problem* const* problems, int count, tmx_batch** out) {
-  if (!problems || count <= 0 || !out) return invalid("bad batch arguments");
-  tmx_batch* b = new (std::nothrow) tmx_batch();
-  if (!b) return invalid("out of host memory");
-  for (int i = 0; i < count; ++i) {
-    if (!problems[i]) {
-      delete b;
-      return invalid("null problem in batch");
-    }
-    for (int j = 0; j < i; ++j)
-      if (problems[j] == problems[i]) {
-        delete b;
-        return invalid("a problem appears twice in the batch");
-      }
-    b->probs.push_back(problems[i]);
-    b->order.push_back(i);
-  }
-  std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
-    return nverts(b->probs[a]) > nverts(b->probs[c]);
-  });
-  *out = b;
-  return TMX_OK;
-}
-
-extern "C" void tmx_batch_destroy(tmx_batch* b) { delete b; }
-
-extern "C" int tmx_batch_set_workers(tmx_batch* b, int workers) {
-  if (!b) return invalid("null batch");
-  if (workers < 1 || workers > 64) return invalid("workers must be in [1, 64]");
-  b->workers = workers;
-  return TMX_OK;
-}
-
-extern "C" int tmx_batch_objective_and_gradient(tmx_batch* b, const double* const* rho,
-                                                double* J, double* const* w,
-                                                const tmx_solve_opts* opts,
-                                                tmx_solve_stats* stats) {
-  if (!b || !rho || !J || !w) return invalid("null argument");
-  const int n = (int)b->probs.size();
-  for (int i = 0; i < n; ++i)
-    if (!rho[i] || !w[i]) return invalid("null design pointer in batch");
-  std::vector<int> rc(n, TMX_OK);
-  std::vector<std::string> msg(n);
-  std::atomic<int> next{0};
-  auto work = [&]() {
-    for (;;) {
-      const int k = next.fetch_add(1);
-      if (k >= n) return;
-      const int i = b->order[k];
-      if (hipSetDevice(b->probs[i]->device) != hipSuccess) {
-        rc[i] = TMX_ERR_HIP;
-        msg[i] = "hipSetDevice failed in a batch worker";
-        continue;
-      }
-      rc[i] = tmx_objective_and_gradient(b->probs[i], rho[i], &J[i], nullptr, nullptr, w[i],
-                                         nullptr, opts, stats ? &stats[i] : nullptr);
-      if (rc[i] == TMX_ERR_NONFINITE) J[i] = NAN;
-      else if (rc[i] != TMX_OK) msg[i] = last_error_cstr();
-    }
-  };
-  const int nw = std::min(b->workers, n);
-  std::vector<std::thread> pool;
-  for (int t = 1; t < nw; ++t) pool.emplace_back(work);
-  work();
-  for (auto& t : pool) t.join();
-  for (int i = 0; i < n; ++i)
-    if (rc[i] != TMX_OK && rc[i] != TMX_ERR_NONFINITE) {
-      set_error("design " + std::to_string(i) + ": " + msg[i]);
-      return rc[i];
-    }
   return TMX_OK;
 }
 

@@ -126,7 +126,7 @@ __device__ __forceinline__ void band_row(const BandLevel<T>& L, const T* __restr
                                          int lx, int ly, int gx, int gy, unsigned g, unsigned nn,
                                          BandRow<T, BS>& R) {
   if constexpr (SRC == kSrcStencil) {
-    const T* __restrict__ st = L.st + g;
+    const auto st = glob(L.st) + g;
 #pragma unroll
     for (int e = 0; e < 9 * BS * BS; ++e) R.a[e] = st[(size_t)e * nn];
   } else {
@@ -362,8 +362,8 @@ __device__ __forceinline__ void band_stage_cells(const BandLevel<T>& L, T* cwl, 
           w[u][0] = w[u][1] = T(0);
           if (cx >= 0 && cy >= 0 && cx < L.nx && cy < L.ny) {
             const unsigned c = (unsigned)cy * (unsigned)L.nx + (unsigned)cx;
-            w[u][0] = L.cw[2 * c];
-            w[u][1] = L.cw[2 * c + 1];
+            w[u][0] = glob(L.cw)[2 * c];
+            w[u][1] = glob(L.cw)[2 * c + 1];
           }
         },
         [&](int u, int rx, int ry) {
@@ -376,22 +376,23 @@ __device__ __forceinline__ void band_stage_cells(const BandLevel<T>& L, T* cwl, 
 
 // tile origin of this workgroup
 template <typename T>
-__device__ __forceinline__ void band_tile(const BandLevel<T>& L, int& X0, int& Y0) {
+__device__ __forceinline__ void band_tile(const BandLevel<T>& L, unsigned bid, int& X0, int& Y0) {
   const unsigned ntx = (unsigned)((L.nx + L.tx) / L.tx);  // = ceil((nx + 1) / tx)
-  X0 = (int)(blockIdx.x % ntx) * L.tx;
-  Y0 = (int)(blockIdx.x / ntx) * L.ty;
+  X0 = (int)(bid % ntx) * L.tx;
+  Y0 = (int)(bid / ntx) * L.ty;
 }
 
 template <typename T, int BS, int SRC, bool ONE, int NT>
-__global__ __launch_bounds__(NT) void k_band_down(BandLevel<T> L,
-                                                            const T* __restrict__ b,
-                                                            T* __restrict__ xo,
-                                                            T* __restrict__ bc) {
+__device__ __forceinline__ void band_down_body(const BlkId blk, const BandLevel<T>& L,
+                                               const T* b_, T* xo_, T* bc_) {
+  const auto b = glob(b_);
+  const auto xo = glob(xo_);
+  const auto bc = glob(bc_);
   extern __shared__ __align__(16) unsigned char band_lds[];
   const int nu = L.nu_down, H = nu + 1;
   const int EX = L.tx + 2 * H, EY = L.ty + 2 * H, NE = EX * EY;
   int X0, Y0;
-  band_tile(L, X0, Y0);
+  band_tile(L, blk.id, X0, Y0);
   const int OX = X0 - H, OY = Y0 - H;
   T* bl = reinterpret_cast<T*>(band_lds);
   T* xa = bl + BS * NE;
@@ -494,16 +495,25 @@ __global__ __launch_bounds__(NT) void k_band_down(BandLevel<T> L,
 }
 
 template <typename T, int BS, int SRC, bool ONE, int NT>
-__global__ __launch_bounds__(NT) void k_band_up(BandLevel<T> L, const T* __restrict__ x,
-                                                          const T* __restrict__ ec,
-                                                          const T* __restrict__ b,
-                                                          T* __restrict__ y,
-                                                          double* __restrict__ dot) {
+__global__ __launch_bounds__(NT) void k_band_down(BandLevel<T> L,
+                                                            const T* __restrict__ b,
+                                                            T* __restrict__ xo,
+                                                            T* __restrict__ bc) {
+  band_down_body<T, BS, SRC, ONE, NT>(hw_blk(), L, b, xo, bc);
+}
+
+template <typename T, int BS, int SRC, bool ONE, int NT>
+__device__ __forceinline__ void band_up_body(const BlkId blk, const BandLevel<T>& L, const T* x_,
+                                             const T* ec_, const T* b_, T* y_, double* dot) {
+  const auto x = glob(x_);
+  const auto ec = glob(ec_);
+  const auto b = glob(b_);
+  const auto y = glob(y_);
   extern __shared__ __align__(16) unsigned char band_lds[];
   const int nu = L.nu_up, H = nu;
   const int EX = L.tx + 2 * H, EY = L.ty + 2 * H, NE = EX * EY;
   int X0, Y0;
-  band_tile(L, X0, Y0);
+  band_tile(L, blk.id, X0, Y0);
   const int OX = X0 - H, OY = Y0 - H;
   T* bl = reinterpret_cast<T*>(band_lds);
   T* xa = bl + BS * NE;
@@ -588,9 +598,18 @@ __global__ __launch_bounds__(NT) void k_band_up(BandLevel<T> L, const T* __restr
     if (dot) {
       const double v[1] = {part};
       double* const o[1] = {dot};
-      block_partials<1>(v, o);
+      block_partials<1>(v, o, blk.id);
     }
   }
+}
+
+template <typename T, int BS, int SRC, bool ONE, int NT>
+__global__ __launch_bounds__(NT) void k_band_up(BandLevel<T> L, const T* __restrict__ x,
+                                                          const T* __restrict__ ec,
+                                                          const T* __restrict__ b,
+                                                          T* __restrict__ y,
+                                                          double* __restrict__ dot) {
+  band_up_body<T, BS, SRC, ONE, NT>(hw_blk(), L, x, ec, b, y, dot);
 }
 
 // ---------------------------------------------------------------- host side
@@ -684,15 +703,13 @@ static bool band_valid(const BandLevel<T>& L) {
 template <typename T, int BS, int SRC, bool ONE, int NT = kBandThreads>
 static void launch_down(const BandLevel<T>& L, const T* b, T* x, T* bc, size_t lds,
                         hipStream_t s) {
-  k_band_down<T, BS, SRC, ONE, NT><<<band_tiles(L.nx, L.ny, L.tx, L.ty), NT, lds, s>>>(L, b, x,
-                                                                                        bc);
+  TMX_LAUNCH_OR_RECORD(NT, (band_down_body<T, BS, SRC, ONE, NT>), band_tiles(L.nx, L.ny, L.tx, L.ty), lds, s, (k_band_down<T, BS, SRC, ONE, NT>), L, b, x, bc);
 }
 
 template <typename T, int BS, int SRC, bool ONE, int NT = kBandThreads>
 static void launch_up(const BandLevel<T>& L, const T* x, const T* ec, const T* b, T* y,
                       double* dot, size_t lds, hipStream_t s) {
-  k_band_up<T, BS, SRC, ONE, NT><<<band_tiles(L.nx, L.ny, L.tx, L.ty), NT, lds, s>>>(L, x, ec, b,
-                                                                                      y, dot);
+  TMX_LAUNCH_OR_RECORD(NT, (band_up_body<T, BS, SRC, ONE, NT>), band_tiles(L.nx, L.ny, L.tx, L.ty), lds, s, (k_band_up<T, BS, SRC, ONE, NT>), L, x, ec, b, y, dot);
 }
 
 template <typename T, int BS>

@@ -167,19 +167,21 @@ __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
 // XCD each, as workgroups are dealt out to the XCDs in turn) take consecutive column groups,
 // so the workgroups that share a row's cache line share that XCD's L2.
 template <typename T, bool INV>
-__global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int groups, FftPlan plan,
-                                                          const V2<T>* __restrict__ tw,
-                                                          const T* __restrict__ src,
-                                                          T* __restrict__ dst,
-                                                          const T* __restrict__ rdot,
-                                                          double* __restrict__ part) {
+__device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, int groups,
+                                               const FftPlan& plan, const V2<T>* tw_,
+                                               const T* src_, T* dst_, const T* rdot_,
+                                               double* part) {
+  const auto tw = glob(tw_);
+  const auto src = glob(src_);
+  const auto dst = glob(dst_);
+  const auto rdot = glob(rdot_);
   constexpr int C = DctPrecond<T>::kCols, NS = C / 2;
   extern __shared__ __align__(16) unsigned char dct_lds[];
   V2<T>* z = reinterpret_cast<V2<T>*>(dct_lds);
   __shared__ V2<T> ends[NS][2];  // each sequence's points 0 and n before the transform
   const int L = 2 * n;
-  const int per = (int)gridDim.x / 8;
-  const int g = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
+  const int per = (int)blk.n / 8;
+  const int g = ((int)blk.id % 8) * per + (int)blk.id / 8;
   double acc = 0.0;
   if (g < groups) {
     const int x0 = g * C;
@@ -245,9 +247,19 @@ __global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int gr
     if (part) {
       const double v[1] = {acc};
       double* const out[1] = {part};
-      block_partials<1>(v, out);
+      block_partials<1>(v, out, blk.id);
     }
   }
+}
+
+template <typename T, bool INV>
+__global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int groups, FftPlan plan,
+                                                          const V2<T>* __restrict__ tw,
+                                                          const T* __restrict__ src,
+                                                          T* __restrict__ dst,
+                                                          const T* __restrict__ rdot,
+                                                          double* __restrict__ part) {
+  dct1_cols_body<T, INV>(hw_blk(), n, mx, groups, plan, tw, src, dst, rdot, part);
 }
 
 // One tridiagonal solve along x per mode row: row m of `modes` is mode m (workgroup m).  Each
@@ -277,8 +289,9 @@ static size_t dct_tri_lds(int mx) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off, double hxhy,
-                                                         double cy, T* __restrict__ modes) {
+__device__ __forceinline__ void dct_tri_body(const BlkId blk, int mx, int n, T a_off,
+                                             double hxhy, double cy, T* modes_) {
+  const auto modes = glob(modes_);
   extern __shared__ __align__(16) unsigned char tri_lds[];
   const int k = (mx + kTriThreads - 1) / kTriThreads;
   const int nblk = (mx + k - 1) / k;
@@ -289,7 +302,7 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
   T* rb = ra + kTriThreads;                 // rows' P, A, G
   T* rc = rb + kTriThreads;
   T* rf = rc + kTriThreads;
-  const int mode = (int)blockIdx.x;
+  const int mode = (int)blk.id;
   T* row = modes + (size_t)mode * mx;
   const double lam = 2.0 - 2.0 * std::cos(M_PI * (double)mode / (double)n);
   const T c = (T)(hxhy + cy * lam);
@@ -392,6 +405,12 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
 }
 
 template <typename T>
+__global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off, double hxhy,
+                                                         double cy, T* __restrict__ modes) {
+  dct_tri_body<T>(hw_blk(), mx, n, a_off, hxhy, cy, modes);
+}
+
+template <typename T>
 bool DctPrecond<T>::supported(int nx, int ny) {
   FftPlan p;
   return ny >= 1 && nx >= 1 && nx + 1 <= kMaxRow && fft_plan(2 * ny, p) &&
@@ -445,13 +464,12 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
   const V2<T>* tw = reinterpret_cast<const V2<T>*>(tw_.get());
   T* t = t_.get();
   // forward DCT-I along y: t[m] = sum_j r[j] cos(pi j m / n)
-  k_dct1_cols<T, false><<<nblk, kFftThreads, lds, s>>>(n, mx, groups, plan_, tw, r, t, nullptr,
-                                                       nullptr);
+  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, false>), nblk, lds, s, (k_dct1_cols<T, false>), n, mx, groups, plan_, tw, r, t, nullptr, nullptr);
   TMX_HIP(hipGetLastError());
   // tridiagonal solves along x, one workgroup per mode (scaled by 1 / D_m there)
   const T a_off = (T)(eps_ * eps_ * hy_ / hx_);
   const double cy = eps_ * eps_ * hx_ / hy_;
-  k_dct_tri<T><<<n + 1, kTriThreads, dct_tri_lds<T>(mx), s>>>(mx, n, a_off, hx_ * hy_, cy, t);
+  TMX_LAUNCH_OR_RECORD(kTriThreads, (dct_tri_body<T>), n + 1, dct_tri_lds<T>(mx), s, (k_dct_tri<T>), mx, n, a_off, hx_ * hy_, cy, t);
   TMX_HIP(hipGetLastError());
   // inverse: z[j] = sum_m t[m] cos(pi j m / n), with the partials of r.z
   double* part = nullptr;
@@ -463,7 +481,7 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
     rz->nblocks = nblk;
     part = rz->partials;
   }
-  k_dct1_cols<T, true><<<nblk, kFftThreads, lds, s>>>(n, mx, groups, plan_, tw, t, z, r, part);
+  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, true>), nblk, lds, s, (k_dct1_cols<T, true>), n, mx, groups, plan_, tw, t, z, r, part);
   TMX_HIP(hipGetLastError());
   return 0;
 }

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "tmx_kernels.h"
+#include "tmx_multi.h"
 
 namespace tmx {
 
@@ -97,8 +98,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// block bid of the launch writes its partials to out[k][bid]
 template <int NRED>
-__device__ __forceinline__ void block_partials(const double (&v)[NRED], double* const (&out)[NRED]) {
+__device__ __forceinline__ void block_partials(const double (&v)[NRED], double* const (&out)[NRED],
+                                               unsigned bid) {
   __shared__ double sh[NRED][kBlock / kWave];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
@@ -113,9 +116,13 @@ __device__ __forceinline__ void block_partials(const double (&v)[NRED], double* 
     for (int k = 0; k < NRED; ++k) {
       double s = 0.0;
       for (int i = 0; i < nw; ++i) s += sh[k][i];
-      out[k][blockIdx.x] = s;
+      out[k][bid] = s;
     }
   }
+}
+template <int NRED>
+__device__ __forceinline__ void block_partials(const double (&v)[NRED], double* const (&out)[NRED]) {
+  block_partials<NRED>(v, out, blockIdx.x);
 }
 
 // fixed-order sum of n partials by one block (all threads receive the result)

@@ -226,6 +226,15 @@ template <typename TI, typename TO>
 hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s);
 
 // ------------------------------------------------------------------ vectors
+// independent vector copies in one launch (tasks: device array of count entries, <= 65535)
+struct CopyTask {
+  const double* src;
+  double* dst;
+  size_t n;
+};
+hipError_t copy_tasks(const CopyTask* tasks, int count, hipStream_t s);
+// *host_dst = *src by a kernel (host_dst: device address of pinned host memory)
+hipError_t publish_scalar(const double* src, double* host_dst, hipStream_t s);
 template <typename T>
 hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s);
 // y += (*a) x with a device scalar

@@ -125,21 +125,19 @@ struct RowIn {
 };
 
 template <typename T, bool XI, int OP>
-__global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const T* __restrict__ x,
-                                                        const T* __restrict__ b, T* __restrict__ y,
-                                                        T omega, double* __restrict__ dpart,
-                                                        SweepExtra<T> ex) {
+__device__ __forceinline__ void elast_sweep_body(const BlkId blk, const ElastOpArgs<T>& a,
+                                                 const T* x, const T* b, T* y, T omega,
+                                                 double* dpart, const SweepExtra<T>& ex) {
   using V = V2<T>;
   constexpr bool RR = (OP == kResidualRestrict);  // residual -> nested restriction (y coarse)
   constexpr bool PS = (OP == kProlongSmooth);     // x + P e, then one Jacobi sweep
   constexpr bool AU = (OP == kApplyUpdate);       // p = z + beta p_old, y = A p
-  const T* __restrict__ ec = ex.ec;
   constexpr bool SMOOTH = (OP == kSmooth || OP == kSmoothZero || PS);
   constexpr int H = RR ? 2 : 1;                   // halo lanes on each side of the strip
   const SweepShape sh = a.sh;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -155,16 +153,17 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
   // cell rows swept: a node row is complete once the cell rows below and above it are done
   const int cy_begin = RR ? r0 - 2 : r0 - 1;
   const int cy_end = !wave_ok ? cy_begin : (RR ? min(rend, ny) + 1 : rend);
-  const V* X = reinterpret_cast<const V*>(x);
-  const V* B = reinterpret_cast<const V*>(b);
-  V* Y = reinterpret_cast<V*>(y);
-  const V* CW = reinterpret_cast<const V*>(a.cellw);
-  const V* EC = reinterpret_cast<const V*>(ec);
-  V* PN = reinterpret_cast<V*>(ex.pnew);
-  const float2* ZL = reinterpret_cast<const float2*>(ex.zlow);
-  const T beta = AU ? (T)*ex.beta : T(0);
-  V* XK = reinterpret_cast<V*>(ex.xk);
-  const T alpha = (AU && ex.xk) ? (T)*ex.alpha : T(0);
+  const auto X = glob(reinterpret_cast<const V*>(x));
+  const auto B = glob(reinterpret_cast<const V*>(b));
+  const auto Y = glob(reinterpret_cast<V*>(y));
+  const auto CW = glob(reinterpret_cast<const V*>(a.cellw));
+  const auto XIP = glob(a.xi);
+  const auto EC = glob(reinterpret_cast<const V*>(ex.ec));
+  const auto PN = glob(reinterpret_cast<V*>(ex.pnew));
+  const auto ZL = glob(reinterpret_cast<const float2*>(ex.zlow));
+  const T beta = AU ? (T)*glob(ex.beta) : T(0);
+  const auto XK = glob(reinterpret_cast<V*>(ex.xk));
+  const T alpha = (AU && ex.xk) ? (T)*glob(ex.alpha) : T(0);
   const ElastConst<T> k = a.k;
 
   auto row_fixed = [&](int iy) {
@@ -205,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     return v;
   };
   auto ld_xi = [&](int iy) -> T {
-    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(unsigned)(iy * nvx + c)];
+    if (XI && colv && iy >= 0 && iy <= ny) return XIP[(unsigned)(iy * nvx + c)];
     return T(0);
   };
 
@@ -334,9 +333,17 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const 
     if (want_dot) {
       const double v[1] = {dot};
       double* const o[1] = {dpart};
-      block_partials<1>(v, o);
+      block_partials<1>(v, o, blk.id);
     }
   }
+}
+
+template <typename T, bool XI, int OP>
+__global__ __launch_bounds__(kBlock) void k_elast_sweep(ElastOpArgs<T> a, const T* __restrict__ x,
+                                                        const T* __restrict__ b, T* __restrict__ y,
+                                                        T omega, double* __restrict__ dpart,
+                                                        SweepExtra<T> ex) {
+  elast_sweep_body<T, XI, OP>(hw_blk(), a, x, b, y, omega, dpart, ex);
 }
 
 template <typename T>
@@ -360,8 +367,9 @@ hipError_t elast_sweep(const ElastOpArgs<T>& a0, int op, const T* x, const T* b,
     dot_xy->nblocks = nb;
   }
   const bool xi = a.xi != nullptr;
-#define TMX_LAUNCH(XIV, OPV) \
-  k_elast_sweep<T, XIV, OPV><<<nb, kBlock, 0, s>>>(a, x, b, y, omega, dp, ex)
+#define TMX_LAUNCH(XIV, OPV)                                                              \
+  TMX_LAUNCH_OR_RECORD(kBlock, (elast_sweep_body<T, XIV, OPV>), nb, 0, s,                   \
+                       (k_elast_sweep<T, XIV, OPV>), a, x, b, y, omega, dp, ex)
 #define TMX_CASE(OPV) \
   case OPV: if (xi) TMX_LAUNCH(true, OPV); else TMX_LAUNCH(false, OPV); break;
   switch (op) {
@@ -395,19 +403,16 @@ template hipError_t elast_sweep<float>(const ElastOpArgs<float>&, int, const flo
 // sweep's node rows cy - 1 and cy and completes its node row cy - 1.  A chunk of R node rows
 // sweeps R + 3 cell rows (node rows r0 - 1 .. rend of the first sweep).
 template <typename T, bool XI, int OP1>
-__global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
-                                                             const T* __restrict__ x,
-                                                             const T* __restrict__ b,
-                                                             T* __restrict__ y, T om1, T om2,
-                                                             double* __restrict__ dpart,
-                                                             const T* __restrict__ ec) {
+__device__ __forceinline__ void elast_sweep_pair_body(const BlkId blk, const ElastOpArgs<T>& a,
+                                                      const T* x, const T* b, T* y, T om1, T om2,
+                                                      double* dpart, const T* ec) {
   using V = V2<T>;
   constexpr bool PS = (OP1 == kProlongSmooth);
   constexpr int H = 2;
   const SweepShape sh = a.sh;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -423,11 +428,12 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
   const int cy_begin = r0 - 2;
   const int cy_end = wave_ok ? rend + 1 : cy_begin;
   const int first_row = max(0, r0 - 1);  // first node row of the first sweep with a full diagonal
-  const V* X = reinterpret_cast<const V*>(x);
-  const V* B = reinterpret_cast<const V*>(b);
-  V* Y = reinterpret_cast<V*>(y);
-  const V* CW = reinterpret_cast<const V*>(a.cellw);
-  const V* EC = reinterpret_cast<const V*>(ec);
+  const auto X = glob(reinterpret_cast<const V*>(x));
+  const auto B = glob(reinterpret_cast<const V*>(b));
+  const auto Y = glob(reinterpret_cast<V*>(y));
+  const auto CW = glob(reinterpret_cast<const V*>(a.cellw));
+  const auto XIP = glob(a.xi);
+  const auto EC = glob(reinterpret_cast<const V*>(ec));
   const ElastConst<T> k = a.k;
   const V zero = {T(0), T(0)};
 
@@ -452,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
     return v;
   };
   auto ld_xi = [&](int iy) -> T {
-    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(unsigned)(iy * nvx + c)];
+    if (XI && colv && iy >= 0 && iy <= ny) return XIP[(unsigned)(iy * nvx + c)];
     return T(0);
   };
 
@@ -491,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
       r.e = EC[(size_t)J0 * (nxc + 1) + I0];
       r.e1 = EC[(size_t)(J0 + oy) * (nxc + 1) + I0 + ox];
     }
-    if constexpr (XI) r.z = a.xi[(unsigned)(nt * nvx + cn)];
+    if constexpr (XI) r.z = XIP[(unsigned)(nt * nvx + cn)];
     else r.w = CW[(unsigned)(cr * nx + cc)];
     r.b = B[(unsigned)(nb * nvx + cn)];
     return r;
@@ -622,8 +628,18 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
   if (want_dot) {
     const double v[1] = {dot};
     double* const o[1] = {dpart};
-    block_partials<1>(v, o);
+    block_partials<1>(v, o, blk.id);
   }
+}
+
+template <typename T, bool XI, int OP1>
+__global__ __launch_bounds__(kBlock) void k_elast_sweep_pair(ElastOpArgs<T> a,
+                                                             const T* __restrict__ x,
+                                                             const T* __restrict__ b,
+                                                             T* __restrict__ y, T om1, T om2,
+                                                             double* __restrict__ dpart,
+                                                             const T* __restrict__ ec) {
+  elast_sweep_pair_body<T, XI, OP1>(hw_blk(), a, x, b, y, om1, om2, dpart, ec);
 }
 
 template <typename T>
@@ -644,8 +660,9 @@ hipError_t elast_sweep_pair(const ElastOpArgs<T>& a0, int op1, int rows, const T
     dot_by->nblocks = nb;
   }
   const bool xi = a.xi != nullptr;
-#define TMX_LAUNCH(XIV, OPV) \
-  k_elast_sweep_pair<T, XIV, OPV><<<nb, kBlock, 0, s>>>(a, x, b, y, om1, om2, dp, ec)
+#define TMX_LAUNCH(XIV, OPV)                                                              \
+  TMX_LAUNCH_OR_RECORD(kBlock, (elast_sweep_pair_body<T, XIV, OPV>), nb, 0, s,              \
+                       (k_elast_sweep_pair<T, XIV, OPV>), a, x, b, y, om1, om2, dp, ec)
   if (op1 == kSmoothZero) {
     if (xi) TMX_LAUNCH(true, kSmoothZero);
     else TMX_LAUNCH(false, kSmoothZero);
@@ -672,16 +689,13 @@ template hipError_t elast_sweep_pair<float>(const ElastOpArgs<float>&, int, int,
 // coarse node row J.  The lanes carry four halo columns on each side (56 output columns); a
 // chunk of R (even) node rows sweeps R + 6 cell rows.
 template <typename T, bool XI>
-__global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a,
-                                                                const T* __restrict__ b,
-                                                                T* __restrict__ y,
-                                                                T* __restrict__ yc, T om1,
-                                                                T om2) {
+__device__ __forceinline__ void elast_sweep_pair_rr_body(const BlkId blk, const ElastOpArgs<T>& a,
+                                                         const T* b, T* y, T* yc, T om1, T om2) {
   using V = V2<T>;
   constexpr int H = 4;
   const SweepShape sh = a.sh;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -697,10 +711,11 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
   const int cy_begin = r0 - 4;
   const int cy_end = wave_ok ? rend + 2 : cy_begin;
   const int first_row = max(0, r0 - 3);  // first node row of the first sweep with a full diagonal
-  const V* B = reinterpret_cast<const V*>(b);
-  V* Y = reinterpret_cast<V*>(y);
-  V* YC = reinterpret_cast<V*>(yc);
-  const V* CW = reinterpret_cast<const V*>(a.cellw);
+  const auto B = glob(reinterpret_cast<const V*>(b));
+  const auto Y = glob(reinterpret_cast<V*>(y));
+  const auto YC = glob(reinterpret_cast<V*>(yc));
+  const auto CW = glob(reinterpret_cast<const V*>(a.cellw));
+  const auto XIP = glob(a.xi);
   const ElastConst<T> k = a.k;
   const V zero = {T(0), T(0)};
 
@@ -708,7 +723,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
     return colfix || ((fixed & kBottom) && iy == 0) || ((fixed & kTop) && iy == ny);
   };
   auto ld_xi = [&](int iy) -> T {
-    if (XI && colv && iy >= 0 && iy <= ny) return a.xi[(unsigned)(iy * nvx + c)];
+    if (XI && colv && iy >= 0 && iy <= ny) return XIP[(unsigned)(iy * nvx + c)];
     return T(0);
   };
 
@@ -740,7 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
     RowIn<T> r;
     const int nt = min(max(row + 1, 0), ny), nb = min(max(row, 0), ny);
     const int cr = min(max(row, 0), ny - 1);
-    if constexpr (XI) r.z = a.xi[(unsigned)(nt * nvx + cn)];
+    if constexpr (XI) r.z = XIP[(unsigned)(nt * nvx + cn)];
     else r.w = CW[(unsigned)(cr * nx + cc)];
     r.b = B[(unsigned)(nb * nvx + cn)];
     return r;
@@ -879,6 +894,15 @@ __global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a
   }
 }
 
+template <typename T, bool XI>
+__global__ __launch_bounds__(kBlock) void k_elast_sweep_pair_rr(ElastOpArgs<T> a,
+                                                                const T* __restrict__ b,
+                                                                T* __restrict__ y,
+                                                                T* __restrict__ yc, T om1,
+                                                                T om2) {
+  elast_sweep_pair_rr_body<T, XI>(hw_blk(), a, b, y, yc, om1, om2);
+}
+
 template <typename T>
 hipError_t elast_sweep_pair_rr(const ElastOpArgs<T>& a0, int rows, const T* b, T* y, T* yc,
                                T om1, T om2, hipStream_t s) {
@@ -888,8 +912,12 @@ hipError_t elast_sweep_pair_rr(const ElastOpArgs<T>& a0, int rows, const T* b, T
   ElastOpArgs<T> a = a0;
   a.sh = SweepShape::make(a0.sh.nx, a0.sh.ny, std::max(2, rows & ~1), kWave - 2 * 4);
   const int nb = a.sh.blocks();
-  if (a.xi) k_elast_sweep_pair_rr<T, true><<<nb, kBlock, 0, s>>>(a, b, y, yc, om1, om2);
-  else k_elast_sweep_pair_rr<T, false><<<nb, kBlock, 0, s>>>(a, b, y, yc, om1, om2);
+  if (a.xi)
+    TMX_LAUNCH_OR_RECORD(kBlock, (elast_sweep_pair_rr_body<T, true>), nb, 0, s,
+                         (k_elast_sweep_pair_rr<T, true>), a, b, y, yc, om1, om2);
+  else
+    TMX_LAUNCH_OR_RECORD(kBlock, (elast_sweep_pair_rr_body<T, false>), nb, 0, s,
+                         (k_elast_sweep_pair_rr<T, false>), a, b, y, yc, om1, om2);
   return hipGetLastError();
 }
 template hipError_t elast_sweep_pair_rr<double>(const ElastOpArgs<double>&, int, const double*,
@@ -898,9 +926,15 @@ template hipError_t elast_sweep_pair_rr<float>(const ElastOpArgs<float>&, int, c
                                                float*, float*, float, float, hipStream_t);
 
 template <typename T>
-__global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
-                                     T wmin, V2<T>* __restrict__ cw) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void elast_cell_weights_body(const BlkId blk, int nx,
+                                                        int ny,
+                                                        const T* xi,
+                                                        int p,
+                                                        T m,
+                                                        T cp,
+                                                        T wmin,
+                                                        V2<T>* cw) {
+  const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (i >= (size_t)nx * ny) return;
   const int cx = (int)(i % nx), cy = (int)(i / nx);
   const size_t v0 = (size_t)cy * (nx + 1) + cx;
@@ -910,10 +944,15 @@ __global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, i
 }
 
 template <typename T>
+__global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
+                                     T wmin, V2<T>* __restrict__ cw) {
+  elast_cell_weights_body<T>(hw_blk(), nx, ny, xi, p, m, cp, wmin, cw);
+}
+
+template <typename T>
 hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,
                               hipStream_t s) {
-  k_elast_cell_weights<T><<<grid1d((size_t)nx * ny), kBlock, 0, s>>>(
-      nx, ny, xi, p, m, cp, wmin, reinterpret_cast<V2<T>*>(cellw));
+  TMX_LAUNCH_OR_RECORD(kBlock, (elast_cell_weights_body<T>), grid1d((size_t)nx * ny), 0, s, (k_elast_cell_weights<T>), nx, ny, xi, p, m, cp, wmin, reinterpret_cast<V2<T>*>(cellw));
   return hipGetLastError();
 }
 template hipError_t elast_cell_weights<double>(int, int, const double*, int, double, double,
@@ -924,9 +963,11 @@ template hipError_t elast_cell_weights<float>(int, int, const float*, int, float
 // coarse triangle A (lower right) = fine (0,0).A + (1,0).A + (1,0).B + (1,1).A
 // coarse triangle B (upper left)  = fine (0,0).B + (0,1).A + (0,1).B + (1,1).B
 template <typename T>
-__global__ void k_restrict_weights(int nxc, int nyc, const V2<T>* __restrict__ f,
-                                   V2<T>* __restrict__ cw) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void restrict_weights_body(const BlkId blk, int nxc,
+                                                      int nyc,
+                                                      const V2<T>* f,
+                                                      V2<T>* cw) {
+  const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (i >= (size_t)nxc * nyc) return;
   const int cx = (int)(i % nxc), cy = (int)(i / nxc);
   const int nxf = 2 * nxc;
@@ -938,9 +979,14 @@ __global__ void k_restrict_weights(int nxc, int nyc, const V2<T>* __restrict__ f
 }
 
 template <typename T>
+__global__ void k_restrict_weights(int nxc, int nyc, const V2<T>* __restrict__ f,
+                                   V2<T>* __restrict__ cw) {
+  restrict_weights_body<T>(hw_blk(), nxc, nyc, f, cw);
+}
+
+template <typename T>
 hipError_t elast_restrict_weights(int nxc, int nyc, const T* fine, T* coarse, hipStream_t s) {
-  k_restrict_weights<T><<<grid1d((size_t)nxc * nyc), kBlock, 0, s>>>(
-      nxc, nyc, reinterpret_cast<const V2<T>*>(fine), reinterpret_cast<V2<T>*>(coarse));
+  TMX_LAUNCH_OR_RECORD(kBlock, (restrict_weights_body<T>), grid1d((size_t)nxc * nyc), 0, s, (k_restrict_weights<T>), nxc, nyc, reinterpret_cast<const V2<T>*>(fine), reinterpret_cast<V2<T>*>(coarse));
   return hipGetLastError();
 }
 template hipError_t elast_restrict_weights<double>(int, int, const double*, double*, hipStream_t);
@@ -955,11 +1001,15 @@ __constant__ double c_tri_grad[2][3][2] = {{{-1, 0}, {1, -1}, {0, 1}}, {{0, -1},
 __device__ inline int st_index(int dx, int dy) { return (dy + 1) * 3 + (dx + 1); }
 
 template <typename T>
-__global__ void k_elast_to_stencil(ElastOpArgs<T> a, T* __restrict__ st,
-                                   double hx, double hy, double lam, double mu) {
+__device__ __forceinline__ void elast_to_stencil_body(const BlkId blk, const ElastOpArgs<T>& a,
+                                                      T* st,
+                                                      double hx,
+                                                      double hy,
+                                                      double lam,
+                                                      double mu) {
   const int nx = a.sh.nx, ny = a.sh.ny;
   const size_t nn = (size_t)(nx + 1) * (ny + 1);
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t v = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
   double acc[9][4];
@@ -1012,6 +1062,12 @@ __global__ void k_elast_to_stencil(ElastOpArgs<T> a, T* __restrict__ st,
     for (int e = 0; e < 4; ++e) st[(size_t)(kk * 4 + e) * nn + v] = (T)acc[kk][e];
 }
 
+template <typename T>
+__global__ void k_elast_to_stencil(ElastOpArgs<T> a, T* __restrict__ st,
+                                   double hx, double hy, double lam, double mu) {
+  elast_to_stencil_body<T>(hw_blk(), a, st, hx, hy, lam, mu);
+}
+
 // lam, mu and a = hy/hx are recovered from the force constants (lam_h = l/2, mu_h = m/2,
 // mu_a = m a/2); the element matrices depend on hx, hy only through a.
 template <>
@@ -1020,7 +1076,7 @@ hipError_t elast_to_stencil<double>(const ElastOpArgs<double>& a, double* st, hi
   const double aa = a.k.mu_a / a.k.mu_h;  // = hy/hx
   const double mu = 2 * a.k.mu_h, lam = 2 * a.k.lam_h;
   const size_t nn = (size_t)(a.sh.nx + 1) * (a.sh.ny + 1);
-  k_elast_to_stencil<double><<<grid1d(nn), kBlock, 0, s>>>(a, st, 1.0, aa, lam, mu);
+  TMX_LAUNCH_OR_RECORD(kBlock, (elast_to_stencil_body<double>), grid1d(nn), 0, s, (k_elast_to_stencil<double>), a, st, 1.0, aa, lam, mu);
   return hipGetLastError();
 }
 
@@ -1058,13 +1114,14 @@ struct SensArgs {
   double lam, mu, hx, hy;
 };
 
-__global__ __launch_bounds__(kBlock) void k_elast_sensitivity(SensArgs a, const double* __restrict__ xi,
-                                                             const double* __restrict__ u,
-                                                             double* __restrict__ out) {
+__device__ __forceinline__ void elast_sensitivity_body(const BlkId blk, const SensArgs& a,
+                                                       const double* xi,
+                                                       const double* u,
+                                                       double* out) {
   using V = V2<double>;
   const SweepShape sh = a.sh;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -1125,6 +1182,12 @@ __global__ __launch_bounds__(kBlock) void k_elast_sensitivity(SensArgs a, const 
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_elast_sensitivity(SensArgs a, const double* __restrict__ xi,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ out) {
+  elast_sensitivity_body(hw_blk(), a, xi, u, out);
+}
+
 hipError_t elast_sensitivity(const SweepShape& sh, const double* xi, const double* u, int p,
                              double m, double lam, double mu, double hx, double hy, double* out,
                              hipStream_t s) {
@@ -1140,7 +1203,7 @@ hipError_t elast_sensitivity(const SweepShape& sh, const double* xi, const doubl
   a.mu = mu;
   a.hx = hx;
   a.hy = hy;
-  k_elast_sensitivity<<<sh.blocks(), kBlock, 0, s>>>(a, xi, u, out);
+  TMX_LAUNCH_OR_RECORD(kBlock, (elast_sensitivity_body), sh.blocks(), 0, s, (k_elast_sensitivity), a, xi, u, out);
   return hipGetLastError();
 }
 
@@ -1230,23 +1293,27 @@ hipError_t elast_assemble_load(const LoadDesc& d, double* F, hipStream_t s) {
 
 // ============================================================================ filter
 template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterConst<T> k,
-                                                         const T* __restrict__ x,
-                                                         const T* __restrict__ b,
-                                                         T* __restrict__ y, T omega,
-                                                         double* __restrict__ dpart,
-                                                         SweepExtra<T> ex) {
+__device__ __forceinline__ void filter_sweep_body(const BlkId blk, const SweepShape& sh,
+                                                  const FilterConst<T>& k, const T* x_,
+                                                  const T* b_, T* y_, T omega, double* dpart,
+                                                  const SweepExtra<T>& ex) {
   constexpr bool RR = (OP == kResidualRestrict);
   constexpr bool PS = (OP == kProlongSmooth);
   constexpr bool AU = (OP == kApplyUpdate);
-  const T* __restrict__ ec = ex.ec;
-  const T beta = AU ? (T)*ex.beta : T(0);
-  const T alpha = (AU && ex.xk) ? (T)*ex.alpha : T(0);
+  const auto x = glob(x_);
+  const auto b = glob(b_);
+  const auto y = glob(y_);
+  const auto ec = glob(ex.ec);
+  const auto zlow = glob(ex.zlow);
+  const auto pnew = glob(ex.pnew);
+  const auto xk = glob(ex.xk);
+  const T beta = AU ? (T)*glob(ex.beta) : T(0);
+  const T alpha = (AU && ex.xk) ? (T)*glob(ex.alpha) : T(0);
   constexpr bool SMOOTH = (OP == kSmooth || OP == kSmoothZero || PS);
   constexpr int H = RR ? 2 : 1;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -1266,7 +1333,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
     if (OP != kSmoothZero && ok) v = x[(size_t)iy * nvx + c];
     if constexpr (AU) {
       po_next = v;
-      if (ok) v = (ex.zlow ? (T)ex.zlow[(size_t)iy * nvx + c] : b[(size_t)iy * nvx + c]) + beta * v;
+      if (ok) v = (ex.zlow ? (T)zlow[(size_t)iy * nvx + c] : b[(size_t)iy * nvx + c]) + beta * v;
     }
     if constexpr (PS) {
       if (ok) {
@@ -1328,8 +1395,8 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
       if constexpr (OP == kApply || AU) {
         y[idx] = acc_b;
         if constexpr (AU) {
-          ex.pnew[idx] = ub;
-          if (ex.xk) ex.xk[idx] += alpha * po_b;  // deferred x update (see SweepExtra)
+          pnew[idx] = ub;
+          if (ex.xk) xk[idx] += alpha * po_b;  // deferred x update (see SweepExtra)
         }
         if (want_dot) dot += (double)ub * (double)acc_b;
       } else if constexpr (OP == kResidual) {
@@ -1351,9 +1418,19 @@ __global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterCo
     if (want_dot) {
       const double v[1] = {dot};
       double* const o[1] = {dpart};
-      block_partials<1>(v, o);
+      block_partials<1>(v, o, blk.id);
     }
   }
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_filter_sweep(SweepShape sh, FilterConst<T> k,
+                                                         const T* __restrict__ x,
+                                                         const T* __restrict__ b,
+                                                         T* __restrict__ y, T omega,
+                                                         double* __restrict__ dpart,
+                                                         SweepExtra<T> ex) {
+  filter_sweep_body<T, OP>(hw_blk(), sh, k, x, b, y, omega, dpart, ex);
 }
 
 template <typename T>
@@ -1377,7 +1454,7 @@ hipError_t filter_sweep(const SweepShape& sh0, const FilterConst<T>& k, int op, 
     dot_xy->nblocks = nb;
   }
 #define TMX_CASE(OPV) \
-  case OPV: k_filter_sweep<T, OPV><<<nb, kBlock, 0, s>>>(sh, k, x, b, y, omega, dp, ex); break;
+  case OPV: TMX_LAUNCH_OR_RECORD(kBlock, (filter_sweep_body<T, OPV>), nb, 0, s, (k_filter_sweep<T, OPV>), sh, k, x, b, y, omega, dp, ex); break;
   switch (op) {
     TMX_CASE(kApply)
     TMX_CASE(kResidual)
@@ -1740,10 +1817,13 @@ __device__ __forceinline__ Blk<T, BS> ldb(const T* p, size_t node) {
 }
 
 template <typename T, int BS>
-__global__ void k_restrict_nested(int nxc, int nyc, unsigned fixed, const T* __restrict__ rf,
-                                  T* __restrict__ bc) {
+__device__ __forceinline__ void restrict_nested_body(const BlkId blk, int nxc,
+                                                     int nyc,
+                                                     unsigned fixed,
+                                                     const T* rf,
+                                                     T* bc) {
   const size_t nn = (size_t)(nxc + 1) * (nyc + 1);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (i >= nn) return;
   const int I = (int)(i % (nxc + 1)), J = (int)(i / (nxc + 1));
   const int nxf = 2 * nxc, nyf = 2 * nyc, nvf = nxf + 1;
@@ -1768,11 +1848,20 @@ __global__ void k_restrict_nested(int nxc, int nyc, unsigned fixed, const T* __r
 }
 
 template <typename T, int BS>
-__global__ void k_prolong_nested(int nxc, int nyc, unsigned fixed, const T* __restrict__ ec,
-                                 T* __restrict__ xf) {
+__global__ void k_restrict_nested(int nxc, int nyc, unsigned fixed, const T* __restrict__ rf,
+                                  T* __restrict__ bc) {
+  restrict_nested_body<T, BS>(hw_blk(), nxc, nyc, fixed, rf, bc);
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void prolong_nested_body(const BlkId blk, int nxc,
+                                                    int nyc,
+                                                    unsigned fixed,
+                                                    const T* ec,
+                                                    T* xf) {
   const int nxf = 2 * nxc, nyf = 2 * nyc;
   const size_t nn = (size_t)(nxf + 1) * (nyf + 1);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (i >= nn) return;
   const int fx = (int)(i % (nxf + 1)), fy = (int)(i / (nxf + 1));
   if (node_fixed(fixed, fx, fy, nxf, nyf)) return;
@@ -1794,13 +1883,19 @@ __global__ void k_prolong_nested(int nxc, int nyc, unsigned fixed, const T* __re
 }
 
 template <typename T, int BS>
+__global__ void k_prolong_nested(int nxc, int nyc, unsigned fixed, const T* __restrict__ ec,
+                                 T* __restrict__ xf) {
+  prolong_nested_body<T, BS>(hw_blk(), nxc, nyc, fixed, ec, xf);
+}
+
+template <typename T, int BS>
 hipError_t restrict_nested(int nxc, int nyc, unsigned fixed, const T* rf, T* bc, hipStream_t s) {
-  k_restrict_nested<T, BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1)), kBlock, 0, s>>>(nxc, nyc, fixed, rf, bc);
+  TMX_LAUNCH_OR_RECORD(kBlock, (restrict_nested_body<T, BS>), grid1d((size_t)(nxc + 1) * (nyc + 1)), 0, s, (k_restrict_nested<T, BS>), nxc, nyc, fixed, rf, bc);
   return hipGetLastError();
 }
 template <typename T, int BS>
 hipError_t prolong_nested(int nxc, int nyc, unsigned fixed, const T* ec, T* xf, hipStream_t s) {
-  k_prolong_nested<T, BS><<<grid1d((size_t)(2 * nxc + 1) * (2 * nyc + 1)), kBlock, 0, s>>>(nxc, nyc, fixed, ec, xf);
+  TMX_LAUNCH_OR_RECORD(kBlock, (prolong_nested_body<T, BS>), grid1d((size_t)(2 * nxc + 1) * (2 * nyc + 1)), 0, s, (k_prolong_nested<T, BS>), nxc, nyc, fixed, ec, xf);
   return hipGetLastError();
 }
 
@@ -1814,35 +1909,55 @@ __device__ __forceinline__ void prolong_bilinear_node(int nxf, int nyf, int nxc,
                                                       T* __restrict__ xf, size_t i);
 
 template <typename T, int BS>
+__device__ __forceinline__ void restrict_bilinear_body(const BlkId blk, int nxf,
+                                                       int nyf,
+                                                       int nxc,
+                                                       int nyc,
+                                                       unsigned fixed,
+                                                       const T* rf,
+                                                       T* bc) {
+  const unsigned nn = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
+  const unsigned i = blk.id * blockDim.x + threadIdx.x;
+  if (i < nn) restrict_bilinear_node<T, BS>(nxf, nyf, nxc, nyc, fixed, rf, bc, i);
+}
+
+template <typename T, int BS>
 __global__ void k_restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                                     const T* __restrict__ rf, T* __restrict__ bc) {
-  const unsigned nn = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nn) restrict_bilinear_node<T, BS>(nxf, nyf, nxc, nyc, fixed, rf, bc, i);
+  restrict_bilinear_body<T, BS>(hw_blk(), nxf, nyf, nxc, nyc, fixed, rf, bc);
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void prolong_bilinear_body(const BlkId blk, int nxf,
+                                                      int nyf,
+                                                      int nxc,
+                                                      int nyc,
+                                                      unsigned fixed,
+                                                      const T* ec,
+                                                      T* xf) {
+  const unsigned nn = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
+  const unsigned i = blk.id * blockDim.x + threadIdx.x;
+  if (i < nn) prolong_bilinear_node<T, BS>(nxf, nyf, nxc, nyc, fixed, ec, xf, i);
 }
 
 template <typename T, int BS>
 __global__ void k_prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
                                    const T* __restrict__ ec, T* __restrict__ xf) {
-  const unsigned nn = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nn) prolong_bilinear_node<T, BS>(nxf, nyf, nxc, nyc, fixed, ec, xf, i);
+  prolong_bilinear_body<T, BS>(hw_blk(), nxf, nyf, nxc, nyc, fixed, ec, xf);
 }
 
 template <typename T, int BS>
 hipError_t restrict_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* rf,
                              T* bc, hipStream_t s) {
   if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
-  k_restrict_bilinear<T, BS><<<grid1d((size_t)(nxc + 1) * (nyc + 1)), kBlock, 0, s>>>(
-      nxf, nyf, nxc, nyc, fixed, rf, bc);
+  TMX_LAUNCH_OR_RECORD(kBlock, (restrict_bilinear_body<T, BS>), grid1d((size_t)(nxc + 1) * (nyc + 1)), 0, s, (k_restrict_bilinear<T, BS>), nxf, nyf, nxc, nyc, fixed, rf, bc);
   return hipGetLastError();
 }
 template <typename T, int BS>
 hipError_t prolong_bilinear(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* ec,
                             T* xf, hipStream_t s) {
   if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
-  k_prolong_bilinear<T, BS><<<grid1d((size_t)(nxf + 1) * (nyf + 1)), kBlock, 0, s>>>(
-      nxf, nyf, nxc, nyc, fixed, ec, xf);
+  TMX_LAUNCH_OR_RECORD(kBlock, (prolong_bilinear_body<T, BS>), grid1d((size_t)(nxf + 1) * (nyf + 1)), 0, s, (k_prolong_bilinear<T, BS>), nxf, nyf, nxc, nyc, fixed, ec, xf);
   return hipGetLastError();
 }
 
@@ -1913,11 +2028,22 @@ __device__ __forceinline__ void s9_node(int nx, int ny, const T* __restrict__ st
 }
 
 template <typename T, int BS, int OP>
+__device__ __forceinline__ void stencil_op_body(const BlkId blk, int nx,
+                                                int ny,
+                                                const T* st,
+                                                const T* x,
+                                                const T* b,
+                                                T* y,
+                                                T omega) {
+  const size_t nn = (size_t)(nx + 1) * (ny + 1);
+  const size_t v = (size_t)blk.id * blockDim.x + threadIdx.x;
+  if (v < nn) s9_node<T, BS, OP>(nx, ny, st, x, b, y, omega, v);
+}
+
+template <typename T, int BS, int OP>
 __global__ void k_stencil_op(int nx, int ny, const T* __restrict__ st, const T* __restrict__ x,
                              const T* __restrict__ b, T* __restrict__ y, T omega) {
-  const size_t nn = (size_t)(nx + 1) * (ny + 1);
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < nn) s9_node<T, BS, OP>(nx, ny, st, x, b, y, omega, v);
+  stencil_op_body<T, BS, OP>(hw_blk(), nx, ny, st, x, b, y, omega);
 }
 
 template <typename T, int BS>
@@ -1925,10 +2051,10 @@ hipError_t stencil_op(int nx, int ny, const T* st, int op, const T* x, const T* 
                       hipStream_t s) {
   const unsigned g = grid1d((size_t)(nx + 1) * (ny + 1), 64);
   switch (op) {
-    case kApply: k_stencil_op<T, BS, kApply><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
-    case kResidual: k_stencil_op<T, BS, kResidual><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
-    case kSmooth: k_stencil_op<T, BS, kSmooth><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
-    default: k_stencil_op<T, BS, kSmoothZero><<<g, 64, 0, s>>>(nx, ny, st, x, b, y, omega); break;
+    case kApply: TMX_LAUNCH_OR_RECORD(64, (stencil_op_body<T, BS, kApply>), g, 0, s, (k_stencil_op<T, BS, kApply>), nx, ny, st, x, b, y, omega); break;
+    case kResidual: TMX_LAUNCH_OR_RECORD(64, (stencil_op_body<T, BS, kResidual>), g, 0, s, (k_stencil_op<T, BS, kResidual>), nx, ny, st, x, b, y, omega); break;
+    case kSmooth: TMX_LAUNCH_OR_RECORD(64, (stencil_op_body<T, BS, kSmooth>), g, 0, s, (k_stencil_op<T, BS, kSmooth>), nx, ny, st, x, b, y, omega); break;
+    default: TMX_LAUNCH_OR_RECORD(64, (stencil_op_body<T, BS, kSmoothZero>), g, 0, s, (k_stencil_op<T, BS, kSmoothZero>), nx, ny, st, x, b, y, omega); break;
   }
   return hipGetLastError();
 }
@@ -2240,7 +2366,7 @@ struct TailRowHolder {
 };
 
 template <typename T, int BS>
-__global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
+__device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc<T>& d) {
   extern __shared__ __align__(16) unsigned char tail_lds[];
   T* lds = reinterpret_cast<T*>(tail_lds);
   const int tid = threadIdx.x;
@@ -2327,13 +2453,18 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
 }
 
 template <typename T, int BS>
+__global__ __launch_bounds__(kTailThreads) void k_tail_vcycle(TailDesc<T> d) {
+  tail_vcycle_body<T, BS>(hw_blk(), d);
+}
+
+template <typename T, int BS>
 hipError_t tail_vcycle(const TailDesc<T>& d, size_t lds_bytes, hipStream_t s) {
   if (d.nlev < 2 || d.nlev > kMaxTail || d.cn > 128 || lds_bytes > kTailLdsBytes ||
       d.nu_pre < 1 || d.nu_pre > kMaxNu || d.nu_post < 0 || d.nu_post > kMaxNu)
     return hipErrorInvalidValue;
   for (int l = 0; l + 1 < d.nlev; ++l)
     if ((d.lev[l].nx + 1) * (d.lev[l].ny + 1) > kTailThreads) return hipErrorInvalidValue;
-  k_tail_vcycle<T, BS><<<1, kTailThreads, lds_bytes, s>>>(d);
+  TMX_LAUNCH_OR_RECORD(kTailThreads, (tail_vcycle_body<T, BS>), 1, lds_bytes, s, (k_tail_vcycle<T, BS>), d);
   return hipGetLastError();
 }
 template hipError_t tail_vcycle<double, 1>(const TailDesc<double>&, size_t, hipStream_t);
@@ -2352,11 +2483,16 @@ template hipError_t tail_vcycle<float, 2>(const TailDesc<float>&, size_t, hipStr
 // clamped into the grid; their weight is zero then).  The nine threads of a node read the
 // same fine entries (cache hits), and the per-thread work is a ninth of a node's.
 template <int BS>
-__global__ __launch_bounds__(64) void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
-                              const double* __restrict__ stf, double* __restrict__ stc) {
+__device__ __forceinline__ void stencil_rap_body(const BlkId blk, int nxf,
+                                                 int nyf,
+                                                 int nxc,
+                                                 int nyc,
+                                                 unsigned fixed,
+                                                 const double* stf,
+                                                 double* stc) {
   const unsigned nnf = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
   const unsigned nnc = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = blk.id * blockDim.x + threadIdx.x;
   const unsigned i = t / 9u;
   const int kc = (int)(t - 9u * i);
   if (i >= nnc) return;
@@ -2408,12 +2544,17 @@ __global__ __launch_bounds__(64) void k_stencil_rap(int nxf, int nyf, int nxc, i
 }
 
 template <int BS>
+__global__ __launch_bounds__(64) void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
+                              const double* __restrict__ stf, double* __restrict__ stc) {
+  stencil_rap_body<BS>(hw_blk(), nxf, nyf, nxc, nyc, fixed, stf, stc);
+}
+
+template <int BS>
 hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
                        const double* stf, double* stc, hipStream_t s) {
   if (nested) return hipErrorInvalidValue;  // 9-point levels are always bilinear coarsenings
   if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
-  k_stencil_rap<BS><<<grid1d(9 * (size_t)(nxc + 1) * (nyc + 1), 64), 64, 0, s>>>(
-      nxf, nyf, nxc, nyc, fixed, stf, stc);
+  TMX_LAUNCH_OR_RECORD(64, (stencil_rap_body<BS>), grid1d(9 * (size_t)(nxc + 1) * (nyc + 1), 64), 0, s, (k_stencil_rap<BS>), nxf, nyf, nxc, nyc, fixed, stf, stc);
   return hipGetLastError();
 }
 template hipError_t stencil_rap<1>(int, int, int, int, int, unsigned, const double*, double*,
@@ -2422,11 +2563,13 @@ template hipError_t stencil_rap<2>(int, int, int, int, int, unsigned, const doub
                                    hipStream_t);
 
 template <int BS>
-__global__ void k_stencil_to_dense(int nx, int ny, const double* __restrict__ st,
-                                   double* __restrict__ dense) {
+__device__ __forceinline__ void stencil_to_dense_body(const BlkId blk, int nx,
+                                                      int ny,
+                                                      const double* st,
+                                                      double* dense) {
   const size_t nn = (size_t)(nx + 1) * (ny + 1);
   const size_t n = BS * nn;
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t v = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
   for (int k = 0; k < 9; ++k) {
@@ -2440,18 +2583,26 @@ __global__ void k_stencil_to_dense(int nx, int ny, const double* __restrict__ st
 }
 
 template <int BS>
+__global__ void k_stencil_to_dense(int nx, int ny, const double* __restrict__ st,
+                                   double* __restrict__ dense) {
+  stencil_to_dense_body<BS>(hw_blk(), nx, ny, st, dense);
+}
+
+template <int BS>
 hipError_t stencil_to_dense(int nx, int ny, const double* st, double* dense, hipStream_t s) {
   const size_t nn = (size_t)(nx + 1) * (ny + 1);
-  hipError_t e = hipMemsetAsync(dense, 0, sizeof(double) * BS * nn * BS * nn, s);
+  // (a fill kernel, not a memset: the merged batch step records kernels only)
+  const hipError_t e = fill_zero_bytes(dense, sizeof(double) * BS * nn * BS * nn, s);
   if (e != hipSuccess) return e;
-  k_stencil_to_dense<BS><<<grid1d(nn), kBlock, 0, s>>>(nx, ny, st, dense);
+  TMX_LAUNCH_OR_RECORD(kBlock, (stencil_to_dense_body<BS>), grid1d(nn), 0, s, (k_stencil_to_dense<BS>), nx, ny, st, dense);
   return hipGetLastError();
 }
 template hipError_t stencil_to_dense<1>(int, int, const double*, double*, hipStream_t);
 template hipError_t stencil_to_dense<2>(int, int, const double*, double*, hipStream_t);
 
 // In-place Gauss-Jordan inversion of an SPD matrix (no pivoting), one workgroup.
-__global__ __launch_bounds__(1024) void k_dense_invert(int n, double* __restrict__ a) {
+__device__ __forceinline__ void dense_invert_body(const BlkId blk, int n,
+                                                  double* a) {
   extern __shared__ double col[];
   for (int k = 0; k < n; ++k) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = a[(size_t)i * n + k];
@@ -2472,8 +2623,13 @@ __global__ __launch_bounds__(1024) void k_dense_invert(int n, double* __restrict
   }
 }
 
+__global__ __launch_bounds__(1024) void k_dense_invert(int n, double* __restrict__ a) {
+  dense_invert_body(hw_blk(), n, a);
+}
+
 // The same elimination with the whole matrix in LDS (n^2 + n doubles <= 64 KB, n <= 90).
-__global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __restrict__ a) {
+__device__ __forceinline__ void dense_invert_lds_body(const BlkId blk, int n,
+                                                      double* a) {
   extern __shared__ double lds_m[];
   double* m = lds_m;
   double* col = lds_m + n * n;
@@ -2496,18 +2652,24 @@ __global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __rest
   for (int e = threadIdx.x; e < nn; e += blockDim.x) a[e] = m[e];
 }
 
+__global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __restrict__ a) {
+  dense_invert_lds_body(hw_blk(), n, a);
+}
+
 hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
   const size_t lds = sizeof(double) * ((size_t)n * n + n);
-  if (lds <= 64 * 1024) k_dense_invert_lds<<<1, 1024, lds, s>>>(n, a);
-  else k_dense_invert<<<1, 1024, sizeof(double) * n, s>>>(n, a);
+  if (lds <= 64 * 1024) TMX_LAUNCH_OR_RECORD(1024, (dense_invert_lds_body), 1, lds, s, (k_dense_invert_lds), n, a);
+  else TMX_LAUNCH_OR_RECORD(1024, (dense_invert_body), 1, sizeof(double) * n, s, (k_dense_invert), n, a);
   return hipGetLastError();
 }
 
 template <typename T>
-__global__ void k_dense_apply(int n, const double* __restrict__ ainv, const T* __restrict__ b,
-                              T* __restrict__ x) {
+__device__ __forceinline__ void dense_apply_body(const BlkId blk, int n,
+                                                 const double* ainv,
+                                                 const T* b,
+                                                 T* x) {
   // 8 lanes per row (rows of the symmetric inverse read contiguously)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blk.id * blockDim.x + threadIdx.x;
   const int row = t >> 3, part = t & 7;
   double s = 0.0;
   if (row < n)
@@ -2519,21 +2681,34 @@ __global__ void k_dense_apply(int n, const double* __restrict__ ainv, const T* _
 }
 
 template <typename T>
+__global__ void k_dense_apply(int n, const double* __restrict__ ainv, const T* __restrict__ b,
+                              T* __restrict__ x) {
+  dense_apply_body<T>(hw_blk(), n, ainv, b, x);
+}
+
+template <typename T>
 hipError_t dense_apply(int n, const double* ainv, const T* b, T* x, hipStream_t s) {
-  k_dense_apply<T><<<(8 * n + 255) / 256, 256, 0, s>>>(n, ainv, b, x);
+  TMX_LAUNCH_OR_RECORD(256, (dense_apply_body<T>), (8 * n + 255) / 256, 0, s, (k_dense_apply<T>), n, ainv, b, x);
   return hipGetLastError();
 }
 template hipError_t dense_apply<double>(int, const double*, const double*, double*, hipStream_t);
 template hipError_t dense_apply<float>(int, const double*, const float*, float*, hipStream_t);
 
 template <typename TI, typename TO>
-__global__ void k_convert(size_t n, const TI* __restrict__ in, TO* __restrict__ out) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void convert_body(const BlkId blk, size_t n, const TI* in_, TO* out_) {
+  const auto in = glob(in_);
+  const auto out = glob(out_);
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x)
     out[i] = (TO)in[i];
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(size_t n, const TI* __restrict__ in, TO* __restrict__ out) {
+  convert_body<TI, TO>(hw_blk(), n, in, out);
 }
 template <typename TI, typename TO>
 hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s) {
-  k_convert<TI, TO><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, in, out);
+  TMX_LAUNCH_OR_RECORD(kBlock, (convert_body<TI, TO>), std::min<unsigned>(grid1d(n), 8192), 0, s, (k_convert<TI, TO>), n, in, out);
   return hipGetLastError();
 }
 template hipError_t convert<double, float>(size_t, const double*, float*, hipStream_t);
@@ -2547,21 +2722,29 @@ static inline unsigned vec_grid(size_t n) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(size_t n, const T* __restrict__ a,
-                                                const T* __restrict__ b, double* __restrict__ part) {
+__device__ __forceinline__ void dot_body(const BlkId blk, size_t n, const T* a_, const T* b_,
+                                         double* part) {
+  const auto a = glob(a_);
+  const auto b = glob(b_);
   double s = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x)
     s += (double)a[i] * (double)b[i];
   const double v[1] = {s};
   double* const o[1] = {part};
-  block_partials<1>(v, o);
+  block_partials<1>(v, o, blk.id);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(size_t n, const T* __restrict__ a,
+                                                const T* __restrict__ b, double* __restrict__ part) {
+  dot_body<T>(hw_blk(), n, a, b, part);
 }
 
 template <typename T>
 hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s) {
   const unsigned g = vec_grid(n);
   r.nblocks = (int)g;
-  k_dot<T><<<g, kBlock, 0, s>>>(n, a, b, r.partials);
+  TMX_LAUNCH_OR_RECORD(kBlock, (dot_body<T>), g, 0, s, (k_dot<T>), n, a, b, r.partials);
   return hipGetLastError();
 }
 template hipError_t dot_t<double>(size_t, const double*, const double*, RedSlot&, hipStream_t);
@@ -2623,31 +2806,45 @@ hipError_t ru_replace(size_t n, const double* r64, float* r, RedSlot& rr, hipStr
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__ part, int n,
-                                                      double* __restrict__ out) {
+__device__ __forceinline__ void reduce_to_body(const BlkId blk, const double* part,
+                                               int n,
+                                               double* out) {
   const double t = sum_partials(part, n);
   if (threadIdx.x == 0) *out = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__ part, int n,
+                                                      double* __restrict__ out) {
+  reduce_to_body(hw_blk(), part, n, out);
+}
+
+template <typename T>
+__device__ __forceinline__ void axpy_dev_body(const BlkId blk, size_t n, const double* a,
+                                              const T* x_, T* y_) {
+  const auto x = glob(x_);
+  const auto y = glob(y_);
+  const T av = (T)*glob(a);
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)blk.n * blockDim.x)
+    y[i] += av * x[i];
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy_dev(size_t n, const double* __restrict__ a,
                                                      const T* __restrict__ x, T* __restrict__ y) {
-  const T av = (T)*a;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    y[i] += av * x[i];
+  axpy_dev_body<T>(hw_blk(), n, a, x, y);
 }
 
 template <typename T>
 hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s) {
-  k_axpy_dev<T><<<vec_grid(n), kBlock, 0, s>>>(n, a, x, y);
+  TMX_LAUNCH_OR_RECORD(kBlock, (axpy_dev_body<T>), vec_grid(n), 0, s, (k_axpy_dev<T>), n, a, x, y);
   return hipGetLastError();
 }
 template hipError_t axpy_dev<double>(size_t, const double*, const double*, double*, hipStream_t);
 template hipError_t axpy_dev<float>(size_t, const double*, const float*, float*, hipStream_t);
 
 hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s) {
-  k_reduce_to<<<1, kBlock, 0, s>>>(r.partials, r.nblocks, out);
+  TMX_LAUNCH_OR_RECORD(kBlock, (reduce_to_body), 1, 0, s, (k_reduce_to), r.partials, r.nblocks, out);
   return hipGetLastError();
 }
 
@@ -2739,10 +2936,15 @@ template hipError_t pcg_update_xr<float>(size_t, const double*, int, float*, flo
                                          const float*, const float*, RedSlot&, hipStream_t);
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
-    size_t n, double* __restrict__ sc, PcgSlots q, const double* __restrict__ pqpart, int npq,
-    T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ qv,
-    double* __restrict__ part, float* __restrict__ rlow) {
+__device__ __forceinline__ void pcg_update_xr_fused_body(const BlkId blk, size_t n, double* sc,
+                                                         PcgSlots q, const double* pqpart, int npq,
+                                                         T* x_, T* r_, const T* p_, const T* qv_,
+                                                         double* part, float* rlow_) {
+  const auto x = glob(x_);
+  const auto r = glob(r_);
+  const auto p = glob(p_);
+  const auto qv = glob(qv_);
+  const auto rlow = glob(rlow_);
   __shared__ double alpha_sh;
   const double pq = sum_partials(pqpart, npq);
   if (threadIdx.x == 0) {
@@ -2751,9 +2953,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
   }
   __syncthreads();
   const T al = (T)alpha_sh;
-  if (!x && blockIdx.x == 0 && threadIdx.x == 0) sc[q.alpha] = alpha_sh;  // for the deferred x
+  if (!x && blk.id == 0 && threadIdx.x == 0) sc[q.alpha] = alpha_sh;  // for the deferred x
   double s = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x) {
     if (x) x[i] += al * p[i];
     const T ri = r[i] - al * qv[i];
     r[i] = ri;
@@ -2762,7 +2964,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
   }
   const double v[1] = {s};
   double* const o[1] = {part};
-  block_partials<1>(v, o);
+  block_partials<1>(v, o, blk.id);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_xr_fused(
+    size_t n, double* __restrict__ sc, PcgSlots q, const double* __restrict__ pqpart, int npq,
+    T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ qv,
+    double* __restrict__ part, float* __restrict__ rlow) {
+  pcg_update_xr_fused_body<T>(hw_blk(), n, sc, q, pqpart, npq, x, r, p, qv, part, rlow);
 }
 
 template <typename T>
@@ -2771,8 +2981,7 @@ hipError_t pcg_update_xr_fused(size_t n, double* sc, const PcgSlots& q, const Re
                                float* rlow) {
   const unsigned g = vec_grid(n);
   rr.nblocks = (int)g;
-  k_pcg_update_xr_fused<T><<<g, kBlock, 0, s>>>(n, sc, q, pq.partials, pq.nblocks, x, r, p, qv,
-                                               rr.partials, rlow);
+  TMX_LAUNCH_OR_RECORD(kBlock, (pcg_update_xr_fused_body<T>), g, 0, s, (k_pcg_update_xr_fused<T>), n, sc, q, pq.partials, pq.nblocks, x, r, p, qv, rr.partials, rlow);
   return hipGetLastError();
 }
 template hipError_t pcg_update_xr_fused<double>(size_t, double*, const PcgSlots&,
@@ -2786,11 +2995,15 @@ template hipError_t pcg_update_xr_fused<float>(size_t, double*, const PcgSlots&,
 // thread keeps its loads in flight) and one barrier, so the kernel costs about one memory
 // round trip plus the pinned-memory report.
 constexpr int kFinThreads = 1024;
-__global__ __launch_bounds__(kFinThreads) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
-                                                              const double* __restrict__ rrp, int nrr,
-                                                              const double* __restrict__ rzp, int nrz,
-                                                              double* __restrict__ sc, PcgSlots q,
-                                                              double* __restrict__ hout) {
+__device__ __forceinline__ void pcg_finalize_body(const BlkId blk, const double* pqp,
+                                                  int npq,
+                                                  const double* rrp,
+                                                  int nrr,
+                                                  const double* rzp,
+                                                  int nrz,
+                                                  double* sc,
+                                                  PcgSlots q,
+                                                  double* hout) {
   __shared__ double sh[3][kFinThreads / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
@@ -2843,15 +3056,24 @@ __global__ __launch_bounds__(kFinThreads) void k_pcg_finalize(const double* __re
   }
 }
 
+__global__ __launch_bounds__(kFinThreads) void k_pcg_finalize(const double* __restrict__ pqp, int npq,
+                                                              const double* __restrict__ rrp, int nrr,
+                                                              const double* __restrict__ rzp, int nrz,
+                                                              double* __restrict__ sc, PcgSlots q,
+                                                              double* __restrict__ hout) {
+  pcg_finalize_body(hw_blk(), pqp, npq, rrp, nrr, rzp, nrz, sc, q, hout);
+}
+
 hipError_t pcg_finalize(const RedSlot& pq, const RedSlot& rr, const RedSlot& rz, double* sc,
                         const PcgSlots& q, hipStream_t s, double* host_out) {
-  k_pcg_finalize<<<1, kFinThreads, 0, s>>>(pq.partials, pq.nblocks, rr.partials, rr.nblocks,
-                                           rz.partials, rz.nblocks, sc, q, host_out);
+  TMX_LAUNCH_OR_RECORD(kFinThreads, (pcg_finalize_body), 1, 0, s, (k_pcg_finalize), pq.partials, pq.nblocks, rr.partials, rr.nblocks, rz.partials, rz.nblocks, sc, q, host_out);
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__ rzp, int nrz,
-                                                      double* __restrict__ sc, PcgSlots q) {
+__device__ __forceinline__ void pcg_start_body(const BlkId blk, const double* rzp,
+                                               int nrz,
+                                               double* sc,
+                                               PcgSlots q) {
   const double rz = sum_partials(rzp, nrz);
   if (threadIdx.x == 0) {
     sc[q.rz] = rz;
@@ -2864,14 +3086,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__ rzp, int nrz,
+                                                      double* __restrict__ sc, PcgSlots q) {
+  pcg_start_body(hw_blk(), rzp, nrz, sc, q);
+}
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_begin(size_t n, const T* __restrict__ b,
-                                                      T* __restrict__ x, T* __restrict__ r,
-                                                      float* __restrict__ rlow,
-                                                      double* __restrict__ part) {
+__device__ __forceinline__ void pcg_begin_body(const BlkId blk, size_t n, const T* b_, T* x_,
+                                               T* r_, float* rlow_, double* part) {
+  const auto b = glob(b_);
+  const auto x = glob(x_);
+  const auto r = glob(r_);
+  const auto rlow = glob(rlow_);
   double s = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)blk.n * blockDim.x) {
     const T v = b[i];
     r[i] = v;
     x[i] = T(0);
@@ -2880,14 +3109,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_begin(size_t n, const T* __restr
   }
   const double v1[1] = {s};
   double* const o[1] = {part};
-  block_partials<1>(v1, o);
+  block_partials<1>(v1, o, blk.id);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_begin(size_t n, const T* __restrict__ b,
+                                                      T* __restrict__ x, T* __restrict__ r,
+                                                      float* __restrict__ rlow,
+                                                      double* __restrict__ part) {
+  pcg_begin_body<T>(hw_blk(), n, b, x, r, rlow, part);
 }
 
 template <typename T>
 hipError_t pcg_begin(size_t n, const T* b, T* x, T* r, float* rlow, RedSlot& bb, hipStream_t s) {
   const unsigned g = vec_grid(n);
   bb.nblocks = (int)g;
-  k_pcg_begin<T><<<g, kBlock, 0, s>>>(n, b, x, r, rlow, bb.partials);
+  TMX_LAUNCH_OR_RECORD(kBlock, (pcg_begin_body<T>), g, 0, s, (k_pcg_begin<T>), n, b, x, r, rlow, bb.partials);
   return hipGetLastError();
 }
 template hipError_t pcg_begin<double>(size_t, const double*, double*, double*, float*, RedSlot&,
@@ -2895,10 +3132,13 @@ template hipError_t pcg_begin<double>(size_t, const double*, double*, double*, f
 template hipError_t pcg_begin<float>(size_t, const float*, float*, float*, float*, RedSlot&,
                                      hipStream_t);
 
-__global__ __launch_bounds__(kBlock) void k_pcg_begin_final(const double* __restrict__ part, int n,
-                                                            double* __restrict__ sc, PcgSlots q,
-                                                            double rtol2, int bb_slot,
-                                                            double* __restrict__ host_bb) {
+__device__ __forceinline__ void pcg_begin_final_body(const BlkId blk, const double* part,
+                                                     int n,
+                                                     double* sc,
+                                                     PcgSlots q,
+                                                     double rtol2,
+                                                     int bb_slot,
+                                                     double* host_bb) {
   const double bb = sum_partials(part, n);
   if (threadIdx.x == 0) {
     sc[bb_slot] = bb;
@@ -2912,14 +3152,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_begin_final(const double* __rest
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_pcg_begin_final(const double* __restrict__ part, int n,
+                                                            double* __restrict__ sc, PcgSlots q,
+                                                            double rtol2, int bb_slot,
+                                                            double* __restrict__ host_bb) {
+  pcg_begin_final_body(hw_blk(), part, n, sc, q, rtol2, bb_slot, host_bb);
+}
+
 hipError_t pcg_begin_final(const RedSlot& bb, double* sc, const PcgSlots& q, double rtol2,
                            int bb_slot, double* host_bb, hipStream_t s) {
-  k_pcg_begin_final<<<1, kBlock, 0, s>>>(bb.partials, bb.nblocks, sc, q, rtol2, bb_slot, host_bb);
+  TMX_LAUNCH_OR_RECORD(kBlock, (pcg_begin_final_body), 1, 0, s, (k_pcg_begin_final), bb.partials, bb.nblocks, sc, q, rtol2, bb_slot, host_bb);
   return hipGetLastError();
 }
 
 hipError_t pcg_start(const RedSlot& rz, double* sc, const PcgSlots& q, hipStream_t s) {
-  k_pcg_start<<<1, kBlock, 0, s>>>(rz.partials, rz.nblocks, sc, q);
+  TMX_LAUNCH_OR_RECORD(kBlock, (pcg_start_body), 1, 0, s, (k_pcg_start), rz.partials, rz.nblocks, sc, q);
   return hipGetLastError();
 }
 
@@ -2939,14 +3186,55 @@ hipError_t pcg_update_p(size_t n, const double* sc, int i_beta, const T* z, T* p
 template hipError_t pcg_update_p<double>(size_t, const double*, int, const double*, double*, hipStream_t);
 template hipError_t pcg_update_p<float>(size_t, const double*, int, const float*, float*, hipStream_t);
 
+// count independent copies in one launch: blockIdx.y picks the task
+__global__ __launch_bounds__(kBlock) void k_copy_tasks(const CopyTask* __restrict__ tasks) {
+  const CopyTask t = tasks[blockIdx.y];
+  const double* __restrict__ src = t.src;
+  double* __restrict__ dst = t.dst;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < t.n; i += (size_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
+hipError_t copy_tasks(const CopyTask* tasks, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > 65535) return hipErrorInvalidValue;
+  k_copy_tasks<<<dim3(64, (unsigned)count), kBlock, 0, s>>>(tasks);
+  return hipGetLastError();
+}
+
+// one device scalar to pinned host memory (the host reads it after the stream has drained)
+__device__ __forceinline__ void publish_body(const BlkId, const double* src, double* host_dst) {
+  if (threadIdx.x == 0) {
+    host_dst[0] = src[0];
+    __threadfence_system();
+  }
+}
+
+__global__ __launch_bounds__(64) void k_publish(const double* __restrict__ src,
+                                                double* __restrict__ host_dst) {
+  publish_body(hw_blk(), src, host_dst);
+}
+
+hipError_t publish_scalar(const double* src, double* host_dst, hipStream_t s) {
+  TMX_LAUNCH_OR_RECORD(64, (publish_body), 1, 0, s, (k_publish), src, host_dst);
+  return hipGetLastError();
+}
+
+template <typename T>
+__device__ __forceinline__ void copy_body(const BlkId blk, size_t n, const T* a_, T* b_) {
+  const auto a = glob(a_);
+  const auto b = glob(b_);
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x)
+    b[i] = a[i];
+}
+
 template <typename T>
 __global__ void k_copy(size_t n, const T* __restrict__ a, T* __restrict__ b) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+  copy_body<T>(hw_blk(), n, a, b);
 }
 template <typename T>
 hipError_t copy_vec(size_t n, const T* a, T* b, hipStream_t s) {
-  k_copy<T><<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, a, b);
+  TMX_LAUNCH_OR_RECORD(kBlock, (copy_body<T>), std::min<unsigned>(grid1d(n), 8192), 0, s, (k_copy<T>), n, a, b);
   return hipGetLastError();
 }
 template hipError_t copy_vec<double>(size_t, const double*, double*, hipStream_t);
@@ -2982,13 +3270,18 @@ hipError_t scale_to(size_t n, const double* sc, int i, const double* in, T* out,
 template hipError_t scale_to<float>(size_t, const double*, int, const double*, float*, hipStream_t);
 template hipError_t scale_to<double>(size_t, const double*, int, const double*, double*, hipStream_t);
 
-__global__ void k_fill_zero(size_t n, unsigned* __restrict__ p) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void fill_zero_body(const BlkId blk, size_t n,
+                                               unsigned* p) {
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x)
     p[i] = 0u;
+}
+
+__global__ void k_fill_zero(size_t n, unsigned* __restrict__ p) {
+  fill_zero_body(hw_blk(), n, p);
 }
 hipError_t fill_zero_bytes(void* p, size_t bytes, hipStream_t s) {
   const size_t n = bytes / 4;
-  k_fill_zero<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, static_cast<unsigned*>(p));
+  TMX_LAUNCH_OR_RECORD(kBlock, (fill_zero_body), std::min<unsigned>(grid1d(n), 8192), 0, s, (k_fill_zero), n, static_cast<unsigned*>(p));
   return hipGetLastError();
 }
 
@@ -3015,12 +3308,18 @@ hipError_t set_scalar(double* sc, int i, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
-__global__ void k_fill(size_t n, double val, double* __restrict__ p) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void fill_body(const BlkId blk, size_t n,
+                                          double val,
+                                          double* p) {
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x)
     p[i] = val;
 }
+
+__global__ void k_fill(size_t n, double val, double* __restrict__ p) {
+  fill_body(hw_blk(), n, val, p);
+}
 hipError_t fill(size_t n, double val, double* p, hipStream_t s) {
-  k_fill<<<std::min<unsigned>(grid1d(n), 8192), kBlock, 0, s>>>(n, val, p);
+  TMX_LAUNCH_OR_RECORD(kBlock, (fill_body), std::min<unsigned>(grid1d(n), 8192), 0, s, (k_fill), n, val, p);
   return hipGetLastError();
 }
 

@@ -219,6 +219,18 @@ class Hierarchy {
           double* rel, hipStream_t s);
   template <typename T>
   T* x_buf();
+  // The launches of pcg() without its host loop, for callers that drive the iterations
+  // themselves (merged batches): the start (r = b, x = 0, |b|^2 and the tolerance into the
+  // device scalars and the pinned ring, then, with init, z = M r and the first direction's
+  // scalars), one iteration of the given parity (the ring slot 4 * parity receives its report),
+  // and the end (the last iteration's deferred x += alpha p, given the number of iterations
+  // launched).  pcg() itself is built from them.
+  template <typename T, typename PC = T>
+  int pcg_enqueue_begin(Workspace& ws, const T* b, double rtol, bool init, hipStream_t s);
+  template <typename T, typename PC = T>
+  int pcg_enqueue_iteration(Workspace& ws, int parity, hipStream_t s);
+  template <typename T>
+  int pcg_enqueue_end(Workspace& ws, int launched, hipStream_t s);
   // Reliable-update PCG on A x = b from x0 = 0 (van der Vorst and Ye 2000, residual
   // replacement): Krylov vectors, operator and V-cycle in FP32, the iterate accumulated in
   // FP64 (x64_), and the recursive FP32 residual replaced by the true FP64 one b - A x64

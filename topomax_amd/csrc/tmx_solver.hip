@@ -929,6 +929,31 @@ static int capture(Hierarchy& h, Workspace& ws, int which, hipGraphExec_t* out, 
 }
 
 template <typename T, typename PC>
+int Hierarchy::pcg_enqueue_begin(Workspace& ws, const T* b, double rtol, bool init,
+                                 hipStream_t s) {
+  RedSlot bbs = ws.region(3);
+  constexpr bool split = !std::is_same<T, PC>::value;
+  TMX_HIP(pcg_begin<T>(n(), b, KrylovBufs<T>::x(*this), KrylovBufs<T>::r(*this),
+                       split ? r32_.get() : nullptr, bbs, s));
+  TMX_HIP(pcg_begin_final(bbs, ws.sc(), kPcg, rtol * rtol, kScBB,
+                          ws.host_dev + Workspace::kScalars + kRingBB, s));
+  if (init) TMX_TRY((enqueue_pcg_init<T, PC>(*this, ws, s)));
+  return 0;
+}
+
+template <typename T, typename PC>
+int Hierarchy::pcg_enqueue_iteration(Workspace& ws, int parity, hipStream_t s) {
+  return enqueue_pcg_iteration<T, PC>(*this, ws, parity, s);
+}
+
+template <typename T>
+int Hierarchy::pcg_enqueue_end(Workspace& ws, int launched, hipStream_t s) {
+  T* p_last = ((launched - 1) & 1) ? KrylovBufs<T>::p(*this) : KrylovBufs<T>::pb(*this);
+  TMX_HIP(axpy_dev<T>(n(), ws.sc() + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s));
+  return 0;
+}
+
+template <typename T, typename PC>
 int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double* bnorm2,
                    int* iters, double* rel, hipStream_t s) {
   const size_t nv = n();
@@ -938,21 +963,14 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   if (use_graphs_)
     for (int w = 0; w < 3; ++w)
       if (!graphs_[prec][w]) TMX_TRY((capture<T, PC>(*this, ws, w, &graphs_[prec][w], s)));
-  double* sc = ws.sc();
   *iters = 0;
   *rel = 1.0;
   // r = b, x = 0 (and the float r of a mixed V-cycle) with |b|^2, the tolerance and the start
   // flag formed on the device: no host round trip before the first iteration
   volatile double* host_bb = ws.host + Workspace::kScalars + kRingBB;
   host_bb[0] = std::numeric_limits<double>::quiet_NaN();
-  RedSlot bbs = ws.region(3);
-  constexpr bool split = !std::is_same<T, PC>::value;
-  TMX_HIP(pcg_begin<T>(nv, b, KrylovBufs<T>::x(*this), KrylovBufs<T>::r(*this),
-                       split ? r32_.get() : nullptr, bbs, s));
-  TMX_HIP(pcg_begin_final(bbs, sc, kPcg, rtol * rtol, kScBB,
-                          ws.host_dev + Workspace::kScalars + kRingBB, s));
+  TMX_TRY((pcg_enqueue_begin<T, PC>(ws, b, rtol, !use_graphs_, s)));
   if (use_graphs_) TMX_HIP(hipGraphLaunch(graphs_[prec][0], s));
-  else TMX_TRY((enqueue_pcg_init<T, PC>(*this, ws, s)));
   // Iteration k runs graph 1 + (k-1)%2, whose last kernel writes (flag, r.r, iterations) and
   // then the sequence number k to ring slot (k-1)%2 in pinned host memory; the host polls that
   // number (no event packets between the graphs).  While the host waits for iteration k it
@@ -1021,10 +1039,7 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   }
   // the last iteration's deferred x += alpha p (alpha is 0 once stopped), queued before the
   // host waits for the last queued iteration's report, so the device does not idle meanwhile
-  if (launched > 0) {
-    T* p_last = ((launched - 1) & 1) ? KrylovBufs<T>::p(*this) : KrylovBufs<T>::pb(*this);
-    TMX_HIP(axpy_dev<T>(nv, sc + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s));
-  }
+  if (launched > 0) TMX_TRY(pcg_enqueue_end<T>(ws, launched, s));
   if (launched > done) TMX_TRY(wait_iter(launched));
   if (launched == 0) {  // no iteration reported |b|^2: read it
     TMX_TRY(ws.read(s, kScBB, 1));
@@ -1526,6 +1541,13 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   return rc;
 }
 
+template int Hierarchy::pcg_enqueue_begin<double, double>(Workspace&, const double*, double, bool,
+                                                          hipStream_t);
+template int Hierarchy::pcg_enqueue_begin<double, float>(Workspace&, const double*, double, bool,
+                                                         hipStream_t);
+template int Hierarchy::pcg_enqueue_iteration<double, double>(Workspace&, int, hipStream_t);
+template int Hierarchy::pcg_enqueue_iteration<double, float>(Workspace&, int, hipStream_t);
+template int Hierarchy::pcg_enqueue_end<double>(Workspace&, int, hipStream_t);
 template int Hierarchy::apply<double>(const double*, double*, RedSlot*, hipStream_t);
 template int Hierarchy::apply_update<double>(const double*, const double*, double*, double*,
                                              const double*, RedSlot*, hipStream_t, const float*,

@@ -499,22 +499,33 @@ class ProblemBatch:
     """Objective + gradient of many independent designs in one call (``tmx_batch_*``).
 
     The problems may differ in size, loads, supports and penalty; each keeps its own
-    hierarchy.  The library runs them concurrently, one host worker per design at a time
-    (``workers``, default min(16, cores), at least 4), largest first, so the latency-bound
-    coarse-grid work of different designs overlaps on the GPU.  Results are identical to
-    calling each problem's :meth:`ElasticityProblem.objective_and_gradient` alone."""
+    hierarchy.  ``mode="merged"`` (the default): the designs that qualify (an objective step in
+    the same precision has run on them before, P1 element, FP64 or mixed) run as one merged
+    step, one kernel launch per PCG phase and V-cycle stage covering all of them; the others
+    run as single-design calls on ``workers`` host threads (default min(16, cores), at least
+    4) meanwhile.  ``mode="alone"``: every design runs as a single-design call on the worker
+    threads, largest first.  Either way the results are identical, bit for bit, to calling
+    each problem's :meth:`ElasticityProblem.objective_and_gradient` alone;
+    :attr:`last_report` tells what the last call did."""
 
-    def __init__(self, problems, workers: int | None = None):
+    MODES = {"merged": _lib.TMX_BATCH_MERGED, "alone": _lib.TMX_BATCH_ALONE}
+
+    def __init__(self, problems, workers: int | None = None, mode: str = "merged"):
         self.problems = list(problems)
         if not self.problems:
             raise ValueError("a batch needs at least one problem")
+        if mode not in self.MODES:
+            raise ValueError(f"mode must be one of {sorted(self.MODES)}")
         self.lib = _lib.load()
         ptrs = (ctypes.c_void_p * len(self.problems))(*[p._h.ptr.value for p in self.problems])
         self.ptr = ctypes.c_void_p()
         _lib.check(self.lib.tmx_batch_create(ptrs, len(self.problems), ctypes.byref(self.ptr)))
         if workers is not None:
             _lib.check(self.lib.tmx_batch_set_workers(self.ptr, int(workers)))
+        _lib.check(self.lib.tmx_batch_set_mode(self.ptr, self.MODES[mode]))
+        self.mode = mode
         self.last_stats = []
+        self.last_report = {}
 
     def objective_and_gradient(self, rhos, *, rtol=None, precision=None):
         """Returns (J list, w list); J is NaN for a design with non-finite entries."""
@@ -541,6 +552,9 @@ class ProblemBatch:
         _lib.check(self.lib.tmx_batch_objective_and_gradient(self.ptr, rp, J, wp,
                                                              ctypes.byref(opts), stats))
         self.last_stats = [s.as_dict() for s in stats]
+        rep = _lib.BatchReport()
+        _lib.check(self.lib.tmx_batch_last_report(self.ptr, ctypes.byref(rep)))
+        self.last_report = rep.as_dict()
         return [float(j) for j in J], ws
 
     def __del__(self):
