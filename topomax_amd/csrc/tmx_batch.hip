@@ -64,16 +64,23 @@ struct Merged {
   unsigned grid;
   size_t lds;
   size_t rec, first, owner;
+  size_t skip;  // offset of the designs' skip-flag pointers, or 0: no skipping
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
 
 // Merges the launch sequences of several designs (one recorder each, same phase) into one
-// launch per kernel instance at each step: at every step the kernel at the head of the most
-// sequences is taken (ties: the one whose sequences have the most launches left), and the
-// heads of all sequences showing it are merged.  Each sequence keeps its order, and the
-// designs are independent, so any such interleaving is a valid schedule.
-void merge_phase(const std::vector<const tmx::Recorder*>& recs, std::vector<Merged>& out,
+// launch per kernel instance at each step: at every step the kernel at the head of the longest
+// remaining sequence is taken (ties: the one at the head of the most sequences), and the heads
+// of all sequences showing it are merged.  Each sequence keeps its order, and the designs are
+// independent, so any such interleaving is a valid schedule.  Taking the critical path first
+// lets the shorter hierarchies' V-cycles meet the deeper ones at the common levels: on the
+// 64-design workload it needs 21 launches per elasticity iteration (18 for the deepest
+// design alone) where taking the most common head needed 45.
+// skip: null, or each design's skip flag (iteration phases: a design whose solve has stopped
+// and applied its last update sits the remaining iterations out).
+void merge_phase(const std::vector<const tmx::Recorder*>& recs,
+                 const std::vector<const double*>* skip, std::vector<Merged>& out,
                  std::vector<unsigned char>& blob) {
   const size_t D = recs.size();
   std::vector<size_t> pos(D, 0);
@@ -101,12 +108,12 @@ void merge_phase(const std::vector<const tmx::Recorder*>& recs, std::vector<Merg
     if (fns.empty()) break;
     size_t best = 0;
     for (size_t k = 1; k < fns.size(); ++k)
-      if (count[k] > count[best] || (count[k] == count[best] && left[k] > left[best])) best = k;
+      if (left[k] > left[best] || (left[k] == left[best] && count[k] > count[best])) best = k;
     const tmx::MultiFn f = fns[best];
     take.clear();
     for (size_t d = 0; d < D; ++d)
       if (pos[d] < recs[d]->recs.size() && recs[d]->recs[pos[d]].fn == f) take.push_back(d);
-    Merged m{f, 0, 0, 0, 0, 0};
+    Merged m{f, 0, 0, 0, 0, 0, 0};
     const size_t rsize = recs[take[0]]->recs[pos[take[0]]].size;
     m.rec = align_up(blob.size(), 16);
     blob.resize(m.rec + rsize * take.size());
@@ -129,6 +136,12 @@ void merge_phase(const std::vector<const tmx::Recorder*>& recs, std::vector<Merg
     first = reinterpret_cast<int*>(blob.data() + m.first);  // (blob may have moved)
     for (size_t i = 0; i < take.size(); ++i)
       for (int b = first[i]; b < first[i + 1]; ++b) owner[b] = (unsigned short)i;
+    if (skip) {
+      m.skip = align_up(blob.size(), 8);
+      blob.resize(m.skip + sizeof(const double*) * take.size());
+      const double** sp = reinterpret_cast<const double**>(blob.data() + m.skip);
+      for (size_t i = 0; i < take.size(); ++i) sp[i] = (*skip)[take[i]];
+    }
     if (m.grid > 0) out.push_back(m);
     for (size_t d : take) ++pos[d];
   }
@@ -143,6 +156,7 @@ bool tables_valid(const std::vector<Merged>& ms, const std::vector<unsigned char
     const int* first = reinterpret_cast<const int*>(blob.data() + m.first);
     const unsigned short* owner = reinterpret_cast<const unsigned short*>(blob.data() + m.owner);
     const size_t nd = (m.owner - m.first) / sizeof(int) - 1;  // (first holds nd + 1 entries)
+    if (m.skip && m.skip + sizeof(const double*) * nd > blob.size()) return false;
     if (nd < 1 || nd > designs || first[0] != 0 || first[nd] != (int)m.grid) return false;
     for (size_t d = 0; d < nd; ++d) {
       if (first[d + 1] < first[d]) return false;
@@ -159,6 +173,10 @@ struct LoopState {
   double rr_end = 0.0;  // r.r of the last report
   bool near = false;
   int iters = 0;
+  // the last report this design makes: once stopped (flag seen in report s), iteration s + 1
+  // still runs (and reports) if the flag was set in iteration s itself, to apply the deferred
+  // x update; the iterations after it are skipped on the device and make no report
+  int last = 1 << 30;
 };
 
 }  // namespace
@@ -330,8 +348,10 @@ bool record_design(tmx_batch* b, tmx_problem* h, const double* rho, double* w,
 int run_phase(tmx_batch* b, int ph) {
   const unsigned char* base = b->dev_blob.get();
   for (const Merged& m : b->phase[ph]) {
-    const tmx::MultiMap mm{reinterpret_cast<const unsigned short*>(base + m.owner),
-                           reinterpret_cast<const int*>(base + m.first)};
+    const tmx::MultiMap mm{
+        reinterpret_cast<const unsigned short*>(base + m.owner),
+        reinterpret_cast<const int*>(base + m.first),
+        m.skip ? reinterpret_cast<const double* const*>(base + m.skip) : nullptr};
     TMX_HIP(m.fn(base + m.rec, mm, m.grid, m.lds, b->stream));
   }
   b->report.launches += (int)b->phase[ph].size();
@@ -352,6 +372,7 @@ int run_loop(tmx_batch* b, const std::vector<tmx_problem*>& ds, const std::vecto
   };
   auto wait_all = [&](int k) -> int {
     for (size_t d = 0; d < ds.size(); ++d) {
+      if (k > st[d].last) continue;
       volatile double* sl = slot(d, k);
       for (long spins = 1; sl[3] < (double)k; ++spins) {
         if ((spins & 1023) == 0) {
@@ -393,7 +414,12 @@ int run_loop(tmx_batch* b, const std::vector<tmx_problem*>& ds, const std::vecto
       if (s.flag != 0.0) continue;
       volatile double* sl = slot(d, done);
       s.flag = sl[0];
-      if (s.flag != 0.0) continue;
+      if (s.flag != 0.0) {  // stopped: its final state, and its last report
+        s.rr_end = sl[1];
+        s.iters = (int)sl[2];
+        s.last = s.iters == done ? done + 1 : done;
+        continue;
+      }
       all_stopped = false;
       const double rr = sl[1];
       if (s.rr_last > 0.0) s.ratio = rr / s.rr_last;
@@ -412,13 +438,13 @@ int run_loop(tmx_batch* b, const std::vector<tmx_problem*>& ds, const std::vecto
   for (size_t d = 0; d < ds.size(); ++d) {
     LoopState& s = st[d];
     if (s.bb < 0.0) s.bb = ds[d]->ws.host[Workspace::kScalars + kRingBB];
-    if (launched > 0) {
+    if (launched == 0) {
+      s.flag = 0.0;
+    } else if (s.flag == 0.0) {  // (not stopped by the last report read: the final one)
       volatile double* sl = slot(d, launched);
       s.flag = sl[0];
       s.rr_end = sl[1];
       s.iters = (int)sl[2];
-    } else {
-      s.flag = 0.0;
     }
   }
   return launched;
@@ -525,13 +551,53 @@ int run_merged(tmx_batch* b, const std::vector<int>& ids, const double* const* r
         alone.push_back(i);
       }
     }
+    if (const char* path = std::getenv("TMX_BATCH_DUMP")) {
+      // the recorded launch sequences (kernel, blocks) of every design and phase, as JSON
+      if (FILE* f = std::fopen(path, "w")) {
+        std::vector<const char*> names;
+        auto id = [&](const char* nm) {
+          for (size_t k = 0; k < names.size(); ++k)
+            if (names[k] == nm) return k;
+          names.push_back(nm);
+          return names.size() - 1;
+        };
+        std::fprintf(f, "{\"phases\": [");
+        for (int ph = 0; ph < kPhases; ++ph) {
+          std::fprintf(f, "%s[", ph ? ", " : "");
+          bool firstd = true;
+          for (size_t k = 0; k < ids.size(); ++k) {
+            if (std::find(used.begin(), used.end(), ids[k]) == used.end()) continue;
+            std::fprintf(f, "%s[", firstd ? "" : ", ");
+            firstd = false;
+            for (size_t r = 0; r < recs[k][ph].recs.size(); ++r)
+              std::fprintf(f, "%s[%zu, %u]", r ? ", " : "", id(recs[k][ph].recs[r].name),
+                           recs[k][ph].recs[r].nblk);
+            std::fprintf(f, "]");
+          }
+          std::fprintf(f, "]");
+        }
+        std::fprintf(f, "], \"names\": [");
+        for (size_t k = 0; k < names.size(); ++k) {
+          std::string nm = names[k];
+          for (auto& ch : nm)
+            if (ch == '"' || ch == '\\') ch = '\'';
+          std::fprintf(f, "%s\"%s\"", k ? ", " : "", nm.c_str());
+        }
+        std::fprintf(f, "]}\n");
+        std::fclose(f);
+      }
+    }
     b->blob.clear();
+    std::vector<const double*> skips;  // each merged design's PCG skip flag
+    for (int i : used) skips.push_back(b->probs[i]->ws.sc() + kPcg.skip);
     for (int ph = 0; ph < kPhases; ++ph) {
       b->phase[ph].clear();
       std::vector<const tmx::Recorder*> rs;
       for (size_t k = 0; k < ids.size(); ++k)
         if (std::find(used.begin(), used.end(), ids[k]) != used.end()) rs.push_back(&recs[k][ph]);
-      merge_phase(rs, b->phase[ph], b->blob);
+      const bool iteration = ph == kF1It0 || ph == kF1It1 || ph == kElIt0 || ph == kElIt1 ||
+                             ph == kF2It0 || ph == kF2It1;
+      merge_phase(rs, iteration ? &skips : nullptr, b->phase[ph], b->blob);
     }
     for (int ph = 0; ph < kPhases; ++ph)
       if (!tables_valid(b->phase[ph], b->blob, used.size())) {

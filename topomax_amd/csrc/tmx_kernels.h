@@ -27,6 +27,9 @@ constexpr int kMaxPartials = 65536;
 struct PcgSlots {
   int rz, pq, rr, alpha, beta, flag, tol2, iters;
   int seq;  // iterations finished, stopped or not (reported to the host with the ring)
+  // 1 once an iteration has run entirely after the stop flag was set (its deferred x update
+  // done): the iterations after it change nothing, and a merged batch step skips them
+  int skip;
 };
 
 // ------------------------------------------------------------------ elasticity

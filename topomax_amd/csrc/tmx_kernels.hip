@@ -2657,8 +2657,16 @@ __global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __rest
 }
 
 hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
+  // the matrix in LDS up to 160 KiB (n <= 144 unknowns; a 64-node coarsest elasticity level
+  // has 128); larger ones stream it from global memory
+  constexpr size_t kLdsMax = 160 * 1024;
   const size_t lds = sizeof(double) * ((size_t)n * n + n);
-  if (lds <= 64 * 1024) TMX_LAUNCH_OR_RECORD(1024, (dense_invert_lds_body), 1, lds, s, (k_dense_invert_lds), n, a);
+  if (lds > 64 * 1024 && lds <= kLdsMax) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dense_invert_lds),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  if (lds <= kLdsMax) TMX_LAUNCH_OR_RECORD(1024, (dense_invert_lds_body), 1, lds, s, (k_dense_invert_lds), n, a);
   else TMX_LAUNCH_OR_RECORD(1024, (dense_invert_body), 1, sizeof(double) * n, s, (k_dense_invert), n, a);
   return hipGetLastError();
 }
@@ -3032,7 +3040,9 @@ __device__ __forceinline__ void pcg_finalize_body(const BlkId blk, const double*
       rr += sh[1][k];
       rz += sh[2][k];
     }
-    if (sc[q.flag] == 0.0) {
+    const bool was_stopped = sc[q.flag] != 0.0;
+    sc[q.skip] = was_stopped ? 1.0 : 0.0;
+    if (!was_stopped) {
       sc[q.pq] = pq;
       sc[q.rr] = rr;
       sc[q.iters] += 1.0;
@@ -3082,6 +3092,7 @@ __device__ __forceinline__ void pcg_start_body(const BlkId blk, const double* rz
     if (sc[q.flag] == 0.0 && !(rz == rz)) sc[q.flag] = 3.0;
     sc[q.iters] = 0.0;
     sc[q.seq] = 0.0;
+    sc[q.skip] = 0.0;
     sc[q.alpha] = 0.0;  // no deferred x update before the first direction
   }
 }

@@ -27,10 +27,13 @@ struct BlkId {
 __device__ __forceinline__ BlkId hw_blk() { return BlkId{blockIdx.x, gridDim.x}; }
 
 // Block -> design table of one merged launch: block b belongs to design owner[b] (an index into
-// the launch's argument records), whose blocks are first[d] .. first[d + 1] - 1.
+// the launch's argument records), whose blocks are first[d] .. first[d + 1] - 1.  With skip,
+// the blocks of design d do nothing while *skip[d] is nonzero (a device flag that only kernels
+// of other launches write).
 struct MultiMap {
   const unsigned short* owner;
   const int* first;
+  const double* const* skip;
 };
 
 // Pointers read from an argument record in memory are generic pointers to the compiler; this
@@ -52,6 +55,7 @@ __device__ __forceinline__ P* glob(P* p) {
 template <int NT, typename Tup, auto Body>
 __global__ __launch_bounds__(NT) void k_multi(const Tup* __restrict__ recs, MultiMap m) {
   const unsigned d = m.owner[blockIdx.x];
+  if (m.skip && *m.skip[d] != 0.0) return;  // (uniform across the block)
   const int f0 = m.first[d], f1 = m.first[d + 1];
   const Tup& r = recs[d];
   const BlkId b{blockIdx.x - (unsigned)f0, (unsigned)(f1 - f0)};
@@ -72,10 +76,17 @@ hipError_t multi_launch(const void* recs, MultiMap m, unsigned grid, size_t lds,
   return hipGetLastError();
 }
 
+// the merged kernel's name with its template arguments (diagnostics)
+template <int NT, typename Tup, auto Body>
+const char* multi_name() {
+  return __PRETTY_FUNCTION__;
+}
+
 // One recorded launch: the merged kernel, this problem's grid and LDS, and where its argument
 // record sits in the recorder's arena.
 struct LaunchRec {
   MultiFn fn;
+  const char* name;
   unsigned nblk;
   unsigned threads;
   size_t lds;
@@ -97,8 +108,8 @@ class Recorder {
     const size_t off = (arena.size() + 15) & ~size_t(15);
     arena.resize(off + sizeof(Tup));
     std::memcpy(arena.data() + off, static_cast<const void*>(&args), sizeof(Tup));
-    recs.push_back(LaunchRec{&multi_launch<NT, Tup, Body>, nblk, (unsigned)NT, lds, off,
-                             sizeof(Tup)});
+    recs.push_back(LaunchRec{&multi_launch<NT, Tup, Body>, multi_name<NT, Tup, Body>(), nblk,
+                             (unsigned)NT, lds, off, sizeof(Tup)});
   }
 };
 

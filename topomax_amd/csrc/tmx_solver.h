@@ -92,7 +92,7 @@ enum : int { kScBB = 0, kScRR64 = 1, kScJ = 2, kScInt = 3, kScTmp = 4, kScVerify
 // the current PCG solve at kRingBB
 constexpr int kRingBB = 24;
 // PCG slots (flag, rr, iters contiguous for one async read)
-constexpr PcgSlots kPcg = {10, 11, 16, 12, 13, 15, 14, 17, 18};
+constexpr PcgSlots kPcg = {10, 11, 16, 12, 13, 15, 14, 17, 18, 19};
 
 enum LevelKind : int { kElem = 0, kStencil = 1, kDense = 2 };
 enum Phys : int { kElasticity = 0, kFilter = 1 };
