@@ -83,9 +83,10 @@ def bench_kernel(problem, kernel, precision, reps=50):
 def batch_throughput(n_designs: int, steps: int, rtol: float):
     """The batched workload: n_designs seeded designs (N in {40, ..., 320}, aspect 1-3, p 1-5,
     1-3 random tractions, optional body force, random fixed sides;
-    topomax_amd.workloads.batch_cases),
-    mixed precision with autotuned tiles; one objective+gradient each, run one after another
-    vs in one tmx_batch call."""
+    topomax_amd.workloads.batch_cases), mixed precision with autotuned tiles; one
+    objective+gradient each, timed three ways: one design after another, as one tmx_batch call
+    of single-design calls on worker threads (mode "alone"), and as one merged batch step
+    (mode "merged": one launch per PCG phase and V-cycle stage across all designs)."""
     import torch
     from topomax_amd import ProblemBatch
     from topomax_amd.workloads import batch_cases, problem_from_case, smoothed_density
@@ -99,27 +100,39 @@ def batch_throughput(n_designs: int, steps: int, rtol: float):
         probs.append(gp)
         rhos.append(torch.as_tensor(smoothed_density(nx, ny, seed=k), device="cuda:0"))
         ndof += 2 * (nx + 1) * (ny + 1)
-    batch = ProblemBatch(probs)  # default workers: min(16, cores)
+    alone = ProblemBatch(probs, mode="alone")  # default workers: min(16, cores)
+    merged = ProblemBatch(probs, mode="merged")
     for gp, r in zip(probs, rhos):
         gp.objective_and_gradient(r)
-    batch.objective_and_gradient(rhos)
+    alone.objective_and_gradient(rhos)
+    merged.objective_and_gradient(rhos)  # (records and merges the launch plan)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for gp, r in zip(probs, rhos):
-            gp.objective_and_gradient(r)
-    torch.cuda.synchronize()
-    t_serial = (time.perf_counter() - t0) / steps
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.objective_and_gradient(rhos)
-    torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / steps
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_serial = timed(lambda: [gp.objective_and_gradient(r) for gp, r in zip(probs, rhos)])
+    t_alone = timed(lambda: alone.objective_and_gradient(rhos))
+    t_merged = timed(lambda: merged.objective_and_gradient(rhos))
+    rep = merged.last_report
+    its = [st["iterations"] for st in merged.last_stats]
     return {"designs": n_designs, "workload": "seeded random designs, N 40-320, aspect 1-3, "
                                               "p 1-5, mixed precision, autotuned tiles",
-            "dofs": ndof, "workers": "default (min(16, cores))", "ms_serial": t_serial * 1e3,
-            "ms_batched": t_batch * 1e3, "GDOF_s_serial": ndof / t_serial / 1e9,
-            "GDOF_s_batched": ndof / t_batch / 1e9, "speedup": t_serial / t_batch}
+            "dofs": ndof, "ms_serial": t_serial * 1e3, "ms_alone": t_alone * 1e3,
+            "ms_batched": t_merged * 1e3, "GDOF_s_serial": ndof / t_serial / 1e9,
+            "GDOF_s_alone": ndof / t_alone / 1e9, "GDOF_s_batched": ndof / t_merged / 1e9,
+            "speedup": t_serial / t_merged, "speedup_over_alone": t_alone / t_merged,
+            "merged_designs": rep["merged_designs"], "launches_per_step": rep["launches"],
+            "merged_iterations": rep["iterations"],
+            "elasticity_iterations": {"min": min(its), "mean": sum(its) / len(its),
+                                      "max": max(its)},
+            "mode": "merged (one launch per PCG phase and V-cycle stage across the designs); "
+                    "alone: worker threads, min(16, cores), one single-design call each"}
 
 
 def instep_kernel_us(problem, rho, steps, precision, rtol, kernel=ROOFLINE_KERNEL):

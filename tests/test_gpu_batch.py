@@ -1,8 +1,9 @@
 """Batched objective + gradient (tmx_batch_*) over uneven designs: different domains, loads,
 supports, mesh sizes and penalties in one call.
 
-The batch runs the same kernels with the same tiles and deterministic reductions as a solo
-call, only concurrently, so every design's J and w must equal its solo result bit for bit.
+The batch runs the same kernel bodies with the same tiles and deterministic reductions as a
+solo call (as single-design calls on worker threads for handles that have not run yet, as one
+merged step for the others), so every design's J and w must equal its solo result bit for bit.
 One design is also checked against the oracle, and a NaN design must come back as NaN without
 disturbing the others.
 """
@@ -103,8 +104,13 @@ def test_batch_64_random_designs():
     for gp in probs:
         gp.autotune()
     batch = ProblemBatch(probs, workers=4)
-    J, w = batch.objective_and_gradient(rhos)
+    J, w = batch.objective_and_gradient(rhos)  # (fresh handles: single-design calls)
     assert all(np.isfinite(J))
+    J2, w2 = batch.objective_and_gradient(rhos)  # merged: every handle has run once
+    print(batch.last_report, flush=True)
+    assert batch.last_report["merged_designs"] == 64
+    for k in range(64):
+        assert J2[k] == J[k] and torch.equal(w2[k], w[k])
     for k in range(0, 64, 7):  # solo spot checks, bitwise
         Js, ws, _ = probs[k].objective_and_gradient(rhos[k])
         assert J[k] == Js and torch.equal(w[k], ws)
