@@ -10,8 +10,8 @@
 //   int_T (a.l)^p l^beta = 2|T| beta! p! / (|beta| + p + 2)! h_p(a with a_i repeated beta_i
 //   more times),
 // h_p the complete homogeneous symmetric polynomial (the Dirichlet moment identity).
-// Every kernel gathers per node over the triangles that contain it, so results are
-// deterministic.
+// Every kernel gathers per node over the triangles that contain it (the operator kernels from
+// per-triangle forces staged in LDS), so results are deterministic.
 #include <cmath>
 
 #include "tmx_device.h"
@@ -166,6 +166,46 @@ __device__ __forceinline__ void tri_force(const double (&W)[6], const double (&d
   }
 }
 
+// forces of one triangle on all six local nodes (the stress coefficients formed once)
+__device__ __forceinline__ void tri_forces_all(const double (&W)[6], const double (&d)[3][2],
+                                               const double (&u)[6][2], double lam, double mu,
+                                               double (&y)[6][2]) {
+  double S[3][3], E[3][3];
+  stress_coeffs(d, u, lam, mu, S, E);
+  double T[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      T[k][e] = wlk(W, 0, k) * S[0][e] + wlk(W, 1, k) * S[1][e] + wlk(W, 2, k) * S[2][e];
+  double tv[3][3][2];  // T_k d_j
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      tv[k][j][0] = T[k][0] * d[j][0] + T[k][1] * d[j][1];
+      tv[k][j][1] = T[k][1] * d[j][0] + T[k][2] * d[j][1];
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double c = k == a ? 3.0 : -1.0;  // 4 delta - 1
+      sx += c * tv[k][a][0];
+      sy += c * tv[k][a][1];
+    }
+    y[a][0] = sx;
+    y[a][1] = sy;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int i = (k + 1) % 3, j = (k + 2) % 3;
+    y[3 + k][0] = 4.0 * (tv[i][j][0] + tv[j][i][0]);
+    y[3 + k][1] = 4.0 * (tv[i][j][1] + tv[j][i][1]);
+  }
+}
+
 __device__ __forceinline__ bool p2_fixed(unsigned fixed, int I, int J, int nX, int nY) {
   return ((fixed & kLeft) && I == 0) || ((fixed & kRight) && I == nX - 1) ||
          ((fixed & kBottom) && J == 0) || ((fixed & kTop) && J == nY - 1);
@@ -200,30 +240,6 @@ __device__ __forceinline__ void for_each_triangle(const P2Geom& g, int I, int J,
   }
 }
 
-// (K u)_a at lattice node (I, J) with fixed nodes of u taken as zero
-__device__ __forceinline__ void node_force(const P2Geom& g, const double* __restrict__ W,
-                                           const double* __restrict__ u, int I, int J,
-                                           double (&acc)[2]) {
-  const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
-  acc[0] = acc[1] = 0.0;
-  for_each_triangle(g, I, J, [&](int t, int cx, int cy, int a, const int (&NI)[6],
-                                 const int (&NJ)[6]) {
-    double w[6], d[3][2], ul[6][2], y[2];
-    load_W(W, g.nx, cx, cy, t, w);
-    tri_grads(t, g.hx, g.hy, d);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const size_t n = (size_t)NJ[q] * nX + NI[q];
-      const bool fx = p2_fixed(g.fixed, NI[q], NJ[q], nX, nY);
-      ul[q][0] = fx ? 0.0 : u[2 * n];
-      ul[q][1] = fx ? 0.0 : u[2 * n + 1];
-    }
-    tri_force(w, d, ul, g.lam, g.mu, a, y);
-    acc[0] += y[0];
-    acc[1] += y[1];
-  });
-}
-
 __global__ void k_p2_weights(P2Geom g, const double* __restrict__ xi, int p, double m,
                              int positive, double* __restrict__ W) {
   const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -250,25 +266,123 @@ __global__ void k_p2_weights(P2Geom g, const double* __restrict__ xi, int p, dou
   }
 }
 
-__global__ void k_p2_apply(P2Geom g, const double* __restrict__ W, const double* __restrict__ u,
-                           const double* __restrict__ b, double* __restrict__ y, int residual) {
+// Tiled operator: a workgroup owns the lattice nodes of kP2TileC x kP2TileC cells.  It stages
+// the (masked) displacement of the cells touching those nodes (one more cell row and column
+// below and to the left) in LDS, forms each of their triangles' forces on all six local nodes
+// once (one thread per triangle), and then every owned node sums its triangles' forces from
+// LDS (cells bottom to top, left to right, triangle A before B).  Forming the forces per node
+// instead would form each triangle's stresses once for each of its six nodes.
+//   MODE 0: y = K u (fixed nodes: y = u);  1: y = b - K u (fixed: b - u);
+//   2: y = x + omega D^-1 (b - K x) (x null: from zero; fixed nodes 0).
+constexpr int kP2TileC = 8;                  // owned cells per side
+constexpr int kP2TileThreads = 256;
+constexpr int kP2ExtC = kP2TileC + 1;        // cells whose triangles reach the owned nodes
+constexpr int kP2ExtN = 2 * kP2TileC + 3;    // lattice nodes of those cells per side
+
+template <int MODE>
+__global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const double* __restrict__ W,
+                                                            const double* __restrict__ D,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ b,
+                                                            double* __restrict__ y, double omega) {
+  __shared__ double us[kP2ExtN * kP2ExtN][2];
+  __shared__ double fs[kP2ExtC * kP2ExtC][2][6][2];
   const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
-  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (size_t)nX * nY) return;
-  const int I = (int)(n % nX), J = (int)(n / nX);
-  double f[2];
-  if (p2_fixed(g.fixed, I, J, nX, nY)) {
-    f[0] = u[2 * n];
-    f[1] = u[2 * n + 1];
-  } else {
-    node_force(g, W, u, I, J, f);
+  const int ntx = (g.nx + kP2TileC - 1) / kP2TileC;
+  const int X0 = (int)(blockIdx.x % (unsigned)ntx) * kP2TileC;
+  const int Y0 = (int)(blockIdx.x / (unsigned)ntx) * kP2TileC;
+  const int LI0 = 2 * X0 - 2, LJ0 = 2 * Y0 - 2;  // lattice origin of the staged region
+  const bool forces = MODE != 2 || x != nullptr;
+  if (forces) {
+    for (int k = (int)threadIdx.x; k < kP2ExtN * kP2ExtN; k += kP2TileThreads) {
+      const int lj = k / kP2ExtN, li = k - lj * kP2ExtN;
+      const int I = LI0 + li, J = LJ0 + lj;
+      double u0 = 0.0, u1 = 0.0;
+      if (I >= 0 && J >= 0 && I < nX && J < nY && !p2_fixed(g.fixed, I, J, nX, nY)) {
+        const size_t n = (size_t)J * nX + I;
+        u0 = x[2 * n];
+        u1 = x[2 * n + 1];
+      }
+      us[k][0] = u0;
+      us[k][1] = u1;
+    }
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < kP2ExtC * kP2ExtC * 2; k += kP2TileThreads) {
+      const int t = k & 1, c = k >> 1;
+      const int lcy = c / kP2ExtC, lcx = c - lcy * kP2ExtC;
+      const int cx = X0 - 1 + lcx, cy = Y0 - 1 + lcy;
+      if (cx < 0 || cy < 0 || cx >= g.nx || cy >= g.ny) continue;  // (no owned node reads it)
+      double w[6], d[3][2], ul[6][2], f[6][2];
+      load_W(W, g.nx, cx, cy, t, w);
+      tri_grads(t, g.hx, g.hy, d);
+      int NI[6], NJ[6];
+      tri_nodes(t, cx, cy, NI, NJ);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int li = (NI[q] - LI0) + (NJ[q] - LJ0) * kP2ExtN;
+        ul[q][0] = us[li][0];
+        ul[q][1] = us[li][1];
+      }
+      tri_forces_all(w, d, ul, g.lam, g.mu, f);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        fs[c][t][q][0] = f[q][0];
+        fs[c][t][q][1] = f[q][1];
+      }
+    }
+    __syncthreads();
   }
-  if (residual) {
-    y[2 * n] = b[2 * n] - f[0];
-    y[2 * n + 1] = b[2 * n + 1] - f[1];
-  } else {
-    y[2 * n] = f[0];
-    y[2 * n + 1] = f[1];
+  // owned lattice nodes [2 X0, IE) x [2 Y0, JE); the last tile of a row or column also owns
+  // the lattice's last line
+  const int IE = X0 + kP2TileC >= g.nx ? nX : 2 * (X0 + kP2TileC);
+  const int JE = Y0 + kP2TileC >= g.ny ? nY : 2 * (Y0 + kP2TileC);
+  const int OW = IE - 2 * X0, OH = JE - 2 * Y0;
+  for (int k = (int)threadIdx.x; k < OW * OH; k += kP2TileThreads) {
+    const int oj = k / OW, oi = k - oj * OW;
+    const int I = 2 * X0 + oi, J = 2 * Y0 + oj;
+    const size_t n = (size_t)J * nX + I;
+    const bool fx = p2_fixed(g.fixed, I, J, nX, nY);
+    double f0 = 0.0, f1 = 0.0;
+    if (forces && !fx) {
+      const int cx0 = (I & 1) ? (I - 1) / 2 : I / 2 - 1, cx1 = (I & 1) ? (I - 1) / 2 : I / 2;
+      const int cy0 = (J & 1) ? (J - 1) / 2 : J / 2 - 1, cy1 = (J & 1) ? (J - 1) / 2 : J / 2;
+      for (int cy = cy0; cy <= cy1; ++cy) {
+        if (cy < 0 || cy >= g.ny) continue;
+        for (int cx = cx0; cx <= cx1; ++cx) {
+          if (cx < 0 || cx >= g.nx) continue;
+          const int c = (cy - Y0 + 1) * kP2ExtC + (cx - X0 + 1);
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            int NI[6], NJ[6];
+            tri_nodes(t, cx, cy, NI, NJ);
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+              if (NI[q] == I && NJ[q] == J) {
+                f0 += fs[c][t][q][0];
+                f1 += fs[c][t][q][1];
+              }
+          }
+        }
+      }
+    }
+    if constexpr (MODE == 0) {
+      y[2 * n] = fx ? x[2 * n] : f0;
+      y[2 * n + 1] = fx ? x[2 * n + 1] : f1;
+    } else if constexpr (MODE == 1) {
+      y[2 * n] = b[2 * n] - (fx ? x[2 * n] : f0);
+      y[2 * n + 1] = b[2 * n + 1] - (fx ? x[2 * n + 1] : f1);
+    } else {
+      if (fx) {
+        y[2 * n] = y[2 * n + 1] = 0.0;
+      } else {
+        const double rx = b[2 * n] - f0, ry = b[2 * n + 1] - f1;
+        const double a = D[3 * n], c = D[3 * n + 1], e = D[3 * n + 2];
+        const double inv = 1.0 / (a * e - c * c);
+        const double x0 = x ? x[2 * n] : 0.0, x1 = x ? x[2 * n + 1] : 0.0;
+        y[2 * n] = x0 + omega * (e * rx - c * ry) * inv;
+        y[2 * n + 1] = x1 + omega * (a * ry - c * rx) * inv;
+      }
+    }
   }
 }
 
@@ -299,27 +413,6 @@ __global__ void k_p2_diag(P2Geom g, const double* __restrict__ W, double* __rest
   D[3 * n] = dxx;
   D[3 * n + 1] = dxy;
   D[3 * n + 2] = dyy;
-}
-
-__global__ void k_p2_smooth(P2Geom g, const double* __restrict__ W, const double* __restrict__ D,
-                            const double* __restrict__ x, const double* __restrict__ b,
-                            double* __restrict__ y, double omega) {
-  const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
-  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (size_t)nX * nY) return;
-  const int I = (int)(n % nX), J = (int)(n / nX);
-  if (p2_fixed(g.fixed, I, J, nX, nY)) {
-    y[2 * n] = y[2 * n + 1] = 0.0;
-    return;
-  }
-  double f[2] = {0.0, 0.0};
-  if (x) node_force(g, W, x, I, J, f);
-  const double rx = b[2 * n] - f[0], ry = b[2 * n + 1] - f[1];
-  const double a = D[3 * n], c = D[3 * n + 1], e = D[3 * n + 2];
-  const double inv = 1.0 / (a * e - c * c);
-  const double x0 = x ? x[2 * n] : 0.0, x1 = x ? x[2 * n + 1] : 0.0;
-  y[2 * n] = x0 + omega * (e * rx - c * ry) * inv;
-  y[2 * n + 1] = x1 + omega * (a * ry - c * rx) * inv;
 }
 
 // r1 at P1 vertex (i, j) = r2(2i, 2j) + 1/2 (the six adjacent midpoints)
@@ -501,14 +594,20 @@ hipError_t p2_weights(const P2Geom& g, const double* xi, int p, double m, bool p
   return hipGetLastError();
 }
 
+static unsigned p2_tiles(const P2Geom& g) {
+  return (unsigned)(((g.nx + kP2TileC - 1) / kP2TileC) * ((g.ny + kP2TileC - 1) / kP2TileC));
+}
+
 hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s) {
-  k_p2_apply<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, u, nullptr, y, 0);
+  if (!u || !y) return hipErrorInvalidValue;
+  k_p2_tile<0><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, u, nullptr, y, 0.0);
   return hipGetLastError();
 }
 
 hipError_t p2_residual(const P2Geom& g, const double* W, const double* x, const double* b,
                        double* r, hipStream_t s) {
-  k_p2_apply<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, x, b, r, 1);
+  if (!x || !b || !r) return hipErrorInvalidValue;
+  k_p2_tile<1><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, x, b, r, 0.0);
   return hipGetLastError();
 }
 
@@ -519,7 +618,8 @@ hipError_t p2_diag(const P2Geom& g, const double* W, double* D, hipStream_t s) {
 
 hipError_t p2_smooth(const P2Geom& g, const double* W, const double* D, const double* x,
                      const double* b, double* y, double omega, hipStream_t s) {
-  k_p2_smooth<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, D, x, b, y, omega);
+  if (!D || !b || !y) return hipErrorInvalidValue;
+  k_p2_tile<2><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, D, x, b, y, omega);
   return hipGetLastError();
 }
 
