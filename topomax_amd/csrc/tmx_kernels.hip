@@ -2656,7 +2656,73 @@ __global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __rest
   dense_invert_lds_body(hw_blk(), n, a);
 }
 
+// The same Gauss-Jordan steps with the matrix in registers, for n <= 128: thread (ti, tj) of a
+// 32 x 32 arrangement holds the 4 x 4 block of rows 4 ti.. and columns 4 tj..; step k publishes
+// column k and row k through LDS (double-buffered, so one barrier per step) and every thread
+// updates its block from them.  Per step each thread reads 8 values from LDS for its 16
+// updates, a quarter of the LDS traffic of the LDS-resident form, with the same arithmetic.
+constexpr int kDenseRegMax = 128;
+__device__ __forceinline__ void dense_invert_reg_body(const BlkId, int n, double* a) {
+  __shared__ double colb[2][kDenseRegMax], rowb[2][kDenseRegMax];
+  const int tid = (int)threadIdx.x, ti = tid >> 5, tj = tid & 31;
+  const int i0 = 4 * ti, j0 = 4 * tj;
+  double m[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      m[r][c] = (i0 + r < n && j0 + c < n) ? a[(size_t)(i0 + r) * n + j0 + c] : 0.0;
+  for (int k = 0; k < n; ++k) {
+    const int buf = k & 1;
+    if (tj == (k >> 2)) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if ((k & 3) == 0) colb[buf][i0 + r] = m[r][0];
+        else if ((k & 3) == 1) colb[buf][i0 + r] = m[r][1];
+        else if ((k & 3) == 2) colb[buf][i0 + r] = m[r][2];
+        else colb[buf][i0 + r] = m[r][3];
+    }
+    if (ti == (k >> 2)) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if ((k & 3) == 0) rowb[buf][j0 + c] = m[0][c];
+        else if ((k & 3) == 1) rowb[buf][j0 + c] = m[1][c];
+        else if ((k & 3) == 2) rowb[buf][j0 + c] = m[2][c];
+        else rowb[buf][j0 + c] = m[3][c];
+    }
+    __syncthreads();
+    const double rp = 1.0 / colb[buf][k];
+    double cl[4], rw[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cl[r] = colb[buf][i0 + r];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) rw[c] = rowb[buf][j0 + c];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = i0 + r, j = j0 + c;
+        const double mkj = (j == k) ? rp : rw[c] * rp;  // the scaled pivot row
+        if (i == k) m[r][c] = mkj;
+        else m[r][c] = (j == k) ? -cl[r] * rp : m[r][c] - cl[r] * mkj;
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (i0 + r < n && j0 + c < n) a[(size_t)(i0 + r) * n + j0 + c] = m[r][c];
+}
+
+__global__ __launch_bounds__(1024) void k_dense_invert_reg(int n, double* __restrict__ a) {
+  dense_invert_reg_body(hw_blk(), n, a);
+}
+
 hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
+  if (n >= 1 && n <= kDenseRegMax) {
+    TMX_LAUNCH_OR_RECORD(1024, (dense_invert_reg_body), 1, 0, s, (k_dense_invert_reg), n, a);
+    return hipGetLastError();
+  }
   // the matrix in LDS up to 160 KiB (n <= 144 unknowns; a 64-node coarsest elasticity level
   // has 128); larger ones stream it from global memory
   constexpr size_t kLdsMax = 160 * 1024;
