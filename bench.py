@@ -80,7 +80,7 @@ def bench_kernel(problem, kernel, precision, reps=50):
     return us.value, nbytes.value
 
 
-def batch_throughput(n_designs: int, steps: int, rtol: float):
+def batch_throughput(n_designs: int, steps: int, rtol: float, merged_only: bool = False):
     """The batched workload: n_designs seeded designs (N in {40, ..., 320}, aspect 1-3, p 1-5,
     1-3 random tractions, optional body force, random fixed sides;
     topomax_amd.workloads.batch_cases), mixed precision with autotuned tiles; one
@@ -104,7 +104,8 @@ def batch_throughput(n_designs: int, steps: int, rtol: float):
     merged = ProblemBatch(probs, mode="merged")
     for gp, r in zip(probs, rhos):
         gp.objective_and_gradient(r)
-    alone.objective_and_gradient(rhos)
+    if not merged_only:
+        alone.objective_and_gradient(rhos)
     merged.objective_and_gradient(rhos)  # (records and merges the launch plan)
     torch.cuda.synchronize()
 
@@ -116,17 +117,22 @@ def batch_throughput(n_designs: int, steps: int, rtol: float):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    t_serial = timed(lambda: [gp.objective_and_gradient(r) for gp, r in zip(probs, rhos)])
-    t_alone = timed(lambda: alone.objective_and_gradient(rhos))
     t_merged = timed(lambda: merged.objective_and_gradient(rhos))
+    t_serial = t_alone = None
+    if not merged_only:
+        t_serial = timed(lambda: [gp.objective_and_gradient(r) for gp, r in zip(probs, rhos)])
+        t_alone = timed(lambda: alone.objective_and_gradient(rhos))
     rep = merged.last_report
     its = [st["iterations"] for st in merged.last_stats]
     return {"designs": n_designs, "workload": "seeded random designs, N 40-320, aspect 1-3, "
                                               "p 1-5, mixed precision, autotuned tiles",
-            "dofs": ndof, "ms_serial": t_serial * 1e3, "ms_alone": t_alone * 1e3,
-            "ms_batched": t_merged * 1e3, "GDOF_s_serial": ndof / t_serial / 1e9,
-            "GDOF_s_alone": ndof / t_alone / 1e9, "GDOF_s_batched": ndof / t_merged / 1e9,
-            "speedup": t_serial / t_merged, "speedup_over_alone": t_alone / t_merged,
+            "dofs": ndof, "ms_serial": t_serial and t_serial * 1e3,
+            "ms_alone": t_alone and t_alone * 1e3, "ms_batched": t_merged * 1e3,
+            "GDOF_s_serial": t_serial and ndof / t_serial / 1e9,
+            "GDOF_s_alone": t_alone and ndof / t_alone / 1e9,
+            "GDOF_s_batched": ndof / t_merged / 1e9,
+            "speedup": t_serial and t_serial / t_merged,
+            "speedup_over_alone": t_alone and t_alone / t_merged,
             "merged_designs": rep["merged_designs"], "launches_per_step": rep["launches"],
             "merged_iterations": rep["iterations"],
             "elasticity_iterations": {"min": min(its), "mean": sum(its) / len(its),
@@ -232,6 +238,10 @@ def main():
     ap.add_argument("--p2-only", action="store_true",
                     help="time only the P2 element's steps (for a kernel trace of them)")
     ap.add_argument("--batch-designs", type=int, default=64)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="time only the batched workload (for a kernel trace of it)")
+    ap.add_argument("--batch-merged-only", action="store_true",
+                    help="with --batch-only: only the merged batch step")
     ap.add_argument("--roofline-only", action="store_true",
                     help="set up the N operator, then launch only the roofline kernel (PCG "
                          "direction update + FP64 operator, 20x) and a plain copy of the 2 n_v "
@@ -247,6 +257,10 @@ def main():
 
     if args.p2_only:
         print(json.dumps({"p2": p2_throughput(args.p2_N, args.steps, args.rtol)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_throughput(args.batch_designs, args.steps, args.rtol,
+                                                    args.batch_merged_only)}), flush=True)
         return
 
     import torch
