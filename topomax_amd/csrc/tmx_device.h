@@ -13,6 +13,14 @@ constexpr int kWave = 64;
 constexpr int kStripCols = kWave - 2;  // output columns per wavefront strip
 constexpr int kBlock = 256;            // threads per block for sweep / vector kernels
 
+// Index of the calling wavefront among the launch's (kBlock-thread workgroups), read from the
+// first lane so the compiler keeps it, and what is derived from it (strip, chunk, row range),
+// in scalar registers: row loops bounded by it then branch on scalars instead of carrying a
+// divergent exit mask.
+__device__ __forceinline__ int wave_index(unsigned block) {
+  return __builtin_amdgcn_readfirstlane((int)(block * (kBlock / kWave) + threadIdx.x / kWave));
+}
+
 template <typename T>
 struct alignas(2 * sizeof(T)) V2 {
   T x, y;

@@ -137,7 +137,7 @@ __device__ __forceinline__ void elast_sweep_body(const BlkId blk, const ElastOpA
   const SweepShape sh = a.sh;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blk.id);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -412,7 +412,7 @@ __device__ __forceinline__ void elast_sweep_pair_body(const BlkId blk, const Ela
   const SweepShape sh = a.sh;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blk.id);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -695,7 +695,7 @@ __device__ __forceinline__ void elast_sweep_pair_rr_body(const BlkId blk, const 
   constexpr int H = 4;
   const SweepShape sh = a.sh;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blk.id);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -1121,7 +1121,7 @@ __device__ __forceinline__ void elast_sensitivity_body(const BlkId blk, const Se
   using V = V2<double>;
   const SweepShape sh = a.sh;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blk.id);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -1313,7 +1313,7 @@ __device__ __forceinline__ void filter_sweep_body(const BlkId blk, const SweepSh
   constexpr int H = RR ? 2 : 1;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blk.id * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blk.id);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1;
@@ -1502,7 +1502,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_pipe_down(SweepShape sh, Filt
                                                              T* __restrict__ yc) {
   constexpr int H = NU + 2;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blockIdx.x);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1, nxc = nx >> 1;
@@ -1600,7 +1600,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_pipe_up(SweepShape sh, Filter
   constexpr int H = NU;
   const bool want_dot = dpart != nullptr;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+  const int wv = wave_index(blockIdx.x);
   const int strip = wv % sh.nstrips, chunk = wv / sh.nstrips;
   const bool wave_ok = chunk < sh.nchunks;
   const int nx = sh.nx, ny = sh.ny, nvx = nx + 1, nxc = nx >> 1;
