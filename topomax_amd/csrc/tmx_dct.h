@@ -54,7 +54,8 @@ class DctPrecond {
   // longest x row: one workgroup of 256 threads solves it, 12 rows per thread, with the row
   // and two coefficient arrays in LDS (40 KB in FP32, 80 KB in FP64 at this length)
   static constexpr int kMaxRow = 3072;
-  // y-columns per FFT workgroup (two per complex sequence of 2 ny points; ny <= 2048)
+  // fewest y-columns per FFT workgroup (two per complex sequence of 2 ny points; ny <= 2048);
+  // four when their sequences and the twiddle table fit 64 KB of LDS
   static constexpr int kCols = 2;
 
  private:

@@ -2,7 +2,7 @@
 //
 // Data layout: vectors are node-major rows, v = iy (nx + 1) + ix, so a y-column is strided and a
 // mode row (fixed y-mode, all x) is contiguous.  The transforms run on y-columns held in LDS
-// (k_dct1_cols: a group of kCols adjacent columns per workgroup, every row read and written as
+// (k_dct1_cols: a group of 2 or 4 adjacent columns per workgroup, every row read and written as
 // one short contiguous segment); the tridiagonal solves run on mode rows (k_dct_tri).
 #include "tmx_dct.h"
 
@@ -35,7 +35,7 @@ bool fft_plan(int L, FftPlan& plan) {
 }
 
 // ---------------------------------------------------------------- complex FFT in LDS
-constexpr int kFftThreads = 256;
+constexpr int kFftThreads = 512;
 constexpr int kFftMaxPoints = 4096;  // points of the complex sequences of one workgroup
 
 template <typename T>
@@ -161,25 +161,26 @@ __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
   __syncthreads();
 }
 
-// DCT-I of the y-columns x0 .. x0 + kCols - 1 (x0 = kCols * group), all n + 1 rows:
+// DCT-I of the y-columns x0 .. x0 + C - 1 (x0 = C * group, C = 2 or 4), all n + 1 rows:
 //   dst[m][x] = sum_j src[j][x] cos(pi j m / n),  m = 0 .. n,
 // with (INV) partials of sum rdot . dst per workgroup.  Workgroups b, b + 8, b + 16, .. (one
 // XCD each, as workgroups are dealt out to the XCDs in turn) take consecutive column groups,
 // so the workgroups that share a row's cache line share that XCD's L2.
 template <typename T, bool INV>
 __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, int groups,
-                                               const FftPlan& plan, const V2<T>* tw_,
+                                               int C, const FftPlan& plan, const V2<T>* tw_,
                                                const T* src_, T* dst_, const T* rdot_,
                                                double* part) {
   const auto tw = glob(tw_);
   const auto src = glob(src_);
   const auto dst = glob(dst_);
   const auto rdot = glob(rdot_);
-  constexpr int C = DctPrecond<T>::kCols, NS = C / 2;
+  const int NS = C / 2;  // complex sequences (C = 2 or 4 columns)
   extern __shared__ __align__(16) unsigned char dct_lds[];
   V2<T>* z = reinterpret_cast<V2<T>*>(dct_lds);
-  __shared__ V2<T> ends[NS][2];  // each sequence's points 0 and n before the transform
+  __shared__ V2<T> ends[2][2];  // each sequence's points 0 and n before the transform
   const int L = 2 * n;
+  V2<T>* twl = z + NS * L;  // the twiddle table, staged with the columns
   const int per = (int)blk.n / 8;
   const int g = ((int)blk.id % 8) * per + (int)blk.id / 8;
   double acc = 0.0;
@@ -190,19 +191,29 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     // column c is the real (c even) or imaginary part of sequence c / 2; point j is stored
     // at j and (0 < j < n) at its mirror L - j.  Eight loads in flight per thread.
     constexpr int NB = 8;
+    const int lc = C == 4 ? 2 : 1;  // log2 C
+    for (int k0 = (int)threadIdx.x; k0 < L; k0 += 4 * kFftThreads) {
+      V2<T> w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k0 + u * kFftThreads < L) w[u] = tw[k0 + u * kFftThreads];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k0 + u * kFftThreads < L) twl[k0 + u * kFftThreads] = w[u];
+    }
     for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads) {
       T v[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const int i = i0 + u * kFftThreads;
-        const int j = i / C, c = i % C;
+        const int j = i >> lc, c = i & (C - 1);
         v[u] = (i < total && x0 + c < mx) ? src[(size_t)j * mx + x0 + c] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const int i = i0 + u * kFftThreads;
         if (i < total) {
-          const int j = i / C, c = i % C;
+          const int j = i >> lc, c = i & (C - 1);
           const int base = 2 * (c / 2) * L + (c & 1);
           zs[base + 2 * j] = v[u];
           if (j > 0 && j < n) zs[base + 2 * (L - j)] = v[u];
@@ -219,27 +230,40 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     for (int st = 0; st < plan.nstages; ++st) {
       const int R = plan.radix[st];
       switch (R) {
-        case 2: fft_stage<T, 2>(z, L, NS, Ns, tw); break;
-        case 3: fft_stage<T, 3>(z, L, NS, Ns, tw); break;
-        case 4: fft_stage<T, 4>(z, L, NS, Ns, tw); break;
-        case 5: fft_stage<T, 5>(z, L, NS, Ns, tw); break;
-        default: fft_stage<T, 7>(z, L, NS, Ns, tw); break;
+        case 2: fft_stage<T, 2>(z, L, NS, Ns, twl); break;
+        case 3: fft_stage<T, 3>(z, L, NS, Ns, twl); break;
+        case 4: fft_stage<T, 4>(z, L, NS, Ns, twl); break;
+        case 5: fft_stage<T, 5>(z, L, NS, Ns, twl); break;
+        default: fft_stage<T, 7>(z, L, NS, Ns, twl); break;
       }
       Ns *= R;
     }
-    // T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2
-    for (int i = (int)threadIdx.x; i < total; i += kFftThreads) {
-      const int m = i / C, c = i % C, x = x0 + c;
-      if (x < mx) {
-        const int f = c / 2;
-        const V2<T> Y = z[f * L + m];
-        const T y = (c & 1) ? Y.y : Y.x;
-        const T r0 = (c & 1) ? ends[f][0].y : ends[f][0].x;
-        const T rn = (c & 1) ? ends[f][1].y : ends[f][1].x;
-        const T val = T(0.5) * (y + r0 + ((m & 1) ? -rn : rn));
-        const size_t o = (size_t)m * mx + x;
-        dst[o] = val;
-        if constexpr (INV) acc += (double)rdot[o] * (double)val;
+    // T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2; INV reads r for the r.z partials NB points at a
+    // time, so its loads are in flight together
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads) {
+      T rv[NB];
+      if constexpr (INV) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const int i = i0 + u * kFftThreads;
+          const int m = i >> lc, x = x0 + (i & (C - 1));
+          rv[u] = (i < total && x < mx) ? rdot[(size_t)m * mx + x] : T(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = i0 + u * kFftThreads;
+        const int m = i >> lc, c = i & (C - 1), x = x0 + c;
+        if (i < total && x < mx) {
+          const int f = c / 2;
+          const V2<T> Y = z[f * L + m];
+          const T y = (c & 1) ? Y.y : Y.x;
+          const T r0 = (c & 1) ? ends[f][0].y : ends[f][0].x;
+          const T rn = (c & 1) ? ends[f][1].y : ends[f][1].x;
+          const T val = T(0.5) * (y + r0 + ((m & 1) ? -rn : rn));
+          dst[(size_t)m * mx + x] = val;
+          if constexpr (INV) acc += (double)rv[u] * (double)val;
+        }
       }
     }
   }
@@ -247,19 +271,20 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     if (part) {
       const double v[1] = {acc};
       double* const out[1] = {part};
-      block_partials<1>(v, out, blk.id);
+      block_partials<1, kFftThreads>(v, out, blk.id);
     }
   }
 }
 
 template <typename T, bool INV>
-__global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int groups, FftPlan plan,
+__global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int groups, int cols,
+                                                          FftPlan plan,
                                                           const V2<T>* __restrict__ tw,
                                                           const T* __restrict__ src,
                                                           T* __restrict__ dst,
                                                           const T* __restrict__ rdot,
                                                           double* __restrict__ part) {
-  dct1_cols_body<T, INV>(hw_blk(), n, mx, groups, plan, tw, src, dst, rdot, part);
+  dct1_cols_body<T, INV>(hw_blk(), n, mx, groups, cols, plan, tw, src, dst, rdot, part);
 }
 
 // One tridiagonal solve along x per mode row: row m of `modes` is mode m (workgroup m).  Each
@@ -274,7 +299,8 @@ __global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int gr
 //   al_e y_{t-1} + (1 - ga_e A'_s) y_t - ga_e G'_s y_{t+1} = ph_e - ga_e P'_s   (' : block t+1),
 // solved by parallel cyclic reduction in LDS (log2(kTriThreads) steps, one equation per
 // thread); then x_i = P_i - A_i y_{t-1} - G_i y_t.  The operator is diagonally dominant, so the
-// eliminations need no pivoting.  ph / P stay in the staged row; al / A and ga / G sit in LDS
+// eliminations need no pivoting, and FP32 takes the hardware reciprocal (1 ulp; the solver is a
+// preconditioner) where FP64 divides.  ph / P stay in the staged row; al / A and ga / G sit in LDS
 // transposed (index j * nblk + t), so a wave's accesses fall in distinct banks.  LDS is sized
 // to the row (40 KB in FP32 at 3001 nodes: four workgroups per CU).
 constexpr int kTriThreads = 256;
@@ -309,7 +335,21 @@ __device__ __forceinline__ void dct_tri_body(const BlkId blk, int mx, int n, T a
   const T inv_d = (mode == 0 || mode == n) ? T(1) / (T)n : T(2) / (T)n;
   const T b_in = a_off * T(2) + c, b_edge = a_off + c * T(0.5);
   const int tid = threadIdx.x;
-  for (int i = tid; i < mx; i += kTriThreads) lrow[i] = row[i] * inv_d;
+  // the row's loads all in flight before the first use (up to kMaxRow / kTriThreads each)
+  {
+    constexpr int NB = (DctPrecond<T>::kMaxRow + kTriThreads - 1) / kTriThreads;
+    T v[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int i = tid + u * kTriThreads;
+      v[u] = i < mx ? row[i] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int i = tid + u * kTriThreads;
+      if (i < mx) lrow[i] = v[u] * inv_d;
+    }
+  }
   const int s0 = tid * k;
   const int jl = tid < nblk ? min(s0 + k, mx) - 1 - s0 : -1;  // last row of the block, local
   __syncthreads();
@@ -320,7 +360,7 @@ __device__ __forceinline__ void dct_tri_body(const BlkId blk, int mx, int n, T a
     const T ai = i > 0 ? -a_off : T(0);
     const T ci = i < mx - 1 ? -a_off : T(0);
     const T bi = (i == 0 || i == mx - 1) ? b_edge : b_in;
-    const T inv = T(1) / (bi - ai * ga);  // ga = 0 at j = 0
+    const T inv = recip(bi - ai * ga);  // ga = 0 at j = 0
     al = j == 0 ? ai * inv : -ai * al * inv;
     ph = (lrow[i] - ai * ph) * inv;
     ga = ci * inv;
@@ -373,13 +413,13 @@ __device__ __forceinline__ void dct_tri_body(const BlkId blk, int mx, int n, T a
     const int im = tid - st, ip = tid + st;
     T ai = T(0), bi = rb[tid], ci = T(0), fi = rf[tid];
     if (im >= 0) {
-      const T k1 = ra[tid] / rb[im];
+      const T k1 = ra[tid] * recip(rb[im]);
       ai = -ra[im] * k1;
       bi -= rc[im] * k1;
       fi -= rf[im] * k1;
     }
     if (ip < kTriThreads) {
-      const T k2 = rc[tid] / rb[ip];
+      const T k2 = rc[tid] * recip(rb[ip]);
       ci = -rc[ip] * k2;
       bi -= ra[ip] * k2;
       fi -= rf[ip] * k2;
@@ -391,7 +431,7 @@ __device__ __forceinline__ void dct_tri_body(const BlkId blk, int mx, int n, T a
     rf[tid] = fi;
     __syncthreads();
   }
-  const T y = rf[tid] / rb[tid];  // y_t
+  const T y = rf[tid] * recip(rb[tid]);  // y_t
   __syncthreads();
   ra[tid] = y;
   __syncthreads();
@@ -408,6 +448,20 @@ template <typename T>
 __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off, double hxhy,
                                                          double cy, T* __restrict__ modes) {
   dct_tri_body<T>(hw_blk(), mx, n, a_off, hxhy, cy, modes);
+}
+
+// LDS of k_dct1_cols: the complex sequences of `cols` columns (2 n points each per pair) and
+// the twiddle table of 2 n points
+template <typename T>
+static size_t dct_cols_lds(int cols, int n) {
+  return ((size_t)(cols / 2) + 1) * 2 * (size_t)n * sizeof(V2<T>);
+}
+
+// columns per FFT workgroup: four (two complex sequences, 16-byte row segments) when their
+// LDS stays within 64 KB, else two
+template <typename T>
+static int dct_cols(int n) {
+  return (4 / 2) * 2 * n <= kFftMaxPoints && dct_cols_lds<T>(4, n) <= 64 * 1024 ? 4 : 2;
 }
 
 template <typename T>
@@ -451,20 +505,21 @@ int DctPrecond<T>::init(int nx, int ny, double hx, double hy, double eps, hipStr
   for (const void* f : {reinterpret_cast<const void*>(&k_dct1_cols<T, false>),
                         reinterpret_cast<const void*>(&k_dct1_cols<T, true>)})
     TMX_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(kFftMaxPoints * sizeof(V2<T>))));
+                                (int)dct_cols_lds<T>(kCols, kFftMaxPoints / 2)));
   return 0;
 }
 
 template <typename T>
 int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
   const int n = ny_, mx = mx_;
-  const int groups = (mx + kCols - 1) / kCols;
+  const int cols = dct_cols<T>(n);
+  const int groups = (mx + cols - 1) / cols;
   const int nblk = 8 * ((groups + 7) / 8);
-  const size_t lds = (size_t)(kCols / 2) * 2 * (size_t)n * sizeof(V2<T>);
+  const size_t lds = dct_cols_lds<T>(cols, n);
   const V2<T>* tw = reinterpret_cast<const V2<T>*>(tw_.get());
   T* t = t_.get();
   // forward DCT-I along y: t[m] = sum_j r[j] cos(pi j m / n)
-  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, false>), nblk, lds, s, (k_dct1_cols<T, false>), n, mx, groups, plan_, tw, r, t, nullptr, nullptr);
+  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, false>), nblk, lds, s, (k_dct1_cols<T, false>), n, mx, groups, cols, plan_, tw, r, t, nullptr, nullptr);
   TMX_HIP(hipGetLastError());
   // tridiagonal solves along x, one workgroup per mode (scaled by 1 / D_m there)
   const T a_off = (T)(eps_ * eps_ * hy_ / hx_);
@@ -481,7 +536,7 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
     rz->nblocks = nblk;
     part = rz->partials;
   }
-  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, true>), nblk, lds, s, (k_dct1_cols<T, true>), n, mx, groups, plan_, tw, t, z, r, part);
+  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, true>), nblk, lds, s, (k_dct1_cols<T, true>), n, mx, groups, cols, plan_, tw, t, z, r, part);
   TMX_HIP(hipGetLastError());
   return 0;
 }

@@ -106,11 +106,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// block bid of the launch writes its partials to out[k][bid]
-template <int NRED>
+// block bid of the launch writes its partials to out[k][bid] (workgroups of at most NT threads)
+template <int NRED, int NT = kBlock>
 __device__ __forceinline__ void block_partials(const double (&v)[NRED], double* const (&out)[NRED],
                                                unsigned bid) {
-  __shared__ double sh[NRED][kBlock / kWave];
+  __shared__ double sh[NRED][NT / kWave];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
 #pragma unroll
