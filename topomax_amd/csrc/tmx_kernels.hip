@@ -2286,6 +2286,9 @@ size_t tail_lds_bytes(const size_t* nn, int nlev, int bs, size_t elem) {
   return n * elem;
 }
 
+// prefetched values of the coarsest inverse per thread (8 threads per row)
+constexpr int kTailDensePre = 8;
+
 template <typename T, int BS>
 struct TailRow {
   T a[9 * BS * BS];  // a[k*BS*BS + e], k = (dy+1)*3 + (dx+1)
@@ -2300,10 +2303,53 @@ __device__ __forceinline__ void tail_load_row(const TailLevel<T>& L, int v, Tail
   }
 }
 
-// one 9-point operation at node v of a level held in LDS (x, b, y LDS pointers)
+// one 9-point operation at node v of a level held in LDS (x, b, y LDS pointers); nb: the
+// node's nine neighbours (clamped into the grid: the stencil entries towards missing
+// neighbours are zero), inv: 1 / det of the 2x2 diagonal block (BS == 2)
 template <typename T, int BS, int OP>
-__device__ __forceinline__ void tail_node(const TailLevel<T>& L, const TailRow<T, BS>& r,
+__device__ __forceinline__ void tail_node(const TailRow<T, BS>& r, const int (&nb)[9], T inv,
                                           const T* x, const T* b, T* y, T omega, int v) {
+  T acc[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) acc[q] = T(0);
+  if (OP != kSmoothZero) {
+#pragma unroll
+    for (int kk = 0; kk < 9; ++kk) {
+      const int j = nb[kk];
+#pragma unroll
+      for (int a = 0; a < BS; ++a)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) acc[a] += r.a[kk * BS * BS + a * BS + c] * x[BS * j + c];
+    }
+  }
+  T res[BS];
+#pragma unroll
+  for (int q = 0; q < BS; ++q) res[q] = b[BS * v + q] - acc[q];
+  if (OP == kResidual) {
+#pragma unroll
+    for (int q = 0; q < BS; ++q) y[BS * v + q] = res[q];
+    return;
+  }
+  T d[BS];
+  if constexpr (BS == 1) {
+    d[0] = res[0] / r.a[4];
+  } else {
+    const T a00 = r.a[16], a01 = r.a[17], a10 = r.a[18], a11 = r.a[19];
+    d[0] = (a11 * res[0] - a01 * res[1]) * inv;
+    d[1] = (a00 * res[1] - a10 * res[0]) * inv;
+  }
+#pragma unroll
+  for (int q = 0; q < BS; ++q) {
+    const T xo = (OP == kSmoothZero) ? T(0) : x[BS * v + q];
+    y[BS * v + q] = xo + omega * d[q];
+  }
+}
+
+// The same operation with the neighbour indices and the diagonal block's inverse determinant
+// formed from the row on the spot (for rows re-read per sweep, where registers are short)
+template <typename T, int BS, int OP>
+__device__ __forceinline__ void tail_node_direct(const TailLevel<T>& L, const TailRow<T, BS>& r,
+                                                 const T* x, const T* b, T* y, T omega, int v) {
   const int nx = L.nx, ny = L.ny;
   if (v >= (nx + 1) * (ny + 1)) return;
   const int ix = v % (nx + 1), iy = v / (nx + 1);
@@ -2349,19 +2395,45 @@ __device__ __forceinline__ void tail_node(const TailLevel<T>& L, const TailRow<T
 // (36 floats, 9 doubles) or the workgroup is at most 256 threads; otherwise (the FP64 2x2-block
 // row, 36 doubles, at 1024 threads) it is re-read from global memory (an L2 hit) for each
 // sweep, which keeps the kernel within 128 VGPRs without spills.  (A 256-thread tail with the
-// 47x16 level moved to the band kernels measured slower at N=1000.)
+// 47x16 level moved to the band kernels measured slower at N=1000.)  The node's neighbour
+// indices and its diagonal block's inverse determinant are formed once per level.
 template <typename T, int BS>
 struct TailRowHolder {
   static constexpr bool kKeep = sizeof(T) * 9 * BS * BS <= 144 || kTailThreads <= 256;
   TailRow<T, BS> row;
+  int nb[9];  // (kKeep only)
+  T inv = T(0);
+  bool live = false;
+  __device__ __forceinline__ static void neighbours(int ix, int iy, int nx, int ny,
+                                                    int (&nb)[9]) {
+#pragma unroll
+    for (int kk = 0; kk < 9; ++kk) {
+      const int jx = min(max(ix + kk % 3 - 1, 0), nx), jy = min(max(iy + kk / 3 - 1, 0), ny);
+      nb[kk] = jy * (nx + 1) + jx;
+    }
+  }
   __device__ __forceinline__ void level(const TailLevel<T>& L, int v) {
-    if constexpr (kKeep) tail_load_row<T, BS>(L, v, row);
+    if constexpr (kKeep) {
+      const int nx = L.nx, ny = L.ny;
+      live = v < (nx + 1) * (ny + 1);
+      if (!live) return;
+      neighbours(v % (nx + 1), v / (nx + 1), nx, ny, nb);
+      tail_load_row<T, BS>(L, v, row);
+      if constexpr (BS == 2) inv = diag_inv(row);
+    }
+  }
+  __device__ __forceinline__ static T diag_inv(const TailRow<T, BS>& r) {
+    return T(1) / (r.a[16] * r.a[19] - r.a[17] * r.a[18]);
   }
   template <int OP>
   __device__ __forceinline__ void sweep(const TailLevel<T>& L, const T* x, const T* b, T* y,
                                         T omega, int v) {
-    if constexpr (!kKeep) tail_load_row<T, BS>(L, v, row);
-    tail_node<T, BS, OP>(L, row, x, b, y, omega, v);
+    if constexpr (kKeep) {
+      if (live) tail_node<T, BS, OP>(row, nb, inv, x, b, y, omega, v);
+    } else {
+      tail_load_row<T, BS>(L, v, row);
+      tail_node_direct<T, BS, OP>(L, row, x, b, y, omega, v);
+    }
   }
 };
 
@@ -2375,6 +2447,19 @@ __device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc
   // iterate is in buffer 0; after the pre-sweeps it is in buffer nu_post%2 on every level
   const int w_first = (d.nu_pre + d.nu_post - 1) & 1;
   const int w_mid = d.nu_post & 1;
+  // this thread's part of its row of the coarsest inverse, loaded now so that its latency
+  // passes during the down leg (systems of at most 8 kPre unknowns; FP32 only: the FP64
+  // kernel has no registers to spare)
+  constexpr int kPre = sizeof(T) == 4 ? kTailDensePre : 1;
+  double dpre[kPre];
+  {
+    const int row = tid >> 3, part = tid & 7, n = d.cn;
+#pragma unroll
+    for (int k = 0; k < kPre; ++k)
+      dpre[k] = (sizeof(T) == 4 && n <= 8 * kPre && row < n && part + 8 * k < n)
+                    ? d.dense[(size_t)row * n + part + 8 * k]
+                    : 0.0;
+  }
   {  // top-level right-hand side into LDS
     const TailLevel<T>& L = d.lev[0];
     const int n = BS * (L.nx + 1) * (L.ny + 1);
@@ -2417,8 +2502,14 @@ __device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc
       const int row = row0 + (tid >> 3);
       double s = 0.0;
       if (row < n) {
-        const double* arow = d.dense + (size_t)row * n;
-        for (int j = part; j < n; j += 8) s += arow[j] * (double)cb[j];
+        if (sizeof(T) == 4 && n <= 8 * kPre && row0 == 0) {  // the row was loaded at the start
+#pragma unroll
+          for (int k = 0; k < kPre; ++k)
+            if (part + 8 * k < n) s += dpre[k] * (double)cb[part + 8 * k];
+        } else {
+          const double* arow = d.dense + (size_t)row * n;
+          for (int j = part; j < n; j += 8) s += arow[j] * (double)cb[j];
+        }
       }
       s += __shfl_down(s, 4, 8);
       s += __shfl_down(s, 2, 8);
