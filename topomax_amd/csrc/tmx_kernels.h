@@ -101,13 +101,16 @@ template <typename T>
 hipError_t elast_sweep_pair_rr(const ElastOpArgs<T>& a, int rows, const T* b, T* y, T* yc,
                                T om1, T om2, hipStream_t s);
 
-// per-cell mean r from nodal xi (for restriction of level-0 weights)
+// per-cell mean r from nodal xi (for restriction of level-0 weights); with cellw32, also
+// their FP32 copy
 template <typename T>
 hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,
-                              hipStream_t s);
-// coarse cell means = mean of the 4 nested fine triangles of each coarse triangle
+                              hipStream_t s, float* cellw32 = nullptr);
+// coarse cell means = mean of the 4 nested fine triangles of each coarse triangle (and with
+// coarse32 their FP32 copy)
 template <typename T>
-hipError_t elast_restrict_weights(int nxc, int nyc, const T* fine, T* coarse, hipStream_t s);
+hipError_t elast_restrict_weights(int nxc, int nyc, const T* fine, T* coarse, hipStream_t s,
+                                  float* coarse32 = nullptr);
 // explicit 9-point block stencil (BS=2, SoA: st[k*4+a*2+b][node]) of an element level
 template <typename T>
 hipError_t elast_to_stencil(const ElastOpArgs<T>& a, T* st, hipStream_t s);
@@ -185,10 +188,11 @@ template <typename T, int BS>
 hipError_t stencil_prolong_smooth(int nxf, int nyf, int nxc, int nyc, unsigned fixed, const T* st,
                                   const T* x, const T* ec, const T* b, T* y, T omega,
                                   hipStream_t s);
-// Galerkin coarse stencil Ac = P^T Af P for the given transfer (nested or bilinear)
+// Galerkin coarse stencil Ac = P^T Af P for the given transfer (nested or bilinear); with
+// stc32, also its FP32 copy
 template <int BS>
 hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
-                       const double* stf, double* stc, hipStream_t s);
+                       const double* stf, double* stc, hipStream_t s, float* stc32 = nullptr);
 template <int BS>
 hipError_t stencil_to_dense(int nx, int ny, const double* st, double* dense, hipStream_t s);
 
@@ -227,6 +231,8 @@ template <typename T>
 hipError_t dense_apply(int n, const double* ainv, const T* b, T* x, hipStream_t s);
 template <typename TI, typename TO>
 hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s);
+// out64 = in and out32 = (float) in in one pass
+hipError_t copy_split(size_t n, const double* in, double* out64, float* out32, hipStream_t s);
 
 // ------------------------------------------------------------------ vectors
 // independent vector copies in one launch (tasks: device array of count entries, <= 65535)

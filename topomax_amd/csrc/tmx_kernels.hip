@@ -933,32 +933,34 @@ __device__ __forceinline__ void elast_cell_weights_body(const BlkId blk, int nx,
                                                         T m,
                                                         T cp,
                                                         T wmin,
-                                                        V2<T>* cw) {
+                                                        V2<T>* cw, V2<float>* cw32) {
   const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (i >= (size_t)nx * ny) return;
   const int cx = (int)(i % nx), cy = (int)(i / nx);
   const size_t v0 = (size_t)cy * (nx + 1) + cx;
   const T a0 = xi[v0], a1 = xi[v0 + 1], a2 = xi[v0 + nx + 1], a3 = xi[v0 + nx + 2];
-  cw[i] = {pc_weight(m + cp * hsym3(a0, a1, a3, p), wmin),
-           pc_weight(m + cp * hsym3(a0, a2, a3, p), wmin)};
+  const V2<T> w = {pc_weight(m + cp * hsym3(a0, a1, a3, p), wmin),
+                   pc_weight(m + cp * hsym3(a0, a2, a3, p), wmin)};
+  cw[i] = w;
+  if (cw32) cw32[i] = {(float)w.x, (float)w.y};
 }
 
 template <typename T>
 __global__ void k_elast_cell_weights(int nx, int ny, const T* __restrict__ xi, int p, T m, T cp,
-                                     T wmin, V2<T>* __restrict__ cw) {
-  elast_cell_weights_body<T>(hw_blk(), nx, ny, xi, p, m, cp, wmin, cw);
+                                     T wmin, V2<T>* __restrict__ cw, V2<float>* __restrict__ cw32) {
+  elast_cell_weights_body<T>(hw_blk(), nx, ny, xi, p, m, cp, wmin, cw, cw32);
 }
 
 template <typename T>
 hipError_t elast_cell_weights(int nx, int ny, const T* xi, int p, T m, T cp, T wmin, T* cellw,
-                              hipStream_t s) {
-  TMX_LAUNCH_OR_RECORD(kBlock, (elast_cell_weights_body<T>), grid1d((size_t)nx * ny), 0, s, (k_elast_cell_weights<T>), nx, ny, xi, p, m, cp, wmin, reinterpret_cast<V2<T>*>(cellw));
+                              hipStream_t s, float* cellw32) {
+  TMX_LAUNCH_OR_RECORD(kBlock, (elast_cell_weights_body<T>), grid1d((size_t)nx * ny), 0, s, (k_elast_cell_weights<T>), nx, ny, xi, p, m, cp, wmin, reinterpret_cast<V2<T>*>(cellw), reinterpret_cast<V2<float>*>(cellw32));
   return hipGetLastError();
 }
 template hipError_t elast_cell_weights<double>(int, int, const double*, int, double, double,
-                                               double, double*, hipStream_t);
+                                               double, double*, hipStream_t, float*);
 template hipError_t elast_cell_weights<float>(int, int, const float*, int, float, float, float,
-                                              float*, hipStream_t);
+                                              float*, hipStream_t, float*);
 
 // coarse triangle A (lower right) = fine (0,0).A + (1,0).A + (1,0).B + (1,1).A
 // coarse triangle B (upper left)  = fine (0,0).B + (0,1).A + (0,1).B + (1,1).B
@@ -966,7 +968,7 @@ template <typename T>
 __device__ __forceinline__ void restrict_weights_body(const BlkId blk, int nxc,
                                                       int nyc,
                                                       const V2<T>* f,
-                                                      V2<T>* cw) {
+                                                      V2<T>* cw, V2<float>* cw32) {
   const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
   if (i >= (size_t)nxc * nyc) return;
   const int cx = (int)(i % nxc), cy = (int)(i / nxc);
@@ -975,22 +977,28 @@ __device__ __forceinline__ void restrict_weights_body(const BlkId blk, int nxc,
   const V2<T> f10 = f[(size_t)(2 * cy) * nxf + 2 * cx + 1];
   const V2<T> f01 = f[(size_t)(2 * cy + 1) * nxf + 2 * cx];
   const V2<T> f11 = f[(size_t)(2 * cy + 1) * nxf + 2 * cx + 1];
-  cw[i] = {T(0.25) * (f00.x + f10.x + f10.y + f11.x), T(0.25) * (f00.y + f01.x + f01.y + f11.y)};
+  const V2<T> w = {T(0.25) * (f00.x + f10.x + f10.y + f11.x),
+                   T(0.25) * (f00.y + f01.x + f01.y + f11.y)};
+  cw[i] = w;
+  if (cw32) cw32[i] = {(float)w.x, (float)w.y};
 }
 
 template <typename T>
 __global__ void k_restrict_weights(int nxc, int nyc, const V2<T>* __restrict__ f,
-                                   V2<T>* __restrict__ cw) {
-  restrict_weights_body<T>(hw_blk(), nxc, nyc, f, cw);
+                                   V2<T>* __restrict__ cw, V2<float>* __restrict__ cw32) {
+  restrict_weights_body<T>(hw_blk(), nxc, nyc, f, cw, cw32);
 }
 
 template <typename T>
-hipError_t elast_restrict_weights(int nxc, int nyc, const T* fine, T* coarse, hipStream_t s) {
-  TMX_LAUNCH_OR_RECORD(kBlock, (restrict_weights_body<T>), grid1d((size_t)nxc * nyc), 0, s, (k_restrict_weights<T>), nxc, nyc, reinterpret_cast<const V2<T>*>(fine), reinterpret_cast<V2<T>*>(coarse));
+hipError_t elast_restrict_weights(int nxc, int nyc, const T* fine, T* coarse, hipStream_t s,
+                                  float* coarse32) {
+  TMX_LAUNCH_OR_RECORD(kBlock, (restrict_weights_body<T>), grid1d((size_t)nxc * nyc), 0, s, (k_restrict_weights<T>), nxc, nyc, reinterpret_cast<const V2<T>*>(fine), reinterpret_cast<V2<T>*>(coarse), reinterpret_cast<V2<float>*>(coarse32));
   return hipGetLastError();
 }
-template hipError_t elast_restrict_weights<double>(int, int, const double*, double*, hipStream_t);
-template hipError_t elast_restrict_weights<float>(int, int, const float*, float*, hipStream_t);
+template hipError_t elast_restrict_weights<double>(int, int, const double*, double*, hipStream_t,
+                                                   float*);
+template hipError_t elast_restrict_weights<float>(int, int, const float*, float*, hipStream_t,
+                                                  float*);
 
 // ---------------------------------------------------------------- explicit stencils
 // local vertex offsets (dx, dy) and gradient factors (gx in 1/hx, gy in 1/hy) of the two
@@ -2580,7 +2588,7 @@ __device__ __forceinline__ void stencil_rap_body(const BlkId blk, int nxf,
                                                  int nyc,
                                                  unsigned fixed,
                                                  const double* stf,
-                                                 double* stc) {
+                                                 double* stc, float* stc32) {
   const unsigned nnf = (unsigned)(nxf + 1) * (unsigned)(nyf + 1);
   const unsigned nnc = (unsigned)(nxc + 1) * (unsigned)(nyc + 1);
   const unsigned t = blk.id * blockDim.x + threadIdx.x;
@@ -2631,27 +2639,31 @@ __device__ __forceinline__ void stencil_rap_body(const BlkId blk, int nxf,
     for (int a = 0; a < BS; ++a) acc[a * BS + a] = 1.0;
   }
 #pragma unroll
-  for (int e = 0; e < B2; ++e) stc[(size_t)(kc * B2 + e) * nnc + i] = acc[e];
+  for (int e = 0; e < B2; ++e) {
+    stc[(size_t)(kc * B2 + e) * nnc + i] = acc[e];
+    if (stc32) stc32[(size_t)(kc * B2 + e) * nnc + i] = (float)acc[e];
+  }
 }
 
 template <int BS>
 __global__ __launch_bounds__(64) void k_stencil_rap(int nxf, int nyf, int nxc, int nyc, unsigned fixed,
-                              const double* __restrict__ stf, double* __restrict__ stc) {
-  stencil_rap_body<BS>(hw_blk(), nxf, nyf, nxc, nyc, fixed, stf, stc);
+                              const double* __restrict__ stf, double* __restrict__ stc,
+                              float* __restrict__ stc32) {
+  stencil_rap_body<BS>(hw_blk(), nxf, nyf, nxc, nyc, fixed, stf, stc, stc32);
 }
 
 template <int BS>
 hipError_t stencil_rap(int nxf, int nyf, int nxc, int nyc, int nested, unsigned fixed,
-                       const double* stf, double* stc, hipStream_t s) {
+                       const double* stf, double* stc, hipStream_t s, float* stc32) {
   if (nested) return hipErrorInvalidValue;  // 9-point levels are always bilinear coarsenings
   if ((size_t)(nxf + 1) * (nyf + 1) >= (1ull << 31)) return hipErrorInvalidValue;
-  TMX_LAUNCH_OR_RECORD(64, (stencil_rap_body<BS>), grid1d(9 * (size_t)(nxc + 1) * (nyc + 1), 64), 0, s, (k_stencil_rap<BS>), nxf, nyf, nxc, nyc, fixed, stf, stc);
+  TMX_LAUNCH_OR_RECORD(64, (stencil_rap_body<BS>), grid1d(9 * (size_t)(nxc + 1) * (nyc + 1), 64), 0, s, (k_stencil_rap<BS>), nxf, nyf, nxc, nyc, fixed, stf, stc, stc32);
   return hipGetLastError();
 }
 template hipError_t stencil_rap<1>(int, int, int, int, int, unsigned, const double*, double*,
-                                   hipStream_t);
+                                   hipStream_t, float*);
 template hipError_t stencil_rap<2>(int, int, int, int, int, unsigned, const double*, double*,
-                                   hipStream_t);
+                                   hipStream_t, float*);
 
 template <int BS>
 __device__ __forceinline__ void stencil_to_dense_body(const BlkId blk, int nx,
@@ -2747,15 +2759,18 @@ __global__ __launch_bounds__(1024) void k_dense_invert_lds(int n, double* __rest
   dense_invert_lds_body(hw_blk(), n, a);
 }
 
-// The same Gauss-Jordan steps with the matrix in registers, for n <= 128: thread (ti, tj) of a
-// 32 x 32 arrangement holds the 4 x 4 block of rows 4 ti.. and columns 4 tj..; step k publishes
+// The same Gauss-Jordan steps with the matrix in registers, for n <= 4 G: thread (ti, tj) of a
+// G x G arrangement holds the 4 x 4 block of rows 4 ti.. and columns 4 tj..; step k publishes
 // column k and row k through LDS (double-buffered, so one barrier per step) and every thread
 // updates its block from them.  Per step each thread reads 8 values from LDS for its 16
 // updates, a quarter of the LDS traffic of the LDS-resident form, with the same arithmetic.
+// G is the smallest of 8, 16, 32 that covers n: a barrier of fewer waves per step (one wave
+// for n <= 32).
 constexpr int kDenseRegMax = 128;
+template <int G>
 __device__ __forceinline__ void dense_invert_reg_body(const BlkId, int n, double* a) {
-  __shared__ double colb[2][kDenseRegMax], rowb[2][kDenseRegMax];
-  const int tid = (int)threadIdx.x, ti = tid >> 5, tj = tid & 31;
+  __shared__ double colb[2][4 * G], rowb[2][4 * G];
+  const int tid = (int)threadIdx.x, ti = tid / G, tj = tid % G;
   const int i0 = 4 * ti, j0 = 4 * tj;
   double m[4][4];
 #pragma unroll
@@ -2805,13 +2820,19 @@ __device__ __forceinline__ void dense_invert_reg_body(const BlkId, int n, double
       if (i0 + r < n && j0 + c < n) a[(size_t)(i0 + r) * n + j0 + c] = m[r][c];
 }
 
-__global__ __launch_bounds__(1024) void k_dense_invert_reg(int n, double* __restrict__ a) {
-  dense_invert_reg_body(hw_blk(), n, a);
+template <int G>
+__global__ __launch_bounds__(G * G) void k_dense_invert_reg(int n, double* __restrict__ a) {
+  dense_invert_reg_body<G>(hw_blk(), n, a);
 }
 
 hipError_t dense_invert(int n, double* a, double* /*work*/, hipStream_t s) {
   if (n >= 1 && n <= kDenseRegMax) {
-    TMX_LAUNCH_OR_RECORD(1024, (dense_invert_reg_body), 1, 0, s, (k_dense_invert_reg), n, a);
+    if (n <= 32)
+      TMX_LAUNCH_OR_RECORD(64, (dense_invert_reg_body<8>), 1, 0, s, (k_dense_invert_reg<8>), n, a);
+    else if (n <= 64)
+      TMX_LAUNCH_OR_RECORD(256, (dense_invert_reg_body<16>), 1, 0, s, (k_dense_invert_reg<16>), n, a);
+    else
+      TMX_LAUNCH_OR_RECORD(1024, (dense_invert_reg_body<32>), 1, 0, s, (k_dense_invert_reg<32>), n, a);
     return hipGetLastError();
   }
   // the matrix in LDS up to 160 KiB (n <= 144 unknowns; a 64-node coarsest elasticity level
@@ -2878,6 +2899,28 @@ hipError_t convert(size_t n, const TI* in, TO* out, hipStream_t s) {
 }
 template hipError_t convert<double, float>(size_t, const double*, float*, hipStream_t);
 template hipError_t convert<float, double>(size_t, const float*, double*, hipStream_t);
+
+__device__ __forceinline__ void copy_split_body(const BlkId blk, size_t n, const double* in_,
+                                                double* out64_, float* out32_) {
+  const auto in = glob(in_);
+  const auto out64 = glob(out64_);
+  const auto out32 = glob(out32_);
+  for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n; i += (size_t)blk.n * blockDim.x) {
+    const double v = in[i];
+    out64[i] = v;
+    out32[i] = (float)v;
+  }
+}
+
+__global__ void k_copy_split(size_t n, const double* __restrict__ in, double* __restrict__ out64,
+                             float* __restrict__ out32) {
+  copy_split_body(hw_blk(), n, in, out64, out32);
+}
+
+hipError_t copy_split(size_t n, const double* in, double* out64, float* out32, hipStream_t s) {
+  TMX_LAUNCH_OR_RECORD(kBlock, (copy_split_body), std::min<unsigned>(grid1d(n), 8192), 0, s, (k_copy_split), n, in, out64, out32);
+  return hipGetLastError();
+}
 
 // ============================================================================ vectors
 constexpr int kVecBlocks = 1024;

@@ -49,8 +49,10 @@ hipError_t stream_wait(hipStream_t s) {
 }
 
 int Workspace::read(hipStream_t s, int first, int count) {
-  TMX_HIP(hipMemcpyAsync(host + first, scalars.get() + first, sizeof(double) * count,
-                         hipMemcpyDeviceToHost, s));
+  // a one-thread kernel per scalar writes the mapped pinned copy: cheaper to queue than a
+  // device-to-host copy, whose host-side cost the device would otherwise sit idle through
+  for (int k = 0; k < count; ++k)
+    TMX_HIP(publish_scalar(scalars.get() + first + k, host_dev + first + k, s));
   TMX_HIP(stream_wait(s));
   return 0;
 }
@@ -303,10 +305,11 @@ int Hierarchy::build_coarse_stencils(hipStream_t s) {
         src = P.st64.get();
       }
       if (bs_ == 2)
-        TMX_HIP(stencil_rap<2>(P.nx, P.ny, L.nx, L.ny, 0, fixed_, src, L.st64.get(), s));
+        TMX_HIP(stencil_rap<2>(P.nx, P.ny, L.nx, L.ny, 0, fixed_, src, L.st64.get(), s,
+                               L.st32.get()));
       else
-        TMX_HIP(stencil_rap<1>(P.nx, P.ny, L.nx, L.ny, 0, fixed_, src, L.st64.get(), s));
-      TMX_HIP((convert<double, float>(L.st64.size(), L.st64.get(), L.st32.get(), s)));
+        TMX_HIP(stencil_rap<1>(P.nx, P.ny, L.nx, L.ny, 0, fixed_, src, L.st64.get(), s,
+                               L.st32.get()));
     } else if (l == last) {
       TMX_TRY(element_stencil(*this, l, L.st64.get(), s));
     }
@@ -328,16 +331,18 @@ int Hierarchy::set_fine_operator(const double* xi, int p, double m, hipStream_t 
   cp_ = (1.0 - m) * 2.0 / ((p + 1.0) * (p + 2.0));
   has_cw0_ = false;
   const Level& L0 = levels_[0];
-  if (xi != xi64buf_.get()) TMX_HIP(copy_vec<double>(L0.nn, xi, xi64buf_.get(), s));
-  TMX_HIP((convert<double, float>(L0.nn, xi64buf_.get(), xi32_.get(), s)));
+  if (xi != xi64buf_.get())
+    TMX_HIP(copy_split(L0.nn, xi, xi64buf_.get(), xi32_.get(), s));
+  else
+    TMX_HIP((convert<double, float>(L0.nn, xi64buf_.get(), xi32_.get(), s)));
   has_operator_ = true;
   return 0;
 }
 
 int Hierarchy::fine_cell_weights(hipStream_t s) {
   const Level& L0 = levels_[0];
-  TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p_, m_, cp_, m_, cw0_.get(), s));
-  TMX_HIP((convert<double, float>(cw0_.size(), cw0_.get(), cw0_32_.get(), s)));
+  TMX_HIP(elast_cell_weights<double>(L0.nx, L0.ny, xi64_, p_, m_, cp_, m_, cw0_.get(), s,
+                                     cw0_32_.get()));
   has_cw0_ = true;
   return 0;
 }
@@ -355,8 +360,7 @@ int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t 
     Level& L = levels_[l];
     if (L.kind != kElem) break;
     const double* src = l == 1 ? cw0_.get() : levels_[l - 1].cw64.get();
-    TMX_HIP(elast_restrict_weights<double>(L.nx, L.ny, src, L.cw64.get(), s));
-    TMX_HIP((convert<double, float>(L.cw64.size(), L.cw64.get(), L.cw32.get(), s)));
+    TMX_HIP(elast_restrict_weights<double>(L.nx, L.ny, src, L.cw64.get(), s, L.cw32.get()));
   }
   return build_coarse_stencils(s);
 }
@@ -1438,8 +1442,7 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
         TMX_TRY(residual<double>(x, b, rr64_.get(), s));
         TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScVerify, s));
         if (defer) {
-          TMX_HIP(hipMemcpyAsync(ws.host + kScVerify, ws.sc() + kScVerify, sizeof(double),
-                                 hipMemcpyDeviceToHost, s));
+          TMX_HIP(publish_scalar(ws.sc() + kScVerify, ws.host_dev + kScVerify, s));
           S.pending_rr = ws.host + kScVerify;
           S.pending_bb = bb;
         } else {
