@@ -1003,8 +1003,11 @@ template hipError_t elast_restrict_weights<float>(int, int, const float*, float*
 // ---------------------------------------------------------------- explicit stencils
 // local vertex offsets (dx, dy) and gradient factors (gx in 1/hx, gy in 1/hy) of the two
 // triangle types: A = (v0, v1, v3), B = (v0, v2, v3)
-__constant__ int c_tri_off[2][3][2] = {{{0, 0}, {1, 0}, {1, 1}}, {{0, 0}, {0, 1}, {1, 1}}};
-__constant__ double c_tri_grad[2][3][2] = {{{-1, 0}, {1, -1}, {0, 1}}, {{0, -1}, {-1, 1}, {1, 0}}};
+// (compile-time tables: with the loops below unrolled, every stencil index is a constant and
+// the accumulators stay in registers)
+__device__ constexpr int c_tri_off[2][3][2] = {{{0, 0}, {1, 0}, {1, 1}}, {{0, 0}, {0, 1}, {1, 1}}};
+__device__ constexpr double c_tri_grad[2][3][2] = {{{-1, 0}, {1, -1}, {0, 1}},
+                                                   {{0, -1}, {-1, 1}, {1, 0}}};
 
 __device__ inline int st_index(int dx, int dy) { return (dy + 1) * 3 + (dx + 1); }
 
@@ -1021,14 +1024,18 @@ __device__ __forceinline__ void elast_to_stencil_body(const BlkId blk, const Ela
   if (v >= nn) return;
   const int ix = (int)(v % (nx + 1)), iy = (int)(v / (nx + 1));
   double acc[9][4];
+#pragma unroll
   for (int kk = 0; kk < 9; ++kk)
+#pragma unroll
     for (int e = 0; e < 4; ++e) acc[kk][e] = 0.0;
   const bool vf = node_fixed(a.fixed, ix, iy, nx, ny);
   if (vf) {
     acc[4][0] = acc[4][3] = 1.0;
   } else {
     const double area = 0.5 * hx * hy;
+#pragma unroll
     for (int cyo = -1; cyo <= 0; ++cyo)
+#pragma unroll
       for (int cxo = -1; cxo <= 0; ++cxo) {
         const int cx = ix + cxo, cy = iy + cyo;
         if (cx < 0 || cy < 0 || cx >= nx || cy >= ny) continue;
@@ -1043,20 +1050,25 @@ __device__ __forceinline__ void elast_to_stencil_body(const BlkId blk, const Ela
           w[0] = a.cellw[2 * ((size_t)cy * nx + cx)];
           w[1] = a.cellw[2 * ((size_t)cy * nx + cx) + 1];
         }
+#pragma unroll
         for (int t = 0; t < 2; ++t) {
-          int li = -1;
+          int li = -1;  // (cx - ix = cxo, cy - iy = cyo: a compile-time search)
+#pragma unroll
           for (int q = 0; q < 3; ++q)
-            if (cx + c_tri_off[t][q][0] == ix && cy + c_tri_off[t][q][1] == iy) li = q;
+            if (cxo + c_tri_off[t][q][0] == 0 && cyo + c_tri_off[t][q][1] == 0) li = q;
           if (li < 0) continue;
           const double gix = c_tri_grad[t][li][0] / hx, giy = c_tri_grad[t][li][1] / hy;
+#pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int jx = cx + c_tri_off[t][q][0], jy = cy + c_tri_off[t][q][1];
             if (node_fixed(a.fixed, jx, jy, nx, ny)) continue;
             const double gjx = c_tri_grad[t][q][0] / hx, gjy = c_tri_grad[t][q][1] / hy;
             const double gi[2] = {gix, giy}, gj[2] = {gjx, gjy};
             const double dotg = gix * gjx + giy * gjy;
-            const int kk = st_index(jx - ix, jy - iy);
+            const int kk = st_index(cxo + c_tri_off[t][q][0], cyo + c_tri_off[t][q][1]);
+#pragma unroll
             for (int p = 0; p < 2; ++p)
+#pragma unroll
               for (int r = 0; r < 2; ++r) {
                 double val = lam * gi[p] * gj[r] + mu * gi[r] * gj[p];
                 if (p == r) val += mu * dotg;
@@ -1066,7 +1078,9 @@ __device__ __forceinline__ void elast_to_stencil_body(const BlkId blk, const Ela
         }
       }
   }
+#pragma unroll
   for (int kk = 0; kk < 9; ++kk)
+#pragma unroll
     for (int e = 0; e < 4; ++e) st[(size_t)(kk * 4 + e) * nn + v] = (T)acc[kk][e];
 }
 
@@ -2609,10 +2623,15 @@ __device__ __forceinline__ void stencil_rap_body(const BlkId blk, int nxf,
   for (int a = 0; a < 5; ++a) {
     const int gx = px - 2 + a, gy = py - 2 + a;
     const bool ox = gx < 0 || gx > nxf, oy = gy < 0 || gy > nyf;
-    wx[a] = ox ? 0.f : (float)interp1_weight(gx, nxf, nxc, I - 1 + dxc);
-    wy[a] = oy ? 0.f : (float)interp1_weight(gy, nyf, nyc, J - 1 + dyc);
-    wx1[a] = ox ? 0.f : (float)interp1_weight(gx, nxf, nxc, I);
-    wy1[a] = oy ? 0.f : (float)interp1_weight(gy, nyf, nyc, J);
+    // one interpolation stencil per fine position, read for both coarse columns (rows)
+    const Interp1 rx = interp1(gx, nxf, nxc), ry = interp1(gy, nyf, nyc);
+    auto wof = [](const Interp1& r, int c) {
+      return (float)((r.c[0] == c ? r.w[0] : 0.0) + (r.n > 1 && r.c[1] == c ? r.w[1] : 0.0));
+    };
+    wx[a] = ox ? 0.f : wof(rx, I - 1 + dxc);
+    wy[a] = oy ? 0.f : wof(ry, J - 1 + dyc);
+    wx1[a] = ox ? 0.f : wof(rx, I);
+    wy1[a] = oy ? 0.f : wof(ry, J);
   }
   const bool self_fixed = node_fixed(fixed, I, J, nxc, nyc);
   const bool live = !node_fixed(fixed, I - 1 + dxc, J - 1 + dyc, nxc, nyc);
@@ -2625,12 +2644,16 @@ __device__ __forceinline__ void stencil_rap_body(const BlkId blk, int nxf,
         const double wf = (double)(wx1[fxi + 1] * wy1[fyi + 1]);
         const int fx = min(max(px - 1 + fxi, 0), nxf), fy = min(max(py - 1 + fyi, 0), nyf);
         const unsigned f = (unsigned)fy * (unsigned)(nxf + 1) + (unsigned)fx;
+        // the fine row's 9 B2 entries, all loads in flight before the first is used
+        double v[9 * B2];
+#pragma unroll
+        for (int q = 0; q < 9 * B2; ++q) v[q] = stf[(size_t)q * nnf + f];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
           const int ax = fxi + k % 3, ay = fyi + k / 3;  // neighbour g at table index (ax, ay)
           const double w = wf * (double)(wx[ax] * wy[ay]);
 #pragma unroll
-          for (int e = 0; e < B2; ++e) acc[e] += w * stf[(size_t)(k * B2 + e) * nnf + f];
+          for (int e = 0; e < B2; ++e) acc[e] += w * v[k * B2 + e];
         }
       }
   }
