@@ -3237,6 +3237,17 @@ __device__ __forceinline__ void pcg_finalize_body(const BlkId blk, const double*
                                                   double* hout) {
   __shared__ double sh[3][kFinThreads / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+  // the scalars the update reads, loaded before the partial sums so that their latency
+  // passes meanwhile (thread 0 alone uses them)
+  double flag0 = 0.0, tol2 = 0.0, rz_old = 0.0, seq0 = 0.0, iters0 = 0.0, rr_old = 0.0;
+  if (tid == 0) {
+    flag0 = sc[q.flag];
+    tol2 = sc[q.tol2];
+    rz_old = sc[q.rz];
+    seq0 = sc[q.seq];
+    iters0 = sc[q.iters];
+    rr_old = sc[q.rr];
+  }
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   const int n = max(npq, max(nrr, nrz));
   for (int i = tid; i < n; i += kFinThreads) {
@@ -3263,26 +3274,30 @@ __device__ __forceinline__ void pcg_finalize_body(const BlkId blk, const double*
       rr += sh[1][k];
       rz += sh[2][k];
     }
-    const bool was_stopped = sc[q.flag] != 0.0;
+    const bool was_stopped = flag0 != 0.0;
     sc[q.skip] = was_stopped ? 1.0 : 0.0;
+    double flag = flag0, rr_now = rr_old, iters = iters0;
     if (!was_stopped) {
       sc[q.pq] = pq;
-      sc[q.rr] = rr;
-      sc[q.iters] += 1.0;
-      if (!(pq == pq) || !(rr == rr) || !(rz == rz) || rr > 1e300) sc[q.flag] = 3.0;
-      else if (!(pq > 0.0)) sc[q.flag] = 2.0;
-      else if (rr <= sc[q.tol2]) sc[q.flag] = 1.0;
+      rr_now = rr;
+      iters = iters0 + 1.0;
+      if (!(pq == pq) || !(rr == rr) || !(rz == rz) || rr > 1e300) flag = 3.0;
+      else if (!(pq > 0.0)) flag = 2.0;
+      else if (rr <= tol2) flag = 1.0;
+      sc[q.rr] = rr_now;
+      sc[q.iters] = iters;
+      sc[q.flag] = flag;
     }
-    const bool stop = sc[q.flag] != 0.0;
-    sc[q.beta] = stop ? 0.0 : rz / sc[q.rz];
+    const bool stop = flag != 0.0;
+    sc[q.beta] = stop ? 0.0 : rz / rz_old;
     if (!stop) sc[q.rz] = rz;
-    const double seq = sc[q.seq] + 1.0;
+    const double seq = seq0 + 1.0;
     sc[q.seq] = seq;
     if (hout) {  // stop flag, r.r and iteration count straight to pinned host memory, then
                  // the sequence number the host polls for, once the rest is visible
-      hout[0] = sc[q.flag];
-      hout[1] = sc[q.rr];
-      hout[2] = sc[q.iters];
+      hout[0] = flag;
+      hout[1] = rr_now;
+      hout[2] = iters;
       __threadfence_system();
       hout[3] = seq;
     }
