@@ -35,7 +35,14 @@ bool fft_plan(int L, FftPlan& plan) {
 }
 
 // ---------------------------------------------------------------- complex FFT in LDS
-constexpr int kFftThreads = 512;
+// threads per FFT workgroup, and whether the twiddle table is staged in LDS: FP32 512 threads
+// and four columns with the table in LDS; FP64 256 threads and two columns with the table
+// read from global memory, as its butterflies take twice the registers and its columns twice
+// the LDS (the FP32 arrangement measured 164 us per FP64 application against 107 us)
+template <typename T>
+constexpr int kFftThreads = sizeof(T) == 4 ? 512 : 256;
+template <typename T>
+constexpr bool kTwiddlesInLds = sizeof(T) == 4;
 constexpr int kFftMaxPoints = 4096;  // points of the complex sequences of one workgroup
 
 template <typename T>
@@ -128,7 +135,7 @@ __device__ __forceinline__ int div_fft(int j, int d, float dinv) {
 template <typename T, int R>
 __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
                                           const V2<T>* __restrict__ tw) {
-  constexpr int MAXB = (kFftMaxPoints / R + kFftThreads - 1) / kFftThreads;
+  constexpr int MAXB = (kFftMaxPoints / R + kFftThreads<T> - 1) / kFftThreads<T>;
   const int nb = L / R;
   const int total = nseq * nb;
   const int span = L / (Ns * R);
@@ -137,7 +144,7 @@ __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
   int dst[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = (int)threadIdx.x + q * kFftThreads;
+    const int b = (int)threadIdx.x + q * kFftThreads<T>;
     dst[q] = -1;
     if (b < total) {
       const int f = b >= nb ? 1 : 0;  // at most two sequences per workgroup
@@ -166,21 +173,21 @@ __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
 // with (INV) partials of sum rdot . dst per workgroup.  Workgroups b, b + 8, b + 16, .. (one
 // XCD each, as workgroups are dealt out to the XCDs in turn) take consecutive column groups,
 // so the workgroups that share a row's cache line share that XCD's L2.
-template <typename T, bool INV>
+template <typename T, bool INV, int C>
 __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, int groups,
-                                               int C, const FftPlan& plan, const V2<T>* tw_,
+                                               const FftPlan& plan, const V2<T>* tw_,
                                                const T* src_, T* dst_, const T* rdot_,
                                                double* part) {
   const auto tw = glob(tw_);
   const auto src = glob(src_);
   const auto dst = glob(dst_);
   const auto rdot = glob(rdot_);
-  const int NS = C / 2;  // complex sequences (C = 2 or 4 columns)
+  constexpr int NS = C / 2;  // complex sequences (C = 2 or 4 columns)
   extern __shared__ __align__(16) unsigned char dct_lds[];
   V2<T>* z = reinterpret_cast<V2<T>*>(dct_lds);
   __shared__ V2<T> ends[2][2];  // each sequence's points 0 and n before the transform
   const int L = 2 * n;
-  V2<T>* twl = z + NS * L;  // the twiddle table, staged with the columns
+  V2<T>* twl = z + NS * L;  // the twiddle table, staged with the columns (FP32)
   const int per = (int)blk.n / 8;
   const int g = ((int)blk.id % 8) * per + (int)blk.id / 8;
   double acc = 0.0;
@@ -191,27 +198,28 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     // column c is the real (c even) or imaginary part of sequence c / 2; point j is stored
     // at j and (0 < j < n) at its mirror L - j.  Eight loads in flight per thread.
     constexpr int NB = 8;
-    const int lc = C == 4 ? 2 : 1;  // log2 C
-    for (int k0 = (int)threadIdx.x; k0 < L; k0 += 4 * kFftThreads) {
+    constexpr int lc = C == 4 ? 2 : 1;  // log2 C
+    if (kTwiddlesInLds<T>)
+    for (int k0 = (int)threadIdx.x; k0 < L; k0 += 4 * kFftThreads<T>) {
       V2<T> w[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (k0 + u * kFftThreads < L) w[u] = tw[k0 + u * kFftThreads];
+        if (k0 + u * kFftThreads<T> < L) w[u] = tw[k0 + u * kFftThreads<T>];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (k0 + u * kFftThreads < L) twl[k0 + u * kFftThreads] = w[u];
+        if (k0 + u * kFftThreads<T> < L) twl[k0 + u * kFftThreads<T>] = w[u];
     }
-    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads) {
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads<T>) {
       T v[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const int i = i0 + u * kFftThreads;
+        const int i = i0 + u * kFftThreads<T>;
         const int j = i >> lc, c = i & (C - 1);
         v[u] = (i < total && x0 + c < mx) ? src[(size_t)j * mx + x0 + c] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const int i = i0 + u * kFftThreads;
+        const int i = i0 + u * kFftThreads<T>;
         if (i < total) {
           const int j = i >> lc, c = i & (C - 1);
           const int base = 2 * (c / 2) * L + (c & 1);
@@ -229,30 +237,32 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     int Ns = 1;
     for (int st = 0; st < plan.nstages; ++st) {
       const int R = plan.radix[st];
+      const V2<T>* twp = kTwiddlesInLds<T> ? static_cast<const V2<T>*>(twl) : tw;
       switch (R) {
-        case 2: fft_stage<T, 2>(z, L, NS, Ns, twl); break;
-        case 3: fft_stage<T, 3>(z, L, NS, Ns, twl); break;
-        case 4: fft_stage<T, 4>(z, L, NS, Ns, twl); break;
-        case 5: fft_stage<T, 5>(z, L, NS, Ns, twl); break;
-        default: fft_stage<T, 7>(z, L, NS, Ns, twl); break;
+        case 2: fft_stage<T, 2>(z, L, NS, Ns, twp); break;
+        case 3: fft_stage<T, 3>(z, L, NS, Ns, twp); break;
+        case 4: fft_stage<T, 4>(z, L, NS, Ns, twp); break;
+        case 5: fft_stage<T, 5>(z, L, NS, Ns, twp); break;
+        default: fft_stage<T, 7>(z, L, NS, Ns, twp); break;
       }
       Ns *= R;
     }
-    // T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2; INV reads r for the r.z partials NB points at a
-    // time, so its loads are in flight together
-    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads) {
-      T rv[NB];
+    // T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2; INV reads r for the r.z partials NO points at a
+    // time, so its loads are in flight together (FP32; FP64 has no registers to spare)
+    constexpr int NO = sizeof(T) == 4 ? NB : 1;
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NO * kFftThreads<T>) {
+      T rv[NO];
       if constexpr (INV) {
 #pragma unroll
-        for (int u = 0; u < NB; ++u) {
-          const int i = i0 + u * kFftThreads;
+        for (int u = 0; u < NO; ++u) {
+          const int i = i0 + u * kFftThreads<T>;
           const int m = i >> lc, x = x0 + (i & (C - 1));
           rv[u] = (i < total && x < mx) ? rdot[(size_t)m * mx + x] : T(0);
         }
       }
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        const int i = i0 + u * kFftThreads;
+      for (int u = 0; u < NO; ++u) {
+        const int i = i0 + u * kFftThreads<T>;
         const int m = i >> lc, c = i & (C - 1), x = x0 + c;
         if (i < total && x < mx) {
           const int f = c / 2;
@@ -271,20 +281,34 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     if (part) {
       const double v[1] = {acc};
       double* const out[1] = {part};
-      block_partials<1, kFftThreads>(v, out, blk.id);
+      block_partials<1, kFftThreads<T>>(v, out, blk.id);
     }
   }
 }
 
-template <typename T, bool INV>
-__global__ __launch_bounds__(kFftThreads) void k_dct1_cols(int n, int mx, int groups, int cols,
+template <typename T, bool INV, int C>
+__global__ __launch_bounds__(kFftThreads<T>) void k_dct1_cols(int n, int mx, int groups,
                                                           FftPlan plan,
                                                           const V2<T>* __restrict__ tw,
                                                           const T* __restrict__ src,
                                                           T* __restrict__ dst,
                                                           const T* __restrict__ rdot,
                                                           double* __restrict__ part) {
-  dct1_cols_body<T, INV>(hw_blk(), n, mx, groups, cols, plan, tw, src, dst, rdot, part);
+  dct1_cols_body<T, INV, C>(hw_blk(), n, mx, groups, plan, tw, src, dst, rdot, part);
+}
+
+// one DCT-I launch over the column groups (C = 2 or 4 columns per workgroup)
+template <typename T, bool INV>
+static void launch_dct1_cols(int cols, unsigned nblk, size_t lds, hipStream_t s, int n, int mx,
+                             int groups, const FftPlan& plan, const V2<T>* tw, const T* src,
+                             T* dst, const T* rdot, double* part) {
+  if constexpr (sizeof(T) == 4) {  // (FP64 always takes two columns)
+    if (cols == 4) {
+      TMX_LAUNCH_OR_RECORD(kFftThreads<T>, (dct1_cols_body<T, INV, 4>), nblk, lds, s, (k_dct1_cols<T, INV, 4>), n, mx, groups, plan, tw, src, dst, rdot, part);
+      return;
+    }
+  }
+  TMX_LAUNCH_OR_RECORD(kFftThreads<T>, (dct1_cols_body<T, INV, 2>), nblk, lds, s, (k_dct1_cols<T, INV, 2>), n, mx, groups, plan, tw, src, dst, rdot, part);
 }
 
 // One tridiagonal solve along x per mode row: row m of `modes` is mode m (workgroup m).  Each
@@ -454,14 +478,16 @@ __global__ __launch_bounds__(kTriThreads) void k_dct_tri(int mx, int n, T a_off,
 // the twiddle table of 2 n points
 template <typename T>
 static size_t dct_cols_lds(int cols, int n) {
-  return ((size_t)(cols / 2) + 1) * 2 * (size_t)n * sizeof(V2<T>);
+  return ((size_t)(cols / 2) + (kTwiddlesInLds<T> ? 1 : 0)) * 2 * (size_t)n * sizeof(V2<T>);
 }
 
-// columns per FFT workgroup: four (two complex sequences, 16-byte row segments) when their
-// LDS stays within 64 KB, else two
+// columns per FFT workgroup: in FP32 four (two complex sequences, 16-byte row segments) when
+// their LDS stays within 64 KB, else two; two in FP64
 template <typename T>
 static int dct_cols(int n) {
-  return (4 / 2) * 2 * n <= kFftMaxPoints && dct_cols_lds<T>(4, n) <= 64 * 1024 ? 4 : 2;
+  return sizeof(T) == 4 && (4 / 2) * 2 * n <= kFftMaxPoints && dct_cols_lds<T>(4, n) <= 64 * 1024
+             ? 4
+             : 2;
 }
 
 template <typename T>
@@ -502,8 +528,13 @@ int DctPrecond<T>::init(int nx, int ny, double hx, double hy, double eps, hipStr
   TMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dct_tri<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)dct_tri_lds<T>(kMaxRow)));
-  for (const void* f : {reinterpret_cast<const void*>(&k_dct1_cols<T, false>),
-                        reinterpret_cast<const void*>(&k_dct1_cols<T, true>)})
+  std::vector<const void*> fns = {reinterpret_cast<const void*>(&k_dct1_cols<T, false, 2>),
+                                   reinterpret_cast<const void*>(&k_dct1_cols<T, true, 2>)};
+  if constexpr (sizeof(T) == 4) {
+    fns.push_back(reinterpret_cast<const void*>(&k_dct1_cols<T, false, 4>));
+    fns.push_back(reinterpret_cast<const void*>(&k_dct1_cols<T, true, 4>));
+  }
+  for (const void* f : fns)
     TMX_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)dct_cols_lds<T>(kCols, kFftMaxPoints / 2)));
   return 0;
@@ -519,7 +550,7 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
   const V2<T>* tw = reinterpret_cast<const V2<T>*>(tw_.get());
   T* t = t_.get();
   // forward DCT-I along y: t[m] = sum_j r[j] cos(pi j m / n)
-  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, false>), nblk, lds, s, (k_dct1_cols<T, false>), n, mx, groups, cols, plan_, tw, r, t, nullptr, nullptr);
+  launch_dct1_cols<T, false>(cols, nblk, lds, s, n, mx, groups, plan_, tw, r, t, nullptr, nullptr);
   TMX_HIP(hipGetLastError());
   // tridiagonal solves along x, one workgroup per mode (scaled by 1 / D_m there)
   const T a_off = (T)(eps_ * eps_ * hy_ / hx_);
@@ -536,7 +567,7 @@ int DctPrecond<T>::apply(const T* r, T* z, RedSlot* rz, hipStream_t s) {
     rz->nblocks = nblk;
     part = rz->partials;
   }
-  TMX_LAUNCH_OR_RECORD(kFftThreads, (dct1_cols_body<T, true>), nblk, lds, s, (k_dct1_cols<T, true>), n, mx, groups, cols, plan_, tw, t, z, r, part);
+  launch_dct1_cols<T, true>(cols, nblk, lds, s, n, mx, groups, plan_, tw, t, z, r, part);
   TMX_HIP(hipGetLastError());
   return 0;
 }
