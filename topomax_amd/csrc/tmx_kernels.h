@@ -325,6 +325,8 @@ hipError_t logit(size_t n, const double* rho, double* psi, hipStream_t s);
 struct MrSlots {
   int beta1, beta, oldb, alfa, dbar, epsln, oldeps, delta, phibar, phi, cs, sn, denom, iters,
       tol, flag;
+  int skip;  // 1 while an iteration runs after the stop: it changes no iterate
+  int seq;   // iterations finished, stopped or not (reported to the host with the ring)
 };
 hipError_t mr_start(const RedSlot& rz, double* sc, const MrSlots& q, double tol, hipStream_t s);
 hipError_t mr_scale(size_t n, const double* sc, const MrSlots& q, const double* z, double* v,

@@ -3734,6 +3734,8 @@ __global__ __launch_bounds__(kBlock) void k_mr_start(const double* __restrict__ 
     sc[q.iters] = 0.0;
     sc[q.tol] = tol;
     sc[q.flag] = !(rz == rz) ? 3.0 : rz < 0.0 ? 2.0 : rz == 0.0 ? 1.0 : 0.0;
+    sc[q.skip] = 0.0;
+    sc[q.seq] = 0.0;
   }
 }
 
@@ -3742,6 +3744,18 @@ __global__ __launch_bounds__(kBlock) void k_mr_step(const double* __restrict__ p
                                                     double* __restrict__ hout) {
   const double rz = sum_partials(part, npart);
   if (threadIdx.x == 0) {
+    const double seq = sc[q.seq] + 1.0;
+    sc[q.seq] = seq;
+    if (sc[q.flag] != 0.0) {  // stopped before this iteration, which then changes nothing
+      sc[q.skip] = 1.0;
+      hout[0] = sc[q.flag];
+      hout[1] = sc[q.phibar] / sc[q.beta1];
+      hout[2] = sc[q.iters];
+      __threadfence_system();
+      hout[3] = seq;
+      return;
+    }
+    sc[q.skip] = 0.0;
     const double alfa = sc[q.alfa];
     const double beta = rz > 0.0 ? sqrt(rz) : 0.0;
     const double cs = sc[q.cs], sn = sc[q.sn], dbar = sc[q.dbar];
@@ -3768,9 +3782,12 @@ __global__ __launch_bounds__(kBlock) void k_mr_step(const double* __restrict__ p
     else if (rz < 0.0) flag = 2.0;  // preconditioner not positive definite
     else if (sn2 * phibar <= sc[q.tol] * sc[q.beta1] || rz == 0.0) flag = 1.0;
     sc[q.flag] = flag;
+    // the report, then the sequence number the host polls for, once the rest is visible
     hout[0] = flag;
     hout[1] = sc[q.phibar] / sc[q.beta1];
     hout[2] = sc[q.iters];
+    __threadfence_system();
+    hout[3] = seq;
   }
 }
 
@@ -3782,6 +3799,7 @@ __global__ __launch_bounds__(kBlock) void k_mr_update(size_t n, const double* __
                                                       const double* __restrict__ w_older,
                                                       const double* __restrict__ w_newer,
                                                       double* __restrict__ x) {
+  if (sc[q.skip] != 0.0) return;  // (uniform: an iteration after the stop)
   const double oldeps = sc[q.oldeps], delta = sc[q.delta], denom = sc[q.denom];
   const double phi = sc[q.phi], beta = sc[q.beta];
   const double rb = beta > 0.0 ? 1.0 / beta : 0.0;

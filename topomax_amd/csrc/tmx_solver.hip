@@ -1247,7 +1247,7 @@ int Hierarchy::pcg_ru(Workspace& ws, const double* b, double rtol, int max_iter,
 }
 
 // MINRES scalar slots (after the PCG slots)
-constexpr MrSlots kMr = {30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45};
+constexpr MrSlots kMr = {30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47};
 
 int minres_core(Workspace& ws, size_t nv, const double* b, double* X, MinresBufs bf,
                 const std::function<int(const double*, double*)>& apply,
@@ -1295,29 +1295,66 @@ int minres_core(Workspace& ws, size_t nv, const double* b, double* X, MinresBufs
     TMX_HIP(fill_zero_bytes(W[1], sizeof(double) * nv, s));
     TMX_HIP(fill_zero_bytes(W[2], sizeof(double) * nv, s));
     TMX_HIP(mr_scale(nv, sc, kMr, Z, V, s));
-    for (; total < max_iter; ++total) {
+    // Iteration k's mr_step writes (flag, |r|/|b| estimate, iterations) and then the sequence
+    // number k to ring slot (k-1)%2; the host keeps iteration k+1 queued while it waits for
+    // iteration k's report, so the device does not idle between iterations.  An iteration
+    // queued behind the one that stops changes no iterate (mr_step marks it skipped and
+    // mr_update leaves x alone); the true residual below synchronises the stream, so no
+    // report of this run is still in flight when the next run resets the ring.
+    auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
+    ring[3] = ring[7] = 0.0;
+    auto launch = [&](int k) -> int {
       TMX_TRY(apply(V, Y));
       TMX_HIP(mr_lanczos(nv, sc, kMr, V, R1, R2, Y, part, s));
       std::swap(R1, R2);  // r1 <- r2
       std::swap(R2, Y);   // r2 <- y (the new Lanczos vector); y <- the old r1 (free)
       TMX_TRY(precond(R2, Z, &rz));
-      TMX_HIP(mr_step(rz, sc, kMr, ring_dev, s));
+      TMX_HIP(mr_step(rz, sc, kMr, ring_dev + 4 * ((k - 1) & 1), s));
       TMX_HIP(mr_update(nv, sc, kMr, V, Z, W[0], W[1], W[2], X, s));
       std::rotate(W, W + 1, W + 3);
-      TMX_HIP(hipStreamSynchronize(s));
-      const double flag = ring[0];
-      if (flag == 2.0) {
-        set_error("MINRES breakdown: preconditioner not positive definite");
-        return -4;
+      return 0;
+    };
+    auto wait_iter = [&](int k) -> int {
+      volatile double* sl = slot(k);
+      for (long spins = 1; sl[3] < (double)k; ++spins) {
+        if ((spins & 1023) == 0) {
+          const hipError_t q = hipStreamQuery(s);
+          if (q == hipSuccess && sl[3] < (double)k) {
+            set_error("internal: MINRES iteration " + std::to_string(k) + " did not report");
+            return -2;
+          }
+          if (q != hipSuccess && q != hipErrorNotReady) TMX_HIP(q);
+        }
       }
-      if (flag == 3.0) {
-        set_error("non-finite value in MINRES");
-        return -5;
-      }
-      if (flag == 1.0) {
-        ++total;
-        break;
-      }
+      std::atomic_thread_fence(std::memory_order_acquire);
+      return 0;
+    };
+    // An iteration queued behind the one that stops runs in full for nothing (a preconditioner
+    // application), so once the residual estimates predict the stop within the next
+    // iteration the host stops queueing ahead and pays one round trip instead.
+    const int budget = max_iter - total;
+    int launched = 0, done = 0;
+    double flag = 0.0, est_last = 1.0, ratio = 1.0;
+    bool near = false;
+    while (done < budget) {
+      while (launched < budget && launched < done + (near ? 1 : 2)) TMX_TRY(launch(++launched));
+      TMX_TRY(wait_iter(done + 1));
+      ++done;
+      flag = slot(done)[0];
+      if (flag != 0.0) break;
+      const double est = slot(done)[1];
+      ratio = est_last > 0.0 ? est / est_last : 1.0;
+      est_last = est;
+      near = est * std::min(ratio, 1.0) <= 2.0 * tol;
+    }
+    total += done;
+    if (flag == 2.0) {
+      set_error("MINRES breakdown: preconditioner not positive definite");
+      return -4;
+    }
+    if (flag == 3.0) {
+      set_error("non-finite value in MINRES");
+      return -5;
     }
   }
   *iters = total;
