@@ -35,12 +35,13 @@ bool fft_plan(int L, FftPlan& plan) {
 }
 
 // ---------------------------------------------------------------- complex FFT in LDS
-// threads per FFT workgroup, and whether the twiddle table is staged in LDS: FP32 512 threads
-// and four columns with the table in LDS; FP64 256 threads and two columns with the table
-// read from global memory, as its butterflies take twice the registers and its columns twice
-// the LDS (the FP32 arrangement measured 164 us per FP64 application against 107 us)
-template <typename T>
-constexpr int kFftThreads = sizeof(T) == 4 ? 512 : 256;
+// Threads per FFT workgroup by its column count: four columns (two complex sequences) take
+// 512 threads, two take 256.  FP32 stages the twiddle table in LDS; FP64 reads it from global
+// memory and always takes two columns, as its butterflies take twice the registers and its
+// columns twice the LDS (four columns on 512 threads with the table in LDS measured 164 us per
+// FP64 application at N=1000, against 103 us).
+template <int C>
+constexpr int kFftThreads = C == 4 ? 512 : 256;
 template <typename T>
 constexpr bool kTwiddlesInLds = sizeof(T) == 4;
 constexpr int kFftMaxPoints = 4096;  // points of the complex sequences of one workgroup
@@ -132,10 +133,10 @@ __device__ __forceinline__ int div_fft(int j, int d, float dinv) {
 // twiddles them by exp(-2 pi i (j mod Ns) r / (Ns R)), transforms them and writes them to
 // (j div Ns) Ns R + (j mod Ns) + r Ns.  Every thread first reads and transforms all its
 // butterflies into registers, then (after a barrier) writes them, so the stage runs in place.
-template <typename T, int R>
+template <typename T, int R, int NT>
 __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
                                           const V2<T>* __restrict__ tw) {
-  constexpr int MAXB = (kFftMaxPoints / R + kFftThreads<T> - 1) / kFftThreads<T>;
+  constexpr int MAXB = (kFftMaxPoints / R + NT - 1) / NT;
   const int nb = L / R;
   const int total = nseq * nb;
   const int span = L / (Ns * R);
@@ -144,7 +145,7 @@ __device__ __forceinline__ void fft_stage(V2<T>* buf, int L, int nseq, int Ns,
   int dst[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = (int)threadIdx.x + q * kFftThreads<T>;
+    const int b = (int)threadIdx.x + q * NT;
     dst[q] = -1;
     if (b < total) {
       const int f = b >= nb ? 1 : 0;  // at most two sequences per workgroup
@@ -183,6 +184,7 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
   const auto dst = glob(dst_);
   const auto rdot = glob(rdot_);
   constexpr int NS = C / 2;  // complex sequences (C = 2 or 4 columns)
+  constexpr int NT = kFftThreads<C>;
   extern __shared__ __align__(16) unsigned char dct_lds[];
   V2<T>* z = reinterpret_cast<V2<T>*>(dct_lds);
   __shared__ V2<T> ends[2][2];  // each sequence's points 0 and n before the transform
@@ -200,26 +202,26 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     constexpr int NB = 8;
     constexpr int lc = C == 4 ? 2 : 1;  // log2 C
     if (kTwiddlesInLds<T>)
-    for (int k0 = (int)threadIdx.x; k0 < L; k0 += 4 * kFftThreads<T>) {
+    for (int k0 = (int)threadIdx.x; k0 < L; k0 += 4 * NT) {
       V2<T> w[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (k0 + u * kFftThreads<T> < L) w[u] = tw[k0 + u * kFftThreads<T>];
+        if (k0 + u * NT < L) w[u] = tw[k0 + u * NT];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (k0 + u * kFftThreads<T> < L) twl[k0 + u * kFftThreads<T>] = w[u];
+        if (k0 + u * NT < L) twl[k0 + u * NT] = w[u];
     }
-    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * kFftThreads<T>) {
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NB * NT) {
       T v[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const int i = i0 + u * kFftThreads<T>;
+        const int i = i0 + u * NT;
         const int j = i >> lc, c = i & (C - 1);
         v[u] = (i < total && x0 + c < mx) ? src[(size_t)j * mx + x0 + c] : T(0);
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const int i = i0 + u * kFftThreads<T>;
+        const int i = i0 + u * NT;
         if (i < total) {
           const int j = i >> lc, c = i & (C - 1);
           const int base = 2 * (c / 2) * L + (c & 1);
@@ -239,30 +241,30 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
       const int R = plan.radix[st];
       const V2<T>* twp = kTwiddlesInLds<T> ? static_cast<const V2<T>*>(twl) : tw;
       switch (R) {
-        case 2: fft_stage<T, 2>(z, L, NS, Ns, twp); break;
-        case 3: fft_stage<T, 3>(z, L, NS, Ns, twp); break;
-        case 4: fft_stage<T, 4>(z, L, NS, Ns, twp); break;
-        case 5: fft_stage<T, 5>(z, L, NS, Ns, twp); break;
-        default: fft_stage<T, 7>(z, L, NS, Ns, twp); break;
+        case 2: fft_stage<T, 2, NT>(z, L, NS, Ns, twp); break;
+        case 3: fft_stage<T, 3, NT>(z, L, NS, Ns, twp); break;
+        case 4: fft_stage<T, 4, NT>(z, L, NS, Ns, twp); break;
+        case 5: fft_stage<T, 5, NT>(z, L, NS, Ns, twp); break;
+        default: fft_stage<T, 7, NT>(z, L, NS, Ns, twp); break;
       }
       Ns *= R;
     }
     // T[m] = (Y[m] + r_0 + (-1)^m r_n) / 2; INV reads r for the r.z partials NO points at a
     // time, so its loads are in flight together (FP32; FP64 has no registers to spare)
     constexpr int NO = sizeof(T) == 4 ? NB : 1;
-    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NO * kFftThreads<T>) {
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += NO * NT) {
       T rv[NO];
       if constexpr (INV) {
 #pragma unroll
         for (int u = 0; u < NO; ++u) {
-          const int i = i0 + u * kFftThreads<T>;
+          const int i = i0 + u * NT;
           const int m = i >> lc, x = x0 + (i & (C - 1));
           rv[u] = (i < total && x < mx) ? rdot[(size_t)m * mx + x] : T(0);
         }
       }
 #pragma unroll
       for (int u = 0; u < NO; ++u) {
-        const int i = i0 + u * kFftThreads<T>;
+        const int i = i0 + u * NT;
         const int m = i >> lc, c = i & (C - 1), x = x0 + c;
         if (i < total && x < mx) {
           const int f = c / 2;
@@ -281,13 +283,13 @@ __device__ __forceinline__ void dct1_cols_body(const BlkId blk, int n, int mx, i
     if (part) {
       const double v[1] = {acc};
       double* const out[1] = {part};
-      block_partials<1, kFftThreads<T>>(v, out, blk.id);
+      block_partials<1, NT>(v, out, blk.id);
     }
   }
 }
 
 template <typename T, bool INV, int C>
-__global__ __launch_bounds__(kFftThreads<T>) void k_dct1_cols(int n, int mx, int groups,
+__global__ __launch_bounds__(kFftThreads<C>) void k_dct1_cols(int n, int mx, int groups,
                                                           FftPlan plan,
                                                           const V2<T>* __restrict__ tw,
                                                           const T* __restrict__ src,
@@ -304,11 +306,11 @@ static void launch_dct1_cols(int cols, unsigned nblk, size_t lds, hipStream_t s,
                              T* dst, const T* rdot, double* part) {
   if constexpr (sizeof(T) == 4) {  // (FP64 always takes two columns)
     if (cols == 4) {
-      TMX_LAUNCH_OR_RECORD(kFftThreads<T>, (dct1_cols_body<T, INV, 4>), nblk, lds, s, (k_dct1_cols<T, INV, 4>), n, mx, groups, plan, tw, src, dst, rdot, part);
+      TMX_LAUNCH_OR_RECORD(kFftThreads<4>, (dct1_cols_body<T, INV, 4>), nblk, lds, s, (k_dct1_cols<T, INV, 4>), n, mx, groups, plan, tw, src, dst, rdot, part);
       return;
     }
   }
-  TMX_LAUNCH_OR_RECORD(kFftThreads<T>, (dct1_cols_body<T, INV, 2>), nblk, lds, s, (k_dct1_cols<T, INV, 2>), n, mx, groups, plan, tw, src, dst, rdot, part);
+  TMX_LAUNCH_OR_RECORD(kFftThreads<2>, (dct1_cols_body<T, INV, 2>), nblk, lds, s, (k_dct1_cols<T, INV, 2>), n, mx, groups, plan, tw, src, dst, rdot, part);
 }
 
 // One tridiagonal solve along x per mode row: row m of `modes` is mode m (workgroup m).  Each
@@ -481,11 +483,14 @@ static size_t dct_cols_lds(int cols, int n) {
   return ((size_t)(cols / 2) + (kTwiddlesInLds<T> ? 1 : 0)) * 2 * (size_t)n * sizeof(V2<T>);
 }
 
-// columns per FFT workgroup: in FP32 four (two complex sequences, 16-byte row segments) when
-// their LDS stays within 64 KB, else two; two in FP64
+// columns per FFT workgroup: in FP32 four (two complex sequences, 16-byte row segments, on
+// 512 threads) for columns of 512 to 1024 cells, whose FFTs give each thread a few butterflies
+// per stage; else two (on 256 threads: shorter columns leave most of 512 threads idle, and
+// longer ones exceed the LDS); two in FP64
 template <typename T>
 static int dct_cols(int n) {
-  return sizeof(T) == 4 && (4 / 2) * 2 * n <= kFftMaxPoints && dct_cols_lds<T>(4, n) <= 64 * 1024
+  return sizeof(T) == 4 && n >= 512 && (4 / 2) * 2 * n <= kFftMaxPoints &&
+                 dct_cols_lds<T>(4, n) <= 64 * 1024
              ? 4
              : 2;
 }
