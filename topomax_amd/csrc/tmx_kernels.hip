@@ -2441,8 +2441,13 @@ struct TailRowHolder {
       if (!live) return;
       neighbours(v % (nx + 1), v / (nx + 1), nx, ny, nb);
       tail_load_row<T, BS>(L, v, row);
-      if constexpr (BS == 2) inv = diag_inv(row);
     }
+  }
+  // the diagonal block's inverse determinant, formed at the level's first sweep (after the
+  // phase that the row loads overlap)
+  __device__ __forceinline__ void finish() {
+    if constexpr (kKeep && BS == 2)
+      if (live) inv = diag_inv(row);
   }
   __device__ __forceinline__ static T diag_inv(const TailRow<T, BS>& r) {
     return T(1) / (r.a[16] * r.a[19] - r.a[17] * r.a[18]);
@@ -2482,6 +2487,12 @@ __device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc
                     ? d.dense[(size_t)row * n + part + 8 * k]
                     : 0.0;
   }
+  // The holder keeps one level's rows.  Each level's rows are requested one phase before
+  // their first sweep (before the right-hand side's load, the restriction or the
+  // prolongation), so their load latency passes during that phase; the deepest 9-point
+  // level's rows serve both legs.
+  TailRowHolder<T, BS> rh;
+  rh.level(d.lev[0], tid);
   {  // top-level right-hand side into LDS
     const TailLevel<T>& L = d.lev[0];
     const int n = BS * (L.nx + 1) * (L.ny + 1);
@@ -2495,9 +2506,8 @@ __device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc
     T* const b0 = lds + L.off;
     T* const b1 = b0 + BS * nn;
     const T* b = b0 + 2 * BS * nn;
-    TailRowHolder<T, BS> rh;
-    rh.level(L, tid);
     int w = w_first;
+    rh.finish();
     rh.template sweep<kSmoothZero>(L, nullptr, b, w ? b1 : b0, d.omega[0], tid);
     __syncthreads();
     for (int k = 1; k < d.nu_pre; ++k) {
@@ -2507,6 +2517,7 @@ __device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc
     }
     T* const spare = w ? b0 : b1;  // the residual goes to the free ping-pong buffer
     rh.template sweep<kResidual>(L, w ? b1 : b0, b, spare, T(0), tid);
+    if (l + 2 < nl) rh.level(d.lev[l + 1], tid);  // (the dense coarsest has no rows)
     __syncthreads();
     const TailLevel<T>& C = d.lev[l + 1];
     const int nnc = (C.nx + 1) * (C.ny + 1);
@@ -2548,13 +2559,14 @@ __device__ __forceinline__ void tail_vcycle_body(const BlkId blk, const TailDesc
     T* const b1 = b0 + BS * nn;
     const T* b = b0 + 2 * BS * nn;
     const T* cx = lds + C.off + (l + 2 == nl ? d.cn : 0);
-    TailRowHolder<T, BS> rh;
-    rh.level(L, tid);  // issued before the prolongation to overlap its latency
     int w = w_mid;
     if (tid < nn) prolong_bilinear_node<T, BS>(L.nx, L.ny, C.nx, C.ny, d.fixed, cx, w ? b1 : b0, tid);
+    if (d.nu_post == 0 && l > 0) rh.level(d.lev[l - 1], tid);
     __syncthreads();
+    if (l + 2 < nl) rh.finish();  // (the deepest level's is still set from the down leg)
     for (int k = 0; k < d.nu_post; ++k) {
       rh.template sweep<kSmooth>(L, w ? b1 : b0, b, w ? b0 : b1, d.omega[k], tid);
+      if (k + 1 == d.nu_post && l > 0) rh.level(d.lev[l - 1], tid);
       __syncthreads();
       w ^= 1;
     }
