@@ -163,9 +163,9 @@ __device__ __forceinline__ void band_row(const BandLevel<T>& L, const T* __restr
   }
   const T* D = R.a + 4 * BS * BS;
   if constexpr (BS == 1) {
-    R.dinv[0] = T(1) / D[0];
+    R.dinv[0] = recip(D[0]);
   } else {
-    const T inv = T(1) / (D[0] * D[3] - D[1] * D[2]);
+    const T inv = recip(D[0] * D[3] - D[1] * D[2]);
     R.dinv[0] = D[3] * inv;
     R.dinv[1] = -D[1] * inv;
     R.dinv[2] = -D[2] * inv;
