@@ -251,8 +251,12 @@ enum tmx_option {
                                    2 (default): 512 threads on the standard tiles; 3: 1024 */
   TMX_OPT_BENCH_COLD = 26,      /* 1: tmx_bench_kernel times each launch alone after reading
                                    512 MiB (no inputs left in the caches); 0: back to back */
-  TMX_OPT_RU_EVERY = 27         /* TMX_MIXED_RU: iterations between residual replacements
+  TMX_OPT_RU_EVERY = 27,        /* TMX_MIXED_RU: iterations between residual replacements
                                    (default 4; none once the residual nears the tolerance) */
+  TMX_OPT_SETUP_OVERLAP = 28    /* 1 (default): an elasticity solve builds the coarse operators
+                                   (Galerkin levels, coarsest inverse) on an internal stream
+                                   while its first V-cycle smooths the finer levels; 0: in
+                                   order on the solve's stream */
 };
 int tmx_set_option(tmx_problem* h, int option, double value);
 /* One multigrid V-cycle z = M^-1 r (the PCG preconditioner).  which: 0 = elasticity with the

@@ -35,6 +35,7 @@ TMX_OPT_PAIR_LEVELS, TMX_OPT_PAIR_ROWS_COARSE, TMX_OPT_PAIR_NU_COARSE = 22, 23, 
 TMX_OPT_BAND_WIDE = 25
 TMX_OPT_BENCH_COLD = 26
 TMX_OPT_RU_EVERY = 27
+TMX_OPT_SETUP_OVERLAP = 28
 TMX_OPT_NU_FINE_FILTER, TMX_OPT_NU_COARSE_FILTER = 8, 9
 TMX_BATCH_MERGED, TMX_BATCH_ALONE = 0, 1
 

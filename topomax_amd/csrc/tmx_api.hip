@@ -375,7 +375,7 @@ static int elasticity_solve(tmx_problem* h, const double* xi, double* u, const t
                             SolveStats* st, bool sync = true) {
   TMX_TRY(need_penalty(h));
   TMX_HIP(hipEventRecord(h->ev_setup[0], h->stream));
-  TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, h->stream));
+  TMX_TRY(h->el.update_elasticity(xi, h->p, h->m, h->stream, h->degree != 2));
   // lambda_max(D^-1 A) of block Jacobi hardly depends on the design (the factors scale D and
   // A alike), so the first operator's estimate (with a margin) serves the later ones
   if (h->el.cheb_auto_ && !h->el.cheb_done_) TMX_TRY(h->el.estimate_lmax(h->ws, h->stream));
@@ -627,6 +627,10 @@ extern "C" int tmx_set_option(tmx_problem* h, int option, double value) {
       break;
     case TMX_OPT_BENCH_COLD:
       h->bench_cold = value != 0.0;
+      break;
+    case TMX_OPT_SETUP_OVERLAP:
+      TMX_TRY(h->el.join_coarse(h->stream));
+      h->el.overlap_setup_ = value != 0.0;
       break;
     case TMX_OPT_RU_EVERY:
       if (value < 1 || value > 1000 || value != std::floor(value))

@@ -171,8 +171,14 @@ class Hierarchy {
  public:
   int init(Phys phys, int nx, int ny, double hx, double hy, unsigned fixed, double lam,
            double mu, double eps, int coarse_max_nodes, hipStream_t s);
-  // elasticity: copy nodal xi (device, fp64) and rebuild the coarse operators, penalty p
-  int update_elasticity(const double* xi, int p, double m, hipStream_t s);
+  // elasticity: copy nodal xi (device, fp64) and rebuild the coarse operators, penalty p.
+  // With overlap, the operators below the cell-weight levels (element-to-stencil, Galerkin
+  // products, coarsest inverse) are built on the hierarchy's side stream, concurrently with
+  // what s runs next; vcycle() makes s wait for them before the first level that reads them,
+  // and solve() before it returns (join_coarse).
+  int update_elasticity(const double* xi, int p, double m, hipStream_t s, bool overlap = false);
+  // make s wait for a coarse-operator build still running on the side stream
+  int join_coarse(hipStream_t s);
   // the preconditioner's per-cell weights of the finest cells from the current xi, p, m
   int fine_cell_weights(hipStream_t s);
   // elasticity fine operator only (copies xi; no coarse rebuild)
@@ -317,6 +323,15 @@ class Hierarchy {
     else return dct32_;
   }
   hipEvent_t solve_ev_[2] = {nullptr, nullptr};  // around an unsynchronised solve
+  // side stream of the overlapped coarse-operator build (default priority), the events that
+  // fork it from and join it to the solve's stream, and the first level whose operator it
+  // builds
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
+  bool coarse_pending_ = false;
+  int coarse_first_ = 0;
+  // build the coarse operators on the side stream (TMX_OPT_SETUP_OVERLAP)
+  bool overlap_setup_ = true;
   // In-step timing of the PCG direction update + operator (apply_update): with au_timing_ on,
   // each launch inside a PCG iteration is bracketed by a pair of events on the solve's stream
   // (up to kAuEvents launches between two reads; direct launches only, not under graphs)

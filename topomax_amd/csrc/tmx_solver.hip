@@ -8,6 +8,7 @@
 #include "tmx_solver.h"
 #include "tmx_dct.h"
 #include "tmx_band.h"
+#include "tmx_multi.h"
 
 #include <algorithm>
 #include <atomic>
@@ -172,7 +173,11 @@ void Hierarchy::drop_graphs() {
 }
 
 Hierarchy::~Hierarchy() {
+  if (side_) (void)hipStreamSynchronize(side_);
   drop_graphs();
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  if (join_ev_) (void)hipEventDestroy(join_ev_);
+  if (side_) (void)hipStreamDestroy(side_);
   for (auto& e : au_ev_)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : solve_ev_)
@@ -352,17 +357,45 @@ bool Hierarchy::pair_post_ok(int l) const {
          !(levels_[l].nx & 1) && !(levels_[l].ny & 1);
 }
 
-int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s) {
+int Hierarchy::update_elasticity(const double* xi, int p, double m, hipStream_t s,
+                                 bool overlap) {
+  TMX_TRY(join_coarse(s));  // (the previous build reads the cell weights rewritten below)
   TMX_TRY(set_fine_operator(xi, p, m, s));
   const int last = (int)levels_.size() - 1;
   TMX_TRY(fine_cell_weights(s));
+  int first = last;  // the first level that build_coarse_stencils writes
   for (int l = 1; l <= last; ++l) {
     Level& L = levels_[l];
-    if (L.kind != kElem) break;
+    if (L.kind != kElem) {
+      first = l;
+      break;
+    }
     const double* src = l == 1 ? cw0_.get() : levels_[l - 1].cw64.get();
     TMX_HIP(elast_restrict_weights<double>(L.nx, L.ny, src, L.cw64.get(), s, L.cw32.get()));
   }
+  // The side stream runs the build while s runs the finest and cell-weight levels of the
+  // first V-cycle.  Not while launches are recorded for a merged batch or captured into
+  // graphs, where a second stream would escape them.
+  if (overlap && overlap_setup_ && !use_graphs_ && !active_recorder() && first >= 1) {
+    if (!side_) TMX_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    if (!fork_ev_) TMX_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    if (!join_ev_) TMX_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
+    TMX_HIP(hipEventRecord(fork_ev_, s));
+    TMX_HIP(hipStreamWaitEvent(side_, fork_ev_, 0));
+    const int rc = build_coarse_stencils(side_);
+    TMX_HIP(hipEventRecord(join_ev_, side_));  // (joined also when a launch failed)
+    coarse_pending_ = true;
+    coarse_first_ = first;
+    return rc;
+  }
   return build_coarse_stencils(s);
+}
+
+int Hierarchy::join_coarse(hipStream_t s) {
+  if (!coarse_pending_) return 0;
+  coarse_pending_ = false;
+  TMX_HIP(hipStreamWaitEvent(s, join_ev_, 0));
+  return 0;
 }
 
 int Hierarchy::build_filter(hipStream_t s) { return build_coarse_stencils(s); }
@@ -648,6 +681,7 @@ static bool band_level(const Hierarchy& h, int l, BandLevel<T>& B) {
 template <typename T>
 int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz) {
   const int last = (int)levels_.size() - 1;
+  if (coarse_pending_ && l >= coarse_first_) TMX_TRY(join_coarse(s));
   if (l == last) {
     TMX_HIP(dense_apply<T>(cn_, dense_.get(), b, out, s));
     if (dot_bz) TMX_HIP(dot_t<T>((size_t)cn_, b, out, *dot_bz, s));
@@ -793,6 +827,7 @@ int Hierarchy::vcycle(int l, const T* b, T* out, hipStream_t s, RedSlot* dot_bz)
 
 int Hierarchy::estimate_lmax(Workspace& ws, hipStream_t s, double margin) {
   constexpr int kIters = 24;
+  TMX_TRY(join_coarse(s));
   RedSlot r = ws.region(3);
   double lmax = 0.0;
   const int last = (int)levels_.size() - 1;
@@ -1393,6 +1428,13 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
                      SolveStats* st, hipStream_t s) {
   const size_t nv = n();
   auto t0 = std::chrono::steady_clock::now();
+  // a coarse-operator build of update_elasticity(overlap) that no V-cycle joined (no
+  // iteration ran) is ordered before whatever s runs after the solve
+  struct JoinAtExit {
+    Hierarchy* h;
+    hipStream_t s;
+    ~JoinAtExit() { (void)h->join_coarse(s); }
+  } join_at_exit{this, s};
   RedSlot gen = ws.region(3);
   SolveStats local;
   SolveStats& S = st ? *st : local;

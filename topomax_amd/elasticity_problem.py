@@ -422,7 +422,7 @@ class ElasticityProblem:
         nu_coarse_filter, omega_elast, omega_filter, filter_rtol, cheb_elast, cheb_filter,
         cheb_ratio, band, band_fine_filter, fine_cell_weights, graphs, sweep_pairs,
         sweep_pairs_post, sweep_pair_restrict, filter_pipe, filter_dct, pair_levels,
-        pair_rows_coarse, pair_nu_coarse, band_wide, bench_cold, ru_every."""
+        pair_rows_coarse, pair_nu_coarse, band_wide, bench_cold, ru_every, setup_overlap."""
         opt = {"fused": _lib.TMX_OPT_FUSED, "tail": _lib.TMX_OPT_TAIL,
                "nu_pre": _lib.TMX_OPT_NU_PRE, "nu_post": _lib.TMX_OPT_NU_POST,
                "nu_coarse": _lib.TMX_OPT_NU_COARSE,
@@ -448,7 +448,8 @@ class ElasticityProblem:
                "pair_nu_coarse": _lib.TMX_OPT_PAIR_NU_COARSE,
                "band_wide": _lib.TMX_OPT_BAND_WIDE,
                "bench_cold": _lib.TMX_OPT_BENCH_COLD,
-               "ru_every": _lib.TMX_OPT_RU_EVERY}[name]
+               "ru_every": _lib.TMX_OPT_RU_EVERY,
+               "setup_overlap": _lib.TMX_OPT_SETUP_OVERLAP}[name]
         _lib.check(self._h.lib.tmx_set_option(self._h.ptr, opt, float(value)))
 
     def precondition(self, r, which: str = "elasticity", xi=None, precision: str = "fp64"):
