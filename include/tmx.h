@@ -105,7 +105,9 @@ typedef struct tmx_solve_stats {
   int iterations;             /* PCG iterations (all inner solves) */
   int outer_iterations;       /* refinement steps (mixed), 1 for FP64 */
   double rel_residual;        /* final |b - A u| / |b| */
-  double setup_ms, solve_ms;  /* operator rebuild (device time) / solve (host clock) */
+  double setup_ms, solve_ms;  /* operator rebuild (device time on the solve's stream; with
+                                 TMX_OPT_SETUP_OVERLAP the coarse part runs beside the solve
+                                 and is not in it) / solve (host clock) */
   int levels;                 /* multigrid levels */
 } tmx_solve_stats;
 
