@@ -246,9 +246,10 @@ hipError_t copy_tasks(const CopyTask* tasks, int count, hipStream_t s);
 hipError_t publish_scalar(const double* src, double* host_dst, hipStream_t s);
 template <typename T>
 hipError_t dot_t(size_t n, const T* a, const T* b, RedSlot& r, hipStream_t s);
-// y += (*a) x with a device scalar
+// y += (*a) x with a device scalar; with gate, only where *gate != 0 when the kernel runs
 template <typename T>
-hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s);
+hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s,
+                    const double* gate = nullptr);
 // out = sum of r's partials (one block)
 hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s);
 // reliable-update PCG: x += xs + (*alpha) p, xs = 0; and r = (float) r64 with partials of |r64|^2

@@ -3063,9 +3063,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__
 
 template <typename T>
 __device__ __forceinline__ void axpy_dev_body(const BlkId blk, size_t n, const double* a,
-                                              const T* x_, T* y_) {
+                                              const T* x_, T* y_, const double* gate) {
   const auto x = glob(x_);
   const auto y = glob(y_);
+  if (gate && *glob(gate) == 0.0) return;
   const T av = (T)*glob(a);
   for (size_t i = (size_t)blk.id * blockDim.x + threadIdx.x; i < n;
        i += (size_t)blk.n * blockDim.x)
@@ -3074,17 +3075,21 @@ __device__ __forceinline__ void axpy_dev_body(const BlkId blk, size_t n, const d
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy_dev(size_t n, const double* __restrict__ a,
-                                                     const T* __restrict__ x, T* __restrict__ y) {
-  axpy_dev_body<T>(hw_blk(), n, a, x, y);
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     const double* __restrict__ gate) {
+  axpy_dev_body<T>(hw_blk(), n, a, x, y, gate);
 }
 
 template <typename T>
-hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s) {
-  TMX_LAUNCH_OR_RECORD(kBlock, (axpy_dev_body<T>), vec_grid(n), 0, s, (k_axpy_dev<T>), n, a, x, y);
+hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s,
+                    const double* gate) {
+  TMX_LAUNCH_OR_RECORD(kBlock, (axpy_dev_body<T>), vec_grid(n), 0, s, (k_axpy_dev<T>), n, a, x, y, gate);
   return hipGetLastError();
 }
-template hipError_t axpy_dev<double>(size_t, const double*, const double*, double*, hipStream_t);
-template hipError_t axpy_dev<float>(size_t, const double*, const float*, float*, hipStream_t);
+template hipError_t axpy_dev<double>(size_t, const double*, const double*, double*, hipStream_t,
+                                     const double*);
+template hipError_t axpy_dev<float>(size_t, const double*, const float*, float*, hipStream_t,
+                                    const double*);
 
 hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s) {
   TMX_LAUNCH_OR_RECORD(kBlock, (reduce_to_body), 1, 0, s, (k_reduce_to), r.partials, r.nblocks, out);

@@ -235,8 +235,11 @@ class Hierarchy {
   int pcg_enqueue_begin(Workspace& ws, const T* b, double rtol, bool init, hipStream_t s);
   template <typename T, typename PC = T>
   int pcg_enqueue_iteration(Workspace& ws, int parity, hipStream_t s);
+  // the deferred x += alpha p of iteration `launched`; gated: only if that iteration has set
+  // the stop flag by the time the update runs (a no-op otherwise: the next iteration's
+  // direction update applies it)
   template <typename T>
-  int pcg_enqueue_end(Workspace& ws, int launched, hipStream_t s);
+  int pcg_enqueue_end(Workspace& ws, int launched, hipStream_t s, bool gated = false);
   // Reliable-update PCG on A x = b from x0 = 0 (van der Vorst and Ye 2000, residual
   // replacement): Krylov vectors, operator and V-cycle in FP32, the iterate accumulated in
   // FP64 (x64_), and the recursive FP32 residual replaced by the true FP64 one b - A x64
@@ -332,6 +335,9 @@ class Hierarchy {
   int coarse_first_ = 0;
   // build the coarse operators on the side stream (TMX_OPT_SETUP_OVERLAP)
   bool overlap_setup_ = true;
+  // pcg(): queue the awaited iteration's x update gated on its stop flag when nothing is
+  // queued behind it
+  bool gate_end_ = true;
   // In-step timing of the PCG direction update + operator (apply_update): with au_timing_ on,
   // each launch inside a PCG iteration is bracketed by a pair of events on the solve's stream
   // (up to kAuEvents launches between two reads; direct launches only, not under graphs)

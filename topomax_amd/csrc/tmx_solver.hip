@@ -986,9 +986,10 @@ int Hierarchy::pcg_enqueue_iteration(Workspace& ws, int parity, hipStream_t s) {
 }
 
 template <typename T>
-int Hierarchy::pcg_enqueue_end(Workspace& ws, int launched, hipStream_t s) {
+int Hierarchy::pcg_enqueue_end(Workspace& ws, int launched, hipStream_t s, bool gated) {
   T* p_last = ((launched - 1) & 1) ? KrylovBufs<T>::p(*this) : KrylovBufs<T>::pb(*this);
-  TMX_HIP(axpy_dev<T>(n(), ws.sc() + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s));
+  TMX_HIP(axpy_dev<T>(n(), ws.sc() + kPcg.alpha, p_last, KrylovBufs<T>::x(*this), s,
+                      gated ? ws.sc() + kPcg.flag : nullptr));
   return 0;
 }
 
@@ -1049,6 +1050,11 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   double flag = 0.0, rr_last = 0.0, ratio = 1.0;
   bool near = false;
   int launched = 0, done = 0;
+  // The iteration whose x update was queued gated on its stop flag: when nothing is queued
+  // behind the awaited iteration (convergence predicted, or the iteration limit), its update
+  // is queued before the wait, so that a converged solve's end does not wait for the host's
+  // round trip; unconverged, it changes nothing and the next iteration applies the update.
+  int gated = 0;
   if (max_iter >= 1) {
     TMX_HIP(launch(1));
     launched = 1;
@@ -1057,6 +1063,10 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
     if (launched == done + 1 && launched < max_iter && !near) {
       TMX_HIP(launch(launched + 1));
       ++launched;
+    }
+    if (launched == done + 1 && gate_end_) {
+      TMX_TRY(pcg_enqueue_end<T>(ws, launched, s, true));
+      gated = launched;
     }
     TMX_TRY(wait_iter(done + 1));
     ++done;
@@ -1078,7 +1088,9 @@ int Hierarchy::pcg(Workspace& ws, const T* b, double rtol, int max_iter, double*
   }
   // the last iteration's deferred x += alpha p (alpha is 0 once stopped), queued before the
   // host waits for the last queued iteration's report, so the device does not idle meanwhile
-  if (launched > 0) TMX_TRY(pcg_enqueue_end<T>(ws, launched, s));
+  // (already queued when the gated update of the last iteration found its stop flag set)
+  if (launched > 0 && !(gated == launched && done == launched && flag != 0.0))
+    TMX_TRY(pcg_enqueue_end<T>(ws, launched, s));
   if (launched > done) TMX_TRY(wait_iter(launched));
   if (launched == 0) {  // no iteration reported |b|^2: read it
     TMX_TRY(ws.read(s, kScBB, 1));
@@ -1629,7 +1641,7 @@ template int Hierarchy::pcg_enqueue_begin<double, float>(Workspace&, const doubl
                                                          hipStream_t);
 template int Hierarchy::pcg_enqueue_iteration<double, double>(Workspace&, int, hipStream_t);
 template int Hierarchy::pcg_enqueue_iteration<double, float>(Workspace&, int, hipStream_t);
-template int Hierarchy::pcg_enqueue_end<double>(Workspace&, int, hipStream_t);
+template int Hierarchy::pcg_enqueue_end<double>(Workspace&, int, hipStream_t, bool);
 template int Hierarchy::apply<double>(const double*, double*, RedSlot*, hipStream_t);
 template int Hierarchy::apply_update<double>(const double*, const double*, double*, double*,
                                              const double*, RedSlot*, hipStream_t, const float*,
