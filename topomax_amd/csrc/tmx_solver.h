@@ -372,6 +372,7 @@ class Hierarchy {
   bool has_operator_ = false;
   // Krylov vectors
   DevBuf<double> x64_, r64_, z64_, p64_, q64_, pb64_;
+  double* xit_ = nullptr;  // the FP64 PCG iterate's buffer while solve() iterates in its output
   DevBuf<float> r32_, z32_, p32_, q32_, d32_, pb32_;
   DevBuf<double> rr64_;         // refinement residual
   DevBuf<double> mrw_[3];       // MINRES direction vectors (elasticity only)

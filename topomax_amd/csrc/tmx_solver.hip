@@ -866,7 +866,7 @@ template <typename T>
 struct KrylovBufs;
 template <>
 struct KrylovBufs<double> {
-  static double* x(Hierarchy& h) { return h.x64_.get(); }
+  static double* x(Hierarchy& h) { return h.xit_ ? h.xit_ : h.x64_.get(); }
   static double* r(Hierarchy& h) { return h.r64_.get(); }
   static double* z(Hierarchy& h) { return h.z64_.get(); }
   static double* p(Hierarchy& h) { return h.p64_.get(); }
@@ -1456,11 +1456,18 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
   int rc = 0;
   if (prm.precision == kSolveFP64 || prm.precision == kSolveMixed ||
       prm.precision == kSolveMixedRU) {
+    // FP64 and mixed PCG iterate in the caller's x itself (no copy at the end), unless
+    // captured graphs hold x64_ or x is b; MINRES and the reliable-update PCG use x64_
+    bool in_x = !use_graphs_ && x != b && prm.precision != kSolveMixedRU;
     auto krylov = [&](int* its, double* relv, double* bbv) {
-      return prm.precision == kSolveFP64 ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s)
-             : prm.precision == kSolveMixed
-                 ? pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s)
-                 : pcg_ru(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s);
+      xit_ = in_x ? x : nullptr;
+      const int r = prm.precision == kSolveFP64
+                        ? pcg<double>(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s)
+                    : prm.precision == kSolveMixed
+                        ? pcg<double, float>(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s)
+                        : pcg_ru(ws, b, prm.rtol, prm.max_iter, bbv, its, relv, s);
+      xit_ = nullptr;
+      return r;
     };
     const bool defer = !prm.sync;
     if (defer) {
@@ -1496,9 +1503,10 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
       rc = minres(ws, b, prm.rtol, std::max(1, prm.max_iter - it), bb, &it2, &rel, s);
       S.iterations += it2;
       S.outer = 2;
+      in_x = false;
     }
     if (rc == 0 || rc == -3) {
-      TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
+      if (!in_x) TMX_HIP(copy_vec<double>(nv, x64_.get(), x, s));
       S.rel_residual = rel;
       if (prm.precision == kSolveMixedRU && rc == 0 && bb > 0.0) {
         // the FP32 recursion stops on its own residual, which drifts from the true one after
