@@ -161,29 +161,42 @@ def instep_kernel_us(problem, rho, steps, precision, rtol, kernel=ROOFLINE_KERNE
     return total.value / max(1, count.value), count.value
 
 
-def p2_throughput(N: int, steps: int, rtol: float):
+def p2_throughput(N: int, steps: int, rtol: float, mixed: bool = True):
     """The reference's own element (P2 displacement, FEM_src/elasisity_problem.py:194-196) on
     the same cantilever at N: FP64 MINRES preconditioned by P2 block Jacobi over the P1
-    multigrid, one objective + gradient per step."""
+    multigrid, one objective + gradient per step; with mixed, also the same solve with the
+    preconditioner's P1 V-cycle in FP32 (precision "mixed")."""
     import torch
     from topomax_amd.workloads import CANTILEVER, problem_from_case, smoothed_density
     nx, ny = 3 * N, N
-    gp = problem_from_case(CANTILEVER, nx, ny, precision="fp64", rtol=rtol, degree=2)
-    gp.set_penalization(3)
     rho = torch.as_tensor(smoothed_density(nx, ny, seed=0, sigma=5.0), device="cuda:0")
-    gp.objective_and_gradient(rho)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        J, _, _ = gp.objective_and_gradient(rho)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+
+    def run(precision):
+        gp = problem_from_case(CANTILEVER, nx, ny, precision=precision, rtol=rtol, degree=2)
+        gp.set_penalization(3)
+        gp.objective_and_gradient(rho)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            J, w, _ = gp.objective_and_gradient(rho)
+        torch.cuda.synchronize()
+        return gp, J, w, (time.perf_counter() - t0) / steps
+
+    gp, J, w, dt = run("fp64")
+    mixed_part = None
+    if mixed:
+        gm, Jm, wm, dtm = run("mixed")
+        mixed_part = {"ms_per_step": dtm * 1e3, "iterations": gm.last_stats["iterations"],
+                      "compliance_rel_err_vs_fp64": abs(Jm - J) / abs(J),
+                      "gradient_rel_err_vs_fp64":
+                          float(torch.linalg.norm(wm - w) / torch.linalg.norm(w)),
+                      "preconditioner": "P2 sweeps FP64, P1 V-cycle FP32"}
     ndof = 2 * (2 * nx + 1) * (2 * ny + 1)
     st = gp.last_stats
     return {"workload": f"cantilever 3x1, N={N}, P2 displacement ({ndof} DOFs), p=3, FP64, "
                         f"rtol {rtol:g}", "dofs": ndof, "ms_per_step": dt * 1e3,
             "GDOF_s": ndof / dt / 1e9, "iterations": st["iterations"],
-            "rel_residual": st["rel_residual"], "compliance": J}
+            "rel_residual": st["rel_residual"], "compliance": J, "mixed": mixed_part}
 
 
 def cpu_baseline(n_cpu: int):

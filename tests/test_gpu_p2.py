@@ -65,27 +65,32 @@ def test_p2_matvec(p):
     assert rel(y, y_ref) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("p", [3, 5])
-def test_p2_objective_and_gradient(p):
+def test_p2_objective_and_gradient(p, precision):
+    """FP64 MINRES to 3e-11 against the oracle's direct P2 solve, with the preconditioner's P1
+    V-cycle in FP64 or (mixed) FP32: the same 1e-8 bar either way."""
     nx, ny = 60, 20
     orc = oracle_problem(CANTILEVER, nx, ny, degree=2)
-    gp = gpu_problem(CANTILEVER, nx, ny, rtol=3e-11, degree=2)
+    gp = gpu_problem(CANTILEVER, nx, ny, rtol=3e-11, degree=2, precision=precision)
     orc.set_penalization(p)
     gp.set_penalization(p)
     rho = smoothed_density(nx, ny, seed=7, sigma=3.0)
     J_ref = orc.calculate_objective(rho)
     w_ref = orc.calculate_objective_gradient()
     J, w, mw = gp.objective_and_gradient(cuda(rho))
-    print("p", p, J, J_ref, rel(host(gp.u), orc.u), rel(host(w), w_ref), gp.last_stats, flush=True)
+    print(precision, "p", p, J, J_ref, rel(host(gp.u), orc.u), rel(host(w), w_ref), gp.last_stats,
+          flush=True)
     assert abs(J - J_ref) < 1e-8 * abs(J_ref)
     assert rel(host(gp.u), orc.u) < 1e-8
     assert rel(host(w), w_ref) < 1e-8
 
 
-def test_p2_indefinite_negative_density():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_p2_indefinite_negative_density(precision):
     nx, ny = 30, 10
     orc = oracle_problem(CANTILEVER, nx, ny, degree=2)
-    gp = gpu_problem(CANTILEVER, nx, ny, rtol=1e-11, degree=2)
+    gp = gpu_problem(CANTILEVER, nx, ny, rtol=1e-11, degree=2, precision=precision)
     orc.set_penalization(3)
     gp.set_penalization(3)
     X, Y = np.meshgrid(np.linspace(0, 3, nx + 1), np.linspace(0, 1, ny + 1))

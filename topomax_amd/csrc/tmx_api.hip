@@ -139,6 +139,8 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
     for (auto& v : h->p2v) TMX_TRY(v.alloc(2 * n2));
     TMX_TRY(h->r1c.alloc(2 * nv));
     TMX_TRY(h->e1c.alloc(2 * nv));
+    TMX_TRY(h->r1c32.alloc(2 * nv));
+    TMX_TRY(h->e1c32.alloc(2 * nv));
     TMX_TRY(h->u_s.alloc(2 * n2));
     TMX_HIP(p2_assemble_load(ld, h->F2.get(), s));
     TMX_HIP(copy_vec<double>(2 * n2, h->F2.get(), h->F2m.get(), s));
@@ -300,7 +302,9 @@ extern "C" int tmx_filter_solve(tmx_problem* h, const double* b, double* w) {
 // dips below zero) preconditioned by a symmetric two-level cycle: nu damped block-Jacobi
 // sweeps on the P2 operator of max(xi, 0), the P1 multigrid V-cycle on P1 (the P1 space of
 // the same mesh is a subspace of P2, and the P1 operator is the Galerkin restriction of the
-// P2 one), nu sweeps back.  FP64 throughout.
+// P2 one), nu sweeps back.  MINRES and the P2 sweeps run in FP64; the P1 V-cycle in FP64, or
+// in FP32 under TMX_MIXED (its input rounded to float, its correction widened back), as the
+// P1 mixed PCG runs it.
 static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolveParams& prm,
                     SolveStats* st) {
   hipStream_t s = h->stream;
@@ -308,6 +312,7 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
   const size_t n = 2 * g.nodes();
   constexpr int kNu = 2;
   constexpr double kOmega = 0.6;  // lambda_max(D^-1 K) of P2 block Jacobi is about 2.7
+  const bool vc32 = prm.precision == kSolveMixed;
   TMX_HIP(p2_weights(g, xi, h->p, h->m, false, h->W2.get(), s));
   TMX_HIP(p2_weights(g, xi, h->p, h->m, true, h->W2pc.get(), s));
   TMX_HIP(p2_diag(g, h->W2pc.get(), h->D2.get(), s));
@@ -331,7 +336,14 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
     }
     TMX_HIP(p2_residual(g, h->W2pc.get(), cur, r, nxt, s));
     TMX_HIP(p2_restrict(g, nxt, h->r1c.get(), s));
-    TMX_TRY(h->el.vcycle<double>(0, h->r1c.get(), h->e1c.get(), s));
+    if (vc32) {
+      const size_t n1 = 2 * nverts(h);
+      TMX_HIP((convert<double, float>(n1, h->r1c.get(), h->r1c32.get(), s)));
+      TMX_TRY(h->el.vcycle<float>(0, h->r1c32.get(), h->e1c32.get(), s));
+      TMX_HIP((convert<float, double>(n1, h->e1c32.get(), h->e1c.get(), s)));
+    } else {
+      TMX_TRY(h->el.vcycle<double>(0, h->r1c.get(), h->e1c.get(), s));
+    }
     TMX_HIP(p2_prolong_add(g, h->e1c.get(), cur, s));
     for (int k = 0; k < kNu; ++k) {
       double* out = k == kNu - 1 ? z : nxt;
