@@ -190,7 +190,7 @@ def p2_throughput(N: int, steps: int, rtol: float, mixed: bool = True):
                       "compliance_rel_err_vs_fp64": abs(Jm - J) / abs(J),
                       "gradient_rel_err_vs_fp64":
                           float(torch.linalg.norm(wm - w) / torch.linalg.norm(w)),
-                      "preconditioner": "P2 sweeps FP64, P1 V-cycle FP32"}
+                      "preconditioner": "two-level cycle in FP32 (P2 sweeps, transfers, P1 V-cycle)"}
     ndof = 2 * (2 * nx + 1) * (2 * ny + 1)
     st = gp.last_stats
     return {"workload": f"cantilever 3x1, N={N}, P2 displacement ({ndof} DOFs), p=3, FP64, "

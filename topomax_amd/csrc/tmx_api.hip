@@ -141,6 +141,9 @@ static int create_impl(const tmx_problem_desc* d, tmx_problem* h) {
     TMX_TRY(h->e1c.alloc(2 * nv));
     TMX_TRY(h->r1c32.alloc(2 * nv));
     TMX_TRY(h->e1c32.alloc(2 * nv));
+    TMX_TRY(h->W2pc32.alloc(12 * ncell));
+    TMX_TRY(h->D2_32.alloc(3 * n2));
+    for (auto& v : h->p2v32) TMX_TRY(v.alloc(2 * n2));
     TMX_TRY(h->u_s.alloc(2 * n2));
     TMX_HIP(p2_assemble_load(ld, h->F2.get(), s));
     TMX_HIP(copy_vec<double>(2 * n2, h->F2.get(), h->F2m.get(), s));
@@ -302,9 +305,10 @@ extern "C" int tmx_filter_solve(tmx_problem* h, const double* b, double* w) {
 // dips below zero) preconditioned by a symmetric two-level cycle: nu damped block-Jacobi
 // sweeps on the P2 operator of max(xi, 0), the P1 multigrid V-cycle on P1 (the P1 space of
 // the same mesh is a subspace of P2, and the P1 operator is the Galerkin restriction of the
-// P2 one), nu sweeps back.  MINRES and the P2 sweeps run in FP64; the P1 V-cycle in FP64, or
-// in FP32 under TMX_MIXED (its input rounded to float, its correction widened back), as the
-// P1 mixed PCG runs it.
+// P2 one), nu sweeps back.  MINRES runs in FP64; the cycle in FP64, or under TMX_MIXED in FP32
+// (the residual rounded to float on entry, the result widened on exit: weights, diagonal
+// blocks, sweeps, transfers and the P1 V-cycle all in float, as the P1 mixed PCG runs its
+// V-cycle).
 static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolveParams& prm,
                     SolveStats* st) {
   hipStream_t s = h->stream;
@@ -316,6 +320,11 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
   TMX_HIP(p2_weights(g, xi, h->p, h->m, false, h->W2.get(), s));
   TMX_HIP(p2_weights(g, xi, h->p, h->m, true, h->W2pc.get(), s));
   TMX_HIP(p2_diag(g, h->W2pc.get(), h->D2.get(), s));
+  if (vc32) {
+    const size_t ncell = (size_t)g.nx * g.ny;
+    TMX_HIP((convert<double, float>(12 * ncell, h->W2pc.get(), h->W2pc32.get(), s)));
+    TMX_HIP((convert<double, float>(3 * g.nodes(), h->D2.get(), h->D2_32.get(), s)));
+  }
   DevBuf<double>* P = h->p2v;
   double* X = P[8].get();
   double* ta = P[9].get();
@@ -326,29 +335,38 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
     TMX_HIP(p2_apply(g, h->W2.get(), x, y, s));
     return 0;
   };
-  auto pc_fn = [&](const double* r, double* z, RedSlot* rz) -> int {
-    double* cur = ta;
-    double* nxt = tb;
-    TMX_HIP(p2_smooth(g, h->W2pc.get(), h->D2.get(), nullptr, r, cur, kOmega, s));
+  // z = C r for the two-level cycle C in precision T (buffers of that precision)
+  auto cycle = [&](auto tag, const auto* Wp, const auto* Dp, const auto* r, auto* z, auto* ca,
+                   auto* cb, auto* r1, auto* e1) -> int {
+    using T = decltype(tag);
+    T* cur = ca;
+    T* nxt = cb;
+    TMX_HIP(p2_smooth<T>(g, Wp, Dp, nullptr, r, cur, kOmega, s));
     for (int k = 1; k < kNu; ++k) {
-      TMX_HIP(p2_smooth(g, h->W2pc.get(), h->D2.get(), cur, r, nxt, kOmega, s));
+      TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, r, nxt, kOmega, s));
       std::swap(cur, nxt);
     }
-    TMX_HIP(p2_residual(g, h->W2pc.get(), cur, r, nxt, s));
-    TMX_HIP(p2_restrict(g, nxt, h->r1c.get(), s));
-    if (vc32) {
-      const size_t n1 = 2 * nverts(h);
-      TMX_HIP((convert<double, float>(n1, h->r1c.get(), h->r1c32.get(), s)));
-      TMX_TRY(h->el.vcycle<float>(0, h->r1c32.get(), h->e1c32.get(), s));
-      TMX_HIP((convert<float, double>(n1, h->e1c32.get(), h->e1c.get(), s)));
-    } else {
-      TMX_TRY(h->el.vcycle<double>(0, h->r1c.get(), h->e1c.get(), s));
-    }
-    TMX_HIP(p2_prolong_add(g, h->e1c.get(), cur, s));
+    TMX_HIP(p2_residual<T>(g, Wp, cur, r, nxt, s));
+    TMX_HIP(p2_restrict<T>(g, nxt, r1, s));
+    TMX_TRY(h->el.vcycle<T>(0, r1, e1, s));
+    TMX_HIP(p2_prolong_add<T>(g, e1, cur, s));
     for (int k = 0; k < kNu; ++k) {
-      double* out = k == kNu - 1 ? z : nxt;
-      TMX_HIP(p2_smooth(g, h->W2pc.get(), h->D2.get(), cur, r, out, kOmega, s));
+      T* out = k == kNu - 1 ? z : nxt;
+      TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, r, out, kOmega, s));
       std::swap(cur, nxt);
+    }
+    return 0;
+  };
+  auto pc_fn = [&](const double* r, double* z, RedSlot* rz) -> int {
+    if (vc32) {
+      float *r32 = h->p2v32[0].get(), *z32 = h->p2v32[1].get();
+      TMX_HIP((convert<double, float>(n, r, r32, s)));
+      TMX_TRY(cycle(float{}, h->W2pc32.get(), h->D2_32.get(), r32, z32, h->p2v32[2].get(),
+                    h->p2v32[3].get(), h->r1c32.get(), h->e1c32.get()));
+      TMX_HIP((convert<float, double>(n, z32, z, s)));
+    } else {
+      TMX_TRY(cycle(double{}, h->W2pc.get(), h->D2.get(), r, z, ta, tb, h->r1c.get(),
+                    h->e1c.get()));
     }
     TMX_HIP(dot_t<double>(n, r, z, *rz, s));
     return 0;

@@ -41,7 +41,9 @@ struct tmx_problem {
   int degree = 1;
   P2Geom g2{};
   DevBuf<double> W2, W2pc, D2, F2, F2m, p2v[11], r1c, e1c;
-  DevBuf<float> r1c32, e1c32;  // the P1 correction's vectors when its V-cycle runs in FP32
+  // the mixed scheme's FP32 preconditioner: the P1 correction's vectors, the P2 sweeps'
+  // weights and diagonal blocks, and their vectors (r, z and two iterates)
+  DevBuf<float> r1c32, e1c32, W2pc32, D2_32, p2v32[4];
   // Merged batches (tmx_batch.hip): `version` changes whenever an entry point may have changed
   // the handle's solver state, so a batch re-records its launches; merge_ready[e][f] is set
   // once an objective step with elasticity scheme e and filter scheme f (kSolveFP64 or

@@ -31,15 +31,19 @@ hipError_t p2_weights(const P2Geom& g, const double* xi, int p, double m, bool p
 hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s);
 // 2x2 diagonal blocks of K (3 per node: xx, xy, yy)
 hipError_t p2_diag(const P2Geom& g, const double* W, double* D, hipStream_t s);
-// one damped block-Jacobi sweep y = x + omega D^-1 (b - K x) (x = nullptr: zero); fixed nodes 0
-hipError_t p2_smooth(const P2Geom& g, const double* W, const double* D, const double* x,
-                     const double* b, double* y, double omega, hipStream_t s);
+// one damped block-Jacobi sweep y = x + omega D^-1 (b - K x) (x = nullptr: zero); fixed nodes
+// 0.  T = double, or float for the mixed scheme's preconditioner (W, D and vectors in float).
+template <typename T>
+hipError_t p2_smooth(const P2Geom& g, const T* W, const T* D, const T* x, const T* b, T* y,
+                     double omega, hipStream_t s);
 // r = b - K x
-hipError_t p2_residual(const P2Geom& g, const double* W, const double* x, const double* b,
-                       double* r, hipStream_t s);
+template <typename T>
+hipError_t p2_residual(const P2Geom& g, const T* W, const T* x, const T* b, T* r, hipStream_t s);
 // P1 <-> P2 transfers on the same mesh (P1 is a subspace of P2): r1 = P^T r2, x2 += P e1
-hipError_t p2_restrict(const P2Geom& g, const double* r2, double* r1, hipStream_t s);
-hipError_t p2_prolong_add(const P2Geom& g, const double* e1, double* x2, hipStream_t s);
+template <typename T>
+hipError_t p2_restrict(const P2Geom& g, const T* r2, T* r1, hipStream_t s);
+template <typename T>
+hipError_t p2_prolong_add(const P2Geom& g, const T* e1, T* x2, hipStream_t s);
 // load vector of the P2 test functions (body force and traction P2 interpolants, exact)
 hipError_t p2_assemble_load(const LoadDesc& d, double* F, hipStream_t s);
 // zero both components of every fixed lattice node

@@ -71,19 +71,21 @@ __device__ __forceinline__ void tri_nodes(int t, int cx, int cy, int (&I)[6], in
 }
 
 // barycentric gradients d_k of triangle t (units 1/hx, 1/hy applied)
-__device__ __forceinline__ void tri_grads(int t, double hx, double hy, double (&d)[3][2]) {
+template <typename T>
+__device__ __forceinline__ void tri_grads(int t, T hx, T hy, T (&d)[3][2]) {
   if (t == 0) {
-    d[0][0] = -1.0 / hx; d[0][1] = 0.0;
-    d[1][0] = 1.0 / hx;  d[1][1] = -1.0 / hy;
-    d[2][0] = 0.0;       d[2][1] = 1.0 / hy;
+    d[0][0] = T(-1) / hx; d[0][1] = T(0);
+    d[1][0] = T(1) / hx;  d[1][1] = T(-1) / hy;
+    d[2][0] = T(0);       d[2][1] = T(1) / hy;
   } else {
-    d[0][0] = 0.0;       d[0][1] = -1.0 / hy;
-    d[1][0] = -1.0 / hx; d[1][1] = 1.0 / hy;
-    d[2][0] = 1.0 / hx;  d[2][1] = 0.0;
+    d[0][0] = T(0);       d[0][1] = T(-1) / hy;
+    d[1][0] = T(-1) / hx; d[1][1] = T(1) / hy;
+    d[2][0] = T(1) / hx;  d[2][1] = T(0);
   }
 }
 
-__device__ __forceinline__ double wlk(const double (&W)[6], int l, int k) {
+template <typename T>
+__device__ __forceinline__ T wlk(const T (&W)[6], int l, int k) {
   const int a = l < k ? l : k, b = l < k ? k : l;
   // packed 00 01 02 11 12 22
   const int idx = a == 0 ? b : (a == 1 ? 2 + b : 5);
@@ -91,10 +93,10 @@ __device__ __forceinline__ double wlk(const double (&W)[6], int l, int k) {
 }
 
 // strain-stress coefficients S_l (xx, xy, yy) of sigma(u) = sum_l l_l S_l, and eps_l
-__device__ __forceinline__ void stress_coeffs(const double (&d)[3][2], const double (&u)[6][2],
-                                              double lam, double mu, double (&S)[3][3],
-                                              double (&E)[3][3]) {
-  double U0[2][2] = {{0, 0}, {0, 0}};
+template <typename T>
+__device__ __forceinline__ void stress_coeffs(const T (&d)[3][2], const T (&u)[6][2], T lam, T mu,
+                                              T (&S)[3][3], T (&E)[3][3]) {
+  T U0[2][2] = {{0, 0}, {0, 0}};
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -103,11 +105,11 @@ __device__ __forceinline__ void stress_coeffs(const double (&d)[3][2], const dou
       for (int j = 0; j < 2; ++j) U0[c][j] += u[k][c] * d[k][j];
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
-    double G[2][2];
+    T G[2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) G[c][j] = 4.0 * u[l][c] * d[l][j] - U0[c][j];
+      for (int j = 0; j < 2; ++j) G[c][j] = T(4) * u[l][c] * d[l][j] - U0[c][j];
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj) {
       if (jj == l) continue;
@@ -115,16 +117,16 @@ __device__ __forceinline__ void stress_coeffs(const double (&d)[3][2], const dou
 #pragma unroll
       for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) G[c][j] += 4.0 * u[m][c] * d[jj][j];
+        for (int j = 0; j < 2; ++j) G[c][j] += T(4) * u[m][c] * d[jj][j];
     }
-    const double exx = G[0][0], eyy = G[1][1], exy = 0.5 * (G[0][1] + G[1][0]);
-    const double tr = exx + eyy;
+    const T exx = G[0][0], eyy = G[1][1], exy = T(0.5) * (G[0][1] + G[1][0]);
+    const T tr = exx + eyy;
     E[l][0] = exx;
     E[l][1] = exy;
     E[l][2] = eyy;
-    S[l][0] = lam * tr + 2.0 * mu * exx;
-    S[l][1] = 2.0 * mu * exy;
-    S[l][2] = lam * tr + 2.0 * mu * eyy;
+    S[l][0] = lam * tr + T(2) * mu * exx;
+    S[l][1] = T(2) * mu * exy;
+    S[l][2] = lam * tr + T(2) * mu * eyy;
   }
 }
 
@@ -167,18 +169,18 @@ __device__ __forceinline__ void tri_force(const double (&W)[6], const double (&d
 }
 
 // forces of one triangle on all six local nodes (the stress coefficients formed once)
-__device__ __forceinline__ void tri_forces_all(const double (&W)[6], const double (&d)[3][2],
-                                               const double (&u)[6][2], double lam, double mu,
-                                               double (&y)[6][2]) {
-  double S[3][3], E[3][3];
+template <typename R>
+__device__ __forceinline__ void tri_forces_all(const R (&W)[6], const R (&d)[3][2],
+                                               const R (&u)[6][2], R lam, R mu, R (&y)[6][2]) {
+  R S[3][3], E[3][3];
   stress_coeffs(d, u, lam, mu, S, E);
-  double T[3][3];
+  R T[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
     for (int e = 0; e < 3; ++e)
       T[k][e] = wlk(W, 0, k) * S[0][e] + wlk(W, 1, k) * S[1][e] + wlk(W, 2, k) * S[2][e];
-  double tv[3][3][2];  // T_k d_j
+  R tv[3][3][2];  // T_k d_j
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -188,10 +190,10 @@ __device__ __forceinline__ void tri_forces_all(const double (&W)[6], const doubl
     }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    double sx = 0.0, sy = 0.0;
+    R sx = R(0), sy = R(0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double c = k == a ? 3.0 : -1.0;  // 4 delta - 1
+      const R c = k == a ? R(3) : R(-1);  // 4 delta - 1
       sx += c * tv[k][a][0];
       sy += c * tv[k][a][1];
     }
@@ -201,8 +203,8 @@ __device__ __forceinline__ void tri_forces_all(const double (&W)[6], const doubl
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int i = (k + 1) % 3, j = (k + 2) % 3;
-    y[3 + k][0] = 4.0 * (tv[i][j][0] + tv[j][i][0]);
-    y[3 + k][1] = 4.0 * (tv[i][j][1] + tv[j][i][1]);
+    y[3 + k][0] = R(4) * (tv[i][j][0] + tv[j][i][0]);
+    y[3 + k][1] = R(4) * (tv[i][j][1] + tv[j][i][1]);
   }
 }
 
@@ -211,9 +213,10 @@ __device__ __forceinline__ bool p2_fixed(unsigned fixed, int I, int J, int nX, i
          ((fixed & kBottom) && J == 0) || ((fixed & kTop) && J == nY - 1);
 }
 
-__device__ __forceinline__ void load_W(const double* __restrict__ W, int nx, int cx, int cy, int t,
-                                       double (&w)[6]) {
-  const double* src = W + 12 * ((size_t)cy * nx + cx) + 6 * t;
+template <typename T>
+__device__ __forceinline__ void load_W(const T* __restrict__ W, int nx, int cx, int cy, int t,
+                                       T (&w)[6]) {
+  const T* src = W + 12 * ((size_t)cy * nx + cx) + 6 * t;
 #pragma unroll
   for (int e = 0; e < 6; ++e) w[e] = src[e];
 }
@@ -274,19 +277,21 @@ __global__ void k_p2_weights(P2Geom g, const double* __restrict__ xi, int p, dou
 // instead would form each triangle's stresses once for each of its six nodes.
 //   MODE 0: y = K u (fixed nodes: y = u);  1: y = b - K u (fixed: b - u);
 //   2: y = x + omega D^-1 (b - K x) (x null: from zero; fixed nodes 0).
+// T: FP64, or FP32 for the mixed scheme's preconditioner (weights, diagonal blocks and vectors
+// in float, the diagonal blocks inverted by the hardware reciprocal).
 constexpr int kP2TileC = 8;                  // owned cells per side
 constexpr int kP2TileThreads = 256;
 constexpr int kP2ExtC = kP2TileC + 1;        // cells whose triangles reach the owned nodes
 constexpr int kP2ExtN = 2 * kP2TileC + 3;    // lattice nodes of those cells per side
 
-template <int MODE>
-__global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const double* __restrict__ W,
-                                                            const double* __restrict__ D,
-                                                            const double* __restrict__ x,
-                                                            const double* __restrict__ b,
-                                                            double* __restrict__ y, double omega) {
-  __shared__ double us[kP2ExtN * kP2ExtN][2];
-  __shared__ double fs[kP2ExtC * kP2ExtC][2][6][2];
+template <int MODE, typename T>
+__global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* __restrict__ W,
+                                                            const T* __restrict__ D,
+                                                            const T* __restrict__ x,
+                                                            const T* __restrict__ b,
+                                                            T* __restrict__ y, T omega) {
+  __shared__ T us[kP2ExtN * kP2ExtN][2];
+  __shared__ T fs[kP2ExtC * kP2ExtC][2][6][2];
   const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
   const int ntx = (g.nx + kP2TileC - 1) / kP2TileC;
   const int X0 = (int)(blockIdx.x % (unsigned)ntx) * kP2TileC;
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const doub
     for (int k = (int)threadIdx.x; k < kP2ExtN * kP2ExtN; k += kP2TileThreads) {
       const int lj = k / kP2ExtN, li = k - lj * kP2ExtN;
       const int I = LI0 + li, J = LJ0 + lj;
-      double u0 = 0.0, u1 = 0.0;
+      T u0 = T(0), u1 = T(0);
       if (I >= 0 && J >= 0 && I < nX && J < nY && !p2_fixed(g.fixed, I, J, nX, nY)) {
         const size_t n = (size_t)J * nX + I;
         u0 = x[2 * n];
@@ -312,9 +317,9 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const doub
       const int lcy = c / kP2ExtC, lcx = c - lcy * kP2ExtC;
       const int cx = X0 - 1 + lcx, cy = Y0 - 1 + lcy;
       if (cx < 0 || cy < 0 || cx >= g.nx || cy >= g.ny) continue;  // (no owned node reads it)
-      double w[6], d[3][2], ul[6][2], f[6][2];
+      T w[6], d[3][2], ul[6][2], f[6][2];
       load_W(W, g.nx, cx, cy, t, w);
-      tri_grads(t, g.hx, g.hy, d);
+      tri_grads(t, (T)g.hx, (T)g.hy, d);
       int NI[6], NJ[6];
       tri_nodes(t, cx, cy, NI, NJ);
 #pragma unroll
@@ -323,7 +328,7 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const doub
         ul[q][0] = us[li][0];
         ul[q][1] = us[li][1];
       }
-      tri_forces_all(w, d, ul, g.lam, g.mu, f);
+      tri_forces_all(w, d, ul, (T)g.lam, (T)g.mu, f);
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         fs[c][t][q][0] = f[q][0];
@@ -342,7 +347,7 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const doub
     const int I = 2 * X0 + oi, J = 2 * Y0 + oj;
     const size_t n = (size_t)J * nX + I;
     const bool fx = p2_fixed(g.fixed, I, J, nX, nY);
-    double f0 = 0.0, f1 = 0.0;
+    T f0 = T(0), f1 = T(0);
     if (forces && !fx) {
       const int cx0 = (I & 1) ? (I - 1) / 2 : I / 2 - 1, cx1 = (I & 1) ? (I - 1) / 2 : I / 2;
       const int cy0 = (J & 1) ? (J - 1) / 2 : J / 2 - 1, cy1 = (J & 1) ? (J - 1) / 2 : J / 2;
@@ -373,12 +378,12 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const doub
       y[2 * n + 1] = b[2 * n + 1] - (fx ? x[2 * n + 1] : f1);
     } else {
       if (fx) {
-        y[2 * n] = y[2 * n + 1] = 0.0;
+        y[2 * n] = y[2 * n + 1] = T(0);
       } else {
-        const double rx = b[2 * n] - f0, ry = b[2 * n + 1] - f1;
-        const double a = D[3 * n], c = D[3 * n + 1], e = D[3 * n + 2];
-        const double inv = 1.0 / (a * e - c * c);
-        const double x0 = x ? x[2 * n] : 0.0, x1 = x ? x[2 * n + 1] : 0.0;
+        const T rx = b[2 * n] - f0, ry = b[2 * n + 1] - f1;
+        const T a = D[3 * n], c = D[3 * n + 1], e = D[3 * n + 2];
+        const T inv = recip(a * e - c * c);
+        const T x0 = x ? x[2 * n] : T(0), x1 = x ? x[2 * n + 1] : T(0);
         y[2 * n] = x0 + omega * (e * rx - c * ry) * inv;
         y[2 * n + 1] = x1 + omega * (a * ry - c * rx) * inv;
       }
@@ -416,12 +421,13 @@ __global__ void k_p2_diag(P2Geom g, const double* __restrict__ W, double* __rest
 }
 
 // r1 at P1 vertex (i, j) = r2(2i, 2j) + 1/2 (the six adjacent midpoints)
-__global__ void k_p2_restrict(P2Geom g, const double* __restrict__ r2, double* __restrict__ r1) {
+template <typename T>
+__global__ void k_p2_restrict(P2Geom g, const T* __restrict__ r2, T* __restrict__ r1) {
   const int nx = g.nx, ny = g.ny, nX = 2 * nx + 1;
   const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= (size_t)(nx + 1) * (ny + 1)) return;
   const int i = (int)(v % (nx + 1)), j = (int)(v / (nx + 1));
-  double sx = 0.0, sy = 0.0;
+  T sx = T(0), sy = T(0);
   if (!node_fixed(g.fixed, i, j, nx, ny)) {
     const int I = 2 * i, J = 2 * j;
     sx = r2[2 * ((size_t)J * nX + I)];
@@ -431,15 +437,16 @@ __global__ void k_p2_restrict(P2Geom g, const double* __restrict__ r2, double* _
       const int mI = I + off[q][0], mJ = J + off[q][1];
       if (mI < 0 || mJ < 0 || mI > 2 * nx || mJ > 2 * ny) continue;
       const size_t m = (size_t)mJ * nX + mI;
-      sx += 0.5 * r2[2 * m];
-      sy += 0.5 * r2[2 * m + 1];
+      sx += T(0.5) * r2[2 * m];
+      sy += T(0.5) * r2[2 * m + 1];
     }
   }
   r1[2 * v] = sx;
   r1[2 * v + 1] = sy;
 }
 
-__global__ void k_p2_prolong_add(P2Geom g, const double* __restrict__ e1, double* __restrict__ x2) {
+template <typename T>
+__global__ void k_p2_prolong_add(P2Geom g, const T* __restrict__ e1, T* __restrict__ x2) {
   const int nx = g.nx, nX = 2 * nx + 1, nY = 2 * g.ny + 1;
   const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (size_t)nX * nY) return;
@@ -449,8 +456,8 @@ __global__ void k_p2_prolong_add(P2Geom g, const double* __restrict__ e1, double
   const int i0 = I / 2, j0 = J / 2;
   const int i1 = (I + 1) / 2, j1 = (J + 1) / 2;
   const size_t a = (size_t)j0 * (nx + 1) + i0, b = (size_t)j1 * (nx + 1) + i1;
-  x2[2 * n] += 0.5 * (e1[2 * a] + e1[2 * b]);
-  x2[2 * n + 1] += 0.5 * (e1[2 * a + 1] + e1[2 * b + 1]);
+  x2[2 * n] += T(0.5) * (e1[2 * a] + e1[2 * b]);
+  x2[2 * n + 1] += T(0.5) * (e1[2 * a + 1] + e1[2 * b + 1]);
 }
 
 // sensitivity at P1 vertex v: sum over its triangles of
@@ -600,38 +607,53 @@ static unsigned p2_tiles(const P2Geom& g) {
 
 hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s) {
   if (!u || !y) return hipErrorInvalidValue;
-  k_p2_tile<0><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, u, nullptr, y, 0.0);
+  k_p2_tile<0, double><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, u, nullptr, y, 0.0);
   return hipGetLastError();
 }
 
-hipError_t p2_residual(const P2Geom& g, const double* W, const double* x, const double* b,
-                       double* r, hipStream_t s) {
+template <typename T>
+hipError_t p2_residual(const P2Geom& g, const T* W, const T* x, const T* b, T* r, hipStream_t s) {
   if (!x || !b || !r) return hipErrorInvalidValue;
-  k_p2_tile<1><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, x, b, r, 0.0);
+  k_p2_tile<1, T><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, x, b, r, T(0));
   return hipGetLastError();
 }
+template hipError_t p2_residual<double>(const P2Geom&, const double*, const double*,
+                                        const double*, double*, hipStream_t);
+template hipError_t p2_residual<float>(const P2Geom&, const float*, const float*, const float*,
+                                       float*, hipStream_t);
 
 hipError_t p2_diag(const P2Geom& g, const double* W, double* D, hipStream_t s) {
   k_p2_diag<<<grid_for(g.nodes()), 256, 0, s>>>(g, W, D);
   return hipGetLastError();
 }
 
-hipError_t p2_smooth(const P2Geom& g, const double* W, const double* D, const double* x,
-                     const double* b, double* y, double omega, hipStream_t s) {
+template <typename T>
+hipError_t p2_smooth(const P2Geom& g, const T* W, const T* D, const T* x, const T* b, T* y,
+                     double omega, hipStream_t s) {
   if (!D || !b || !y) return hipErrorInvalidValue;
-  k_p2_tile<2><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, D, x, b, y, omega);
+  k_p2_tile<2, T><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, D, x, b, y, (T)omega);
   return hipGetLastError();
 }
+template hipError_t p2_smooth<double>(const P2Geom&, const double*, const double*, const double*,
+                                      const double*, double*, double, hipStream_t);
+template hipError_t p2_smooth<float>(const P2Geom&, const float*, const float*, const float*,
+                                     const float*, float*, double, hipStream_t);
 
-hipError_t p2_restrict(const P2Geom& g, const double* r2, double* r1, hipStream_t s) {
-  k_p2_restrict<<<grid_for((size_t)(g.nx + 1) * (g.ny + 1)), 256, 0, s>>>(g, r2, r1);
+template <typename T>
+hipError_t p2_restrict(const P2Geom& g, const T* r2, T* r1, hipStream_t s) {
+  k_p2_restrict<T><<<grid_for((size_t)(g.nx + 1) * (g.ny + 1)), 256, 0, s>>>(g, r2, r1);
   return hipGetLastError();
 }
+template hipError_t p2_restrict<double>(const P2Geom&, const double*, double*, hipStream_t);
+template hipError_t p2_restrict<float>(const P2Geom&, const float*, float*, hipStream_t);
 
-hipError_t p2_prolong_add(const P2Geom& g, const double* e1, double* x2, hipStream_t s) {
-  k_p2_prolong_add<<<grid_for(g.nodes()), 256, 0, s>>>(g, e1, x2);
+template <typename T>
+hipError_t p2_prolong_add(const P2Geom& g, const T* e1, T* x2, hipStream_t s) {
+  k_p2_prolong_add<T><<<grid_for(g.nodes()), 256, 0, s>>>(g, e1, x2);
   return hipGetLastError();
 }
+template hipError_t p2_prolong_add<double>(const P2Geom&, const double*, double*, hipStream_t);
+template hipError_t p2_prolong_add<float>(const P2Geom&, const float*, float*, hipStream_t);
 
 hipError_t p2_assemble_load(const LoadDesc& d, double* F, hipStream_t s) {
   k_p2_load<<<grid_for((size_t)(2 * d.nx + 1) * (2 * d.ny + 1)), 256, 0, s>>>(d, F);
