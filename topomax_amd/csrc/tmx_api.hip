@@ -335,39 +335,43 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
     TMX_HIP(p2_apply(g, h->W2.get(), x, y, s));
     return 0;
   };
-  // z = C r for the two-level cycle C in precision T (buffers of that precision)
-  auto cycle = [&](auto tag, const auto* Wp, const auto* Dp, const auto* r, auto* z, auto* ca,
-                   auto* cb, auto* r1, auto* e1) -> int {
+  // z = C r for the two-level cycle C in precision T (weights, diagonal blocks and vectors of
+  // that precision); the float cycle's first sweep reads the double r (writing r rounded to
+  // float for the later steps) and its last writes the double z
+  auto cycle = [&](auto tag, const auto* Wp, const auto* Dp, const double* r, double* z,
+                   auto* rt, auto* ca, auto* cb, auto* r1, auto* e1) -> int {
     using T = decltype(tag);
+    constexpr bool f32 = sizeof(T) == 4;
     T* cur = ca;
     T* nxt = cb;
-    TMX_HIP(p2_smooth<T>(g, Wp, Dp, nullptr, r, cur, kOmega, s));
+    if constexpr (f32) TMX_HIP(p2_smooth_first32(g, Wp, Dp, r, rt, cur, kOmega, s));
+    else TMX_HIP(p2_smooth<T>(g, Wp, Dp, nullptr, rt, cur, kOmega, s));
     for (int k = 1; k < kNu; ++k) {
-      TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, r, nxt, kOmega, s));
+      TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, rt, nxt, kOmega, s));
       std::swap(cur, nxt);
     }
-    TMX_HIP(p2_residual<T>(g, Wp, cur, r, nxt, s));
+    TMX_HIP(p2_residual<T>(g, Wp, cur, rt, nxt, s));
     TMX_HIP(p2_restrict<T>(g, nxt, r1, s));
     TMX_TRY(h->el.vcycle<T>(0, r1, e1, s));
     TMX_HIP(p2_prolong_add<T>(g, e1, cur, s));
     for (int k = 0; k < kNu; ++k) {
-      T* out = k == kNu - 1 ? z : nxt;
-      TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, r, out, kOmega, s));
-      std::swap(cur, nxt);
+      if (k == kNu - 1) {
+        if constexpr (f32) TMX_HIP(p2_smooth_last32(g, Wp, Dp, cur, rt, z, kOmega, s));
+        else TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, rt, z, kOmega, s));
+      } else {
+        TMX_HIP(p2_smooth<T>(g, Wp, Dp, cur, rt, nxt, kOmega, s));
+        std::swap(cur, nxt);
+      }
     }
     return 0;
   };
   auto pc_fn = [&](const double* r, double* z, RedSlot* rz) -> int {
-    if (vc32) {
-      float *r32 = h->p2v32[0].get(), *z32 = h->p2v32[1].get();
-      TMX_HIP((convert<double, float>(n, r, r32, s)));
-      TMX_TRY(cycle(float{}, h->W2pc32.get(), h->D2_32.get(), r32, z32, h->p2v32[2].get(),
-                    h->p2v32[3].get(), h->r1c32.get(), h->e1c32.get()));
-      TMX_HIP((convert<float, double>(n, z32, z, s)));
-    } else {
-      TMX_TRY(cycle(double{}, h->W2pc.get(), h->D2.get(), r, z, ta, tb, h->r1c.get(),
-                    h->e1c.get()));
-    }
+    if (vc32)
+      TMX_TRY(cycle(float{}, h->W2pc32.get(), h->D2_32.get(), r, z, h->p2v32[0].get(),
+                    h->p2v32[2].get(), h->p2v32[3].get(), h->r1c32.get(), h->e1c32.get()));
+    else
+      TMX_TRY(cycle(double{}, h->W2pc.get(), h->D2.get(), r, z, const_cast<double*>(r), ta, tb,
+                    h->r1c.get(), h->e1c.get()));
     TMX_HIP(dot_t<double>(n, r, z, *rz, s));
     return 0;
   };

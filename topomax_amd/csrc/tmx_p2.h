@@ -36,6 +36,12 @@ hipError_t p2_diag(const P2Geom& g, const double* W, double* D, hipStream_t s);
 template <typename T>
 hipError_t p2_smooth(const P2Geom& g, const T* W, const T* D, const T* x, const T* b, T* y,
                      double omega, hipStream_t s);
+// the float cycle's first sweep (from zero) reading a double b, which it also writes rounded
+// to b32, and its last sweep writing a double y
+hipError_t p2_smooth_first32(const P2Geom& g, const float* W, const float* D, const double* b,
+                             float* b32, float* y, double omega, hipStream_t s);
+hipError_t p2_smooth_last32(const P2Geom& g, const float* W, const float* D, const float* x,
+                            const float* b, double* y, double omega, hipStream_t s);
 // r = b - K x
 template <typename T>
 hipError_t p2_residual(const P2Geom& g, const T* W, const T* x, const T* b, T* r, hipStream_t s);

@@ -278,18 +278,21 @@ __global__ void k_p2_weights(P2Geom g, const double* __restrict__ xi, int p, dou
 //   MODE 0: y = K u (fixed nodes: y = u);  1: y = b - K u (fixed: b - u);
 //   2: y = x + omega D^-1 (b - K x) (x null: from zero; fixed nodes 0).
 // T: FP64, or FP32 for the mixed scheme's preconditioner (weights, diagonal blocks and vectors
-// in float, the diagonal blocks inverted by the hardware reciprocal).
+// in float, the diagonal blocks inverted by the hardware reciprocal).  TB / TY: the types of b
+// and y (a double right-hand side read, or a double result written, by the float cycle's
+// first and last sweeps); with bcopy, b rounded to T is also written there.
 constexpr int kP2TileC = 8;                  // owned cells per side
 constexpr int kP2TileThreads = 256;
 constexpr int kP2ExtC = kP2TileC + 1;        // cells whose triangles reach the owned nodes
 constexpr int kP2ExtN = 2 * kP2TileC + 3;    // lattice nodes of those cells per side
 
-template <int MODE, typename T>
+template <int MODE, typename T, typename TB = T, typename TY = T>
 __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* __restrict__ W,
                                                             const T* __restrict__ D,
                                                             const T* __restrict__ x,
-                                                            const T* __restrict__ b,
-                                                            T* __restrict__ y, T omega) {
+                                                            const TB* __restrict__ b,
+                                                            TY* __restrict__ y, T omega,
+                                                            T* __restrict__ bcopy = nullptr) {
   __shared__ T us[kP2ExtN * kP2ExtN][2];
   __shared__ T fs[kP2ExtC * kP2ExtC][2][6][2];
   const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
@@ -377,15 +380,20 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
       y[2 * n] = b[2 * n] - (fx ? x[2 * n] : f0);
       y[2 * n + 1] = b[2 * n + 1] - (fx ? x[2 * n + 1] : f1);
     } else {
+      const T b0 = (T)b[2 * n], b1 = (T)b[2 * n + 1];
+      if (bcopy) {
+        bcopy[2 * n] = b0;
+        bcopy[2 * n + 1] = b1;
+      }
       if (fx) {
-        y[2 * n] = y[2 * n + 1] = T(0);
+        y[2 * n] = y[2 * n + 1] = TY(0);
       } else {
-        const T rx = b[2 * n] - f0, ry = b[2 * n + 1] - f1;
+        const T rx = b0 - f0, ry = b1 - f1;
         const T a = D[3 * n], c = D[3 * n + 1], e = D[3 * n + 2];
         const T inv = recip(a * e - c * c);
         const T x0 = x ? x[2 * n] : T(0), x1 = x ? x[2 * n + 1] : T(0);
-        y[2 * n] = x0 + omega * (e * rx - c * ry) * inv;
-        y[2 * n + 1] = x1 + omega * (a * ry - c * rx) * inv;
+        y[2 * n] = (TY)(x0 + omega * (e * rx - c * ry) * inv);
+        y[2 * n + 1] = (TY)(x1 + omega * (a * ry - c * rx) * inv);
       }
     }
   }
@@ -607,7 +615,7 @@ static unsigned p2_tiles(const P2Geom& g) {
 
 hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s) {
   if (!u || !y) return hipErrorInvalidValue;
-  k_p2_tile<0, double><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, u, nullptr, y, 0.0);
+  k_p2_tile<0, double, double, double><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, u, nullptr, y, 0.0);
   return hipGetLastError();
 }
 
@@ -632,6 +640,22 @@ hipError_t p2_smooth(const P2Geom& g, const T* W, const T* D, const T* x, const 
                      double omega, hipStream_t s) {
   if (!D || !b || !y) return hipErrorInvalidValue;
   k_p2_tile<2, T><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, D, x, b, y, (T)omega);
+  return hipGetLastError();
+}
+
+hipError_t p2_smooth_first32(const P2Geom& g, const float* W, const float* D, const double* b,
+                             float* b32, float* y, double omega, hipStream_t s) {
+  if (!D || !b || !b32 || !y) return hipErrorInvalidValue;
+  k_p2_tile<2, float, double, float><<<p2_tiles(g), kP2TileThreads, 0, s>>>(
+      g, W, D, nullptr, b, y, (float)omega, b32);
+  return hipGetLastError();
+}
+
+hipError_t p2_smooth_last32(const P2Geom& g, const float* W, const float* D, const float* x,
+                            const float* b, double* y, double omega, hipStream_t s) {
+  if (!D || !x || !b || !y) return hipErrorInvalidValue;
+  k_p2_tile<2, float, float, double><<<p2_tiles(g), kP2TileThreads, 0, s>>>(
+      g, W, D, x, b, y, (float)omega);
   return hipGetLastError();
 }
 template hipError_t p2_smooth<double>(const P2Geom&, const double*, const double*, const double*,
