@@ -15,7 +15,8 @@ Prints one JSON line (rank 0, N=1).  `value` = GDOF/s = DOFs / (solve+grad secon
 Beside it: `roofline` (the PCG direction update + FP64 operator, timed with HIP events around
 each of its launches inside the timed steps), `kernels` (fine-grid kernels, each launch timed
 alone after 512 MiB of other writes), `step_bytes` (the elasticity PCG's algorithmic streams
-per step over the step time), `p2` (the reference's P2 element at N=400, FP64), `batch` (the
+per step over the step time), `p2` (the reference's P2 element at N=400: FP64, and `mixed` with
+the preconditioner in FP32), `batch` (the
 64-design workload) and `cpu_baseline` (the CPU oracle at N=400).
 """
 
