@@ -286,6 +286,67 @@ constexpr int kP2TileThreads = 256;
 constexpr int kP2ExtC = kP2TileC + 1;        // cells whose triangles reach the owned nodes
 constexpr int kP2ExtN = 2 * kP2TileC + 3;    // lattice nodes of those cells per side
 
+// The triangles that hold a lattice node (I, J), by its parity class (I & 1) + 2 (J & 1): the
+// offset (dx, dy) of their cell from cell (I >> 1, J >> 1), the triangle t and the node's local
+// index q, listed cells bottom to top, left to right, triangle A before B.  A vertex lies in six
+// triangles, an edge midpoint in two.  Derived at compile time from the local node layout of
+// tri_nodes (translation invariant), so a node's gather is a fixed list of LDS reads.
+struct P2Hold {
+  int dx, dy, t, q;
+};
+struct P2HoldSet {
+  int n;
+  P2Hold e[6];
+};
+constexpr P2HoldSet p2_holds(int cls) {
+  P2HoldSet hs{0, {}};
+  const int pi = cls & 1, pj = cls >> 1;
+  for (int dy = -1; dy <= 0; ++dy)
+    for (int dx = -1; dx <= 0; ++dx)
+      for (int t = 0; t < 2; ++t) {
+        int vi[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {  // (tri_vertex, relative to lattice node (2 ci, 2 cj))
+          const int ox = k == 0 ? 0 : (k == 2 ? 1 : (t == 0 ? 1 : 0));
+          const int oy = k == 0 ? 0 : (k == 2 ? 1 : (t == 0 ? 0 : 1));
+          vi[k] = 2 * (dx + ox);
+          vj[k] = 2 * (dy + oy);
+        }
+        for (int q = 0; q < 6; ++q) {
+          int ni = 0, nj = 0;
+          if (q < 3) {
+            ni = vi[q];
+            nj = vj[q];
+          } else {  // midpoint opposite vertex q - 3
+            const int a = (q - 3 + 1) % 3, b = (q - 3 + 2) % 3;
+            ni = (vi[a] + vi[b]) / 2;
+            nj = (vj[a] + vj[b]) / 2;
+          }
+          if (ni == pi && nj == pj && hs.n < 6) hs.e[hs.n++] = P2Hold{dx, dy, t, q};
+        }
+      }
+  return hs;
+}
+static_assert(p2_holds(0).n == 6 && p2_holds(1).n == 2 && p2_holds(2).n == 2 &&
+                  p2_holds(3).n == 2,
+              "a vertex lies in six triangles, a midpoint in two");
+
+// the sum over the triangles holding owned node (I, J) of their forces on it (LDS fs)
+template <int CLS, typename T>
+__device__ __forceinline__ void p2_gather(const T (&fs)[kP2ExtC * kP2ExtC][2][6][2],
+                                          const P2Geom& g, int I, int J, int X0, int Y0, T& f0,
+                                          T& f1) {
+  constexpr P2HoldSet H = p2_holds(CLS);
+  const int ci = I >> 1, cj = J >> 1;
+#pragma unroll
+  for (int k = 0; k < H.n; ++k) {
+    const int cx = ci + H.e[k].dx, cy = cj + H.e[k].dy;
+    if (cx < 0 || cy < 0 || cx >= g.nx || cy >= g.ny) continue;
+    const int c = (cy - Y0 + 1) * kP2ExtC + (cx - X0 + 1);
+    f0 += fs[c][H.e[k].t][H.e[k].q][0];
+    f1 += fs[c][H.e[k].t][H.e[k].q][1];
+  }
+}
+
 template <int MODE, typename T, typename TB = T, typename TY = T>
 __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* __restrict__ W,
                                                             const T* __restrict__ D,
@@ -352,25 +413,11 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
     const bool fx = p2_fixed(g.fixed, I, J, nX, nY);
     T f0 = T(0), f1 = T(0);
     if (forces && !fx) {
-      const int cx0 = (I & 1) ? (I - 1) / 2 : I / 2 - 1, cx1 = (I & 1) ? (I - 1) / 2 : I / 2;
-      const int cy0 = (J & 1) ? (J - 1) / 2 : J / 2 - 1, cy1 = (J & 1) ? (J - 1) / 2 : J / 2;
-      for (int cy = cy0; cy <= cy1; ++cy) {
-        if (cy < 0 || cy >= g.ny) continue;
-        for (int cx = cx0; cx <= cx1; ++cx) {
-          if (cx < 0 || cx >= g.nx) continue;
-          const int c = (cy - Y0 + 1) * kP2ExtC + (cx - X0 + 1);
-#pragma unroll
-          for (int t = 0; t < 2; ++t) {
-            int NI[6], NJ[6];
-            tri_nodes(t, cx, cy, NI, NJ);
-#pragma unroll
-            for (int q = 0; q < 6; ++q)
-              if (NI[q] == I && NJ[q] == J) {
-                f0 += fs[c][t][q][0];
-                f1 += fs[c][t][q][1];
-              }
-          }
-        }
+      switch ((I & 1) + 2 * (J & 1)) {
+        case 0: p2_gather<0>(fs, g, I, J, X0, Y0, f0, f1); break;
+        case 1: p2_gather<1>(fs, g, I, J, X0, Y0, f0, f1); break;
+        case 2: p2_gather<2>(fs, g, I, J, X0, Y0, f0, f1); break;
+        default: p2_gather<3>(fs, g, I, J, X0, Y0, f0, f1); break;
       }
     }
     if constexpr (MODE == 0) {
