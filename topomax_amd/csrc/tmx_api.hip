@@ -519,8 +519,8 @@ extern "C" int tmx_objective_and_gradient(tmx_problem* h, const double* rho, dou
   if (rc != TMX_OK && rc != TMX_ERR_NONFINITE) return finish(rc);
   // J = F.u reduced into a device scalar now, read after the step's final synchronisation
   RedSlot gen = h->ws.region(3);
-  TMX_HIP(dot(ndofs(h), load_F(h), uv, gen, h->ws.sc() + kScJ, h->stream));
-  TMX_HIP(publish_scalar(h->ws.sc() + kScJ, h->ws.host_dev + kScJ, h->stream));
+  TMX_HIP(dot(ndofs(h), load_F(h), uv, gen, h->ws.sc() + kScJ, h->stream,
+              h->ws.host_dev + kScJ));
   if (w || Mw) {
     double* wv = w ? w : h->w_s.get();
     TMX_TRY(sensitivity(h, xiv, uv, h->b_s.get()));

@@ -251,7 +251,8 @@ template <typename T>
 hipError_t axpy_dev(size_t n, const double* a, const T* x, T* y, hipStream_t s,
                     const double* gate = nullptr);
 // out = sum of r's partials (one block)
-hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s);
+// (with host_out, the sum is also written to that mapped host memory, as publish_scalar does)
+hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s, double* host_out = nullptr);
 // reliable-update PCG: x += xs + (*alpha) p, xs = 0; and r = (float) r64 with partials of |r64|^2
 hipError_t ru_flush(size_t n, const double* alpha, const float* p, float* xs, double* x,
                     hipStream_t s);
@@ -260,7 +261,7 @@ hipError_t ru_replace(size_t n, const double* r64, float* r, RedSlot& rr, hipStr
 hipError_t add_vec(size_t n, const double* a, double* y, hipStream_t s);
 // dot_t + reduce_to
 hipError_t dot(size_t n, const double* a, const double* b, RedSlot& r, double* out,
-               hipStream_t s);
+               hipStream_t s, double* host_out = nullptr);
 // PCG scalars on the device.  flag: 0 running, 1 converged, 2 p.Ap <= 0, 3 non-finite.
 // alpha: pq = sum; alpha = rz/pq (0 once stopped).  rr: rr = sum; converged if rr <= tol2.
 // beta: rznew = sum; beta = rznew/rz; rz = rznew.

@@ -3051,14 +3051,21 @@ hipError_t ru_replace(size_t n, const double* r64, float* r, RedSlot& rr, hipStr
 
 __device__ __forceinline__ void reduce_to_body(const BlkId blk, const double* part,
                                                int n,
-                                               double* out) {
+                                               double* out, double* host_out) {
   const double t = sum_partials(part, n);
-  if (threadIdx.x == 0) *out = t;
+  if (threadIdx.x == 0) {
+    *out = t;
+    if (host_out) {  // (as publish_scalar: the value to mapped host memory)
+      host_out[0] = t;
+      __threadfence_system();
+    }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_to(const double* __restrict__ part, int n,
-                                                      double* __restrict__ out) {
-  reduce_to_body(hw_blk(), part, n, out);
+                                                      double* __restrict__ out,
+                                                      double* __restrict__ host_out) {
+  reduce_to_body(hw_blk(), part, n, out, host_out);
 }
 
 template <typename T>
@@ -3091,16 +3098,16 @@ template hipError_t axpy_dev<double>(size_t, const double*, const double*, doubl
 template hipError_t axpy_dev<float>(size_t, const double*, const float*, float*, hipStream_t,
                                     const double*);
 
-hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s) {
-  TMX_LAUNCH_OR_RECORD(kBlock, (reduce_to_body), 1, 0, s, (k_reduce_to), r.partials, r.nblocks, out);
+hipError_t reduce_to(const RedSlot& r, double* out, hipStream_t s, double* host_out) {
+  TMX_LAUNCH_OR_RECORD(kBlock, (reduce_to_body), 1, 0, s, (k_reduce_to), r.partials, r.nblocks, out, host_out);
   return hipGetLastError();
 }
 
 hipError_t dot(size_t n, const double* a, const double* b, RedSlot& r, double* out,
-               hipStream_t s) {
+               hipStream_t s, double* host_out) {
   hipError_t e = dot_t<double>(n, a, b, r, s);
   if (e != hipSuccess) return e;
-  return reduce_to(r, out, s);
+  return reduce_to(r, out, s, host_out);
 }
 
 // PCG scalar steps (one block).  sc layout: see PcgSlots.

@@ -1539,9 +1539,9 @@ int Hierarchy::solve(Workspace& ws, const double* b, double* x, const SolveParam
         }
       } else if (prm.verify && bb > 0.0) {  // the true residual
         TMX_TRY(residual<double>(x, b, rr64_.get(), s));
-        TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScVerify, s));
+        TMX_HIP(dot(nv, rr64_.get(), rr64_.get(), gen, ws.sc() + kScVerify, s,
+                    defer ? ws.host_dev + kScVerify : nullptr));
         if (defer) {
-          TMX_HIP(publish_scalar(ws.sc() + kScVerify, ws.host_dev + kScVerify, s));
           S.pending_rr = ws.host + kScVerify;
           S.pending_bb = bb;
         } else {
