@@ -102,6 +102,12 @@ int Hierarchy::init(Phys phys, int nx, int ny, double hx, double hy, unsigned fi
     L.sh32 = SweepShape::make(L.nx, L.ny, rows_default(L.nx, L.ny));
   }
   find_tail();
+  if (phys == kElasticity) {  // the overlapped coarse build's stream and events, made up front
+                              // so that no solve's setup pays for their creation
+    if (!side_) TMX_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    if (!fork_ev_) TMX_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    if (!join_ev_) TMX_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
+  }
   Level& last = levels_.back();
   cnx_ = last.nx;
   cny_ = last.ny;
