@@ -393,8 +393,18 @@ static int p2_solve(tmx_problem* h, const double* xi, double* u, const SolvePara
   auto t0 = std::chrono::steady_clock::now();
   int it = 0;
   double rel = 0.0;
+  // the operator launch also forms MINRES's first Lanczos update and the v.y partials
+  MinresApplyL1 apply_l1 = nullptr;
+  if (p2_apply_lanczos_ok(g))
+    apply_l1 = [&](const double* v, const double* r1, double* y, RedSlot* part,
+                   const double* beta, const double* oldb) -> int {
+      TMX_HIP(p2_apply_lanczos(g, h->W2.get(), v, r1, y, beta, oldb, part->partials,
+                               &part->nblocks, s));
+      return 0;
+    };
   const int rc = minres_core(h->ws, n, h->F2m.get(), X, bf, apply_fn, pc_fn, prm.rtol,
-                             prm.max_iter, bb, &it, &rel, s, std::getenv("TMX_TRACE") != nullptr);
+                             prm.max_iter, bb, &it, &rel, s, std::getenv("TMX_TRACE") != nullptr,
+                             apply_l1);
   st->iterations = it;
   st->rel_residual = rel;
   if (rc == TMX_OK || rc == TMX_ERR_NOT_CONVERGED) TMX_HIP(copy_vec<double>(n, X, u, s));

@@ -336,6 +336,10 @@ hipError_t mr_scale(size_t n, const double* sc, const MrSlots& q, const double* 
 // y -= (beta/oldb) r1; alfa = v.y; y -= (alfa/beta) r2
 hipError_t mr_lanczos(size_t n, double* sc, const MrSlots& q, const double* v, const double* r1,
                       const double* r2, double* y, RedSlot& part, hipStream_t s);
+// its second half alone: alfa from the partials of v.y in part; y -= (alfa/beta) r2 (for an
+// operator whose launch already formed y - (beta/oldb) r1 and those partials)
+hipError_t mr_lanczos2(size_t n, double* sc, const MrSlots& q, const RedSlot& part,
+                       const double* r2, double* y, hipStream_t s);
 hipError_t mr_step(const RedSlot& rz, double* sc, const MrSlots& q, double* host_out,
                    hipStream_t s);
 hipError_t mr_update(size_t n, const double* sc, const MrSlots& q, double* v, const double* z,

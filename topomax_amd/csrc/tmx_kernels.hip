@@ -3863,6 +3863,11 @@ hipError_t mr_lanczos(size_t n, double* sc, const MrSlots& q, const double* v, c
   k_mr_lanczos2<<<g, kBlock, 0, s>>>(n, sc, q, part.partials, part.nblocks, r2, y);
   return hipGetLastError();
 }
+hipError_t mr_lanczos2(size_t n, double* sc, const MrSlots& q, const RedSlot& part,
+                       const double* r2, double* y, hipStream_t s) {
+  k_mr_lanczos2<<<vec_grid(n), kBlock, 0, s>>>(n, sc, q, part.partials, part.nblocks, r2, y);
+  return hipGetLastError();
+}
 hipError_t mr_step(const RedSlot& rz, double* sc, const MrSlots& q, double* host_out,
                    hipStream_t s) {
   k_mr_step<<<1, kBlock, 0, s>>>(rz.partials, rz.nblocks, sc, q, host_out);

@@ -29,6 +29,13 @@ hipError_t p2_weights(const P2Geom& g, const double* xi, int p, double m, bool p
                       double* W, hipStream_t s);
 // y = K u for the Dirichlet-masked P2 stiffness (identity rows and columns on fixed nodes)
 hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s);
+// MINRES's operator step: y = K v - (*beta / *oldb) r1 (0 when *oldb is 0) with the partials of
+// v.y in part (one per tile; their count in *nparts)
+hipError_t p2_apply_lanczos(const P2Geom& g, const double* W, const double* v, const double* r1,
+                            double* y, const double* beta, const double* oldb, double* part,
+                            int* nparts, hipStream_t s);
+// whether the grid's tiles fit the partials buffer of p2_apply_lanczos
+bool p2_apply_lanczos_ok(const P2Geom& g);
 // 2x2 diagonal blocks of K (3 per node: xx, xy, yy)
 hipError_t p2_diag(const P2Geom& g, const double* W, double* D, hipStream_t s);
 // one damped block-Jacobi sweep y = x + omega D^-1 (b - K x) (x = nullptr: zero); fixed nodes

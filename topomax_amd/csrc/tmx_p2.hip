@@ -276,7 +276,9 @@ __global__ void k_p2_weights(P2Geom g, const double* __restrict__ xi, int p, dou
 // LDS (cells bottom to top, left to right, triangle A before B).  Forming the forces per node
 // instead would form each triangle's stresses once for each of its six nodes.
 //   MODE 0: y = K u (fixed nodes: y = u);  1: y = b - K u (fixed: b - u);
-//   2: y = x + omega D^-1 (b - K x) (x null: from zero; fixed nodes 0).
+//   2: y = x + omega D^-1 (b - K x) (x null: from zero; fixed nodes 0);
+//   3: MINRES's operator step y = K u - (*mb / *mo) b (0 when *mo is 0), with the partial of
+//      u.y of the workgroup's owned nodes in part[workgroup] (FP64).
 // T: FP64, or FP32 for the mixed scheme's preconditioner (weights, diagonal blocks and vectors
 // in float, the diagonal blocks inverted by the hardware reciprocal).  TB / TY: the types of b
 // and y (a double right-hand side read, or a double result written, by the float cycle's
@@ -353,7 +355,10 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
                                                             const T* __restrict__ x,
                                                             const TB* __restrict__ b,
                                                             TY* __restrict__ y, T omega,
-                                                            T* __restrict__ bcopy = nullptr) {
+                                                            T* __restrict__ bcopy = nullptr,
+                                                            const double* __restrict__ mb = nullptr,
+                                                            const double* __restrict__ mo = nullptr,
+                                                            double* __restrict__ part = nullptr) {
   __shared__ T us[kP2ExtN * kP2ExtN][2];
   __shared__ T fs[kP2ExtC * kP2ExtC][2][6][2];
   const int nX = 2 * g.nx + 1, nY = 2 * g.ny + 1;
@@ -406,6 +411,9 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
   const int IE = X0 + kP2TileC >= g.nx ? nX : 2 * (X0 + kP2TileC);
   const int JE = Y0 + kP2TileC >= g.ny ? nY : 2 * (Y0 + kP2TileC);
   const int OW = IE - 2 * X0, OH = JE - 2 * Y0;
+  double dot_part = 0.0;  // (MODE 3)
+  double cl = 0.0;
+  if constexpr (MODE == 3) cl = *mo != 0.0 ? *mb / *mo : 0.0;
   for (int k = (int)threadIdx.x; k < OW * OH; k += kP2TileThreads) {
     const int oj = k / OW, oi = k - oj * OW;
     const int I = 2 * X0 + oi, J = 2 * Y0 + oj;
@@ -423,6 +431,12 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
     if constexpr (MODE == 0) {
       y[2 * n] = fx ? x[2 * n] : f0;
       y[2 * n + 1] = fx ? x[2 * n + 1] : f1;
+    } else if constexpr (MODE == 3) {
+      const T u0 = x[2 * n], u1 = x[2 * n + 1];
+      const T y0 = (fx ? u0 : f0) - cl * b[2 * n], y1 = (fx ? u1 : f1) - cl * b[2 * n + 1];
+      y[2 * n] = y0;
+      y[2 * n + 1] = y1;
+      dot_part += u0 * y0 + u1 * y1;
     } else if constexpr (MODE == 1) {
       y[2 * n] = b[2 * n] - (fx ? x[2 * n] : f0);
       y[2 * n + 1] = b[2 * n + 1] - (fx ? x[2 * n + 1] : f1);
@@ -443,6 +457,11 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
         y[2 * n + 1] = (TY)(x1 + omega * (a * ry - c * rx) * inv);
       }
     }
+  }
+  if constexpr (MODE == 3) {
+    const double vv[1] = {dot_part};
+    double* const o[1] = {part};
+    block_partials<1, kP2TileThreads>(vv, o, blockIdx.x);
   }
 }
 
@@ -659,10 +678,21 @@ hipError_t p2_weights(const P2Geom& g, const double* xi, int p, double m, bool p
 static unsigned p2_tiles(const P2Geom& g) {
   return (unsigned)(((g.nx + kP2TileC - 1) / kP2TileC) * ((g.ny + kP2TileC - 1) / kP2TileC));
 }
+bool p2_apply_lanczos_ok(const P2Geom& g) { return p2_tiles(g) <= (unsigned)kMaxPartials; }
 
 hipError_t p2_apply(const P2Geom& g, const double* W, const double* u, double* y, hipStream_t s) {
   if (!u || !y) return hipErrorInvalidValue;
   k_p2_tile<0, double, double, double><<<p2_tiles(g), kP2TileThreads, 0, s>>>(g, W, nullptr, u, nullptr, y, 0.0);
+  return hipGetLastError();
+}
+
+hipError_t p2_apply_lanczos(const P2Geom& g, const double* W, const double* v, const double* r1,
+                            double* y, const double* beta, const double* oldb, double* part,
+                            int* nparts, hipStream_t s) {
+  if (!v || !r1 || !y || !part || p2_tiles(g) > (unsigned)kMaxPartials) return hipErrorInvalidValue;
+  *nparts = (int)p2_tiles(g);
+  k_p2_tile<3, double, double, double><<<p2_tiles(g), kP2TileThreads, 0, s>>>(
+      g, W, nullptr, v, r1, y, 0.0, nullptr, beta, oldb, part);
   return hipGetLastError();
 }
 

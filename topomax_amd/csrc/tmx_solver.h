@@ -157,11 +157,15 @@ struct SolveStats {
 struct MinresBufs {
   double *r1, *r2, *y, *z, *v, *w0, *w1, *w2;
 };
+// apply_l1 (optional): y = A v - (*beta / *oldb) r1 (0 when *oldb is 0) with the partials of
+// v.y in part, in one launch; used in place of apply and the first Lanczos update.
+using MinresApplyL1 = std::function<int(const double* v, const double* r1, double* y,
+                                        RedSlot* part, const double* beta, const double* oldb)>;
 int minres_core(Workspace& ws, size_t n, const double* b, double* x, MinresBufs bufs,
                 const std::function<int(const double*, double*)>& apply,
                 const std::function<int(const double*, double*, RedSlot*)>& precond,
                 double rtol, int max_iter, double bnorm2, int* iters, double* rel,
-                hipStream_t s, bool trace);
+                hipStream_t s, bool trace, const MinresApplyL1& apply_l1 = nullptr);
 
 // Geometric multigrid + PCG for one operator family on one grid.
 template <typename T>

@@ -1306,7 +1306,7 @@ int minres_core(Workspace& ws, size_t nv, const double* b, double* X, MinresBufs
                 const std::function<int(const double*, double*)>& apply,
                 const std::function<int(const double*, double*, RedSlot*)>& precond,
                 double rtol, int max_iter, double bnorm2, int* iters, double* rel,
-                hipStream_t s, bool trace) {
+                hipStream_t s, bool trace, const MinresApplyL1& apply_l1) {
   constexpr int kMaxRestarts = 8;
   double* sc = ws.sc();
   double *R1 = bf.r1, *R2 = bf.r2, *Y = bf.y, *Z = bf.z, *V = bf.v;
@@ -1357,8 +1357,13 @@ int minres_core(Workspace& ws, size_t nv, const double* b, double* X, MinresBufs
     auto slot = [&](int k) { return ring + 4 * ((k - 1) & 1); };
     ring[3] = ring[7] = 0.0;
     auto launch = [&](int k) -> int {
-      TMX_TRY(apply(V, Y));
-      TMX_HIP(mr_lanczos(nv, sc, kMr, V, R1, R2, Y, part, s));
+      if (apply_l1) {
+        TMX_TRY(apply_l1(V, R1, Y, &part, sc + kMr.beta, sc + kMr.oldb));
+        TMX_HIP(mr_lanczos2(nv, sc, kMr, part, R2, Y, s));
+      } else {
+        TMX_TRY(apply(V, Y));
+        TMX_HIP(mr_lanczos(nv, sc, kMr, V, R1, R2, Y, part, s));
+      }
       std::swap(R1, R2);  // r1 <- r2
       std::swap(R2, Y);   // r2 <- y (the new Lanczos vector); y <- the old r1 (free)
       TMX_TRY(precond(R2, Z, &rz));
