@@ -70,17 +70,18 @@ __device__ __forceinline__ void tri_nodes(int t, int cx, int cy, int (&I)[6], in
   }
 }
 
-// barycentric gradients d_k of triangle t (units 1/hx, 1/hy applied)
+// barycentric gradients d_k of triangle t, given ihx = 1/hx and ihy = 1/hy (formed once per
+// thread by the caller: the same values as dividing here, without a division per triangle)
 template <typename T>
-__device__ __forceinline__ void tri_grads(int t, T hx, T hy, T (&d)[3][2]) {
+__device__ __forceinline__ void tri_grads(int t, T ihx, T ihy, T (&d)[3][2]) {
   if (t == 0) {
-    d[0][0] = T(-1) / hx; d[0][1] = T(0);
-    d[1][0] = T(1) / hx;  d[1][1] = T(-1) / hy;
-    d[2][0] = T(0);       d[2][1] = T(1) / hy;
+    d[0][0] = -ihx; d[0][1] = T(0);
+    d[1][0] = ihx;  d[1][1] = -ihy;
+    d[2][0] = T(0); d[2][1] = ihy;
   } else {
-    d[0][0] = T(0);       d[0][1] = T(-1) / hy;
-    d[1][0] = T(-1) / hx; d[1][1] = T(1) / hy;
-    d[2][0] = T(1) / hx;  d[2][1] = T(0);
+    d[0][0] = T(0); d[0][1] = -ihy;
+    d[1][0] = -ihx; d[1][1] = ihy;
+    d[2][0] = ihx;  d[2][1] = T(0);
   }
 }
 
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(kP2TileThreads) void k_p2_tile(P2Geom g, const T* _
       if (cx < 0 || cy < 0 || cx >= g.nx || cy >= g.ny) continue;  // (no owned node reads it)
       T w[6], d[3][2], ul[6][2], f[6][2];
       load_W(W, g.nx, cx, cy, t, w);
-      tri_grads(t, (T)g.hx, (T)g.hy, d);
+      tri_grads(t, T(1) / (T)g.hx, T(1) / (T)g.hy, d);
       int NI[6], NJ[6];
       tri_nodes(t, cx, cy, NI, NJ);
 #pragma unroll
@@ -471,11 +472,12 @@ __global__ void k_p2_diag(P2Geom g, const double* __restrict__ W, double* __rest
   if (n >= (size_t)nX * (2 * g.ny + 1)) return;
   const int I = (int)(n % nX), J = (int)(n / nX);
   double dxx = 0.0, dxy = 0.0, dyy = 0.0;
+  const double ihx = 1.0 / g.hx, ihy = 1.0 / g.hy;
   for_each_triangle(g, I, J, [&](int t, int cx, int cy, int a, const int (&)[6],
                                  const int (&)[6]) {
     double w[6], d[3][2], ul[6][2], y[2];
     load_W(W, g.nx, cx, cy, t, w);
-    tri_grads(t, g.hx, g.hy, d);
+    tri_grads(t, ihx, ihy, d);
     for (int c = 0; c < 2; ++c) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) ul[q][0] = ul[q][1] = 0.0;
@@ -546,10 +548,11 @@ __global__ void k_p2_sensitivity(P2Geom g, const double* __restrict__ xi,
   const double area = 0.5 * g.hx * g.hy;
   const double coef = -(double)p * (1.0 - m) * 2.0 * area * factorial(p - 1) / factorial(p + 4);
   double acc = 0.0;
+  const double ihx = 1.0 / g.hx, ihy = 1.0 / g.hy;
   for_each_triangle(g, 2 * iv, 2 * jv, [&](int t, int cx, int cy, int li, const int (&NI)[6],
                                            const int (&NJ)[6]) {
     double d[3][2], ul[6][2], a[3];
-    tri_grads(t, g.hx, g.hy, d);
+    tri_grads(t, ihx, ihy, d);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       const size_t n = (size_t)NJ[q] * nX + NI[q];
